@@ -1,0 +1,184 @@
+"""Benchmark of the cut-element hot path on one MI355X.
+
+Workload (default): the 3D plain configuration -- unit cube, 64^3 cells x 6
+Kuhn tets = 1,572,864 tets, P1 level set |x - (0.5,0.5,0.5)| - 0.35 +
+0.01 sin(7x) sin(7y) sin(7z) (perturbed as InterpolateToP1 does), order-4
+straight cut rules on NEG and IF, P2 H1 mass + stiffness element matrices.
+
+One step = the fused rule + element-matrix pass (xg_cut_pipeline) over all cut
+elements for both domain types, inputs resident in HBM.  value = cut
+elements / second.  Marking (xg_mark) is timed separately.
+
+    python bench.py [--steps K] [--warmup W] [--precision fp32|fp64]
+"""
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cut-elem/s (rule+elmat) plain vs learned/mixed at ≤1e-6 rel err; %HBM/%MFMA"
+PEAK_FP32_MFMA = 157.3e12
+PEAK_FP64_MFMA = 78.6e12
+PEAK_HBM = 8.0e12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=64, help="cells per direction")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def read_traffic(name):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
+    scaled by the copy-kernel calibration) from the committed summary."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch\s*[:=]\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the path shards by element "
+              "ranges; see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from ngsxfem_amd import _abi
+    from ngsxfem_amd import workloads as W
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    ctx = _abi.Context(0, stream=stream)
+
+    D, p, order = 3, 2, 4
+    t0 = time.time()
+    mesh, lset = W.config2(args.n)
+    print("# workload built in %.1fs: %d tets, %d vertices" % (time.time() - t0, mesh.ne, mesh.nv),
+          file=sys.stderr, flush=True)
+    d_el = torch.from_numpy(mesh.el2v).to(dev)
+    d_x = torch.from_numpy(mesh.coords).to(dev)
+    d_l = torch.from_numpy(lset).to(dev)
+    ne = mesh.ne
+    nw = (ne + 31) // 32
+    dt = torch.empty(ne, dtype=torch.uint8, device=dev)
+    cr = torch.empty(ne, dtype=torch.float64, device=dev)
+    bits = torch.empty(8 * nw, dtype=torch.int32, device=dev)
+    cut = torch.empty(ne, dtype=torch.int32, device=dev)
+    ncut = torch.zeros(1, dtype=torch.int32, device=dev)
+
+    # ---- marking (timed separately) ----
+    def mark():
+        ctx.mark(D, d_el, d_l, dt, cr, bits, cut, ncut)
+
+    for _ in range(3):
+        mark()
+    ctx.sync()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    nmark = 20
+    ev[0].record(stream)
+    for _ in range(nmark):
+        mark()
+    ev[1].record(stream)
+    ctx.sync()
+    mark_ms = ev[0].elapsed_time(ev[1]) / nmark
+    n_cut = int(ncut.item())
+    cut_list = cut[:n_cut].clone()
+
+    # ---- algorithmic work per step ----
+    npts = {}
+    for dtv in (_abi.XG_NEG, _abi.XG_IF):
+        off = torch.zeros(n_cut + 1, dtype=torch.int32, device=dev)
+        ctx.cut_rule_count(D, d_el, d_l, dtv, order, off, elems=cut_list)
+        ctx.sync()
+        npts[dtv] = int(off[-1].item())
+    nd = 10
+    flops = sum(2.0 * npts[k] * nd * nd * (1 + D) for k in npts)
+    bytes_alg = n_cut * (4 * 4 + 8 * 4 + 24 * 4 + 4) + n_cut * 2 * nd * nd * (4 if args.precision == "fp32" else 8)
+
+    prec = _abi.XG_PREC_FP32 if args.precision == "fp32" else _abi.XG_PREC_FP64
+    out = torch.empty(n_cut * 2 * nd * nd, dtype=torch.float32 if prec == _abi.XG_PREC_FP32 else torch.float64,
+                      device=dev)
+    dts = [_abi.XG_NEG, _abi.XG_IF]
+
+    def step():
+        ctx.cut_pipeline(D, p, order, d_el, d_x, d_l, dts, 3, prec, out, elems=cut_list)
+
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    g0 = torch.cuda.Event(enable_timing=True)
+    g1 = torch.cuda.Event(enable_timing=True)
+    g0.record(stream)
+    for k in range(args.steps):
+        evs[k][0].record(stream)
+        step()
+        evs[k][1].record(stream)
+    g1.record(stream)
+    ctx.sync()
+    torch.cuda.synchronize()
+    total_ms = g0.elapsed_time(g1)
+    launch_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    ms_per_step = total_ms / args.steps
+    value = n_cut / (ms_per_step * 1e-3)
+
+    peak = PEAK_FP32_MFMA if prec == _abi.XG_PREC_FP32 else PEAK_FP64_MFMA
+    achieved = flops / (launch_ms * 1e-3)
+    roof = {"bound": "mfma", "kernel": "k_pipeline<3,2,%s>" % ("float" if prec == 1 else "double"),
+            "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+            "frac": achieved / peak, "traffic": read_traffic("pipeline_c2_pmc.txt"),
+            "algorithmic_flops_per_launch": flops, "algorithmic_bytes_per_launch": bytes_alg,
+            "hbm_frac_algorithmic": bytes_alg / (launch_ms * 1e-3) / PEAK_HBM,
+            "avg_launch_ms": launch_ms}
+
+    # ---- CPU baseline: FP64 restatement of the reference algorithm ----
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        cores = min(16, os.cpu_count() or 1)
+        cl = cut_list.cpu().numpy()
+        reps, t_cpu = 0, 0.0
+        while t_cpu < args.cpu_seconds and reps < 100:
+            t1 = time.perf_counter()
+            O.cut_pipeline(D, p, order, cl, mesh.el2v, mesh.coords, lset, dts, mask=3, nthreads=cores)
+            t_cpu += time.perf_counter() - t1
+            reps += 1
+        cpu = {"value": reps * n_cut / t_cpu, "unit": "cut-elem/s", "cores": cores, "kind": "port",
+               "sample": "%d passes over all %d cut tets (FP64 oracle, %d threads, %.1fs)" % (reps, n_cut, cores, t_cpu)}
+
+    res = {
+        "metric": METRIC, "value": value, "unit": "cut-elem/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
+        "config": {"workload": "3D plain: unit cube %d^3 x 6 Kuhn tets (%d tets), P1 sphere r=0.35 + 0.01 "
+                               "sin(7x)sin(7y)sin(7z), order-4 straight cut rule NEG+IF, P2 mass+stiffness "
+                               "elmat (%s MFMA)" % (args.n, ne, args.precision),
+                   "cut_elements": n_cut, "points_neg": npts[0], "points_if": npts[2]},
+        "roofline": roof, "cpu_baseline": cpu,
+        "marking": {"ms": mark_ms, "tets_per_s": ne / (mark_ms * 1e-3),
+                    "hbm_frac_algorithmic": ne * 26.4 / (mark_ms * 1e-3) / PEAK_HBM},
+    }
+    print(json.dumps(res), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

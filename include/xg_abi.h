@@ -1,0 +1,133 @@
+/*
+ * xg_abi.h -- C ABI of the MI355X (gfx950) cut-element hot path.
+ *
+ * Plain pointers and sizes only.  Unless stated otherwise every array
+ * argument is a DEVICE pointer (hipMalloc'ed or a torch CUDA tensor's
+ * data_ptr); outputs are caller-allocated.  All calls are asynchronous on the
+ * context's stream; results are valid after xg_sync() (or any later work on
+ * the same stream).  Every function returns XG_OK (0) or a negative XG_ERR_*
+ * code; xg_last_error() gives the message of the last failure on the calling
+ * host thread.  One context per host thread.
+ *
+ * Conventions (shared with the reference):
+ *   DOMAIN_TYPE      NEG = 0, POS = 1, IF = 2          (utils/ngsxstd.hpp:16-20)
+ *   COMBINED_DOMAIN  NO=0 NEG=1 POS=2 UNCUT=3 IF=4 HASNEG=5 HASPOS=6 ANY=7
+ *                                                       (utils/ngsxstd.hpp:23-31)
+ *   bit arrays       uint32 words, element e is bit (e & 31) of word e >> 5;
+ *                    byte-for-byte the layout of an NGSolve BitArray.
+ *   mesh             coords: nv x D FP64, el2v: ne x (D+1) int32.
+ *   reference simplex vertices (1,0,0), (0,1,0), (0,0,1), (0,0,0); the element
+ *                    map sends reference vertex k to el2v[e][k].
+ *   rules            CSR: offsets int32[n+1]; points double4 (x, y, z, w) in
+ *                    reference coordinates; the physical weight is w * |det F|
+ *                    (straightcutrule.cpp:580-581: "a scaling with
+ *                    mip.GetMeasure() gives the correct weight").
+ */
+#ifndef XG_ABI_H
+#define XG_ABI_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XG_OK 0
+#define XG_ERR_ARG (-1)        /* invalid argument / size */
+#define XG_ERR_HIP (-2)        /* HIP runtime failure */
+#define XG_ERR_DEGENERATE (-3) /* straightcutrule.cpp:45-49,148,193: cut of a vertex value 0 */
+#define XG_ERR_CAPACITY (-4)   /* output capacity exceeded */
+#define XG_ERR_SUBDIV (-5)     /* xintegration.cpp:1386 "did not expect this.. -3-" */
+#define XG_ERR_UNSUPPORTED (-6)
+
+#define XG_NEG 0
+#define XG_POS 1
+#define XG_IF 2
+
+/* element-matrix arithmetic */
+#define XG_PREC_FP64 0   /* FP64 MFMA (v_mfma_f64_16x16x4f64), reference mode */
+#define XG_PREC_FP32 1   /* FP32 MFMA (v_mfma_f32_16x16x4f32), FP32 accumulate */
+#define XG_PREC_BF16X3 2 /* 3-way split BF16 MFMA (6 products), FP32 accumulate */
+
+/* element-matrix form mask */
+#define XG_FORM_MASS 1      /* u*v */
+#define XG_FORM_LAPLACE 2   /* grad(u)*grad(v) */
+
+typedef struct xg_ctx xg_ctx;
+
+/* ---- context --------------------------------------------------------- */
+/* No reference counterpart (NGSolve's LocalHeap + TaskManager play this role,
+   utils/ngsxstd.cpp:3-31): binds a device and a stream. */
+int xg_ctx_create(int device, xg_ctx **out);
+int xg_ctx_destroy(xg_ctx *ctx);
+/* stream: a hipStream_t (0 = the null stream) */
+int xg_ctx_set_stream(xg_ctx *ctx, void *stream);
+int xg_sync(xg_ctx *ctx);
+const char *xg_last_error(void);
+/* number of quadrature points of the base rule of dimension D, order */
+int xg_rule_npts(int D, int order);
+
+/* ---- a2: InterpolateP1 perturbation (utils/p1interpol.cpp:104-107) ----- */
+/* v[i] = eps if |v[i]| < eps, in place (n values, device). */
+int xg_p1_perturb(xg_ctx *ctx, int n, double *v, double eps);
+
+/* ---- a5, a12: CutInformation::Update marking (xfem/cutinfo.cpp:61-133) */
+/* D = element dimension (1, 2, 3); el2v: ne x (D+1).
+   dt (nullable): uint8[ne] DOMAIN_TYPE per element.
+   cut_ratio (nullable): FP64[ne] = part_vol[NEG] / (part_vol[NEG] + part_vol[POS]).
+   bits: uint32[8][nw], nw = ceil(ne/32): the 8 COMBINED_DOMAIN_TYPE arrays.
+   cut_list (nullable): int32[ne], receives the IF elements in ascending order;
+   n_cut (nullable, device int32[1]) receives their number.
+   Bit-exact with the reference algorithm (FP64, no contraction). */
+int xg_mark(xg_ctx *ctx, int D, int ne, const int32_t *el2v, const double *lset_v,
+            uint8_t *dt, double *cut_ratio, uint32_t *bits, int32_t *cut_list,
+            int32_t *n_cut);
+
+/* ---- a13: GetFacetsWithNeighborTypes (xfem/cutinfo.cpp:476-538) ------- */
+/* facet f has neighbours f2el[f][0], f2el[f][1] (-1 = none).  a, b: element
+   bit arrays; result bit f = use_and ? (a(L)&b(R) | a(R)&b(L))
+   : (a(L)|b(R) ... as the reference); boundary sides use bnd_val_a/b. */
+int xg_facets_with_neighbor_types(xg_ctx *ctx, int nf, const int32_t *f2el,
+                                  const uint32_t *a, const uint32_t *b, int bnd_val_a,
+                                  int bnd_val_b, int use_and, uint32_t *out);
+
+/* ---- a8-a11: StraightCutIntegrationRule (cutint/straightcutrule.cpp:582-677) */
+/* Pass 1: counts.  elems (nullable = 0..nel-1).  offsets: int32[nel+1]
+   (exclusive prefix sum, offsets[nel] = total points); nsub (nullable):
+   uint8[nel] sub-simplex count (0 = uncut). */
+int xg_cut_rule_count(xg_ctx *ctx, int D, int nel, const int32_t *elems,
+                      const int32_t *el2v, const double *lset_v, int dt, int order,
+                      int32_t *offsets, uint8_t *nsub);
+/* Pass 2: points (x, y, z, w) per element at offsets[i] (double4 array of
+   offsets[nel] entries).  Interface weights include |F^{-T} n|. */
+int xg_cut_rule_fill(xg_ctx *ctx, int D, int nel, const int32_t *elems,
+                     const int32_t *el2v, const double *coords, const double *lset_v,
+                     int dt, int order, const int32_t *offsets, double *pts);
+
+/* ---- a16: IntegrateX (cutint/python_cutint.cpp:21-110) ----------------- */
+/* f = sum_t coef[t] x^expo[3t] y^expo[3t+1] z^expo[3t+2] (host arrays,
+   nterms <= 32).  per_elem (nullable, device FP64[nel]); total: device FP64[1]
+   (overwritten). */
+int xg_integrate(xg_ctx *ctx, int D, int nel, const int32_t *elems, const int32_t *el2v,
+                 const double *coords, const int32_t *offsets, const double *pts,
+                 int nterms, const double *coef, const int32_t *expo, double *per_elem,
+                 double *total);
+
+/* ---- a14: SymbolicCutBilinearFormIntegrator::CalcElementMatrix ---------
+   (xfem/symboliccutbfi.cpp:37-46, 76-304) on a materialised rule.
+   out: nel x ndof x ndof (FP64 for XG_PREC_FP64, FP32 otherwise), row = test
+   dof, overwritten (CalcElementMatrix semantics). */
+int xg_elmat(xg_ctx *ctx, int D, int p, int nel, const int32_t *elems, const int32_t *el2v,
+             const double *coords, const int32_t *offsets, const double *pts, int mask,
+             int precision, void *out);
+
+/* ---- fused: rule + element matrix, points never leave the chip ---------
+   For each element i and each dts[t] (host array, ndt <= 3): the straight cut
+   rule of order `order` and CalcElementMatrix on it.
+   out: nel x ndt x ndof x ndof (FP64 / FP32 by precision). */
+int xg_cut_pipeline(xg_ctx *ctx, int D, int p, int order, int nel, const int32_t *elems,
+                    const int32_t *el2v, const double *coords, const double *lset_v,
+                    int ndt, const int *dts, int mask, int precision, void *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

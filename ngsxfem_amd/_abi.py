@@ -1,0 +1,159 @@
+"""ctypes binding of libxg.so, the C ABI declared in include/xg_abi.h.
+
+Device buffers are torch CUDA tensors (PyTorch is used here only for device
+memory and streams); every call passes raw device pointers to the HIP
+kernels.  There is no CPU fallback: loading fails loudly when libxg.so is
+missing, and every call raises XgError on a non-zero status.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libxg.so")
+
+XG_OK = 0
+XG_NEG, XG_POS, XG_IF = 0, 1, 2
+XG_PREC_FP64, XG_PREC_FP32, XG_PREC_BF16X3 = 0, 1, 2
+XG_FORM_MASS, XG_FORM_LAPLACE = 1, 2
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+
+
+class XgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("xg error %d: %s" % (code, msg))
+        self.code = code
+
+
+# signatures: name -> argtypes (all return int)
+_SIGS = {
+    "xg_ctx_create": [_i, ctypes.POINTER(_vp)],
+    "xg_ctx_destroy": [_vp],
+    "xg_ctx_set_stream": [_vp, _vp],
+    "xg_sync": [_vp],
+    "xg_rule_npts": [_i, _i],
+    "xg_p1_perturb": [_vp, _i, _vp, ctypes.c_double],
+    "xg_mark": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "xg_facets_with_neighbor_types": [_vp, _i, _vp, _vp, _vp, _i, _i, _i, _vp],
+    "xg_cut_rule_count": [_vp, _i, _i, _vp, _vp, _vp, _i, _i, _vp, _vp],
+    "xg_cut_rule_fill": [_vp, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp],
+    "xg_integrate": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp],
+    "xg_elmat": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
+    "xg_cut_pipeline": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _i, _vp],
+}
+
+
+def declared_symbols():
+    """Entry points declared in include/xg_abi.h (parsed from the header)."""
+    import re
+    hdr = os.path.join(HERE, "..", "include", "xg_abi.h")
+    txt = open(hdr).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(xg_\w+)\s*\(", txt, re.M)))
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise ImportError("libxg.so not built (%s); run __graft_entry__.build() or "
+                              "python -m ngsxfem_amd.build_ext" % LIBPATH)
+        L = ctypes.CDLL(LIBPATH)
+        for name, args in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _i
+        L.xg_last_error.argtypes = []
+        L.xg_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def check(code):
+    if code != XG_OK:
+        raise XgError(code, lib().xg_last_error().decode(errors="replace"))
+    return code
+
+
+def ptr(t):
+    """device pointer of a torch tensor (or None)"""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Context:
+    """xg_ctx bound to a device and a (torch) stream."""
+
+    def __init__(self, device=0, stream=None):
+        self.h = ctypes.c_void_p()
+        check(lib().xg_ctx_create(device, ctypes.byref(self.h)))
+        self.device = device
+        if stream is not None:
+            self.set_stream(stream)
+
+    def set_stream(self, stream):
+        s = stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream)
+        check(lib().xg_ctx_set_stream(self.h, ctypes.c_void_p(s)))
+
+    def sync(self):
+        check(lib().xg_sync(self.h))
+
+    def close(self):
+        if self.h:
+            lib().xg_ctx_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- thin wrappers (device tensors in, device tensors out) -------------
+    def p1_perturb(self, v, eps=1e-14):
+        check(lib().xg_p1_perturb(self.h, v.numel(), ptr(v), eps))
+
+    def mark(self, D, el2v, lset, dt=None, cut_ratio=None, bits=None, cut_list=None, n_cut=None):
+        ne = el2v.shape[0]
+        check(lib().xg_mark(self.h, D, ne, ptr(el2v), ptr(lset), ptr(dt), ptr(cut_ratio), ptr(bits),
+                            ptr(cut_list), ptr(n_cut)))
+
+    def facets_with_neighbor_types(self, f2el, a, b, bnd_val_a, bnd_val_b, use_and, out):
+        check(lib().xg_facets_with_neighbor_types(self.h, f2el.shape[0], ptr(f2el), ptr(a), ptr(b),
+                                                  int(bnd_val_a), int(bnd_val_b), int(use_and), ptr(out)))
+
+    def cut_rule_count(self, D, el2v, lset, dt, order, offsets, elems=None, nsub=None):
+        nel = elems.shape[0] if elems is not None else el2v.shape[0]
+        check(lib().xg_cut_rule_count(self.h, D, nel, ptr(elems), ptr(el2v), ptr(lset), dt, order,
+                                      ptr(offsets), ptr(nsub)))
+
+    def cut_rule_fill(self, D, el2v, coords, lset, dt, order, offsets, pts, elems=None):
+        nel = elems.shape[0] if elems is not None else el2v.shape[0]
+        check(lib().xg_cut_rule_fill(self.h, D, nel, ptr(elems), ptr(el2v), ptr(coords), ptr(lset), dt,
+                                     order, ptr(offsets), ptr(pts)))
+
+    def integrate(self, D, el2v, coords, offsets, pts, terms, total, per_elem=None, elems=None):
+        nel = elems.shape[0] if elems is not None else el2v.shape[0]
+        n = len(terms)
+        coef = (ctypes.c_double * max(n, 1))(*[t[0] for t in terms])
+        ex = (ctypes.c_int32 * max(3 * n, 1))(*[e for t in terms for e in t[1]])
+        check(lib().xg_integrate(self.h, D, nel, ptr(elems), ptr(el2v), ptr(coords), ptr(offsets),
+                                 ptr(pts), n, ctypes.cast(coef, _vp), ctypes.cast(ex, _vp),
+                                 ptr(per_elem), ptr(total)))
+
+    def elmat(self, D, p, el2v, coords, offsets, pts, mask, precision, out, elems=None):
+        nel = elems.shape[0] if elems is not None else el2v.shape[0]
+        check(lib().xg_elmat(self.h, D, p, nel, ptr(elems), ptr(el2v), ptr(coords), ptr(offsets),
+                             ptr(pts), mask, precision, ptr(out)))
+
+    def cut_pipeline(self, D, p, order, el2v, coords, lset, dts, mask, precision, out, elems=None,
+                     nel=None):
+        if nel is None:
+            nel = elems.shape[0] if elems is not None else el2v.shape[0]
+        arr = (ctypes.c_int * len(dts))(*dts)
+        check(lib().xg_cut_pipeline(self.h, D, p, order, nel, ptr(elems), ptr(el2v), ptr(coords),
+                                    ptr(lset), len(dts), ctypes.cast(arr, _vp), mask, precision, ptr(out)))

@@ -1,0 +1,116 @@
+// xg_ctx.hip -- context, error reporting and scratch management of the C ABI.
+//
+// The reference runs every hot call inside an NGSolve LocalHeap on
+// TaskManager threads (utils/ngsxstd.cpp:3-31) and reports failures with C++
+// exceptions.  Here a context binds a device + stream + scratch; failures
+// are status codes with a thread-local message, and data-dependent failures
+// detected inside kernels (the conditions on which the reference throws,
+// e.g. straightcutrule.cpp:148,193) raise a sticky device flag that
+// xg_sync() reports and clears.
+#include <stdio.h>
+
+#include <string>
+
+#include "xg_internal.hpp"
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+namespace xg {
+
+int set_error(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int check_hip(hipError_t e, const char *what) {
+  g_last_error = std::string("HIP error ") + hipGetErrorString(e) + " in " + what;
+  return XG_ERR_HIP;
+}
+
+int ensure_ws(xg_ctx *ctx, size_t bytes) {
+  if (ctx->ws_size >= bytes) return XG_OK;
+  if (ctx->ws) {
+    XG_CHECK(hipStreamSynchronize(ctx->stream));
+    XG_CHECK(hipFree(ctx->ws));
+    ctx->ws = nullptr;
+    ctx->ws_size = 0;
+  }
+  size_t sz = 1 << 20;
+  while (sz < bytes) sz <<= 1;
+  XG_CHECK(hipMalloc(&ctx->ws, sz));
+  ctx->ws_size = sz;
+  return XG_OK;
+}
+
+}  // namespace xg
+
+extern "C" const char *xg_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int xg_ctx_create(int device, xg_ctx **out) {
+  if (!out) return xg::set_error(XG_ERR_ARG, "xg_ctx_create: null out");
+  int ndev = 0;
+  XG_CHECK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return xg::set_error(XG_ERR_ARG, "xg_ctx_create: no such device");
+  XG_CHECK(hipSetDevice(device));
+  xg_ctx *c = new xg_ctx();
+  c->device = device;
+  c->stream = nullptr;
+  hipError_t e = hipMalloc(&c->d_err, sizeof(int32_t));
+  if (e != hipSuccess) {
+    delete c;
+    return xg::check_hip(e, "hipMalloc(err)");
+  }
+  e = hipHostMalloc(&c->h_err, sizeof(int32_t), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    hipFree(c->d_err);
+    delete c;
+    return xg::check_hip(e, "hipHostMalloc(err)");
+  }
+  *c->h_err = 0;
+  e = hipMemset(c->d_err, 0, sizeof(int32_t));
+  if (e != hipSuccess) {
+    hipFree(c->d_err);
+    hipHostFree(c->h_err);
+    delete c;
+    return xg::check_hip(e, "hipMemset(err)");
+  }
+  *out = c;
+  return XG_OK;
+}
+
+extern "C" int xg_ctx_destroy(xg_ctx *ctx) {
+  if (!ctx) return XG_OK;
+  hipSetDevice(ctx->device);
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  else hipDeviceSynchronize();
+  if (ctx->ws) hipFree(ctx->ws);
+  if (ctx->d_err) hipFree(ctx->d_err);
+  if (ctx->h_err) hipHostFree(ctx->h_err);
+  delete ctx;
+  return XG_OK;
+}
+
+extern "C" int xg_ctx_set_stream(xg_ctx *ctx, void *stream) {
+  if (!ctx) return xg::set_error(XG_ERR_ARG, "xg_ctx_set_stream: null ctx");
+  ctx->stream = (hipStream_t)stream;
+  return XG_OK;
+}
+
+extern "C" int xg_sync(xg_ctx *ctx) {
+  if (!ctx) return xg::set_error(XG_ERR_ARG, "xg_sync: null ctx");
+  XG_CHECK(hipMemcpyAsync(ctx->h_err, ctx->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  XG_CHECK(hipStreamSynchronize(ctx->stream));
+  const int32_t flag = *ctx->h_err;
+  if (flag) {
+    XG_CHECK(hipMemsetAsync(ctx->d_err, 0, sizeof(int32_t), ctx->stream));
+    XG_CHECK(hipStreamSynchronize(ctx->stream));
+    if (flag & 2) return xg::set_error(XG_ERR_SUBDIV, "subdivision rule: did not expect this.. -3-");
+    if (flag & 4) return xg::set_error(XG_ERR_CAPACITY, "output capacity exceeded");
+    return xg::set_error(XG_ERR_DEGENERATE,
+                         "You tried to cut a simplex with a plain geometry lset function "
+                         "(a vertex level-set value is exactly 0 next to a negative one)");
+  }
+  return XG_OK;
+}

@@ -1,0 +1,40 @@
+// xg_internal.hpp -- host-side internals shared by the C-ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/xg_abi.h"
+
+struct xg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // grow-only scratch (block sums, scan partials, staging)
+  void *ws = nullptr;
+  size_t ws_size = 0;
+  // sticky device error word: set by kernels that meet a condition on which
+  // the reference throws; read and cleared by xg_sync()
+  int32_t *d_err = nullptr;
+  int32_t *h_err = nullptr;  // pinned host mirror
+};
+
+namespace xg {
+
+int set_error(int code, const std::string &msg);
+int check_hip(hipError_t e, const char *what);
+// make sure ctx->ws holds at least `bytes`
+int ensure_ws(xg_ctx *ctx, size_t bytes);
+// exclusive scan of n int32 values in place (device), total written to
+// data[n] when `with_total` (data must then hold n+1 entries)
+int scan_exclusive_i32(xg_ctx *ctx, int32_t *data, int n, bool with_total);
+
+}  // namespace xg
+
+#define XG_CHECK(expr)                                  \
+  do {                                                  \
+    hipError_t _e = (expr);                             \
+    if (_e != hipSuccess) return xg::check_hip(_e, #expr); \
+  } while (0)
+
+#define XG_LAUNCH_CHECK() XG_CHECK(hipGetLastError())

@@ -1,0 +1,349 @@
+// xg_mark.hip -- element marking (CutInformation::Update), compaction of the
+// cut elements, P1 perturbation, facet marking, device scan.
+//
+// Reference: xfem/cutinfo.cpp:61-133 (marking), :476-538 (facets),
+//            utils/p1interpol.cpp:104-107 (perturbation).
+//
+// Layout: one thread per element, 256-thread blocks (4 waves).  The element
+// row of el2v is read as one int4 (tets) -- 16 B per lane, coalesced -- and
+// the D+1 level-set values are gathered (FP64, L2-resident for meshes of a
+// few million vertices).  The 8 COMBINED_DOMAIN_TYPE bit arrays are built
+// with one wave ballot per array: lane 0 stores the 64-bit mask, i.e. two
+// uint32 words, so no atomics touch the bit arrays.
+#include "xg_device.hpp"
+#include "xg_internal.hpp"
+
+using namespace xg;
+
+namespace {
+
+constexpr int BLK = 256;
+
+template <int D>
+__device__ __forceinline__ void load_lset(const int32_t *__restrict__ el2v,
+                                          const double *__restrict__ lset, int e, double *l) {
+  if constexpr (D == 3) {
+    const int4 v = reinterpret_cast<const int4 *>(el2v)[e];
+    l[0] = lset[v.x];
+    l[1] = lset[v.y];
+    l[2] = lset[v.z];
+    l[3] = lset[v.w];
+  } else {
+#pragma unroll
+    for (int k = 0; k <= D; ++k) l[k] = lset[el2v[(size_t)e * (D + 1) + k]];
+  }
+}
+
+// part_vol[NEG], part_vol[POS] from the order-0 rules (cutinfo.cpp:87-101):
+// the weight sums of StraightCutIntegrationRule(..., np, order 0).  Returns
+// false when the reference would throw.
+template <int D>
+__device__ __forceinline__ bool part_volumes(const double *l, double *pv) {
+  const double w0 = xg_rule_data_d[xg_rule_off_d[D][1]][3];  // order-0 weight = |ref simplex|
+  pv[NEG] = 0.0;
+  pv[POS] = 0.0;
+  const int eldom = check_if_straight_cut<D>(l);
+  if (eldom != IF) {
+    pv[eldom] = w0;
+    return true;
+  }
+  bool ok = true;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int np = t == 0 ? POS : NEG;
+    Decomp dc;
+    if (!decompose<D>(l, np, dc)) {
+      ok = false;
+      continue;
+    }
+    double P[D + 1 + 4][3];
+    cut_point_array<D>(l, dc, P);
+    double s = 0.0;
+    for (int k = 0; k < dc.ns; ++k) s += w0 * simplex_volume(dc.nv, dc.v[k], P);
+    pv[np] = s;
+  }
+  return ok;
+}
+
+template <int D>
+__global__ __launch_bounds__(BLK) void k_mark(int ne, const int32_t *__restrict__ el2v,
+                                              const double *__restrict__ lset,
+                                              uint8_t *__restrict__ dt_out,
+                                              double *__restrict__ cut_ratio,
+                                              uint32_t *__restrict__ bits, int nw,
+                                              int32_t *__restrict__ blk_if, int32_t *err) {
+  __shared__ int s_cnt[BLK / 64];
+  const int e = blockIdx.x * BLK + threadIdx.x;
+  const bool valid = e < ne;
+  int cdt = 0;
+  if (valid) {
+    double l[D + 1];
+    load_lset<D>(el2v, lset, e, l);
+    double pv[2];
+    if (!part_volumes<D>(l, pv)) atomicOr(err, 1);
+    if (cut_ratio) cut_ratio[e] = pv[NEG] / (pv[NEG] + pv[POS]);
+    // cutinfo.cpp:104-124
+    if (pv[NEG] > 0.0)
+      cdt = (pv[POS] > 0.0) ? 4 : 1;
+    else
+      cdt = 2;
+    if (dt_out) dt_out[e] = (uint8_t)(cdt == 4 ? IF : (cdt == 1 ? NEG : POS));
+  }
+  const int lane = threadIdx.x & 63;
+  const int wave_e0 = e - lane;  // multiple of 64
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    // combined arrays (cutinfo.cpp:72-78,127-132): bit set iff (c & cdt) != 0,
+    // CDOM_ANY everywhere, CDOM_NO nowhere
+    const bool on = valid && ((c == 7) || (c & cdt));
+    const unsigned long long m = __ballot(on);
+    if (lane == 0) {
+      const int w = wave_e0 >> 5;
+      uint32_t *row = bits + (size_t)c * nw;
+      if (w < nw) row[w] = (uint32_t)(m & 0xffffffffull);
+      if (w + 1 < nw) row[w + 1] = (uint32_t)(m >> 32);
+    }
+  }
+  const unsigned long long mif = __ballot(valid && cdt == 4);
+  if (lane == 0) s_cnt[threadIdx.x >> 6] = __popcll(mif);
+  __syncthreads();
+  if (threadIdx.x == 0 && blk_if) {
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < BLK / 64; ++k) s += s_cnt[k];
+    blk_if[blockIdx.x] = s;
+  }
+}
+
+// writes the IF elements of block b to cut_list[blk_off[b] ...] in ascending
+// order
+__global__ __launch_bounds__(BLK) void k_compact(int ne, const uint32_t *__restrict__ if_bits,
+                                                 const int32_t *__restrict__ blk_off,
+                                                 int32_t *__restrict__ cut_list) {
+  __shared__ int s_cnt[BLK / 64];
+  const int e = blockIdx.x * BLK + threadIdx.x;
+  const bool on = e < ne && ((if_bits[e >> 5] >> (e & 31)) & 1u);
+  const unsigned long long m = __ballot(on);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                              __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+  if (lane == 0) s_cnt[wv] = __popcll(m);
+  __syncthreads();
+  int off = blk_off[blockIdx.x];
+  for (int k = 0; k < wv; ++k) off += s_cnt[k];
+  if (on) cut_list[off + below] = e;
+}
+
+__global__ void k_perturb(int n, double *v, double eps) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const double x = v[i];
+    if (fabs(x) < eps) v[i] = eps;
+  }
+}
+
+// ---------------------------------------------------------------- scan
+constexpr int SCAN_BLK = 1024;
+constexpr int SCAN_ITEMS = 4;
+constexpr int SCAN_TILE = SCAN_BLK * SCAN_ITEMS;
+
+__device__ __forceinline__ int block_exclusive_scan(int v, int *s_wave, int &total) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_wave[wv] = x;
+  __syncthreads();
+  if (wv == 0) {
+    const int nwv = blockDim.x >> 6;
+    int t = lane < nwv ? s_wave[lane] : 0;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(t, o, 64);
+      if (lane >= o) t += y;
+    }
+    if (lane < nwv) s_wave[lane] = t;
+  }
+  __syncthreads();
+  total = s_wave[(blockDim.x >> 6) - 1];
+  const int wave_off = wv > 0 ? s_wave[wv - 1] : 0;
+  __syncthreads();
+  return wave_off + x - v;
+}
+
+// pass 1: scan each tile locally, write tile sums
+__global__ __launch_bounds__(SCAN_BLK) void k_scan_tiles(int32_t *data, int n, int32_t *sums) {
+  __shared__ int s_wave[SCAN_BLK / 64];
+  const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+  int v[SCAN_ITEMS], s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    v[k] = (base + k < n) ? data[base + k] : 0;
+    s += v[k];
+  }
+  int total;
+  int off = block_exclusive_scan(s, s_wave, total);
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    if (base + k < n) data[base + k] = off;
+    off += v[k];
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// pass 2 (single block): exclusive scan of the tile sums, serial over
+// chunks of SCAN_BLK
+__global__ __launch_bounds__(SCAN_BLK) void k_scan_sums(int32_t *sums, int nt, int32_t *total_out) {
+  __shared__ int s_wave[SCAN_BLK / 64];
+  int carry = 0;
+  for (int base = 0; base < nt; base += SCAN_BLK) {
+    const int i = base + threadIdx.x;
+    const int v = i < nt ? sums[i] : 0;
+    int total;
+    const int off = block_exclusive_scan(v, s_wave, total);
+    if (i < nt) sums[i] = carry + off;
+    carry += total;
+  }
+  if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+__global__ __launch_bounds__(SCAN_BLK) void k_scan_add(int32_t *data, int n, const int32_t *sums) {
+  const int add = sums[blockIdx.x];
+  const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k)
+    if (base + k < n) data[base + k] += add;
+}
+
+// ------------------------------------------------- facets (cutinfo.cpp:476-538)
+__device__ __forceinline__ bool bit_of(const uint32_t *b, int e) { return (b[e >> 5] >> (e & 31)) & 1u; }
+
+__global__ void k_facets(int nf, const int32_t *__restrict__ f2el, const uint32_t *__restrict__ a,
+                         const uint32_t *__restrict__ b, int bva, int bvb, int use_and,
+                         uint32_t *__restrict__ out, int nwf) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  bool r = false;
+  if (f < nf) {
+    const int e0 = f2el[2 * f], e1 = f2el[2 * f + 1];
+    bool a0, a1, b0, b1;
+    if (e1 < 0) {
+      // boundary facet: the missing neighbour takes the boundary values
+      a0 = bit_of(a, e0);
+      b0 = bit_of(b, e0);
+      a1 = bva != 0;
+      b1 = bvb != 0;
+    } else {
+      a0 = bit_of(a, e0);
+      a1 = bit_of(a, e1);
+      b0 = bit_of(b, e0);
+      b1 = bit_of(b, e1);
+    }
+    if (use_and)
+      r = (a0 && b1) || (a1 && b0);
+    else
+      r = (a0 || b1) || (a1 || b0);
+  }
+  const unsigned long long m = __ballot(r);
+  const int lane = threadIdx.x & 63;
+  if (lane == 0) {
+    const int w = (f - lane) >> 5;
+    if (w < nwf) out[w] = (uint32_t)(m & 0xffffffffull);
+    if (w + 1 < nwf) out[w + 1] = (uint32_t)(m >> 32);
+  }
+}
+
+}  // namespace
+
+namespace xg {
+
+int scan_exclusive_i32(xg_ctx *ctx, int32_t *data, int n, bool with_total) {
+  if (n <= 0) {
+    if (with_total) XG_CHECK(hipMemsetAsync(data, 0, sizeof(int32_t), ctx->stream));
+    return XG_OK;
+  }
+  const int nt = (n + SCAN_TILE - 1) / SCAN_TILE;
+  int r = ensure_ws(ctx, sizeof(int32_t) * (size_t)(nt + 1));
+  if (r) return r;
+  int32_t *sums = (int32_t *)ctx->ws;
+  k_scan_tiles<<<nt, SCAN_BLK, 0, ctx->stream>>>(data, n, sums);
+  XG_LAUNCH_CHECK();
+  k_scan_sums<<<1, SCAN_BLK, 0, ctx->stream>>>(sums, nt, with_total ? data + n : nullptr);
+  XG_LAUNCH_CHECK();
+  k_scan_add<<<nt, SCAN_BLK, 0, ctx->stream>>>(data, n, sums);
+  XG_LAUNCH_CHECK();
+  return XG_OK;
+}
+
+}  // namespace xg
+
+extern "C" int xg_p1_perturb(xg_ctx *ctx, int n, double *v, double eps) {
+  if (!ctx || n < 0 || (n > 0 && !v)) return set_error(XG_ERR_ARG, "xg_p1_perturb: bad argument");
+  if (n == 0) return XG_OK;
+  k_perturb<<<(n + 255) / 256, 256, 0, ctx->stream>>>(n, v, eps);
+  XG_LAUNCH_CHECK();
+  return XG_OK;
+}
+
+extern "C" int xg_mark(xg_ctx *ctx, int D, int ne, const int32_t *el2v, const double *lset_v,
+                       uint8_t *dt, double *cut_ratio, uint32_t *bits, int32_t *cut_list,
+                       int32_t *n_cut) {
+  if (!ctx || D < 1 || D > 3 || ne < 0 || !bits || (ne > 0 && (!el2v || !lset_v)))
+    return set_error(XG_ERR_ARG, "xg_mark: bad argument");
+  const int nw = (ne + 31) / 32;
+  if (ne == 0) {
+    if (n_cut) XG_CHECK(hipMemsetAsync(n_cut, 0, sizeof(int32_t), ctx->stream));
+    return XG_OK;
+  }
+  const int nblk = (ne + BLK - 1) / BLK;
+  const bool compact = cut_list || n_cut;
+  // block counts live after the scan scratch: [0, nblk] counts (+ total)
+  int32_t *blk = nullptr;
+  if (compact) {
+    const size_t scan_ws = sizeof(int32_t) * (size_t)((nblk + 1 + SCAN_TILE - 1) / SCAN_TILE + 1);
+    int r = ensure_ws(ctx, scan_ws + sizeof(int32_t) * (size_t)(nblk + 1) + 256);
+    if (r) return r;
+    blk = (int32_t *)((char *)ctx->ws + ((scan_ws + 255) / 256) * 256);
+  }
+  switch (D) {
+    case 1: k_mark<1><<<nblk, BLK, 0, ctx->stream>>>(ne, el2v, lset_v, dt, cut_ratio, bits, nw, blk, ctx->d_err); break;
+    case 2: k_mark<2><<<nblk, BLK, 0, ctx->stream>>>(ne, el2v, lset_v, dt, cut_ratio, bits, nw, blk, ctx->d_err); break;
+    default: k_mark<3><<<nblk, BLK, 0, ctx->stream>>>(ne, el2v, lset_v, dt, cut_ratio, bits, nw, blk, ctx->d_err); break;
+  }
+  XG_LAUNCH_CHECK();
+  if (!compact) return XG_OK;
+  // scan of block counts: blk[nblk] = total.  The scan scratch is the start
+  // of ctx->ws, disjoint from blk.
+  {
+    const int n = nblk;
+    const int nt = (n + SCAN_TILE - 1) / SCAN_TILE;
+    int32_t *sums = (int32_t *)ctx->ws;
+    k_scan_tiles<<<nt, SCAN_BLK, 0, ctx->stream>>>(blk, n, sums);
+    XG_LAUNCH_CHECK();
+    k_scan_sums<<<1, SCAN_BLK, 0, ctx->stream>>>(sums, nt, blk + n);
+    XG_LAUNCH_CHECK();
+    k_scan_add<<<nt, SCAN_BLK, 0, ctx->stream>>>(blk, n, sums);
+    XG_LAUNCH_CHECK();
+  }
+  if (n_cut) XG_CHECK(hipMemcpyAsync(n_cut, blk + nblk, sizeof(int32_t), hipMemcpyDeviceToDevice, ctx->stream));
+  if (cut_list) {
+    k_compact<<<nblk, BLK, 0, ctx->stream>>>(ne, bits + (size_t)4 * nw, blk, cut_list);
+    XG_LAUNCH_CHECK();
+  }
+  return XG_OK;
+}
+
+extern "C" int xg_facets_with_neighbor_types(xg_ctx *ctx, int nf, const int32_t *f2el,
+                                             const uint32_t *a, const uint32_t *b, int bnd_val_a,
+                                             int bnd_val_b, int use_and, uint32_t *out) {
+  if (!ctx || nf < 0 || (nf > 0 && (!f2el || !a || !b || !out)))
+    return set_error(XG_ERR_ARG, "xg_facets_with_neighbor_types: bad argument");
+  if (nf == 0) return XG_OK;
+  const int nwf = (nf + 31) / 32;
+  k_facets<<<(nf + 255) / 256, 256, 0, ctx->stream>>>(nf, f2el, a, b, bnd_val_a, bnd_val_b, use_and,
+                                                      out, nwf);
+  XG_LAUNCH_CHECK();
+  return XG_OK;
+}

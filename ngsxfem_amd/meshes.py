@@ -1,0 +1,172 @@
+"""Structured simplex meshes (stand-ins for ngsolve.meshes / netgen meshes).
+
+The reference's tests build their inputs with ``ngsolve.meshes.Make1DMesh``,
+``MakeStructured2DMesh`` and ``MakeStructured3DMesh`` (e.g.
+tests/pytests/test_straightcutrule.py:14,44,113,195).  NGSolve is not part of
+this build, so these generators produce the same geometry with this
+project's own numbering:
+
+* 1D: vertices x_i = i/n, segments (i, i+1).
+* 2D: each cell [i,i+1]x[j,j+1] is split along its (0,0)-(1,1) diagonal into
+  (v00, v10, v11) and (v00, v11, v01).
+* 3D: each cube is split into the 6 Kuhn tetrahedra along its main diagonal
+  (v000 -> v111), one per permutation of the axes.
+
+Boundary elements (segments in 2D, triangles in 3D, points in 1D) are
+listed as well; they carry the same vertex-array layout as the volume
+elements one dimension lower.
+"""
+import itertools
+
+import numpy as np
+
+
+class Mesh:
+    """Simplex mesh held as flat arrays.
+
+    coords : (nv, D) float64 vertex coordinates
+    el2v   : (ne, D+1) int32 element -> vertex table
+    bnd2v  : (nse, D) int32 boundary element -> vertex table
+    """
+
+    def __init__(self, coords, el2v, bnd2v=None):
+        self.coords = np.ascontiguousarray(coords, dtype=np.float64)
+        if self.coords.ndim == 1:
+            self.coords = self.coords[:, None]
+        self.el2v = np.ascontiguousarray(el2v, dtype=np.int32)
+        self.dim = self.coords.shape[1]
+        self._bnd2v = None if bnd2v is None else np.ascontiguousarray(bnd2v, dtype=np.int32)
+        self._facets = None
+
+    @property
+    def bnd2v(self):
+        """boundary elements; computed from the facets when not given"""
+        if self._bnd2v is None:
+            if self.dim == 1:
+                self._bnd2v = np.zeros((0, 1), np.int32)
+            else:
+                f2v, _, f2el = self.facets()
+                self._bnd2v = np.ascontiguousarray(f2v[f2el[:, 1] < 0], dtype=np.int32)
+        return self._bnd2v
+
+    @bnd2v.setter
+    def bnd2v(self, v):
+        self._bnd2v = np.ascontiguousarray(v, dtype=np.int32)
+
+    @property
+    def nv(self):
+        return self.coords.shape[0]
+
+    @property
+    def ne(self):
+        return self.el2v.shape[0]
+
+    @property
+    def nse(self):
+        return self.bnd2v.shape[0]
+
+    def GetNE(self, vb="VOL"):
+        return self.ne if str(vb).endswith("VOL") else self.nse
+
+    def facets(self):
+        """(facet2v (nf, D) sorted vertex tuples, el2f (ne, D+1), f2el (nf, 2)).
+
+        Facet k of an element is the one opposite local vertex k.  f2el holds
+        the two neighbouring elements (-1 on the boundary)."""
+        if self._facets is None:
+            D = self.dim
+            ne = self.ne
+            loc = []
+            for k in range(D + 1):
+                cols = [c for c in range(D + 1) if c != k]
+                loc.append(np.sort(self.el2v[:, cols], axis=1))
+            allf = np.concatenate(loc, axis=0)  # (D+1)*ne rows, local facet k block k
+            uniq, inv = np.unique(allf, axis=0, return_inverse=True)
+            inv = inv.reshape(-1)
+            el2f = inv.reshape(D + 1, ne).T.copy().astype(np.int32)
+            nf = uniq.shape[0]
+            f2el = -np.ones((nf, 2), np.int64)
+            elid = np.tile(np.arange(ne), D + 1)
+            order = np.argsort(inv, kind="stable")
+            sinv = inv[order]
+            sel = elid[order]
+            first = np.ones(len(sinv), bool)
+            first[1:] = sinv[1:] != sinv[:-1]
+            f2el[sinv[first], 0] = sel[first]
+            second = ~first
+            f2el[sinv[second], 1] = sel[second]
+            self._facets = (uniq.astype(np.int32), el2f, f2el.astype(np.int32))
+        return self._facets
+
+    def GetNFacets(self):
+        return self.facets()[0].shape[0]
+
+
+def Make1DMesh(n, mapping=None):
+    x = np.arange(n + 1, dtype=np.float64) / n
+    if mapping is not None:
+        x = np.array([mapping(v) for v in x], dtype=np.float64)
+    el = np.stack([np.arange(n), np.arange(1, n + 1)], axis=1)
+    bnd = np.array([[0], [n]], np.int32)
+    return Mesh(x[:, None], el, bnd)
+
+
+def MakeStructured2DMesh(quads=False, nx=10, ny=10, mapping=None):
+    if quads:
+        raise NotImplementedError("quadrilateral meshes are outside the simplex hot path")
+    i, j = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), indexing="ij")
+    coords = np.stack([i.ravel() / nx, j.ravel() / ny], axis=1).astype(np.float64)
+    if mapping is not None:
+        coords = np.array([mapping(*p) for p in coords], dtype=np.float64)
+
+    def vid(a, b):
+        return a * (ny + 1) + b
+
+    ci, cj = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    ci = ci.ravel()
+    cj = cj.ravel()
+    v00, v10, v11, v01 = vid(ci, cj), vid(ci + 1, cj), vid(ci + 1, cj + 1), vid(ci, cj + 1)
+    t1 = np.stack([v00, v10, v11], axis=1)
+    t2 = np.stack([v00, v11, v01], axis=1)
+    el = np.stack([t1, t2], axis=1).reshape(-1, 3)
+    bnd = []
+    for a in range(nx):
+        bnd.append((vid(a, 0), vid(a + 1, 0)))
+        bnd.append((vid(a + 1, ny), vid(a, ny)))
+    for b in range(ny):
+        bnd.append((vid(nx, b), vid(nx, b + 1)))
+        bnd.append((vid(0, b + 1), vid(0, b)))
+    return Mesh(coords, el, np.array(bnd, np.int32))
+
+
+_KUHN = [p for p in itertools.permutations(range(3))]
+
+
+def MakeStructured3DMesh(hexes=False, nx=10, ny=None, nz=None, mapping=None):
+    if hexes:
+        raise NotImplementedError("hexahedral meshes are outside the simplex hot path")
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    i, j, k = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    coords = np.stack([i.ravel() / nx, j.ravel() / ny, k.ravel() / nz], axis=1).astype(np.float64)
+    if mapping is not None:
+        coords = np.array([mapping(*p) for p in coords], dtype=np.float64)
+
+    def vid(a, b, c):
+        return (a * (ny + 1) + b) * (nz + 1) + c
+
+    ci, cj, ck = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    ci = ci.ravel()
+    cj = cj.ravel()
+    ck = ck.ravel()
+    tets = []
+    for perm in _KUHN:
+        off = np.zeros(3, np.int64)
+        verts = [vid(ci, cj, ck)]
+        for ax in perm:
+            off = off.copy()
+            off[ax] += 1
+            verts.append(vid(ci + off[0], cj + off[1], ck + off[2]))
+        tets.append(np.stack(verts, axis=1))
+    el = np.stack(tets, axis=1).reshape(-1, 4)
+    return Mesh(coords, el)

@@ -1,0 +1,60 @@
+"""Synthetic workloads shared by tests and bench.py (inputs only, no compute).
+
+C1: [-1.5,1.5]^2, 60x60 cells x 2 triangles, phi = |x| - 1 (P1, perturbed).
+C2: unit cube, n^3 cells x 6 Kuhn tets, phi = |x - c| - 0.35 + 0.01 sin(7x)
+    sin(7y) sin(7z) (P1, perturbed); n = 64 is the full size (1,572,864 tets).
+"""
+import math
+
+import numpy as np
+
+from ngsxfem_amd.meshes import MakeStructured2DMesh, MakeStructured3DMesh
+
+
+def p1_perturb(v, eps=1e-14):
+    v = np.array(v, dtype=np.float64)
+    v[np.abs(v) < eps] = eps
+    return v
+
+
+def config1():
+    mesh = MakeStructured2DMesh(nx=60, ny=60, mapping=lambda x, y: (-1.5 + 3 * x, -1.5 + 3 * y))
+    x, y = mesh.coords[:, 0], mesh.coords[:, 1]
+    lset = p1_perturb(np.sqrt(x * x + y * y) - 1.0)
+    return mesh, lset
+
+
+def sphere_lset(coords):
+    x, y, z = coords[:, 0], coords[:, 1], coords[:, 2]
+    r = np.sqrt((x - 0.5) ** 2 + (y - 0.5) ** 2 + (z - 0.5) ** 2)
+    return p1_perturb(r - 0.35 + 0.01 * np.sin(7 * x) * np.sin(7 * y) * np.sin(7 * z))
+
+
+def config2(n=64):
+    mesh = MakeStructured3DMesh(nx=n, ny=n, nz=n)
+    return mesh, sphere_lset(mesh.coords)
+
+
+def random_quadric_p2(nel, seed=42):
+    """C4 inputs: P2 nodal values (10 per tet, this project's dof order) of
+    x^T A x + b^T x + c on the reference tet, A sym N(0,1), b N(0,1), c shifted
+    by the median of the nodal values so the tet is cut."""
+    from itertools import product  # noqa: F401
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((nel, 3, 3))
+    A = 0.5 * (A + A.transpose(0, 2, 1))
+    b = rng.standard_normal((nel, 3))
+    nodes = p2_nodes()
+    vals = np.einsum("ki,nij,kj->nk", nodes, A, nodes) + b @ nodes.T
+    c = -np.median(vals, axis=1)
+    return vals + c[:, None]
+
+
+def p2_nodes():
+    """reference coordinates of the 10 P2 Lagrange nodes in dof order"""
+    V = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [0, 0, 0]], float)
+    nodes = [V[k] for k in range(4)]
+    for i in range(4):
+        for j in range(i + 1, 4):
+            nodes.append(0.5 * (V[i] + V[j]))
+    return np.array(nodes)
