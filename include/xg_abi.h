@@ -127,6 +127,22 @@ int xg_cut_pipeline(xg_ctx *ctx, int D, int p, int order, int nel, const int32_t
                     const int32_t *el2v, const double *coords, const double *lset_v,
                     int ndt, const int *dts, int mask, int precision, void *out);
 
+/* ---- a17: subdivision rule for non-P1 level sets ----------------------
+   (cutint/xintegration.cpp:1626-1754 CutIntegrationRule with subdivlvl = L,
+   NumericalIntegrationStrategy :617-1063, CutSimplex<3>::MakeQuad :1084-1388)
+   on tetrahedra whose level set is given by its Lagrange nodal values of
+   degree lset_order (1 or 2; lset_nodal: nel x 4 or nel x 10, this
+   library's dof order).  The rule (thousands of points at L = 4) is
+   contracted on the fly into fixed-node weights
+       W_k = sum_points w psi_k(x),  psi = degree-q Lagrange basis (q == 4),
+   which integrate every polynomial of degree <= q exactly like the rule.
+   out_neg / out_if (nullable): nel x 35 FP64; finv (nullable): nel x 9
+   element Jacobian inverses for the interface weights (identity if null);
+   dom (nullable): uint8[nel] element domain (NEG/POS/IF).  L <= 5. */
+int xg_subdiv_weights(xg_ctx *ctx, int nel, const double *lset_nodal, int lset_order, int L,
+                      int order, int q, const double *finv, double *out_neg, double *out_if,
+                      uint8_t *dom);
+
 #ifdef __cplusplus
 }
 #endif

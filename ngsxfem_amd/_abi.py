@@ -41,6 +41,7 @@ _SIGS = {
     "xg_integrate": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp],
     "xg_elmat": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "xg_cut_pipeline": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _i, _vp],
+    "xg_subdiv_weights": [_vp, _i, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp],
 }
 
 
@@ -149,6 +150,12 @@ class Context:
         nel = elems.shape[0] if elems is not None else el2v.shape[0]
         check(lib().xg_elmat(self.h, D, p, nel, ptr(elems), ptr(el2v), ptr(coords), ptr(offsets),
                              ptr(pts), mask, precision, ptr(out)))
+
+    def subdiv_weights(self, lset_nodal, lset_order, L, order, q, out_neg=None, out_if=None, dom=None,
+                       finv=None):
+        nel = lset_nodal.shape[0]
+        check(lib().xg_subdiv_weights(self.h, nel, ptr(lset_nodal), lset_order, L, order, q, ptr(finv),
+                                      ptr(out_neg), ptr(out_if), ptr(dom)))
 
     def cut_pipeline(self, D, p, order, el2v, coords, lset, dts, mask, precision, out, elems=None,
                      nel=None):

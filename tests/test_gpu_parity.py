@@ -258,3 +258,52 @@ def test_facets_with_neighbor_types(dev):
     for f in np.nonzero(ref)[0]:
         refbits[f >> 5] |= np.uint32(1 << (f & 31))
     assert np.array_equal(got, refbits)
+
+
+@pytest.mark.parametrize("L", [0, 1, 2, 3, 4])
+def test_subdiv_weights_parity(orc, dev, L):
+    """GPU subdivision rule (contracted to P4 lattice weights) vs the FP64
+    oracle: identical cut decisions (element domain), weights to 1e-12
+    relative to the element's weight scale (different summation order)."""
+    torch, _abi, ctx = dev
+    nel = 256 if L <= 3 else 64
+    vals = W.random_quadric_p2(nel, seed=11 + L)
+    rng = np.random.default_rng(L)
+    # random affine maps for the interface-weight transform
+    Fs = np.eye(3)[None] + 0.3 * rng.standard_normal((nel, 3, 3))
+    Finv = np.linalg.inv(Fs)
+    ref_n = np.zeros((nel, 35))
+    ref_i = np.zeros((nel, 35))
+    ref_dom = np.zeros(nel, np.uint8)
+    for e in range(nel):
+        d, mn, mi = orc.subdiv_moments3(vals[e], 2, L, 4, 4, Finv=Finv[e])
+        ref_n[e], ref_i[e], ref_dom[e] = mn, mi, d
+    on = torch.zeros(nel * 35, dtype=torch.float64, device="cuda")
+    oi = torch.zeros(nel * 35, dtype=torch.float64, device="cuda")
+    dom = torch.zeros(nel, dtype=torch.uint8, device="cuda")
+    ctx.subdiv_weights(up(torch, vals), 2, L, 4, 4, on, oi, dom, finv=up(torch, Finv.reshape(nel, 9)))
+    ctx.sync()
+    gn = on.cpu().numpy().reshape(nel, 35)
+    gi = oi.cpu().numpy().reshape(nel, 35)
+    assert np.array_equal(dom.cpu().numpy(), ref_dom)
+    sn = np.abs(ref_n).max(axis=1, keepdims=True) + 1e-300
+    si = np.abs(ref_i).max(axis=1, keepdims=True) + 1e-300
+    assert (np.abs(gn - ref_n) / sn).max() < 1e-12
+    assert (np.abs(gi - ref_i) / si).max() < 1e-12
+
+
+def test_subdiv_p1_levelset_equals_straight_rule(orc, dev):
+    """P1 nodal input: moments equal the straight cut rule's (any L)"""
+    torch, _abi, ctx = dev
+    rng = np.random.default_rng(5)
+    l = rng.standard_normal((128, 4))
+    l = l[(l < 0).any(1) & (l > 0).any(1)]
+    nel = l.shape[0]
+    on = torch.zeros(nel * 35, dtype=torch.float64, device="cuda")
+    ctx.subdiv_weights(up(torch, l), 1, 2, 4, 4, on, None, None)
+    ctx.sync()
+    g = on.cpu().numpy().reshape(nel, 35)
+    for e in range(nel):
+        p, w, _ = orc.straight_cut_rule(3, l[e], NEG, 4)
+        psi = np.array([orc.h1_shape(3, 4, x)[0] for x in p])
+        np.testing.assert_allclose(g[e], psi.T @ w, rtol=0, atol=1e-13)
