@@ -143,6 +143,38 @@ int xg_subdiv_weights(xg_ctx *ctx, int nel, const double *lset_nodal, int lset_o
                       int order, int q, const double *finv, double *out_neg, double *out_if,
                       uint8_t *dom);
 
+/* ---- element matrices on a fixed-node rule ----------------------------
+   (symboliccutbfi.cpp:76-304 on the rule {(xi_k, W_k)}: xi_k = the 35 nodes of
+   the degree-4 Lagrange lattice of the reference tet, W_k per element from
+   xg_subdiv_weights or the learned rule).  coords == NULL: reference tets
+   (identity map).  p <= 2 (P2 mass + stiffness integrands have degree <= 4).
+   out: nel x ndof x ndof. */
+int xg_elmat_weights(xg_ctx *ctx, int p, int q, int nel, const int32_t *elems, const int32_t *el2v,
+                     const double *coords, const double *weights, int mask, int precision, void *out);
+
+/* ---- learned rule: MLP I -> H -> H -> O (tanh), FP32 parameters --------
+   No reference counterpart: the rebuild's learned quadrature (predicts the
+   fixed-node weights of the NEG part of a P2-cut tet from its normalised
+   nodal values; trained on xg_subdiv_weights).
+   params layout: W1[H][I], b1[H], W2[H][H], b2[H], W3[O][H], b3[O]. */
+#define XG_PREC_BF16 3   /* plain BF16 MFMA, FP32 accumulation */
+int xg_mlp_nparams(int I, int H, int O);
+/* precision XG_PREC_FP32 (VALU; work: 2*B*H floats) or XG_PREC_BF16 (fused
+   MFMA kernel, I = 10, H = 128, O <= 48; work unused). */
+int xg_mlp_forward(xg_ctx *ctx, int I, int H, int O, const float *params, int B, const float *x,
+                   float *y, float *work, int precision);
+/* One Adam step (FP32 forward + backward); loss: device float[1]; grad
+   (nullable) receives the gradient; work: 4*B*H + 2*B*O floats.  lr <= 0
+   skips the update (gradient only). */
+int xg_mlp_train_step(xg_ctx *ctx, int I, int H, int O, float *params, float *adam_m, float *adam_v,
+                      int step, float lr, int B, const float *x, const float *target, float *loss,
+                      float *grad, float *work);
+/* Moment-residual check: rel[e] = |scale*pred - ref|_2 / |ref|_2 (FP64);
+   accept[e] = rel <= thr; out = accepted ? scale*pred : ref (fallback to the
+   plain rule).  pred: FP32 nel x nq, ref: FP64 nel x nq; outputs nullable. */
+int xg_moment_residual(xg_ctx *ctx, int nel, int nq, const float *pred, double scale, const double *ref,
+                       double thr, double *rel, uint8_t *accept, double *out);
+
 #ifdef __cplusplus
 }
 #endif

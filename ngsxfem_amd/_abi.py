@@ -13,7 +13,7 @@ LIBPATH = os.path.join(HERE, "libxg.so")
 
 XG_OK = 0
 XG_NEG, XG_POS, XG_IF = 0, 1, 2
-XG_PREC_FP64, XG_PREC_FP32, XG_PREC_BF16X3 = 0, 1, 2
+XG_PREC_FP64, XG_PREC_FP32, XG_PREC_BF16X3, XG_PREC_BF16 = 0, 1, 2, 3
 XG_FORM_MASS, XG_FORM_LAPLACE = 1, 2
 
 _vp = ctypes.c_void_p
@@ -42,6 +42,11 @@ _SIGS = {
     "xg_elmat": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "xg_cut_pipeline": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _i, _vp],
     "xg_subdiv_weights": [_vp, _i, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp],
+    "xg_elmat_weights": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _vp],
+    "xg_mlp_nparams": [_i, _i, _i],
+    "xg_mlp_forward": [_vp, _i, _i, _i, _vp, _i, _vp, _vp, _vp, _i],
+    "xg_mlp_train_step": [_vp, _i, _i, _i, _vp, _vp, _vp, _i, ctypes.c_float, _i, _vp, _vp, _vp, _vp, _vp],
+    "xg_moment_residual": [_vp, _i, _i, _vp, ctypes.c_double, _vp, ctypes.c_double, _vp, _vp, _vp],
 }
 
 

@@ -1,0 +1,594 @@
+// python_xfem.cpp -- pybind11 module `xfem` of the MI355X build: the host
+// objects of the cut-element path over the C ABI (include/xg_abi.h).
+//
+// Mirrors the reference bindings:
+//   CutInfo                      xfem/python_xfem.cpp:47-199 (CutInformation,
+//                                xfem/cutinfo.cpp:14-133, 263-284)
+//   GetFacetsWithNeighborTypes   xfem/python_xfem.cpp:346-405 (cutinfo.cpp:476-538)
+//   IntegrateX                   cutint/python_cutint.cpp:21-225
+//   InterpolateToP1 (perturb)    utils/python_utils.cpp:134,166 (p1interpol.cpp:104-107)
+//   enums                        python/python_ngsxfem.cpp:15-46
+// plus CutElementMatrices, the element-matrix kernel behind
+// SymbolicCutBFI / dCut bilinear forms (xfem/symboliccutbfi.cpp:37-486).
+//
+// NGSolve is not part of this build: meshes, grid functions and coefficient
+// functions are the lightweight Python stand-ins of ngsxfem_amd.xfem; this
+// module reads their numpy arrays.  All heavy work runs in HIP kernels;
+// there is no CPU fallback.
+#include <hip/hip_runtime_api.h>
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <array>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/xg_abi.h"
+
+namespace py = pybind11;
+
+namespace {
+
+enum DOMAIN_TYPE { NEG = 0, POS = 1, IF = 2 };
+enum COMBINED_DOMAIN_TYPE { CDOM_NO = 0, CDOM_NEG = 1, CDOM_POS = 2, CDOM_UNCUT = 3, CDOM_IF = 4,
+                            CDOM_HASNEG = 5, CDOM_HASPOS = 6, CDOM_ANY = 7 };
+enum SWAP_DIMENSIONS_POLICY { FIRST_ALLOWED = 0, FIND_OPTIMAL = 1, ALWAYS_NONE = 2 };
+enum VorB { VOL = 0, BND = 1 };
+
+int to_cdt(DOMAIN_TYPE a) { return a == NEG ? CDOM_NEG : (a == POS ? CDOM_POS : CDOM_IF); }
+
+void check(int rc) {
+  if (rc != XG_OK) throw std::runtime_error(std::string("xg: ") + xg_last_error());
+}
+void hipcheck(hipError_t e, const char *what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+xg_ctx *ctx() {
+  static thread_local xg_ctx *c = nullptr;
+  if (!c) check(xg_ctx_create(0, &c));
+  return c;
+}
+
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t n_) { resize(n_); }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() {
+    if (p) hipFree(p);
+  }
+  void resize(size_t n_) {
+    if (n_ <= n && p) return;
+    if (p) hipFree(p);
+    p = nullptr;
+    n = n_;
+    if (n_) hipcheck(hipMalloc(&p, sizeof(T) * n_), "hipMalloc");
+  }
+  void upload(const T *h, size_t cnt) {
+    resize(cnt);
+    if (cnt) hipcheck(hipMemcpy(p, h, sizeof(T) * cnt, hipMemcpyHostToDevice), "upload");
+  }
+  void download(T *h, size_t cnt) const {
+    if (cnt) hipcheck(hipMemcpy(h, p, sizeof(T) * cnt, hipMemcpyDeviceToHost), "download");
+  }
+};
+
+// ---------------------------------------------------------------- BitArray
+// Same memory layout as NGSolve's BitArray: bit i of byte i/8 (LSB first);
+// stored as uint32 words.
+class BitArray {
+ public:
+  explicit BitArray(size_t n) : n_(n), w_((n + 31) / 32, 0u) {}
+  size_t Size() const { return n_; }
+  bool Test(size_t i) const { return (w_.at(i >> 5) >> (i & 31)) & 1u; }
+  void SetBit(size_t i) { w_.at(i >> 5) |= 1u << (i & 31); }
+  void ClearBit(size_t i) { w_.at(i >> 5) &= ~(1u << (i & 31)); }
+  void Set() {
+    for (auto &x : w_) x = ~0u;
+    trim();
+  }
+  void Clear() {
+    for (auto &x : w_) x = 0u;
+  }
+  size_t NumSet() const {
+    size_t c = 0;
+    for (auto x : w_) c += __builtin_popcount(x);
+    return c;
+  }
+  std::vector<uint32_t> &words() { return w_; }
+  const std::vector<uint32_t> &words() const { return w_; }
+  void trim() {
+    if (n_ & 31) w_.back() &= (1u << (n_ & 31)) - 1u;
+  }
+
+ private:
+  size_t n_;
+  std::vector<uint32_t> w_;
+};
+using BA = std::shared_ptr<BitArray>;
+
+// ---------------------------------------------------------------- mesh
+struct MeshAccess {
+  py::object pymesh;
+  int dim = 0, nv = 0, ne = 0, nse = 0;
+  DevBuf<double> coords;
+  DevBuf<int32_t> el2v, bnd2v, f2el;
+  int nf = -1;
+  std::vector<int32_t> h_el2v;
+
+  explicit MeshAccess(py::object m) : pymesh(m) {
+    auto c = py::array_t<double, py::array::c_style | py::array::forcecast>(m.attr("coords"));
+    auto e = py::array_t<int32_t, py::array::c_style | py::array::forcecast>(m.attr("el2v"));
+    dim = (int)c.shape(1);
+    nv = (int)c.shape(0);
+    ne = (int)e.shape(0);
+    coords.upload(c.data(), (size_t)nv * dim);
+    el2v.upload(e.data(), (size_t)ne * (dim + 1));
+    h_el2v.assign(e.data(), e.data() + (size_t)ne * (dim + 1));
+    auto b = py::array_t<int32_t, py::array::c_style | py::array::forcecast>(m.attr("bnd2v"));
+    nse = (int)b.shape(0);
+    if (nse) bnd2v.upload(b.data(), (size_t)nse * dim);
+  }
+  int GetNE(VorB vb) const { return vb == VOL ? ne : nse; }
+  void ensure_facets() {
+    if (nf >= 0) return;
+    py::tuple t = pymesh.attr("facets")();
+    auto f = py::array_t<int32_t, py::array::c_style | py::array::forcecast>(t[2]);
+    nf = (int)f.shape(0);
+    f2el.upload(f.data(), (size_t)nf * 2);
+  }
+};
+using MA = std::shared_ptr<MeshAccess>;
+
+std::shared_ptr<MeshAccess> as_mesh(py::object m) {
+  if (py::isinstance<MeshAccess>(m)) return m.cast<MA>();
+  if (py::hasattr(m, "_xg_access")) {
+    py::object a = m.attr("_xg_access");
+    if (!a.is_none()) return a.cast<MA>();
+  }
+  auto ma = std::make_shared<MeshAccess>(m);
+  try {
+    m.attr("_xg_access") = py::cast(ma);
+  } catch (...) {
+  }
+  return ma;
+}
+
+// vertex values of a P1 level set: a grid function stand-in (attribute
+// `vec` and `order`) or any array of vertex values
+std::vector<double> p1_values(py::object lset, int nv, bool &is_p1) {
+  py::object vec = py::hasattr(lset, "vec") ? lset.attr("vec") : lset;
+  int order = py::hasattr(lset, "order") ? lset.attr("order").cast<int>() : 1;
+  auto a = py::array_t<double, py::array::c_style | py::array::forcecast>(vec);
+  is_p1 = order <= 1;
+  if (is_p1 && a.size() != nv) throw std::runtime_error("level set: expected one value per vertex");
+  return std::vector<double>(a.data(), a.data() + a.size());
+}
+
+// ---------------------------------------------------------------- CutInfo
+class CutInformation {
+ public:
+  MA ma;
+  std::array<BA, 8> elems, selems, facets;
+  std::vector<double> cut_ratio[2];
+  DevBuf<double> d_lset;
+
+  explicit CutInformation(MA m) : ma(m) {
+    for (int c = 0; c < 8; ++c) {
+      elems[c] = std::make_shared<BitArray>(ma->ne);
+      selems[c] = std::make_shared<BitArray>(ma->nse);
+    }
+    // cutinfo.cpp:22-26: facet arrays are only initialised here, indexed by
+    // DOMAIN_TYPE values (NEG = 0 -> all set, POS = 1 and IF = 2 cleared)
+    int nf = 0;
+    try {
+      ma->ensure_facets();
+      nf = ma->nf;
+    } catch (...) {
+      nf = 0;
+    }
+    for (int c = 0; c < 8; ++c) facets[c] = std::make_shared<BitArray>(nf);
+    facets[NEG]->Set();
+    facets[POS]->Clear();
+    facets[IF]->Clear();
+    cut_ratio[VOL].assign(ma->ne, 0.0);
+    cut_ratio[BND].assign(ma->nse, 0.0);
+  }
+
+  void Update(py::object lset, int subdivlvl, int time_order) {
+    if (time_order > 0)
+      py::print("Warning: you used CutInfo.Update with time_order > 0. time_order = 0 suffices to detect the "
+                "cut topology and should be prefered.");
+    bool is_p1 = true;
+    std::vector<double> v = p1_values(lset, ma->nv, is_p1);
+    if (!is_p1 || subdivlvl != 0)
+      throw std::runtime_error("CutInfo.Update: this build marks P1 level sets (subdivlvl = 0); "
+                               "P2 level sets go through xg_subdiv_weights");
+    d_lset.upload(v.data(), v.size());
+    for (int vb = 0; vb < 2; ++vb) {
+      const int ne = ma->GetNE((VorB)vb);
+      auto &arr = vb == VOL ? elems : selems;
+      if (ne == 0) {
+        for (auto &b : arr) b->Clear();
+        continue;
+      }
+      const int D = vb == VOL ? ma->dim : ma->dim - 1;
+      const int nw = (ne + 31) / 32;
+      DevBuf<uint32_t> bits((size_t)8 * nw);
+      DevBuf<double> cr(ne);
+      if (D < 1) continue;  // point boundaries of 1D meshes
+      check(xg_mark(ctx(), D, ne, vb == VOL ? ma->el2v.p : ma->bnd2v.p, d_lset.p, nullptr, cr.p, bits.p, nullptr,
+                    nullptr));
+      check(xg_sync(ctx()));
+      std::vector<uint32_t> hb((size_t)8 * nw);
+      bits.download(hb.data(), hb.size());
+      for (int c = 0; c < 8; ++c) {
+        auto &w = arr[c]->words();
+        std::copy(hb.begin() + (size_t)c * nw, hb.begin() + (size_t)(c + 1) * nw, w.begin());
+      }
+      cut_ratio[vb].resize(ne);
+      cr.download(cut_ratio[vb].data(), ne);
+    }
+  }
+
+  BA GetElementsOfDomainType(int cdt, VorB vb) const { return vb == VOL ? elems.at(cdt) : selems.at(cdt); }
+  BA GetFacetsOfDomainType(int cdt) const { return facets.at(cdt); }
+
+  // cutinfo.cpp:263-284
+  BA GetElementsWithThresholdContribution(DOMAIN_TYPE dt, double threshold, VorB vb) const {
+    const auto &cr = cut_ratio[vb];
+    auto ret = std::make_shared<BitArray>(cr.size());
+    if (dt == POS) threshold = 1 - threshold;
+    for (size_t e = 0; e < cr.size(); ++e) {
+      if (dt == NEG && cr[e] >= threshold) ret->SetBit(e);
+      else if (dt == POS && cr[e] <= threshold) ret->SetBit(e);
+    }
+    return ret;
+  }
+};
+
+int parse_cdt(py::object dt) {
+  if (py::isinstance<py::int_>(dt)) return dt.cast<int>();
+  try {
+    return (int)dt.cast<COMBINED_DOMAIN_TYPE>();
+  } catch (...) {
+  }
+  try {
+    return to_cdt(dt.cast<DOMAIN_TYPE>());
+  } catch (...) {
+  }
+  throw std::runtime_error(" unknown type for dt ");
+}
+
+int parse_dt(py::object dt) {
+  try {
+    return (int)dt.cast<DOMAIN_TYPE>();
+  } catch (...) {
+  }
+  if (py::isinstance<py::int_>(dt)) return dt.cast<int>();
+  throw std::runtime_error("domain_type must be NEG, POS or IF");
+}
+
+// ---------------------------------------------------------------- rules
+struct DeviceRule {
+  int nel = 0;
+  DevBuf<int32_t> elems, offsets;
+  DevBuf<double> pts;
+  int total = 0;
+};
+
+void build_rule(MA ma, const double *d_lset, int dt, int order, const std::vector<int32_t> *elems, DeviceRule &r) {
+  r.nel = elems ? (int)elems->size() : ma->ne;
+  if (elems) r.elems.upload(elems->data(), elems->size());
+  r.offsets.resize(r.nel + 1);
+  check(xg_cut_rule_count(ctx(), ma->dim, r.nel, elems ? r.elems.p : nullptr, ma->el2v.p, d_lset, dt, order,
+                          r.offsets.p, nullptr));
+  check(xg_sync(ctx()));
+  int32_t tot = 0;
+  hipcheck(hipMemcpy(&tot, r.offsets.p + r.nel, sizeof(int32_t), hipMemcpyDeviceToHost), "rule total");
+  r.total = tot;
+  r.pts.resize((size_t)std::max(tot, 1) * 4);
+  check(xg_cut_rule_fill(ctx(), ma->dim, r.nel, elems ? r.elems.p : nullptr, ma->el2v.p, ma->coords.p, d_lset, dt,
+                         order, r.offsets.p, r.pts.p));
+  check(xg_sync(ctx()));
+}
+
+// IntegrateX (python_cutint.cpp:21-225) for straight-cut (P1) level sets.
+// cf: an object with poly_terms() -> [(coef, (a, b, c)), ...] runs on the GPU
+// reduction kernel; any other cf must provide evaluate(xyz ndarray) -> values
+// and is evaluated on the host at the GPU-built rule's physical points.
+py::object IntegrateX(py::dict lsetdom, py::object mesh, py::object cf, py::object deformation,
+                      py::object ip_container, bool element_wise, int heapsize) {
+  if (!deformation.is_none()) throw std::runtime_error("IntegrateX: deformation is not supported in this build");
+  MA ma = as_mesh(mesh);
+  py::object lset = lsetdom["levelset"];
+  const int dt = parse_dt(lsetdom["domain_type"]);
+  const int order = lsetdom.contains("order") ? lsetdom["order"].cast<int>() : 5;
+  const int subdiv = lsetdom.contains("subdivlvl") ? lsetdom["subdivlvl"].cast<int>() : 0;
+  bool is_p1 = true;
+  std::vector<double> v = p1_values(lset, ma->nv, is_p1);
+  if (!is_p1 || subdiv != 0)
+    throw std::runtime_error("IntegrateX: this build integrates on P1 level sets (straight cut rules)");
+  DevBuf<double> d_l;
+  d_l.upload(v.data(), v.size());
+  DeviceRule r;
+  build_rule(ma, d_l.p, dt, order, nullptr, r);
+  const int D = ma->dim;
+  std::vector<double> pe(ma->ne, 0.0);
+  double total = 0.0;
+  if (cf.is_none() || py::hasattr(cf, "poly_terms") || py::isinstance<py::float_>(cf) || py::isinstance<py::int_>(cf)) {
+    std::vector<double> coef;
+    std::vector<int32_t> ex;
+    if (cf.is_none()) {
+      coef = {0.0};
+      ex = {0, 0, 0};
+    } else if (py::isinstance<py::float_>(cf) || py::isinstance<py::int_>(cf)) {
+      coef = {cf.cast<double>()};
+      ex = {0, 0, 0};
+    } else {
+      for (auto t : cf.attr("poly_terms")()) {
+        py::tuple tt = t.cast<py::tuple>();
+        coef.push_back(tt[0].cast<double>());
+        py::tuple e = tt[1].cast<py::tuple>();
+        for (int d = 0; d < 3; ++d) ex.push_back(d < (int)e.size() ? e[d].cast<int>() : 0);
+      }
+    }
+    if (coef.size() > 32) throw std::runtime_error("IntegrateX: polynomial with more than 32 terms");
+    DevBuf<double> d_pe(ma->ne), d_tot(1);
+    check(xg_integrate(ctx(), D, ma->ne, nullptr, ma->el2v.p, ma->coords.p, r.offsets.p, r.pts.p, (int)coef.size(),
+                       coef.data(), ex.data(), d_pe.p, d_tot.p));
+    check(xg_sync(ctx()));
+    d_pe.download(pe.data(), ma->ne);
+    d_tot.download(&total, 1);
+  } else {
+    // host evaluation of a general coefficient function at the rule points
+    std::vector<int32_t> off(ma->ne + 1);
+    r.offsets.download(off.data(), off.size());
+    std::vector<double> p((size_t)r.total * 4);
+    r.pts.download(p.data(), p.size());
+    auto c = py::array_t<double, py::array::c_style | py::array::forcecast>(mesh.attr("coords"));
+    const double *X = c.data();
+    py::array_t<double> xyz({(py::ssize_t)r.total, (py::ssize_t)3});
+    auto xv = xyz.mutable_unchecked<2>();
+    std::vector<double> wphys(r.total);
+    for (int e = 0; e < ma->ne; ++e) {
+      const int32_t *vv = &ma->h_el2v[(size_t)e * (D + 1)];
+      double F[9], det;
+      for (int rr = 0; rr < D; ++rr)
+        for (int cc = 0; cc < D; ++cc) F[rr * D + cc] = X[(size_t)vv[cc] * D + rr] - X[(size_t)vv[D] * D + rr];
+      if (D == 1) det = F[0];
+      else if (D == 2) det = F[0] * F[3] - F[1] * F[2];
+      else
+        det = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+              F[2] * (F[3] * F[7] - F[4] * F[6]);
+      for (int q = off[e]; q < off[e + 1]; ++q) {
+        for (int rr = 0; rr < 3; ++rr) {
+          double a = rr < D ? X[(size_t)vv[D] * D + rr] : 0.0;
+          for (int cc = 0; cc < D && rr < D; ++cc) a += F[rr * D + cc] * p[(size_t)4 * q + cc];
+          xv(q, rr) = a;
+        }
+        wphys[q] = std::abs(det) * p[(size_t)4 * q + 3];
+      }
+    }
+    auto fv = py::array_t<double, py::array::c_style | py::array::forcecast>(cf.attr("evaluate")(xyz));
+    const double *f = fv.data();
+    for (int e = 0; e < ma->ne; ++e) {
+      double s = 0.0;
+      for (int q = off[e]; q < off[e + 1]; ++q) s += wphys[q] * f[q];
+      pe[e] = s;
+      total += s;
+    }
+    if (!ip_container.is_none()) {
+      py::list l = ip_container.cast<py::list>();
+      for (int q = 0; q < r.total; ++q) l.append(py::make_tuple(xv(q, 0), xv(q, 1), xv(q, 2), wphys[q]));
+    }
+  }
+  if (element_wise) return py::array_t<double>((py::ssize_t)pe.size(), pe.data());
+  return py::float_(total);
+}
+
+// element matrices of (mass | laplace) on the cut rule of dt for the listed
+// elements (all if None): the kernel behind SymbolicCutBFI / dCut forms
+py::tuple CutElementMatrices(py::object mesh, py::object lset, int dt, int order, int p, int mask,
+                             py::object elements, int precision) {
+  MA ma = as_mesh(mesh);
+  bool is_p1 = true;
+  std::vector<double> v = p1_values(lset, ma->nv, is_p1);
+  if (!is_p1) throw std::runtime_error("CutElementMatrices: P1 level sets only");
+  DevBuf<double> d_l;
+  d_l.upload(v.data(), v.size());
+  std::vector<int32_t> el;
+  if (elements.is_none()) {
+    el.resize(ma->ne);
+    for (int e = 0; e < ma->ne; ++e) el[e] = e;
+  } else if (py::isinstance<BitArray>(elements)) {
+    const BitArray &b = elements.cast<const BitArray &>();
+    for (size_t e = 0; e < b.Size(); ++e)
+      if (b.Test(e)) el.push_back((int32_t)e);
+  } else {
+    auto a = py::array_t<int32_t, py::array::c_style | py::array::forcecast>(elements);
+    el.assign(a.data(), a.data() + a.size());
+  }
+  const int D = ma->dim;
+  int nd = 1;
+  for (int k = 1; k <= D; ++k) nd = nd * (p + k) / k;
+  const int nel = (int)el.size();
+  DevBuf<int32_t> d_el;
+  d_el.upload(el.data(), el.size());
+  const int dts[1] = {dt};
+  py::array_t<int32_t> elarr((py::ssize_t)nel, el.data());
+  if (precision == XG_PREC_FP64) {
+    DevBuf<double> out((size_t)nel * nd * nd);
+    if (nel) check(xg_cut_pipeline(ctx(), D, p, order, nel, d_el.p, ma->el2v.p, ma->coords.p, d_l.p, 1, dts, mask,
+                                   precision, out.p));
+    check(xg_sync(ctx()));
+    py::array_t<double> h({(py::ssize_t)nel, (py::ssize_t)nd, (py::ssize_t)nd});
+    out.download(h.mutable_data(), (size_t)nel * nd * nd);
+    return py::make_tuple(elarr, h);
+  }
+  DevBuf<float> out((size_t)nel * nd * nd);
+  if (nel) check(xg_cut_pipeline(ctx(), D, p, order, nel, d_el.p, ma->el2v.p, ma->coords.p, d_l.p, 1, dts, mask,
+                                 precision, out.p));
+  check(xg_sync(ctx()));
+  py::array_t<float> h({(py::ssize_t)nel, (py::ssize_t)nd, (py::ssize_t)nd});
+  out.download(h.mutable_data(), (size_t)nel * nd * nd);
+  return py::make_tuple(elarr, h);
+}
+
+}  // namespace
+
+PYBIND11_MODULE(xfem, m) {
+  m.doc() = "MI355X cut-element hot path (HIP kernels behind the xg C ABI)";
+
+  py::enum_<DOMAIN_TYPE>(m, "DOMAIN_TYPE").value("POS", POS).value("NEG", NEG).value("IF", IF).export_values();
+  py::enum_<COMBINED_DOMAIN_TYPE>(m, "COMBINED_DOMAIN_TYPE")
+      .value("NO", CDOM_NO)
+      .value("CDOM_NEG", CDOM_NEG)
+      .value("CDOM_POS", CDOM_POS)
+      .value("UNCUT", CDOM_UNCUT)
+      .value("CDOM_IF", CDOM_IF)
+      .value("HASNEG", CDOM_HASNEG)
+      .value("HASPOS", CDOM_HASPOS)
+      .value("ANY", CDOM_ANY)
+      .export_values();
+  py::enum_<SWAP_DIMENSIONS_POLICY>(m, "QUAD_DIRECTION_POLICY")
+      .value("FIRST", FIRST_ALLOWED)
+      .value("OPTIMAL", FIND_OPTIMAL)
+      .value("FALLBACK", ALWAYS_NONE)
+      .export_values();
+  py::enum_<VorB>(m, "VorB").value("VOL", VOL).value("BND", BND).export_values();
+
+  py::class_<BitArray, BA>(m, "BitArray")
+      .def(py::init<size_t>())
+      .def("__len__", &BitArray::Size)
+      .def("__getitem__", [](const BitArray &b, long i) {
+        if (i < 0) i += (long)b.Size();
+        if (i < 0 || (size_t)i >= b.Size()) throw py::index_error();
+        return b.Test((size_t)i);
+      })
+      .def("__setitem__", [](BitArray &b, long i, bool v) {
+        if (i < 0 || (size_t)i >= b.Size()) throw py::index_error();
+        if (v) b.SetBit(i);
+        else b.ClearBit(i);
+      })
+      .def("NumSet", &BitArray::NumSet)
+      .def("Set", [](BitArray &b) { b.Set(); })
+      .def("Clear", [](BitArray &b) { b.Clear(); })
+      .def("__or__", [](const BitArray &a, const BitArray &b) {
+        if (a.Size() != b.Size()) throw std::runtime_error("BitArray size mismatch");
+        auto r = std::make_shared<BitArray>(a.Size());
+        for (size_t k = 0; k < r->words().size(); ++k) r->words()[k] = a.words()[k] | b.words()[k];
+        return r;
+      })
+      .def("__and__", [](const BitArray &a, const BitArray &b) {
+        if (a.Size() != b.Size()) throw std::runtime_error("BitArray size mismatch");
+        auto r = std::make_shared<BitArray>(a.Size());
+        for (size_t k = 0; k < r->words().size(); ++k) r->words()[k] = a.words()[k] & b.words()[k];
+        return r;
+      })
+      .def("__invert__", [](const BitArray &a) {
+        auto r = std::make_shared<BitArray>(a.Size());
+        for (size_t k = 0; k < r->words().size(); ++k) r->words()[k] = ~a.words()[k];
+        r->trim();
+        return r;
+      })
+      .def("words", [](const BitArray &b) {
+        return py::array_t<uint32_t>((py::ssize_t)b.words().size(), b.words().data());
+      })
+      .def("numpy", [](const BitArray &b) {
+        py::array_t<bool> r((py::ssize_t)b.Size());
+        auto v = r.mutable_unchecked<1>();
+        for (size_t i = 0; i < b.Size(); ++i) v(i) = b.Test(i);
+        return r;
+      });
+
+  py::class_<MeshAccess, MA>(m, "MeshAccess")
+      .def(py::init([](py::object mesh) { return as_mesh(mesh); }))
+      .def_readonly("dim", &MeshAccess::dim)
+      .def_readonly("ne", &MeshAccess::ne)
+      .def_readonly("nv", &MeshAccess::nv);
+
+  py::class_<CutInformation, std::shared_ptr<CutInformation>>(m, "CutInfo")
+      .def(py::init([](py::object mesh, py::object levelset, int subdivlvl, int time_order, int heapsize) {
+             auto ci = std::make_shared<CutInformation>(as_mesh(mesh));
+             if (!levelset.is_none()) ci->Update(levelset, subdivlvl, time_order);
+             return ci;
+           }),
+           py::arg("mesh"), py::arg("levelset") = py::none(), py::arg("subdivlvl") = 0, py::arg("time_order") = -1,
+           py::arg("heapsize") = 1000000)
+      .def("Update",
+           [](CutInformation &self, py::object lset, int subdivlvl, int time_order, int heapsize) {
+             self.Update(lset, subdivlvl, time_order);
+           },
+           py::arg("levelset"), py::arg("subdivlvl") = 0, py::arg("time_order") = -1, py::arg("heapsize") = 1000000)
+      .def("Mesh", [](CutInformation &self) { return self.ma->pymesh; })
+      .def("GetElementsOfType",
+           [](CutInformation &self, py::object dt, VorB vb) { return self.GetElementsOfDomainType(parse_cdt(dt), vb); },
+           py::arg("domain_type") = IF, py::arg("VOL_or_BND") = VOL)
+      .def("GetFacetsOfType", [](CutInformation &self, py::object dt) { return self.GetFacetsOfDomainType(parse_cdt(dt)); },
+           py::arg("domain_type") = IF)
+      .def("GetCutRatios",
+           [](CutInformation &self, VorB vb) {
+             const auto &c = self.cut_ratio[vb];
+             return py::array_t<double>((py::ssize_t)c.size(), c.data());
+           },
+           py::arg("VOL_or_BND") = VOL)
+      .def("GetElementsWithThresholdContribution",
+           [](CutInformation &self, py::object dt, double threshold, VorB vb) {
+             int d = -1;
+             try {
+               d = (int)dt.cast<DOMAIN_TYPE>();
+             } catch (...) {
+             }
+             if (d != NEG && d != POS)
+               throw std::runtime_error("Unknown/Invalid type for dt: Only POS, NEG are implemented a.t.m.");
+             return self.GetElementsWithThresholdContribution((DOMAIN_TYPE)d, threshold, vb);
+           },
+           py::arg("domain_type") = NEG, py::arg("threshold") = 1.0, py::arg("VOL_or_BND") = VOL);
+
+  m.def("GetFacetsWithNeighborTypes",
+        [](py::object mesh, BA a, bool bv_a, bool bv_b, bool use_and, py::object bb, int heapsize) {
+          MA ma = as_mesh(mesh);
+          BA b = bb.is_none() ? a : bb.cast<BA>();
+          ma->ensure_facets();
+          auto ret = std::make_shared<BitArray>(ma->nf);
+          if (ma->nf == 0) return ret;
+          DevBuf<uint32_t> da, db, dout;
+          da.upload(a->words().data(), a->words().size());
+          db.upload(b->words().data(), b->words().size());
+          dout.resize(ret->words().size());
+          check(xg_facets_with_neighbor_types(ctx(), ma->nf, ma->f2el.p, da.p, db.p, bv_a, bv_b, use_and, dout.p));
+          check(xg_sync(ctx()));
+          dout.download(ret->words().data(), ret->words().size());
+          return ret;
+        },
+        py::arg("mesh"), py::arg("a"), py::arg("bnd_val_a") = true, py::arg("bnd_val_b") = true,
+        py::arg("use_and") = true, py::arg("b") = py::none(), py::arg("heapsize") = 1000000);
+
+  m.def("IntegrateX", &IntegrateX, py::arg("levelset_domain"), py::arg("mesh"), py::arg("cf") = py::none(),
+        py::arg("deformation") = py::none(), py::arg("ip_container") = py::none(), py::arg("element_wise") = false,
+        py::arg("heapsize") = 1000000);
+
+  m.def("PerturbP1",
+        [](py::array_t<double, py::array::c_style | py::array::forcecast> v, double eps) {
+          DevBuf<double> d;
+          d.upload(v.data(), v.size());
+          check(xg_p1_perturb(ctx(), (int)v.size(), d.p, eps));
+          check(xg_sync(ctx()));
+          py::array_t<double> r(v.size());
+          d.download(r.mutable_data(), v.size());
+          return r;
+        },
+        py::arg("values"), py::arg("eps_perturbation") = 1e-14);
+
+  m.def("CutElementMatrices", &CutElementMatrices, py::arg("mesh"), py::arg("levelset"), py::arg("domain_type"),
+        py::arg("order"), py::arg("p"), py::arg("mask") = 3, py::arg("elements") = py::none(),
+        py::arg("precision") = XG_PREC_FP32);
+}

@@ -38,6 +38,8 @@ struct Mfma<float> {
   static __device__ __forceinline__ acc_t run(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  // output row of accumulator element v in lane group c = lane / 16
+  static __device__ __forceinline__ int row(int c, int v) { return 4 * c + v; }
 };
 template <>
 struct Mfma<double> {
@@ -45,6 +47,8 @@ struct Mfma<double> {
   static __device__ __forceinline__ acc_t run(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  // the FP64 variant interleaves the rows: element v of lane group c is row c + 4 v
+  static __device__ __forceinline__ int row(int c, int v) { return c + 4 * v; }
 };
 
 __device__ __forceinline__ void wave_sync() {
@@ -63,67 +67,116 @@ struct Cfg {
   static constexpr size_t LDS_PER_WAVE = sizeof(T) * (4 * NB * 16 * PAD + CH);
 };
 
-// Point source: either a materialised CSR rule or the straight cut rule built
-// on the fly.  prepare() sets up the element, count() gives the number of
-// points, point() gives reference coordinates and the reference weight.
-template <int D>
-struct FusedSrc {
-  double l[D + 1];
-  Decomp dc;
-  double P[D + 1 + 4][3];
-  int uncut, nq, off, Ds, npts, bad;
-  double ifac;
-  __device__ void prepare(const double *lv, const double *Finv, int dt, int order) {
+// Point sources.  point(g, x, w) gives the reference coordinates x (in the
+// accumulation type T) and the reference weight w of point g; elmat_one
+// multiplies by |det F|.
+//
+// StagedSrc: the straight cut rule built on the fly.  The topology comes from
+// the compile-time decomposition table (wave-uniform sign mask -> scalar
+// loads), the FP64 cut points, sub-simplex volumes and weight scales are
+// computed once per (element, dt) by a few lanes and staged in LDS, together
+// with the base rules; each lane then maps its point with
+// x = ow V0 + sum_m xi_m V_{m+1} (straightcutrule.cpp:80-100) in precision T.
+constexpr int PIPE_NQMAX = 125;  // base rule points (order <= 8 tets)
+
+template <int D, typename T>
+struct StageLds {
+  double P[D + 5][3];  // reference vertices, then cut points (FP64)
+  T V[3][4][3];        // sub-simplex vertices (mapping order)
+  T wsc[3];            // volume * interface factor per sub-simplex
+};
+
+template <int D, typename T>
+struct StagedSrc {
+  StageLds<D, T> *st;
+  const T (*R)[4];  // base rule (xi0, xi1, xi2, w), in LDS
+  int nq, nv1;      // points per sub-simplex, vertices per sub-simplex
+  __device__ void point(int g, T *x, T &w) const {
+    const int s = (g >= nq) + (g >= 2 * nq);
+    const int k = g - s * nq;
+    const T *r = R[k];
+    T ow = T(1);
+    for (int m = 0; m + 1 < nv1; ++m) ow -= r[m];
 #pragma unroll
-    for (int k = 0; k <= D; ++k) l[k] = lv[k];
-    const int eldom = check_if_straight_cut<D>(l);
-    npts = 0;
-    bad = 0;
-    ifac = 1.0;
-    if (eldom != IF) {
-      uncut = 1;
-      Ds = D;
-      nq = rule_cnt(D, order);
-      off = rule_off(D, order);
-      npts = (eldom == dt) ? nq : 0;
-      return;
-    }
-    uncut = 0;
-    if (!decompose<D>(l, dt, dc)) {  // the reference throws here
-      bad = 1;
-      return;
-    }
-    cut_point_array<D>(l, dc, P);
-    Ds = dc.nv - 1;
-    nq = rule_cnt(Ds, order);
-    off = rule_off(Ds, order);
-    npts = dc.ns * nq;
-    if (dt == IF) ifac = interface_factor<D>(l, Finv);
-  }
-  __device__ void point(int g, double *x, double &w) const {
-    if (uncut) {
-      const double *r = xg_rule_data_d[off + g];
-      x[0] = r[0];
-      x[1] = D > 1 ? r[1] : 0.0;
-      x[2] = D > 2 ? r[2] : 0.0;
-      w = r[3];
-      return;
-    }
-    const int s = g / nq, k = g - s * nq;
-    const int8_t *idx = dc.v[s];
-    const double tf = simplex_volume(dc.nv, idx, P);
-    const double *r = xg_rule_data_d[off + k];
-    double ow = 1.0;
-    for (int m = 0; m < Ds; ++m) ow -= r[m];
+    for (int d = 0; d < 3; ++d) x[d] = ow * st->V[s][0][d];
+    for (int m = 0; m + 1 < nv1; ++m)
 #pragma unroll
-    for (int d = 0; d < 3; ++d) x[d] = ow * P[idx[0]][d];
-    for (int m = 0; m < Ds; ++m)
-#pragma unroll
-      for (int d = 0; d < 3; ++d) x[d] += r[m] * P[idx[m + 1]][d];
-    w = r[3] * tf;
-    w = w * ifac;
+      for (int d = 0; d < 3; ++d) x[d] += r[m] * st->V[s][m + 1][d];
+    w = r[3] * st->wsc[s];
   }
 };
+
+__device__ __forceinline__ void wave_fence() { wave_sync(); }
+
+// Builds the stage for (element with level set l, dt); returns the number of
+// points (0 = no rule).  Wave-uniform.  *bad set when the reference throws.
+template <int D, typename T>
+__device__ int build_stage(StageLds<D, T> *st, const double *l, const double *Finv, int dt, int nq_vol, int nq_if,
+                           int &nq, int &nv1, bool &bad) {
+  const int lane = threadIdx.x & 63;
+  const int eldom = check_if_straight_cut<D>(l);
+  bad = false;
+  if (eldom != IF) {
+    if (eldom != dt) return 0;
+    // uncut: the base rule itself = one "sub-simplex" (origin, e_1, .., e_D)
+    if (lane < (D + 1) * 3) {
+      const int m = lane / 3, d = lane % 3;
+      st->V[0][m][d] = (m > 0 && d == m - 1) ? T(1) : T(0);
+    }
+    if (lane == 0) st->wsc[0] = T(1);
+    nq = nq_vol;
+    nv1 = D + 1;
+    wave_fence();
+    return nq;
+  }
+  if (!cut_is_valid<D>(l)) {
+    bad = true;
+    return 0;
+  }
+  const int mask = sign_mask<D>(l);
+  const DecompEntry &E = xg_decomp_d.e[D - 1][dt][mask];
+  const int ns = E.ns, nv = E.nv, nc = E.nc;
+  if (!E.ok) {
+    bad = true;
+    return 0;
+  }
+  // reference vertices and cut points (FP64, straightcutrule.cpp:50)
+  if (lane < D + 1) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) st->P[lane][d] = (d == lane && lane < D) ? 1.0 : 0.0;
+  } else if (lane < D + 1 + nc) {
+    const int c = lane - (D + 1);
+    const int i = E.ce[c][0], j = E.ce[c][1];
+    double li = 0.0, lj = 0.0;
+#pragma unroll
+    for (int k = 0; k <= D; ++k) {
+      li = (k == i) ? l[k] : li;
+      lj = (k == j) ? l[k] : lj;
+    }
+    const double t = li / (li - lj);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double pi = (d == i && i < D) ? 1.0 : 0.0;
+      const double pj = (d == j && j < D) ? 1.0 : 0.0;
+      st->P[lane][d] = pi + t * (pj - pi);
+    }
+  }
+  wave_fence();
+  double ifac = 1.0;
+  if (dt == IF) ifac = interface_factor<D>(l, Finv);
+  if (lane < ns) {
+    const double tf = simplex_volume(nv, E.v[lane], st->P);
+    st->wsc[lane] = (T)(tf * ifac);
+  }
+  if (lane < ns * nv * 3) {
+    const int s = lane / (nv * 3), m = (lane / 3) % nv, d = lane % 3;
+    st->V[s][m][d] = (T)st->P[E.v[s][m]][d];
+  }
+  wave_fence();
+  nq = (dt == IF) ? nq_if : nq_vol;
+  nv1 = nv;
+  return ns * nq;
+}
 
 // one element (or one (element, dt) pair): accumulate and store
 template <int D, int P, typename T, typename SRC>
@@ -151,14 +204,14 @@ __device__ __forceinline__ void elmat_one(const SRC &src, int npts, const double
       T phi[N], dphi[N][D];
       T w = T(0);
       if (g < npts) {
-        double x[3], wd;
-        src.point(g, x, wd);
-        w = (T)(wd * absdet);
+        T x[3], wr;
+        src.point(g, x, wr);
+        w = wr * (T)absdet;
         T lam[D + 1];
         T s = T(1);
 #pragma unroll
         for (int m = 0; m < D; ++m) {
-          lam[m] = (T)x[m];
+          lam[m] = x[m];
           s -= lam[m];
         }
         lam[D] = s;
@@ -197,14 +250,14 @@ __device__ __forceinline__ void elmat_one(const SRC &src, int npts, const double
     }
     wave_sync();
   }
-  // D[row][col]: lane holds rows 4*(lane/16)+v, col lane%16
+  // D[row][col]: lane holds col lane%16 and rows Mfma<T>::row(lane/16, v)
 #pragma unroll
   for (int bi = 0; bi < NB; ++bi)
 #pragma unroll
     for (int bj = bi; bj < NB; ++bj)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int gi = bi * 16 + 4 * c + v, gj = bj * 16 + ii;
+        const int gi = bi * 16 + Mfma<T>::row(c, v), gj = bj * 16 + ii;
         if (gi < N && gj < N) {
           out[gi * N + gj] = acc[bi][bj][v];
           if (bi < bj) out[gj * N + gi] = acc[bi][bj][v];
@@ -232,13 +285,24 @@ __global__ __launch_bounds__(128) void k_pipeline(int nel, const int32_t *__rest
                                                   int mask, T *__restrict__ out, int32_t *err) {
   using C = Cfg<D, P, T>;
   extern __shared__ char smem[];
+  __shared__ StageLds<D, T> s_stage[C::WAVES];
+  __shared__ T s_rule[2][PIPE_NQMAX][4];  // base rules of dimension D and D-1
   const int wv = threadIdx.x >> 6;
   T *sB = reinterpret_cast<T *>(smem) + (size_t)wv * (C::LDS_PER_WAVE / sizeof(T));
   T *sW = sB + 4 * C::NB * 16 * C::PAD;
   const int lane = threadIdx.x & 63;
   // zero the padding rows once (dofs >= N never get written)
   for (int k = lane; k < 4 * C::NB * 16 * C::PAD; k += 64) sB[k] = T(0);
-  wave_sync();
+  const int nq_vol = rule_cnt(D, order), nq_if = rule_cnt(D - 1, order);
+  const int off_vol = rule_off(D, order), off_if = rule_off(D - 1, order);
+  for (int k = threadIdx.x; k < nq_vol + nq_if; k += blockDim.x) {
+    const int which = k < nq_vol ? 0 : 1;
+    const int kk = which ? k - nq_vol : k;
+    const double *r = xg_rule_data_d[(which ? off_if : off_vol) + kk];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) s_rule[which][kk][c] = (T)r[c];
+  }
+  __syncthreads();  // the only block barrier: before the per-wave loops
   const int wid = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const int nwaves = (gridDim.x * blockDim.x) >> 6;
   constexpr int NN = C::N * C::N;
@@ -250,23 +314,26 @@ __global__ __launch_bounds__(128) void k_pipeline(int nel, const int32_t *__rest
     jacobian<D>(V, F, Finv, absdet);
     for (int t = 0; t < ndt; ++t) {
       const int dt = (dts >> (2 * t)) & 3;
-      FusedSrc<D> src;
-      src.prepare(l, Finv, dt, order);
-      if (src.bad && (threadIdx.x & 63) == 0) atomicOr(err, 1);
-      elmat_one<D, P, T>(src, src.npts, Finv, absdet, mask, sB, sW, out + ((size_t)i * ndt + t) * NN);
+      int nq = 0, nv1 = 0;
+      bool bad = false;
+      const int npts = build_stage<D, T>(&s_stage[wv], l, Finv, dt, nq_vol, nq_if, nq, nv1, bad);
+      if (bad && lane == 0) atomicOr(err, 1);
+      StagedSrc<D, T> src{&s_stage[wv], s_rule[dt == IF ? 1 : 0], nq, nv1};
+      elmat_one<D, P, T>(src, npts, Finv, absdet, mask, sB, sW, out + ((size_t)i * ndt + t) * NN);
     }
   }
 }
 
+template <typename T>
 struct CsrSrc {
   const double4 *pts;
   int base;
-  __device__ void point(int g, double *x, double &w) const {
+  __device__ void point(int g, T *x, T &w) const {
     const double4 p = pts[base + g];
-    x[0] = p.x;
-    x[1] = p.y;
-    x[2] = p.z;
-    w = p.w;
+    x[0] = (T)p.x;
+    x[1] = (T)p.y;
+    x[2] = (T)p.z;
+    w = (T)p.w;
   }
 };
 
@@ -294,8 +361,53 @@ __global__ __launch_bounds__(128) void k_elmat_csr(int nel, const int32_t *__res
     load_geom<D>(el2v, coords, nullptr, e, V, nullptr);
     double F[D * D], Finv[D * D], absdet;
     jacobian<D>(V, F, Finv, absdet);
-    CsrSrc src{pts, offsets[i]};
+    CsrSrc<T> src{pts, offsets[i]};
     elmat_one<D, P, T>(src, offsets[i + 1] - offsets[i], Finv, absdet, mask, sB, sW, out + (size_t)i * NN);
+  }
+}
+
+// Fixed-node rule: the degree-Q Lagrange lattice nodes of the reference tet
+// with per-element weights (learned rule or contracted subdivision rule).
+template <int Q, typename T>
+struct WeightSrc {
+  const double *w;
+  __device__ void point(int g, T *x, T &wt) const {
+    constexpr Lattice<3, Q> L{};
+    x[0] = (T)L.a[g][0] / (T)Q;
+    x[1] = (T)L.a[g][1] / (T)Q;
+    x[2] = (T)L.a[g][2] / (T)Q;
+    wt = (T)w[g];
+  }
+};
+
+template <int P, int Q, typename T>
+__global__ __launch_bounds__(128) void k_elmat_weights(int nel, const int32_t *__restrict__ elems,
+                                                       const int32_t *__restrict__ el2v,
+                                                       const double *__restrict__ coords,
+                                                       const double *__restrict__ weights, int mask,
+                                                       T *__restrict__ out) {
+  using C = Cfg<3, P, T>;
+  constexpr int NQ = ndof(3, Q);
+  extern __shared__ char smem[];
+  const int wv = threadIdx.x >> 6;
+  T *sB = reinterpret_cast<T *>(smem) + (size_t)wv * (C::LDS_PER_WAVE / sizeof(T));
+  T *sW = sB + 4 * C::NB * 16 * C::PAD;
+  const int lane = threadIdx.x & 63;
+  for (int k = lane; k < 4 * C::NB * 16 * C::PAD; k += 64) sB[k] = T(0);
+  wave_sync();
+  const int wid = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int nwaves = (gridDim.x * blockDim.x) >> 6;
+  constexpr int NN = C::N * C::N;
+  for (int i = wid; i < nel; i += nwaves) {
+    double Finv[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, absdet = 1.0;
+    if (coords) {
+      const int e = elems ? elems[i] : i;
+      double V[4][3], F[9];
+      load_geom<3>(el2v, coords, nullptr, e, V, nullptr);
+      jacobian<3>(V, F, Finv, absdet);
+    }
+    WeightSrc<Q, T> src{weights + (size_t)i * NQ};
+    elmat_one<3, P, T>(src, NQ, Finv, absdet, mask, sB, sW, out + (size_t)i * NN);
   }
 }
 
@@ -349,6 +461,8 @@ extern "C" int xg_cut_pipeline(xg_ctx *ctx, int D, int p, int order, int nel, co
   if (!ctx || D < 1 || D > 3 || p < 1 || p > 3 || nel < 0 || ndt < 1 || ndt > 3 || !dts || order < 0 ||
       order / 2 + 1 > XG_RULE_NMAX || mask < 1 || mask > 3)
     return set_error(XG_ERR_ARG, "xg_cut_pipeline: bad argument");
+  if (xg_rule_cnt[D][order / 2 + 1] > PIPE_NQMAX)
+    return set_error(XG_ERR_UNSUPPORTED, "xg_cut_pipeline: order too high for the fused kernel (use xg_elmat)");
   int packed = 0;
   for (int t = 0; t < ndt; ++t) {
     if (dts[t] < 0 || dts[t] > 2) return set_error(XG_ERR_ARG, "xg_cut_pipeline: bad domain type");
@@ -361,6 +475,32 @@ extern "C" int xg_cut_pipeline(xg_ctx *ctx, int D, int p, int order, int nel, co
   if (precision == XG_PREC_FP64) XG_DISPATCH(D, p, double, CALL_P);
 #undef CALL_P
   return set_error(XG_ERR_UNSUPPORTED, "xg_cut_pipeline: unsupported precision");
+}
+
+template <int P, int Q, typename T>
+int launch_weights(xg_ctx *ctx, int nel, const int32_t *elems, const int32_t *el2v, const double *coords,
+                   const double *weights, int mask, void *out) {
+  using C = Cfg<3, P, T>;
+  const size_t lds = C::LDS_PER_WAVE * C::WAVES;
+  k_elmat_weights<P, Q, T><<<grid_waves(nel), 64 * C::WAVES, lds, ctx->stream>>>(nel, elems, el2v, coords, weights,
+                                                                                mask, (T *)out);
+  XG_LAUNCH_CHECK();
+  return XG_OK;
+}
+
+extern "C" int xg_elmat_weights(xg_ctx *ctx, int p, int q, int nel, const int32_t *elems, const int32_t *el2v,
+                                const double *coords, const double *weights, int mask, int precision, void *out) {
+  if (!ctx || p < 1 || p > 2 || q != 4 || nel < 0 || mask < 1 || mask > 3)
+    return set_error(XG_ERR_ARG, "xg_elmat_weights: bad argument (p <= 2, q == 4)");
+  if (nel == 0) return XG_OK;
+  if (!weights || !out || (coords && !el2v)) return set_error(XG_ERR_ARG, "xg_elmat_weights: null pointer");
+  if (precision == XG_PREC_FP32)
+    return p == 1 ? launch_weights<1, 4, float>(ctx, nel, elems, el2v, coords, weights, mask, out)
+                  : launch_weights<2, 4, float>(ctx, nel, elems, el2v, coords, weights, mask, out);
+  if (precision == XG_PREC_FP64)
+    return p == 1 ? launch_weights<1, 4, double>(ctx, nel, elems, el2v, coords, weights, mask, out)
+                  : launch_weights<2, 4, double>(ctx, nel, elems, el2v, coords, weights, mask, out);
+  return set_error(XG_ERR_UNSUPPORTED, "xg_elmat_weights: unsupported precision");
 }
 
 extern "C" int xg_elmat(xg_ctx *ctx, int D, int p, int nel, const int32_t *elems, const int32_t *el2v,

@@ -1,0 +1,446 @@
+"""ngsxfem's Python surface for the cut-element path, on the MI355X build.
+
+Mirror of the reference package ``xfem`` (python/__init__.py): the compiled
+module ``xfem.xfem`` (here ``ngsxfem_amd.xfem.xfem``, pybind11 over the HIP
+C ABI) provides CutInfo, BitArray, GetFacetsWithNeighborTypes, IntegrateX
+and the enums; this file adds HAS, Integrate, dCut, SymbolicCutBFI,
+InterpolateToP1 and the small NGSolve stand-ins the path needs (meshes, P1
+grid functions, polynomial coefficient functions, u*v / grad(u)*grad(v)
+forms, BilinearForm.Assemble).
+
+    from ngsxfem_amd.xfem import *
+    mesh = MakeStructured2DMesh(nx=30, ny=30)
+    lsetp1 = GridFunction(H1(mesh, order=1))
+    InterpolateToP1(sqrt(x*x + y*y) - 0.5, lsetp1)
+    ci = CutInfo(mesh, lsetp1)
+    a = BilinearForm(H1(mesh, order=2))
+    a += (grad(u)*grad(v) + u*v) * dCut(lsetp1, NEG, definedonelements=ci.GetElementsOfType(HASNEG))
+    a.Assemble()
+    Integrate({"levelset": lsetp1, "domain_type": IF}, cf=1, mesh=mesh, order=2)
+"""
+import math
+
+import numpy as np
+
+from ..meshes import Make1DMesh, MakeStructured2DMesh, MakeStructured3DMesh, Mesh  # noqa: F401
+from .xfem import (ANY, BND, CDOM_IF, CDOM_NEG, CDOM_POS, FALLBACK, FIRST, HASNEG, HASPOS, IF, NEG, NO,  # noqa: F401
+                   OPTIMAL, POS, UNCUT, VOL, BitArray, COMBINED_DOMAIN_TYPE, CutElementMatrices, CutInfo,
+                   DOMAIN_TYPE, GetFacetsWithNeighborTypes, IntegrateX, PerturbP1, QUAD_DIRECTION_POLICY, VorB)
+
+PREC_FP64, PREC_FP32 = 0, 1
+
+
+class _Globals:
+    """ngsxfemglobals (utils/python_utils.cpp:232-280): the epsilons of the path"""
+
+    def __init__(self):
+        self.SetDefaults()
+
+    def SetDefaults(self):
+        self.eps_P1_perturbation = 1e-14
+        self.simd_eval = True
+        self.elmat_precision = PREC_FP32
+
+    def MultiplyAllEps(self, fac):
+        self.eps_P1_perturbation *= fac
+
+    def Output(self):
+        print("eps_P1_perturbation =", self.eps_P1_perturbation)
+
+
+ngsxfemglobals = _Globals()
+
+
+def HAS(domain_type):
+    """python/__init__.py:30-42"""
+    if domain_type == NEG:
+        return HASNEG
+    elif domain_type == POS:
+        return HASPOS
+    elif domain_type == IF:
+        return IF
+    else:
+        raise Exception("invalid domain type")
+
+
+# ------------------------------------------------------------------ coefficient functions
+class CoefficientFunction:
+    """Scalar coefficient function of the physical coordinates.  Polynomials
+    (built from x, y, z, constants, +, -, *, integer powers) are kept as
+    monomial term lists and integrated by the GPU reduction kernel; anything
+    else is evaluated on the host (``evaluate``)."""
+
+    def __init__(self, val=None, terms=None, fn=None):
+        if terms is not None:
+            self._terms = {k: v for k, v in terms.items() if v != 0.0}
+            self._fn = None
+        elif fn is not None:
+            self._terms = None
+            self._fn = fn
+        else:
+            self._terms = {(0, 0, 0): float(val)} if float(val) != 0.0 else {}
+            self._fn = None
+
+    @staticmethod
+    def _lift(o):
+        return o if isinstance(o, CoefficientFunction) else CoefficientFunction(float(o))
+
+    def is_poly(self):
+        return self._terms is not None
+
+    def poly_terms(self):
+        if not self.is_poly():
+            raise TypeError("not a polynomial")
+        t = [(c, e) for e, c in sorted(self._terms.items())]
+        return t if t else [(0.0, (0, 0, 0))]
+
+    def evaluate(self, X):
+        X = np.asarray(X, dtype=np.float64)
+        if self.is_poly():
+            out = np.zeros(X.shape[0])
+            for (a, b, c), co in self._terms.items():
+                out += co * X[:, 0] ** a * X[:, 1] ** b * X[:, 2] ** c
+            return out
+        return np.asarray(self._fn(X), dtype=np.float64)
+
+    def __call__(self, *p):
+        X = np.zeros((1, 3))
+        X[0, :len(p)] = p
+        return float(self.evaluate(X)[0])
+
+    def _binop(self, o, op):
+        o = self._lift(o)
+        if self.is_poly() and o.is_poly():
+            if op == "+" or op == "-":
+                s = -1.0 if op == "-" else 1.0
+                t = dict(self._terms)
+                for e, c in o._terms.items():
+                    t[e] = t.get(e, 0.0) + s * c
+                return CoefficientFunction(terms=t)
+            if op == "*":
+                t = {}
+                for e1, c1 in self._terms.items():
+                    for e2, c2 in o._terms.items():
+                        e = (e1[0] + e2[0], e1[1] + e2[1], e1[2] + e2[2])
+                        t[e] = t.get(e, 0.0) + c1 * c2
+                return CoefficientFunction(terms=t)
+        a, b = self, o
+        fns = {"+": lambda X: a.evaluate(X) + b.evaluate(X), "-": lambda X: a.evaluate(X) - b.evaluate(X),
+               "*": lambda X: a.evaluate(X) * b.evaluate(X), "/": lambda X: a.evaluate(X) / b.evaluate(X)}
+        return CoefficientFunction(fn=fns[op])
+
+    def __add__(self, o):
+        return self._binop(o, "+")
+
+    def __radd__(self, o):
+        return self._lift(o)._binop(self, "+")
+
+    def __sub__(self, o):
+        return self._binop(o, "-")
+
+    def __rsub__(self, o):
+        return self._lift(o)._binop(self, "-")
+
+    def __mul__(self, o):
+        if isinstance(o, (_Form, _CutDifferentialSymbol, _Measure)):
+            return NotImplemented
+        return self._binop(o, "*")
+
+    def __rmul__(self, o):
+        return self._lift(o)._binop(self, "*")
+
+    def __truediv__(self, o):
+        o = self._lift(o)
+        if o.is_poly() and list(o._terms.keys()) in ([(0, 0, 0)],):
+            return self * (1.0 / o._terms[(0, 0, 0)])
+        return self._binop(o, "/")
+
+    def __neg__(self):
+        return self * (-1.0)
+
+    def __pow__(self, k):
+        if isinstance(k, int) and k >= 0 and self.is_poly():
+            r = CoefficientFunction(1.0)
+            for _ in range(k):
+                r = r * self
+            return r
+        a = self
+        return CoefficientFunction(fn=lambda X: a.evaluate(X) ** k)
+
+
+x = CoefficientFunction(terms={(1, 0, 0): 1.0})
+y = CoefficientFunction(terms={(0, 1, 0): 1.0})
+z = CoefficientFunction(terms={(0, 0, 1): 1.0})
+
+
+def _unary(f):
+    def g(cf):
+        if not isinstance(cf, CoefficientFunction):
+            return f(cf)
+        return CoefficientFunction(fn=lambda X: f(cf.evaluate(X)))
+    return g
+
+
+sqrt = _unary(np.sqrt)
+sin = _unary(np.sin)
+cos = _unary(np.cos)
+exp = _unary(np.exp)
+
+
+# ------------------------------------------------------------------ spaces, grid functions
+class H1:
+    """Lagrange space of degree `order` (P1 vertex dofs are what the cut path
+    reads; higher orders define the element matrices' basis)."""
+
+    def __init__(self, mesh, order=1, **kwargs):
+        self.mesh = mesh
+        self.order = order
+
+    @property
+    def ndof(self):
+        if self.order == 1:
+            return self.mesh.nv
+        raise NotImplementedError("global numbering is built for P1 spaces in this build")
+
+    def TrialFunction(self):
+        return _Proxy("u", self)
+
+    def TestFunction(self):
+        return _Proxy("v", self)
+
+    def TnT(self):
+        return self.TrialFunction(), self.TestFunction()
+
+
+class GridFunction:
+    def __init__(self, space, name="gf"):
+        self.space = space
+        self.order = space.order
+        self.vec = np.zeros(space.mesh.nv if space.order == 1 else 0)
+
+    def Set(self, cf):
+        """nodal interpolation at the vertices (P1)"""
+        self.vec = _eval_at_vertices(cf, self.space.mesh)
+
+
+def _eval_at_vertices(cf, mesh):
+    X = np.zeros((mesh.nv, 3))
+    X[:, :mesh.dim] = mesh.coords
+    if isinstance(cf, CoefficientFunction):
+        return cf.evaluate(X)
+    if callable(cf):
+        return np.array([cf(*p) for p in mesh.coords], dtype=np.float64)
+    return np.full(mesh.nv, float(cf))
+
+
+def InterpolateToP1(gf_or_cf, gf_p1, eps_perturbation=None, heapsize=1000000):
+    """utils/p1interpol.cpp:27-113: vertex values, |v| < eps -> +eps (GPU)"""
+    eps = ngsxfemglobals.eps_P1_perturbation if eps_perturbation is None else eps_perturbation
+    if isinstance(gf_or_cf, GridFunction):
+        vals = np.asarray(gf_or_cf.vec, dtype=np.float64).copy()
+    else:
+        vals = _eval_at_vertices(gf_or_cf, gf_p1.space.mesh)
+    gf_p1.vec = PerturbP1(vals, eps)
+
+
+# ------------------------------------------------------------------ Integrate
+def Integrate_X_special_args(levelset_domain={}, cf=None, mesh=None, VOL_or_BND=VOL, order=5, time_order=-1,
+                             region_wise=False, element_wise=False, heapsize=1000000, deformation=None,
+                             ip_container=None):
+    """python/__init__.py:339-358"""
+    lsd = dict(levelset_domain)
+    if "order" not in lsd or lsd["order"] == -1:
+        lsd["order"] = order
+    if "time_order" not in lsd or lsd["time_order"] == -1:
+        lsd["time_order"] = time_order
+    if isinstance(cf, (int, float)):
+        cf = CoefficientFunction(float(cf))
+    return IntegrateX(levelset_domain=lsd, mesh=mesh, cf=cf, deformation=deformation,
+                      ip_container=ip_container, element_wise=element_wise, heapsize=heapsize)
+
+
+def Integrate(levelset_domain=None, *args, **kwargs):
+    """python/__init__.py:361-439: level-set integration when a dict is given;
+    otherwise `cf * dCut(...)` integrals"""
+    if levelset_domain is not None and type(levelset_domain) == dict:
+        return Integrate_X_special_args(levelset_domain, *args, **kwargs)
+    if isinstance(levelset_domain, _SumOfIntegrals):
+        mesh = args[0] if args else kwargs["mesh"]
+        return sum(i.integrate(mesh, kwargs.get("element_wise", False)) for i in levelset_domain.parts)
+    raise NotImplementedError("plain (non level-set) Integrate is NGSolve's and not part of this build")
+
+
+# ------------------------------------------------------------------ forms
+class _Proxy:
+    def __init__(self, kind, space, deriv=0):
+        self.kind, self.space, self.deriv = kind, space, deriv
+
+    def __mul__(self, o):
+        if isinstance(o, _Proxy):
+            if {self.kind, o.kind} != {"u", "v"} or self.deriv != o.deriv:
+                raise NotImplementedError("forms: u*v and grad(u)*grad(v) only")
+            return _Form({("mass" if self.deriv == 0 else "laplace"): 1.0}, self.space)
+        return NotImplemented
+
+
+def grad(p):
+    if isinstance(p, _Proxy):
+        return _Proxy(p.kind, p.space, 1)
+    raise NotImplementedError("grad of a proxy function only")
+
+
+class _Form:
+    def __init__(self, parts, space):
+        self.parts, self.space = dict(parts), space
+
+    def __add__(self, o):
+        d = dict(self.parts)
+        for k, c in o.parts.items():
+            d[k] = d.get(k, 0.0) + c
+        return _Form(d, self.space)
+
+    def __mul__(self, o):
+        if isinstance(o, (_CutDifferentialSymbol, _Measure)):
+            return _SumOfIntegrals([_CutIntegral(self, o)])
+        if isinstance(o, (int, float)):
+            return _Form({k: c * o for k, c in self.parts.items()}, self.space)
+        return NotImplemented
+
+    __rmul__ = lambda self, o: self.__mul__(o) if isinstance(o, (int, float)) else NotImplemented
+
+
+class _Measure:
+    """NGSolve dx(definedonelements=...): the uncut base rule on the listed
+    elements (a constant negative level set selects it)."""
+
+    def __init__(self, definedonelements=None, order=None):
+        self.definedonelements, self.order = definedonelements, order
+
+
+def dx(definedonelements=None, bonus_intorder=0, **kwargs):
+    return _Measure(definedonelements)
+
+
+class _CutDifferentialSymbol:
+    """CutDifferentialSymbol (cutint/cutintegral.hpp:55-82)"""
+
+    def __init__(self, lsetdom, definedon=None, vb=VOL, element_boundary=False, element_vb=VOL, skeleton=False,
+                 deformation=None, definedonelements=None):
+        if deformation is not None or skeleton or element_boundary or vb != VOL:
+            raise NotImplementedError("dCut: VOL element integrals without deformation in this build")
+        self.lsetdom = lsetdom
+        self.definedonelements = definedonelements
+
+    def __rmul__(self, o):
+        if isinstance(o, _Form):
+            return o.__mul__(self)
+        if isinstance(o, (CoefficientFunction, int, float)):
+            return _SumOfIntegrals([_CutIntegral(CoefficientFunction._lift(o), self)])
+        return NotImplemented
+
+
+def dCut(levelset, domain_type, order=None, subdivlvl=None, time_order=-1, levelset_domain=None, **kwargs):
+    """python/__init__.py:781-864"""
+    if levelset_domain is not None and type(levelset_domain) == dict:
+        lsetdom = levelset_domain
+    else:
+        lsetdom = {"levelset": levelset, "domain_type": domain_type}
+    if order is not None and "order" not in lsetdom:
+        lsetdom["order"] = order
+    if subdivlvl is not None and "subdivlvl" not in lsetdom:
+        lsetdom["subdivlvl"] = subdivlvl
+    if time_order > -1 and "time_order" not in lsetdom:
+        lsetdom["time_order"] = time_order
+    if "tref" in kwargs:
+        raise NotImplementedError("tref (space-time) is outside this build")
+    return _CutDifferentialSymbol(lsetdom, **kwargs)
+
+
+class _CutIntegral:
+    """CutIntegral (cutint/cutintegral.hpp:21-51)"""
+
+    def __init__(self, integrand, measure):
+        self.integrand, self.measure = integrand, measure
+
+    def _rule_args(self, mesh, p):
+        m = self.measure
+        if isinstance(m, _Measure):
+            lset = np.full(mesh.nv, -1.0)
+            return lset, NEG, (m.order if m.order is not None else 2 * p), m.definedonelements
+        d = m.lsetdom
+        if d.get("subdivlvl", 0):
+            raise NotImplementedError("subdivlvl > 0 element matrices: use the P2 subdivision kernels")
+        return d["levelset"], int(d["domain_type"]), d.get("order", 2 * p), m.definedonelements
+
+    def element_matrices(self, mesh, p, precision):
+        lset, dt, order, els = self._rule_args(mesh, p)
+        mask = (1 if "mass" in self.integrand.parts else 0) | (2 if "laplace" in self.integrand.parts else 0)
+        coefs = self.integrand.parts
+        if mask == 3 and coefs["mass"] != coefs["laplace"]:
+            e1, m1 = CutElementMatrices(mesh, lset, dt, order, p, 1, els, precision)
+            e2, m2 = CutElementMatrices(mesh, lset, dt, order, p, 2, els, precision)
+            return e1, coefs["mass"] * m1 + coefs["laplace"] * m2
+        e, m = CutElementMatrices(mesh, lset, dt, order, p, mask, els, precision)
+        c = coefs.get("mass", coefs.get("laplace", 1.0))
+        return e, c * m
+
+    def integrate(self, mesh, element_wise=False):
+        lset, dt, order, els = self._rule_args(mesh, 0)
+        lsd = {"levelset": lset, "domain_type": dt, "order": order if order is not None else 5}
+        r = IntegrateX(levelset_domain=lsd, mesh=mesh, cf=self.integrand, element_wise=True)
+        if els is not None:
+            r = r * np.asarray(els.numpy(), dtype=np.float64)
+        return r if element_wise else float(np.sum(r))
+
+
+class _SumOfIntegrals:
+    def __init__(self, parts):
+        self.parts = list(parts)
+
+    def __add__(self, o):
+        return _SumOfIntegrals(self.parts + o.parts)
+
+
+def SymbolicCutBFI(levelset_domain, form, VOL_or_BND=VOL, element_boundary=False, skeleton=False, definedon=None,
+                   definedonelements=None, deformation=None):
+    """xfem/python_xfem.cpp:855-931"""
+    return _SumOfIntegrals([_CutIntegral(form, _CutDifferentialSymbol(
+        levelset_domain, definedonelements=definedonelements, deformation=deformation,
+        element_boundary=element_boundary, skeleton=skeleton))])
+
+
+class BilinearForm:
+    """Collects cut integrals; Assemble() computes the element matrices on the
+    GPU (xg_cut_pipeline) and scatters them into a CSR matrix for P1 spaces."""
+
+    def __init__(self, space, symmetric=False, **kwargs):
+        self.space = space
+        self.integrals = []
+        self.elmats = []
+        self.mat = None
+
+    def __iadd__(self, o):
+        if isinstance(o, _SumOfIntegrals):
+            self.integrals += o.parts
+        else:
+            raise TypeError("BilinearForm += expects cut integrals")
+        return self
+
+    def Assemble(self, precision=None):
+        prec = ngsxfemglobals.elmat_precision if precision is None else precision
+        mesh = self.space.mesh
+        p = self.space.order
+        self.elmats = [ci.element_matrices(mesh, p, prec) for ci in self.integrals]
+        if p == 1:
+            from scipy.sparse import coo_matrix
+            nd = mesh.dim + 1
+            rows, cols, vals = [], [], []
+            for els, mats in self.elmats:
+                dofs = mesh.el2v[els]
+                rows.append(np.repeat(dofs, nd, axis=1).ravel())
+                cols.append(np.tile(dofs, (1, nd)).ravel())
+                vals.append(np.asarray(mats, dtype=np.float64).ravel())
+            n = mesh.nv
+            self.mat = coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                                  shape=(n, n)).tocsr()
+        return self

@@ -1,0 +1,131 @@
+"""Learned quadrature and fixed-node element matrices on the GPU, through the
+C ABI, checked against the FP64 oracle.
+
+* MLP gradients of the library's FP32 backward kernels vs the oracle's FP64
+  backward (itself pinned by finite differences): 1e-4 relative.
+* BF16 MFMA inference vs the FP32 forward: BF16 rounding of activations and
+  weights, 3e-2 relative (Frobenius) -- also validates the MFMA lane layout.
+* A trained network beats the untrained one by > 10x and beats the plain
+  rule at subdivision level 1 (the cheap plain alternative) against the
+  level-4 plain rule.
+* Element matrices from fixed-node weights equal those on the materialised
+  subdivision rule (exactness of the degree-4 lattice for P2 integrands).
+"""
+import numpy as np
+import pytest
+
+from ngsxfem_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+NEG, POS, IF = 0, 1, 2
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+    assert torch.cuda.is_available()
+    from ngsxfem_amd import _abi
+    ctx = _abi.Context(0)
+    yield torch, _abi, ctx
+    ctx.close()
+
+
+def test_mlp_gradient_vs_oracle(orc, dev):
+    torch, _abi, ctx = dev
+    I, H, O, B = 10, 128, 35, 64
+    rng = np.random.default_rng(0)
+    npar = orc.mlp_nparams(I, H, O)
+    P = (0.2 * rng.standard_normal(npar)).astype(np.float32)
+    x = rng.standard_normal((B, I)).astype(np.float32)
+    t = rng.standard_normal((B, O)).astype(np.float32)
+    loss_ref, g_ref = orc.mlp_loss_grad(I, H, O, P.astype(np.float64), x.astype(np.float64), t.astype(np.float64))
+    dP = torch.from_numpy(P).cuda()
+    g = torch.zeros(npar, dtype=torch.float32, device="cuda")
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    work = torch.zeros(4 * B * H + 2 * B * O, dtype=torch.float32, device="cuda")
+    _abi.check(_abi.lib().xg_mlp_train_step(ctx.h, I, H, O, _abi.ptr(dP), None, None, 1, 0.0, B,
+                                            _abi.ptr(torch.from_numpy(x).cuda()), _abi.ptr(torch.from_numpy(t).cuda()),
+                                            _abi.ptr(loss), _abi.ptr(g), _abi.ptr(work)))
+    ctx.sync()
+    gg = g.cpu().numpy().astype(np.float64)
+    assert abs(loss.item() - loss_ref) < 1e-5 * abs(loss_ref)
+    assert np.linalg.norm(gg - g_ref) < 1e-4 * np.linalg.norm(g_ref)
+    # parameters untouched when lr = 0
+    assert np.array_equal(dP.cpu().numpy(), P)
+
+
+def test_mlp_forward_fp32_and_bf16(orc, dev):
+    torch, _abi, ctx = dev
+    I, H, O, B = 10, 128, 35, 1000
+    rng = np.random.default_rng(1)
+    npar = orc.mlp_nparams(I, H, O)
+    P = (0.3 * rng.standard_normal(npar)).astype(np.float32)
+    x = rng.uniform(-1, 1, (B, I)).astype(np.float32)
+    ref = orc.mlp_forward(I, H, O, P.astype(np.float64), x.astype(np.float64))
+    dP = torch.from_numpy(P).cuda()
+    dx = torch.from_numpy(x).cuda()
+    y32 = torch.zeros(B * O, dtype=torch.float32, device="cuda")
+    y16 = torch.zeros(B * O, dtype=torch.float32, device="cuda")
+    work = torch.zeros(2 * B * H, dtype=torch.float32, device="cuda")
+    lib = _abi.lib()
+    _abi.check(lib.xg_mlp_forward(ctx.h, I, H, O, _abi.ptr(dP), B, _abi.ptr(dx), _abi.ptr(y32), _abi.ptr(work),
+                                  _abi.XG_PREC_FP32))
+    _abi.check(lib.xg_mlp_forward(ctx.h, I, H, O, _abi.ptr(dP), B, _abi.ptr(dx), _abi.ptr(y16), None,
+                                  _abi.XG_PREC_BF16))
+    ctx.sync()
+    a32 = y32.cpu().numpy().reshape(B, O)
+    a16 = y16.cpu().numpy().reshape(B, O)
+    assert np.abs(a32 - ref).max() < 1e-4 * max(1, np.abs(ref).max())
+    assert np.linalg.norm(a16 - ref) < 3e-2 * np.linalg.norm(ref)
+
+
+def test_learned_rule_gain(orc, dev):
+    """training on the GPU reduces the error against the level-4 plain rule by
+    > 10x and beats the level-1 plain rule"""
+    torch, _abi, ctx = dev
+    from ngsxfem_amd.learned import LearnedRule, random_cut_quadrics
+    lr_ = LearnedRule(ctx, seed=0)
+    untrained = LearnedRule(ctx, seed=0)
+    untrained.fold()
+    hist = lr_.train(n_train=40000, epochs=8, batch=512, lr=2e-3, L=4, seed=3)
+    xv = random_cut_quadrics(4000, 99)
+    w4, _, _ = lr_.plain_weights(xv, L=4)
+    w1, _, _ = lr_.plain_weights(xv, L=1)
+    p = lr_.predict(xv).to(torch.float64)
+    pu = untrained.predict(xv).to(torch.float64)
+    ctx.sync()
+    def err(a):
+        return float(torch.linalg.norm(a - w4) / torch.linalg.norm(w4))
+    e_nn, e_un, e_l1 = err(p), err(pu), err(w1)
+    print("rel err vs L4: trained %.3e untrained %.3e plain L1 %.3e" % (e_nn, e_un, e_l1))
+    assert e_nn < e_un / 10
+    assert e_nn < e_l1
+    # fallback: threshold 0 -> everything falls back to the plain weights
+    out, acc, rel = lr_.with_fallback(p.to(torch.float32).contiguous(), w4, thr=0.0)
+    ctx.sync()
+    assert int(acc.sum().item()) == 0
+    assert torch.equal(out, w4)
+    out, acc, rel = lr_.with_fallback(p.to(torch.float32).contiguous(), w4, thr=1e30)
+    ctx.sync()
+    assert int(acc.sum().item()) == w4.shape[0]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_fixed_node_elmat_matches_materialised_rule(orc, dev, prec):
+    torch, _abi, ctx = dev
+    vals = W.random_quadric_p2(24, seed=5)
+    n = vals.shape[0]
+    wn = torch.zeros(n * 35, dtype=torch.float64, device="cuda")
+    ctx.subdiv_weights(torch.from_numpy(vals).cuda(), 2, 2, 4, 4, wn, None, None)
+    P = _abi.XG_PREC_FP64 if prec == "fp64" else _abi.XG_PREC_FP32
+    out = torch.zeros(n * 100, dtype=torch.float64 if prec == "fp64" else torch.float32, device="cuda")
+    _abi.check(_abi.lib().xg_elmat_weights(ctx.h, 2, 4, n, None, None, None, _abi.ptr(wn), 3, P, _abi.ptr(out)))
+    ctx.sync()
+    g = out.cpu().numpy().astype(np.float64).reshape(n, 10, 10)
+    Vref = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [0, 0, 0]], float)
+    for e in range(n):
+        dom, rules = orc.subdiv_rule3(vals[e], 2, 2, 4)
+        p, w = rules[NEG]
+        ref = orc.elmat(3, 2, Vref, p, w, mask=3)
+        err = np.linalg.norm(g[e] - ref) / np.linalg.norm(ref)
+        assert err < (1e-12 if prec == "fp64" else 1e-6), (e, err)

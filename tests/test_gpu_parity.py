@@ -89,7 +89,10 @@ def test_mark_fuzz_special_values(orc, dev, D):
     l = rng.choice(choices, size=ne * (D + 1)) * rng.uniform(0.5, 2.0, size=ne * (D + 1))
     l = l.reshape(ne, D + 1)
     # exact zeros only in elements without negative values
+    # (an all-zero element raises as well, so one entry stays positive)
     z = rng.random((ne, D + 1)) < 0.1
+    z[:, 0] = False
+    l[:, 0] = np.abs(l[:, 0])
     nonneg = (l >= 0).all(axis=1)
     l[z & nonneg[:, None]] = 0.0
     l = l.reshape(-1)

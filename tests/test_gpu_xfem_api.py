@@ -1,0 +1,157 @@
+"""The reference's user-facing tests, through this build's `xfem` surface
+(pybind11 module over the HIP C ABI), on the GPU.
+
+Transcribed known answers: tests/pytests/test_straightcutrule.py:111-126,
+191-207 (closed-form cut measures), test_localgeom.py:24-45; CutInfo /
+GetFacetsWithNeighborTypes usage as in tests/pytests/test_straightcutrule.py:
+329-331; dCut bilinear forms as in test_differential_symbol.py:15-72 (dCut
+vs SymbolicCutBFI must agree).
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def X():
+    import torch
+    assert torch.cuda.is_available()
+    from ngsxfem_amd import xfem
+    return xfem
+
+
+def tol(order):
+    return 5e-15 * (order + 1) ** 2
+
+
+@pytest.mark.parametrize("order", [2, 4, 8])
+@pytest.mark.parametrize("domain", ["NEG", "POS", "IF"])
+@pytest.mark.parametrize("N", [1, 10, 30])
+def test_integrate_straight_cut_2d(X, order, domain, N):
+    mesh = X.MakeStructured2DMesh(nx=N, ny=N)
+    lset = X.GridFunction(X.H1(mesh, order=1))
+    X.InterpolateToP1(1 - 2 * X.x - 2 * X.y, lset)
+    ref = {"NEG": 7 / 8, "POS": 1 / 8, "IF": 1 / math.sqrt(2)}
+    dt = getattr(X, domain)
+    val = X.Integrate(levelset_domain={"levelset": lset, "domain_type": dt}, cf=X.CoefficientFunction(1),
+                      mesh=mesh, order=order)
+    assert abs(val - ref[domain]) < tol(order)
+
+
+@pytest.mark.parametrize("order", [2, 4, 8])
+@pytest.mark.parametrize("domain", ["NEG", "POS", "IF"])
+def test_integrate_straight_cut_3d(X, order, domain):
+    mesh = X.MakeStructured3DMesh(nx=1, ny=1, nz=1)
+    lset = X.GridFunction(X.H1(mesh, order=1))
+    X.InterpolateToP1(1 - 2 * X.x - 2 * X.y - 2 * X.z, lset)
+    ref = {"POS": 1 / 48, "NEG": 47 / 48, "IF": math.sqrt(3) / 8}
+    val = X.Integrate({"levelset": lset, "domain_type": getattr(X, domain)}, cf=1, mesh=mesh, order=order)
+    assert abs(val - ref[domain]) < tol(order)
+
+
+@pytest.mark.parametrize("alpha", [0, 1, 2])
+def test_integrate_polynomial_3d(X, alpha):
+    mesh = X.MakeStructured3DMesh(nx=1, ny=1, nz=1)
+    lset = X.GridFunction(X.H1(mesh, order=1))
+    X.InterpolateToP1(1 - 2 * X.x - 2 * X.y - 2 * X.z, lset)
+    val_pos = 2 ** (-alpha - 3) / (alpha ** 3 + 6 * alpha * alpha + 11 * alpha + 6)
+    for dt, ref in ((X.POS, val_pos), (X.NEG, 1 / (alpha + 1) - val_pos)):
+        val = X.Integrate({"levelset": lset, "domain_type": dt}, cf=X.x ** alpha, mesh=mesh, order=4)
+        assert abs(val - ref) < tol(4)
+
+
+def test_integrate_nonpolynomial_cf(X):
+    """general coefficient functions are evaluated at the GPU-built rule"""
+    mesh = X.MakeStructured2DMesh(nx=20, ny=20)
+    lset = X.GridFunction(X.H1(mesh, order=1))
+    X.InterpolateToP1(1 - 2 * X.x - 2 * X.y, lset)
+    v1 = X.Integrate({"levelset": lset, "domain_type": X.NEG}, cf=X.sqrt(X.x * X.x + 1), mesh=mesh, order=8)
+    # reference by a fine midpoint sum over the exact region x + y > 1/2
+    n = 2000
+    g = (np.arange(n) + 0.5) / n
+    xx, yy = np.meshgrid(g, g, indexing="ij")
+    ref = np.sum(np.sqrt(xx ** 2 + 1) * (xx + yy > 0.5)) / n / n
+    assert abs(v1 - ref) < 2e-3
+
+
+def test_cutinfo_and_facets(X):
+    from ngsxfem_amd import workloads as W
+    from oracle import oracle as O
+    mesh, lsetv = W.config2(8)
+    lset = X.GridFunction(X.H1(mesh, order=1))
+    lset.vec = lsetv
+    ci = X.CutInfo(mesh, lset)
+    dt, cr, bits = O.mark(3, mesh.el2v, lsetv)
+    for cdt in range(8):
+        ba = ci.GetElementsOfType(X.COMBINED_DOMAIN_TYPE(cdt))
+        assert np.array_equal(ba.words(), bits[cdt])
+    assert ci.GetElementsOfType(X.IF).NumSet() == int(np.sum(dt == 2))
+    assert np.array_equal(ci.GetCutRatios().view(np.int64), cr.view(np.int64))
+    # the returned BitArray is the live array: Update mutates it
+    hasneg = ci.GetElementsOfType(X.HASNEG)
+    n0 = hasneg.NumSet()
+    lset2 = X.GridFunction(X.H1(mesh, order=1))
+    lset2.vec = lsetv - 0.1
+    ci.Update(lset2)
+    assert hasneg.NumSet() > n0
+    # boundary elements are marked too
+    assert len(ci.GetElementsOfType(X.IF, X.BND)) == mesh.nse
+    th = ci.GetElementsWithThresholdContribution(X.NEG, 0.5)
+    assert th.NumSet() == int(np.sum(ci.GetCutRatios() >= 0.5))
+    ci.Update(lset)
+    ba = ci.GetElementsOfType(X.IF)
+    fa = X.GetFacetsWithNeighborTypes(mesh, a=ba, b=ba, use_and=True)
+    f2v, el2f, f2el = mesh.facets()
+    ifm = dt == 2
+    ref = np.array([(ifm[l] and r >= 0 and ifm[r]) for l, r in f2el])
+    assert np.array_equal(fa.numpy(), ref)
+    with pytest.raises(Exception):
+        ci.GetElementsWithThresholdContribution(X.IF, 0.5)
+
+
+def test_localgeom_cut_triangle(X):
+    coords = np.array([[0, 0], [1, 0], [0, 1], [0.5, 0], [0.5, 0.5], [0, 0.5]], float)
+    el = np.array([[0, 3, 5], [3, 1, 4], [5, 4, 2], [3, 4, 5]], np.int32)
+    mesh = X.Mesh(coords, el)
+    lset = X.GridFunction(X.H1(mesh, order=1))
+    X.InterpolateToP1(X.x + X.y - 0.25, lset)
+    for order in range(16):
+        neg = X.Integrate({"levelset": lset, "domain_type": X.NEG, "subdivlvl": 0}, cf=1.0, mesh=mesh, order=order)
+        pos = X.Integrate({"levelset": lset, "domain_type": X.POS, "subdivlvl": 0}, cf=1.0, mesh=mesh, order=order)
+        t = 5e-16 * (order + 1) ** 2
+        assert abs(neg - 1 / 32) < t and abs(pos - 0.5 + 1 / 32) < t
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_dcut_bilinear_form(X, prec):
+    """dCut form == SymbolicCutBFI form (test_differential_symbol.py:15-72);
+    the assembled P1 matrix reproduces int_NEG 1 = c1^T M c1 and K c1 = 0"""
+    mesh = X.MakeStructured2DMesh(nx=24, ny=24)
+    lset = X.GridFunction(X.H1(mesh, order=1))
+    X.InterpolateToP1(X.sqrt((X.x - 0.5) ** 2 + (X.y - 0.5) ** 2) - 0.3, lset)
+    ci = X.CutInfo(mesh, lset)
+    V = X.H1(mesh, order=1)
+    u, v = V.TnT()
+    a = X.BilinearForm(V)
+    a += (X.grad(u) * X.grad(v) + u * v) * X.dCut(lset, X.NEG, definedonelements=ci.GetElementsOfType(X.HASNEG))
+    a.Assemble(precision=prec)
+    b = X.BilinearForm(V)
+    b += X.SymbolicCutBFI({"levelset": lset, "domain_type": X.NEG}, X.grad(u) * X.grad(v) + u * v,
+                          definedonelements=ci.GetElementsOfType(X.HASNEG))
+    b.Assemble(precision=prec)
+    assert abs(a.mat - b.mat).max() < 1e-12
+    one = np.ones(mesh.nv)
+    vol = X.Integrate({"levelset": lset, "domain_type": X.NEG}, cf=1, mesh=mesh, order=2)
+    m = X.BilinearForm(V)
+    m += (u * v) * X.dCut(lset, X.NEG, definedonelements=ci.GetElementsOfType(X.HASNEG))
+    m.Assemble(precision=prec)
+    k = X.BilinearForm(V)
+    k += (X.grad(u) * X.grad(v)) * X.dCut(lset, X.NEG, definedonelements=ci.GetElementsOfType(X.HASNEG))
+    k.Assemble(precision=prec)
+    t = 1e-12 if prec == 0 else 1e-6
+    assert abs(one @ (m.mat @ one) - vol) < t * vol
+    assert np.abs(k.mat @ one).max() < t * abs(k.mat).max() * 10
+    assert abs(vol - math.pi * 0.09) < 2e-3
