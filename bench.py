@@ -38,6 +38,10 @@ def parse():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--config", default="c2", choices=["c2", "c4"],
+                    help="c2: 3D plain straight-cut pipeline (headline); c4: learned quadrature on P2-cut tets")
+    ap.add_argument("--c4-n", type=int, default=2000000)
+    ap.add_argument("--c4-train", type=int, default=100000)
     return ap.parse_args()
 
 
@@ -51,8 +55,87 @@ def read_traffic(name):
     return float(m.group(1)) if m else None
 
 
+def bench_c4(args):
+    """Learned quadrature (P2 level sets on single tets): plain subdivision
+    rule (level 4, FP64) vs the learned rule (BF16 MLP), both followed by the
+    P2 mass + stiffness element matrices on the fixed-node rule (FP32)."""
+    import torch
+
+    from ngsxfem_amd import _abi
+    from ngsxfem_amd.learned import LearnedRule, random_cut_quadrics
+
+    stream = torch.cuda.current_stream()
+    ctx = _abi.Context(0, stream=stream)
+    n = args.c4_n
+    t0 = time.time()
+    lr_ = LearnedRule(ctx, seed=0)
+    hist = lr_.train(n_train=args.c4_train, epochs=10, batch=1024, lr=2e-3, L=4, seed=3)
+    t_train = time.time() - t0
+    x = random_cut_quadrics(n, 42)
+    xn = None
+    from ngsxfem_amd.learned import normalise_inputs
+    xn = normalise_inputs(x)
+    wp = torch.empty(n, 35, dtype=torch.float64, device="cuda")
+    wl = torch.empty(n, 35, dtype=torch.float32, device="cuda")
+    wl64 = torch.empty(n, 35, dtype=torch.float64, device="cuda")
+    em = torch.empty(n * 100, dtype=torch.float32, device="cuda")
+
+    def plain():
+        ctx.subdiv_weights(x, 2, 4, 4, 4, wp, None, None)
+        _abi.check(_abi.lib().xg_elmat_weights(ctx.h, 2, 4, n, None, None, None, _abi.ptr(wp), 3,
+                                               _abi.XG_PREC_FP32, _abi.ptr(em)))
+
+    def learned():
+        lr_.predict(x, x_norm=xn, out=wl)
+        wl64.copy_(wl)
+        _abi.check(_abi.lib().xg_elmat_weights(ctx.h, 2, 4, n, None, None, None, _abi.ptr(wl64), 3,
+                                               _abi.XG_PREC_FP32, _abi.ptr(em)))
+
+    def timed(f, reps):
+        f()
+        ctx.sync()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            f()
+        b.record(stream)
+        ctx.sync()
+        return a.elapsed_time(b) / reps
+
+    ms_plain = timed(plain, 2)
+    ms_learned = timed(learned, 10)
+    err = (torch.linalg.norm(wl.to(torch.float64) - wp, dim=1) / torch.linalg.norm(wp, dim=1)).cpu().numpy()
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        cores = min(16, os.cpu_count() or 1)
+        ns = 4000
+        xs = x[:ns].cpu().numpy()
+        t1 = time.perf_counter()
+        O.subdiv_moments3_batch(xs, 2, 4, 4, 4, nthreads=cores)
+        dt_ = time.perf_counter() - t1
+        cpu = {"value": ns / dt_, "unit": "cut-elem/s", "cores": cores, "kind": "port",
+               "sample": "%d of the %d tets, subdivision level 4 weights only (FP64 oracle, %d threads, %.1fs)"
+                         % (ns, n, cores, dt_)}
+    res = {"metric": METRIC, "config": {"workload": "learned quadrature: %d cut tets, random quadric P2 level "
+                                                    "sets, plain = subdivision L=4 FP64 + P2 elmat FP32, learned = "
+                                                    "MLP 10-128-128-35 BF16 + P2 elmat FP32" % n},
+           "plain": {"ms": ms_plain, "cut_elem_per_s": n / (ms_plain * 1e-3)},
+           "learned": {"ms": ms_learned, "cut_elem_per_s": n / (ms_learned * 1e-3),
+                       "rel_err_vs_plain_median": float(np.median(err)),
+                       "rel_err_vs_plain_p99": float(np.percentile(err, 99)),
+                       "frac_within_1e-6": float(np.mean(err <= 1e-6)),
+                       "frac_within_1e-3": float(np.mean(err <= 1e-3))},
+           "train_s": t_train, "train_loss_last": hist[-1], "cpu_baseline": cpu}
+    print(json.dumps(res), flush=True)
+    ctx.close()
+
+
 def main():
     args = parse()
+    if args.config == "c4":
+        return bench_c4(args)
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the path shards by element "
               "ranges; see DESIGN.md); run with --gpus 1", file=sys.stderr)

@@ -323,7 +323,9 @@ py::object IntegrateX(py::dict lsetdom, py::object mesh, py::object cf, py::obje
   const int D = ma->dim;
   std::vector<double> pe(ma->ne, 0.0);
   double total = 0.0;
-  if (cf.is_none() || py::hasattr(cf, "poly_terms") || py::isinstance<py::float_>(cf) || py::isinstance<py::int_>(cf)) {
+  const bool poly = cf.is_none() || py::isinstance<py::float_>(cf) || py::isinstance<py::int_>(cf) ||
+                    (py::hasattr(cf, "poly_terms") && (!py::hasattr(cf, "is_poly") || cf.attr("is_poly")().cast<bool>()));
+  if (poly) {
     std::vector<double> coef;
     std::vector<int32_t> ex;
     if (cf.is_none()) {

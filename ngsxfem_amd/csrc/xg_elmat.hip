@@ -82,20 +82,22 @@ constexpr int PIPE_NQMAX = 125;  // base rule points (order <= 8 tets)
 template <int D, typename T>
 struct StageLds {
   double P[D + 5][3];  // reference vertices, then cut points (FP64)
-  T V[3][4][3];        // sub-simplex vertices (mapping order)
-  T wsc[3];            // volume * interface factor per sub-simplex
+  double V[3][4][3];   // sub-simplex vertices (mapping order, FP64)
+  double wsc[3];       // volume * interface factor per sub-simplex
 };
 
 template <int D, typename T>
 struct StagedSrc {
   StageLds<D, T> *st;
-  const T (*R)[4];  // base rule (xi0, xi1, xi2, w), in LDS
+  const double (*R)[4];  // base rule (xi0, xi1, xi2, w), in LDS
   int nq, nv1;      // points per sub-simplex, vertices per sub-simplex
-  __device__ void point(int g, T *x, T &w) const {
+  // point mapping in FP64 (the barycentric lam_D = 1 - sum xi of the element
+  // is formed from these coordinates; FP32 here costs ~1e-6 for P3)
+  __device__ void point(int g, double *x, double &w) const {
     const int s = (g >= nq) + (g >= 2 * nq);
     const int k = g - s * nq;
-    const T *r = R[k];
-    T ow = T(1);
+    const double *r = R[k];
+    double ow = 1.0;
     for (int m = 0; m + 1 < nv1; ++m) ow -= r[m];
 #pragma unroll
     for (int d = 0; d < 3; ++d) x[d] = ow * st->V[s][0][d];
@@ -121,9 +123,9 @@ __device__ int build_stage(StageLds<D, T> *st, const double *l, const double *Fi
     // uncut: the base rule itself = one "sub-simplex" (origin, e_1, .., e_D)
     if (lane < (D + 1) * 3) {
       const int m = lane / 3, d = lane % 3;
-      st->V[0][m][d] = (m > 0 && d == m - 1) ? T(1) : T(0);
+      st->V[0][m][d] = (m > 0 && d == m - 1) ? 1.0 : 0.0;
     }
-    if (lane == 0) st->wsc[0] = T(1);
+    if (lane == 0) st->wsc[0] = 1.0;
     nq = nq_vol;
     nv1 = D + 1;
     wave_fence();
@@ -166,11 +168,11 @@ __device__ int build_stage(StageLds<D, T> *st, const double *l, const double *Fi
   if (dt == IF) ifac = interface_factor<D>(l, Finv);
   if (lane < ns) {
     const double tf = simplex_volume(nv, E.v[lane], st->P);
-    st->wsc[lane] = (T)(tf * ifac);
+    st->wsc[lane] = tf * ifac;
   }
   if (lane < ns * nv * 3) {
     const int s = lane / (nv * 3), m = (lane / 3) % nv, d = lane % 3;
-    st->V[s][m][d] = (T)st->P[E.v[s][m]][d];
+    st->V[s][m][d] = st->P[E.v[s][m]][d];
   }
   wave_fence();
   nq = (dt == IF) ? nq_if : nq_vol;
@@ -204,17 +206,17 @@ __device__ __forceinline__ void elmat_one(const SRC &src, int npts, const double
       T phi[N], dphi[N][D];
       T w = T(0);
       if (g < npts) {
-        T x[3], wr;
+        double x[3], wr;
         src.point(g, x, wr);
-        w = wr * (T)absdet;
+        w = (T)(wr * absdet);
         T lam[D + 1];
-        T s = T(1);
+        double s = 1.0;
 #pragma unroll
         for (int m = 0; m < D; ++m) {
-          lam[m] = x[m];
-          s -= lam[m];
+          lam[m] = (T)x[m];
+          s -= x[m];
         }
-        lam[D] = s;
+        lam[D] = (T)s;
         h1_shape<D, P, T>(lam, phi, dphi);
       } else {
 #pragma unroll
@@ -238,15 +240,24 @@ __device__ __forceinline__ void elmat_one(const SRC &src, int npts, const double
       }
     }
     wave_sync();
-    for (int pp = 0; pp < nch; ++pp) {
-      const T w = sW[pp];
-      T av[NB];
+    // contraction over the chunk's points, 8 at a time: all LDS operand loads
+    // of a group are issued before its MFMAs (points past npts carry zero
+    // weight and zero shape rows, so rounding nch up to 8 adds nothing)
+    const int nch8 = min(CH, (nch + 7) & ~7);
+    for (int p0 = 0; p0 < nch8; p0 += 8) {
+      T av[8][NB], wv[8];
 #pragma unroll
-      for (int b = 0; b < NB; ++b) av[b] = sB[(c * NB * 16 + b * 16 + ii) * PAD + pp];
+      for (int u = 0; u < 8; ++u) {
+        wv[u] = sW[p0 + u];
 #pragma unroll
-      for (int bi = 0; bi < NB; ++bi)
+        for (int b = 0; b < NB; ++b) av[u][b] = sB[(c * NB * 16 + b * 16 + ii) * PAD + p0 + u];
+      }
 #pragma unroll
-        for (int bj = bi; bj < NB; ++bj) acc[bi][bj] = Mfma<T>::run(av[bi], av[bj] * w, acc[bi][bj]);
+      for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int bi = 0; bi < NB; ++bi)
+#pragma unroll
+          for (int bj = bi; bj < NB; ++bj) acc[bi][bj] = Mfma<T>::run(av[u][bi], av[u][bj] * wv[u], acc[bi][bj]);
     }
     wave_sync();
   }
@@ -286,7 +297,7 @@ __global__ __launch_bounds__(128) void k_pipeline(int nel, const int32_t *__rest
   using C = Cfg<D, P, T>;
   extern __shared__ char smem[];
   __shared__ StageLds<D, T> s_stage[C::WAVES];
-  __shared__ T s_rule[2][PIPE_NQMAX][4];  // base rules of dimension D and D-1
+  __shared__ double s_rule[2][PIPE_NQMAX][4];  // base rules of dimension D and D-1
   const int wv = threadIdx.x >> 6;
   T *sB = reinterpret_cast<T *>(smem) + (size_t)wv * (C::LDS_PER_WAVE / sizeof(T));
   T *sW = sB + 4 * C::NB * 16 * C::PAD;
@@ -300,7 +311,7 @@ __global__ __launch_bounds__(128) void k_pipeline(int nel, const int32_t *__rest
     const int kk = which ? k - nq_vol : k;
     const double *r = xg_rule_data_d[(which ? off_if : off_vol) + kk];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) s_rule[which][kk][c] = (T)r[c];
+    for (int c = 0; c < 4; ++c) s_rule[which][kk][c] = r[c];
   }
   __syncthreads();  // the only block barrier: before the per-wave loops
   const int wid = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
@@ -328,12 +339,12 @@ template <typename T>
 struct CsrSrc {
   const double4 *pts;
   int base;
-  __device__ void point(int g, T *x, T &w) const {
+  __device__ void point(int g, double *x, double &w) const {
     const double4 p = pts[base + g];
-    x[0] = (T)p.x;
-    x[1] = (T)p.y;
-    x[2] = (T)p.z;
-    w = (T)p.w;
+    x[0] = p.x;
+    x[1] = p.y;
+    x[2] = p.z;
+    w = p.w;
   }
 };
 
@@ -371,12 +382,12 @@ __global__ __launch_bounds__(128) void k_elmat_csr(int nel, const int32_t *__res
 template <int Q, typename T>
 struct WeightSrc {
   const double *w;
-  __device__ void point(int g, T *x, T &wt) const {
+  __device__ void point(int g, double *x, double &wt) const {
     constexpr Lattice<3, Q> L{};
-    x[0] = (T)L.a[g][0] / (T)Q;
-    x[1] = (T)L.a[g][1] / (T)Q;
-    x[2] = (T)L.a[g][2] / (T)Q;
-    wt = (T)w[g];
+    x[0] = (double)L.a[g][0] / Q;
+    x[1] = (double)L.a[g][1] / Q;
+    x[2] = (double)L.a[g][2] / Q;
+    wt = w[g];
   }
 };
 

@@ -43,9 +43,11 @@ def test_mlp_gradient_vs_oracle(orc, dev):
     g = torch.zeros(npar, dtype=torch.float32, device="cuda")
     loss = torch.zeros(1, dtype=torch.float32, device="cuda")
     work = torch.zeros(4 * B * H + 2 * B * O, dtype=torch.float32, device="cuda")
+    dx = torch.from_numpy(x).cuda()  # keep the device tensors alive across the call
+    dt = torch.from_numpy(t).cuda()
     _abi.check(_abi.lib().xg_mlp_train_step(ctx.h, I, H, O, _abi.ptr(dP), None, None, 1, 0.0, B,
-                                            _abi.ptr(torch.from_numpy(x).cuda()), _abi.ptr(torch.from_numpy(t).cuda()),
-                                            _abi.ptr(loss), _abi.ptr(g), _abi.ptr(work)))
+                                            _abi.ptr(dx), _abi.ptr(dt), _abi.ptr(loss), _abi.ptr(g),
+                                            _abi.ptr(work)))
     ctx.sync()
     gg = g.cpu().numpy().astype(np.float64)
     assert abs(loss.item() - loss_ref) < 1e-5 * abs(loss_ref)

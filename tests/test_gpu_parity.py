@@ -233,7 +233,10 @@ def test_csr_elmat_matches_pipeline(orc, dev, prec):
     out2 = torch.zeros_like(out)
     ctx.cut_pipeline(D, p, order, d_el, d_x, up(torch, lset), [NEG], 3, P, out2, elems=d_e)
     ctx.sync()
-    assert torch.equal(out, out2)
+    # same rule, built in FP64 by different kernels (materialised vs staged)
+    a = out.cpu().numpy().astype(np.float64).reshape(-1, 10, 10)
+    b = out2.cpu().numpy().astype(np.float64).reshape(-1, 10, 10)
+    assert rel_fro(a, b).max() < (1e-13 if prec == "fp64" else 1e-6)
 
 
 def test_facets_with_neighbor_types(dev):
