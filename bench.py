@@ -38,7 +38,7 @@ def parse():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", default="c2", choices=["c2", "c4"],
+    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4"],
                     help="c2: 3D plain straight-cut pipeline (headline); c4: learned quadrature on P2-cut tets")
     ap.add_argument("--c4-n", type=int, default=2000000)
     ap.add_argument("--c4-train", type=int, default=100000)
@@ -132,10 +132,93 @@ def bench_c4(args):
     ctx.close()
 
 
+def bench_c3(args):
+    """Batched uneven: 256 cube meshes (n log-uniform in [8, 48], own ellipsoid
+    level set, own degree p in {1,2,3}, rule order 2p) in one captured launch
+    sequence (marking + plan + persistent fused rule/elmat kernels), FP32."""
+    import torch
+
+    from ngsxfem_amd import _abi, autotune
+    from ngsxfem_amd import workloads as W
+
+    stream = torch.cuda.current_stream()
+    ctx = _abi.Context(0, stream=stream)
+    t0 = time.time()
+    coords, el2v, lset, off, ps, ns, axes = W.config3(256)
+    print("# c3 workload built in %.1fs: %d tets" % (time.time() - t0, off[-1]), file=sys.stderr, flush=True)
+    d_x = torch.from_numpy(coords).cuda()
+    d_e = torch.from_numpy(el2v).cuda()
+    d_l = torch.from_numpy(lset).cuda()
+    batch = _abi.Batch(ctx, d_x, d_e, d_l, off, ps)
+    tuning = autotune.load()
+    autotune.apply(batch, tuning)
+    cnt, oo = batch.plan()
+    n_cut = int(cnt.sum())
+    out = torch.empty(int(oo[-1]), dtype=torch.float32, device="cuda")
+    for _ in range(args.warmup):
+        batch.run(out, use_graph=True)
+    ctx.sync()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(args.steps):
+        batch.run(out, use_graph=True)
+    b.record(stream)
+    ctx.sync()
+    ms = a.elapsed_time(b) / args.steps
+    # algorithmic FLOPs: 2 npts ndof^2 (1 + D) per (cut element, dt)
+    flops = 0.0
+    starts = np.concatenate([[0], np.cumsum(cnt)])
+    cut = batch.cut_list_host(n_cut)
+    for p in (1, 2, 3):
+        sel = np.concatenate([cut[starts[m]:starts[m + 1]] for m in range(len(ps)) if ps[m] == p] or [np.zeros(0, np.int32)])
+        if len(sel) == 0:
+            continue
+        nd = {1: 4, 2: 10, 3: 20}[p]
+        dsel = torch.from_numpy(sel.astype(np.int32)).cuda()
+        for dtv in (_abi.XG_NEG, _abi.XG_IF):
+            o = torch.zeros(len(sel) + 1, dtype=torch.int32, device="cuda")
+            ctx.cut_rule_count(3, d_e, d_l, dtv, 2 * p, o, elems=dsel)
+            ctx.sync()
+            flops += 2.0 * int(o[-1].item()) * nd * nd * 4
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        cores = min(16, os.cpu_count() or 1)
+        t_cpu, done = 0.0, 0
+        for m in range(len(ps)):
+            if t_cpu > args.cpu_seconds:
+                break
+            seg = cut[starts[m]:starts[m + 1]]
+            if len(seg) == 0:
+                continue
+            t1 = time.perf_counter()
+            O.cut_pipeline(3, int(ps[m]), 2 * int(ps[m]), seg, el2v, coords, lset, [0, 2], mask=3, nthreads=cores)
+            t_cpu += time.perf_counter() - t1
+            done += len(seg)
+        cpu = {"value": done / t_cpu, "unit": "cut-elem/s", "cores": cores, "kind": "port",
+               "sample": "the first meshes' %d cut tets (mixed p, FP64 oracle, %d threads, %.1fs)" % (done, cores, t_cpu)}
+    res = {"metric": METRIC, "value": n_cut / (ms * 1e-3), "unit": "cut-elem/s", "ms_per_step": ms,
+           "config": {"workload": "batched uneven: 256 cube meshes n in [8,48] log-uniform (%d tets), ellipsoid "
+                                  "P1 level sets, p in {1,2,3}, rule order 2p, NEG+IF, FP32 MFMA, one hipGraph "
+                                  "(marking + plan + persistent kernels)" % int(off[-1]),
+                      "cut_elements": n_cut, "meshes_per_p": [int(np.sum(ps == q)) for q in (1, 2, 3)],
+                      "tuning": tuning},
+           "roofline": {"bound": "mfma", "achieved": flops / (ms * 1e-3) / 1e12, "peak": PEAK_FP32_MFMA / 1e12,
+                        "unit": "TFLOP/s", "frac": flops / (ms * 1e-3) / PEAK_FP32_MFMA,
+                        "note": "whole graph time (marking of all tets included)"},
+           "cpu_baseline": cpu}
+    print(json.dumps(res), flush=True)
+    batch.close()
+    ctx.close()
+
+
 def main():
     args = parse()
     if args.config == "c4":
         return bench_c4(args)
+    if args.config == "c3":
+        return bench_c3(args)
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the path shards by element "
               "ranges; see DESIGN.md); run with --gpus 1", file=sys.stderr)

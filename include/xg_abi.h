@@ -152,6 +152,38 @@ int xg_subdiv_weights(xg_ctx *ctx, int nel, const double *lset_nodal, int lset_o
 int xg_elmat_weights(xg_ctx *ctx, int p, int q, int nel, const int32_t *elems, const int32_t *el2v,
                      const double *coords, const double *weights, int mask, int precision, void *out);
 
+/* ---- batched multi-mesh pipeline ---------------------------------------
+   No single reference counterpart: many independent meshes (each one
+   CutInformation::Update + BilinearForm::Assemble of a dCut form in the
+   reference) concatenated into one device-side sequence -- marking and
+   compaction over all elements, a per-mesh plan, and one persistent fused
+   rule + element-matrix kernel per H1 degree with dynamic work distribution,
+   captured once into a hipGraph and replayed.
+   Mesh m owns elements [elem_off[m], elem_off[m+1]) of the concatenated
+   el2v (global vertex ids) and uses degree mesh_p[m] (1..3) with rule order
+   2 p (host arrays, copied).  el2v / coords / lset are device arrays that must
+   stay valid; D == 3, nmesh <= 1024.  Output (FP32): per mesh, its cut
+   elements in ascending order, ndt x ndof(p)^2 floats each, at out_off[m]. */
+typedef struct xg_batch xg_batch;
+int xg_batch_create(xg_ctx *ctx, int D, int nmesh, const int32_t *elem_off, const int32_t *mesh_p, int ndt,
+                    const int *dts, int mask, const int32_t *el2v, const double *coords, const double *lset,
+                    xg_batch **out);
+int xg_batch_destroy(xg_batch *b);
+/* launch shape of the degree-p kernel: waves per block (1|2), resident
+   blocks per CU, elements per work grab */
+int xg_batch_tune(xg_batch *b, int p, int waves_per_block, int blocks_per_cu, int grab);
+/* marking + plan only; then xg_batch_info */
+int xg_batch_plan(xg_ctx *ctx, xg_batch *b);
+/* synchronises; host outputs (nullable): cut elements per mesh (nmesh),
+   output offsets in floats (nmesh + 1) */
+int xg_batch_info(xg_ctx *ctx, xg_batch *b, int32_t *cut_per_mesh, long long *out_off);
+/* device array of the cut element ids of the last plan / run */
+const int32_t *xg_batch_cut_list(xg_batch *b);
+/* full batch into out (cap floats; overflow -> XG_ERR_CAPACITY at xg_sync).
+   use_graph != 0: capture once per (out, cap, tuning) and replay.
+   only_p != 0: run the kernel of that degree only (eager; for tuning). */
+int xg_batch_run(xg_ctx *ctx, xg_batch *b, float *out, long long cap, int use_graph, int only_p);
+
 /* ---- learned rule: MLP I -> H -> H -> O (tanh), FP32 parameters --------
    No reference counterpart: the rebuild's learned quadrature (predicts the
    fixed-node weights of the NEG part of a P2-cut tet from its normalised

@@ -47,6 +47,12 @@ _SIGS = {
     "xg_mlp_forward": [_vp, _i, _i, _i, _vp, _i, _vp, _vp, _vp, _i],
     "xg_mlp_train_step": [_vp, _i, _i, _i, _vp, _vp, _vp, _i, ctypes.c_float, _i, _vp, _vp, _vp, _vp, _vp],
     "xg_moment_residual": [_vp, _i, _i, _vp, ctypes.c_double, _vp, ctypes.c_double, _vp, _vp, _vp],
+    "xg_batch_create": [_vp, _i, _i, _vp, _vp, _i, _vp, _i, _vp, _vp, _vp, ctypes.POINTER(_vp)],
+    "xg_batch_destroy": [_vp],
+    "xg_batch_tune": [_vp, _i, _i, _i, _i],
+    "xg_batch_plan": [_vp, _vp],
+    "xg_batch_info": [_vp, _vp, _vp, _vp],
+    "xg_batch_run": [_vp, _vp, _vp, ctypes.c_longlong, _i, _i],
 }
 
 
@@ -55,7 +61,7 @@ def declared_symbols():
     import re
     hdr = os.path.join(HERE, "..", "include", "xg_abi.h")
     txt = open(hdr).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(xg_\w+)\s*\(", txt, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*|const int32_t \*)\s*(xg_\w+)\s*\(", txt, re.M)))
 
 
 _lib = None
@@ -72,6 +78,8 @@ def lib():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = _i
+        L.xg_batch_cut_list.argtypes = [_vp]
+        L.xg_batch_cut_list.restype = _vp
         L.xg_last_error.argtypes = []
         L.xg_last_error.restype = ctypes.c_char_p
         _lib = L
@@ -169,3 +177,66 @@ class Context:
         arr = (ctypes.c_int * len(dts))(*dts)
         check(lib().xg_cut_pipeline(self.h, D, p, order, nel, ptr(elems), ptr(el2v), ptr(coords),
                                     ptr(lset), len(dts), ctypes.cast(arr, _vp), mask, precision, ptr(out)))
+
+
+class Batch:
+    """xg_batch: many meshes, one captured launch sequence (see xg_abi.h)."""
+
+    def __init__(self, ctx, coords, el2v, lset, elem_off, mesh_p, dts=(XG_NEG, XG_IF), mask=3):
+        import numpy as np
+        self.ctx = ctx
+        self.nmesh = len(mesh_p)
+        self._keep = (coords, el2v, lset)  # device arrays must outlive the batch
+        off = (ctypes.c_int32 * (self.nmesh + 1))(*[int(v) for v in elem_off])
+        pp = (ctypes.c_int32 * self.nmesh)(*[int(v) for v in mesh_p])
+        d = (ctypes.c_int * len(dts))(*dts)
+        self.h = ctypes.c_void_p()
+        check(lib().xg_batch_create(ctx.h, 3, self.nmesh, ctypes.cast(off, _vp), ctypes.cast(pp, _vp), len(dts),
+                                    ctypes.cast(d, _vp), mask, ptr(el2v), ptr(coords), ptr(lset),
+                                    ctypes.byref(self.h)))
+        self.mesh_p = np.asarray(mesh_p)
+        self.ndt = len(dts)
+
+    def tune(self, p, waves, blocks_per_cu, grab):
+        check(lib().xg_batch_tune(self.h, p, waves, blocks_per_cu, grab))
+
+    def plan(self):
+        check(lib().xg_batch_plan(self.ctx.h, self.h))
+        return self.info()
+
+    def info(self):
+        import numpy as np
+        cnt = np.zeros(self.nmesh, np.int32)
+        off = np.zeros(self.nmesh + 1, np.int64)
+        check(lib().xg_batch_info(self.ctx.h, self.h, cnt.ctypes.data_as(_vp), off.ctypes.data_as(_vp)))
+        return cnt, off
+
+    def cut_list_ptr(self):
+        return lib().xg_batch_cut_list(self.h)
+
+    def cut_list_host(self, n):
+        """the first n cut element ids (after plan/run), copied to the host"""
+        import numpy as np
+        out = np.zeros(n, np.int32)
+        if n:
+            hip = ctypes.CDLL("libamdhip64.so")
+            hip.hipDeviceSynchronize()
+            rc = hip.hipMemcpy(ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(self.cut_list_ptr()),
+                               ctypes.c_size_t(4 * n), 2)  # hipMemcpyDeviceToHost
+            if rc != 0:
+                raise XgError(-2, "hipMemcpy of the cut list failed (%d)" % rc)
+        return out
+
+    def run(self, out, use_graph=True, only_p=0):
+        check(lib().xg_batch_run(self.ctx.h, self.h, ptr(out), out.numel(), int(use_graph), int(only_p)))
+
+    def close(self):
+        if self.h:
+            lib().xg_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

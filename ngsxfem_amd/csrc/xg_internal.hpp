@@ -28,6 +28,10 @@ int ensure_ws(xg_ctx *ctx, size_t bytes);
 // exclusive scan of n int32 values in place (device), total written to
 // data[n] when `with_total` (data must then hold n+1 entries)
 int scan_exclusive_i32(xg_ctx *ctx, int32_t *data, int n, bool with_total);
+// graph-capturable marking + compaction (kernel launches only)
+size_t mark_scratch_ints(int ne);
+int mark_compact(hipStream_t st, int32_t *d_err, int D, int ne, const int32_t *el2v, const double *lset,
+                 uint8_t *dt, double *cr, uint32_t *bits, int32_t *scratch, int32_t *cut_list, int32_t *n_cut);
 
 }  // namespace xg
 

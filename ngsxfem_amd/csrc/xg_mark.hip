@@ -196,7 +196,8 @@ __global__ __launch_bounds__(SCAN_BLK) void k_scan_tiles(int32_t *data, int n, i
 
 // pass 2 (single block): exclusive scan of the tile sums, serial over
 // chunks of SCAN_BLK
-__global__ __launch_bounds__(SCAN_BLK) void k_scan_sums(int32_t *sums, int nt, int32_t *total_out) {
+__global__ __launch_bounds__(SCAN_BLK) void k_scan_sums(int32_t *sums, int nt, int32_t *total_out,
+                                                        int32_t *total_out2 = nullptr) {
   __shared__ int s_wave[SCAN_BLK / 64];
   int carry = 0;
   for (int base = 0; base < nt; base += SCAN_BLK) {
@@ -208,6 +209,7 @@ __global__ __launch_bounds__(SCAN_BLK) void k_scan_sums(int32_t *sums, int nt, i
     carry += total;
   }
   if (threadIdx.x == 0 && total_out) *total_out = carry;
+  if (threadIdx.x == 0 && total_out2) *total_out2 = carry;
 }
 
 __global__ __launch_bounds__(SCAN_BLK) void k_scan_add(int32_t *data, int n, const int32_t *sums) {
@@ -279,6 +281,38 @@ int scan_exclusive_i32(xg_ctx *ctx, int32_t *data, int n, bool with_total) {
 
 }  // namespace xg
 
+namespace xg {
+
+size_t mark_scratch_ints(int ne) {
+  const int nblk = (ne + BLK - 1) / BLK;
+  return (size_t)(nblk + 1) + (size_t)((nblk + SCAN_TILE - 1) / SCAN_TILE + 1);
+}
+
+// marking + compaction on an explicit stream with caller-owned scratch
+// (mark_scratch_ints(ne) int32); only kernel launches, so the sequence can be
+// captured into a hipGraph.  n_cut (nullable) receives the cut count.
+int mark_compact(hipStream_t st, int32_t *d_err, int D, int ne, const int32_t *el2v, const double *lset,
+                 uint8_t *dt, double *cr, uint32_t *bits, int32_t *scratch, int32_t *cut_list, int32_t *n_cut) {
+  const int nw = (ne + 31) / 32;
+  const int nblk = (ne + BLK - 1) / BLK;
+  int32_t *blk = scratch, *sums = scratch + nblk + 1;
+  switch (D) {
+    case 1: k_mark<1><<<nblk, BLK, 0, st>>>(ne, el2v, lset, dt, cr, bits, nw, blk, d_err); break;
+    case 2: k_mark<2><<<nblk, BLK, 0, st>>>(ne, el2v, lset, dt, cr, bits, nw, blk, d_err); break;
+    default: k_mark<3><<<nblk, BLK, 0, st>>>(ne, el2v, lset, dt, cr, bits, nw, blk, d_err); break;
+  }
+  XG_LAUNCH_CHECK();
+  const int nt = (nblk + SCAN_TILE - 1) / SCAN_TILE;
+  k_scan_tiles<<<nt, SCAN_BLK, 0, st>>>(blk, nblk, sums);
+  k_scan_sums<<<1, SCAN_BLK, 0, st>>>(sums, nt, blk + nblk, n_cut);
+  k_scan_add<<<nt, SCAN_BLK, 0, st>>>(blk, nblk, sums);
+  k_compact<<<nblk, BLK, 0, st>>>(ne, bits + (size_t)4 * nw, blk, cut_list);
+  XG_LAUNCH_CHECK();
+  return XG_OK;
+}
+
+}  // namespace xg
+
 extern "C" int xg_p1_perturb(xg_ctx *ctx, int n, double *v, double eps) {
   if (!ctx || n < 0 || (n > 0 && !v)) return set_error(XG_ERR_ARG, "xg_p1_perturb: bad argument");
   if (n == 0) return XG_OK;
@@ -322,12 +356,11 @@ extern "C" int xg_mark(xg_ctx *ctx, int D, int ne, const int32_t *el2v, const do
     int32_t *sums = (int32_t *)ctx->ws;
     k_scan_tiles<<<nt, SCAN_BLK, 0, ctx->stream>>>(blk, n, sums);
     XG_LAUNCH_CHECK();
-    k_scan_sums<<<1, SCAN_BLK, 0, ctx->stream>>>(sums, nt, blk + n);
+    k_scan_sums<<<1, SCAN_BLK, 0, ctx->stream>>>(sums, nt, blk + n, n_cut);
     XG_LAUNCH_CHECK();
     k_scan_add<<<nt, SCAN_BLK, 0, ctx->stream>>>(blk, n, sums);
     XG_LAUNCH_CHECK();
   }
-  if (n_cut) XG_CHECK(hipMemcpyAsync(n_cut, blk + nblk, sizeof(int32_t), hipMemcpyDeviceToDevice, ctx->stream));
   if (cut_list) {
     k_compact<<<nblk, BLK, 0, ctx->stream>>>(ne, bits + (size_t)4 * nw, blk, cut_list);
     XG_LAUNCH_CHECK();

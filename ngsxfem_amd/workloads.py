@@ -58,3 +58,34 @@ def p2_nodes():
         for j in range(i + 1, 4):
             nodes.append(0.5 * (V[i] + V[j]))
     return np.array(nodes)
+
+
+def config3(nmesh=256, seed0=1000, nmin=8, nmax=48):
+    """Batched uneven configuration: nmesh independent unit-cube meshes of
+    n^3 cells x 6 Kuhn tets, n = round(exp(U[ln nmin, ln nmax])), each with
+    the P1 interpolant (perturbed) of its own ellipsoid
+    ((x-1/2)/a)^2 + ((y-1/2)/b)^2 + ((z-1/2)/c)^2 - 1, semi-axes U[0.15, 0.45],
+    and its own H1 degree p in {1, 2, 3} (rule order 2p).  Mesh m draws from
+    MT19937(seed0 + m).  Returns the concatenation (coords, el2v with global
+    vertex ids, lset) and per-mesh (elem_off, p, n, semi-axes)."""
+    coords, el2v, lset, elem_off, ps, ns, axes = [], [], [], [0], [], [], []
+    voff = 0
+    for m in range(nmesh):
+        rng = np.random.Generator(np.random.MT19937(seed0 + m))
+        n = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
+        ax = rng.uniform(0.15, 0.45, 3)
+        p = int(rng.integers(1, 4))
+        mesh = MakeStructured3DMesh(nx=n, ny=n, nz=n)
+        X = mesh.coords
+        phi = ((X[:, 0] - 0.5) / ax[0]) ** 2 + ((X[:, 1] - 0.5) / ax[1]) ** 2 + ((X[:, 2] - 0.5) / ax[2]) ** 2 - 1.0
+        coords.append(X)
+        el2v.append(mesh.el2v + voff)
+        lset.append(p1_perturb(phi))
+        voff += mesh.nv
+        elem_off.append(elem_off[-1] + mesh.ne)
+        ps.append(p)
+        ns.append(n)
+        axes.append(ax)
+    return (np.concatenate(coords), np.ascontiguousarray(np.concatenate(el2v), dtype=np.int32),
+            np.concatenate(lset), np.array(elem_off, np.int32), np.array(ps, np.int32), np.array(ns),
+            np.array(axes))
