@@ -135,8 +135,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_batch(int order, int ndt, int dt
   __shared__ int s_mesh[MAXMESH];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   T *sB = reinterpret_cast<T *>(smem) + (size_t)wv * (C::LDS_PER_WAVE / sizeof(T));
-  T *sW = sB + 4 * C::NB * 16 * C::PAD;
-  for (int k = lane; k < 4 * C::NB * 16 * C::PAD; k += 64) sB[k] = T(0);
+  T *sW = sB + C::SB_SIZE;
+  for (int k = lane; k < C::SB_SIZE; k += 64) sB[k] = T(0);
   const int nq_vol = rule_cnt(D, order), nq_if = rule_cnt(D - 1, order);
   const int off_vol = rule_off(D, order), off_if = rule_off(D - 1, order);
   for (int k = threadIdx.x; k < nq_vol + nq_if; k += blockDim.x) {

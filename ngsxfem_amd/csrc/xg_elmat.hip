@@ -42,10 +42,10 @@ __global__ __launch_bounds__(128) void k_pipeline(int nel, const int32_t *__rest
   __shared__ double s_rule[2][PIPE_NQMAX][4];  // base rules of dimension D and D-1
   const int wv = threadIdx.x >> 6;
   T *sB = reinterpret_cast<T *>(smem) + (size_t)wv * (C::LDS_PER_WAVE / sizeof(T));
-  T *sW = sB + 4 * C::NB * 16 * C::PAD;
+  T *sW = sB + C::SB_SIZE;
   const int lane = threadIdx.x & 63;
   // zero the padding rows once (dofs >= N never get written)
-  for (int k = lane; k < 4 * C::NB * 16 * C::PAD; k += 64) sB[k] = T(0);
+  for (int k = lane; k < C::SB_SIZE; k += 64) sB[k] = T(0);
   const int nq_vol = rule_cnt(D, order), nq_if = rule_cnt(D - 1, order);
   const int off_vol = rule_off(D, order), off_if = rule_off(D - 1, order);
   for (int k = threadIdx.x; k < nq_vol + nq_if; k += blockDim.x) {
@@ -101,9 +101,9 @@ __global__ __launch_bounds__(128) void k_elmat_csr(int nel, const int32_t *__res
   extern __shared__ char smem[];
   const int wv = threadIdx.x >> 6;
   T *sB = reinterpret_cast<T *>(smem) + (size_t)wv * (C::LDS_PER_WAVE / sizeof(T));
-  T *sW = sB + 4 * C::NB * 16 * C::PAD;
+  T *sW = sB + C::SB_SIZE;
   const int lane = threadIdx.x & 63;
-  for (int k = lane; k < 4 * C::NB * 16 * C::PAD; k += 64) sB[k] = T(0);
+  for (int k = lane; k < C::SB_SIZE; k += 64) sB[k] = T(0);
   wave_sync();
   const int wid = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const int nwaves = (gridDim.x * blockDim.x) >> 6;
@@ -144,9 +144,9 @@ __global__ __launch_bounds__(128) void k_elmat_weights(int nel, const int32_t *_
   extern __shared__ char smem[];
   const int wv = threadIdx.x >> 6;
   T *sB = reinterpret_cast<T *>(smem) + (size_t)wv * (C::LDS_PER_WAVE / sizeof(T));
-  T *sW = sB + 4 * C::NB * 16 * C::PAD;
+  T *sW = sB + C::SB_SIZE;
   const int lane = threadIdx.x & 63;
-  for (int k = lane; k < 4 * C::NB * 16 * C::PAD; k += 64) sB[k] = T(0);
+  for (int k = lane; k < C::SB_SIZE; k += 64) sB[k] = T(0);
   wave_sync();
   const int wid = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const int nwaves = (gridDim.x * blockDim.x) >> 6;

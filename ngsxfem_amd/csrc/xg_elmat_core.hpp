@@ -44,8 +44,10 @@ struct Cfg {
   static constexpr int NB = (N + 15) / 16;
   static constexpr int CH = (sizeof(T) == 4) ? (NB == 1 ? 64 : 32) : (NB == 1 ? 32 : 16);
   static constexpr int PAD = CH + 1;
+  static constexpr int ROWS = N;                      // LDS rows per component (real dofs only)
+  static constexpr int SB_SIZE = 4 * ROWS * PAD;      // shape/gradient table per wave
   static constexpr int WAVES = 2;  // 128-thread blocks, <= 35 KB LDS per block
-  static constexpr size_t LDS_PER_WAVE = sizeof(T) * (4 * NB * 16 * PAD + CH);
+  static constexpr size_t LDS_PER_WAVE = sizeof(T) * (SB_SIZE + CH);
 };
 
 // Point sources.  point(g, x, w) gives the reference coordinates x (in the
@@ -210,13 +212,13 @@ __device__ __forceinline__ void elmat_one(const SRC &src, int npts, const double
       sW[lane] = w;
 #pragma unroll
       for (int n = 0; n < N; ++n) {
-        sB[(0 * NB * 16 + n) * PAD + lane] = m_mass ? phi[n] : T(0);
+        sB[(0 * N + n) * PAD + lane] = m_mass ? phi[n] : T(0);
 #pragma unroll
         for (int r = 0; r < D; ++r) {
           T gr = T(0);
 #pragma unroll
           for (int cc = 0; cc < D; ++cc) gr += FinvT[cc * D + r] * dphi[n][cc];
-          sB[((1 + r) * NB * 16 + n) * PAD + lane] = m_lap ? gr : T(0);
+          sB[((1 + r) * N + n) * PAD + lane] = m_lap ? gr : T(0);
         }
       }
     }
@@ -231,7 +233,10 @@ __device__ __forceinline__ void elmat_one(const SRC &src, int npts, const double
       for (int u = 0; u < 8; ++u) {
         wv[u] = sW[p0 + u];
 #pragma unroll
-        for (int b = 0; b < NB; ++b) av[u][b] = sB[(c * NB * 16 + b * 16 + ii) * PAD + p0 + u];
+        for (int b = 0; b < NB; ++b) {
+          const int dof = b * 16 + ii;  // dofs >= N are zero rows of the MFMA tile
+          av[u][b] = dof < N ? sB[(c * N + dof) * PAD + p0 + u] : T(0);
+        }
       }
 #pragma unroll
       for (int u = 0; u < 8; ++u)
