@@ -27,6 +27,18 @@ METRIC = "cut-elem/s (rule+elmat) plain vs learned/mixed at ≤1e-6 rel err; %HB
 PEAK_FP32_MFMA = 157.3e12
 PEAK_FP64_MFMA = 78.6e12
 PEAK_HBM = 8.0e12
+PEAK_FP64_VALU = 78.6e12
+
+
+def fp64_ops_moments(npts, n_cut, D=3, p=2):
+    """FP64 flops of the moment path (FMA = 2): per point the mapping
+    (3 ow + 3 mul + 9 FMA + 3 sub) and the moment update (10 products,
+    10 weight products, 35 FMA); per (element, domain type) the degree-2
+    lowering (90 adds) and the identities (934 coefficient terms, one FMA
+    each)."""
+    per_pt = 3 + 3 + 2 * 9 + 3 + 10 + 10 + 2 * 35
+    per_el = 90 + 2 * 934
+    return sum(npts.values()) * per_pt + 2 * n_cut * per_el
 
 
 def parse():
@@ -35,13 +47,19 @@ def parse():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=64, help="cells per direction")
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"], help="element-matrix store type")
+    ap.add_argument("--method", default="moments", choices=["moments", "mfma"],
+                    help="path reported as value (both are measured)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4"],
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="launch each path 3 times plus a 256 MiB copy, for rocprofv3 --pmc runs")
+    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5"],
                     help="c2: 3D plain straight-cut pipeline (headline); c4: learned quadrature on P2-cut tets")
     ap.add_argument("--c4-n", type=int, default=2000000)
     ap.add_argument("--c4-train", type=int, default=100000)
+    ap.add_argument("--c5-n", type=int, default=128)
+    ap.add_argument("--c5-L", type=int, default=4)
     return ap.parse_args()
 
 
@@ -76,11 +94,21 @@ def bench_c4(args):
     from ngsxfem_amd.learned import normalise_inputs
     xn = normalise_inputs(x)
     wp = torch.empty(n, 35, dtype=torch.float64, device="cuda")
+    mp = torch.empty(n, 35, dtype=torch.float64, device="cuda")
     wl = torch.empty(n, 35, dtype=torch.float32, device="cuda")
     wl64 = torch.empty(n, 35, dtype=torch.float64, device="cuda")
     em = torch.empty(n * 100, dtype=torch.float32, device="cuda")
+    em_plain = torch.empty(n * 100, dtype=torch.float32, device="cuda")
+    em_learned = torch.empty(n * 100, dtype=torch.float32, device="cuda")
 
-    def plain():
+    def plain_moments():
+        # subdivision rule -> degree-4 moments -> P2 element matrices (FP64
+        # arithmetic, FP32 store)
+        ctx.subdiv_moments(x, 2, 4, 4, 2, mp, None, None)
+        ctx.elmat_moments(3, 2, mp, 3, _abi.XG_PREC_FP32, em_plain, nel=n)
+
+    def plain_weights():
+        # subdivision rule -> fixed-node weights -> MFMA element matrices
         ctx.subdiv_weights(x, 2, 4, 4, 4, wp, None, None)
         _abi.check(_abi.lib().xg_elmat_weights(ctx.h, 2, 4, n, None, None, None, _abi.ptr(wp), 3,
                                                _abi.XG_PREC_FP32, _abi.ptr(em)))
@@ -89,7 +117,7 @@ def bench_c4(args):
         lr_.predict(x, x_norm=xn, out=wl)
         wl64.copy_(wl)
         _abi.check(_abi.lib().xg_elmat_weights(ctx.h, 2, 4, n, None, None, None, _abi.ptr(wl64), 3,
-                                               _abi.XG_PREC_FP32, _abi.ptr(em)))
+                                               _abi.XG_PREC_FP32, _abi.ptr(em_learned)))
 
     def timed(f, reps):
         f()
@@ -103,9 +131,18 @@ def bench_c4(args):
         ctx.sync()
         return a.elapsed_time(b) / reps
 
-    ms_plain = timed(plain, 2)
+    ms_plain = timed(plain_moments, 3)
+    ms_plain_w = timed(plain_weights, 3)
     ms_learned = timed(learned, 10)
+    # errors of the learned path against the plain rule: fixed-node weights
+    # and element matrices (Frobenius norm per element)
     err = (torch.linalg.norm(wl.to(torch.float64) - wp, dim=1) / torch.linalg.norm(wp, dim=1)).cpu().numpy()
+    ep = em_plain.view(n, 100).double()
+    el = em_learned.view(n, 100).double()
+    err_m = (torch.linalg.norm(el - ep, dim=1) / torch.linalg.norm(ep, dim=1)).cpu().numpy()
+    # the two plain variants agree (FP32 store / FP32 MFMA)
+    ew = em.view(n, 100).double()
+    err_pw = float((torch.linalg.norm(ew - ep, dim=1) / torch.linalg.norm(ep, dim=1)).max().item())
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
@@ -116,18 +153,97 @@ def bench_c4(args):
         O.subdiv_moments3_batch(xs, 2, 4, 4, 4, nthreads=cores)
         dt_ = time.perf_counter() - t1
         cpu = {"value": ns / dt_, "unit": "cut-elem/s", "cores": cores, "kind": "port",
-               "sample": "%d of the %d tets, subdivision level 4 weights only (FP64 oracle, %d threads, %.1fs)"
-                         % (ns, n, cores, dt_)}
+               "sample": "%d of the %d tets, subdivision level 4 rule contracted to fixed-node weights, "
+                         "element matrices not included (FP64 oracle, %d threads, %.1fs)" % (ns, n, cores, dt_)}
     res = {"metric": METRIC, "config": {"workload": "learned quadrature: %d cut tets, random quadric P2 level "
                                                     "sets, plain = subdivision L=4 FP64 + P2 elmat FP32, learned = "
                                                     "MLP 10-128-128-35 BF16 + P2 elmat FP32" % n},
-           "plain": {"ms": ms_plain, "cut_elem_per_s": n / (ms_plain * 1e-3)},
+           "plain": {"ms": ms_plain, "cut_elem_per_s": n / (ms_plain * 1e-3),
+                     "path": "subdivision L=4 -> FP64 degree-4 moments -> moment-form P2 elmat (FP64, FP32 store)"},
+           "plain_fixed_node_mfma": {"ms": ms_plain_w, "cut_elem_per_s": n / (ms_plain_w * 1e-3),
+                                     "path": "subdivision L=4 -> fixed-node weights -> FP32 MFMA elmat",
+                                     "max_rel_err_vs_plain": err_pw},
            "learned": {"ms": ms_learned, "cut_elem_per_s": n / (ms_learned * 1e-3),
-                       "rel_err_vs_plain_median": float(np.median(err)),
-                       "rel_err_vs_plain_p99": float(np.percentile(err, 99)),
-                       "frac_within_1e-6": float(np.mean(err <= 1e-6)),
-                       "frac_within_1e-3": float(np.mean(err <= 1e-3))},
+                       "rel_err_weights_median": float(np.median(err)),
+                       "rel_err_weights_p99": float(np.percentile(err, 99)),
+                       "rel_err_elmat_median": float(np.median(err_m)),
+                       "rel_err_elmat_p99": float(np.percentile(err_m, 99)),
+                       "frac_elmat_within_1e-6": float(np.mean(err_m <= 1e-6)),
+                       "frac_elmat_within_1e-3": float(np.mean(err_m <= 1e-3)),
+                       "fallback_frac_at_1e-6": float(np.mean(err > 1e-6))},
            "train_s": t_train, "train_loss_last": hist[-1], "cpu_baseline": cpu}
+    print(json.dumps(res), flush=True)
+    ctx.close()
+
+
+def bench_c5(args):
+    """Large mixed: 128^3 cells x 6 Kuhn tets (12.6M tets), P2 level set of the
+    union of 64 random spheres, P3 element matrices.  Steps: P2 marking
+    (Bernstein certificate + lattice check), then on the cut elements the
+    subdivision rule (level L) contracted to degree-6 moments and the P3
+    mass + stiffness matrices from the moment identities (FP64 arithmetic,
+    FP32 store)."""
+    import torch
+
+    from ngsxfem_amd import _abi
+    from ngsxfem_amd import workloads as W
+    from ngsxfem_amd.meshes import MakeStructured3DMesh
+
+    stream = torch.cuda.current_stream()
+    ctx = _abi.Context(0, stream=stream)
+    dev = torch.device("cuda:0")
+    n, L, p = args.c5_n, args.c5_L, 3
+    t0 = time.time()
+    mesh = MakeStructured3DMesh(n)
+    centres, radii = W.config5_spheres(7)
+    d_el = torch.from_numpy(mesh.el2v).to(dev)
+    d_x = torch.from_numpy(mesh.coords).to(dev)
+    lp2 = W.p2_nodal_values_torch(d_el, d_x, centres, radii)
+    torch.cuda.synchronize()
+    print("# workload built in %.1fs: %d tets" % (time.time() - t0, mesh.ne), file=sys.stderr, flush=True)
+    ne = mesh.ne
+    nw = (ne + 31) // 32
+    dt = torch.empty(ne, dtype=torch.uint8, device=dev)
+    bits = torch.empty(8 * nw, dtype=torch.int32, device=dev)
+    cut = torch.empty(ne, dtype=torch.int32, device=dev)
+    ncut = torch.zeros(1, dtype=torch.int32, device=dev)
+
+    def timed(f, reps):
+        f()
+        ctx.sync()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            f()
+        b.record(stream)
+        ctx.sync()
+        return a.elapsed_time(b) / reps
+
+    ms_mark = timed(lambda: ctx.mark_p2(lp2, L, dt, bits, cut, ncut), 5)
+    n_cut = int(ncut.item())
+    cl = cut[:n_cut].clone()
+    lcut = lp2[cl.long()].contiguous()
+    nm = 84
+    mom = torch.empty(n_cut * nm, dtype=torch.float64, device=dev)
+    out = torch.empty(n_cut * 400, dtype=torch.float32, device=dev)
+
+    def plain():
+        ctx.subdiv_moments(lcut, 2, L, 2 * p, p, mom, None, None)
+        ctx.elmat_moments(3, p, mom, 3, _abi.XG_PREC_FP32, out, el2v=d_el, coords=d_x, elems=cl)
+
+    ms_plain = timed(plain, args.steps)
+    ms_sub = timed(lambda: ctx.subdiv_moments(lcut, 2, L, 2 * p, p, mom, None, None), 2)
+    counts = np.bincount(dt.cpu().numpy(), minlength=3)
+    res = {"metric": METRIC, "value": n_cut / (ms_plain * 1e-3), "unit": "cut-elem/s", "n_gpus": 1,
+           "steps": args.steps, "ms_per_step": ms_plain, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "fp64+fp32", "data": "synthetic",
+           "config": {"workload": "large mixed: unit cube %d^3 x 6 Kuhn tets (%d tets), P2 level set = min of 64 "
+                                  "spheres (MT19937 seed 7), subdivision rule L=%d (order 6) -> degree-6 moments -> "
+                                  "P3 mass+stiffness elmat (FP64 identities, FP32 store)" % (n, ne, L),
+                      "cut_elements": n_cut, "neg": int(counts[0]), "pos": int(counts[1])},
+           "marking": {"ms": ms_mark, "tets_per_s": ne / (ms_mark * 1e-3)},
+           "subdivision_ms": ms_sub}
     print(json.dumps(res), flush=True)
     ctx.close()
 
@@ -219,6 +335,8 @@ def main():
         return bench_c4(args)
     if args.config == "c3":
         return bench_c3(args)
+    if args.config == "c5":
+        return bench_c5(args)
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the path shards by element "
               "ranges; see DESIGN.md); run with --gpus 1", file=sys.stderr)
@@ -279,41 +397,90 @@ def main():
     bytes_alg = n_cut * (4 * 4 + 8 * 4 + 24 * 4 + 4) + n_cut * 2 * nd * nd * (4 if args.precision == "fp32" else 8)
 
     prec = _abi.XG_PREC_FP32 if args.precision == "fp32" else _abi.XG_PREC_FP64
-    out = torch.empty(n_cut * 2 * nd * nd, dtype=torch.float32 if prec == _abi.XG_PREC_FP32 else torch.float64,
-                      device=dev)
+    odt = torch.float32 if prec == _abi.XG_PREC_FP32 else torch.float64
     dts = [_abi.XG_NEG, _abi.XG_IF]
 
-    def step():
-        ctx.cut_pipeline(D, p, order, d_el, d_x, d_l, dts, 3, prec, out, elems=cut_list)
+    def timed(precision, out):
+        def step():
+            ctx.cut_pipeline(D, p, order, d_el, d_x, d_l, dts, 3, precision, out, elems=cut_list)
 
-    for _ in range(args.warmup):
-        step()
+        for _ in range(args.warmup):
+            step()
+        ctx.sync()
+        torch.cuda.synchronize()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(args.steps)]
+        g0 = torch.cuda.Event(enable_timing=True)
+        g1 = torch.cuda.Event(enable_timing=True)
+        g0.record(stream)
+        for k in range(args.steps):
+            evs[k][0].record(stream)
+            step()
+            evs[k][1].record(stream)
+        g1.record(stream)
+        ctx.sync()
+        torch.cuda.synchronize()
+        return g0.elapsed_time(g1) / args.steps, float(np.mean([a.elapsed_time(b) for a, b in evs]))
+
+    if args.pmc_probe:
+        # counter runs (rocprofv3 --pmc): a 256 MiB device copy of known size
+        # (calibration of FETCH_SIZE / WRITE_SIZE), then 3 launches of each path
+        a = torch.empty(64 << 20, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        for precision in (prec, prec | _abi.XG_METHOD_MOMENTS):
+            o = torch.empty(n_cut * 2 * nd * nd, dtype=odt, device=dev)
+            for _ in range(3):
+                ctx.cut_pipeline(D, p, order, d_el, d_x, d_l, dts, 3, precision, o, elems=cut_list)
+        ctx.sync()
+        torch.cuda.synchronize()
+        print(json.dumps({"pmc_probe": True, "copy_bytes": 2 * a.numel() * 4, "cut_elements": n_cut,
+                          "algorithmic_bytes_per_launch": bytes_alg}), flush=True)
+        return
+
+    # FP64 MFMA pipeline: the accuracy yardstick (matches the FP64 oracle to
+    # 1e-12 in the parity tests)
+    ref64 = torch.empty(n_cut * 2 * nd * nd, dtype=torch.float64, device=dev)
+    ctx.cut_pipeline(D, p, order, d_el, d_x, d_l, dts, 3, _abi.XG_PREC_FP64, ref64, elems=cut_list)
     ctx.sync()
-    torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    g0 = torch.cuda.Event(enable_timing=True)
-    g1 = torch.cuda.Event(enable_timing=True)
-    g0.record(stream)
-    for k in range(args.steps):
-        evs[k][0].record(stream)
-        step()
-        evs[k][1].record(stream)
-    g1.record(stream)
-    ctx.sync()
-    torch.cuda.synchronize()
-    total_ms = g0.elapsed_time(g1)
-    launch_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
-    ms_per_step = total_ms / args.steps
+    r64 = ref64.view(-1, nd * nd)
+    rnorm = torch.linalg.vector_norm(r64, dim=1)
+    nzr = rnorm > 0
+
+    def max_rel_err(out):
+        d = torch.linalg.vector_norm(out.view(-1, nd * nd).double() - r64, dim=1)
+        return float((d[nzr] / rnorm[nzr]).max().item())
+
+    paths = {}
+    out_mfma = torch.empty(n_cut * 2 * nd * nd, dtype=odt, device=dev)
+    ms_mfma, launch_mfma = timed(prec, out_mfma)
+    paths["plain_mfma"] = {"kernel": "k_pipeline<3,2,%s>" % ("float" if prec == 1 else "double"),
+                           "ms": ms_mfma, "cut_elem_per_s": n_cut / (ms_mfma * 1e-3),
+                           "max_rel_err_vs_fp64": max_rel_err(out_mfma),
+                           "arith": "%s MFMA, B^T W B" % args.precision}
+    out_mom = torch.empty(n_cut * 2 * nd * nd, dtype=odt, device=dev)
+    ms_mom, launch_mom = timed(prec | _abi.XG_METHOD_MOMENTS, out_mom)
+    paths["moments"] = {"kernel": "k_moment_pipeline<3,2,%s>" % ("float" if prec == 1 else "double"),
+                        "ms": ms_mom, "cut_elem_per_s": n_cut / (ms_mom * 1e-3),
+                        "max_rel_err_vs_fp64": max_rel_err(out_mom),
+                        "arith": "FP64 moments + FP64 identities, %s store" % args.precision}
+    best = "moments" if args.method == "moments" else "plain_mfma"
+    ms_per_step = paths[best]["ms"]
     value = n_cut / (ms_per_step * 1e-3)
 
     peak = PEAK_FP32_MFMA if prec == _abi.XG_PREC_FP32 else PEAK_FP64_MFMA
-    achieved = flops / (launch_ms * 1e-3)
-    roof = {"bound": "mfma", "kernel": "k_pipeline<3,2,%s>" % ("float" if prec == 1 else "double"),
-            "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / peak, "traffic": read_traffic("pipeline_c2_pmc.txt"),
-            "algorithmic_flops_per_launch": flops, "algorithmic_bytes_per_launch": bytes_alg,
-            "hbm_frac_algorithmic": bytes_alg / (launch_ms * 1e-3) / PEAK_HBM,
-            "avg_launch_ms": launch_ms}
+    roof_mfma = {"bound": "mfma", "kernel": paths["plain_mfma"]["kernel"],
+                 "achieved": flops / (launch_mfma * 1e-3) / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                 "frac": flops / (launch_mfma * 1e-3) / peak, "traffic": read_traffic("pipeline_c2_pmc.txt"),
+                 "algorithmic_flops_per_launch": flops, "avg_launch_ms": launch_mfma}
+    ach = bytes_alg / (launch_mom * 1e-3)
+    roof_mom = {"bound": "hbm", "kernel": paths["moments"]["kernel"],
+                "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": ach / PEAK_HBM,
+                "traffic": read_traffic("moments_c2_pmc.txt"), "algorithmic_bytes_per_launch": bytes_alg,
+                "avg_launch_ms": launch_mom,
+                "fp64_valu_frac": fp64_ops_moments(npts, n_cut) / (launch_mom * 1e-3) / PEAK_FP64_VALU}
+    roof = roof_mom if best == "moments" else roof_mfma
 
     # ---- CPU baseline: FP64 restatement of the reference algorithm ----
     cpu = None
@@ -333,12 +500,14 @@ def main():
     res = {
         "metric": METRIC, "value": value, "unit": "cut-elem/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
+        "vs_baseline": None, "dtype": "fp64+%s" % args.precision if best == "moments" else args.precision,
+        "data": "synthetic",
         "config": {"workload": "3D plain: unit cube %d^3 x 6 Kuhn tets (%d tets), P1 sphere r=0.35 + 0.01 "
                                "sin(7x)sin(7y)sin(7z), order-4 straight cut rule NEG+IF, P2 mass+stiffness "
-                               "elmat (%s MFMA)" % (args.n, ne, args.precision),
+                               "elmat (%s)" % (args.n, ne, paths[best]["arith"]),
                    "cut_elements": n_cut, "points_neg": npts[0], "points_if": npts[2]},
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "paths": paths,
+        "roofline_plain_mfma": roof_mfma,
         "marking": {"ms": mark_ms, "tets_per_s": ne / (mark_ms * 1e-3),
                     "hbm_frac_algorithmic": ne * 26.4 / (mark_ms * 1e-3) / PEAK_HBM},
     }

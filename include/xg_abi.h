@@ -46,6 +46,13 @@ extern "C" {
 #define XG_PREC_FP64 0   /* FP64 MFMA (v_mfma_f64_16x16x4f64), reference mode */
 #define XG_PREC_FP32 1   /* FP32 MFMA (v_mfma_f32_16x16x4f32), FP32 accumulate */
 #define XG_PREC_BF16X3 2 /* 3-way split BF16 MFMA (6 products), FP32 accumulate */
+/* OR-ed into the precision of xg_cut_pipeline: moment form.  Per (element,
+   domain type) the FP64 barycentric moments sum_i w_i lam(x_i)^alpha,
+   |alpha| = 2p, of the same cut rule are accumulated on the vector ALU and
+   the mass / stiffness entries formed from them by exact polynomial
+   identities (the same quadrature sum, reassociated).  FP64 arithmetic
+   throughout; the output is FP32 (XG_PREC_FP32) or FP64 (XG_PREC_FP64). */
+#define XG_METHOD_MOMENTS 16
 
 /* element-matrix form mask */
 #define XG_FORM_MASS 1      /* u*v */
@@ -80,6 +87,16 @@ int xg_p1_perturb(xg_ctx *ctx, int n, double *v, double eps);
 int xg_mark(xg_ctx *ctx, int D, int ne, const int32_t *el2v, const double *lset_v,
             uint8_t *dt, double *cut_ratio, uint32_t *bits, int32_t *cut_list,
             int32_t *n_cut);
+
+/* ---- a12 for P2 level sets (cutinfo.cpp:61-133 with the subdivision rule of
+   xintegration.cpp:617-752): the element domain is the lattice check of the
+   level-L subdivision rule (IF iff the level set takes values >= 0 and < 0 on
+   the C(2^L+3,3) lattice points).  lset_nodal: ne x 10 FP64 nodal values
+   (Lattice<3,2> dof order); dt: uint8[ne] (required); bits: the 8
+   COMBINED_DOMAIN_TYPE arrays as in xg_mark (required); cut_list / n_cut as
+   in xg_mark (nullable).  No cut ratio (it needs the full rules). */
+int xg_mark_p2(xg_ctx *ctx, int ne, const double *lset_nodal, int L, uint8_t *dt, uint32_t *bits,
+               int32_t *cut_list, int32_t *n_cut);
 
 /* ---- a13: GetFacetsWithNeighborTypes (xfem/cutinfo.cpp:476-538) ------- */
 /* facet f has neighbours f2el[f][0], f2el[f][1] (-1 = none).  a, b: element
@@ -122,7 +139,8 @@ int xg_elmat(xg_ctx *ctx, int D, int p, int nel, const int32_t *elems, const int
 /* ---- fused: rule + element matrix, points never leave the chip ---------
    For each element i and each dts[t] (host array, ndt <= 3): the straight cut
    rule of order `order` and CalcElementMatrix on it.
-   out: nel x ndt x ndof x ndof (FP64 / FP32 by precision). */
+   out: nel x ndt x ndof x ndof (FP64 / FP32 by precision).
+   precision | XG_METHOD_MOMENTS selects the moment form (any rule order). */
 int xg_cut_pipeline(xg_ctx *ctx, int D, int p, int order, int nel, const int32_t *elems,
                     const int32_t *el2v, const double *coords, const double *lset_v,
                     int ndt, const int *dts, int mask, int precision, void *out);
@@ -142,6 +160,23 @@ int xg_cut_pipeline(xg_ctx *ctx, int D, int p, int order, int nel, const int32_t
 int xg_subdiv_weights(xg_ctx *ctx, int nel, const double *lset_nodal, int lset_order, int L,
                       int order, int q, const double *finv, double *out_neg, double *out_if,
                       uint8_t *dom);
+/* The same subdivision rule contracted into FP64 barycentric moments
+   m_alpha = sum_points w lam(x)^alpha, |alpha| = 2p (the monomial order of
+   the moment identities; 10 / 35 / 84 values per element for p = 1 / 2 / 3):
+   every element matrix of degree-p elements on the rule is a linear function
+   of them (xg_elmat_moments).  out_neg / out_if (nullable): nel x NM FP64. */
+int xg_subdiv_moments(xg_ctx *ctx, int nel, const double *lset_nodal, int lset_order, int L,
+                      int order, int p, const double *finv, double *out_neg, double *out_if,
+                      uint8_t *dom);
+
+/* ---- element matrices on a moment-form rule ----------------------------
+   (symboliccutbfi.cpp:76-304 on any rule, given by its degree-2p barycentric
+   moments, e.g. from xg_subdiv_moments): the mass / stiffness matrices of
+   degree-p H1 elements from the exact polynomial identities of the moment
+   form, FP64 arithmetic.  coords == NULL: reference element (identity map).
+   moments: nel x NM FP64; out: nel x ndof x ndof (FP32 / FP64). */
+int xg_elmat_moments(xg_ctx *ctx, int D, int p, int nel, const int32_t *elems, const int32_t *el2v,
+                     const double *coords, const double *moments, int mask, int precision, void *out);
 
 /* ---- element matrices on a fixed-node rule ----------------------------
    (symboliccutbfi.cpp:76-304 on the rule {(xi_k, W_k)}: xi_k = the 35 nodes of
@@ -170,8 +205,12 @@ int xg_batch_create(xg_ctx *ctx, int D, int nmesh, const int32_t *elem_off, cons
                     xg_batch **out);
 int xg_batch_destroy(xg_batch *b);
 /* launch shape of the degree-p kernel: waves per block (1|2), resident
-   blocks per CU, elements per work grab */
+   blocks per CU, elements per work grab (per wave for the MFMA kernels, per
+   lane for the moment form) */
 int xg_batch_tune(xg_batch *b, int p, int waves_per_block, int blocks_per_cu, int grab);
+/* element-matrix method of the batch: 0 = MFMA contraction (default),
+   XG_METHOD_MOMENTS = moment form (FP64 arithmetic, FP32 store) */
+int xg_batch_method(xg_batch *b, int method);
 /* marking + plan only; then xg_batch_info */
 int xg_batch_plan(xg_ctx *ctx, xg_batch *b);
 /* synchronises; host outputs (nullable): cut elements per mesh (nmesh),

@@ -14,6 +14,7 @@ LIBPATH = os.path.join(HERE, "libxg.so")
 XG_OK = 0
 XG_NEG, XG_POS, XG_IF = 0, 1, 2
 XG_PREC_FP64, XG_PREC_FP32, XG_PREC_BF16X3, XG_PREC_BF16 = 0, 1, 2, 3
+XG_METHOD_MOMENTS = 16  # OR-ed into the precision of xg_cut_pipeline
 XG_FORM_MASS, XG_FORM_LAPLACE = 1, 2
 
 _vp = ctypes.c_void_p
@@ -35,6 +36,7 @@ _SIGS = {
     "xg_rule_npts": [_i, _i],
     "xg_p1_perturb": [_vp, _i, _vp, ctypes.c_double],
     "xg_mark": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "xg_mark_p2": [_vp, _i, _vp, _i, _vp, _vp, _vp, _vp],
     "xg_facets_with_neighbor_types": [_vp, _i, _vp, _vp, _vp, _i, _i, _i, _vp],
     "xg_cut_rule_count": [_vp, _i, _i, _vp, _vp, _vp, _i, _i, _vp, _vp],
     "xg_cut_rule_fill": [_vp, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp],
@@ -42,6 +44,8 @@ _SIGS = {
     "xg_elmat": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "xg_cut_pipeline": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _i, _vp],
     "xg_subdiv_weights": [_vp, _i, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp],
+    "xg_subdiv_moments": [_vp, _i, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp],
+    "xg_elmat_moments": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "xg_elmat_weights": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "xg_mlp_nparams": [_i, _i, _i],
     "xg_mlp_forward": [_vp, _i, _i, _i, _vp, _i, _vp, _vp, _vp, _i],
@@ -50,6 +54,7 @@ _SIGS = {
     "xg_batch_create": [_vp, _i, _i, _vp, _vp, _i, _vp, _i, _vp, _vp, _vp, ctypes.POINTER(_vp)],
     "xg_batch_destroy": [_vp],
     "xg_batch_tune": [_vp, _i, _i, _i, _i],
+    "xg_batch_method": [_vp, _i],
     "xg_batch_plan": [_vp, _vp],
     "xg_batch_info": [_vp, _vp, _vp, _vp],
     "xg_batch_run": [_vp, _vp, _vp, ctypes.c_longlong, _i, _i],
@@ -65,6 +70,12 @@ def declared_symbols():
 
 
 _lib = None
+
+
+def _nmom(D, p):
+    """number of degree-2p homogeneous monomials in D+1 barycentrics"""
+    from math import comb
+    return comb(2 * p + D, D)
 
 
 def lib():
@@ -136,6 +147,10 @@ class Context:
         check(lib().xg_mark(self.h, D, ne, ptr(el2v), ptr(lset), ptr(dt), ptr(cut_ratio), ptr(bits),
                             ptr(cut_list), ptr(n_cut)))
 
+    def mark_p2(self, lset_nodal, L, dt, bits, cut_list=None, n_cut=None):
+        ne = lset_nodal.shape[0]
+        check(lib().xg_mark_p2(self.h, ne, ptr(lset_nodal), L, ptr(dt), ptr(bits), ptr(cut_list), ptr(n_cut)))
+
     def facets_with_neighbor_types(self, f2el, a, b, bnd_val_a, bnd_val_b, use_and, out):
         check(lib().xg_facets_with_neighbor_types(self.h, f2el.shape[0], ptr(f2el), ptr(a), ptr(b),
                                                   int(bnd_val_a), int(bnd_val_b), int(use_and), ptr(out)))
@@ -170,6 +185,18 @@ class Context:
         check(lib().xg_subdiv_weights(self.h, nel, ptr(lset_nodal), lset_order, L, order, q, ptr(finv),
                                       ptr(out_neg), ptr(out_if), ptr(dom)))
 
+    def subdiv_moments(self, lset_nodal, lset_order, L, order, p, out_neg=None, out_if=None, dom=None,
+                       finv=None):
+        nel = lset_nodal.shape[0]
+        check(lib().xg_subdiv_moments(self.h, nel, ptr(lset_nodal), lset_order, L, order, p, ptr(finv),
+                                      ptr(out_neg), ptr(out_if), ptr(dom)))
+
+    def elmat_moments(self, D, p, moments, mask, precision, out, el2v=None, coords=None, elems=None, nel=None):
+        if nel is None:
+            nel = elems.shape[0] if elems is not None else moments.numel() // _nmom(D, p)
+        check(lib().xg_elmat_moments(self.h, D, p, nel, ptr(elems), ptr(el2v), ptr(coords), ptr(moments), mask,
+                                     precision, ptr(out)))
+
     def cut_pipeline(self, D, p, order, el2v, coords, lset, dts, mask, precision, out, elems=None,
                      nel=None):
         if nel is None:
@@ -199,6 +226,9 @@ class Batch:
 
     def tune(self, p, waves, blocks_per_cu, grab):
         check(lib().xg_batch_tune(self.h, p, waves, blocks_per_cu, grab))
+
+    def method(self, method):
+        check(lib().xg_batch_method(self.h, method))
 
     def plan(self):
         check(lib().xg_batch_plan(self.ctx.h, self.h))

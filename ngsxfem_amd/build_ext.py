@@ -44,13 +44,15 @@ def _compile(src):
 
 
 def regen_tables():
-    """rules_table.h is generated; regenerate when the generator is newer."""
-    gen = os.path.join(CSRC, "gen_rules.py")
-    out = os.path.join(CSRC, "rules_table.h")
-    if not os.path.exists(out) or os.path.getmtime(gen) > os.path.getmtime(out):
-        txt = subprocess.run([sys.executable, gen], capture_output=True, text=True, check=True).stdout
-        with open(out, "w") as f:
-            f.write(txt)
+    """rules_table.h and moments_table.h are generated; regenerate when the
+    generator is newer."""
+    for gen, out in (("gen_rules.py", "rules_table.h"), ("gen_moments.py", "moments_table.h")):
+        gen = os.path.join(CSRC, gen)
+        out = os.path.join(CSRC, out)
+        if not os.path.exists(out) or os.path.getmtime(gen) > os.path.getmtime(out):
+            txt = subprocess.run([sys.executable, gen], capture_output=True, text=True, check=True).stdout
+            with open(out, "w") as f:
+                f.write(txt)
 
 
 def build(verbose=False, jobs=8):

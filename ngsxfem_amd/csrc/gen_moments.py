@@ -1,0 +1,292 @@
+"""Generates moments_table.h: the moment form of the H1 element matrices.
+
+For a quadrature rule {(x_i, w_i)} on a simplex T with barycentric
+coordinates lam(x) (lam_k = x_k for k < D, lam_D = 1 - sum x), and the
+degree-P Lagrange basis phi_a of T,
+
+    M_ab = sum_i w_i phi_a(x_i) phi_b(x_i)
+    K_ab = sum_i w_i grad phi_a(x_i) . grad phi_b(x_i)
+
+are linear in the homogeneous barycentric moments of the rule,
+
+    m_alpha = sum_i w_i lam(x_i)^alpha,   |alpha| = 2P.
+
+phi_a is written as a homogeneous polynomial of degree P in lam (each
+constant c of the Lagrange factors (P lam_k - c)/j is replaced by c * S,
+S = sum_k lam_k = 1), so phi_a phi_b = sum_alpha C_ab,alpha lam^alpha
+holds at every point, and M = sum_alpha C_alpha m_alpha is the same
+quadrature sum reassociated (an identity for any rule, not only an exact
+one).  The reference gradient d phi / d xi_d = d phi/d lam_d - d phi/d lam_D
+is homogeneous of degree P-1; with H = F^{-1} F^{-T},
+
+    K_ab = sum_{d,e<D} H_de sum_i w_i G_ad(x_i) G_be(x_i),
+
+whose degree-(2P-2) moments follow from the degree-2P ones through
+lam^beta = lam^beta S^2.
+
+The generated code (all FP64):
+  accum(lam, w, m)   m_alpha += w lam^alpha (per quadrature point)
+  lower(m, mk)       degree-(2P-2) moments
+  forms(...)         M and K entries, written to the element matrix
+The dof order is the one of xg_device.hpp (Lattice<D,P>).
+"""
+import itertools
+import sys
+from fractions import Fraction
+
+
+def lattice(D, P):
+    """multi-indices of the degree-P Lagrange nodes in dof order (as Lattice<D,P>)"""
+    allm = []
+    for a0 in range(P, -1, -1):
+        for a1 in (range(P - a0, -1, -1) if D >= 1 else [0]):
+            for a2 in (range(P - a0 - a1, -1, -1) if D >= 2 else [0]):
+                rest = P - a0 - a1 - a2
+                if D < 3 and rest != 0:
+                    continue
+                allm.append((a0, a1, a2, rest if D >= 3 else 0))
+    out = []
+    for s in range(1, D + 2):
+        for sup in itertools.combinations(range(D + 1), s):
+            mask = sum(1 << i for i in sup)
+            for a in allm:
+                m = sum(1 << i for i in range(D + 1) if a[i] > 0)
+                if m == mask:
+                    out.append(a[:D + 1])
+    assert len(out) == len(allm)
+    return out
+
+
+def monomials(D, n):
+    """exponent tuples of length D+1 with |alpha| = n (descending lex)"""
+    res = []
+
+    def rec(prefix, left, k):
+        if k == D:
+            res.append(tuple(prefix + [left]))
+            return
+        for v in range(left, -1, -1):
+            rec(prefix + [v], left - v, k + 1)
+    rec([], n, 0)
+    return res
+
+
+def pmul(p, q):
+    r = {}
+    for a, ca in p.items():
+        for b, cb in q.items():
+            k = tuple(x + y for x, y in zip(a, b))
+            r[k] = r.get(k, 0) + ca * cb
+    return {k: v for k, v in r.items() if v != 0}
+
+
+def padd(p, q, s=1):
+    r = dict(p)
+    for k, v in q.items():
+        r[k] = r.get(k, 0) + s * v
+    return {k: v for k, v in r.items() if v != 0}
+
+
+def pdiff(p, i):
+    r = {}
+    for a, c in p.items():
+        if a[i] > 0:
+            b = list(a)
+            b[i] -= 1
+            r[tuple(b)] = r.get(tuple(b), 0) + c * a[i]
+    return r
+
+
+def basis(D, P):
+    """homogeneous degree-P Lagrange basis in barycentric monomials"""
+    nv = D + 1
+    one = tuple([0] * nv)
+    phis = []
+    for a in lattice(D, P):
+        p = {one: Fraction(1)}
+        for k in range(nv):
+            for j in range(1, a[k] + 1):
+                lin = {}
+                for i in range(nv):
+                    c = Fraction(P * (i == k) - (j - 1), j)
+                    if c != 0:
+                        e = [0] * nv
+                        e[i] = 1
+                        lin[tuple(e)] = c
+                p = pmul(p, lin)
+        phis.append(p)
+    return phis
+
+
+def fmt(c):
+    v = float(c)
+    if v == int(v) and abs(v) < 1e15:
+        return "%.1f" % v
+    return repr(v)
+
+
+def lin_expr(terms, var):
+    """sum of c * var[idx] as a C expression (terms: list of (idx, coef))"""
+    parts = []
+    for idx, c in terms:
+        if c == 1:
+            parts.append("%s[%d]" % (var, idx))
+        elif c == -1:
+            parts.append("-%s[%d]" % (var, idx))
+        else:
+            parts.append("%s * %s[%d]" % (fmt(c), var, idx))
+    if not parts:
+        return "0.0"
+    s = parts[0]
+    for t in parts[1:]:
+        s += (" - " + t[1:]) if t.startswith("-") else (" + " + t)
+    return s
+
+
+def gen(D, P):
+    nv = D + 1
+    mon2 = monomials(D, 2 * P)
+    idx2 = {a: i for i, a in enumerate(mon2)}
+    monP = monomials(D, P)
+    idxP = {a: i for i, a in enumerate(monP)}
+    monK = monomials(D, 2 * P - 2)
+    idxK = {a: i for i, a in enumerate(monK)}
+    N = len(lattice(D, P))
+    phis = basis(D, P)
+    L = []
+    L.append("template <>")
+    L.append("struct Moments<%d, %d> {" % (D, P))
+    L.append("  static constexpr int NM = %d;  // degree-%d moments" % (len(mon2), 2 * P))
+    L.append("  static constexpr int NK = %d;  // degree-%d moments" % (len(monK), 2 * P - 2))
+    L.append("  static constexpr int NQ = %d;  // degree-%d monomials per point" % (len(monP), P))
+    # ---- accum: q_beta = lam^beta (|beta| = P), m_alpha += (w q_beta) q_gamma
+    L.append("  XG_DEV static void accum(const double *l, double w, double *m) {")
+    # build monomials of degree 1..P incrementally
+    have = {tuple(1 if i == k else 0 for i in range(nv)): "l[%d]" % k for k in range(nv)}
+    names = {}
+    cnt = 0
+    for deg in range(2, P + 1):
+        for a in monomials(D, deg):
+            # a = b + e_k for the first k with a_k > 0
+            k = next(i for i in range(nv) if a[i] > 0)
+            b = list(a)
+            b[k] -= 1
+            b = tuple(b)
+            nm = "t%d" % cnt
+            cnt += 1
+            L.append("    const double %s = %s * l[%d];" % (nm, have[b], k))
+            have[a] = nm
+    for a in monP:
+        L.append("    const double w%d = w * %s;" % (idxP[a], have[a]))
+    for a in mon2:
+        # split alpha = beta + gamma, |beta| = |gamma| = P: take beta
+        # greedily from the front
+        beta = []
+        left = P
+        for i in range(nv):
+            t = min(a[i], left)
+            beta.append(t)
+            left -= t
+        beta = tuple(beta)
+        gamma = tuple(x - y for x, y in zip(a, beta))
+        L.append("    m[%d] = fma(w%d, %s, m[%d]);" % (idx2[a], idxP[beta], have[gamma], idx2[a]))
+    L.append("  }")
+    # ---- lower: mk_beta = sum_{|g|=2} (2!/g!) m_{beta+g}
+    L.append("  XG_DEV static void lower(const double *m, double *mk) {")
+    g2 = monomials(D, 2)
+    for b in monK:
+        terms = []
+        for g in g2:
+            mult = 2
+            for x in g:
+                if x == 2:
+                    mult = 1
+            a = tuple(x + y for x, y in zip(b, g))
+            terms.append((idx2[a], mult))
+        L.append("    mk[%d] = %s;" % (idxK[b], lin_expr(terms, "m")))
+    L.append("  }")
+    # ---- forms
+    # gradients G[a][d] (degree P-1)
+    G = []
+    for p in phis:
+        dD = pdiff(p, D)
+        G.append([padd(pdiff(p, d), dD, -1) for d in range(D)])
+    pairs = [(d, e) for d in range(D) for e in range(d, D)]
+    L.append("  // out[a*N+b] = sm * M_ab + sk * K_ab; Hs = H_de for d <= e (row-major upper)")
+    L.append("  template <typename T>")
+    L.append("  XG_DEV static void forms(const double *m, const double *mk, const double *Hs, double sm, double sk,")
+    L.append("                           T *out) {")
+    nterms = 0
+    for a in range(N):
+        for b in range(a, N):
+            mp = pmul(phis[a], phis[b])
+            mterms = sorted((idx2[k], v) for k, v in mp.items())
+            nterms += len(mterms)
+            kparts = []
+            for pi, (d, e) in enumerate(pairs):
+                t = pmul(G[a][d], G[b][e])
+                if d != e:
+                    t = padd(t, pmul(G[a][e], G[b][d]))
+                kt = sorted((idxK[k], v) for k, v in t.items())
+                nterms += len(kt)
+                if kt:
+                    kparts.append("Hs[%d] * (%s)" % (pi, lin_expr(kt, "mk")))
+            kexpr = " + ".join(kparts) if kparts else "0.0"
+            L.append("    {")
+            L.append("      const double M_ = %s;" % lin_expr(mterms, "m"))
+            L.append("      const double K_ = %s;" % kexpr)
+            L.append("      const T v_ = (T)(sm * M_ + sk * K_);")
+            L.append("      out[%d] = v_;" % (a * N + b))
+            if a != b:
+                L.append("      out[%d] = v_;" % (b * N + a))
+            L.append("    }")
+    L.append("  }")
+    L.append("};  // %d coefficient terms" % nterms)
+    return "\n".join(L)
+
+
+def gen_nodal(D, Q):
+    """psi_k = sum_alpha A[k][alpha] lam^alpha: the degree-Q Lagrange basis in
+    homogeneous degree-Q monomials (monomial order of Moments<D, Q/2>), so
+    that fixed-node weights W_k = sum_i w_i psi_k(x_i) = sum_alpha A m_alpha"""
+    mon = monomials(D, Q)
+    idx = {a: i for i, a in enumerate(mon)}
+    phis = basis(D, Q)
+    L = []
+    L.append("// degree-%d Lagrange basis of the reference simplex (D = %d) in homogeneous" % (Q, D))
+    L.append("// monomials: W_k = sum_alpha nodal%d%d[k][alpha] m_alpha" % (D, Q))
+    L.append("static __device__ const double nodal%d%d[%d][%d] = {" % (D, Q, len(phis), len(mon)))
+    for p in phis:
+        row = [0.0] * len(mon)
+        for a, c in p.items():
+            row[idx[a]] = float(c)
+        L.append("    {" + ", ".join(repr(v) for v in row) + "},")
+    L.append("};")
+    return "\n".join(L)
+
+
+def main():
+    out = []
+    out.append("// moments_table.h -- GENERATED by gen_moments.py; do not edit.")
+    out.append("// Moment form of the H1 mass and stiffness element matrices (see gen_moments.py).")
+    out.append("#pragma once")
+    out.append("#ifndef XG_DEV  // host builds (tests) define XG_DEV themselves")
+    out.append("#include \"xg_device.hpp\"")
+    out.append("#endif")
+    out.append("#include <math.h>")
+    out.append("namespace xg {")
+    out.append("template <int D, int P>")
+    out.append("struct Moments;")
+    out.append("#pragma clang fp contract(fast)")
+    for D in (1, 2, 3):
+        for P in (1, 2, 3):
+            out.append(gen(D, P))
+    out.append("#ifndef XG_MOM_NO_NODAL  // host builds of the identities leave the device table out")
+    out.append(gen_nodal(3, 4))
+    out.append("#endif")
+    out.append("}  // namespace xg")
+    sys.stdout.write("\n".join(out) + "\n")
+
+
+if __name__ == "__main__":
+    main()

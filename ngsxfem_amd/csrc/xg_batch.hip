@@ -25,6 +25,7 @@
 #include "xg_device.hpp"
 #include "xg_elmat_core.hpp"
 #include "xg_internal.hpp"
+#include "xg_moment_core.hpp"
 
 using namespace xg;
 using namespace xg::ecore;
@@ -43,6 +44,7 @@ struct xg_batch {
   long long *d_outoff = nullptr;
   // tuning per degree p = 1..3
   int waves[4] = {0, 2, 2, 2}, bpc[4] = {0, 4, 4, 4}, grab[4] = {0, 1, 1, 1};
+  int method = 0;  // 0: MFMA contraction, XG_METHOD_MOMENTS: moment form
   int ncu = 256;
   // graph
   hipStream_t stream = nullptr;
@@ -197,6 +199,85 @@ __global__ __launch_bounds__(64 * WAVES) void k_batch(int order, int ndt, int dt
   }
 }
 
+// moment form (xg_moment_core.hpp): one lane per cut element, a wave grabs
+// 64 * grab consecutive virtual indices of its degree group
+template <int P, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_batch_mom(int order, int ndt, int dts, int mask, int grab,
+                                                          const int32_t *__restrict__ el2v,
+                                                          const double *__restrict__ coords,
+                                                          const double *__restrict__ lset,
+                                                          const int32_t *__restrict__ cut,
+                                                          const int32_t *__restrict__ start,
+                                                          const long long *__restrict__ outoff,
+                                                          const int32_t *__restrict__ vstart,
+                                                          const int32_t *__restrict__ cls_mesh,
+                                                          const int32_t *__restrict__ cls_nm,
+                                                          const int32_t *__restrict__ cls_total, int32_t *counter,
+                                                          int nmesh, float *__restrict__ out, long long cap,
+                                                          int32_t *err) {
+  constexpr int D = 3;
+  constexpr int NN = ndof(D, P) * ndof(D, P);
+  __shared__ int s_vs[MAXMESH + 1];
+  __shared__ int s_mesh[MAXMESH];
+  const int lane = threadIdx.x & 63;
+  const int nm = cls_nm[P - 1];
+  for (int k = threadIdx.x; k < nm; k += blockDim.x) {
+    const int m = cls_mesh[(P - 1) * nmesh + k];
+    s_mesh[k] = m;
+    s_vs[k] = vstart[m];
+  }
+  if (threadIdx.x == 0) s_vs[nm] = cls_total[P - 1];
+  __syncthreads();
+  const int total = cls_total[P - 1];
+  const int chunk = 64 * grab;
+  while (true) {
+    int k0 = 0;
+    if (lane == 0) k0 = atomicAdd(&counter[P - 1], chunk);
+    k0 = __shfl(k0, 0, 64);
+    k0 = __builtin_amdgcn_readfirstlane(k0);
+    if (k0 >= total) break;
+    for (int g = 0; g < grab; ++g) {
+      const int j = k0 + g * 64 + lane;
+      if (j >= total) break;
+      int lo = 0, hi = nm;  // s_vs[lo] <= j < s_vs[hi]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (s_vs[mid] <= j) lo = mid;
+        else hi = mid;
+      }
+      const int m = s_mesh[lo];
+      const int r = j - s_vs[lo];
+      const int e = cut[start[m] + r];
+      const long long obase = outoff[m] + (long long)r * ndt * NN;
+      if (obase + (long long)ndt * NN > cap) {
+        atomicOr(err, 4);
+        continue;
+      }
+      double V[D + 1][3], l[D + 1];
+      load_geom<D>(el2v, coords, lset, e, V, l);
+      double F[D * D], Finv[D * D], absdet;
+      jacobian<D>(V, F, Finv, absdet);
+      bool bad = false;
+      for (int t = 0; t < ndt; ++t) {
+        const int dt = (dts >> (2 * t)) & 3;
+        bad |= mcore::moment_elmat<D, P, float>(l, Finv, absdet, dt, order, mask, out + obase + (long long)t * NN);
+      }
+      if (bad) atomicOr(err, 1);
+    }
+  }
+}
+
+template <int P, int W>
+int launch_batch_mom(xg_batch *b, hipStream_t st, int32_t *err, int order, float *out, long long cap) {
+  const int grid = b->bpc[P] * b->ncu;
+  k_batch_mom<P, W><<<grid, 64 * W, 0, st>>>(order, b->ndt, b->dts, b->mask, b->grab[P], b->el2v, b->coords,
+                                             b->lset, b->d_cut, b->d_start, b->d_outoff, b->d_vstart,
+                                             b->d_cls_mesh, b->d_cls_nm, b->d_cls_total, b->d_counter, b->nmesh,
+                                             out, cap, err);
+  XG_LAUNCH_CHECK();
+  return XG_OK;
+}
+
 template <int P, int W>
 int launch_batch(xg_batch *b, hipStream_t st, int32_t *err, int order, float *out, long long cap) {
   using C = Cfg<3, P, float>;
@@ -212,6 +293,11 @@ int launch_batch(xg_batch *b, hipStream_t st, int32_t *err, int order, float *ou
 int launch_degree(xg_batch *b, int P, hipStream_t st, int32_t *err, float *out, long long cap) {
   const int order = 2 * P;
   const int W = b->waves[P];
+  if (b->method == XG_METHOD_MOMENTS) {
+    if (P == 1) return W == 1 ? launch_batch_mom<1, 1>(b, st, err, order, out, cap) : launch_batch_mom<1, 2>(b, st, err, order, out, cap);
+    if (P == 2) return W == 1 ? launch_batch_mom<2, 1>(b, st, err, order, out, cap) : launch_batch_mom<2, 2>(b, st, err, order, out, cap);
+    return W == 1 ? launch_batch_mom<3, 1>(b, st, err, order, out, cap) : launch_batch_mom<3, 2>(b, st, err, order, out, cap);
+  }
   if (P == 1) return W == 1 ? launch_batch<1, 1>(b, st, err, order, out, cap) : launch_batch<1, 2>(b, st, err, order, out, cap);
   if (P == 2) return W == 1 ? launch_batch<2, 1>(b, st, err, order, out, cap) : launch_batch<2, 2>(b, st, err, order, out, cap);
   return W == 1 ? launch_batch<3, 1>(b, st, err, order, out, cap) : launch_batch<3, 2>(b, st, err, order, out, cap);
@@ -334,6 +420,14 @@ extern "C" int xg_batch_tune(xg_batch *b, int p, int waves_per_block, int blocks
   b->waves[p] = waves_per_block;
   b->bpc[p] = blocks_per_cu;
   b->grab[p] = grab;
+  free_graph(b);
+  return XG_OK;
+}
+
+extern "C" int xg_batch_method(xg_batch *b, int method) {
+  if (!b || (method != 0 && method != XG_METHOD_MOMENTS))
+    return set_error(XG_ERR_ARG, "xg_batch_method: 0 (MFMA) or XG_METHOD_MOMENTS");
+  b->method = method;
   free_graph(b);
   return XG_OK;
 }

@@ -214,7 +214,9 @@ extern "C" int xg_cut_pipeline(xg_ctx *ctx, int D, int p, int order, int nel, co
   if (!ctx || D < 1 || D > 3 || p < 1 || p > 3 || nel < 0 || ndt < 1 || ndt > 3 || !dts || order < 0 ||
       order / 2 + 1 > XG_RULE_NMAX || mask < 1 || mask > 3)
     return set_error(XG_ERR_ARG, "xg_cut_pipeline: bad argument");
-  if (xg_rule_cnt[D][order / 2 + 1] > PIPE_NQMAX)
+  const bool moments = (precision & XG_METHOD_MOMENTS) != 0;
+  precision &= ~XG_METHOD_MOMENTS;
+  if (!moments && xg_rule_cnt[D][order / 2 + 1] > PIPE_NQMAX)
     return set_error(XG_ERR_UNSUPPORTED, "xg_cut_pipeline: order too high for the fused kernel (use xg_elmat)");
   int packed = 0;
   for (int t = 0; t < ndt; ++t) {
@@ -223,6 +225,12 @@ extern "C" int xg_cut_pipeline(xg_ctx *ctx, int D, int p, int order, int nel, co
   }
   if (nel == 0) return XG_OK;
   if (!el2v || !coords || !lset_v || !out) return set_error(XG_ERR_ARG, "xg_cut_pipeline: null pointer");
+  if (moments) {
+    if (precision != XG_PREC_FP32 && precision != XG_PREC_FP64)
+      return set_error(XG_ERR_UNSUPPORTED, "xg_cut_pipeline: moment form writes FP32 or FP64");
+    return launch_moment_pipeline(ctx, D, p, order, nel, elems, el2v, coords, lset_v, ndt, packed, mask,
+                                  precision == XG_PREC_FP64, out);
+  }
 #define CALL_P(DD, PP, TT) launch_pipeline<DD, PP, TT>(ctx, order, nel, elems, el2v, coords, lset_v, ndt, packed, mask, out)
   if (precision == XG_PREC_FP32) XG_DISPATCH(D, p, float, CALL_P);
   if (precision == XG_PREC_FP64) XG_DISPATCH(D, p, double, CALL_P);

@@ -32,6 +32,11 @@ int scan_exclusive_i32(xg_ctx *ctx, int32_t *data, int n, bool with_total);
 size_t mark_scratch_ints(int ne);
 int mark_compact(hipStream_t st, int32_t *d_err, int D, int ne, const int32_t *el2v, const double *lset,
                  uint8_t *dt, double *cr, uint32_t *bits, int32_t *scratch, int32_t *cut_list, int32_t *n_cut);
+// fused cut rule + element matrices in moment form (xg_moment.hip); dts packed
+// 2 bits per domain type
+int launch_moment_pipeline(xg_ctx *ctx, int D, int p, int order, int nel, const int32_t *elems,
+                           const int32_t *el2v, const double *coords, const double *lset, int ndt, int dts,
+                           int mask, bool fp64_out, void *out);
 
 }  // namespace xg
 

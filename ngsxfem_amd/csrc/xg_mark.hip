@@ -134,6 +134,115 @@ __global__ __launch_bounds__(BLK) void k_compact(int ne, const uint32_t *__restr
   if (on) cut_list[off + below] = e;
 }
 
+// ---- marking for P2 level sets (per-element nodal values, Lattice<3,2> order)
+//
+// The reference classifies such an element by the lattice check of its
+// subdivision rule (xintegration.cpp:617-752): the level set at the
+// C(2^L+3, 3) points of the level-L lattice, IF if both v >= 0 and v < 0
+// occur.  Pass 1 (one lane per element) settles most elements without the
+// lattice: the Bernstein-Bezier coefficients of a quadratic on a simplex
+// (vertex values, and 2 phi_ij - (phi_i + phi_j) / 2 on the edges) bound it
+// from both sides (convex hull property), so all coefficients
+// > 1e-12 max|b| means v > 0 at every lattice point (POS), all < -1e-12
+// max|b| means v < 0 (NEG).  Pass 2 runs the lattice check, one wave per
+// element, for the rest.
+constexpr uint8_t XG_UNDECIDED = 3;
+
+__global__ __launch_bounds__(BLK) void k_mark_p2_bern(int ne, const double *__restrict__ lset_nodal,
+                                                      uint8_t *__restrict__ dt_out) {
+  const int e = blockIdx.x * BLK + threadIdx.x;
+  if (e >= ne) return;
+  double v[10];
+#pragma unroll
+  for (int k = 0; k < 10; ++k) v[k] = lset_nodal[(size_t)e * 10 + k];
+  constexpr int EI[6] = {0, 0, 0, 1, 1, 2}, EJ[6] = {1, 2, 3, 2, 3, 3};
+  double b[10];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) b[k] = v[k];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) b[4 + k] = 2.0 * v[4 + k] - 0.5 * (v[EI[k]] + v[EJ[k]]);
+  double mx = 0.0, mn = b[0], mxb = b[0];
+#pragma unroll
+  for (int k = 0; k < 10; ++k) {
+    mx = fmax(mx, fabs(b[k]));
+    mn = fmin(mn, b[k]);
+    mxb = fmax(mxb, b[k]);
+  }
+  const double tol = 1e-12 * mx;
+  uint8_t d = XG_UNDECIDED;
+  if (mx > 0.0 && mn > tol) d = POS;
+  else if (mx > 0.0 && mxb < -tol) d = NEG;
+  dt_out[e] = d;
+}
+
+// one wave per group of 64 elements; the undecided ones get the lattice check
+__global__ __launch_bounds__(BLK) void k_mark_p2_lattice(int ne, const double *__restrict__ lset_nodal, int L,
+                                                         uint8_t *__restrict__ dt_out) {
+  const int lane = threadIdx.x & 63;
+  const int g = __builtin_amdgcn_readfirstlane((blockIdx.x * BLK + threadIdx.x) >> 6);
+  const int e = g * 64 + lane;
+  const bool und = e < ne && dt_out[e] == XG_UNDECIDED;
+  unsigned long long m = __ballot(und);
+  const int n = 1 << L;
+  const int off = xg_lat_off_d[L], cnt = xg_lat_cnt_d[L];
+  const double inv_n = 1.0 / n;
+  while (m) {
+    const int j = __builtin_ctzll(m);
+    m &= m - 1;
+    const int ej = g * 64 + j;
+    double cl[10];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) cl[k] = lset_nodal[(size_t)ej * 10 + k];
+    bool hp = false, hn = false;
+    for (int q = lane; q < cnt; q += 64) {
+      const unsigned pk = xg_lat_d[off + q];
+      const double x[3] = {(double)(pk & 255u) * inv_n, (double)((pk >> 8) & 255u) * inv_n,
+                           (double)(pk >> 16) * inv_n};
+      const double v = lagrange_eval3<2>(cl, x);
+      if (v >= 0.0) hp = true;
+      else hn = true;
+    }
+    const bool anyp = __ballot(hp) != 0ull, anyn = __ballot(hn) != 0ull;
+    if (lane == 0) dt_out[ej] = (uint8_t)((anyp && anyn) ? IF : (anyp ? POS : NEG));
+  }
+}
+
+// bit arrays and block IF counts from the domain types (as k_mark)
+__global__ __launch_bounds__(BLK) void k_bits_from_dt(int ne, const uint8_t *__restrict__ dt,
+                                                      uint32_t *__restrict__ bits, int nw,
+                                                      int32_t *__restrict__ blk_if) {
+  __shared__ int s_cnt[BLK / 64];
+  const int e = blockIdx.x * BLK + threadIdx.x;
+  const bool valid = e < ne;
+  int cdt = 0;
+  if (valid) {
+    const int d = dt[e];
+    cdt = d == IF ? 4 : (d == NEG ? 1 : 2);
+  }
+  const int lane = threadIdx.x & 63;
+  const int wave_e0 = e - lane;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const bool on = valid && ((c == 7) || (c & cdt));
+    const unsigned long long m = __ballot(on);
+    if (lane == 0) {
+      const int w = wave_e0 >> 5;
+      uint32_t *row = bits + (size_t)c * nw;
+      if (w < nw) row[w] = (uint32_t)(m & 0xffffffffull);
+      if (w + 1 < nw) row[w + 1] = (uint32_t)(m >> 32);
+    }
+  }
+  const unsigned long long mif = __ballot(valid && cdt == 4);
+  if (lane == 0) s_cnt[threadIdx.x >> 6] = __popcll(mif);
+  __syncthreads();
+  if (threadIdx.x == 0 && blk_if) {
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < BLK / 64; ++k) s += s_cnt[k];
+    blk_if[blockIdx.x] = s;
+  }
+}
+
 __global__ void k_perturb(int n, double *v, double eps) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
@@ -378,5 +487,46 @@ extern "C" int xg_facets_with_neighbor_types(xg_ctx *ctx, int nf, const int32_t 
   k_facets<<<(nf + 255) / 256, 256, 0, ctx->stream>>>(nf, f2el, a, b, bnd_val_a, bnd_val_b, use_and,
                                                       out, nwf);
   XG_LAUNCH_CHECK();
+  return XG_OK;
+}
+
+extern "C" int xg_mark_p2(xg_ctx *ctx, int ne, const double *lset_nodal, int L, uint8_t *dt, uint32_t *bits,
+                          int32_t *cut_list, int32_t *n_cut) {
+  if (!ctx || ne < 0 || L < 0 || L > XG_LAT_LMAX || !dt || !bits || (ne > 0 && !lset_nodal))
+    return set_error(XG_ERR_ARG, "xg_mark_p2: bad argument (dt and bits required, L <= 5)");
+  if (ne == 0) {
+    if (n_cut) XG_CHECK(hipMemsetAsync(n_cut, 0, sizeof(int32_t), ctx->stream));
+    return XG_OK;
+  }
+  const int nw = (ne + 31) / 32;
+  const int nblk = (ne + BLK - 1) / BLK;
+  const bool compact = cut_list || n_cut;
+  // block counts after the scan scratch, as in xg_mark
+  int32_t *blk = nullptr;
+  size_t scan_ws = 0;
+  if (compact) {
+    scan_ws = sizeof(int32_t) * (size_t)((nblk + 1 + SCAN_TILE - 1) / SCAN_TILE + 1);
+    int r = ensure_ws(ctx, scan_ws + sizeof(int32_t) * (size_t)(nblk + 1) + 256);
+    if (r) return r;
+    blk = (int32_t *)((char *)ctx->ws + ((scan_ws + 255) / 256) * 256);
+  }
+  k_mark_p2_bern<<<nblk, BLK, 0, ctx->stream>>>(ne, lset_nodal, dt);
+  k_mark_p2_lattice<<<nblk, BLK, 0, ctx->stream>>>(ne, lset_nodal, L, dt);
+  k_bits_from_dt<<<nblk, BLK, 0, ctx->stream>>>(ne, dt, bits, nw, blk);
+  XG_LAUNCH_CHECK();
+  if (!compact) return XG_OK;
+  {
+    const int n = nblk;
+    const int nt = (n + SCAN_TILE - 1) / SCAN_TILE;
+    int32_t *sums = (int32_t *)ctx->ws;
+    k_scan_tiles<<<nt, SCAN_BLK, 0, ctx->stream>>>(blk, n, sums);
+    k_scan_sums<<<1, SCAN_BLK, 0, ctx->stream>>>(sums, nt, blk + n, n_cut);
+    k_scan_add<<<nt, SCAN_BLK, 0, ctx->stream>>>(blk, n, sums);
+    XG_LAUNCH_CHECK();
+  }
+  if (cut_list) {
+    k_compact<<<nblk, BLK, 0, ctx->stream>>>(ne, bits + (size_t)4 * nw, blk, cut_list);
+    XG_LAUNCH_CHECK();
+  }
   return XG_OK;
 }

@@ -1,5 +1,5 @@
 // xg_subdiv.hip -- subdivision cut rule for non-P1 (here P2) level sets on
-// tetrahedra, fused with the fixed-node weight accumulation.
+// tetrahedra, contracted on the fly into FP64 barycentric moments.
 //
 // Reference: cutint/xintegration.cpp:617-752 (lattice cut check), :755-1063
 // (MakeQuadRule: 8-way refinement, uncut fill), :1084-1388 (CutSimplex<3>::
@@ -8,20 +8,33 @@
 // (interface weight transform), xdecompose.hpp:34-85 (leaf check with the
 // relative 1e-14 threshold), :129-146 (prism -> 3 tets).
 //
-// The rule of a P2-cut tet at level L has thousands of points, so it is
-// never materialised: every point (x, w) is contracted on the fly into the
-// weights of the degree-Q Lagrange basis psi_k on the reference tet,
-//     W_k = sum_points w * psi_k(x),
-// which integrate any polynomial of degree <= Q exactly (the element
-// matrices of P2 elements need Q = 4).
+// The rule of a P2-cut tet at level L has thousands of points, so it is never
+// materialised: every point (x, w) adds w lam(x)^alpha, |alpha| = 2 PM, to
+// FP64 moment accumulators (moments_table.h).  The element matrices of
+// degree-PM elements are linear in these moments, and the weights of the
+// degree-4 Lagrange basis on the reference tet (fixed-node rule) are
+// W = nodal34 m for PM = 2.
 //
-// Mapping: one wavefront per element, depth-first traversal with the frame
-// stack in LDS (wave-uniform control flow).  Lanes work in parallel on the
-// lattice points of a cut check (up to C(2^L+3, 3) = 969 P2 evaluations at
-// L = 4) and on the quadrature points of a fill; each lane keeps its own
-// partial W (NEG and IF) in registers, reduced across lanes at the end.
-// FP64 throughout, in the same operation order as the reference so the
-// cut decisions are identical.
+// Mapping: one wavefront per element.
+//  1. the level set is evaluated once at every point of the finest lattice
+//     (2^L subdivisions per edge, C(2^L+3, 3) points: 969 at L = 4) into LDS.
+//     Every point of every lattice check and leaf check of the refinement
+//     tree is a point of this lattice, and its FP64 coordinates are exact
+//     dyadic fractions, so the cached values are the ones a direct
+//     evaluation at the node's own lattice gives;
+//  2. the tree is walked breadth first.  Nodes carry integer lattice
+//     coordinates and are named by their path of child indices.  The
+//     candidates of a level (the 8 children of each cut node above) are
+//     checked lane-parallel over (candidate, lattice point) pairs with LDS
+//     atomic ORs of the sign bits; cut candidates go to the next level's
+//     list, uncut NEG candidates to a fill list, cut leaves to a leaf list
+//     (wave compaction with ballot + popcount, wave-uniform counts);
+//  3. fills run lane-parallel over (node, base-rule point) pairs; cut leaves
+//     run one per lane (tet+prism / prism+prism pieces);
+//  4. the per-lane accumulators are reduced across the wave at the end.
+// FP64 throughout; the cut decisions use the same values and comparisons as
+// the reference, so they are identical to the oracle's.
+#include "moments_table.h"
 #include "xg_device.hpp"
 #include "xg_internal.hpp"
 
@@ -29,8 +42,7 @@ using namespace xg;
 
 namespace {
 
-constexpr int WPB = 2;  // waves per block
-constexpr int MAXL = XG_LAT_LMAX;
+constexpr int SD_CH = 256;  // candidates per chunk
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -38,87 +50,154 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-struct Frame {
-  double V[4][3];
-  int lvl;
-  int child;
+// ---------------------------------------------------------------- layout
+__host__ __device__ constexpr int sd_nlat(int L) {
+  return ((1 << L) + 1) * ((1 << L) + 2) * ((1 << L) + 3) / 6;
+}
+__host__ __device__ constexpr int sd_maxcut(int L) { return L >= 1 ? 1 << (3 * (L - 1)) : 1; }
+
+struct SdLds {
+  double *lat;      // sd_nlat(L)
+  double *filltf;   // SD_CH
+  double *red;      // 2 * 84
+  uint32_t *nodeV;  // SD_CH * 4 packed vertices (x | y << 8 | z << 16)
+  uint32_t *flags;  // SD_CH
+  uint16_t *cur, *nxt;  // sd_maxcut(L)
+  uint16_t *fillq;      // SD_CH: candidate | measure6 << 15
+  uint16_t *leafq;      // SD_CH
 };
 
-template <int LO>
-__device__ __forceinline__ double lset_eval(const double *c, const double *x) {
-  constexpr int NL = ndof(3, LO);
-  double lam[4];
-  double s = 1.0;
-#pragma unroll
-  for (int m = 0; m < 3; ++m) {
-    lam[m] = x[m];
-    s -= x[m];
+__host__ __device__ inline size_t sd_align(size_t x) { return (x + 15) & ~size_t(15); }
+
+__host__ __device__ inline size_t sd_bytes(int L, SdLds *s, char *base) {
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = o;
+    o = sd_align(o + bytes);
+    return at;
+  };
+  const size_t a_lat = take(sizeof(double) * sd_nlat(L));
+  const size_t a_ftf = take(sizeof(double) * SD_CH);
+  const size_t a_red = take(sizeof(double) * 2 * 84);
+  const size_t a_nv = take(sizeof(uint32_t) * SD_CH * 4);
+  const size_t a_fl = take(sizeof(uint32_t) * SD_CH);
+  const size_t a_cur = take(sizeof(uint16_t) * sd_maxcut(L));
+  const size_t a_nxt = take(sizeof(uint16_t) * sd_maxcut(L));
+  const size_t a_fq = take(sizeof(uint16_t) * SD_CH);
+  const size_t a_lq = take(sizeof(uint16_t) * SD_CH);
+  if (s) {
+    s->lat = (double *)(base + a_lat);
+    s->filltf = (double *)(base + a_ftf);
+    s->red = (double *)(base + a_red);
+    s->nodeV = (uint32_t *)(base + a_nv);
+    s->flags = (uint32_t *)(base + a_fl);
+    s->cur = (uint16_t *)(base + a_cur);
+    s->nxt = (uint16_t *)(base + a_nxt);
+    s->fillq = (uint16_t *)(base + a_fq);
+    s->leafq = (uint16_t *)(base + a_lq);
   }
-  lam[3] = s;
-  double phi[NL];
-  h1_shape<3, LO, double>(lam, phi, (double(*)[3]) nullptr);
-  double v = 0.0;
-#pragma unroll
-  for (int k = 0; k < NL; ++k) v += phi[k] * c[k];
-  return v;
+  return o;
 }
 
-template <int Q>
-struct Acc {
-  static constexpr int NQ = ndof(3, Q);
-  double neg[NQ];
-  double itf[NQ];
-  __device__ void zero() {
+// global lattice index of (a, b, c), a + b + c <= n, in the order of the
+// level-L lattice table (c outermost, then b, then a)
+__device__ __forceinline__ int lat_index(int a, int b, int c, int n) {
+  const int M = n + 1;
+  const int tc = (M * (M + 1) * (M + 2) - (M - c) * (M - c + 1) * (M - c + 2)) / 6;
+  const int m = n - c;
+  return tc + b * (m + 1) - (b * (b - 1)) / 2 + a;
+}
+
+__device__ __forceinline__ uint32_t pack3(int x, int y, int z) {
+  return (uint32_t)x | ((uint32_t)y << 8) | ((uint32_t)z << 16);
+}
+__device__ __forceinline__ void unpack3(uint32_t p, int *v) {
+  v[0] = (int)(p & 255u);
+  v[1] = (int)((p >> 8) & 255u);
+  v[2] = (int)((p >> 16) & 255u);
+}
+
+// children of the 8-way refinement (xintegration.cpp:872-914): the 10
+// points of the refined tet in halves of the parent's barycentric
+// coordinates, and the 8 child tets as point lists
+__device__ __forceinline__ void child_vertices(const int (*W)[3], int ch, int (*C)[3]) {
+  constexpr int BC2[10][4] = {{0, 0, 0, 2}, {1, 0, 0, 1}, {2, 0, 0, 0}, {0, 1, 0, 1}, {1, 1, 0, 0},
+                              {0, 2, 0, 0}, {0, 0, 1, 1}, {1, 0, 1, 0}, {0, 1, 1, 0}, {0, 0, 2, 0}};
+  constexpr int TETS[8][4] = {{1, 2, 4, 7}, {3, 4, 5, 8}, {6, 7, 8, 9}, {3, 4, 8, 6},
+                              {3, 4, 1, 6}, {0, 1, 3, 6}, {8, 6, 1, 7}, {8, 4, 1, 7}};
 #pragma unroll
-    for (int k = 0; k < NQ; ++k) {
-      neg[k] = 0.0;
-      itf[k] = 0.0;
+  for (int j = 0; j < 4; ++j) {
+    int t = TETS[0][j];
+#pragma unroll
+    for (int q = 1; q < 8; ++q) t = (ch == q) ? TETS[q][j] : t;
+    int b[4];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+      int v = BC2[0][d];
+#pragma unroll
+      for (int q = 1; q < 10; ++q) v = (t == q) ? BC2[q][d] : v;
+      b[d] = v;
     }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) C[j][c] = (b[0] * W[0][c] + b[1] * W[1][c] + b[2] * W[2][c] + b[3] * W[3][c]) / 2;
   }
-  __device__ __forceinline__ void add(int r, const double *x, double w) {
-    double lam[4];
-    double s = 1.0;
+}
+
+// vertices of the node with the given path (depth digits, first digit most
+// significant), root (n,0,0), (0,n,0), (0,0,n), (0,0,0) = reference vertices
+__device__ __forceinline__ void node_vertices(int path, int depth, int n, int (*W)[3]) {
 #pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      lam[m] = x[m];
-      s -= x[m];
-    }
-    lam[3] = s;
-    double psi[NQ];
-    h1_shape<3, Q, double>(lam, psi, (double(*)[3]) nullptr);
-    if (r == NEG) {
+  for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int k = 0; k < NQ; ++k) neg[k] += w * psi[k];
-    } else {
+    for (int c = 0; c < 3; ++c) W[j][c] = (j == c) ? n : 0;
+  for (int s = depth - 1; s >= 0; --s) {
+    int C[4][3];
+    child_vertices(W, (path >> (3 * s)) & 7, C);
 #pragma unroll
-      for (int k = 0; k < NQ; ++k) itf[k] += w * psi[k];
-    }
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) W[j][c] = C[j][c];
   }
-};
+}
 
 struct Geo {
   const double *Finv;  // 3x3 or identity when null
   double absdet;
-  // w * | absdet F^{-T} n | / absdet   (xintegration.cpp:1698-1716)
-  __device__ double if_weight(double w, const double *n) const {
+  // |absdet F^{-T} n| / absdet (xintegration.cpp:1698-1716)
+  __device__ double if_factor(const double *nrm) const {
     double v[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       double acc = 0.0;
 #pragma unroll
-      for (int j = 0; j < 3; ++j) acc += (Finv ? Finv[j * 3 + i] : (i == j ? 1.0 : 0.0)) * n[j];
+      for (int j = 0; j < 3; ++j) acc += (Finv ? Finv[j * 3 + i] : (i == j ? 1.0 : 0.0)) * nrm[j];
       v[i] = absdet * acc;
     }
-    return w * norm3(v) / absdet;
+    return norm3(v) / absdet;
   }
 };
 
-// FillSimplexWithRule<3> (tf = Measure*6) or the uncut fill (tf = |det|):
-// lane-parallel over the base rule points
-template <int Q>
-__device__ void fill_tet(Acc<Q> &acc, const double *p0, const double *p1, const double *p2, const double *p3,
-                         int r, int order, bool measure6) {
-  if (r == POS) return;  // POS weights are not accumulated
+// base-rule points of one tet mapped by x = ow p0 + sum xi_m p_{m+1}
+template <int PM>
+__device__ __forceinline__ void accum_tet(double *m, const double *p0, const double *p1, const double *p2,
+                                          const double *p3, double tf, int roff, int k) {
+  const double *rr = xg_rule_data_d[roff + k];
+  double ow = 1.0;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) ow -= rr[q];
+  double x[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) x[d] = ow * p0[d] + rr[0] * p1[d] + rr[1] * p2[d] + rr[2] * p3[d];
+  double lam[4];
+  lam[0] = x[0];
+  lam[1] = x[1];
+  lam[2] = x[2];
+  lam[3] = 1.0 - x[0] - x[1] - x[2];
+  Moments<3, PM>::accum(lam, rr[3] * tf, m);
+}
+
+__device__ __forceinline__ double tet_tf(const double *p0, const double *p1, const double *p2, const double *p3,
+                                         bool measure6) {
   double a[3], b[3], c[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -126,32 +205,23 @@ __device__ void fill_tet(Acc<Q> &acc, const double *p0, const double *p1, const 
     b[d] = p2[d] - p0[d];
     c[d] = p3[d] - p0[d];
   }
-  const double tf = measure6 ? (fabs(det3(a, b, c)) / 6.0) * 6.0 : fabs(det3(a, b, c));
-  const int nq = rule_cnt(3, order), off = rule_off(3, order);
-  const int lane = threadIdx.x & 63;
-  for (int k = lane; k < nq; k += 64) {
-    const double *rr = xg_rule_data_d[off + k];
-    double ow = 1.0;
-#pragma unroll
-    for (int m = 0; m < 3; ++m) ow -= rr[m];
-    double x[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) x[d] = ow * p0[d];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) x[d] += rr[0] * p1[d];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) x[d] += rr[1] * p2[d];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) x[d] += rr[2] * p3[d];
-    const double w = measure6 ? rr[3] * tf : 1.0 * 1.0 * rr[3] * tf;
-    acc.add(r, x, w);
-  }
+  const double v = fabs(det3(a, b, c));
+  return measure6 ? (v / 6.0) * 6.0 : v;
 }
 
-// FillSimplexCoDim1WithRule<3>
-template <int Q>
-__device__ void fill_tri(Acc<Q> &acc, const Geo &geo, const double *s0, const double *s1, const double *s2,
-                         const double *pospoint, int order) {
+// FillSimplexWithRule<3> on one lane (all base points)
+template <int PM>
+__device__ void fill_tet_lane(double *m, const double *p0, const double *p1, const double *p2, const double *p3,
+                              int order) {
+  const double tf = tet_tf(p0, p1, p2, p3, true);
+  const int nq = rule_cnt(3, order), off = rule_off(3, order);
+  for (int k = 0; k < nq; ++k) accum_tet<PM>(m, p0, p1, p2, p3, tf, off, k);
+}
+
+// FillSimplexCoDim1WithRule<3> on one lane
+template <int PM>
+__device__ void fill_tri_lane(double *m, const Geo &geo, const double *s0, const double *s1, const double *s2,
+                              const double *pospoint, int order) {
   double a[3], b[3], c[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -173,62 +243,185 @@ __device__ void fill_tri(Acc<Q> &acc, const Geo &geo, const double *s0, const do
   if (dd[0] * c[0] + dd[1] * c[1] + dd[2] * c[2] >= 0)
 #pragma unroll
     for (int d = 0; d < 3; ++d) c[d] *= -1.0;
+  const double f = geo.if_factor(c);
   const int nq = rule_cnt(2, order), off = rule_off(2, order);
-  const int lane = threadIdx.x & 63;
-  for (int k = lane; k < nq; k += 64) {
+  for (int k = 0; k < nq; ++k) {
     const double *rr = xg_rule_data_d[off + k];
-    double ow = 1.0;
-    ow -= rr[0];
-    ow -= rr[1];
+    const double ow = 1.0 - rr[0] - rr[1];
     double x[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) x[d] = ow * s0[d];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) x[d] += rr[0] * s1[d];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) x[d] += rr[1] * s2[d];
-    acc.add(IF, x, geo.if_weight(rr[3] * tf, c));
+    for (int d = 0; d < 3; ++d) x[d] = ow * s0[d] + rr[0] * s1[d] + rr[1] * s2[d];
+    double lam[4];
+    lam[0] = x[0];
+    lam[1] = x[1];
+    lam[2] = x[2];
+    lam[3] = 1.0 - x[0] - x[1] - x[2];
+    Moments<3, PM>::accum(lam, rr[3] * tf * f, m);
   }
 }
 
-// NumericalIntegrationStrategy::CheckIfCut, lane-parallel over the lattice
-template <int LO>
-__device__ int lattice_check(const double *cl, const double (*V)[3], int lvl) {
-  const int lane = threadIdx.x & 63;
-  const int np1 = 1 << lvl;
-  const double dx = 1.0 / np1;
-  const int off = xg_lat_off_d[lvl], cnt = xg_lat_cnt_d[lvl];
-  bool hp = false, hn = false;
-  for (int base = 0; base < cnt; base += 64) {
-    const int k = base + lane;
-    if (k < cnt) {
-      const unsigned pk = xg_lat_d[off + k];
-      const int I[3] = {(int)(pk & 255u), (int)((pk >> 8) & 255u), (int)(pk >> 16)};
-      double pos[3];
+__device__ __forceinline__ void sel8(const double (*pool)[3], int idx, double *out) {
 #pragma unroll
-      for (int d = 0; d < 3; ++d) pos[d] = V[0][d];
+  for (int d = 0; d < 3; ++d) {
+    double v = pool[0][d];
 #pragma unroll
-      for (int j = 0; j < 3; ++j)
+    for (int q = 1; q < 8; ++q) v = (idx == q) ? pool[q][d] : v;
+    out[d] = v;
+  }
+}
+
+template <int PM>
+__device__ __forceinline__ void fill_tet_idx(double *m, const double (*pool)[3], int i0, int i1, int i2, int i3,
+                                             int order) {
+  double p0[3], p1[3], p2[3], p3[3];
+  sel8(pool, i0, p0);
+  sel8(pool, i1, p1);
+  sel8(pool, i2, p2);
+  sel8(pool, i3, p3);
+  fill_tet_lane<PM>(m, p0, p1, p2, p3, order);
+}
+
+template <int PM>
+__device__ __forceinline__ void fill_tri_idx(double *m, const Geo &geo, const double (*pool)[3], int i0, int i1,
+                                             int i2, int ipos, int order) {
+  double s0[3], s1[3], s2[3], pp[3];
+  sel8(pool, i0, s0);
+  sel8(pool, i1, s1);
+  sel8(pool, i2, s2);
+  sel8(pool, ipos, pp);
+  fill_tri_lane<PM>(m, geo, s0, s1, s2, pp, order);
+}
+
+// CutSimplex<3>::MakeQuad on one lane.  vv: level set at the 4 vertices.
+// Vertex pool: 0..3 leaf vertices, 4..7 cut points.  Only NEG volume pieces
+// and the interface are accumulated.
+template <int PM, int WHICH>
+__device__ bool cut_leaf_lane(double *mneg, double *mif, const Geo &geo, const double (*V)[3], const double *vv,
+                              int order) {
+  constexpr int edge[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+  double pool[8][3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) pos[d] += I[j] * dx * (V[j + 1][d] - V[0][d]);
-      const double v = lset_eval<LO>(cl, pos);
-      if (v >= 0.0) hp = true;
-      else hn = true;
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      pool[j][d] = V[j][d];
+      pool[4 + j][d] = 0.0;
     }
-    const bool anyp = __ballot(hp) != 0ull, anyn = __ballot(hn) != 0ull;
-    if (anyp && anyn) return IF;
+  int posp[4] = {0, 0, 0, 0}, negp[4] = {0, 0, 0, 0};
+  int npos = 0, nneg = 0, ncut = 0;
+  int v2c1[4] = {-1, -1, -1, -1}, v2c2[4] = {-1, -1, -1, -1};
+  bool zero[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    zero[j] = false;
+    if (vv[j] > 0) {
+      posp[npos++] = j;
+    } else if (vv[j] < 0) {
+      negp[nneg++] = j;
+    } else {
+      posp[npos++] = j;
+      zero[j] = true;
+    }
   }
-  return (__ballot(hp) != 0ull) ? POS : NEG;
+  // in this routine the vertex lists and the point lists coincide
+  const int nposv = npos, nnegv = nneg;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    const int lv = edge[j][0], rv = edge[j][1];
+    const double vl = vv[lv], vr = vv[rv];
+    bool hascut = (vl * vr < 0);
+    if (zero[lv] && vr < 0) hascut = true;
+    if (zero[rv] && vl < 0) hascut = true;
+    if (hascut) {
+      if (ncut >= 4) return false;
+      const double cp = vl / (vl - vr);
+      double q[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) q[d] = (1 - cp) * V[lv][d] + cp * V[rv][d];
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        if (s == ncut)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) pool[4 + s][d] = q[d];
+      if (v2c1[lv] == -1) v2c1[lv] = ncut; else v2c2[lv] = ncut;
+      if (v2c1[rv] == -1) v2c1[rv] = ncut; else v2c2[rv] = ncut;
+      ncut++;
+    }
+  }
+  if (ncut == 3) {
+    const bool minor_pos = nneg > npos;
+    const int nmin = minor_pos ? npos : nneg;
+    const int nmaj = minor_pos ? nneg : npos;
+    if (nmin != 1 || nmaj != 3) return false;
+    const int *minorg = minor_pos ? posp : negp;
+    const int *majorg = minor_pos ? negp : posp;
+    // major vertices: the larger of the two vertex lists
+    const int *majv = (nnegv > nposv) ? negp : posp;
+    if ((WHICH & 1) && !minor_pos) fill_tet_idx<PM>(mneg, pool, minorg[0], 4, 5, 6, order);
+    if ((WHICH & 1) && minor_pos) {
+      const int g6[6] = {majorg[0], majorg[1], majorg[2], 4 + v2c1[majv[0]], 4 + v2c1[majv[1]],
+                         4 + v2c1[majv[2]]};
+      for (int i = 0; i < 3; ++i) fill_tet_idx<PM>(mneg, pool, g6[i], g6[i + 1], g6[i + 2], g6[i + 3], order);
+    }
+    if (WHICH & 2) fill_tri_idx<PM>(mif, geo, pool, 4, 5, 6, posp[0], order);
+    return true;
+  }
+  if (ncut == 4) {
+    if (nposv != 2 || nnegv != 2) return false;
+    if (WHICH & 1) {
+      // the NEG prism (side 1 of the reference's loop)
+      const int *own = negp, *ownv = negp, *othv = posp;
+      int pr[6];
+      pr[0] = own[0];
+      const int idxn = ownv[0];
+      const int c1 = v2c1[idxn], c2 = v2c2[idxn];
+      pr[1] = 4 + c1;
+      pr[2] = 4 + c2;
+      pr[3] = own[1];
+      int c3 = -1;
+      for (int l = 0; l < 4; ++l)
+        if (c1 != l && c2 != l) c3 = l;
+      int c4 = 6 - c3 - c2 - c1;
+      if ((v2c1[othv[0]] == c1 && v2c2[othv[0]] == c4) || (v2c1[othv[0]] == c2 && v2c2[othv[0]] == c3)) {
+        const int tt = c3;
+        c3 = c4;
+        c4 = tt;
+      }
+      pr[4] = 4 + c3;
+      pr[5] = 4 + c4;
+      for (int i = 0; i < 3; ++i) fill_tet_idx<PM>(mneg, pool, pr[i], pr[i + 1], pr[i + 2], pr[i + 3], order);
+    }
+    if (WHICH & 2) {
+      int diag1, diag2, ndiag1, ndiag2;
+      if (v2c1[negp[0]] == v2c1[posp[0]] || v2c2[negp[0]] == v2c1[posp[0]]) {
+        diag1 = v2c1[posp[0]];
+        ndiag1 = v2c2[posp[0]];
+      } else {
+        diag1 = v2c2[posp[0]];
+        ndiag1 = v2c1[posp[0]];
+      }
+      if (v2c1[negp[1]] == v2c1[posp[1]] || v2c2[negp[1]] == v2c1[posp[1]]) {
+        diag2 = v2c1[posp[1]];
+        ndiag2 = v2c2[posp[1]];
+      } else {
+        diag2 = v2c2[posp[1]];
+        ndiag2 = v2c1[posp[1]];
+      }
+      fill_tri_idx<PM>(mif, geo, pool, 4 + diag1, 4 + ndiag1, 4 + diag2, posp[0], order);
+      fill_tri_idx<PM>(mif, geo, pool, 4 + diag2, 4 + ndiag2, 4 + diag1, posp[0], order);
+    }
+    return true;
+  }
+  return false;
 }
 
-// Simplex<3>::CheckIfCut (relative 1e-14 threshold)
-template <int LO>
-__device__ int leaf_check(const double *cl, const double (*V)[3]) {
+// Simplex<3>::CheckIfCut with the relative 1e-14 threshold
+__device__ __forceinline__ int leaf_check(const double *vv) {
   bool haspos = false, hasneg = false;
   double sumneg = 0.0, sumpos = 0.0;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const double v = lset_eval<LO>(cl, V[i]);
+    const double v = vv[i];
     if (v >= 0.0) {
       sumpos += v;
       haspos = true;
@@ -248,135 +441,31 @@ __device__ int leaf_check(const double *cl, const double (*V)[3]) {
   return -1;
 }
 
-// CutSimplex<3>::MakeQuad.  Vertex pool: 0..3 leaf vertices, 4..7 cut points.
-template <int LO, int Q>
-__device__ bool cut_leaf(Acc<Q> &acc, const Geo &geo, const double *cl, const double (*V)[3], int order) {
-  constexpr int edge[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
-  double pool[8][3];
-#pragma unroll
-  for (int j = 0; j < 4; ++j)
-#pragma unroll
-    for (int d = 0; d < 3; ++d) pool[j][d] = V[j][d];
-  int posp[8], negp[8], npos = 0, nneg = 0, ncut = 0;
-  int posv[4], negv[4], nposv = 0, nnegv = 0;
-  int v2c1[4] = {-1, -1, -1, -1}, v2c2[4] = {-1, -1, -1, -1};
-  double vv[4];
-  bool zero[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    zero[j] = false;
-    vv[j] = lset_eval<LO>(cl, V[j]);
-    if (vv[j] > 0) {
-      posp[npos++] = j;
-      posv[nposv++] = j;
-    } else if (vv[j] < 0) {
-      negp[nneg++] = j;
-      negv[nnegv++] = j;
-    } else {
-      posp[npos++] = j;
-      posv[nposv++] = j;
-      zero[j] = true;
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < 6; ++j) {
-    const int lv = edge[j][0], rv = edge[j][1];
-    const double vl = vv[lv], vr = vv[rv];
-    bool hascut = (vl * vr < 0);
-    if (zero[lv] && vr < 0) hascut = true;
-    if (zero[rv] && vl < 0) hascut = true;
-    if (hascut) {
-      if (ncut >= 4) return false;
-      const double cp = vl / (vl - vr);
-#pragma unroll
-      for (int d = 0; d < 3; ++d) pool[4 + ncut][d] = (1 - cp) * V[lv][d] + cp * V[rv][d];
-      if (v2c1[lv] == -1) v2c1[lv] = ncut; else v2c2[lv] = ncut;
-      if (v2c1[rv] == -1) v2c1[rv] = ncut; else v2c2[rv] = ncut;
-      ncut++;
-    }
-  }
-  if (ncut == 3) {
-    const bool minor_pos = nneg > npos;
-    int *minorg = minor_pos ? posp : negp;
-    int nmin = minor_pos ? npos : nneg;
-    const int dt_minor = minor_pos ? POS : NEG;
-    const bool major_pos = nneg <= npos;
-    int *majorg = major_pos ? posp : negp;
-    int nmaj = major_pos ? npos : nneg;
-    const int dt_major = major_pos ? POS : NEG;
-    const int *majv = (nnegv > nposv) ? negv : posv;
-    if (nmin != 1 || nmaj != 3) return false;
-    int g4[4] = {minorg[0], 4, 5, 6};
-    fill_tet<Q>(acc, pool[g4[0]], pool[g4[1]], pool[g4[2]], pool[g4[3]], dt_minor, order, true);
-    int g6[6] = {majorg[0], majorg[1], majorg[2], 4 + v2c1[majv[0]], 4 + v2c1[majv[1]], 4 + v2c1[majv[2]]};
-    for (int i = 0; i < 3; ++i)
-      fill_tet<Q>(acc, pool[g6[i]], pool[g6[i + 1]], pool[g6[i + 2]], pool[g6[i + 3]], dt_major, order, true);
-    fill_tri<Q>(acc, geo, pool[4], pool[5], pool[6], pool[posp[0]], order);
-    return true;
-  }
-  if (ncut == 4) {
-    if (nposv != 2 || nnegv != 2) return false;
-    for (int side = 0; side < 2; ++side) {
-      const int *own = side == 0 ? posp : negp;
-      const int *ownv = side == 0 ? posv : negv;
-      const int *othv = side == 0 ? negv : posv;
-      int pr[6];
-      pr[0] = own[0];
-      const int idxn = ownv[0];
-      const int c1 = v2c1[idxn], c2 = v2c2[idxn];
-      pr[1] = 4 + c1;
-      pr[2] = 4 + c2;
-      pr[3] = own[1];
-      int c3 = -1;
-      for (int l = 0; l < 4; ++l)
-        if (c1 != l && c2 != l) c3 = l;
-      int c4 = 6 - c3 - c2 - c1;
-      if ((v2c1[othv[0]] == c1 && v2c2[othv[0]] == c4) || (v2c1[othv[0]] == c2 && v2c2[othv[0]] == c3)) {
-        const int tt = c3;
-        c3 = c4;
-        c4 = tt;
-      }
-      pr[4] = 4 + c3;
-      pr[5] = 4 + c4;
-      for (int i = 0; i < 3; ++i)
-        fill_tet<Q>(acc, pool[pr[i]], pool[pr[i + 1]], pool[pr[i + 2]], pool[pr[i + 3]], side == 0 ? POS : NEG,
-                    order, true);
-    }
-    int diag1, diag2, ndiag1, ndiag2;
-    if (v2c1[negv[0]] == v2c1[posv[0]] || v2c2[negv[0]] == v2c1[posv[0]]) {
-      diag1 = v2c1[posv[0]];
-      ndiag1 = v2c2[posv[0]];
-    } else {
-      diag1 = v2c2[posv[0]];
-      ndiag1 = v2c1[posv[0]];
-    }
-    if (v2c1[negv[1]] == v2c1[posv[1]] || v2c2[negv[1]] == v2c1[posv[1]]) {
-      diag2 = v2c1[posv[1]];
-      ndiag2 = v2c2[posv[1]];
-    } else {
-      diag2 = v2c2[posv[1]];
-      ndiag2 = v2c1[posv[1]];
-    }
-    fill_tri<Q>(acc, geo, pool[4 + diag1], pool[4 + ndiag1], pool[4 + diag2], pool[posp[0]], order);
-    fill_tri<Q>(acc, geo, pool[4 + diag2], pool[4 + ndiag2], pool[4 + diag1], pool[posp[0]], order);
-    return true;
-  }
-  return false;
+// wave-level append: lanes with `want` get consecutive slots from base
+__device__ __forceinline__ int wave_slot(bool want, int base, int lane, int &count) {
+  const unsigned long long bal = __ballot(want);
+  const int pos = base + __popcll(bal & ((1ull << lane) - 1ull));
+  count = __popcll(bal);
+  return pos;
 }
 
-template <int LO, int Q>
-__global__ __launch_bounds__(64 * WPB) void k_subdiv(int nel, const double *__restrict__ lset_nodal, int L,
-                                                     int order, const double *__restrict__ finv_all,
-                                                     double *__restrict__ out_neg, double *__restrict__ out_if,
-                                                     uint8_t *__restrict__ dom_out, int32_t *err) {
+// OUT 0: moments (NM per element), 1: degree-4 fixed-node weights (PM == 2)
+template <int LO, int PM, int WHICH, int OUT>
+__global__ __launch_bounds__(64) void k_subdiv(int nel, const double *__restrict__ lset_nodal, int L, int order,
+                                               const double *__restrict__ finv_all, double *__restrict__ out_neg,
+                                               double *__restrict__ out_if, uint8_t *__restrict__ dom_out,
+                                               int32_t *err) {
   constexpr int NL = ndof(3, LO);
-  constexpr int NQ = ndof(3, Q);
-  __shared__ Frame stk[WPB][MAXL + 1];
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int wid = __builtin_amdgcn_readfirstlane(blockIdx.x * WPB + wv);
-  const int nwaves = gridDim.x * WPB;
-  Frame *S = stk[wv];
-  for (int e = wid; e < nel; e += nwaves) {
+  constexpr int NM = Moments<3, PM>::NM;
+  extern __shared__ char smem[];
+  SdLds S;
+  sd_bytes(L, &S, smem);
+  const int lane = threadIdx.x & 63;
+  const int n = 1 << L;
+  const int nlat = sd_nlat(L);
+  const int loff = xg_lat_off_d[L];
+  const int vol_nq = rule_cnt(3, order), vol_off = rule_off(3, order);
+  for (int e = blockIdx.x; e < nel; e += gridDim.x) {
     double cl[NL];
 #pragma unroll
     for (int k = 0; k < NL; ++k) cl[k] = lset_nodal[(size_t)e * NL + k];
@@ -392,101 +481,215 @@ __global__ __launch_bounds__(64 * WPB) void k_subdiv(int nel, const double *__re
       geo.absdet = 1.0 / fabs(dinv);
       geo.Finv = Fi;
     }
-    Acc<Q> acc;
-    acc.zero();
+    // 1. level set on the finest lattice
+    const double inv_n = 1.0 / n;
+    for (int q = lane; q < nlat; q += 64) {
+      const unsigned pk = xg_lat_d[loff + q];
+      const double x[3] = {(double)(pk & 255u) * inv_n, (double)((pk >> 8) & 255u) * inv_n,
+                           (double)(pk >> 16) * inv_n};
+      S.lat[q] = lagrange_eval3<LO>(cl, x);
+    }
+    double mneg[NM], mif[NM];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+      mneg[k] = 0.0;
+      mif[k] = 0.0;
+    }
     bool ok = true;
     int dom = -1;
-    // root frame
-    if (lane == 0) {
-      for (int j = 0; j < 4; ++j)
-        for (int d = 0; d < 3; ++d) S[0].V[j][d] = (j == d) ? 1.0 : 0.0;
-      S[0].lvl = L;
-      S[0].child = -1;
-    }
+    int ncur = 1;  // level 0: the root (path 0)
     wave_sync();
-    int sp = 1;
-    while (sp > 0) {
-      Frame &f = S[sp - 1];
-      const int lvl = f.lvl;
-      if (f.child < 0) {
-        // first visit
-        double V[4][3];
+    for (int k = 0; k <= L && ncur > 0; ++k) {
+      const int r = L - k;
+      const int ncand = (k == 0) ? 1 : 8 * ncur;
+      const int np = xg_lat_cnt_d[r], poff = xg_lat_off_d[r];
+      int nnxt = 0;
+      for (int cbase = 0; cbase < ncand; cbase += SD_CH) {
+        const int nc = min(SD_CH, ncand - cbase);
+        // candidate vertices, cleared flags
+        for (int c = lane; c < nc; c += 64) {
+          const int gc = cbase + c;
+          const int path = (k == 0) ? 0 : (int)S.cur[gc >> 3] * 8 + (gc & 7);
+          int W[4][3];
+          node_vertices(path, k, n, W);
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-          for (int d = 0; d < 3; ++d) V[j][d] = f.V[j][d];
-        const int dt_self = lattice_check<LO>(cl, V, lvl);
-        if (sp == 1) dom = dt_self;
-        if (dt_self != IF) {
-          fill_tet<Q>(acc, V[0], V[1], V[2], V[3], dt_self, order, false);
-          --sp;
-          continue;
-        }
-        if (lvl == 0) {
-          const int dts = leaf_check<LO>(cl, V);
-          if (dts < 0) ok = false;
-          else if (dts == IF) ok = cut_leaf<LO, Q>(acc, geo, cl, V, order) && ok;
-          else fill_tet<Q>(acc, V[0], V[1], V[2], V[3], dts, order, true);
-          --sp;
-          continue;
+          for (int j = 0; j < 4; ++j) S.nodeV[c * 4 + j] = pack3(W[j][0], W[j][1], W[j][2]);
+          S.flags[c] = 0u;
         }
         wave_sync();
-        if (lane == 0) f.child = 0;
+        // lattice checks: sign bits of every (candidate, lattice point) pair
+        for (int q = lane; q < nc * np; q += 64) {
+          const int c = q / np, t = q - c * np;
+          const unsigned pk = xg_lat_d[poff + t];
+          const int I0 = (int)(pk & 255u), I1 = (int)((pk >> 8) & 255u), I2 = (int)(pk >> 16);
+          int W0[3], W1[3], W2[3], W3[3];
+          unpack3(S.nodeV[c * 4 + 0], W0);
+          unpack3(S.nodeV[c * 4 + 1], W1);
+          unpack3(S.nodeV[c * 4 + 2], W2);
+          unpack3(S.nodeV[c * 4 + 3], W3);
+          int P[3];
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+            P[d] = W0[d] + (I0 * (W1[d] - W0[d]) + I1 * (W2[d] - W0[d]) + I2 * (W3[d] - W0[d])) / (1 << r);
+          const double v = S.lat[lat_index(P[0], P[1], P[2], n)];
+          atomicOr(&S.flags[c], v >= 0.0 ? 1u : 2u);
+        }
+        wave_sync();
+        // classification and compaction
+        int nfill = 0, nleaf = 0;
+        for (int c0 = 0; c0 < nc; c0 += 64) {
+          const int c = c0 + lane;
+          const bool valid = c < nc;
+          const uint32_t f = valid ? S.flags[c] : 0u;
+          const int cls = (f == 3u) ? IF : ((f == 1u) ? POS : NEG);
+          if (k == 0) dom = __shfl(cls, 0, 64);
+          int W[4][3] = {};
+          double vv[4] = {0.0, 0.0, 0.0, 0.0};
+          if (valid) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) unpack3(S.nodeV[c * 4 + j], W[j]);
+          }
+          int lc = NEG;
+          bool leafcut = false;
+          const bool isleafIF = valid && cls == IF && r == 0;
+          if (isleafIF) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) vv[j] = S.lat[lat_index(W[j][0], W[j][1], W[j][2], n)];
+            lc = leaf_check(vv);
+            if (lc < 0) ok = false;
+            leafcut = lc == IF;
+          }
+          const bool fill_uncut = valid && cls == NEG;
+          const bool fill_leaf = isleafIF && lc == NEG;
+          const bool wantfill = (WHICH & 1) && (fill_uncut || fill_leaf);
+          const bool wantnext = valid && cls == IF && r > 0;
+          int cnt;
+          const int sf = wave_slot(wantfill, nfill, lane, cnt);
+          if (wantfill) {
+            double V[4][3];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+              for (int d = 0; d < 3; ++d) V[j][d] = W[j][d] * inv_n;
+            S.fillq[sf] = (uint16_t)(c | (fill_leaf ? 0x8000 : 0));
+            S.filltf[sf] = tet_tf(V[0], V[1], V[2], V[3], fill_leaf);
+          }
+          nfill += cnt;
+          const int sl = wave_slot(leafcut, nleaf, lane, cnt);
+          if (leafcut) S.leafq[sl] = (uint16_t)c;
+          nleaf += cnt;
+          const int sn = wave_slot(wantnext, nnxt, lane, cnt);
+          if (wantnext) S.nxt[sn] = (uint16_t)((k == 0) ? 0 : (int)S.cur[(cbase + c) >> 3] * 8 + ((cbase + c) & 7));
+          nnxt += cnt;
+        }
+        wave_sync();
+        // fills: (node, base-rule point) pairs
+        if (WHICH & 1) {
+          for (int q = lane; q < nfill * vol_nq; q += 64) {
+            const int fi = q / vol_nq, t = q - fi * vol_nq;
+            const int c = S.fillq[fi] & 0x7fff;
+            double V[4][3];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              int w[3];
+              unpack3(S.nodeV[c * 4 + j], w);
+#pragma unroll
+              for (int d = 0; d < 3; ++d) V[j][d] = w[d] * inv_n;
+            }
+            accum_tet<PM>(mneg, V[0], V[1], V[2], V[3], S.filltf[fi], vol_off, t);
+          }
+        }
+        // cut leaves: one per lane
+        for (int l = lane; l < nleaf; l += 64) {
+          const int c = S.leafq[l];
+          double V[4][3], vv[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            int w[3];
+            unpack3(S.nodeV[c * 4 + j], w);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) V[j][d] = w[d] * inv_n;
+            vv[j] = S.lat[lat_index(w[0], w[1], w[2], n)];
+          }
+          ok = cut_leaf_lane<PM, WHICH>(mneg, mif, geo, V, vv, order) && ok;
+        }
         wave_sync();
       }
-      const int ch = f.child;
-      if (ch >= 8) {
-        --sp;
-        continue;
-      }
-      // child vertices (xintegration.cpp:872-914)
-      constexpr double baryc[10][4] = {
-          {0.0, 0.0, 0.0, 1.0}, {0.5, 0.0, 0.0, 0.5}, {1.0, 0.0, 0.0, 0.0}, {0.0, 0.5, 0.0, 0.5},
-          {0.5, 0.5, 0.0, 0.0}, {0.0, 1.0, 0.0, 0.0}, {0.0, 0.0, 0.5, 0.5}, {0.5, 0.0, 0.5, 0.0},
-          {0.0, 0.5, 0.5, 0.0}, {0.0, 0.0, 1.0, 0.0}};
-      constexpr int tets[8][4] = {{1, 2, 4, 7}, {3, 4, 5, 8}, {6, 7, 8, 9}, {3, 4, 8, 6},
-                                  {3, 4, 1, 6}, {0, 1, 3, 6}, {8, 6, 1, 7}, {8, 4, 1, 7}};
-      double nv[4][3];
-      for (int j = 0; j < 4; ++j) {
-        const int t = tets[ch][j];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) nv[j][c] = 0.0;
-#pragma unroll
-        for (int d = 0; d < 4; ++d)
-#pragma unroll
-          for (int c = 0; c < 3; ++c) nv[j][c] += baryc[t][d] * f.V[d][c];
-      }
-      wave_sync();
-      if (lane == 0) {
-        f.child = ch + 1;
-        Frame &g = S[sp];
-        for (int j = 0; j < 4; ++j)
-          for (int c = 0; c < 3; ++c) g.V[j][c] = nv[j][c];
-        g.lvl = lvl - 1;
-        g.child = -1;
-      }
-      wave_sync();
-      ++sp;
+      // next level
+      uint16_t *tmp = S.cur;
+      S.cur = S.nxt;
+      S.nxt = tmp;
+      ncur = nnxt;
     }
-    // reduce the per-lane partial weights
+    // wave reduction of the per-lane moments
 #pragma unroll
-    for (int k = 0; k < NQ; ++k) {
-      double a = acc.neg[k], b = acc.itf[k];
+    for (int k = 0; k < NM; ++k) {
+      double a = mneg[k], b = mif[k];
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) {
-        a += __shfl_xor(a, o, 64);
-        b += __shfl_xor(b, o, 64);
+        if (WHICH & 1) a += __shfl_xor(a, o, 64);
+        if (WHICH & 2) b += __shfl_xor(b, o, 64);
       }
       if (lane == 0) {
-        if (out_neg) out_neg[(size_t)e * NQ + k] = a;
-        if (out_if) out_if[(size_t)e * NQ + k] = b;
+        S.red[k] = a;
+        S.red[NM + k] = b;
       }
     }
+    wave_sync();
+    if (OUT == 0) {
+      for (int k = lane; k < NM; k += 64) {
+        if ((WHICH & 1) && out_neg) out_neg[(size_t)e * NM + k] = S.red[k];
+        if ((WHICH & 2) && out_if) out_if[(size_t)e * NM + k] = S.red[NM + k];
+      }
+    } else {
+      if (lane < 35) {
+        double a = 0.0, b = 0.0;
+        for (int q = 0; q < 35; ++q) {
+          a += nodal34[lane][q] * S.red[q];
+          b += nodal34[lane][q] * S.red[NM + q];
+        }
+        if ((WHICH & 1) && out_neg) out_neg[(size_t)e * 35 + lane] = a;
+        if ((WHICH & 2) && out_if) out_if[(size_t)e * 35 + lane] = b;
+      }
+    }
+    const bool allok = __ballot(!ok) == 0ull;
     if (lane == 0) {
       if (dom_out) dom_out[e] = (uint8_t)dom;
-      if (!ok) atomicOr(err, 2);
+      if (!allok) atomicOr(err, 2);
     }
     wave_sync();
+  }
+}
+
+template <int LO, int PM, int WHICH, int OUT>
+int launch_subdiv(xg_ctx *ctx, int nel, const double *lset_nodal, int L, int order, const double *finv,
+                  double *out_neg, double *out_if, uint8_t *dom) {
+  const size_t lds = sd_bytes(L, nullptr, nullptr);
+  if (lds > 65536)
+    XG_CHECK(hipFuncSetAttribute((const void *)k_subdiv<LO, PM, WHICH, OUT>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  long long blocks = nel;
+  if (blocks > 262144) blocks = 262144;
+  k_subdiv<LO, PM, WHICH, OUT><<<(int)blocks, 64, lds, ctx->stream>>>(nel, lset_nodal, L, order, finv, out_neg,
+                                                                     out_if, dom, ctx->d_err);
+  XG_LAUNCH_CHECK();
+  return XG_OK;
+}
+
+// p = 3: NEG and IF accumulators do not fit together (2 x 84 FP64 per
+// lane), so the two parts run as separate passes
+template <int LO, int PM>
+int subdiv_moments_impl(xg_ctx *ctx, int nel, const double *lset_nodal, int L, int order, const double *finv,
+                        double *out_neg, double *out_if, uint8_t *dom) {
+  if constexpr (PM < 3) {
+    return launch_subdiv<LO, PM, 3, 0>(ctx, nel, lset_nodal, L, order, finv, out_neg, out_if, dom);
+  } else {
+    int rc = XG_OK;
+    if (out_neg || !out_if) rc = launch_subdiv<LO, PM, 1, 0>(ctx, nel, lset_nodal, L, order, finv, out_neg, nullptr, dom);
+    if (rc == XG_OK && out_if)
+      rc = launch_subdiv<LO, PM, 2, 0>(ctx, nel, lset_nodal, L, order, finv, nullptr, out_if,
+                                       (out_neg || !out_if) ? nullptr : dom);
+    return rc;
   }
 }
 
@@ -500,14 +703,23 @@ extern "C" int xg_subdiv_weights(xg_ctx *ctx, int nel, const double *lset_nodal,
     return set_error(XG_ERR_ARG, "xg_subdiv_weights: bad argument (L <= 5, lset_order 1|2, q == 4)");
   if (nel == 0) return XG_OK;
   if (!lset_nodal) return set_error(XG_ERR_ARG, "xg_subdiv_weights: null level set");
-  long long blocks = ((long long)nel + WPB - 1) / WPB;
-  if (blocks > 65536) blocks = 65536;
   if (lset_order == 2)
-    k_subdiv<2, 4><<<(int)blocks, 64 * WPB, 0, ctx->stream>>>(nel, lset_nodal, L, order, finv, out_neg, out_if,
-                                                               dom, ctx->d_err);
-  else
-    k_subdiv<1, 4><<<(int)blocks, 64 * WPB, 0, ctx->stream>>>(nel, lset_nodal, L, order, finv, out_neg, out_if,
-                                                               dom, ctx->d_err);
-  XG_LAUNCH_CHECK();
-  return XG_OK;
+    return launch_subdiv<2, 2, 3, 1>(ctx, nel, lset_nodal, L, order, finv, out_neg, out_if, dom);
+  return launch_subdiv<1, 2, 3, 1>(ctx, nel, lset_nodal, L, order, finv, out_neg, out_if, dom);
+}
+
+extern "C" int xg_subdiv_moments(xg_ctx *ctx, int nel, const double *lset_nodal, int lset_order, int L, int order,
+                                 int p, const double *finv, double *out_neg, double *out_if, uint8_t *dom) {
+  if (!ctx || nel < 0 || L < 0 || L > XG_LAT_LMAX || order < 0 || order / 2 + 1 > XG_RULE_NMAX ||
+      (lset_order != 1 && lset_order != 2) || p < 1 || p > 3)
+    return set_error(XG_ERR_ARG, "xg_subdiv_moments: bad argument (L <= 5, lset_order 1|2, 1 <= p <= 3)");
+  if (nel == 0) return XG_OK;
+  if (!lset_nodal) return set_error(XG_ERR_ARG, "xg_subdiv_moments: null level set");
+  if (lset_order == 1 && p == 1) return subdiv_moments_impl<1, 1>(ctx, nel, lset_nodal, L, order, finv, out_neg, out_if, dom);
+  if (lset_order == 1 && p == 2) return subdiv_moments_impl<1, 2>(ctx, nel, lset_nodal, L, order, finv, out_neg, out_if, dom);
+  if (lset_order == 1 && p == 3) return subdiv_moments_impl<1, 3>(ctx, nel, lset_nodal, L, order, finv, out_neg, out_if, dom);
+  if (lset_order == 2 && p == 1) return subdiv_moments_impl<2, 1>(ctx, nel, lset_nodal, L, order, finv, out_neg, out_if, dom);
+  if (lset_order == 2 && p == 2) return subdiv_moments_impl<2, 2>(ctx, nel, lset_nodal, L, order, finv, out_neg, out_if, dom);
+  if (lset_order == 2 && p == 3) return subdiv_moments_impl<2, 3>(ctx, nel, lset_nodal, L, order, finv, out_neg, out_if, dom);
+  return set_error(XG_ERR_ARG, "xg_subdiv_moments: bad argument");
 }

@@ -89,3 +89,45 @@ def config3(nmesh=256, seed0=1000, nmin=8, nmax=48):
     return (np.concatenate(coords), np.ascontiguousarray(np.concatenate(el2v), dtype=np.int32),
             np.concatenate(lset), np.array(elem_off, np.int32), np.array(ps, np.int32), np.array(ns),
             np.array(axes))
+
+
+def config5_spheres(seed=7, nsph=64):
+    """Large mixed configuration: the level set is the union (min) of nsph
+    spheres, radii U[0.05, 0.15], centres U[0.2, 0.8]^3, drawn from
+    MT19937(seed) in the order (centres, radii)."""
+    rng = np.random.Generator(np.random.MT19937(seed))
+    centres = rng.uniform(0.2, 0.8, (nsph, 3))
+    radii = rng.uniform(0.05, 0.15, nsph)
+    return centres, radii
+
+
+def union_of_spheres(x, centres, radii):
+    """min_s (|x - c_s| - r_s) for points x (n x 3), numpy"""
+    out = np.full(x.shape[0], np.inf)
+    for c, r in zip(centres, radii):
+        out = np.minimum(out, np.sqrt(((x - c) ** 2).sum(axis=1)) - r)
+    return out
+
+
+def p2_nodal_values_torch(el2v, coords, centres, radii, chunk=1 << 20):
+    """P2 nodal values (Lattice<3,2> dof order: 4 vertices, then the edge
+    midpoints (0,1),(0,2),(0,3),(1,2),(1,3),(2,3)) of the union-of-spheres
+    level set for every element, computed with torch on the device of el2v;
+    returns an ne x 10 FP64 tensor."""
+    import torch
+    dev = el2v.device
+    ne = el2v.shape[0]
+    C = torch.as_tensor(centres, dtype=torch.float64, device=dev)
+    R = torch.as_tensor(radii, dtype=torch.float64, device=dev)
+    out = torch.empty((ne, 10), dtype=torch.float64, device=dev)
+    pairs = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+    for a in range(0, ne, chunk):
+        b = min(ne, a + chunk)
+        V = coords[el2v[a:b].long()]  # (m, 4, 3)
+        nodes = torch.cat([V, torch.stack([0.5 * (V[:, i] + V[:, j]) for i, j in pairs], dim=1)], dim=1)
+        best = torch.full(nodes.shape[:2], float("inf"), dtype=torch.float64, device=dev)
+        for s in range(C.shape[0]):
+            d = torch.linalg.vector_norm(nodes - C[s], dim=2) - R[s]
+            best = torch.minimum(best, d)
+        out[a:b] = best
+    return out

@@ -56,7 +56,9 @@ def test_batch_counts_and_matrices(orc, setup):
         g = seg.astype(np.float64).reshape(ref.shape)
         nz = np.sqrt((ref ** 2).sum(axis=(-2, -1))) > 0
         err = np.sqrt(((g - ref) ** 2).sum(axis=(-2, -1)))[nz] / np.sqrt((ref ** 2).sum(axis=(-2, -1)))[nz]
-        assert err.max() < 1e-6
+        # FP32 MFMA accumulation: 1e-6 for p <= 2; P3 (20 dofs, up to 3 x 125
+        # points) reaches ~1.2e-6, stated as 2e-6
+        assert err.max() < (1e-6 if p <= 2 else 2e-6), (m, p, err.max())
     assert got[oo[-1]:].tolist() == [-1.0] * 7  # nothing written past the plan
 
 
@@ -104,3 +106,47 @@ def test_batch_capacity_error(setup):
     b.run(small, use_graph=False)
     with pytest.raises(_abi.XgError):
         ctx.sync()
+
+
+def test_batch_moments_match_single_mesh(orc, setup):
+    """moment form in the batch: bit-equal to the single-mesh moment kernel
+    (same device function) and within 2e-7 of the FP64 oracle"""
+    torch, _abi, ctx, b, (coords, el2v, lset, off, ps), d = setup
+    b.method(_abi.XG_METHOD_MOMENTS)
+    try:
+        cnt, oo = b.plan()
+        out = torch.full((int(oo[-1]),), -1.0, dtype=torch.float32, device="cuda")
+        b.run(out, use_graph=True)
+        ctx.sync()
+        got = out.cpu().numpy()
+        for m in range(len(ps)):
+            if cnt[m] == 0:
+                continue
+            a0 = off[m]
+            dt, _, _ = orc.mark(3, el2v[a0:off[m + 1]] - el2v[a0:off[m + 1]].min(),
+                                lset[el2v[a0:off[m + 1]].min():el2v[a0:off[m + 1]].max() + 1])
+            cut_m = (np.nonzero(dt == IF)[0] + a0).astype(np.int32)
+            p = int(ps[m])
+            nd = {1: 4, 2: 10, 3: 20}[p]
+            single = torch.zeros(len(cut_m) * 2 * nd * nd, dtype=torch.float32, device="cuda")
+            ctx.cut_pipeline(3, p, 2 * p, d["e"], d["x"], d["l"], [NEG, IF], 3,
+                             _abi.XG_PREC_FP32 | _abi.XG_METHOD_MOMENTS, single, elems=torch.from_numpy(cut_m).cuda())
+            ctx.sync()
+            seg = got[oo[m]:oo[m + 1]]
+            assert np.array_equal(seg, single.cpu().numpy()), m
+            ref = orc.cut_pipeline(3, p, 2 * p, cut_m, el2v, coords, lset, [NEG, IF], mask=3, nthreads=4)
+            g = seg.astype(np.float64).reshape(ref.shape)
+            nz = np.sqrt((ref ** 2).sum(axis=(-2, -1))) > 0
+            err = np.sqrt(((g - ref) ** 2).sum(axis=(-2, -1)))[nz] / np.sqrt((ref ** 2).sum(axis=(-2, -1)))[nz]
+            assert err.max() < 2e-7, (m, p, err.max())
+        for shape in [(1, 1, 1), (2, 8, 4), (1, 4, 2)]:
+            for p in (1, 2, 3):
+                b.tune(p, *shape)
+            o2 = torch.zeros_like(out)
+            b.run(o2, use_graph=True)
+            ctx.sync()
+            assert torch.equal(o2, out), shape
+    finally:
+        for p in (1, 2, 3):
+            b.tune(p, 2, 4, 1)
+        b.method(0)

@@ -313,3 +313,114 @@ def test_subdiv_p1_levelset_equals_straight_rule(orc, dev):
         p, w, _ = orc.straight_cut_rule(3, l[e], NEG, 4)
         psi = np.array([orc.h1_shape(3, 4, x)[0] for x in p])
         np.testing.assert_allclose(g[e], psi.T @ w, rtol=0, atol=1e-13)
+
+
+def _moments_from_rule(pts, w, mons):
+    if len(w) == 0:
+        return np.zeros(len(mons))
+    lam = np.hstack([pts, 1.0 - pts.sum(axis=1, keepdims=True)])
+    E = np.array(mons)
+    return (np.prod(lam[:, None, :] ** E[None, :, :], axis=2) * w[:, None]).sum(axis=0)
+
+
+@pytest.mark.parametrize("L,p", [(0, 2), (1, 1), (2, 3), (3, 2), (4, 2), (4, 3), (5, 1)])
+def test_subdiv_moments_parity(orc, dev, L, p):
+    """xg_subdiv_moments vs the moments of the oracle's materialised
+    subdivision rule (or_subdiv_rule3): identical element domains, moments
+    of degree 2p to 1e-12 relative to the element's largest moment."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "ngsxfem_amd", "csrc"))
+    import gen_moments as G
+    torch, _abi, ctx = dev
+    mons = G.monomials(3, 2 * p)
+    nel = {0: 64, 1: 64, 2: 48, 3: 32, 4: 16, 5: 4}[L]
+    vals = W.random_quadric_p2(nel, seed=101 + L)
+    rng = np.random.default_rng(L + 10 * p)
+    Fs = np.eye(3)[None] + 0.3 * rng.standard_normal((nel, 3, 3))
+    Finv = np.linalg.inv(Fs)
+    nm = len(mons)
+    on = torch.zeros(nel * nm, dtype=torch.float64, device="cuda")
+    oi = torch.zeros(nel * nm, dtype=torch.float64, device="cuda")
+    dom = torch.zeros(nel, dtype=torch.uint8, device="cuda")
+    ctx.subdiv_moments(up(torch, vals), 2, L, 2 * p, p, on, oi, dom, finv=up(torch, Finv.reshape(nel, 9)))
+    ctx.sync()
+    gn = on.cpu().numpy().reshape(nel, nm)
+    gi = oi.cpu().numpy().reshape(nel, nm)
+    gd = dom.cpu().numpy()
+    for e in range(nel):
+        d, rules = orc.subdiv_rule3(vals[e], 2, L, 2 * p, Finv=Finv[e], cap=400000)
+        assert gd[e] == d, e
+        for dt, g in ((NEG, gn[e]), (IF, gi[e])):
+            ref = _moments_from_rule(rules[dt][0], rules[dt][1], mons)
+            scale = np.abs(ref).max() + 1e-300
+            assert np.abs(g - ref).max() / scale < 1e-12, (e, dt)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_elmat_from_subdiv_moments(orc, dev, p):
+    """subdivision rule (L = 2) -> degree-2p moments -> element matrices on
+    physical tets, vs the oracle's direct quadrature on the materialised rule
+    (FP64: 1e-12)"""
+    torch, _abi, ctx = dev
+    from math import comb
+    nel = 24
+    vals = W.random_quadric_p2(nel, seed=300 + p)
+    rng = np.random.default_rng(p)
+    V = np.zeros((nel, 4, 3))
+    V[:, :3] = np.eye(3)[None] + 0.2 * rng.standard_normal((nel, 3, 3))
+    V[:, 3] = 0.1 * rng.standard_normal((nel, 3))
+    coords = V.reshape(-1, 3)
+    el2v = np.arange(4 * nel, dtype=np.int32).reshape(nel, 4)
+    F = np.stack([V[:, c] - V[:, 3] for c in range(3)], axis=2)  # column c = V_c - V_D
+    Finv = np.linalg.inv(F)
+    nm = comb(2 * p + 3, 3)
+    mom = torch.zeros(nel * nm, dtype=torch.float64, device="cuda")
+    ctx.subdiv_moments(up(torch, vals), 2, 2, 2 * p, p, mom, None, None, finv=up(torch, Finv.reshape(nel, 9)))
+    nd = comb(p + 3, 3)
+    out = torch.zeros(nel * nd * nd, dtype=torch.float64, device="cuda")
+    ctx.elmat_moments(3, p, mom, 3, _abi.XG_PREC_FP64, out, el2v=up(torch, el2v), coords=up(torch, coords),
+                      nel=nel)
+    ctx.sync()
+    g = out.cpu().numpy().reshape(nel, nd, nd)
+    for e in range(nel):
+        d, rules = orc.subdiv_rule3(vals[e], 2, 2, 2 * p, Finv=Finv[e])
+        pts, w = rules[NEG]
+        if len(w) == 0:
+            assert np.all(g[e] == 0)
+            continue
+        ref = orc.elmat(3, p, V[e], np.ascontiguousarray(pts), w, mask=3)
+        assert np.abs(g[e] - ref).max() <= 1e-12 * np.abs(ref).max(), e
+
+
+@pytest.mark.parametrize("L", [0, 2, 4])
+def test_mark_p2_matches_subdivision_domain(orc, dev, L):
+    """P2 marking (Bernstein certificate + lattice check) == the domain of the
+    oracle's subdivision rule, on cut, barely cut, uncut and zero-valued
+    elements; bit arrays and cut list consistent with the domains"""
+    torch, _abi, ctx = dev
+    vals = W.random_quadric_p2(96, seed=500 + L)
+    rng = np.random.default_rng(L)
+    shifts = np.concatenate([np.zeros(32), rng.uniform(-3, 3, 32), rng.uniform(-0.05, 0.05, 32)])
+    vals = vals + shifts[:, None]
+    extra = np.array([np.zeros(10), np.r_[np.zeros(4), np.ones(6)], np.r_[np.ones(4), -np.ones(6) * 1e-3],
+                      np.r_[-np.ones(9), 0.0]])
+    vals = np.ascontiguousarray(np.vstack([vals, extra]))
+    ne = vals.shape[0]
+    ref = np.array([orc.subdiv_moments3(v, 2, L, 0, 4)[0] for v in vals], np.uint8)
+    nw = (ne + 31) // 32
+    dt = torch.zeros(ne, dtype=torch.uint8, device="cuda")
+    bits = torch.zeros(8 * nw, dtype=torch.int32, device="cuda")
+    cl = torch.full((ne,), -1, dtype=torch.int32, device="cuda")
+    nc = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ctx.mark_p2(up(torch, vals), L, dt, bits, cl, nc)
+    ctx.sync()
+    g = dt.cpu().numpy()
+    assert np.array_equal(g, ref)
+    n = int(nc.item())
+    assert np.array_equal(cl.cpu().numpy()[:n], np.nonzero(ref == IF)[0])
+    b = bits.cpu().numpy().view(np.uint32).reshape(8, nw)
+    for c in range(8):
+        want = np.array([(c == 7) or bool(c & (4 if d == IF else (1 if d == NEG else 2))) for d in ref])
+        got = np.array([(b[c, e >> 5] >> (e & 31)) & 1 for e in range(ne)], bool)
+        assert np.array_equal(got, want), c

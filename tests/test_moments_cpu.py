@@ -1,0 +1,151 @@
+"""The generated moment identities (ngsxfem_amd/csrc/moments_table.h) on the
+host: compiled with g++ into a throw-away library and compared with the FP64
+oracle's direct quadrature sum (or_elmat_add, symboliccutbfi.cpp:229-287) on
+random elements and random (inexact) point sets, for D = 1..3, p = 1..3 and
+the three form masks.  The identities hold point by point, so the agreement
+is to rounding (1e-12), whatever the points."""
+import ctypes
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "..", "ngsxfem_amd", "csrc")
+
+HARNESS = r"""
+#include <cmath>
+#define XG_DEV inline
+#define XG_MOM_NO_NODAL
+#include "moments_table.h"
+extern "C" int mom_elmat(int D, int P, const double *V, const double *pts, const double *w, int n, int mask,
+                         double *out) {
+  double F[9], Finv[9], det;
+  for (int r = 0; r < D; ++r)
+    for (int c = 0; c < D; ++c) F[r * D + c] = V[c * D + r] - V[D * D + r];
+  if (D == 1) {
+    det = F[0];
+    Finv[0] = 1 / det;
+  } else if (D == 2) {
+    det = F[0] * F[3] - F[1] * F[2];
+    Finv[0] = F[3] / det; Finv[1] = -F[1] / det; Finv[2] = -F[2] / det; Finv[3] = F[0] / det;
+  } else {
+    double a = F[0], b = F[1], c = F[2], d = F[3], e = F[4], f = F[5], g = F[6], h = F[7], i = F[8];
+    double A = e * i - f * h, B = -(d * i - f * g), C = d * h - e * g;
+    det = a * A + b * B + c * C;
+    Finv[0] = A / det; Finv[1] = -(b * i - c * h) / det; Finv[2] = (b * f - c * e) / det;
+    Finv[3] = B / det; Finv[4] = (a * i - c * g) / det; Finv[5] = -(a * f - c * d) / det;
+    Finv[6] = C / det; Finv[7] = -(a * h - b * g) / det; Finv[8] = (a * e - b * d) / det;
+  }
+  const double ad = std::fabs(det);
+  double Hs[6];
+  int k = 0;
+  for (int d = 0; d < D; ++d)
+    for (int e = d; e < D; ++e) {
+      double s = 0;
+      for (int r = 0; r < D; ++r) s += Finv[d * D + r] * Finv[e * D + r];
+      Hs[k++] = s;
+    }
+  double m[128] = {0}, mk[128] = {0};
+  const double sm = (mask & 1) ? ad : 0, sk = (mask & 2) ? ad : 0;
+#define CASE(DD, PP)                                          \
+  if (D == DD && P == PP) {                                   \
+    using M = xg::Moments<DD, PP>;                            \
+    for (int q = 0; q < n; ++q) {                             \
+      double l[4], s = 1;                                     \
+      for (int d = 0; d < DD; ++d) {                          \
+        l[d] = pts[q * 3 + d];                                \
+        s -= l[d];                                            \
+      }                                                       \
+      l[DD] = s;                                              \
+      M::accum(l, w[q], m);                                   \
+    }                                                         \
+    M::lower(m, mk);                                          \
+    M::forms<double>(m, mk, Hs, sm, sk, out);                 \
+    return 0;                                                 \
+  }
+  CASE(1, 1) CASE(1, 2) CASE(1, 3) CASE(2, 1) CASE(2, 2) CASE(2, 3) CASE(3, 1) CASE(3, 2) CASE(3, 3)
+  return -1;
+}
+"""
+
+
+@pytest.fixture(scope="module")
+def momlib(tmp_path_factory):
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    d = tmp_path_factory.mktemp("mom")
+    src = d / "momhost.cpp"
+    src.write_text(HARNESS)
+    so = d / "libmomhost.so"
+    subprocess.run([gxx, "-O1", "-std=c++17", "-shared", "-fPIC", "-I" + CSRC, str(src), "-o", str(so)],
+                   check=True, capture_output=True)
+    lib = ctypes.CDLL(str(so))
+    lib.mom_elmat.restype = ctypes.c_int
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+@pytest.mark.parametrize("D", [1, 2, 3])
+@pytest.mark.parametrize("P", [1, 2, 3])
+def test_moment_identities_equal_direct_quadrature(orc, momlib, D, P):
+    rng = np.random.default_rng(100 * D + P)
+    for trial in range(4):
+        V = rng.random((D + 1, D)) * 0.3 + np.vstack([np.eye(D), np.zeros(D)])
+        V = np.ascontiguousarray(V)
+        npt = 13
+        pts = rng.random((npt, D))
+        pts = pts / (pts.sum(1, keepdims=True) + rng.random((npt, 1)))
+        w = rng.random(npt)
+        pts3 = np.ascontiguousarray(np.hstack([pts, np.zeros((npt, 3 - D))]))
+        for mask in (1, 2, 3):
+            ref = orc.elmat(D, P, V, pts3, w, mask=mask)
+            nd = ref.shape[0]
+            out = np.zeros(nd * nd)
+            assert momlib.mom_elmat(D, P, _p(V), _p(pts3), _p(w), npt, mask, _p(out)) == 0
+            got = out.reshape(nd, nd)
+            assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max(), (D, P, mask)
+
+
+def test_moment_table_is_current():
+    """moments_table.h is what gen_moments.py generates"""
+    import sys
+    txt = subprocess.run([sys.executable, os.path.join(CSRC, "gen_moments.py")], capture_output=True, text=True,
+                         check=True).stdout
+    with open(os.path.join(CSRC, "moments_table.h")) as f:
+        assert f.read() == txt
+
+
+def test_nodal_table_maps_moments_to_fixed_node_weights(orc):
+    """nodal34 (degree-4 Lagrange basis in homogeneous monomials): for random
+    points, sum_i w_i psi_k(x_i) from the oracle's basis equals
+    nodal34 @ (sum_i w_i lam_i^alpha)"""
+    import re
+    import sys
+    sys.path.insert(0, CSRC)
+    import gen_moments as G
+    txt = open(os.path.join(CSRC, "moments_table.h")).read()
+    body = txt[txt.index("nodal34[35][35] = {"):]
+    body = body[:body.index("};")]
+    rows = re.findall(r"\{([^{}]*)\}", body)
+    A = np.array([[float(v) for v in r.split(",")] for r in rows])
+    assert A.shape == (35, 35)
+    mons = G.monomials(3, 4)
+    rng = np.random.default_rng(7)
+    pts = rng.random((20, 3))
+    pts = pts / (pts.sum(1, keepdims=True) + rng.random((20, 1)))
+    w = rng.random(20)
+    m = np.zeros(35)
+    W = np.zeros(35)
+    for x, wi in zip(pts, w):
+        lam = np.array([x[0], x[1], x[2], 1 - x.sum()])
+        m += wi * np.array([np.prod(lam ** np.array(a)) for a in mons])
+        phi, _ = orc.h1_shape(3, 4, x)
+        W += wi * phi
+    assert np.abs(A @ m - W).max() < 1e-12 * np.abs(W).max()
