@@ -208,9 +208,10 @@ int xg_batch_destroy(xg_batch *b);
    blocks per CU, elements per work grab (per wave for the MFMA kernels, per
    lane for the moment form) */
 int xg_batch_tune(xg_batch *b, int p, int waves_per_block, int blocks_per_cu, int grab);
-/* element-matrix method of the batch: 0 = MFMA contraction (default),
-   XG_METHOD_MOMENTS = moment form (FP64 arithmetic, FP32 store) */
-int xg_batch_method(xg_batch *b, int method);
+/* element-matrix method of the degree-p kernel (p = 0: all degrees):
+   0 = MFMA contraction (default), XG_METHOD_MOMENTS = moment form (FP64
+   arithmetic, FP32 store) */
+int xg_batch_method(xg_batch *b, int p, int method);
 /* marking + plan only; then xg_batch_info */
 int xg_batch_plan(xg_ctx *ctx, xg_batch *b);
 /* synchronises; host outputs (nullable): cut elements per mesh (nmesh),

@@ -54,7 +54,7 @@ _SIGS = {
     "xg_batch_create": [_vp, _i, _i, _vp, _vp, _i, _vp, _i, _vp, _vp, _vp, ctypes.POINTER(_vp)],
     "xg_batch_destroy": [_vp],
     "xg_batch_tune": [_vp, _i, _i, _i, _i],
-    "xg_batch_method": [_vp, _i],
+    "xg_batch_method": [_vp, _i, _i],
     "xg_batch_plan": [_vp, _vp],
     "xg_batch_info": [_vp, _vp, _vp, _vp],
     "xg_batch_run": [_vp, _vp, _vp, ctypes.c_longlong, _i, _i],
@@ -227,8 +227,8 @@ class Batch:
     def tune(self, p, waves, blocks_per_cu, grab):
         check(lib().xg_batch_tune(self.h, p, waves, blocks_per_cu, grab))
 
-    def method(self, method):
-        check(lib().xg_batch_method(self.h, method))
+    def method(self, method, p=0):
+        check(lib().xg_batch_method(self.h, p, method))
 
     def plan(self):
         check(lib().xg_batch_plan(self.ctx.h, self.h))

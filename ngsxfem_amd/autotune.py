@@ -1,8 +1,9 @@
 """Autotuner for the batched persistent pipeline (xg_batch).
 
-For each H1 degree p the launch shape of the persistent degree-p kernel --
-waves per block, resident blocks per CU, cut elements per work grab -- is
-swept on the batched workload; every candidate is timed with HIP events on
+For each H1 degree p the element-matrix method (MFMA contraction or moment
+form) and the launch shape of the persistent degree-p kernel -- waves per
+block, resident blocks per CU, cut elements per work grab -- are swept on the
+batched workload; every candidate is timed with HIP events on
 the context's stream (median of `reps` eager runs of that kernel alone) and
 the fastest is written to tuning/<name>.json, which bench.py and
 ngsxfem_amd.batch read.
@@ -19,6 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_OUT = os.path.join(HERE, "tuning", "c3_gfx950.json")
 
 CANDIDATES = [(w, b, g) for w in (1, 2) for b in (1, 2, 4, 8) for g in (1, 4)]
+METHODS = {"mfma": 0, "moments": 16}
 
 
 def load(path=DEFAULT_OUT):
@@ -35,6 +37,7 @@ def apply(batch, tuning):
         t = tuning.get("p%d" % p)
         if t:
             batch.tune(p, t["waves"], t["blocks_per_cu"], t["grab"])
+            batch.method(METHODS[t.get("method", "mfma")], p)
 
 
 def tune(batch, out, ctx, reps=3, log=print):
@@ -45,7 +48,8 @@ def tune(batch, out, ctx, reps=3, log=print):
         if not np.any(batch.mesh_p == p):
             continue
         best = None
-        for (w, b, g) in CANDIDATES:
+        for mname, (w, b, g) in [(m, c) for m in METHODS for c in CANDIDATES]:
+            batch.method(METHODS[mname], p)
             batch.tune(p, w, b, g)
             times = []
             for _ in range(reps + 1):
@@ -59,10 +63,12 @@ def tune(batch, out, ctx, reps=3, log=print):
                 times.append(e0.elapsed_time(e1))
             t = float(np.median(times[1:]))
             if log:
-                log("p=%d waves=%d blocks/CU=%d grab=%d: %.3f ms" % (p, w, b, g, t))
+                log("p=%d %s waves=%d blocks/CU=%d grab=%d: %.3f ms" % (p, mname, w, b, g, t))
             if best is None or t < best[0]:
-                best = (t, w, b, g)
-        res["p%d" % p] = {"waves": best[1], "blocks_per_cu": best[2], "grab": best[3], "ms": best[0]}
+                best = (t, w, b, g, mname)
+        res["p%d" % p] = {"method": best[4], "waves": best[1], "blocks_per_cu": best[2], "grab": best[3],
+                          "ms": best[0]}
+        batch.method(METHODS[best[4]], p)
         batch.tune(p, best[1], best[2], best[3])
     return res
 

@@ -44,7 +44,7 @@ struct xg_batch {
   long long *d_outoff = nullptr;
   // tuning per degree p = 1..3
   int waves[4] = {0, 2, 2, 2}, bpc[4] = {0, 4, 4, 4}, grab[4] = {0, 1, 1, 1};
-  int method = 0;  // 0: MFMA contraction, XG_METHOD_MOMENTS: moment form
+  int method[4] = {0, 0, 0, 0};  // per p: 0 MFMA contraction, XG_METHOD_MOMENTS moment form
   int ncu = 256;
   // graph
   hipStream_t stream = nullptr;
@@ -293,7 +293,7 @@ int launch_batch(xg_batch *b, hipStream_t st, int32_t *err, int order, float *ou
 int launch_degree(xg_batch *b, int P, hipStream_t st, int32_t *err, float *out, long long cap) {
   const int order = 2 * P;
   const int W = b->waves[P];
-  if (b->method == XG_METHOD_MOMENTS) {
+  if (b->method[P] == XG_METHOD_MOMENTS) {
     if (P == 1) return W == 1 ? launch_batch_mom<1, 1>(b, st, err, order, out, cap) : launch_batch_mom<1, 2>(b, st, err, order, out, cap);
     if (P == 2) return W == 1 ? launch_batch_mom<2, 1>(b, st, err, order, out, cap) : launch_batch_mom<2, 2>(b, st, err, order, out, cap);
     return W == 1 ? launch_batch_mom<3, 1>(b, st, err, order, out, cap) : launch_batch_mom<3, 2>(b, st, err, order, out, cap);
@@ -424,10 +424,11 @@ extern "C" int xg_batch_tune(xg_batch *b, int p, int waves_per_block, int blocks
   return XG_OK;
 }
 
-extern "C" int xg_batch_method(xg_batch *b, int method) {
-  if (!b || (method != 0 && method != XG_METHOD_MOMENTS))
-    return set_error(XG_ERR_ARG, "xg_batch_method: 0 (MFMA) or XG_METHOD_MOMENTS");
-  b->method = method;
+extern "C" int xg_batch_method(xg_batch *b, int p, int method) {
+  if (!b || p < 0 || p > 3 || (method != 0 && method != XG_METHOD_MOMENTS))
+    return set_error(XG_ERR_ARG, "xg_batch_method: p in 0..3, method 0 (MFMA) or XG_METHOD_MOMENTS");
+  for (int q = 1; q <= 3; ++q)
+    if (p == 0 || p == q) b->method[q] = method;
   free_graph(b);
   return XG_OK;
 }

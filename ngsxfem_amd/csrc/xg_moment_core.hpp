@@ -63,8 +63,20 @@ __device__ __forceinline__ double piece_volume(int nv, const double (*W)[3]) {
 // point) mapped by x = ow W0 + sum_q xi_q W_{q+1}
 template <int D, int P, int NC>
 __device__ __forceinline__ void accum_piece(int roff, int nq, const double (*W)[3], double wsc, double *m) {
+  // the next point's rule entry is loaded (scalar loads, wave-uniform)
+  // before the current point is processed, so the load latency overlaps
+  // the FP64 work
+  double nx[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) nx[c] = xg_rule_data_d[roff][c];
   for (int k = 0; k < nq; ++k) {
-    const double *r = xg_rule_data_d[roff + k];
+    double r[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) r[c] = nx[c];
+    if (k + 1 < nq) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) nx[c] = xg_rule_data_d[roff + k + 1][c];
+    }
     double ow = 1.0;
 #pragma unroll
     for (int q = 0; q < NC; ++q) ow -= r[q];
@@ -143,14 +155,22 @@ __device__ __forceinline__ bool moment_elmat(const double *l, const double *Finv
         for (int d = 0; d < 3; ++d) W[k][d] = (k > 0 && d == k - 1) ? 1.0 : 0.0;
     } else {
 #pragma unroll
-      for (int k = 0; k <= D; ++k)
-        if (k < nvs) piece_vertex<D>(E->v[s][k], E, tcut, W[k]);
+      for (int k = 0; k <= D; ++k) {
+        if (k < nvs) {
+          piece_vertex<D>(E->v[s][k], E, tcut, W[k]);
+        } else {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) W[k][d] = 0.0;
+        }
+      }
       wsc = piece_volume(nvs, W) * ifac;
     }
-    if (dt == IF)
-      accum_piece<D, P, D - 1>(roff, nq, W, wsc, m);
-    else
-      accum_piece<D, P, D>(roff, nq, W, wsc, m);
+    // one code path for volume and interface pieces (a single instantiation
+    // keeps the register allocation of the two from adding up): the rule
+    // coordinates past the piece's dimension are exact zeros in the rule
+    // table and the unused vertices are zero, so the extra terms add exact
+    // zeros
+    accum_piece<D, P, D>(roff, nq, W, wsc, m);
   }
   double mk[MM::NK];
   MM::lower(m, mk);
