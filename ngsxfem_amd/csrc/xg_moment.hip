@@ -41,6 +41,8 @@ __global__ __launch_bounds__(256) void k_moment_pipeline(int nel, const int32_t 
                                                          int dts, int mask, T *__restrict__ out,
                                                          int32_t *err) {
   constexpr int N = ndof(D, P);
+  extern __shared__ char smem[];
+  const MomLds S = stage_lds<D>(smem, order);  // block barrier: before any exit
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const int ntiles = (nel + 63) >> 6;
@@ -54,7 +56,7 @@ __global__ __launch_bounds__(256) void k_moment_pipeline(int nel, const int32_t 
   ecore::load_geom<D>(el2v, coords, lset, e, V, l);
   double F[D * D], Finv[D * D], absdet;
   jacobian<D>(V, F, Finv, absdet);
-  if (mcore::moment_elmat<D, P, T>(l, Finv, absdet, dt, order, mask, out + ((size_t)i * ndt + t) * (size_t)(N * N)))
+  if (mcore::moment_elmat<D, P, T>(S, l, Finv, absdet, dt, mask, out + ((size_t)i * ndt + t) * (size_t)(N * N)))
     atomicOr(err, 1);
 }
 
@@ -114,7 +116,7 @@ int launch(xg_ctx *ctx, int order, int nel, const int32_t *elems, const int32_t 
   const long long waves = (long long)ndt * ((nel + 63) / 64);
   const long long blocks = (waves + 3) / 4;
   if (blocks > 0x7fffffffLL) return set_error(XG_ERR_ARG, "xg_cut_pipeline: too many elements");
-  k_moment_pipeline<D, P, T><<<(unsigned)blocks, 256, 0, ctx->stream>>>(nel, elems, el2v, coords, lset, order,
+  k_moment_pipeline<D, P, T><<<(unsigned)blocks, 256, mom_lds_bytes(D, order), ctx->stream>>>(nel, elems, el2v, coords, lset, order,
                                                                       ndt, dts, mask, (T *)out, ctx->d_err);
   XG_LAUNCH_CHECK();
   return XG_OK;

@@ -8,20 +8,40 @@
 namespace xg {
 namespace mcore {
 
+// a DecompEntry held in registers as six 32-bit words (byte layout of the
+// struct: ns, nv, nc, ok, v[3][4], ce[4][2])
+struct PackedEntry {
+  uint32_t w[6];
+  __device__ __forceinline__ int byte(int b) const {  // b: compile-time after unrolling
+    return (int)(int8_t)((w[b >> 2] >> (8 * (b & 3))) & 255u);
+  }
+  __device__ __forceinline__ int ns() const { return byte(0); }
+  __device__ __forceinline__ int nc() const { return byte(2); }
+  __device__ __forceinline__ int ok() const { return byte(3); }
+  __device__ __forceinline__ int v(int s, int k) const { return byte(4 + 4 * s + k); }
+  __device__ __forceinline__ int ce(int c, int j) const { return byte(16 + 2 * c + j); }
+};
+static_assert(sizeof(DecompEntry) == 24, "DecompEntry layout");
+
 // vertex `code` of a decomposition entry in reference coordinates (FP64,
 // same operation order as cut_point_array)
 template <int D>
-__device__ __forceinline__ void piece_vertex(int code, const DecompEntry *E, const double *t, double *p) {
+__device__ __forceinline__ void piece_vertex(int code, const PackedEntry &E, double t0, double t1, double t2,
+                                             double t3, double *p) {
   if (code <= D) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) p[d] = (d == code && code < D) ? 1.0 : 0.0;
     return;
   }
   const int c = code - (D + 1);
-  const int i = E->ce[c][0], j = E->ce[c][1];
-  double tc = t[0];
+  int i = E.ce(0, 0), j = E.ce(0, 1);
 #pragma unroll
-  for (int k = 1; k < 4; ++k) tc = (c == k) ? t[k] : tc;
+  for (int k = 1; k < 4; ++k) {
+    i = (c == k) ? E.ce(k, 0) : i;
+    j = (c == k) ? E.ce(k, 1) : j;
+  }
+  // explicit selects (an indexed array here ends up in scratch memory)
+  const double tc = (c == 0) ? t0 : ((c == 1) ? t1 : ((c == 2) ? t2 : t3));
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const double pi = (d == i && i < D) ? 1.0 : 0.0;
@@ -62,20 +82,21 @@ __device__ __forceinline__ double piece_volume(int nv, const double (*W)[3]) {
 // moments of one sub-simplex: base rule (NC reference coordinates per
 // point) mapped by x = ow W0 + sum_q xi_q W_{q+1}
 template <int D, int P, int NC>
-__device__ __forceinline__ void accum_piece(int roff, int nq, const double (*W)[3], double wsc, double *m) {
-  // the next point's rule entry is loaded (scalar loads, wave-uniform)
-  // before the current point is processed, so the load latency overlaps
-  // the FP64 work
+__device__ __forceinline__ void accum_piece(const double (*rule)[4], int nq, const double (*W)[3], double wsc,
+                                            double *m) {
+  // the rule sits in LDS (wave-uniform addresses: broadcast reads); the next
+  // point's entry is read before the current point is processed, so the
+  // read latency overlaps the FP64 work
   double nx[4];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) nx[c] = xg_rule_data_d[roff][c];
+  for (int c = 0; c < 4; ++c) nx[c] = rule[0][c];
   for (int k = 0; k < nq; ++k) {
     double r[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) r[c] = nx[c];
     if (k + 1 < nq) {
 #pragma unroll
-      for (int c = 0; c < 4; ++c) nx[c] = xg_rule_data_d[roff + k + 1][c];
+      for (int c = 0; c < 4; ++c) nx[c] = rule[k + 1][c];
     }
     double ow = 1.0;
 #pragma unroll
@@ -99,32 +120,83 @@ __device__ __forceinline__ void accum_piece(int roff, int nq, const double (*W)[
   }
 }
 
+// Block-shared staging in LDS: the decomposition table of dimension D (all
+// three domain types) and the base rules of dimensions D and D-1 for the
+// rule order.  Every thread of the block must call stage_lds (it ends with a
+// block barrier).  Layout: [3][16] DecompEntry, then (nq_vol + nq_if) x 4
+// doubles.
+struct MomLds {
+  const DecompEntry *tab;  // [3][16]
+  const double (*rule_vol)[4];
+  const double (*rule_if)[4];
+  int nq_vol, nq_if;
+};
+
+inline size_t mom_lds_bytes(int D, int order) {  // host
+  const int n1 = order / 2 + 1;
+  const size_t nv = (size_t)xg_rule_cnt[D][n1], ni = (size_t)xg_rule_cnt[D - 1][n1];
+  return ((sizeof(DecompEntry) * 48 + 15) & ~size_t(15)) + sizeof(double) * 4 * (nv + ni);
+}
+
+template <int D>
+__device__ __forceinline__ MomLds stage_lds(char *smem, int order) {
+  MomLds S;
+  DecompEntry *tab = reinterpret_cast<DecompEntry *>(smem);
+  double(*rv)[4] = reinterpret_cast<double(*)[4]>(smem + ((sizeof(DecompEntry) * 48 + 15) & ~size_t(15)));
+  S.nq_vol = rule_cnt(D, order);
+  S.nq_if = rule_cnt(D - 1, order);
+  double(*ri)[4] = rv + S.nq_vol;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int k = tid; k < 48; k += nt) tab[k] = xg_decomp_d.e[D - 1][k >> 4][k & 15];
+  const int ov = rule_off(D, order), oi = rule_off(D - 1, order);
+  for (int k = tid; k < 4 * (S.nq_vol + S.nq_if); k += nt) {
+    const int q = k >> 2, c = k & 3;
+    if (q < S.nq_vol) rv[q][c] = xg_rule_data_d[ov + q][c];
+    else ri[q - S.nq_vol][c] = xg_rule_data_d[oi + q - S.nq_vol][c];
+  }
+  __syncthreads();
+  S.tab = tab;
+  S.rule_vol = rv;
+  S.rule_if = ri;
+  return S;
+}
+
 // One (element, domain type): rule topology, moments, identities; writes the
 // ndof x ndof matrix to out.  Returns true where the reference throws
 // (straightcutrule.cpp:45-49, :148, :193).
 template <int D, int P, typename T>
-__device__ __forceinline__ bool moment_elmat(const double *l, const double *Finv, double absdet, int dt, int order,
-                                             int mask, T *out) {
+__device__ __forceinline__ bool moment_elmat(const MomLds &S, const double *l, const double *Finv, double absdet,
+                                             int dt, int mask, T *out) {
   using MM = Moments<D, P>;
   // topology of the (element, dt) rule
   const int nvs = (dt == IF) ? D : D + 1;  // vertices per sub-simplex (wave-uniform)
   const int eldom = check_if_straight_cut<D>(l);
-  const DecompEntry *E = nullptr;
+  PackedEntry E;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) E.w[q] = 0u;
+  bool cut = false;
   int ns = 0;
   bool bad = false;
-  double tcut[4] = {0.0, 0.0, 0.0, 0.0};
+  double tc0 = 0.0, tc1 = 0.0, tc2 = 0.0, tc3 = 0.0;
   if (eldom != IF) {
     ns = (eldom == dt) ? 1 : 0;  // uncut: the base rule on the element itself
   } else if (!cut_is_valid<D>(l)) {
     bad = true;
   } else {
-    E = &xg_decomp_d.e[D - 1][dt][sign_mask<D>(l)];
-    if (!E->ok) {
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(&S.tab[dt * 16 + sign_mask<D>(l)]);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) E.w[q] = src[q];
+    cut = true;
+    if (!E.ok()) {
       bad = true;
+      ns = 0;
     } else {
-      ns = E->ns;
-      for (int c = 0; c < E->nc; ++c) {
-        const int a = E->ce[c][0], b = E->ce[c][1];
+      ns = E.ns();
+      const int nc = E.nc();
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (c >= nc) break;
+        const int a = E.ce(c, 0), b = E.ce(c, 1);
         double la = l[0], lb = l[0];
 #pragma unroll
         for (int k = 1; k <= D; ++k) {
@@ -132,22 +204,26 @@ __device__ __forceinline__ bool moment_elmat(const double *l, const double *Finv
           lb = (k == b) ? l[k] : lb;
         }
         const double tc = la / (la - lb);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) tcut[k] = (c == k) ? tc : tcut[k];
+        tc0 = (c == 0) ? tc : tc0;
+        tc1 = (c == 1) ? tc : tc1;
+        tc2 = (c == 2) ? tc : tc2;
+        tc3 = (c == 3) ? tc : tc3;
       }
     }
   }
   const double ifac = (dt == IF && ns > 0) ? interface_factor<D>(l, Finv) : 1.0;
-  const int rd = (dt == IF) ? D - 1 : D;
-  const int nq = rule_cnt(rd, order), roff = rule_off(rd, order);
+  const int nq = (dt == IF) ? S.nq_if : S.nq_vol;
+  const double(*rule)[4] = (dt == IF) ? S.rule_if : S.rule_vol;
 
   double m[MM::NM];
 #pragma unroll
   for (int k = 0; k < MM::NM; ++k) m[k] = 0.0;
-  for (int s = 0; s < ns; ++s) {
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    if (s >= ns) break;
     double W[D + 1][3];
     double wsc = 1.0;
-    if (E == nullptr) {
+    if (!cut) {
       // uncut element: origin, e_0, .., e_{D-1} (identity map)
 #pragma unroll
       for (int k = 0; k <= D; ++k)
@@ -157,7 +233,7 @@ __device__ __forceinline__ bool moment_elmat(const double *l, const double *Finv
 #pragma unroll
       for (int k = 0; k <= D; ++k) {
         if (k < nvs) {
-          piece_vertex<D>(E->v[s][k], E, tcut, W[k]);
+          piece_vertex<D>(E.v(s, k), E, tc0, tc1, tc2, tc3, W[k]);
         } else {
 #pragma unroll
           for (int d = 0; d < 3; ++d) W[k][d] = 0.0;
@@ -170,7 +246,7 @@ __device__ __forceinline__ bool moment_elmat(const double *l, const double *Finv
     // coordinates past the piece's dimension are exact zeros in the rule
     // table and the unused vertices are zero, so the extra terms add exact
     // zeros
-    accum_piece<D, P, D>(roff, nq, W, wsc, m);
+    accum_piece<D, P, D>(rule, nq, W, wsc, m);
   }
   double mk[MM::NK];
   MM::lower(m, mk);
