@@ -60,6 +60,10 @@ struct SdLds {
   double *lat;      // sd_nlat(L)
   double *filltf;   // SD_CH
   double *red;      // 2 * 84
+  double (*rule_vol)[4], (*rule_if)[4];  // base rules of order `order`
+  double (*pool)[8][3];                  // 64 cut leaves x 8 points
+  double *negtf, *ifw;                   // 192 / 128 piece weight factors
+  uint32_t *negrec, *ifrec;              // 192 / 128 piece records
   uint32_t *nodeV;  // SD_CH * 4 packed vertices (x | y << 8 | z << 16)
   uint32_t *flags;  // SD_CH
   uint16_t *cur, *nxt;  // sd_maxcut(L)
@@ -69,7 +73,7 @@ struct SdLds {
 
 __host__ __device__ inline size_t sd_align(size_t x) { return (x + 15) & ~size_t(15); }
 
-__host__ __device__ inline size_t sd_bytes(int L, SdLds *s, char *base) {
+__host__ __device__ inline size_t sd_bytes(int L, int nq_vol, int nq_if, SdLds *s, char *base) {
   size_t o = 0;
   auto take = [&](size_t bytes) {
     const size_t at = o;
@@ -79,6 +83,13 @@ __host__ __device__ inline size_t sd_bytes(int L, SdLds *s, char *base) {
   const size_t a_lat = take(sizeof(double) * sd_nlat(L));
   const size_t a_ftf = take(sizeof(double) * SD_CH);
   const size_t a_red = take(sizeof(double) * 2 * 84);
+  const size_t a_rv = take(sizeof(double) * 4 * nq_vol);
+  const size_t a_ri = take(sizeof(double) * 4 * nq_if);
+  const size_t a_pool = take(sizeof(double) * 64 * 8 * 3);
+  const size_t a_ntf = take(sizeof(double) * 192);
+  const size_t a_ifw = take(sizeof(double) * 128);
+  const size_t a_nrec = take(sizeof(uint32_t) * 192);
+  const size_t a_irec = take(sizeof(uint32_t) * 128);
   const size_t a_nv = take(sizeof(uint32_t) * SD_CH * 4);
   const size_t a_fl = take(sizeof(uint32_t) * SD_CH);
   const size_t a_cur = take(sizeof(uint16_t) * sd_maxcut(L));
@@ -89,6 +100,13 @@ __host__ __device__ inline size_t sd_bytes(int L, SdLds *s, char *base) {
     s->lat = (double *)(base + a_lat);
     s->filltf = (double *)(base + a_ftf);
     s->red = (double *)(base + a_red);
+    s->rule_vol = (double(*)[4])(base + a_rv);
+    s->rule_if = (double(*)[4])(base + a_ri);
+    s->pool = (double(*)[8][3])(base + a_pool);
+    s->negtf = (double *)(base + a_ntf);
+    s->ifw = (double *)(base + a_ifw);
+    s->negrec = (uint32_t *)(base + a_nrec);
+    s->ifrec = (uint32_t *)(base + a_irec);
     s->nodeV = (uint32_t *)(base + a_nv);
     s->flags = (uint32_t *)(base + a_fl);
     s->cur = (uint16_t *)(base + a_cur);
@@ -180,8 +198,8 @@ struct Geo {
 // base-rule points of one tet mapped by x = ow p0 + sum xi_m p_{m+1}
 template <int PM>
 __device__ __forceinline__ void accum_tet(double *m, const double *p0, const double *p1, const double *p2,
-                                          const double *p3, double tf, int roff, int k) {
-  const double *rr = xg_rule_data_d[roff + k];
+                                          const double *p3, double tf, const double (*rule)[4], int k) {
+  const double *rr = rule[k];
   double ow = 1.0;
 #pragma unroll
   for (int q = 0; q < 3; ++q) ow -= rr[q];
@@ -209,97 +227,41 @@ __device__ __forceinline__ double tet_tf(const double *p0, const double *p1, con
   return measure6 ? (v / 6.0) * 6.0 : v;
 }
 
-// FillSimplexWithRule<3> on one lane (all base points)
-template <int PM>
-__device__ void fill_tet_lane(double *m, const double *p0, const double *p1, const double *p2, const double *p3,
-                              int order) {
-  const double tf = tet_tf(p0, p1, p2, p3, true);
-  const int nq = rule_cnt(3, order), off = rule_off(3, order);
-  for (int k = 0; k < nq; ++k) accum_tet<PM>(m, p0, p1, p2, p3, tf, off, k);
+// 4-bit fields (vertex -> cut index, 0xF = none)
+__device__ __forceinline__ int nib(uint32_t x, int v) { return (int)((x >> (4 * v)) & 15u); }
+__device__ __forceinline__ uint32_t set_nib(uint32_t x, int v, int val) {
+  return (x & ~(15u << (4 * v))) | ((uint32_t)val << (4 * v));
 }
-
-// FillSimplexCoDim1WithRule<3> on one lane
-template <int PM>
-__device__ void fill_tri_lane(double *m, const Geo &geo, const double *s0, const double *s1, const double *s2,
-                              const double *pospoint, int order) {
-  double a[3], b[3], c[3];
+// k-th set bit (ascending) of a 4-bit mask
+__device__ __forceinline__ int kth_bit(int mask, int k) {
+  int r = 0;
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    a[d] = s1[d] - s0[d];
-    b[d] = s2[d] - s0[d];
+  for (int j = 3; j >= 0; --j) {
+    const int below = __popc(mask & ((1 << j) - 1));
+    if (((mask >> j) & 1) && below == k) r = j;
   }
-  c[0] = a[1] * b[2] - a[2] * b[1];
-  c[1] = a[2] * b[0] - a[0] * b[2];
-  c[2] = a[0] * b[1] - a[1] * b[0];
-  const double tf = norm3(c);
-  const double na = norm3(a), nb = norm3(b);
-  const double rel = na > nb ? na : nb;
-  if (tf < 1e-14 * rel) return;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) c[d] /= tf;
-  double dd[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) dd[d] = pospoint[d] - s0[d];
-  if (dd[0] * c[0] + dd[1] * c[1] + dd[2] * c[2] >= 0)
-#pragma unroll
-    for (int d = 0; d < 3; ++d) c[d] *= -1.0;
-  const double f = geo.if_factor(c);
-  const int nq = rule_cnt(2, order), off = rule_off(2, order);
-  for (int k = 0; k < nq; ++k) {
-    const double *rr = xg_rule_data_d[off + k];
-    const double ow = 1.0 - rr[0] - rr[1];
-    double x[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) x[d] = ow * s0[d] + rr[0] * s1[d] + rr[1] * s2[d];
-    double lam[4];
-    lam[0] = x[0];
-    lam[1] = x[1];
-    lam[2] = x[2];
-    lam[3] = 1.0 - x[0] - x[1] - x[2];
-    Moments<3, PM>::accum(lam, rr[3] * tf * f, m);
-  }
+  return r;
 }
 
-__device__ __forceinline__ void sel8(const double (*pool)[3], int idx, double *out) {
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    double v = pool[0][d];
-#pragma unroll
-    for (int q = 1; q < 8; ++q) v = (idx == q) ? pool[q][d] : v;
-    out[d] = v;
-  }
+// The pieces of a cut leaf (CutSimplex<3>::MakeQuad, xintegration.cpp:
+// 1084-1388): the point pool (0..3 leaf vertices, 4..7 cut points in edge
+// order) and index lists of the NEG volume tets (3 bits per vertex) and of
+// the interface triangles (3 vertices + the first POS vertex, which orients
+// the normal).  Returns false where the reference throws.
+struct LeafPieces {
+  int nneg, nif;
+  uint32_t neg[3];  // i0 | i1 << 3 | i2 << 6 | i3 << 9
+  uint32_t itf[2];  // i0 | i1 << 3 | i2 << 6 | ipos << 9
+};
+
+__device__ __forceinline__ uint32_t pack4i(int a, int b, int c, int d) {
+  return (uint32_t)a | ((uint32_t)b << 3) | ((uint32_t)c << 6) | ((uint32_t)d << 9);
 }
 
-template <int PM>
-__device__ __forceinline__ void fill_tet_idx(double *m, const double (*pool)[3], int i0, int i1, int i2, int i3,
-                                             int order) {
-  double p0[3], p1[3], p2[3], p3[3];
-  sel8(pool, i0, p0);
-  sel8(pool, i1, p1);
-  sel8(pool, i2, p2);
-  sel8(pool, i3, p3);
-  fill_tet_lane<PM>(m, p0, p1, p2, p3, order);
-}
-
-template <int PM>
-__device__ __forceinline__ void fill_tri_idx(double *m, const Geo &geo, const double (*pool)[3], int i0, int i1,
-                                             int i2, int ipos, int order) {
-  double s0[3], s1[3], s2[3], pp[3];
-  sel8(pool, i0, s0);
-  sel8(pool, i1, s1);
-  sel8(pool, i2, s2);
-  sel8(pool, ipos, pp);
-  fill_tri_lane<PM>(m, geo, s0, s1, s2, pp, order);
-}
-
-// CutSimplex<3>::MakeQuad on one lane.  vv: level set at the 4 vertices.
-// Vertex pool: 0..3 leaf vertices, 4..7 cut points.  Only NEG volume pieces
-// and the interface are accumulated.
-template <int PM, int WHICH>
-__device__ bool cut_leaf_lane(double *mneg, double *mif, const Geo &geo, const double (*V)[3], const double *vv,
-                              int order) {
+__device__ bool leaf_topology(const double (*V)[3], const double *vv, double (*pool)[3], LeafPieces &P) {
   constexpr int edge[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
-  double pool[8][3];
+  P.nneg = 0;
+  P.nif = 0;
 #pragma unroll
   for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -307,31 +269,26 @@ __device__ bool cut_leaf_lane(double *mneg, double *mif, const Geo &geo, const d
       pool[j][d] = V[j][d];
       pool[4 + j][d] = 0.0;
     }
-  int posp[4] = {0, 0, 0, 0}, negp[4] = {0, 0, 0, 0};
-  int npos = 0, nneg = 0, ncut = 0;
-  int v2c1[4] = {-1, -1, -1, -1}, v2c2[4] = {-1, -1, -1, -1};
-  bool zero[4];
+  int posm = 0, negm = 0, zerom = 0;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    zero[j] = false;
-    if (vv[j] > 0) {
-      posp[npos++] = j;
-    } else if (vv[j] < 0) {
-      negp[nneg++] = j;
-    } else {
-      posp[npos++] = j;
-      zero[j] = true;
+    if (vv[j] > 0) posm |= 1 << j;
+    else if (vv[j] < 0) negm |= 1 << j;
+    else {
+      posm |= 1 << j;  // a zero vertex value counts as POS (xintegration.cpp:1159-1164)
+      zerom |= 1 << j;
     }
   }
-  // in this routine the vertex lists and the point lists coincide
-  const int nposv = npos, nnegv = nneg;
+  const int npos = __popc(posm), nneg = __popc(negm);
+  int ncut = 0;
+  uint32_t v2c1 = 0xFFFFu, v2c2 = 0xFFFFu;
 #pragma unroll
-  for (int j = 0; j < 6; ++j) {
-    const int lv = edge[j][0], rv = edge[j][1];
+  for (int e = 0; e < 6; ++e) {
+    const int lv = edge[e][0], rv = edge[e][1];
     const double vl = vv[lv], vr = vv[rv];
     bool hascut = (vl * vr < 0);
-    if (zero[lv] && vr < 0) hascut = true;
-    if (zero[rv] && vl < 0) hascut = true;
+    if (((zerom >> lv) & 1) && vr < 0) hascut = true;
+    if (((zerom >> rv) & 1) && vl < 0) hascut = true;
     if (hascut) {
       if (ncut >= 4) return false;
       const double cp = vl / (vl - vr);
@@ -339,80 +296,88 @@ __device__ bool cut_leaf_lane(double *mneg, double *mif, const Geo &geo, const d
 #pragma unroll
       for (int d = 0; d < 3; ++d) q[d] = (1 - cp) * V[lv][d] + cp * V[rv][d];
 #pragma unroll
-      for (int s = 0; s < 4; ++s)
-        if (s == ncut)
+      for (int t = 0; t < 4; ++t)
+        if (t == ncut)
 #pragma unroll
-          for (int d = 0; d < 3; ++d) pool[4 + s][d] = q[d];
-      if (v2c1[lv] == -1) v2c1[lv] = ncut; else v2c2[lv] = ncut;
-      if (v2c1[rv] == -1) v2c1[rv] = ncut; else v2c2[rv] = ncut;
+          for (int d = 0; d < 3; ++d) pool[4 + t][d] = q[d];
+      if (nib(v2c1, lv) == 15) v2c1 = set_nib(v2c1, lv, ncut); else v2c2 = set_nib(v2c2, lv, ncut);
+      if (nib(v2c1, rv) == 15) v2c1 = set_nib(v2c1, rv, ncut); else v2c2 = set_nib(v2c2, rv, ncut);
       ncut++;
     }
   }
   if (ncut == 3) {
     const bool minor_pos = nneg > npos;
     const int nmin = minor_pos ? npos : nneg;
-    const int nmaj = minor_pos ? nneg : npos;
-    if (nmin != 1 || nmaj != 3) return false;
-    const int *minorg = minor_pos ? posp : negp;
-    const int *majorg = minor_pos ? negp : posp;
-    // major vertices: the larger of the two vertex lists
-    const int *majv = (nnegv > nposv) ? negp : posp;
-    if ((WHICH & 1) && !minor_pos) fill_tet_idx<PM>(mneg, pool, minorg[0], 4, 5, 6, order);
-    if ((WHICH & 1) && minor_pos) {
-      const int g6[6] = {majorg[0], majorg[1], majorg[2], 4 + v2c1[majv[0]], 4 + v2c1[majv[1]],
-                         4 + v2c1[majv[2]]};
-      for (int i = 0; i < 3; ++i) fill_tet_idx<PM>(mneg, pool, g6[i], g6[i + 1], g6[i + 2], g6[i + 3], order);
+    if (nmin != 1 || 4 - nmin != 3) return false;
+    const int minm = minor_pos ? posm : negm, majm = minor_pos ? negm : posm;
+    if (!minor_pos) {
+      P.neg[0] = pack4i(kth_bit(minm, 0), 4, 5, 6);
+      P.nneg = 1;
+    } else {
+      const int m0 = kth_bit(majm, 0), m1 = kth_bit(majm, 1), m2 = kth_bit(majm, 2);
+      const int g3 = 4 + nib(v2c1, m0), g4 = 4 + nib(v2c1, m1), g5 = 4 + nib(v2c1, m2);
+      P.neg[0] = pack4i(m0, m1, m2, g3);
+      P.neg[1] = pack4i(m1, m2, g3, g4);
+      P.neg[2] = pack4i(m2, g3, g4, g5);
+      P.nneg = 3;
     }
-    if (WHICH & 2) fill_tri_idx<PM>(mif, geo, pool, 4, 5, 6, posp[0], order);
+    P.itf[0] = pack4i(4, 5, 6, kth_bit(posm, 0));
+    P.nif = 1;
     return true;
   }
   if (ncut == 4) {
-    if (nposv != 2 || nnegv != 2) return false;
-    if (WHICH & 1) {
-      // the NEG prism (side 1 of the reference's loop)
-      const int *own = negp, *ownv = negp, *othv = posp;
-      int pr[6];
-      pr[0] = own[0];
-      const int idxn = ownv[0];
-      const int c1 = v2c1[idxn], c2 = v2c2[idxn];
-      pr[1] = 4 + c1;
-      pr[2] = 4 + c2;
-      pr[3] = own[1];
-      int c3 = -1;
-      for (int l = 0; l < 4; ++l)
-        if (c1 != l && c2 != l) c3 = l;
-      int c4 = 6 - c3 - c2 - c1;
-      if ((v2c1[othv[0]] == c1 && v2c2[othv[0]] == c4) || (v2c1[othv[0]] == c2 && v2c2[othv[0]] == c3)) {
-        const int tt = c3;
-        c3 = c4;
-        c4 = tt;
-      }
-      pr[4] = 4 + c3;
-      pr[5] = 4 + c4;
-      for (int i = 0; i < 3; ++i) fill_tet_idx<PM>(mneg, pool, pr[i], pr[i + 1], pr[i + 2], pr[i + 3], order);
+    if (npos != 2 || nneg != 2) return false;
+    const int n0 = kth_bit(negm, 0), n1 = kth_bit(negm, 1), p0 = kth_bit(posm, 0), p1 = kth_bit(posm, 1);
+    // NEG prism (the reference's second side: own = NEG vertices)
+    const int c1 = nib(v2c1, n0), c2 = nib(v2c2, n0);
+    int c3 = -1;
+#pragma unroll
+    for (int l = 0; l < 4; ++l)
+      if (c1 != l && c2 != l) c3 = l;
+    int c4 = 6 - c3 - c2 - c1;
+    if ((nib(v2c1, p0) == c1 && nib(v2c2, p0) == c4) || (nib(v2c1, p0) == c2 && nib(v2c2, p0) == c3)) {
+      const int tt = c3;
+      c3 = c4;
+      c4 = tt;
     }
-    if (WHICH & 2) {
-      int diag1, diag2, ndiag1, ndiag2;
-      if (v2c1[negp[0]] == v2c1[posp[0]] || v2c2[negp[0]] == v2c1[posp[0]]) {
-        diag1 = v2c1[posp[0]];
-        ndiag1 = v2c2[posp[0]];
-      } else {
-        diag1 = v2c2[posp[0]];
-        ndiag1 = v2c1[posp[0]];
-      }
-      if (v2c1[negp[1]] == v2c1[posp[1]] || v2c2[negp[1]] == v2c1[posp[1]]) {
-        diag2 = v2c1[posp[1]];
-        ndiag2 = v2c2[posp[1]];
-      } else {
-        diag2 = v2c2[posp[1]];
-        ndiag2 = v2c1[posp[1]];
-      }
-      fill_tri_idx<PM>(mif, geo, pool, 4 + diag1, 4 + ndiag1, 4 + diag2, posp[0], order);
-      fill_tri_idx<PM>(mif, geo, pool, 4 + diag2, 4 + ndiag2, 4 + diag1, posp[0], order);
+    const int q1 = 4 + c1, q2 = 4 + c2, q4 = 4 + c3, q5 = 4 + c4;
+    P.neg[0] = pack4i(n0, q1, q2, n1);
+    P.neg[1] = pack4i(q1, q2, n1, q4);
+    P.neg[2] = pack4i(q2, n1, q4, q5);
+    P.nneg = 3;
+    int diag1, diag2, ndiag1, ndiag2;
+    if (nib(v2c1, n0) == nib(v2c1, p0) || nib(v2c2, n0) == nib(v2c1, p0)) {
+      diag1 = nib(v2c1, p0);
+      ndiag1 = nib(v2c2, p0);
+    } else {
+      diag1 = nib(v2c2, p0);
+      ndiag1 = nib(v2c1, p0);
     }
+    if (nib(v2c1, n1) == nib(v2c1, p1) || nib(v2c2, n1) == nib(v2c1, p1)) {
+      diag2 = nib(v2c1, p1);
+      ndiag2 = nib(v2c2, p1);
+    } else {
+      diag2 = nib(v2c2, p1);
+      ndiag2 = nib(v2c1, p1);
+    }
+    P.itf[0] = pack4i(4 + diag1, 4 + ndiag1, 4 + diag2, p0);
+    P.itf[1] = pack4i(4 + diag2, 4 + ndiag2, 4 + diag1, p0);
+    P.nif = 2;
     return true;
   }
   return false;
+}
+
+// exclusive prefix sum over the wave (and the total)
+__device__ __forceinline__ int wave_excl_scan(int x, int lane, int &total) {
+  int incl = x;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  total = __shfl(incl, 63, 64);
+  return incl - x;
 }
 
 // Simplex<3>::CheckIfCut with the relative 1e-14 threshold
@@ -459,12 +424,21 @@ __global__ __launch_bounds__(64) void k_subdiv(int nel, const double *__restrict
   constexpr int NM = Moments<3, PM>::NM;
   extern __shared__ char smem[];
   SdLds S;
-  sd_bytes(L, &S, smem);
+  const int vol_nq = rule_cnt(3, order), if_nq = rule_cnt(2, order);
+  sd_bytes(L, vol_nq, if_nq, &S, smem);
   const int lane = threadIdx.x & 63;
   const int n = 1 << L;
   const int nlat = sd_nlat(L);
   const int loff = xg_lat_off_d[L];
-  const int vol_nq = rule_cnt(3, order), vol_off = rule_off(3, order);
+  {
+    const int ov = rule_off(3, order), oi = rule_off(2, order);
+    for (int k = lane; k < 4 * (vol_nq + if_nq); k += 64) {
+      const int q = k >> 2, c = k & 3;
+      if (q < vol_nq) S.rule_vol[q][c] = xg_rule_data_d[ov + q][c];
+      else S.rule_if[q - vol_nq][c] = xg_rule_data_d[oi + q - vol_nq][c];
+    }
+    wave_sync();
+  }
   for (int e = blockIdx.x; e < nel; e += gridDim.x) {
     double cl[NL];
 #pragma unroll
@@ -596,22 +570,109 @@ __global__ __launch_bounds__(64) void k_subdiv(int nel, const double *__restrict
 #pragma unroll
               for (int d = 0; d < 3; ++d) V[j][d] = w[d] * inv_n;
             }
-            accum_tet<PM>(mneg, V[0], V[1], V[2], V[3], S.filltf[fi], vol_off, t);
+            accum_tet<PM>(mneg, V[0], V[1], V[2], V[3], S.filltf[fi], S.rule_vol, t);
           }
         }
-        // cut leaves: one per lane
-        for (int l = lane; l < nleaf; l += 64) {
-          const int c = S.leafq[l];
-          double V[4][3], vv[4];
+        // cut leaves, 64 at a time: topology one lane per leaf (point pool
+        // and piece lists into LDS), then the pieces' base-rule points
+        // lane-parallel over (piece, point) pairs
+        for (int l0 = 0; l0 < nleaf; l0 += 64) {
+          const int l = l0 + lane;
+          LeafPieces PC;
+          PC.nneg = 0;
+          PC.nif = 0;
+          if (l < nleaf) {
+            const int c = S.leafq[l];
+            double V[4][3], vv[4], pool[8][3];
 #pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            int w[3];
-            unpack3(S.nodeV[c * 4 + j], w);
+            for (int j = 0; j < 4; ++j) {
+              int w[3];
+              unpack3(S.nodeV[c * 4 + j], w);
 #pragma unroll
-            for (int d = 0; d < 3; ++d) V[j][d] = w[d] * inv_n;
-            vv[j] = S.lat[lat_index(w[0], w[1], w[2], n)];
+              for (int d = 0; d < 3; ++d) V[j][d] = w[d] * inv_n;
+              vv[j] = S.lat[lat_index(w[0], w[1], w[2], n)];
+            }
+            if (!leaf_topology(V, vv, pool, PC)) {
+              ok = false;
+              PC.nneg = 0;
+              PC.nif = 0;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+#pragma unroll
+              for (int d = 0; d < 3; ++d) S.pool[lane][j][d] = pool[j][d];
           }
-          ok = cut_leaf_lane<PM, WHICH>(mneg, mif, geo, V, vv, order) && ok;
+          if (!(WHICH & 1)) PC.nneg = 0;
+          if (!(WHICH & 2)) PC.nif = 0;
+          int tot_neg, tot_if;
+          const int bneg = wave_excl_scan(PC.nneg, lane, tot_neg);
+          const int bif = wave_excl_scan(PC.nif, lane, tot_if);
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+            if (k < PC.nneg) S.negrec[bneg + k] = (uint32_t)lane | (PC.neg[k] << 8);
+#pragma unroll
+          for (int k = 0; k < 2; ++k)
+            if (k < PC.nif) S.ifrec[bif + k] = (uint32_t)lane | (PC.itf[k] << 8);
+          wave_sync();
+          // weight factors per piece (FillSimplexWithRule: (|det|/6)*6;
+          // FillSimplexCoDim1WithRule: |cross| and the interface transform)
+          for (int k = lane; k < tot_neg; k += 64) {
+            const uint32_t r = S.negrec[k];
+            const int sl = r & 63;
+            const double(*pl)[3] = S.pool[sl];
+            S.negtf[k] = tet_tf(pl[(r >> 8) & 7], pl[(r >> 11) & 7], pl[(r >> 14) & 7], pl[(r >> 17) & 7], true);
+          }
+          for (int k = lane; k < tot_if; k += 64) {
+            const uint32_t r = S.ifrec[k];
+            const double(*pl)[3] = S.pool[r & 63];
+            const double *s0 = pl[(r >> 8) & 7], *s1 = pl[(r >> 11) & 7], *s2 = pl[(r >> 14) & 7];
+            const double *pp = pl[(r >> 17) & 7];
+            double a[3], b[3], cc[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              a[d] = s1[d] - s0[d];
+              b[d] = s2[d] - s0[d];
+            }
+            cc[0] = a[1] * b[2] - a[2] * b[1];
+            cc[1] = a[2] * b[0] - a[0] * b[2];
+            cc[2] = a[0] * b[1] - a[1] * b[0];
+            const double tf = norm3(cc);
+            const double na = norm3(a), nb = norm3(b);
+            const double rel = na > nb ? na : nb;
+            double wf = 0.0;
+            if (!(tf < 1e-14 * rel)) {
+#pragma unroll
+              for (int d = 0; d < 3; ++d) cc[d] /= tf;
+              const double dd0 = pp[0] - s0[0], dd1 = pp[1] - s0[1], dd2 = pp[2] - s0[2];
+              if (dd0 * cc[0] + dd1 * cc[1] + dd2 * cc[2] >= 0)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) cc[d] *= -1.0;
+              wf = tf * geo.if_factor(cc);
+            }
+            S.ifw[k] = wf;
+          }
+          wave_sync();
+          for (int q = lane; q < tot_neg * vol_nq; q += 64) {
+            const int k = q / vol_nq, t = q - k * vol_nq;
+            const uint32_t r = S.negrec[k];
+            const double(*pl)[3] = S.pool[r & 63];
+            accum_tet<PM>(mneg, pl[(r >> 8) & 7], pl[(r >> 11) & 7], pl[(r >> 14) & 7], pl[(r >> 17) & 7],
+                          S.negtf[k], S.rule_vol, t);
+          }
+          for (int q = lane; q < tot_if * if_nq; q += 64) {
+            const int k = q / if_nq, t = q - k * if_nq;
+            const uint32_t r = S.ifrec[k];
+            const double(*pl)[3] = S.pool[r & 63];
+            const double *s0 = pl[(r >> 8) & 7], *s1 = pl[(r >> 11) & 7], *s2 = pl[(r >> 14) & 7];
+            const double *rr = S.rule_if[t];
+            const double ow = 1.0 - rr[0] - rr[1];
+            double lam[4];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) lam[d] = ow * s0[d] + rr[0] * s1[d] + rr[1] * s2[d];
+            lam[3] = 1.0 - lam[0] - lam[1] - lam[2];
+            Moments<3, PM>::accum(lam, rr[3] * S.ifw[k], mif);
+          }
+          wave_sync();
         }
         wave_sync();
       }
@@ -664,7 +725,7 @@ __global__ __launch_bounds__(64) void k_subdiv(int nel, const double *__restrict
 template <int LO, int PM, int WHICH, int OUT>
 int launch_subdiv(xg_ctx *ctx, int nel, const double *lset_nodal, int L, int order, const double *finv,
                   double *out_neg, double *out_if, uint8_t *dom) {
-  const size_t lds = sd_bytes(L, nullptr, nullptr);
+  const size_t lds = sd_bytes(L, xg_rule_cnt[3][order / 2 + 1], xg_rule_cnt[2][order / 2 + 1], nullptr, nullptr);
   if (lds > 65536)
     XG_CHECK(hipFuncSetAttribute((const void *)k_subdiv<LO, PM, WHICH, OUT>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
