@@ -31,14 +31,17 @@ PEAK_FP64_VALU = 78.6e12
 
 
 def fp64_ops_moments(npts, n_cut, D=3, p=2):
-    """FP64 flops of the moment path (FMA = 2): per point the mapping
-    (3 ow + 3 mul + 9 FMA + 3 sub) and the moment update (10 products,
-    10 weight products, 35 FMA); per (element, domain type) the degree-2
-    lowering (90 adds) and the identities (934 coefficient terms, one FMA
-    each)."""
+    """FP64 flops of the moment path (FMA = 2).  The order-4 rule counts npts
+    (27-point tets, 9-point triangles) give the numbers of pieces; the moment
+    form evaluates each piece with the minimal exact rule of degree 4 (14 /
+    6 points).  Per point the mapping (3 ow + 3 mul + 9 FMA + 3 sub) and the
+    moment update (10 products, 10 weight products, 35 FMA); per (element,
+    domain type) the degree-2 lowering (90 adds) and the identities (934
+    coefficient terms, one FMA each)."""
+    pts = npts[0] // 27 * 14 + npts[2] // 9 * 6
     per_pt = 3 + 3 + 2 * 9 + 3 + 10 + 10 + 2 * 35
     per_el = 90 + 2 * 934
-    return sum(npts.values()) * per_pt + 2 * n_cut * per_el
+    return pts * per_pt + 2 * n_cut * per_el
 
 
 def parse():

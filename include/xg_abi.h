@@ -145,6 +145,26 @@ int xg_cut_pipeline(xg_ctx *ctx, int D, int p, int order, int nel, const int32_t
                     const int32_t *el2v, const double *coords, const double *lset_v,
                     int ndt, const int *dts, int mask, int precision, void *out);
 
+/* ---- linear forms: SymbolicCutLinearFormIntegrator::T_CalcElementVector
+   (xfem/symboliccutlfi.cpp:57-140) for the source term f = 1 on the straight
+   cut rule, moment form (FP64 arithmetic): out[i][t][a] = sum_q w_q |det F|
+   phi_a(x_q).  out: nel x ndt x ndof (FP32 / FP64 by precision). */
+int xg_cut_elvec(xg_ctx *ctx, int D, int p, int order, int nel, const int32_t *elems, const int32_t *el2v,
+                 const double *coords, const double *lset_v, int ndt, const int *dts, int precision,
+                 void *out);
+
+/* ---- global assembly: the scatter of NGSolve's BilinearForm::Assemble /
+   LinearForm::Assemble (the callers of CalcElementMatrix / CalcElementVector)
+   on the device.  dofs: nel x ndof global dof numbers (< 0: skipped); CSR
+   pattern row_ptr / col_idx (columns sorted per row) must contain every
+   (dofs[e][a], dofs[e][b]); values / out: FP64, accumulated with atomic adds
+   (summation order not fixed: results agree to rounding, not bitwise).  A
+   missing pattern entry is reported by xg_sync (XG_ERR_CAPACITY). */
+int xg_assemble_csr(xg_ctx *ctx, int nel, int ndof, const int32_t *dofs, const void *elmat, int precision,
+                    const int32_t *row_ptr, const int32_t *col_idx, double *values);
+int xg_assemble_vec(xg_ctx *ctx, int nel, int ndof, const int32_t *dofs, const void *elvec, int precision,
+                    double *out);
+
 /* ---- a17: subdivision rule for non-P1 level sets ----------------------
    (cutint/xintegration.cpp:1626-1754 CutIntegrationRule with subdivlvl = L,
    NumericalIntegrationStrategy :617-1063, CutSimplex<3>::MakeQuad :1084-1388)

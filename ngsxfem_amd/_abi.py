@@ -46,6 +46,9 @@ _SIGS = {
     "xg_subdiv_weights": [_vp, _i, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp],
     "xg_subdiv_moments": [_vp, _i, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp],
     "xg_elmat_moments": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _vp],
+    "xg_cut_elvec": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp],
+    "xg_assemble_csr": [_vp, _i, _i, _vp, _vp, _i, _vp, _vp, _vp],
+    "xg_assemble_vec": [_vp, _i, _i, _vp, _vp, _i, _vp],
     "xg_elmat_weights": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "xg_mlp_nparams": [_i, _i, _i],
     "xg_mlp_forward": [_vp, _i, _i, _i, _vp, _i, _vp, _vp, _vp, _i],
@@ -190,6 +193,21 @@ class Context:
         nel = lset_nodal.shape[0]
         check(lib().xg_subdiv_moments(self.h, nel, ptr(lset_nodal), lset_order, L, order, p, ptr(finv),
                                       ptr(out_neg), ptr(out_if), ptr(dom)))
+
+    def cut_elvec(self, D, p, order, el2v, coords, lset, dts, precision, out, elems=None):
+        nel = elems.shape[0] if elems is not None else el2v.shape[0]
+        arr = (ctypes.c_int * len(dts))(*dts)
+        check(lib().xg_cut_elvec(self.h, D, p, order, nel, ptr(elems), ptr(el2v), ptr(coords), ptr(lset),
+                                 len(dts), ctypes.cast(arr, _vp), precision, ptr(out)))
+
+    def assemble_csr(self, dofs, elmat, precision, row_ptr, col_idx, values):
+        nel, ndof = dofs.shape
+        check(lib().xg_assemble_csr(self.h, nel, ndof, ptr(dofs), ptr(elmat), precision, ptr(row_ptr),
+                                    ptr(col_idx), ptr(values)))
+
+    def assemble_vec(self, dofs, elvec, precision, out):
+        nel, ndof = dofs.shape
+        check(lib().xg_assemble_vec(self.h, nel, ndof, ptr(dofs), ptr(elvec), precision, ptr(out)))
 
     def elmat_moments(self, D, p, moments, mask, precision, out, el2v=None, coords=None, elems=None, nel=None):
         if nel is None:

@@ -118,6 +118,41 @@ def basis(D, P):
     return phis
 
 
+def basis_affine(D, P):
+    """the same Lagrange basis with its constants kept (not homogenised); its
+    partial derivatives in the barycentrics are sparse (a vertex function
+    depends on its own barycentric only)"""
+    nv = D + 1
+    one = tuple([0] * nv)
+    phis = []
+    for a in lattice(D, P):
+        p = {one: Fraction(1)}
+        for k in range(nv):
+            for j in range(1, a[k] + 1):
+                lin = {}
+                e = [0] * nv
+                e[k] = 1
+                lin[tuple(e)] = Fraction(P, j)
+                if j > 1:
+                    lin[one] = Fraction(-(j - 1), j)
+                p = pmul(p, lin)
+        phis.append(p)
+    return phis
+
+
+def homogenise(p, D, n):
+    """multiply each term of degree d < n by S^(n-d), S = sum lam (= 1)"""
+    nv = D + 1
+    S1 = {tuple(1 if i == k else 0 for i in range(nv)): Fraction(1) for k in range(nv)}
+    out = {}
+    for a, c in p.items():
+        t = {a: c}
+        for _ in range(n - sum(a)):
+            t = pmul(t, S1)
+        out = padd(out, t)
+    return out
+
+
 def fmt(c):
     v = float(c)
     if v == int(v) and abs(v) < 1e15:
@@ -206,15 +241,18 @@ def gen(D, P):
         L.append("    mk[%d] = %s;" % (idxK[b], lin_expr(terms, "m")))
     L.append("  }")
     # ---- forms
-    # gradients G[a][d] (degree P-1)
-    G = []
-    for p in phis:
-        dD = pdiff(p, D)
-        G.append([padd(pdiff(p, d), dD, -1) for d in range(D)])
-    pairs = [(d, e) for d in range(D) for e in range(d, D)]
-    L.append("  // out[a*N+b] = sm * M_ab + sk * K_ab; Hs = H_de for d <= e (row-major upper)")
+    # stiffness through the barycentric gradients: grad phi = sum_i
+    # (d phi / d lam_i) grad lam_i for any extension of phi off the plane
+    # sum lam = 1 (the grad lam_i sum to zero); with the affine form of the
+    # basis the partials are sparse.  K_ab = sum_{i<=j} Gs_ij <...>, Gs_ij =
+    # grad lam_i . grad lam_j, the products homogenised to degree 2P-2.
+    aff = basis_affine(D, P)
+    dG = [[pdiff(p, i) for i in range(nv)] for p in aff]
+    gpairs = [(i, j) for i in range(nv) for j in range(i, nv)]
+    L.append("  // out[a*N+b] = sm * M_ab + sk * K_ab; Gs = grad lam_i . grad lam_j, i <= j (row-major upper)")
+    L.append("  static constexpr int NG = %d;" % len(gpairs))
     L.append("  template <typename T>")
-    L.append("  XG_DEV static void forms(const double *m, const double *mk, const double *Hs, double sm, double sk,")
+    L.append("  XG_DEV static void forms(const double *m, const double *mk, const double *Gs, double sm, double sk,")
     L.append("                           T *out) {")
     nterms = 0
     for a in range(N):
@@ -223,14 +261,17 @@ def gen(D, P):
             mterms = sorted((idx2[k], v) for k, v in mp.items())
             nterms += len(mterms)
             kparts = []
-            for pi, (d, e) in enumerate(pairs):
-                t = pmul(G[a][d], G[b][e])
-                if d != e:
-                    t = padd(t, pmul(G[a][e], G[b][d]))
+            for pi, (i, j) in enumerate(gpairs):
+                t = pmul(dG[a][i], dG[b][j])
+                if i != j:
+                    t = padd(t, pmul(dG[a][j], dG[b][i]))
+                if not t:
+                    continue
+                t = homogenise(t, D, 2 * P - 2)
                 kt = sorted((idxK[k], v) for k, v in t.items())
                 nterms += len(kt)
                 if kt:
-                    kparts.append("Hs[%d] * (%s)" % (pi, lin_expr(kt, "mk")))
+                    kparts.append("Gs[%d] * (%s)" % (pi, lin_expr(kt, "mk")))
             kexpr = " + ".join(kparts) if kparts else "0.0"
             L.append("    {")
             L.append("      const double M_ = %s;" % lin_expr(mterms, "m"))
@@ -240,6 +281,18 @@ def gen(D, P):
             if a != b:
                 L.append("      out[%d] = v_;" % (b * N + a))
             L.append("    }")
+    L.append("  }")
+    # ---- vec: F_a = sum_i w_i phi_a(x_i) = sum_alpha (phi_a S^P)_alpha m_alpha
+    S1 = {tuple(1 if i == k else 0 for i in range(nv)): Fraction(1) for k in range(nv)}
+    SP = {tuple([0] * nv): Fraction(1)}
+    for _ in range(P):
+        SP = pmul(SP, S1)
+    L.append("  // out[a] = s * sum_i w_i phi_a(x_i)  (source term of a linear form)")
+    L.append("  template <typename T>")
+    L.append("  XG_DEV static void vec(const double *m, double s, T *out) {")
+    for a in range(N):
+        t = sorted((idx2[k], v) for k, v in pmul(phis[a], SP).items())
+        L.append("    out[%d] = (T)(s * (%s));" % (a, lin_expr(t, "m")))
     L.append("  }")
     L.append("};  // %d coefficient terms" % nterms)
     return "\n".join(L)

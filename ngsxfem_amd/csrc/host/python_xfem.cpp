@@ -444,6 +444,91 @@ py::tuple CutElementMatrices(py::object mesh, py::object lset, int dt, int order
   return py::make_tuple(elarr, h);
 }
 
+// element list argument: None (all), BitArray or integer array
+std::vector<int32_t> element_list(py::object elements, int ne) {
+  std::vector<int32_t> el;
+  if (elements.is_none()) {
+    el.resize(ne);
+    for (int e = 0; e < ne; ++e) el[e] = e;
+  } else if (py::isinstance<BitArray>(elements)) {
+    const BitArray &b = elements.cast<const BitArray &>();
+    for (size_t e = 0; e < b.Size(); ++e)
+      if (b.Test(e)) el.push_back((int32_t)e);
+  } else {
+    auto a = py::array_t<int32_t, py::array::c_style | py::array::forcecast>(elements);
+    el.assign(a.data(), a.data() + a.size());
+  }
+  return el;
+}
+
+// SymbolicCutLinearFormIntegrator::T_CalcElementVector (xfem/symboliccutlfi.cpp:
+// 57-140) for f = 1 on the straight cut rule (moment form, FP64): returns
+// (elements, nel x ndof element vectors)
+py::tuple CutElementVectors(py::object mesh, py::object lset, int dt, int order, int p, py::object elements) {
+  MA ma = as_mesh(mesh);
+  bool is_p1 = true;
+  std::vector<double> v = p1_values(lset, ma->nv, is_p1);
+  if (!is_p1) throw std::runtime_error("CutElementVectors: P1 level sets only");
+  DevBuf<double> d_l;
+  d_l.upload(v.data(), v.size());
+  std::vector<int32_t> el = element_list(elements, ma->ne);
+  const int D = ma->dim;
+  int nd = 1;
+  for (int k = 1; k <= D; ++k) nd = nd * (p + k) / k;
+  const int nel = (int)el.size();
+  DevBuf<int32_t> d_el;
+  d_el.upload(el.data(), el.size());
+  const int dts[1] = {dt};
+  DevBuf<double> out((size_t)nel * nd);
+  if (nel)
+    check(xg_cut_elvec(ctx(), D, p, order, nel, d_el.p, ma->el2v.p, ma->coords.p, d_l.p, 1, dts, XG_PREC_FP64, out.p));
+  check(xg_sync(ctx()));
+  py::array_t<double> h({(py::ssize_t)nel, (py::ssize_t)nd});
+  out.download(h.mutable_data(), (size_t)nel * nd);
+  py::array_t<int32_t> elarr((py::ssize_t)nel, el.data());
+  return py::make_tuple(elarr, h);
+}
+
+// device scatter of element matrices into a CSR pattern (the assembly loop of
+// NGSolve's BilinearForm::Assemble); returns the FP64 values array
+py::array_t<double> AssembleCSR(py::array_t<int32_t, py::array::c_style | py::array::forcecast> dofs,
+                                py::array_t<double, py::array::c_style | py::array::forcecast> elmats,
+                                py::array_t<int32_t, py::array::c_style | py::array::forcecast> row_ptr,
+                                py::array_t<int32_t, py::array::c_style | py::array::forcecast> col_idx) {
+  if (dofs.ndim() != 2) throw std::runtime_error("AssembleCSR: dofs must be nel x ndof");
+  const int nel = (int)dofs.shape(0), nd = (int)dofs.shape(1);
+  if ((size_t)elmats.size() != (size_t)nel * nd * nd) throw std::runtime_error("AssembleCSR: elmats size");
+  const size_t nnz = (size_t)col_idx.size();
+  DevBuf<int32_t> d_dofs, d_rp, d_ci;
+  DevBuf<double> d_m, d_v(nnz ? nnz : 1);
+  d_dofs.upload(dofs.data(), dofs.size());
+  d_m.upload(elmats.data(), elmats.size());
+  d_rp.upload(row_ptr.data(), row_ptr.size());
+  d_ci.upload(col_idx.data(), col_idx.size());
+  if (nnz) hipcheck(hipMemset(d_v.p, 0, sizeof(double) * nnz), "hipMemset");
+  if (nel) check(xg_assemble_csr(ctx(), nel, nd, d_dofs.p, d_m.p, XG_PREC_FP64, d_rp.p, d_ci.p, d_v.p));
+  check(xg_sync(ctx()));
+  py::array_t<double> h((py::ssize_t)nnz);
+  d_v.download(h.mutable_data(), nnz);
+  return h;
+}
+
+py::array_t<double> AssembleVector(py::array_t<int32_t, py::array::c_style | py::array::forcecast> dofs,
+                                   py::array_t<double, py::array::c_style | py::array::forcecast> elvecs, int n) {
+  const int nel = (int)dofs.shape(0), nd = (int)dofs.shape(1);
+  if ((size_t)elvecs.size() != (size_t)nel * nd) throw std::runtime_error("AssembleVector: elvecs size");
+  DevBuf<int32_t> d_dofs;
+  DevBuf<double> d_e, d_o((size_t)(n ? n : 1));
+  d_dofs.upload(dofs.data(), dofs.size());
+  d_e.upload(elvecs.data(), elvecs.size());
+  if (n) hipcheck(hipMemset(d_o.p, 0, sizeof(double) * n), "hipMemset");
+  if (nel) check(xg_assemble_vec(ctx(), nel, nd, d_dofs.p, d_e.p, XG_PREC_FP64, d_o.p));
+  check(xg_sync(ctx()));
+  py::array_t<double> h((py::ssize_t)n);
+  d_o.download(h.mutable_data(), (size_t)n);
+  return h;
+}
+
 }  // namespace
 
 PYBIND11_MODULE(xfem, m) {
@@ -590,6 +675,13 @@ PYBIND11_MODULE(xfem, m) {
         },
         py::arg("values"), py::arg("eps_perturbation") = 1e-14);
 
+  m.def("CutElementVectors", &CutElementVectors, py::arg("mesh"), py::arg("levelset"), py::arg("domain_type"),
+        py::arg("order"), py::arg("p"), py::arg("elements") = py::none(),
+        "element vectors of the source term f = 1 on the straight cut rule (FP64)");
+  m.def("AssembleCSR", &AssembleCSR, py::arg("dofs"), py::arg("elmats"), py::arg("row_ptr"), py::arg("col_idx"),
+        "device scatter of element matrices into a CSR pattern (FP64 values)");
+  m.def("AssembleVector", &AssembleVector, py::arg("dofs"), py::arg("elvecs"), py::arg("n"),
+        "device scatter of element vectors");
   m.def("CutElementMatrices", &CutElementMatrices, py::arg("mesh"), py::arg("levelset"), py::arg("domain_type"),
         py::arg("order"), py::arg("p"), py::arg("mask") = 3, py::arg("elements") = py::none(),
         py::arg("precision") = XG_PREC_FP32);

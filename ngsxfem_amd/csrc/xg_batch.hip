@@ -220,7 +220,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_batch_mom(int order, int ndt, in
   __shared__ int s_vs[MAXMESH + 1];
   __shared__ int s_mesh[MAXMESH];
   extern __shared__ char smem[];
-  const mcore::MomLds S = mcore::stage_lds<D>(smem, order);
+  const mcore::MomLds S = mcore::stage_lds<D, P>(smem, order);
   const int lane = threadIdx.x & 63;
   const int nm = cls_nm[P - 1];
   for (int k = threadIdx.x; k < nm; k += blockDim.x) {
@@ -272,7 +272,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_batch_mom(int order, int ndt, in
 template <int P, int W>
 int launch_batch_mom(xg_batch *b, hipStream_t st, int32_t *err, int order, float *out, long long cap) {
   const int grid = b->bpc[P] * b->ncu;
-  k_batch_mom<P, W><<<grid, 64 * W, mcore::mom_lds_bytes(3, order), st>>>(order, b->ndt, b->dts, b->mask, b->grab[P], b->el2v, b->coords,
+  k_batch_mom<P, W><<<grid, 64 * W, mcore::mom_lds_bytes(3, P, order), st>>>(order, b->ndt, b->dts, b->mask, b->grab[P], b->el2v, b->coords,
                                              b->lset, b->d_cut, b->d_start, b->d_outoff, b->d_vstart,
                                              b->d_cls_mesh, b->d_cls_nm, b->d_cls_total, b->d_counter, b->nmesh,
                                              out, cap, err);

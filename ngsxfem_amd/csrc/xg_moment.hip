@@ -33,7 +33,7 @@ namespace {
 
 using namespace xg::mcore;
 
-template <int D, int P, typename T>
+template <int D, int P, typename T, int KIND>
 __global__ __launch_bounds__(256) void k_moment_pipeline(int nel, const int32_t *__restrict__ elems,
                                                          const int32_t *__restrict__ el2v,
                                                          const double *__restrict__ coords,
@@ -42,7 +42,7 @@ __global__ __launch_bounds__(256) void k_moment_pipeline(int nel, const int32_t 
                                                          int32_t *err) {
   constexpr int N = ndof(D, P);
   extern __shared__ char smem[];
-  const MomLds S = stage_lds<D>(smem, order);  // block barrier: before any exit
+  const MomLds S = stage_lds<D, P>(smem, order);  // block barrier: before any exit
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const int ntiles = (nel + 63) >> 6;
@@ -56,8 +56,12 @@ __global__ __launch_bounds__(256) void k_moment_pipeline(int nel, const int32_t 
   ecore::load_geom<D>(el2v, coords, lset, e, V, l);
   double F[D * D], Finv[D * D], absdet;
   jacobian<D>(V, F, Finv, absdet);
-  if (mcore::moment_elmat<D, P, T>(S, l, Finv, absdet, dt, mask, out + ((size_t)i * ndt + t) * (size_t)(N * N)))
-    atomicOr(err, 1);
+  bool bad;
+  if constexpr (KIND == 0)
+    bad = mcore::moment_elmat<D, P, T>(S, l, Finv, absdet, dt, mask, out + ((size_t)i * ndt + t) * (size_t)(N * N));
+  else
+    bad = mcore::moment_elvec<D, P, T>(S, l, Finv, absdet, dt, out + ((size_t)i * ndt + t) * (size_t)N);
+  if (bad) atomicOr(err, 1);
 }
 
 // element matrices from given moments (one lane per element): the rule is
@@ -86,15 +90,28 @@ __global__ __launch_bounds__(256) void k_elmat_from_moments(int nel, const int32
   for (int k = 0; k < MM::NM; ++k) m[k] = mom[(size_t)i * MM::NM + k];
   double mk[MM::NK];
   MM::lower(m, mk);
-  double Hs[D * (D + 1) / 2];
+  // Gram matrix of the physical barycentric gradients: grad lam_i = row i
+  // of F^{-1} (i < D), grad lam_D = -sum of the others
+  double Gr[D + 1][D];
+#pragma unroll
+  for (int r = 0; r < D; ++r) {
+    double sacc = 0.0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      Gr[i][r] = Finv[i * D + r];
+      sacc += Finv[i * D + r];
+    }
+    Gr[D][r] = -sacc;
+  }
+  double Hs[(D + 1) * (D + 2) / 2];
   int h = 0;
 #pragma unroll
-  for (int d = 0; d < D; ++d)
+  for (int i = 0; i <= D; ++i)
 #pragma unroll
-    for (int q = d; q < D; ++q) {
+    for (int j = i; j <= D; ++j) {
       double acc = 0.0;
 #pragma unroll
-      for (int r = 0; r < D; ++r) acc = fma(Finv[d * D + r], Finv[q * D + r], acc);
+      for (int r = 0; r < D; ++r) acc = fma(Gr[i][r], Gr[j][r], acc);
       Hs[h++] = acc;
     }
   const double sm = (mask & 1) ? absdet : 0.0, sk = (mask & 2) ? absdet : 0.0;
@@ -110,13 +127,13 @@ int launch_from_moments(xg_ctx *ctx, int nel, const int32_t *elems, const int32_
   return XG_OK;
 }
 
-template <int D, int P, typename T>
+template <int D, int P, typename T, int KIND = 0>
 int launch(xg_ctx *ctx, int order, int nel, const int32_t *elems, const int32_t *el2v, const double *coords,
            const double *lset, int ndt, int dts, int mask, void *out) {
   const long long waves = (long long)ndt * ((nel + 63) / 64);
   const long long blocks = (waves + 3) / 4;
   if (blocks > 0x7fffffffLL) return set_error(XG_ERR_ARG, "xg_cut_pipeline: too many elements");
-  k_moment_pipeline<D, P, T><<<(unsigned)blocks, 256, mom_lds_bytes(D, order), ctx->stream>>>(nel, elems, el2v, coords, lset, order,
+  k_moment_pipeline<D, P, T, KIND><<<(unsigned)blocks, 256, mom_lds_bytes(D, P, order), ctx->stream>>>(nel, elems, el2v, coords, lset, order,
                                                                       ndt, dts, mask, (T *)out, ctx->d_err);
   XG_LAUNCH_CHECK();
   return XG_OK;
@@ -140,6 +157,19 @@ int launch_moment_pipeline(xg_ctx *ctx, int D, int p, int order, int nel, const 
   return set_error(XG_ERR_ARG, "xg_cut_pipeline: bad D or p");
 }
 
+int launch_moment_elvec(xg_ctx *ctx, int D, int p, int order, int nel, const int32_t *elems, const int32_t *el2v,
+                        const double *coords, const double *lset, int ndt, int dts, bool fp64_out, void *out) {
+#define XG_VCALL(DD, PP)                                                                                          \
+  if (D == DD && p == PP)                                                                                         \
+    return fp64_out ? launch<DD, PP, double, 1>(ctx, order, nel, elems, el2v, coords, lset, ndt, dts, 1, out)    \
+                    : launch<DD, PP, float, 1>(ctx, order, nel, elems, el2v, coords, lset, ndt, dts, 1, out);
+  XG_VCALL(1, 1) XG_VCALL(1, 2) XG_VCALL(1, 3)
+  XG_VCALL(2, 1) XG_VCALL(2, 2) XG_VCALL(2, 3)
+  XG_VCALL(3, 1) XG_VCALL(3, 2) XG_VCALL(3, 3)
+#undef XG_VCALL
+  return set_error(XG_ERR_ARG, "xg_cut_elvec: bad D or p");
+}
+
 }  // namespace xg
 
 extern "C" int xg_elmat_moments(xg_ctx *ctx, int D, int p, int nel, const int32_t *elems, const int32_t *el2v,
@@ -160,4 +190,99 @@ extern "C" int xg_elmat_moments(xg_ctx *ctx, int D, int p, int nel, const int32_
   XG_ECALL(3, 1) XG_ECALL(3, 2) XG_ECALL(3, 3)
 #undef XG_ECALL
   return set_error(XG_ERR_ARG, "xg_elmat_moments: bad D or p");
+}
+
+extern "C" int xg_cut_elvec(xg_ctx *ctx, int D, int p, int order, int nel, const int32_t *elems, const int32_t *el2v,
+                            const double *coords, const double *lset_v, int ndt, const int *dts, int precision,
+                            void *out) {
+  if (!ctx || D < 1 || D > 3 || p < 1 || p > 3 || nel < 0 || ndt < 1 || ndt > 3 || !dts || order < 0 ||
+      order / 2 + 1 > XG_RULE_NMAX)
+    return set_error(XG_ERR_ARG, "xg_cut_elvec: bad argument");
+  if (precision != XG_PREC_FP32 && precision != XG_PREC_FP64)
+    return set_error(XG_ERR_UNSUPPORTED, "xg_cut_elvec: FP32 or FP64 output");
+  int packed = 0;
+  for (int t = 0; t < ndt; ++t) {
+    if (dts[t] < 0 || dts[t] > 2) return set_error(XG_ERR_ARG, "xg_cut_elvec: bad domain type");
+    packed |= dts[t] << (2 * t);
+  }
+  if (nel == 0) return XG_OK;
+  if (!el2v || !coords || !lset_v || !out) return set_error(XG_ERR_ARG, "xg_cut_elvec: null pointer");
+  return xg::launch_moment_elvec(ctx, D, p, order, nel, elems, el2v, coords, lset_v, ndt, packed,
+                                 precision == XG_PREC_FP64, out);
+}
+
+// ---- global assembly (NGSolve's BilinearForm::Assemble / LinearForm::Assemble
+// scatter): FP64 atomic adds into a CSR matrix / a vector
+namespace {
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_assemble_csr(long long nent, int ndof, const int32_t *__restrict__ dofs,
+                                                      const T *__restrict__ elmat,
+                                                      const int32_t *__restrict__ row_ptr,
+                                                      const int32_t *__restrict__ col_idx, double *values,
+                                                      int32_t *err) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nent) return;
+  const int nn = ndof * ndof;
+  const long long e = q / nn;
+  const int ab = (int)(q - e * nn), a = ab / ndof, b = ab - a * ndof;
+  const int r = dofs[e * ndof + a], c = dofs[e * ndof + b];
+  if (r < 0 || c < 0) return;  // dof not in the space (e.g. Dirichlet-free dofs)
+  int lo = row_ptr[r], hi = row_ptr[r + 1];
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (col_idx[mid] <= c) lo = mid;
+    else hi = mid;
+  }
+  if (lo >= row_ptr[r + 1] || col_idx[lo] != c) {
+    atomicOr(err, 4);  // entry missing from the sparsity pattern
+    return;
+  }
+  atomicAdd(&values[lo], (double)elmat[q]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_assemble_vec(long long nent, const int32_t *__restrict__ dofs,
+                                                      const T *__restrict__ elvec, double *out) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nent) return;
+  const int r = dofs[q];
+  if (r >= 0) atomicAdd(&out[r], (double)elvec[q]);
+}
+
+}  // namespace
+
+extern "C" int xg_assemble_csr(xg_ctx *ctx, int nel, int ndof, const int32_t *dofs, const void *elmat,
+                               int precision, const int32_t *row_ptr, const int32_t *col_idx, double *values) {
+  if (!ctx || nel < 0 || ndof < 1 || ndof > 64 || (precision != XG_PREC_FP32 && precision != XG_PREC_FP64))
+    return set_error(XG_ERR_ARG, "xg_assemble_csr: bad argument");
+  if (nel == 0) return XG_OK;
+  if (!dofs || !elmat || !row_ptr || !col_idx || !values) return set_error(XG_ERR_ARG, "xg_assemble_csr: null pointer");
+  const long long nent = (long long)nel * ndof * ndof;
+  const long long blocks = (nent + 255) / 256;
+  if (blocks > 0x7fffffffLL) return set_error(XG_ERR_ARG, "xg_assemble_csr: too many entries");
+  if (precision == XG_PREC_FP64)
+    k_assemble_csr<double><<<(unsigned)blocks, 256, 0, ctx->stream>>>(nent, ndof, dofs, (const double *)elmat, row_ptr,
+                                                                      col_idx, values, ctx->d_err);
+  else
+    k_assemble_csr<float><<<(unsigned)blocks, 256, 0, ctx->stream>>>(nent, ndof, dofs, (const float *)elmat, row_ptr,
+                                                                     col_idx, values, ctx->d_err);
+  XG_LAUNCH_CHECK();
+  return XG_OK;
+}
+
+extern "C" int xg_assemble_vec(xg_ctx *ctx, int nel, int ndof, const int32_t *dofs, const void *elvec, int precision,
+                               double *out) {
+  if (!ctx || nel < 0 || ndof < 1 || (precision != XG_PREC_FP32 && precision != XG_PREC_FP64))
+    return set_error(XG_ERR_ARG, "xg_assemble_vec: bad argument");
+  if (nel == 0) return XG_OK;
+  if (!dofs || !elvec || !out) return set_error(XG_ERR_ARG, "xg_assemble_vec: null pointer");
+  const long long nent = (long long)nel * ndof;
+  const long long blocks = (nent + 255) / 256;
+  if (precision == XG_PREC_FP64)
+    k_assemble_vec<double><<<(unsigned)blocks, 256, 0, ctx->stream>>>(nent, dofs, (const double *)elvec, out);
+  else
+    k_assemble_vec<float><<<(unsigned)blocks, 256, 0, ctx->stream>>>(nent, dofs, (const float *)elvec, out);
+  XG_LAUNCH_CHECK();
+  return XG_OK;
 }

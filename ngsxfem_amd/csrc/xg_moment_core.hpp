@@ -132,27 +132,42 @@ struct MomLds {
   int nq_vol, nq_if;
 };
 
-inline size_t mom_lds_bytes(int D, int order) {  // host
-  const int n1 = order / 2 + 1;
-  const size_t nv = (size_t)xg_rule_cnt[D][n1], ni = (size_t)xg_rule_cnt[D - 1][n1];
+// Rule used by the moment form: when the requested order is at least the
+// integrand degree 2P, every exact rule gives the same moments (the exact
+// integrals of lam^alpha over the piece), so the minimal symmetric rule of
+// degree 2P is used (tet 14 instead of 27 points for P = 2, triangle 6
+// instead of 9); below 2P the rule of the requested order itself.
+inline size_t mom_lds_bytes(int D, int P, int order) {  // host
+  size_t nv, ni;
+  if (order >= 2 * P) {
+    nv = (size_t)xg_minrule_cnt[D][P];
+    ni = (size_t)xg_minrule_cnt[D - 1][P];
+  } else {
+    const int n1 = order / 2 + 1;
+    nv = (size_t)xg_rule_cnt[D][n1];
+    ni = (size_t)xg_rule_cnt[D - 1][n1];
+  }
   return ((sizeof(DecompEntry) * 48 + 15) & ~size_t(15)) + sizeof(double) * 4 * (nv + ni);
 }
 
-template <int D>
+template <int D, int P>
 __device__ __forceinline__ MomLds stage_lds(char *smem, int order) {
   MomLds S;
   DecompEntry *tab = reinterpret_cast<DecompEntry *>(smem);
   double(*rv)[4] = reinterpret_cast<double(*)[4]>(smem + ((sizeof(DecompEntry) * 48 + 15) & ~size_t(15)));
-  S.nq_vol = rule_cnt(D, order);
-  S.nq_if = rule_cnt(D - 1, order);
+  const bool minimal = order >= 2 * P;
+  S.nq_vol = minimal ? xg_minrule_cnt_d[D][P] : rule_cnt(D, order);
+  S.nq_if = minimal ? xg_minrule_cnt_d[D - 1][P] : rule_cnt(D - 1, order);
+  const double(*src_v)[4] = minimal ? &xg_minrule_data_d[xg_minrule_off_d[D][P]] : &xg_rule_data_d[rule_off(D, order)];
+  const double(*src_i)[4] =
+      minimal ? &xg_minrule_data_d[xg_minrule_off_d[D - 1][P]] : &xg_rule_data_d[rule_off(D - 1, order)];
   double(*ri)[4] = rv + S.nq_vol;
   const int tid = threadIdx.x, nt = blockDim.x;
   for (int k = tid; k < 48; k += nt) tab[k] = xg_decomp_d.e[D - 1][k >> 4][k & 15];
-  const int ov = rule_off(D, order), oi = rule_off(D - 1, order);
   for (int k = tid; k < 4 * (S.nq_vol + S.nq_if); k += nt) {
     const int q = k >> 2, c = k & 3;
-    if (q < S.nq_vol) rv[q][c] = xg_rule_data_d[ov + q][c];
-    else ri[q - S.nq_vol][c] = xg_rule_data_d[oi + q - S.nq_vol][c];
+    if (q < S.nq_vol) rv[q][c] = src_v[q][c];
+    else ri[q - S.nq_vol][c] = src_i[q - S.nq_vol][c];
   }
   __syncthreads();
   S.tab = tab;
@@ -161,12 +176,12 @@ __device__ __forceinline__ MomLds stage_lds(char *smem, int order) {
   return S;
 }
 
-// One (element, domain type): rule topology, moments, identities; writes the
-// ndof x ndof matrix to out.  Returns true where the reference throws
-// (straightcutrule.cpp:45-49, :148, :193).
-template <int D, int P, typename T>
-__device__ __forceinline__ bool moment_elmat(const MomLds &S, const double *l, const double *Finv, double absdet,
-                                             int dt, int mask, T *out) {
+// The degree-2P moments of the cut rule of one (element, domain type) (rule
+// topology, sub-simplex volumes, interface factor, point loop).  Returns true
+// where the reference throws (straightcutrule.cpp:45-49, :148, :193).
+template <int D, int P>
+__device__ __forceinline__ bool moment_accumulate(const MomLds &S, const double *l, const double *Finv, int dt,
+                                                  double *m) {
   using MM = Moments<D, P>;
   // topology of the (element, dt) rule
   const int nvs = (dt == IF) ? D : D + 1;  // vertices per sub-simplex (wave-uniform)
@@ -215,7 +230,6 @@ __device__ __forceinline__ bool moment_elmat(const MomLds &S, const double *l, c
   const int nq = (dt == IF) ? S.nq_if : S.nq_vol;
   const double(*rule)[4] = (dt == IF) ? S.rule_if : S.rule_vol;
 
-  double m[MM::NM];
 #pragma unroll
   for (int k = 0; k < MM::NM; ++k) m[k] = 0.0;
 #pragma unroll
@@ -248,21 +262,58 @@ __device__ __forceinline__ bool moment_elmat(const MomLds &S, const double *l, c
     // zeros
     accum_piece<D, P, D>(rule, nq, W, wsc, m);
   }
+  return bad;
+}
+
+// One (element, domain type): rule topology, moments, identities; writes the
+// ndof x ndof matrix to out.  Returns true where the reference throws
+// (straightcutrule.cpp:45-49, :148, :193).
+template <int D, int P, typename T>
+__device__ __forceinline__ bool moment_elmat(const MomLds &S, const double *l, const double *Finv, double absdet,
+                                             int dt, int mask, T *out) {
+  using MM = Moments<D, P>;
+  double m[MM::NM];
+  const bool bad = moment_accumulate<D, P>(S, l, Finv, dt, m);
   double mk[MM::NK];
   MM::lower(m, mk);
-  double Hs[D * (D + 1) / 2];
+  // Gram matrix of the physical barycentric gradients: grad lam_i = row i
+  // of F^{-1} (i < D), grad lam_D = -sum of the others
+  double Gr[D + 1][D];
+#pragma unroll
+  for (int r = 0; r < D; ++r) {
+    double sacc = 0.0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      Gr[i][r] = Finv[i * D + r];
+      sacc += Finv[i * D + r];
+    }
+    Gr[D][r] = -sacc;
+  }
+  double Hs[(D + 1) * (D + 2) / 2];
   int h = 0;
 #pragma unroll
-  for (int d = 0; d < D; ++d)
+  for (int i = 0; i <= D; ++i)
 #pragma unroll
-    for (int q = d; q < D; ++q) {
+    for (int j = i; j <= D; ++j) {
       double acc = 0.0;
 #pragma unroll
-      for (int r = 0; r < D; ++r) acc = fma(Finv[d * D + r], Finv[q * D + r], acc);
+      for (int r = 0; r < D; ++r) acc = fma(Gr[i][r], Gr[j][r], acc);
       Hs[h++] = acc;
     }
   const double sm = (mask & 1) ? absdet : 0.0, sk = (mask & 2) ? absdet : 0.0;
   MM::template forms<T>(m, mk, Hs, sm, sk, out);
+  return bad;
+}
+
+// element vector of the source term f = 1 (SymbolicCutLinearFormIntegrator,
+// xfem/symboliccutlfi.cpp:57-140): out[a] = |det F| sum_i w_i phi_a(x_i)
+template <int D, int P, typename T>
+__device__ __forceinline__ bool moment_elvec(const MomLds &S, const double *l, const double *Finv, double absdet,
+                                             int dt, T *out) {
+  using MM = Moments<D, P>;
+  double m[MM::NM];
+  const bool bad = moment_accumulate<D, P>(S, l, Finv, dt, m);
+  MM::template vec<T>(m, absdet, out);
   return bad;
 }
 

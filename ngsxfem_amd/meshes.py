@@ -57,6 +57,23 @@ class Mesh:
     def nv(self):
         return self.coords.shape[0]
 
+    def dofs(self, p):
+        """global dof numbers of the degree-p Lagrange space, per element in
+        the element basis' dof order (vertices, then edges (0,1),(0,2),...,
+        lower local vertex first): (element dof table ne x ndof, ndof).
+        Vertex dofs come first, then one dof per mesh edge (p = 2)."""
+        if p == 1:
+            return self.el2v, self.nv
+        if p != 2:
+            raise NotImplementedError("global dof numbering for p = 1, 2")
+        D = self.dim
+        pairs = [(i, j) for i in range(D + 1) for j in range(i + 1, D + 1)]
+        ev = np.stack([np.sort(self.el2v[:, [i, j]], axis=1) for i, j in pairs], axis=1)  # ne x nE x 2
+        key = ev[..., 0].astype(np.int64) * self.nv + ev[..., 1]
+        uniq, inv = np.unique(key.ravel(), return_inverse=True)
+        edofs = self.nv + inv.reshape(key.shape)
+        return np.ascontiguousarray(np.hstack([self.el2v, edofs]).astype(np.int32)), self.nv + len(uniq)
+
     @property
     def ne(self):
         return self.el2v.shape[0]

@@ -24,7 +24,8 @@ import numpy as np
 
 from ..meshes import Make1DMesh, MakeStructured2DMesh, MakeStructured3DMesh, Mesh  # noqa: F401
 from .xfem import (ANY, BND, CDOM_IF, CDOM_NEG, CDOM_POS, FALLBACK, FIRST, HASNEG, HASPOS, IF, NEG, NO,  # noqa: F401
-                   OPTIMAL, POS, UNCUT, VOL, BitArray, COMBINED_DOMAIN_TYPE, CutElementMatrices, CutInfo,
+                   OPTIMAL, POS, UNCUT, VOL, AssembleCSR, AssembleVector, BitArray, COMBINED_DOMAIN_TYPE,
+                   CutElementMatrices, CutElementVectors, CutInfo,
                    DOMAIN_TYPE, GetFacetsWithNeighborTypes, IntegrateX, PerturbP1, QUAD_DIRECTION_POLICY, VorB)
 
 PREC_FP64, PREC_FP32 = 0, 1
@@ -198,9 +199,11 @@ class H1:
 
     @property
     def ndof(self):
-        if self.order == 1:
-            return self.mesh.nv
-        raise NotImplementedError("global numbering is built for P1 spaces in this build")
+        return self.mesh.dofs(self.order)[1]
+
+    def dofs(self):
+        """element dof table (ne x ndof_el) and the number of global dofs"""
+        return self.mesh.dofs(self.order)
 
     def TrialFunction(self):
         return _Proxy("u", self)
@@ -280,7 +283,30 @@ class _Proxy:
             if {self.kind, o.kind} != {"u", "v"} or self.deriv != o.deriv:
                 raise NotImplementedError("forms: u*v and grad(u)*grad(v) only")
             return _Form({("mass" if self.deriv == 0 else "laplace"): 1.0}, self.space)
+        if isinstance(o, (int, float)) and self.kind == "v" and self.deriv == 0:
+            return _LinForm(float(o), self.space)
         return NotImplemented
+
+    def __rmul__(self, o):
+        if isinstance(o, (int, float)) and self.kind == "v" and self.deriv == 0:
+            return _LinForm(float(o), self.space)
+        return NotImplemented
+
+
+class _LinForm:
+    """f * v with a constant f (SymbolicCutLinearFormIntegrator's source term)"""
+
+    def __init__(self, f, space):
+        self.f, self.space = f, space
+
+    def __mul__(self, o):
+        if isinstance(o, (_CutDifferentialSymbol, _Measure)):
+            return _SumOfIntegrals([_CutIntegral(self, o)])
+        if isinstance(o, (int, float)):
+            return _LinForm(self.f * o, self.space)
+        return NotImplemented
+
+    __rmul__ = lambda self, o: self.__mul__(o) if isinstance(o, (int, float)) else NotImplemented
 
 
 def grad(p):
@@ -332,7 +358,7 @@ class _CutDifferentialSymbol:
         self.definedonelements = definedonelements
 
     def __rmul__(self, o):
-        if isinstance(o, _Form):
+        if isinstance(o, (_Form, _LinForm)):
             return o.__mul__(self)
         if isinstance(o, (CoefficientFunction, int, float)):
             return _SumOfIntegrals([_CutIntegral(CoefficientFunction._lift(o), self)])
@@ -384,6 +410,11 @@ class _CutIntegral:
         c = coefs.get("mass", coefs.get("laplace", 1.0))
         return e, c * m
 
+    def element_vectors(self, mesh, p):
+        lset, dt, order, els = self._rule_args(mesh, p)
+        e, v = CutElementVectors(mesh, lset, dt, order, p, els)
+        return e, self.integrand.f * v
+
     def integrate(self, mesh, element_wise=False):
         lset, dt, order, els = self._rule_args(mesh, 0)
         lsd = {"levelset": lset, "domain_type": dt, "order": order if order is not None else 5}
@@ -409,9 +440,22 @@ def SymbolicCutBFI(levelset_domain, form, VOL_or_BND=VOL, element_boundary=False
         element_boundary=element_boundary, skeleton=skeleton))])
 
 
+def _csr_pattern(dofs, n):
+    """sorted CSR sparsity pattern of the element dof couplings (host)"""
+    nd = dofs.shape[1]
+    rows = np.repeat(dofs, nd, axis=1).ravel().astype(np.int64)
+    cols = np.tile(dofs, (1, nd)).ravel().astype(np.int64)
+    key = np.unique(rows * n + cols)
+    r, c = key // n, key % n
+    row_ptr = np.zeros(n + 1, np.int32)
+    np.add.at(row_ptr, r + 1, 1)
+    return np.cumsum(row_ptr).astype(np.int32), c.astype(np.int32)
+
+
 class BilinearForm:
     """Collects cut integrals; Assemble() computes the element matrices on the
-    GPU (xg_cut_pipeline) and scatters them into a CSR matrix for P1 spaces."""
+    GPU (xg_cut_pipeline) and scatters them on the GPU (xg_assemble_csr) into
+    a CSR matrix (scipy) of the P1 or P2 space."""
 
     def __init__(self, space, symmetric=False, **kwargs):
         self.space = space
@@ -424,6 +468,51 @@ class BilinearForm:
             self.integrals += o.parts
         else:
             raise TypeError("BilinearForm += expects cut integrals")
+        return self
+
+    def Assemble(self, precision=None):
+        from scipy.sparse import csr_matrix
+        prec = ngsxfemglobals.elmat_precision if precision is None else precision
+        mesh = self.space.mesh
+        p = self.space.order
+        self.elmats = [ci.element_matrices(mesh, p, prec) for ci in self.integrals]
+        if p in (1, 2):
+            dofs, n = self.space.dofs()
+            row_ptr, col_idx = _csr_pattern(dofs, n)
+            values = np.zeros(len(col_idx))
+            for els, mats in self.elmats:
+                values += AssembleCSR(np.ascontiguousarray(dofs[els]), np.asarray(mats, dtype=np.float64),
+                                      row_ptr, col_idx)
+            self.mat = csr_matrix((values, col_idx, row_ptr), shape=(n, n))
+        return self
+
+
+class LinearForm:
+    """Collects cut source terms f*v*dCut(...); Assemble() computes the element
+    vectors on the GPU (xg_cut_elvec, SymbolicCutLinearFormIntegrator) and
+    scatters them on the GPU (xg_assemble_vec)."""
+
+    def __init__(self, space, **kwargs):
+        self.space = space
+        self.integrals = []
+        self.vec = None
+
+    def __iadd__(self, o):
+        if isinstance(o, _SumOfIntegrals):
+            self.integrals += o.parts
+        else:
+            raise TypeError("LinearForm += expects cut integrals")
+        return self
+
+    def Assemble(self):
+        mesh = self.space.mesh
+        p = self.space.order
+        dofs, n = self.space.dofs()
+        out = np.zeros(n)
+        for ci in self.integrals:
+            els, vecs = ci.element_vectors(mesh, p)
+            out += AssembleVector(np.ascontiguousarray(dofs[els]), np.asarray(vecs, dtype=np.float64), n)
+        self.vec = out
         return self
 
     def Assemble(self, precision=None):

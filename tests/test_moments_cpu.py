@@ -40,12 +40,23 @@ extern "C" int mom_elmat(int D, int P, const double *V, const double *pts, const
     Finv[6] = C / det; Finv[7] = -(a * h - b * g) / det; Finv[8] = (a * e - b * d) / det;
   }
   const double ad = std::fabs(det);
-  double Hs[6];
+  // Gram matrix of the barycentric gradients (rows of F^{-1}, the last one
+  // minus their sum)
+  double G[4][3];
+  for (int r = 0; r < D; ++r) {
+    double s = 0;
+    for (int i = 0; i < D; ++i) {
+      G[i][r] = Finv[i * D + r];
+      s += Finv[i * D + r];
+    }
+    G[D][r] = -s;
+  }
+  double Hs[10];
   int k = 0;
-  for (int d = 0; d < D; ++d)
-    for (int e = d; e < D; ++e) {
+  for (int i = 0; i <= D; ++i)
+    for (int j = i; j <= D; ++j) {
       double s = 0;
-      for (int r = 0; r < D; ++r) s += Finv[d * D + r] * Finv[e * D + r];
+      for (int r = 0; r < D; ++r) s += G[i][r] * G[j][r];
       Hs[k++] = s;
     }
   double m[128] = {0}, mk[128] = {0};
