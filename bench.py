@@ -340,16 +340,16 @@ def bench_c3(args):
 
 def main():
     args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (every element is independent, so the path "
+              "shards by contiguous element ranges, one range per GPU); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
     if args.config == "c4":
         return bench_c4(args)
     if args.config == "c3":
         return bench_c3(args)
     if args.config == "c5":
         return bench_c5(args)
-    if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (the path shards by element "
-              "ranges; see DESIGN.md); run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
     import torch
 
     from ngsxfem_amd import _abi

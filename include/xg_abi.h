@@ -106,6 +106,37 @@ int xg_facets_with_neighbor_types(xg_ctx *ctx, int nf, const int32_t *f2el,
                                   const uint32_t *a, const uint32_t *b, int bnd_val_a,
                                   int bnd_val_b, int use_and, uint32_t *out);
 
+/* ---- node and dof marking (xfem/cutinfo.cpp:135-224, 540-647) -------- */
+/* Bit arrays have the BitArray layout above (bit i of word i/32).
+   out (nn bits) is cleared, then bit d is set for every node d of every item
+   whose bit is set in mark (mark == NULL: every item).  item2node: n x k
+   int32 node numbers; negative entries are skipped; an entry >= nn is
+   reported by xg_sync (XG_ERR_ARG).  Items = elements with their dof table:
+   GetDofsOfElements (cutinfo.cpp:598-622); items = facets with their dof
+   table: GetDofsOfFacets (cutinfo.cpp:624-647). */
+int xg_mark_from_items(xg_ctx *ctx, int n, int k, const int32_t *item2node, const uint32_t *mark,
+                       int nn, uint32_t *out);
+/* GetElementsWithNeighborFacets (cutinfo.cpp:540-564): f2el as in
+   xg_facets_with_neighbor_types; out: ne bits. */
+int xg_elements_with_neighbor_facets(xg_ctx *ctx, int nf, const int32_t *f2el,
+                                     const uint32_t *facet_mark, int ne, uint32_t *out);
+/* GetElementsWithSharedVertex (cutinfo.cpp:566-596): elements with a vertex
+   of a marked element (the marked elements included); out: ne bits.  Uses
+   ceil(nv/32) words of the context scratch. */
+int xg_elements_with_shared_vertex(xg_ctx *ctx, int D, int ne, const int32_t *el2v,
+                                   const uint32_t *elem_mark, int nv, uint32_t *out);
+/* cut_neighboring_node / dom_of_node of one node type (cutinfo.cpp:135-224)
+   from the element domains dt (uint8, NEG 0 / POS 1 / IF 2, as written by
+   xg_mark).  el2node: ne x k node numbers of the type (vertices, edges or
+   faces); cutnb (nullable, nn bits): the nodes of IF elements are OR-ed in
+   (not cleared: the reference clears these arrays only in the CutInformation
+   constructor, cutinfo.cpp:28-34); dom (nullable, uint8[nn]): NEG for nodes
+   of NEG elements, else POS.  el2node == NULL selects the element level
+   (nn == ne): cutnb |= the IF elements, dom = NEG where cut_ratio > 0.5,
+   else POS. */
+int xg_node_domains(xg_ctx *ctx, int ne, int k, const int32_t *el2node, const uint8_t *dt,
+                    const double *cut_ratio, int nn, uint32_t *cutnb, uint8_t *dom);
+
 /* ---- a8-a11: StraightCutIntegrationRule (cutint/straightcutrule.cpp:582-677) */
 /* Pass 1: counts.  elems (nullable = 0..nel-1).  offsets: int32[nel+1]
    (exclusive prefix sum, offsets[nel] = total points); nsub (nullable):
@@ -118,6 +149,30 @@ int xg_cut_rule_count(xg_ctx *ctx, int D, int nel, const int32_t *elems,
 int xg_cut_rule_fill(xg_ctx *ctx, int D, int nel, const int32_t *elems,
                      const int32_t *el2v, const double *coords, const double *lset_v,
                      int dt, int order, const int32_t *offsets, double *pts);
+
+/* ---- several level sets on simplices ------------------------------------
+   CreateMultiLevelsetCutIntegrationRule (cutint/mlsetintegration.cpp:8-158)
+   with StraightCutsIntegrationRule (cutint/straightcutrule.cpp:679-800).
+   lsets: HOST array of M DEVICE pointers to P1 vertex values (1 <= M <= 8);
+   dtts: HOST array of K x M domain types (1 <= K <= 64), the domain tuples
+   of the LevelsetIntegrationDomain.  Per element, the rules of the tuples it
+   is compatible with are concatenated in tuple order (uncut: the element's
+   base rule; cut: the element cut by each cutting level set in turn, the
+   base rule of the final sub-simplices, codimension 1 / 2 / 3 weights as the
+   reference).  Rule layout and weights as xg_cut_rule_count / _fill.  More
+   than 256 sub-simplices per element -> XG_ERR_CAPACITY at xg_sync. */
+int xg_mlset_rule_count(xg_ctx *ctx, int D, int nel, const int32_t *elems, const int32_t *el2v, int M,
+                        const double *const *lsets, int K, const int8_t *dtts, int order, int32_t *offsets);
+int xg_mlset_rule_fill(xg_ctx *ctx, int D, int nel, const int32_t *elems, const int32_t *el2v,
+                       const double *coords, int M, const double *const *lsets, int K, const int8_t *dtts,
+                       int order, const int32_t *offsets, double *pts);
+/* MultiLevelsetCutInformation (xfem/cutinfo.cpp:286-415): of_type
+   (nullable, ne bits): the element's tuple of CheckIfStraightCut values is
+   one of dtts (GetElementsOfType); contrib (nullable, ne bits): the order-0
+   rule of dtts has a positive weight sum (GetElementsWithContribution; uses
+   8 ne bytes of the context scratch). */
+int xg_mlset_mark(xg_ctx *ctx, int D, int ne, const int32_t *el2v, const double *coords, int M,
+                  const double *const *lsets, int K, const int8_t *dtts, uint32_t *of_type, uint32_t *contrib);
 
 /* ---- a16: IntegrateX (cutint/python_cutint.cpp:21-110) ----------------- */
 /* f = sum_t coef[t] x^expo[3t] y^expo[3t+1] z^expo[3t+2] (host arrays,

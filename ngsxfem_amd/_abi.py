@@ -38,6 +38,13 @@ _SIGS = {
     "xg_mark": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "xg_mark_p2": [_vp, _i, _vp, _i, _vp, _vp, _vp, _vp],
     "xg_facets_with_neighbor_types": [_vp, _i, _vp, _vp, _vp, _i, _i, _i, _vp],
+    "xg_mark_from_items": [_vp, _i, _i, _vp, _vp, _i, _vp],
+    "xg_mlset_rule_count": [_vp, _i, _i, _vp, _vp, _i, _vp, _i, _vp, _i, _vp],
+    "xg_mlset_rule_fill": [_vp, _i, _i, _vp, _vp, _vp, _i, _vp, _i, _vp, _i, _vp, _vp],
+    "xg_mlset_mark": [_vp, _i, _i, _vp, _vp, _i, _vp, _i, _vp, _vp, _vp],
+    "xg_elements_with_neighbor_facets": [_vp, _i, _vp, _vp, _i, _vp],
+    "xg_elements_with_shared_vertex": [_vp, _i, _i, _vp, _vp, _i, _vp],
+    "xg_node_domains": [_vp, _i, _i, _vp, _vp, _vp, _i, _vp, _vp],
     "xg_cut_rule_count": [_vp, _i, _i, _vp, _vp, _vp, _i, _i, _vp, _vp],
     "xg_cut_rule_fill": [_vp, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp],
     "xg_integrate": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp],
@@ -81,9 +88,22 @@ def _nmom(D, p):
     return comb(2 * p + D, D)
 
 
+def load_hip_runtime():
+    """Make the process use a single HIP runtime.  PyTorch-ROCm wheels bundle
+    their own libamdhip64; when libxg.so (linked against the system one) is
+    loaded first, two runtimes end up in the process and the second to
+    initialise sees no device.  Importing torch first lets libxg.so bind to
+    the runtime torch already loaded (same soname)."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     global _lib
     if _lib is None:
+        load_hip_runtime()
         if not os.path.exists(LIBPATH):
             raise ImportError("libxg.so not built (%s); run __graft_entry__.build() or "
                               "python -m ngsxfem_amd.build_ext" % LIBPATH)
@@ -157,6 +177,54 @@ class Context:
     def facets_with_neighbor_types(self, f2el, a, b, bnd_val_a, bnd_val_b, use_and, out):
         check(lib().xg_facets_with_neighbor_types(self.h, f2el.shape[0], ptr(f2el), ptr(a), ptr(b),
                                                   int(bnd_val_a), int(bnd_val_b), int(use_and), ptr(out)))
+
+    def mark_from_items(self, item2node, mark, nn, out):
+        """out bit d for each node d of a marked item (mark None: all items)"""
+        n, k = item2node.shape
+        check(lib().xg_mark_from_items(self.h, n, k, ptr(item2node), ptr(mark), nn, ptr(out)))
+
+    def elements_with_neighbor_facets(self, f2el, facet_mark, ne, out):
+        check(lib().xg_elements_with_neighbor_facets(self.h, f2el.shape[0], ptr(f2el), ptr(facet_mark), ne,
+                                                     ptr(out)))
+
+    def elements_with_shared_vertex(self, D, el2v, elem_mark, nv, out):
+        check(lib().xg_elements_with_shared_vertex(self.h, D, el2v.shape[0], ptr(el2v), ptr(elem_mark), nv,
+                                                   ptr(out)))
+
+    def node_domains(self, el2node, dt, cut_ratio, nn, cutnb=None, dom=None):
+        """el2node None: the element level (nn == number of elements)"""
+        ne = dt.shape[0]
+        k = el2node.shape[1] if el2node is not None else 0
+        check(lib().xg_node_domains(self.h, ne, k, ptr(el2node), ptr(dt), ptr(cut_ratio), nn, ptr(cutnb),
+                                    ptr(dom)))
+
+    @staticmethod
+    def _mlset_args(lsets, dtts):
+        M = len(lsets)
+        arr = (ctypes.c_void_p * M)(*[t.data_ptr() for t in lsets])
+        ptrs = ctypes.cast(arr, ctypes.c_void_p)
+        ptrs._keep = arr
+        import numpy as np
+        dt = np.ascontiguousarray(np.asarray(dtts, dtype=np.int8).reshape(-1, M))
+        return M, ptrs, dt.shape[0], dt
+
+    def mlset_rule_count(self, D, el2v, lsets, dtts, order, offsets, elems=None):
+        """lsets: list of device tensors (vertex values), dtts: (K, M) domain tuples"""
+        nel = elems.shape[0] if elems is not None else el2v.shape[0]
+        M, ptrs, K, dt = self._mlset_args(lsets, dtts)
+        check(lib().xg_mlset_rule_count(self.h, D, nel, ptr(elems), ptr(el2v), M, ptrs, K,
+                                        dt.ctypes.data_as(ctypes.c_void_p), order, ptr(offsets)))
+
+    def mlset_rule_fill(self, D, el2v, coords, lsets, dtts, order, offsets, pts, elems=None):
+        nel = elems.shape[0] if elems is not None else el2v.shape[0]
+        M, ptrs, K, dt = self._mlset_args(lsets, dtts)
+        check(lib().xg_mlset_rule_fill(self.h, D, nel, ptr(elems), ptr(el2v), ptr(coords), M, ptrs, K,
+                                       dt.ctypes.data_as(ctypes.c_void_p), order, ptr(offsets), ptr(pts)))
+
+    def mlset_mark(self, D, el2v, coords, lsets, dtts, of_type=None, contrib=None):
+        M, ptrs, K, dt = self._mlset_args(lsets, dtts)
+        check(lib().xg_mlset_mark(self.h, D, el2v.shape[0], ptr(el2v), ptr(coords), M, ptrs, K,
+                                  dt.ctypes.data_as(ctypes.c_void_p), ptr(of_type), ptr(contrib)))
 
     def cut_rule_count(self, D, el2v, lset, dt, order, offsets, elems=None, nsub=None):
         nel = elems.shape[0] if elems is not None else el2v.shape[0]

@@ -6,7 +6,8 @@ Every .hip file under csrc/ is compiled with hipcc for gfx950 and linked into
 ngsxfem_amd/libxg.so.  Flags:
   -ffp-contract=off      FP64 geometry in IEEE source order (bit-exact marking);
                          hot FP32 loops opt back in with a local pragma
-  -packed-fp32-ops off   no v_pk_{add,mul,fma}_f32
+  -packed-fp32-ops off   no v_pk_{add,mul,fma}_f32: some of their forms have given
+                         wrong results on this hardware now and then
 """
 import concurrent.futures as cf
 import os

@@ -26,11 +26,13 @@ lam^beta = lam^beta S^2.
 
 The generated code (all FP64):
   accum(lam, w, m)   m_alpha += w lam^alpha (per quadrature point)
+  complement(m)      m_alpha <- (exact moment of the whole simplex) - m_alpha
   lower(m, mk)       degree-(2P-2) moments
   forms(...)         M and K entries, written to the element matrix
 The dof order is the one of xg_device.hpp (Lattice<D,P>).
 """
 import itertools
+import math
 import sys
 from fractions import Fraction
 
@@ -225,6 +227,18 @@ def gen(D, P):
         beta = tuple(beta)
         gamma = tuple(x - y for x, y in zip(a, beta))
         L.append("    m[%d] = fma(w%d, %s, m[%d]);" % (idx2[a], idxP[beta], have[gamma], idx2[a]))
+    L.append("  }")
+    # ---- complement: m <- (moments of the whole reference simplex) - m.
+    # int_T lam^alpha = |T| D! alpha! / (|alpha| + D)! with |T| = 1/D! (the
+    # rule weights of the reference simplex sum to its volume)
+    L.append("  // m_alpha <- int_T lam^alpha - m_alpha over the whole reference simplex T")
+    L.append("  XG_DEV static void complement(double *m) {")
+    for a in mon2:
+        num = 1
+        for x in a:
+            num *= math.factorial(x)
+        full = Fraction(num, math.factorial(2 * P + D))
+        L.append("    m[%d] = %s - m[%d];" % (idx2[a], repr(float(full)), idx2[a]))
     L.append("  }")
     # ---- lower: mk_beta = sum_{|g|=2} (2!/g!) m_{beta+g}
     L.append("  XG_DEV static void lower(const double *m, double *mk) {")

@@ -5,6 +5,9 @@
 //   CutInfo                      xfem/python_xfem.cpp:47-199 (CutInformation,
 //                                xfem/cutinfo.cpp:14-133, 263-284)
 //   GetFacetsWithNeighborTypes   xfem/python_xfem.cpp:346-405 (cutinfo.cpp:476-538)
+//   GetElementsWithNeighborFacets, GetElementsWithSharedVertex,
+//   GetDofsOfElements, GetDofsOfFacets
+//                                xfem/python_xfem.cpp:407-490 (cutinfo.cpp:540-647)
 //   IntegrateX                   cutint/python_cutint.cpp:21-225
 //   InterpolateToP1 (perturb)    utils/python_utils.cpp:134,166 (p1interpol.cpp:104-107)
 //   enums                        python/python_ngsxfem.cpp:15-46
@@ -20,6 +23,7 @@
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include <algorithm>
 #include <array>
 #include <memory>
 #include <stdexcept>
@@ -37,6 +41,7 @@ enum COMBINED_DOMAIN_TYPE { CDOM_NO = 0, CDOM_NEG = 1, CDOM_POS = 2, CDOM_UNCUT 
                             CDOM_HASNEG = 5, CDOM_HASPOS = 6, CDOM_ANY = 7 };
 enum SWAP_DIMENSIONS_POLICY { FIRST_ALLOWED = 0, FIND_OPTIMAL = 1, ALWAYS_NONE = 2 };
 enum VorB { VOL = 0, BND = 1 };
+enum NODE_TYPE { NT_VERTEX = 0, NT_EDGE = 1, NT_FACE = 2, NT_CELL = 3, NT_ELEMENT = 4, NT_FACET = 5 };
 
 int to_cdt(DOMAIN_TYPE a) { return a == NEG ? CDOM_NEG : (a == POS ? CDOM_POS : CDOM_IF); }
 
@@ -119,8 +124,8 @@ struct MeshAccess {
   py::object pymesh;
   int dim = 0, nv = 0, ne = 0, nse = 0;
   DevBuf<double> coords;
-  DevBuf<int32_t> el2v, bnd2v, f2el;
-  int nf = -1;
+  DevBuf<int32_t> el2v, bnd2v, f2el, el2f, el2e;
+  int nf = -1, nedges = -1;
   std::vector<int32_t> h_el2v;
 
   explicit MeshAccess(py::object m) : pymesh(m) {
@@ -143,6 +148,55 @@ struct MeshAccess {
     auto f = py::array_t<int32_t, py::array::c_style | py::array::forcecast>(t[2]);
     nf = (int)f.shape(0);
     f2el.upload(f.data(), (size_t)nf * 2);
+    auto ef = py::array_t<int32_t, py::array::c_style | py::array::forcecast>(t[1]);
+    el2f.upload(ef.data(), (size_t)ef.size());
+  }
+  void ensure_edges() {
+    if (nedges >= 0) return;
+    py::tuple t = pymesh.attr("edges")();
+    auto ev = py::array_t<int32_t, py::array::c_style | py::array::forcecast>(t[0]);
+    auto ee = py::array_t<int32_t, py::array::c_style | py::array::forcecast>(t[1]);
+    nedges = (int)ev.shape(0);
+    el2e.upload(ee.data(), (size_t)ee.size());
+  }
+  // NGSolve's GetNNodes for NT_VERTEX, NT_EDGE, NT_FACE, NT_CELL
+  int nnodes(int nt) {
+    switch (nt) {
+      case NT_VERTEX: return nv;
+      case NT_EDGE:
+        if (dim == 1) return ne;
+        ensure_edges();
+        return nedges;
+      case NT_FACE:
+        if (dim == 2) return ne;
+        if (dim == 3) {
+          ensure_facets();
+          return nf;
+        }
+        return 0;
+      case NT_CELL: return dim == 3 ? ne : 0;
+    }
+    return 0;
+  }
+  // element -> node table of node type nt, or nullptr where the nodes of
+  // that type are the elements themselves
+  const int32_t *el2node(int nt, int &k) {
+    if (nt == NT_VERTEX) {
+      k = dim + 1;
+      return el2v.p;
+    }
+    if (nt == NT_EDGE && dim >= 2) {
+      ensure_edges();
+      k = dim == 2 ? 3 : 6;
+      return el2e.p;
+    }
+    if (nt == NT_FACE && dim == 3) {
+      ensure_facets();
+      k = 4;
+      return el2f.p;
+    }
+    k = 0;
+    return nullptr;
   }
 };
 using MA = std::shared_ptr<MeshAccess>;
@@ -179,8 +233,18 @@ class CutInformation {
   std::array<BA, 8> elems, selems, facets;
   std::vector<double> cut_ratio[2];
   DevBuf<double> d_lset;
+  // cut_neighboring_node / dom_of_node of NT_VERTEX .. NT_CELL
+  // (cutinfo.hpp:30-33); NT_ELEMENT and NT_FACET alias them by dimension
+  std::array<BA, 4> cutnb;
+  std::array<std::vector<uint8_t>, 4> dom;
 
   explicit CutInformation(MA m) : ma(m) {
+    // cutinfo.cpp:28-34: cleared cut-neighbour arrays, every node NEG
+    for (int nt = 0; nt < 4; ++nt) {
+      const int nn = ma->nnodes(nt);
+      cutnb[nt] = std::make_shared<BitArray>(nn);
+      dom[nt].assign(nn, (uint8_t)NEG);
+    }
     for (int c = 0; c < 8; ++c) {
       elems[c] = std::make_shared<BitArray>(ma->ne);
       selems[c] = std::make_shared<BitArray>(ma->nse);
@@ -223,9 +287,11 @@ class CutInformation {
       const int nw = (ne + 31) / 32;
       DevBuf<uint32_t> bits((size_t)8 * nw);
       DevBuf<double> cr(ne);
+      DevBuf<uint8_t> dt(ne);
       if (D < 1) continue;  // point boundaries of 1D meshes
-      check(xg_mark(ctx(), D, ne, vb == VOL ? ma->el2v.p : ma->bnd2v.p, d_lset.p, nullptr, cr.p, bits.p, nullptr,
+      check(xg_mark(ctx(), D, ne, vb == VOL ? ma->el2v.p : ma->bnd2v.p, d_lset.p, dt.p, cr.p, bits.p, nullptr,
                     nullptr));
+      if (vb == VOL) UpdateNodes(dt.p, cr.p);
       check(xg_sync(ctx()));
       std::vector<uint32_t> hb((size_t)8 * nw);
       bits.download(hb.data(), hb.size());
@@ -236,6 +302,31 @@ class CutInformation {
       cut_ratio[vb].resize(ne);
       cr.download(cut_ratio[vb].data(), ne);
     }
+  }
+
+  // cutinfo.cpp:135-224.  The cut-neighbour arrays are not cleared here (the
+  // reference clears them only in the constructor): they collect the nodes
+  // of the cut elements of every Update so far.
+  void UpdateNodes(const uint8_t *d_dt, const double *d_cr) {
+    for (int nt = 0; nt < 4; ++nt) {
+      const int nn = ma->nnodes(nt);
+      if (nn == 0) continue;
+      int k = 0;
+      const int32_t *tab = ma->el2node(nt, k);
+      DevBuf<uint32_t> dc;
+      DevBuf<uint8_t> dd(nn);
+      dc.upload(cutnb[nt]->words().data(), cutnb[nt]->words().size());
+      check(xg_node_domains(ctx(), ma->ne, k, tab, d_dt, d_cr, nn, dc.p, dd.p));
+      check(xg_sync(ctx()));
+      dc.download(cutnb[nt]->words().data(), cutnb[nt]->words().size());
+      dd.download(dom[nt].data(), nn);
+    }
+  }
+  int node_index(int nt) const {
+    if (nt == NT_ELEMENT) return ma->dim == 1 ? NT_EDGE : (ma->dim == 2 ? NT_FACE : NT_CELL);
+    if (nt == NT_FACET) return ma->dim == 1 ? NT_VERTEX : (ma->dim == 2 ? NT_EDGE : NT_FACE);
+    if (nt < 0 || nt > 3) throw std::runtime_error("node type: NT_VERTEX .. NT_FACET");
+    return nt;
   }
 
   BA GetElementsOfDomainType(int cdt, VorB vb) const { return vb == VOL ? elems.at(cdt) : selems.at(cdt); }
@@ -300,6 +391,149 @@ void build_rule(MA ma, const double *d_lset, int dt, int order, const std::vecto
   check(xg_sync(ctx()));
 }
 
+// ---------------------------------------------------------------- several level sets
+// A set of P1 level sets on the device (one array of vertex values each) and
+// a list of domain tuples (LevelsetIntegrationDomain with several level sets,
+// cutint/lsetintdom.cpp:178-301).
+struct MlsetDomain {
+  int M = 0, K = 0;
+  std::vector<std::unique_ptr<DevBuf<double>>> lsets;
+  std::vector<const double *> ptrs;
+  std::vector<int8_t> dtts;  // K x M
+};
+
+bool is_level_set_tuple(py::object o) { return py::isinstance<py::tuple>(o) || py::isinstance<py::list>(o); }
+
+// domain tuples: a tuple of domain types, a list of such tuples, or any
+// object with `as_list` (DomainTypeArray)
+std::vector<std::vector<int>> parse_dtts(py::object o) {
+  if (py::hasattr(o, "as_list")) o = o.attr("as_list");
+  std::vector<std::vector<int>> out;
+  auto one = [](py::object t) {
+    std::vector<int> v;
+    for (auto x : t) v.push_back(parse_dt(py::reinterpret_borrow<py::object>(x)));
+    return v;
+  };
+  if (py::isinstance<py::tuple>(o)) {
+    out.push_back(one(o));
+  } else if (py::isinstance<py::list>(o)) {
+    for (auto t : o) out.push_back(one(py::reinterpret_borrow<py::object>(t)));
+  } else {
+    throw std::runtime_error("domain_type of several level sets: a tuple of domain types or a list of tuples");
+  }
+  return out;
+}
+
+std::vector<std::vector<double>> level_set_values(MA ma, py::object lsets) {
+  std::vector<std::vector<double>> out;
+  for (auto l : lsets) {
+    bool is_p1 = true;
+    out.push_back(p1_values(py::reinterpret_borrow<py::object>(l), ma->nv, is_p1));
+    if (!is_p1) throw std::runtime_error("several level sets: P1 level sets only");
+  }
+  if (out.empty()) throw std::runtime_error("several level sets: empty level set tuple");
+  return out;
+}
+
+void make_mlset(const std::vector<std::vector<double>> &vals, py::object dts, MlsetDomain &md) {
+  md.M = (int)vals.size();
+  for (const auto &v : vals) {
+    md.lsets.emplace_back(new DevBuf<double>());
+    md.lsets.back()->upload(v.data(), v.size());
+    md.ptrs.push_back(md.lsets.back()->p);
+  }
+  auto t = parse_dtts(dts);
+  md.K = (int)t.size();
+  for (auto &tt : t) {
+    if ((int)tt.size() != md.M)
+      throw std::runtime_error("several level sets: the domain tuples must have one entry per level set");
+    for (int v : tt) md.dtts.push_back((int8_t)v);
+  }
+}
+
+void build_mlset_rule(MA ma, const MlsetDomain &md, int order, const std::vector<int32_t> *elems, DeviceRule &r) {
+  r.nel = elems ? (int)elems->size() : ma->ne;
+  if (elems) r.elems.upload(elems->data(), elems->size());
+  r.offsets.resize(r.nel + 1);
+  check(xg_mlset_rule_count(ctx(), ma->dim, r.nel, elems ? r.elems.p : nullptr, ma->el2v.p, md.M, md.ptrs.data(),
+                            md.K, md.dtts.data(), order, r.offsets.p));
+  check(xg_sync(ctx()));
+  int32_t tot = 0;
+  hipcheck(hipMemcpy(&tot, r.offsets.p + r.nel, sizeof(int32_t), hipMemcpyDeviceToHost), "rule total");
+  r.total = tot;
+  r.pts.resize((size_t)std::max(tot, 1) * 4);
+  check(xg_mlset_rule_fill(ctx(), ma->dim, r.nel, elems ? r.elems.p : nullptr, ma->el2v.p, ma->coords.p, md.M,
+                           md.ptrs.data(), md.K, md.dtts.data(), order, r.offsets.p, r.pts.p));
+  check(xg_sync(ctx()));
+}
+
+// MultiLevelsetCutInformation (xfem/cutinfo.cpp:286-415,
+// python_xfem.cpp:600-720).  As in the reference, the level set values are
+// copied at construction and by Update only, and the element marks of each
+// requested list of domain tuples are kept: asking again for the same list
+// (as a multiset of tuples) returns the same BitArray, and Update recomputes
+// every kept array in place.
+class MultiLevelsetCutInformation {
+ public:
+  MA ma;
+  std::vector<std::vector<double>> vals;
+  struct Kept {
+    BA bits;
+    std::vector<std::vector<int>> dtts;  // sorted
+    bool contrib;
+    py::object dts;
+  };
+  std::vector<Kept> kept;
+
+  MultiLevelsetCutInformation(MA m, py::object l) : ma(m) {
+    check_lsets(l);
+    vals = level_set_values(ma, l);
+  }
+  static void check_lsets(py::object l) {
+    if (!is_level_set_tuple(l)) throw std::runtime_error("MultiLevelsetCutInfo: a tuple of level sets");
+    for (auto x : l)
+      if (!py::hasattr(x, "vec"))
+        throw std::runtime_error("MultiLevelsetCutInfo: the level sets must be (P1) grid functions");
+  }
+  void compute(py::object dts, bool contrib, BitArray &out) const {
+    MlsetDomain md;
+    make_mlset(vals, dts, md);
+    if (ma->ne == 0) return;
+    DevBuf<uint32_t> d(out.words().size());
+    check(xg_mlset_mark(ctx(), ma->dim, ma->ne, ma->el2v.p, ma->coords.p, md.M, md.ptrs.data(), md.K,
+                        md.dtts.data(), contrib ? nullptr : d.p, contrib ? d.p : nullptr));
+    check(xg_sync(ctx()));
+    d.download(out.words().data(), out.words().size());
+  }
+  void Update(py::object l) {
+    check_lsets(l);
+    if (py::len(l) != vals.size()) throw std::runtime_error("MultiLevelsetCutInfo.Update: number of level sets");
+    vals = level_set_values(ma, l);
+    for (auto &k : kept) compute(k.dts, k.contrib, *k.bits);
+  }
+  BA mark(py::object dts, bool contrib, VorB vb) {
+    if (vb != VOL) throw std::runtime_error("MultiLevelsetCutInfo: VOL elements in this build");
+    auto key = parse_dtts(dts);
+    for (auto &t : key)
+      if (t.size() != vals.size())
+        throw std::runtime_error("MultiLevelsetCutInfo: the domain tuples must have one entry per level set");
+    std::sort(key.begin(), key.end());
+    for (auto &k : kept)
+      if (k.contrib == contrib && k.dtts == key) return k.bits;
+    auto ret = std::make_shared<BitArray>(ma->ne);
+    compute(dts, contrib, *ret);
+    py::list copy;
+    py::object src = py::hasattr(dts, "as_list") ? dts.attr("as_list") : dts;
+    if (py::isinstance<py::list>(src)) {
+      for (auto t : src) copy.append(t);
+      kept.push_back({ret, key, contrib, copy});
+    } else {
+      kept.push_back({ret, key, contrib, src});
+    }
+    return ret;
+  }
+};
+
 // IntegrateX (python_cutint.cpp:21-225) for straight-cut (P1) level sets.
 // cf: an object with poly_terms() -> [(coef, (a, b, c)), ...] runs on the GPU
 // reduction kernel; any other cf must provide evaluate(xyz ndarray) -> values
@@ -309,17 +543,25 @@ py::object IntegrateX(py::dict lsetdom, py::object mesh, py::object cf, py::obje
   if (!deformation.is_none()) throw std::runtime_error("IntegrateX: deformation is not supported in this build");
   MA ma = as_mesh(mesh);
   py::object lset = lsetdom["levelset"];
-  const int dt = parse_dt(lsetdom["domain_type"]);
   const int order = lsetdom.contains("order") ? lsetdom["order"].cast<int>() : 5;
   const int subdiv = lsetdom.contains("subdivlvl") ? lsetdom["subdivlvl"].cast<int>() : 0;
-  bool is_p1 = true;
-  std::vector<double> v = p1_values(lset, ma->nv, is_p1);
-  if (!is_p1 || subdiv != 0)
-    throw std::runtime_error("IntegrateX: this build integrates on P1 level sets (straight cut rules)");
-  DevBuf<double> d_l;
-  d_l.upload(v.data(), v.size());
   DeviceRule r;
-  build_rule(ma, d_l.p, dt, order, nullptr, r);
+  if (is_level_set_tuple(lset)) {
+    // several level sets (mlsetintegration.cpp)
+    if (subdiv != 0) throw std::runtime_error("IntegrateX: several level sets need subdivlvl = 0");
+    MlsetDomain md;
+    make_mlset(level_set_values(ma, lset), lsetdom["domain_type"], md);
+    build_mlset_rule(ma, md, order, nullptr, r);
+  } else {
+    const int dt = parse_dt(lsetdom["domain_type"]);
+    bool is_p1 = true;
+    std::vector<double> v = p1_values(lset, ma->nv, is_p1);
+    if (!is_p1 || subdiv != 0)
+      throw std::runtime_error("IntegrateX: this build integrates on P1 level sets (straight cut rules)");
+    DevBuf<double> d_l;
+    d_l.upload(v.data(), v.size());
+    build_rule(ma, d_l.p, dt, order, nullptr, r);
+  }
   const int D = ma->dim;
   std::vector<double> pe(ma->ne, 0.0);
   double total = 0.0;
@@ -444,6 +686,52 @@ py::tuple CutElementMatrices(py::object mesh, py::object lset, int dt, int order
   return py::make_tuple(elarr, h);
 }
 
+// element matrices on the rule of several level sets (the dCut forms of a
+// LevelsetIntegrationDomain with several level sets): GPU mlset rule, then
+// the element-matrix kernel on the materialised rule
+py::tuple MlsetElementMatrices(py::object mesh, py::object lsets, py::object dts, int order, int p, int mask,
+                               py::object elements, int precision) {
+  MA ma = as_mesh(mesh);
+  MlsetDomain md;
+  make_mlset(level_set_values(ma, lsets), dts, md);
+  std::vector<int32_t> el;
+  if (elements.is_none()) {
+    el.resize(ma->ne);
+    for (int e = 0; e < ma->ne; ++e) el[e] = e;
+  } else if (py::isinstance<BitArray>(elements)) {
+    const BitArray &b = elements.cast<const BitArray &>();
+    for (size_t e = 0; e < b.Size(); ++e)
+      if (b.Test(e)) el.push_back((int32_t)e);
+  } else {
+    auto a = py::array_t<int32_t, py::array::c_style | py::array::forcecast>(elements);
+    el.assign(a.data(), a.data() + a.size());
+  }
+  const int D = ma->dim;
+  int nd = 1;
+  for (int k = 1; k <= D; ++k) nd = nd * (p + k) / k;
+  const int nel = (int)el.size();
+  py::array_t<int32_t> elarr((py::ssize_t)nel, el.data());
+  DeviceRule r;
+  build_mlset_rule(ma, md, order, &el, r);
+  const bool f64 = precision == XG_PREC_FP64;
+  DevBuf<double> o64;
+  DevBuf<float> o32;
+  if (f64) o64.resize((size_t)std::max(nel, 1) * nd * nd);
+  else o32.resize((size_t)std::max(nel, 1) * nd * nd);
+  if (nel)
+    check(xg_elmat(ctx(), D, p, nel, r.elems.p, ma->el2v.p, ma->coords.p, r.offsets.p, r.pts.p, mask, precision,
+                   f64 ? (void *)o64.p : (void *)o32.p));
+  check(xg_sync(ctx()));
+  if (f64) {
+    py::array_t<double> h({(py::ssize_t)nel, (py::ssize_t)nd, (py::ssize_t)nd});
+    o64.download(h.mutable_data(), (size_t)nel * nd * nd);
+    return py::make_tuple(elarr, h);
+  }
+  py::array_t<float> h({(py::ssize_t)nel, (py::ssize_t)nd, (py::ssize_t)nd});
+  o32.download(h.mutable_data(), (size_t)nel * nd * nd);
+  return py::make_tuple(elarr, h);
+}
+
 // element list argument: None (all), BitArray or integer array
 std::vector<int32_t> element_list(py::object elements, int ne) {
   std::vector<int32_t> el;
@@ -551,6 +839,19 @@ PYBIND11_MODULE(xfem, m) {
       .value("FALLBACK", ALWAYS_NONE)
       .export_values();
   py::enum_<VorB>(m, "VorB").value("VOL", VOL).value("BND", BND).export_values();
+  py::enum_<NODE_TYPE>(m, "NODE_TYPE")
+      .value("VERTEX", NT_VERTEX)
+      .value("EDGE", NT_EDGE)
+      .value("FACE", NT_FACE)
+      .value("CELL", NT_CELL)
+      .value("ELEMENT", NT_ELEMENT)
+      .value("FACET", NT_FACET);
+  m.attr("NT_VERTEX") = NT_VERTEX;
+  m.attr("NT_EDGE") = NT_EDGE;
+  m.attr("NT_FACE") = NT_FACE;
+  m.attr("NT_CELL") = NT_CELL;
+  m.attr("NT_ELEMENT") = NT_ELEMENT;
+  m.attr("NT_FACET") = NT_FACET;
 
   py::class_<BitArray, BA>(m, "BitArray")
       .def(py::init<size_t>())
@@ -621,6 +922,15 @@ PYBIND11_MODULE(xfem, m) {
            py::arg("domain_type") = IF, py::arg("VOL_or_BND") = VOL)
       .def("GetFacetsOfType", [](CutInformation &self, py::object dt) { return self.GetFacetsOfDomainType(parse_cdt(dt)); },
            py::arg("domain_type") = IF)
+      .def("GetCutNeighboringNodes",
+           [](CutInformation &self, int nt) { return self.cutnb.at(self.node_index(nt)); }, py::arg("node_type"),
+           "cut_neighboring_node[node_type] (cutinfo.hpp:30): the nodes of cut elements")
+      .def("GetDomainOfNodes",
+           [](CutInformation &self, int nt) {
+             const auto &d = self.dom.at(self.node_index(nt));
+             return py::array_t<uint8_t>((py::ssize_t)d.size(), d.data());
+           },
+           py::arg("node_type"), "dom_of_node[node_type] (cutinfo.hpp:32) as DOMAIN_TYPE codes (uint8)")
       .def("GetCutRatios",
            [](CutInformation &self, VorB vb) {
              const auto &c = self.cut_ratio[vb];
@@ -659,6 +969,78 @@ PYBIND11_MODULE(xfem, m) {
         py::arg("mesh"), py::arg("a"), py::arg("bnd_val_a") = true, py::arg("bnd_val_b") = true,
         py::arg("use_and") = true, py::arg("b") = py::none(), py::arg("heapsize") = 1000000);
 
+  // cutinfo.cpp:540-647 through xg_nodes.hip
+  m.def("GetElementsWithNeighborFacets",
+        [](py::object mesh, BA a, int heapsize) {
+          MA ma = as_mesh(mesh);
+          ma->ensure_facets();
+          if ((int)a->Size() != ma->nf) throw std::runtime_error("GetElementsWithNeighborFacets: a must mark facets");
+          auto ret = std::make_shared<BitArray>(ma->ne);
+          DevBuf<uint32_t> da, dout(ret->words().size());
+          da.upload(a->words().data(), a->words().size());
+          check(xg_elements_with_neighbor_facets(ctx(), ma->nf, ma->f2el.p, da.p, ma->ne, dout.p));
+          check(xg_sync(ctx()));
+          dout.download(ret->words().data(), ret->words().size());
+          return ret;
+        },
+        py::arg("mesh"), py::arg("a"), py::arg("heapsize") = 1000000);
+  m.def("GetElementsWithSharedVertex",
+        [](py::object mesh, BA a, int heapsize) {
+          MA ma = as_mesh(mesh);
+          if ((int)a->Size() != ma->ne) throw std::runtime_error("GetElementsWithSharedVertex: a must mark elements");
+          auto ret = std::make_shared<BitArray>(ma->ne);
+          DevBuf<uint32_t> da, dout(ret->words().size());
+          da.upload(a->words().data(), a->words().size());
+          check(xg_elements_with_shared_vertex(ctx(), ma->dim, ma->ne, ma->el2v.p, da.p, ma->nv, dout.p));
+          check(xg_sync(ctx()));
+          dout.download(ret->words().data(), ret->words().size());
+          return ret;
+        },
+        py::arg("mesh"), py::arg("a"), py::arg("heapsize") = 1000000);
+  // the space stand-in gives its dof tables: dofs() -> (ne x nd, ndof) and
+  // mesh.facet_dofs(order) -> nf x nfd
+  auto dofs_of = [](py::object space, BA a, bool facets) {
+    py::tuple t = space.attr("dofs")();
+    const int ndof = t[1].cast<int>();
+    py::object tab_obj = facets ? space.attr("mesh").attr("facet_dofs")(space.attr("order")) : t[0];
+    auto tab = py::array_t<int32_t, py::array::c_style | py::array::forcecast>(tab_obj);
+    const int n = (int)tab.shape(0), k = (int)tab.shape(1);
+    if ((int)a->Size() != n)
+      throw std::runtime_error(facets ? "GetDofsOfFacets: a must mark facets" : "GetDofsOfElements: a must mark elements");
+    auto ret = std::make_shared<BitArray>(ndof);
+    DevBuf<int32_t> dtab;
+    DevBuf<uint32_t> da, dout(ret->words().size());
+    dtab.upload(tab.data(), (size_t)n * k);
+    da.upload(a->words().data(), a->words().size());
+    check(xg_mark_from_items(ctx(), n, k, dtab.p, da.p, ndof, dout.p));
+    check(xg_sync(ctx()));
+    dout.download(ret->words().data(), ret->words().size());
+    return ret;
+  };
+  m.def("GetDofsOfElements", [dofs_of](py::object space, BA a, int heapsize) { return dofs_of(space, a, false); },
+        py::arg("space"), py::arg("a"), py::arg("heapsize") = 1000000);
+  m.def("GetDofsOfFacets", [dofs_of](py::object space, BA a, int heapsize) { return dofs_of(space, a, true); },
+        py::arg("space"), py::arg("a"), py::arg("heapsize") = 1000000);
+
+  py::class_<MultiLevelsetCutInformation, std::shared_ptr<MultiLevelsetCutInformation>>(m, "MultiLevelsetCutInfo")
+      .def(py::init([](py::object mesh, py::object lsets) {
+             return std::make_shared<MultiLevelsetCutInformation>(as_mesh(mesh), lsets);
+           }),
+           py::arg("mesh"), py::arg("levelset"))
+      .def("Update", [](MultiLevelsetCutInformation &self, py::object l, int heapsize) { self.Update(l); },
+           py::arg("levelset"), py::arg("heapsize") = 1000000)
+      .def("Mesh", [](MultiLevelsetCutInformation &self) { return self.ma->pymesh; })
+      .def("GetElementsOfType",
+           [](MultiLevelsetCutInformation &self, py::object dts, VorB vb, int heapsize) {
+             return self.mark(dts, false, vb);
+           },
+           py::arg("domain_type"), py::arg("VOL_or_BND") = VOL, py::arg("heapsize") = 1000000)
+      .def("GetElementsWithContribution",
+           [](MultiLevelsetCutInformation &self, py::object dts, VorB vb, int heapsize) {
+             return self.mark(dts, true, vb);
+           },
+           py::arg("domain_type"), py::arg("VOL_or_BND") = VOL, py::arg("heapsize") = 1000000);
+
   m.def("IntegrateX", &IntegrateX, py::arg("levelset_domain"), py::arg("mesh"), py::arg("cf") = py::none(),
         py::arg("deformation") = py::none(), py::arg("ip_container") = py::none(), py::arg("element_wise") = false,
         py::arg("heapsize") = 1000000);
@@ -682,6 +1064,10 @@ PYBIND11_MODULE(xfem, m) {
         "device scatter of element matrices into a CSR pattern (FP64 values)");
   m.def("AssembleVector", &AssembleVector, py::arg("dofs"), py::arg("elvecs"), py::arg("n"),
         "device scatter of element vectors");
+  m.def("MlsetElementMatrices", &MlsetElementMatrices, py::arg("mesh"), py::arg("levelsets"),
+        py::arg("domain_type"), py::arg("order"), py::arg("p"), py::arg("mask") = 3,
+        py::arg("elements") = py::none(), py::arg("precision") = XG_PREC_FP32,
+        "element matrices of (mass | laplace) on the rule of several level sets");
   m.def("CutElementMatrices", &CutElementMatrices, py::arg("mesh"), py::arg("levelset"), py::arg("domain_type"),
         py::arg("order"), py::arg("p"), py::arg("mask") = 3, py::arg("elements") = py::none(),
         py::arg("precision") = XG_PREC_FP32);

@@ -108,6 +108,7 @@ extern "C" int xg_sync(xg_ctx *ctx) {
     XG_CHECK(hipStreamSynchronize(ctx->stream));
     if (flag & 2) return xg::set_error(XG_ERR_SUBDIV, "subdivision rule: did not expect this.. -3-");
     if (flag & 4) return xg::set_error(XG_ERR_CAPACITY, "output capacity exceeded");
+    if (flag & 8) return xg::set_error(XG_ERR_ARG, "node or dof number out of range");
     return xg::set_error(XG_ERR_DEGENERATE,
                          "You tried to cut a simplex with a plain geometry lset function "
                          "(a vertex level-set value is exactly 0 next to a negative one)");

@@ -18,10 +18,14 @@
 // holds for any rule, not only exact ones.  All arithmetic is FP64 (CDNA4
 // issues v_fma_f64 at the FP32 rate), the output is cast to T at the store.
 //
-// Mapping: one lane per (element, domain type); a wave holds 64 elements of
-// one domain type, so the number of base-rule points per sub-simplex is
-// wave-uniform and the base rule is read with scalar loads.  No LDS, no
-// barriers: occupancy is set by the VGPR count of the moment accumulators.
+// Mapping: one lane per (element, domain type); a block holds 256 elements
+// of one domain type, so the number of base-rule points per sub-simplex is
+// block-uniform.  The decomposition table and the base rules are staged in
+// LDS once per block.  The block then sorts its elements by the number of
+// sub-simplices their rule sums (0..3, xg_moment_core.hpp make_plan) with a
+// counting sort in LDS, so that a wave's lanes run the same number of point
+// loops instead of all running the largest.  Occupancy is set by the VGPR
+// count of the moment accumulators.
 #include "xg_device.hpp"
 #include "xg_elmat_core.hpp"
 #include "xg_internal.hpp"
@@ -42,15 +46,37 @@ __global__ __launch_bounds__(256) void k_moment_pipeline(int nel, const int32_t 
                                                          int32_t *err) {
   constexpr int N = ndof(D, P);
   extern __shared__ char smem[];
+  __shared__ int cnt[4];
+  __shared__ int perm[256];
+  const int tid = threadIdx.x;
+  if (tid < 4) cnt[tid] = 0;
   const MomLds S = stage_lds<D, P>(smem, order);  // block barrier: before any exit
-  const int lane = threadIdx.x & 63;
-  const int wid = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  const int ntiles = (nel + 63) >> 6;
-  const int t = wid / ntiles;
-  if (t >= ndt) return;
-  const int i = (wid - t * ntiles) * 64 + lane;
-  if (i >= nel) return;  // no cross-lane operations below
+  const int ntiles = (nel + 255) >> 8;
+  const int t = blockIdx.x / ntiles;  // domain-type slot (block-uniform)
+  const int base = (blockIdx.x - t * ntiles) * 256;
   const int dt = (dts >> (2 * t)) & 3;
+  const int nitems = min(256, nel - base);
+  // counting sort of the block's elements by the number of sub-simplices
+  const int i0 = base + tid;
+  int key = 0, pos = 0;
+  if (tid < nitems) {
+    const int e = elems ? elems[i0] : i0;
+    double l[D + 1];
+#pragma unroll
+    for (int k = 0; k <= D; ++k) l[k] = lset[el2v[(size_t)e * (D + 1) + k]];
+    key = make_plan<D>(S, l, dt).ns;
+    pos = atomicAdd(&cnt[key], 1);
+  }
+  __syncthreads();
+  if (tid < nitems) {
+    int off = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) off += (k < key) ? cnt[k] : 0;
+    perm[off + pos] = i0;
+  }
+  __syncthreads();
+  if (tid >= nitems) return;  // no cross-lane operations below
+  const int i = perm[tid];
   const int e = elems ? elems[i] : i;
   double V[D + 1][3], l[D + 1];
   ecore::load_geom<D>(el2v, coords, lset, e, V, l);
@@ -130,8 +156,8 @@ int launch_from_moments(xg_ctx *ctx, int nel, const int32_t *elems, const int32_
 template <int D, int P, typename T, int KIND = 0>
 int launch(xg_ctx *ctx, int order, int nel, const int32_t *elems, const int32_t *el2v, const double *coords,
            const double *lset, int ndt, int dts, int mask, void *out) {
-  const long long waves = (long long)ndt * ((nel + 63) / 64);
-  const long long blocks = (waves + 3) / 4;
+  const long long blocks = (long long)ndt * ((nel + 255) / 256);
+  if (blocks == 0) return XG_OK;
   if (blocks > 0x7fffffffLL) return set_error(XG_ERR_ARG, "xg_cut_pipeline: too many elements");
   k_moment_pipeline<D, P, T, KIND><<<(unsigned)blocks, 256, mom_lds_bytes(D, P, order), ctx->stream>>>(nel, elems, el2v, coords, lset, order,
                                                                       ndt, dts, mask, (T *)out, ctx->d_err);

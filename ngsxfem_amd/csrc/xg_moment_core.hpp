@@ -130,6 +130,7 @@ struct MomLds {
   const double (*rule_vol)[4];
   const double (*rule_if)[4];
   int nq_vol, nq_if;
+  bool exact;  // the rules are exact for degree 2P: moments are exact integrals
 };
 
 // Rule used by the moment form: when the requested order is at least the
@@ -173,7 +174,89 @@ __device__ __forceinline__ MomLds stage_lds(char *smem, int order) {
   S.tab = tab;
   S.rule_vol = rv;
   S.rule_if = ri;
+  S.exact = minimal;
   return S;
+}
+
+// How the moments of one (element, domain type) are obtained: the number of
+// sub-simplices whose points are summed (0..3), and whether the result is
+// the complement of the other side's moments.
+//
+// With exact rules (S.exact) the moments are the exact integrals of
+// lam^alpha over the domain, so
+//   * an uncut element needs no points: its moments are those of the whole
+//     simplex (or zero);
+//   * a volume part with D vertices of the element (the reference splits it
+//     into D sub-simplices, straightcutrule.cpp:131-196) is the element minus
+//     the single sub-simplex at the other vertex: one sub-simplex and the
+//     exact whole-simplex moments (Moments::complement).  Only used when the
+//     subtracted simplex holds at most half of the element, so the
+//     subtraction loses at most one bit of relative accuracy.
+// Below degree 2P the rule's own points are kept (any rule gives the
+// quadrature sum of that rule).
+struct Plan {
+  int edt;   // table entry used: dt, or the other side when comp
+  int ns;    // sub-simplices to sum
+  bool cut;  // the element is cut (ns pieces from the table entry)
+  bool comp; // moments = whole simplex - those of the ns pieces
+  bool full; // uncut, exact: moments of the whole simplex, no points
+  bool bad;  // the reference throws (straightcutrule.cpp:45-49, :148, :193)
+};
+
+template <int D>
+__device__ __forceinline__ Plan make_plan(const MomLds &S, const double *l, int dt) {
+  Plan pl;
+  pl.edt = dt;
+  pl.ns = 0;
+  pl.cut = pl.comp = pl.full = pl.bad = false;
+  const int eldom = check_if_straight_cut<D>(l);
+  if (eldom != IF) {
+    if (S.exact) pl.full = (eldom == dt);
+    else pl.ns = (eldom == dt) ? 1 : 0;  // the base rule on the element itself
+    return pl;
+  }
+  if (!cut_is_valid<D>(l)) {
+    pl.bad = true;
+    return pl;
+  }
+  const int msk = sign_mask<D>(l);
+  const DecompEntry &E = S.tab[dt * 16 + msk];
+  if (!E.ok) {
+    pl.bad = true;
+    return pl;
+  }
+  pl.cut = true;
+  pl.ns = E.ns;
+  if (S.exact && dt != IF && E.ns > 1) {
+    // a valid cut has no zero values, so msk bit i = (l_i > 0); the other
+    // side is a single vertex v when exactly one bit differs from dt's side
+    const int npos = __builtin_popcount(msk);
+    const int nother = (dt == NEG) ? npos : (D + 1 - npos);
+    if (nother == 1) {
+      int v = 0;
+#pragma unroll
+      for (int i = 0; i <= D; ++i)
+        if ((((msk >> i) & 1) != 0) == (dt == NEG)) v = i;
+      double lv = l[0];
+#pragma unroll
+      for (int i = 1; i <= D; ++i) lv = (i == v) ? l[i] : lv;
+      // volume fraction of the simplex at v: prod_w lv / (lv - lw) (each
+      // factor in (0, 1))
+      double num = 1.0, den = 1.0;
+#pragma unroll
+      for (int i = 0; i <= D; ++i) {
+        if (i == v) continue;
+        num *= lv;
+        den *= lv - l[i];
+      }
+      if (fabs(num) <= 0.5 * fabs(den)) {
+        pl.comp = true;
+        pl.edt = (dt == NEG) ? POS : NEG;
+        pl.ns = 1;
+      }
+    }
+  }
+  return pl;
 }
 
 // The degree-2P moments of the cut rule of one (element, domain type) (rule
@@ -183,49 +266,35 @@ template <int D, int P>
 __device__ __forceinline__ bool moment_accumulate(const MomLds &S, const double *l, const double *Finv, int dt,
                                                   double *m) {
   using MM = Moments<D, P>;
-  // topology of the (element, dt) rule
+  const Plan pl = make_plan<D>(S, l, dt);
   const int nvs = (dt == IF) ? D : D + 1;  // vertices per sub-simplex (wave-uniform)
-  const int eldom = check_if_straight_cut<D>(l);
   PackedEntry E;
 #pragma unroll
   for (int q = 0; q < 6; ++q) E.w[q] = 0u;
-  bool cut = false;
-  int ns = 0;
-  bool bad = false;
   double tc0 = 0.0, tc1 = 0.0, tc2 = 0.0, tc3 = 0.0;
-  if (eldom != IF) {
-    ns = (eldom == dt) ? 1 : 0;  // uncut: the base rule on the element itself
-  } else if (!cut_is_valid<D>(l)) {
-    bad = true;
-  } else {
-    const uint32_t *src = reinterpret_cast<const uint32_t *>(&S.tab[dt * 16 + sign_mask<D>(l)]);
+  if (pl.cut) {
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(&S.tab[pl.edt * 16 + sign_mask<D>(l)]);
 #pragma unroll
     for (int q = 0; q < 6; ++q) E.w[q] = src[q];
-    cut = true;
-    if (!E.ok()) {
-      bad = true;
-      ns = 0;
-    } else {
-      ns = E.ns();
-      const int nc = E.nc();
+    const int nc = E.nc();
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        if (c >= nc) break;
-        const int a = E.ce(c, 0), b = E.ce(c, 1);
-        double la = l[0], lb = l[0];
+    for (int c = 0; c < 4; ++c) {
+      if (c >= nc) break;
+      const int a = E.ce(c, 0), b = E.ce(c, 1);
+      double la = l[0], lb = l[0];
 #pragma unroll
-        for (int k = 1; k <= D; ++k) {
-          la = (k == a) ? l[k] : la;
-          lb = (k == b) ? l[k] : lb;
-        }
-        const double tc = la / (la - lb);
-        tc0 = (c == 0) ? tc : tc0;
-        tc1 = (c == 1) ? tc : tc1;
-        tc2 = (c == 2) ? tc : tc2;
-        tc3 = (c == 3) ? tc : tc3;
+      for (int k = 1; k <= D; ++k) {
+        la = (k == a) ? l[k] : la;
+        lb = (k == b) ? l[k] : lb;
       }
+      const double tc = la / (la - lb);
+      tc0 = (c == 0) ? tc : tc0;
+      tc1 = (c == 1) ? tc : tc1;
+      tc2 = (c == 2) ? tc : tc2;
+      tc3 = (c == 3) ? tc : tc3;
     }
   }
+  const int ns = pl.ns;
   const double ifac = (dt == IF && ns > 0) ? interface_factor<D>(l, Finv) : 1.0;
   const int nq = (dt == IF) ? S.nq_if : S.nq_vol;
   const double(*rule)[4] = (dt == IF) ? S.rule_if : S.rule_vol;
@@ -237,7 +306,7 @@ __device__ __forceinline__ bool moment_accumulate(const MomLds &S, const double 
     if (s >= ns) break;
     double W[D + 1][3];
     double wsc = 1.0;
-    if (!cut) {
+    if (!pl.cut) {
       // uncut element: origin, e_0, .., e_{D-1} (identity map)
 #pragma unroll
       for (int k = 0; k <= D; ++k)
@@ -262,7 +331,8 @@ __device__ __forceinline__ bool moment_accumulate(const MomLds &S, const double 
     // zeros
     accum_piece<D, P, D>(rule, nq, W, wsc, m);
   }
-  return bad;
+  if (pl.comp || pl.full) MM::complement(m);  // full: whole simplex - 0
+  return pl.bad;
 }
 
 // One (element, domain type): rule topology, moments, identities; writes the

@@ -66,13 +66,22 @@ class Mesh:
             return self.el2v, self.nv
         if p != 2:
             raise NotImplementedError("global dof numbering for p = 1, 2")
-        D = self.dim
-        pairs = [(i, j) for i in range(D + 1) for j in range(i + 1, D + 1)]
-        ev = np.stack([np.sort(self.el2v[:, [i, j]], axis=1) for i, j in pairs], axis=1)  # ne x nE x 2
-        key = ev[..., 0].astype(np.int64) * self.nv + ev[..., 1]
-        uniq, inv = np.unique(key.ravel(), return_inverse=True)
-        edofs = self.nv + inv.reshape(key.shape)
-        return np.ascontiguousarray(np.hstack([self.el2v, edofs]).astype(np.int32)), self.nv + len(uniq)
+        e2v, el2e = self.edges()
+        return np.ascontiguousarray(np.hstack([self.el2v, self.nv + el2e]).astype(np.int32)), self.nv + len(e2v)
+
+    def edges(self):
+        """(edge2v (nedge, 2) sorted vertex pairs in ascending order, el2e
+        (ne, nE)): the element's edges in the local order (0,1),(0,2),...,
+        (1,2),... .  In 1D the edges are the elements."""
+        if getattr(self, "_edges", None) is None:
+            D = self.dim
+            pairs = [(i, j) for i in range(D + 1) for j in range(i + 1, D + 1)]
+            ev = np.stack([np.sort(self.el2v[:, [i, j]], axis=1) for i, j in pairs], axis=1)  # ne x nE x 2
+            key = ev[..., 0].astype(np.int64) * self.nv + ev[..., 1]
+            uniq, inv = np.unique(key.ravel(), return_inverse=True)
+            e2v = np.stack([uniq // self.nv, uniq % self.nv], axis=1).astype(np.int32)
+            self._edges = (e2v, np.ascontiguousarray(inv.reshape(key.shape).astype(np.int32)))
+        return self._edges
 
     @property
     def ne(self):
@@ -117,6 +126,25 @@ class Mesh:
 
     def GetNFacets(self):
         return self.facets()[0].shape[0]
+
+    def facet_dofs(self, p):
+        """global dof numbers of the degree-p Lagrange space on each facet
+        (nf x ndof_facet): the dofs of a neighbouring element that do not
+        involve the vertex opposite the facet."""
+        dofs, _ = self.dofs(p)
+        f2v, el2f, f2el = self.facets()
+        D = self.dim
+        e = f2el[:, 0]
+        k = np.argmax(el2f[e] == np.arange(len(f2el))[:, None], axis=1)  # local facet = opposite vertex
+        locs = []
+        for kk in range(D + 1):
+            loc = [j for j in range(D + 1) if j != kk]
+            if p == 2:
+                pairs = [(i, j) for i in range(D + 1) for j in range(i + 1, D + 1)]
+                loc += [D + 1 + t for t, (i, j) in enumerate(pairs) if kk not in (i, j)]
+            locs.append(loc)
+        locs = np.array(locs)
+        return np.ascontiguousarray(np.take_along_axis(dofs[e], locs[k], axis=1).astype(np.int32))
 
 
 def Make1DMesh(n, mapping=None):

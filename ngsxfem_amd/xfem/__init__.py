@@ -2,8 +2,9 @@
 
 Mirror of the reference package ``xfem`` (python/__init__.py): the compiled
 module ``xfem.xfem`` (here ``ngsxfem_amd.xfem.xfem``, pybind11 over the HIP
-C ABI) provides CutInfo, BitArray, GetFacetsWithNeighborTypes, IntegrateX
-and the enums; this file adds HAS, Integrate, dCut, SymbolicCutBFI,
+C ABI) provides CutInfo, BitArray, GetFacetsWithNeighborTypes,
+GetElementsWithNeighborFacets, GetElementsWithSharedVertex,
+GetDofsOfElements, GetDofsOfFacets, IntegrateX and the enums; this file adds HAS, Integrate, dCut, SymbolicCutBFI,
 InterpolateToP1 and the small NGSolve stand-ins the path needs (meshes, P1
 grid functions, polynomial coefficient functions, u*v / grad(u)*grad(v)
 forms, BilinearForm.Assemble).
@@ -22,11 +23,18 @@ import math
 
 import numpy as np
 
+from .._abi import load_hip_runtime
 from ..meshes import Make1DMesh, MakeStructured2DMesh, MakeStructured3DMesh, Mesh  # noqa: F401
+
+load_hip_runtime()  # before the host module loads libxg.so: one HIP runtime per process
+
 from .xfem import (ANY, BND, CDOM_IF, CDOM_NEG, CDOM_POS, FALLBACK, FIRST, HASNEG, HASPOS, IF, NEG, NO,  # noqa: F401
                    OPTIMAL, POS, UNCUT, VOL, AssembleCSR, AssembleVector, BitArray, COMBINED_DOMAIN_TYPE,
-                   CutElementMatrices, CutElementVectors, CutInfo,
-                   DOMAIN_TYPE, GetFacetsWithNeighborTypes, IntegrateX, PerturbP1, QUAD_DIRECTION_POLICY, VorB)
+                   CutElementMatrices, CutElementVectors, CutInfo, DOMAIN_TYPE, GetDofsOfElements,
+                   GetDofsOfFacets, GetElementsWithNeighborFacets, GetElementsWithSharedVertex,
+                   GetFacetsWithNeighborTypes, IntegrateX, MlsetElementMatrices, MultiLevelsetCutInfo, NODE_TYPE,
+                   NT_CELL, NT_EDGE, NT_ELEMENT, NT_FACE, NT_FACET, NT_VERTEX, PerturbP1, QUAD_DIRECTION_POLICY,
+                   VorB)
 
 PREC_FP64, PREC_FP32 = 0, 1
 
@@ -396,17 +404,26 @@ class _CutIntegral:
         d = m.lsetdom
         if d.get("subdivlvl", 0):
             raise NotImplementedError("subdivlvl > 0 element matrices: use the P2 subdivision kernels")
+        if isinstance(d["levelset"], (tuple, list)):
+            return tuple(d["levelset"]), d["domain_type"], d.get("order", 2 * p), m.definedonelements
         return d["levelset"], int(d["domain_type"]), d.get("order", 2 * p), m.definedonelements
 
     def element_matrices(self, mesh, p, precision):
         lset, dt, order, els = self._rule_args(mesh, p)
         mask = (1 if "mass" in self.integrand.parts else 0) | (2 if "laplace" in self.integrand.parts else 0)
         coefs = self.integrand.parts
+        if isinstance(lset, (tuple, list)):
+            # several level sets: dt holds the domain tuples
+            def kernel(msk):
+                return MlsetElementMatrices(mesh, tuple(lset), dt, order, p, msk, els, precision)
+        else:
+            def kernel(msk):
+                return CutElementMatrices(mesh, lset, dt, order, p, msk, els, precision)
         if mask == 3 and coefs["mass"] != coefs["laplace"]:
-            e1, m1 = CutElementMatrices(mesh, lset, dt, order, p, 1, els, precision)
-            e2, m2 = CutElementMatrices(mesh, lset, dt, order, p, 2, els, precision)
+            e1, m1 = kernel(1)
+            e2, m2 = kernel(2)
             return e1, coefs["mass"] * m1 + coefs["laplace"] * m2
-        e, m = CutElementMatrices(mesh, lset, dt, order, p, mask, els, precision)
+        e, m = kernel(mask)
         c = coefs.get("mass", coefs.get("laplace", 1.0))
         return e, c * m
 
@@ -513,23 +530,4 @@ class LinearForm:
             els, vecs = ci.element_vectors(mesh, p)
             out += AssembleVector(np.ascontiguousarray(dofs[els]), np.asarray(vecs, dtype=np.float64), n)
         self.vec = out
-        return self
-
-    def Assemble(self, precision=None):
-        prec = ngsxfemglobals.elmat_precision if precision is None else precision
-        mesh = self.space.mesh
-        p = self.space.order
-        self.elmats = [ci.element_matrices(mesh, p, prec) for ci in self.integrals]
-        if p == 1:
-            from scipy.sparse import coo_matrix
-            nd = mesh.dim + 1
-            rows, cols, vals = [], [], []
-            for els, mats in self.elmats:
-                dofs = mesh.el2v[els]
-                rows.append(np.repeat(dofs, nd, axis=1).ravel())
-                cols.append(np.tile(dofs, (1, nd)).ravel())
-                vals.append(np.asarray(mats, dtype=np.float64).ravel())
-            n = mesh.nv
-            self.mat = coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
-                                  shape=(n, n)).tocsr()
         return self

@@ -36,6 +36,10 @@ def _load():
     lib.or_straight_cut_rule.argtypes = [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp,
                                          _dp, _dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     lib.or_mark.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _dp, _bp, _dp, _up]
+    lib.or_mlset_rule.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _ip, ctypes.c_int, _dp, _dp,
+                                  ctypes.c_double, _dp, _dp, ctypes.c_int]
+    lib.or_mlset_mark.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _dp, ctypes.c_int, _dp, ctypes.c_int,
+                                  ctypes.c_int, _ip, _bp, _bp]
     lib.or_h1_ndof.argtypes = [ctypes.c_int, ctypes.c_int]
     lib.or_h1_shape.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
     lib.or_h1_node.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
@@ -134,6 +138,43 @@ def mark(D, el2v, lset_v):
     if r:
         raise RuntimeError("oracle mark error %d" % r)
     return dt, cr, bits
+
+
+def mlset_rule(D, lsets, dtts, order, V=None, cap=20000):
+    """mlset rule of one element (CreateMultiLevelsetCutIntegrationRule):
+    lsets (M, D+1) vertex values, dtts (K, M) domain tuples, V (D+1, D)
+    physical vertices (None: the reference element).  Returns (pts[n,3],
+    w[n]) with reference weights (physical = w * |det F|)."""
+    l = _f64(lsets)
+    M = l.shape[0]
+    dt = np.ascontiguousarray(np.asarray(dtts, dtype=np.int32).reshape(-1, M))
+    if V is None:
+        V = np.vstack([np.eye(D), np.zeros(D)])
+    F, Fi, ad = jacobian(V)
+    pts = np.zeros((cap, 3))
+    w = np.zeros(cap)
+    n = lib().or_mlset_rule(D, M, _d(l), dt.shape[0], dt.ctypes.data_as(_ip), order, _d(_f64(F)), _d(_f64(Fi)),
+                            ad, _d(pts), _d(w), cap)
+    if n < 0:
+        raise RuntimeError("oracle mlset rule error %d" % n)
+    return pts[:n], w[:n]
+
+
+def mlset_mark(D, el2v, coords, lset_v, dtts, contrib=True):
+    """(of_type, contrib) per element (MultiLevelsetCutInformation)"""
+    lv = _f64(lset_v)
+    M, nv = lv.shape
+    dt = np.ascontiguousarray(np.asarray(dtts, dtype=np.int32).reshape(-1, M))
+    e = np.ascontiguousarray(el2v, dtype=np.int32)
+    ne = e.shape[0]
+    ot = np.zeros(ne, np.uint8)
+    co = np.zeros(ne, np.uint8) if contrib else None
+    r = lib().or_mlset_mark(D, ne, e.ctypes.data_as(_ip), _d(_f64(coords)), M, _d(lv), nv, dt.shape[0],
+                            dt.ctypes.data_as(_ip), ot.ctypes.data_as(_bp),
+                            co.ctypes.data_as(_bp) if co is not None else None)
+    if r < 0:
+        raise RuntimeError("oracle mlset mark error %d" % r)
+    return ot.astype(bool), (co.astype(bool) if co is not None else None)
 
 
 def h1_ndof(D, p):

@@ -302,11 +302,11 @@ static void simp_set(Simp *s, int np, const double (*pts[])[3])
   for (int k = 0; k < np; ++k) memcpy(s->p[k], *pts[k], sizeof(double) * 3);
 }
 
-/* LevelsetCutSimplex::Decompose, straightcutrule.cpp:131-196 */
-static int decompose(int D, const double *l, int dt, Simp *out, int *nsub)
+/* LevelsetCutSimplex::Decompose, straightcutrule.cpp:131-196, of the
+   simplex with vertices P[0..D] (any simplex of dimension D in R^3) */
+static int decompose_pts(int D, double P[4][3], const double *l, int dt, Simp *out, int *nsub)
 {
-  double P[4][3], C[4][3];
-  ref_vertices(D, P);
+  double C[4][3];
   if (or_check_if_straight_cut(D + 1, l) != OR_IF) return OR_ERR_NOT_IMPL;
   int nc = 0;
   int err = calc_if_polytope(D, l, P, C, &nc);
@@ -361,6 +361,13 @@ static int decompose(int D, const double *l, int dt, Simp *out, int *nsub)
   } else
     return OR_ERR_NOT_IMPL;
   return 0;
+}
+
+static int decompose(int D, const double *l, int dt, Simp *out, int *nsub)
+{
+  double P[4][3];
+  ref_vertices(D, P);
+  return decompose_pts(D, P, l, dt, out, nsub);
 }
 
 /* SimpleX::GetVolume, straightcutrule.cpp:66-72 */
@@ -452,6 +459,235 @@ int or_straight_cut_rule(int D, const double *lset, int dt, int order,
     for (int q = 0; q < n; ++q) w[q] = w[q] * len;
   }
   return n;
+}
+
+/* ====================================================================== */
+/* multiple level sets                                                    */
+/* ====================================================================== */
+
+/* LevelsetWrapper of a P1 level set on a simplex (straightcutrule.cpp:503-
+   555): coefficients from the vertex values (GetCoeffsFromVals, SEGM / TRIG /
+   TET), evaluation in the loop order of operator(), and the 1e-14 clamp of
+   update_initial_coefs */
+static void lsw_coefs(int D, const double *vals, double c[4])
+{
+  c[0] = vals[D];
+  c[1] = D >= 1 ? vals[0] - vals[D] : 0.0;
+  c[2] = D >= 2 ? vals[1] - vals[D] : 0.0;
+  c[3] = D >= 3 ? vals[2] - vals[D] : 0.0;
+}
+
+static double lsw_eval(const double c[4], const double *p)
+{
+  /* sum over c[i][j][k] p0^i p1^j p2^k in the order (000),(001),(010),(100);
+     the bilinear coefficients are zero for simplices */
+  double v = 0.0;
+  v += c[0];
+  v += c[3] * p[2];
+  v += c[2] * p[1];
+  v += c[1] * p[0];
+  return v;
+}
+
+static double lsw_clamped(const double c[4], const double *p)
+{
+  double d = lsw_eval(c, p);
+  return fabs(d) > 1e-14 ? d : 1e-14;
+}
+
+/* unit normal of the level set on the reference element (GetNormal) */
+static void lsw_normal(const double c[4], double n[3])
+{
+  n[0] = c[1];
+  n[1] = c[2];
+  n[2] = c[3];
+  double len = norm3(n);
+  for (int d = 0; d < 3; ++d) n[d] /= len;
+}
+
+#define OR_MLSET_MAXSIMP 729
+
+/* StraightCutsIntegrationRule, straightcutrule.cpp:679-800 (simplices).
+   lsets: M x (D+1) vertex values of the level sets that cut the element,
+   dts: their target domain types; F, Finv: D x D Jacobian and its inverse,
+   absdet = |det F|.  Appends to pts / w from n0; returns the new count. */
+static int mlset_cut_rule(int D, int M, const double *lsets, const int *dts, int order, const double *F,
+                          const double *Finv, double absdet, double *pts, double *w, int cap, int n0)
+{
+  static __thread Simp bufA[OR_MLSET_MAXSIMP], bufB[OR_MLSET_MAXSIMP];
+  Simp *cur = bufA, *nxt = bufB;
+  int ncur = 0;
+  double P[4][3];
+  ref_vertices(D, P);
+  {
+    int ns = 0;
+    int err = decompose_pts(D, P, &lsets[0], dts[0], cur, &ns);
+    if (err) return err;
+    ncur = ns;
+  }
+  double c[4];
+  for (int i = 1; i < M; ++i) {
+    lsw_coefs(D, &lsets[i * (D + 1)], c);
+    int nn = 0;
+    for (int s = 0; s < ncur; ++s) {
+      double vals[4];
+      for (int k = 0; k < cur[s].np; ++k) vals[k] = lsw_clamped(c, cur[s].p[k]);
+      int dts_s = or_check_if_straight_cut(cur[s].np, vals);
+      if (dts_s == OR_IF) {
+        if (nn + MAXSUB > OR_MLSET_MAXSIMP) return OR_ERR_CAP;
+        int ns = 0;
+        int err = decompose_pts(cur[s].np - 1, cur[s].p, vals, dts[i], &nxt[nn], &ns);
+        if (err) return err;
+        nn += ns;
+      } else if (dts_s == dts[i]) {
+        if (nn + 1 > OR_MLSET_MAXSIMP) return OR_ERR_CAP;
+        nxt[nn++] = cur[s];
+      }
+    }
+    Simp *t = cur;
+    cur = nxt;
+    nxt = t;
+    ncur = nn;
+  }
+  int n = n0;
+  for (int s = 0; s < ncur; ++s) {
+    n = simp_rule(&cur[s], order, pts, w, cap, n);
+    if (n < 0) return n;
+  }
+  int ifi[4], nif = 0;
+  for (int i = 0; i < M; ++i)
+    if (dts[i] == OR_IF) {
+      if (nif >= 3) return OR_ERR_NOT_IMPL;  /* "Codim possibilities available" */
+      ifi[nif++] = i;
+    }
+  double fac = 1.0;
+  if (nif == 1) {
+    /* TransformQuadUntrafoToIRInterface: |F^{-T} n_ref| */
+    double nr[3];
+    lsw_coefs(D, &lsets[ifi[0] * (D + 1)], c);
+    lsw_normal(c, nr);
+    double v[3] = {0, 0, 0};
+    for (int r = 0; r < D; ++r) {
+      double acc = 0.0;
+      for (int q = 0; q < D; ++q) acc += Finv[q * D + r] * nr[q];
+      v[r] = acc;
+    }
+    fac = norm3(v);
+  } else if (nif == 2) {
+    if (D == 2) {
+      fac = 1.0 / absdet;
+    } else if (D == 3) {
+      double n0v[3], n1v[3], cp[3];
+      lsw_coefs(D, &lsets[ifi[0] * (D + 1)], c);
+      lsw_normal(c, n0v);
+      lsw_coefs(D, &lsets[ifi[1] * (D + 1)], c);
+      lsw_normal(c, n1v);
+      cross3(n0v, n1v, cp);
+      double cpf = 1.0 / norm3(cp);
+      double v[3];
+      for (int r = 0; r < 3; ++r) {
+        double acc = 0.0;
+        for (int q = 0; q < 3; ++q) acc += F[r * 3 + q] * cp[q];
+        v[r] = cpf * acc;
+      }
+      fac = norm3(v) / absdet;
+    } else
+      return OR_ERR_NOT_IMPL;
+  } else if (nif == 3) {
+    if (D != 3) return OR_ERR_NOT_IMPL;
+    fac = 1.0 / absdet;
+  }
+  if (nif > 0)
+    for (int q = n0; q < n; ++q) w[q] = w[q] * fac;
+  return n;
+}
+
+/* CreateMultiLevelsetCutIntegrationRule, mlsetintegration.cpp:8-158, on one
+   element: lsets M x (D+1) vertex values, dtts K x M domain tuples (the
+   LevelsetIntegrationDomain's domain list).  Returns the number of points
+   (0 when no tuple contributes). */
+int or_mlset_rule(int D, int M, const double *lsets, int K, const int *dtts, int order, const double *F,
+                  const double *Finv, double absdet, double *pts, double *w, int cap)
+{
+  int ldt[16];
+  if (M > 16) return OR_ERR_CAP;
+  for (int i = 0; i < M; ++i) ldt[i] = or_check_if_straight_cut(D + 1, &lsets[i * (D + 1)]);
+  int n = 0;
+  for (int k = 0; k < K; ++k) {
+    const int *dts = &dtts[k * M];
+    int compatible = 1, cut = 0;
+    for (int i = 0; i < M; ++i) {
+      if (ldt[i] != OR_IF && ldt[i] != dts[i]) compatible = 0;
+      else if (ldt[i] == OR_IF) cut = 1;
+    }
+    if (!compatible) continue;
+    if (!cut) {
+      const Rule *r = get_rule(D, order);
+      if (n + r->n > cap) return OR_ERR_CAP;
+      for (int q = 0; q < r->n; ++q) {
+        double x[3] = {0, 0, 0};
+        for (int m = 0; m < D; ++m) x[m] = r->xi[q * D + m];
+        memcpy(&pts[3 * (n + q)], x, sizeof(x));
+        w[n + q] = r->w[q];
+      }
+      n += r->n;
+      continue;
+    }
+    double cl[16 * 4];
+    int cd[16], nc = 0;
+    for (int i = 0; i < M; ++i)
+      if (ldt[i] == OR_IF || ldt[i] != dts[i]) {
+        memcpy(&cl[nc * (D + 1)], &lsets[i * (D + 1)], sizeof(double) * (D + 1));
+        cd[nc++] = dts[i];
+      }
+    int r = mlset_cut_rule(D, nc, cl, cd, order, F, Finv, absdet, pts, w, cap, n);
+    if (r < 0) return r;
+    n = r;
+  }
+  return n;
+}
+
+/* MultiLevelsetCutInformation (cutinfo.cpp:286-415) on a mesh: bit e of
+   of_type = the element's level-set domain tuple is one of dtts;
+   contrib (nullable): the mlset rule of order 0 has positive weight sum. */
+int or_mlset_mark(int D, int ne, const int32_t *el2v, const double *coords, int M, const double *lset_v, int nv,
+                  int K, const int *dtts, uint8_t *of_type, uint8_t *contrib)
+{
+  double *pts = malloc(sizeof(double) * 3 * 4096), *w = malloc(sizeof(double) * 4096);
+  int err = 0;
+  for (int e = 0; e < ne && !err; ++e) {
+    double l[16 * 4];
+    int ldt[16];
+    for (int i = 0; i < M; ++i) {
+      for (int k = 0; k <= D; ++k) l[i * (D + 1) + k] = lset_v[(size_t)i * nv + el2v[(size_t)e * (D + 1) + k]];
+      ldt[i] = or_check_if_straight_cut(D + 1, &l[i * (D + 1)]);
+    }
+    int match = 0;
+    for (int k = 0; k < K; ++k) {
+      int ok = 1;
+      for (int i = 0; i < M; ++i)
+        if (dtts[k * M + i] != ldt[i]) ok = 0;
+      if (ok) match = 1;
+    }
+    of_type[e] = (uint8_t)match;
+    if (contrib) {
+      double V[4 * 3], F[9], Fi[9], ad;
+      for (int k = 0; k <= D; ++k)
+        for (int d = 0; d < D; ++d) V[k * D + d] = coords[(size_t)el2v[(size_t)e * (D + 1) + k] * D + d];
+      or_jacobian(D, V, F, Fi, &ad);
+      int n = or_mlset_rule(D, M, l, K, dtts, 0, F, Fi, ad, pts, w, 4096);
+      if (n < 0) {
+        err = n;
+        break;
+      }
+      double sw = 0.0;
+      for (int q = 0; q < n; ++q) sw += w[q];
+      contrib[e] = sw > 0.0;
+    }
+  }
+  free(pts);
+  free(w);
+  return err;
 }
 
 /* ====================================================================== */

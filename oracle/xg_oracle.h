@@ -58,6 +58,20 @@ int or_straight_cut_rule(int D, const double *lset, int dt, int order,
                          const double *Finv, double *pts, double *w, int cap,
                          int *nsub);
 
+/* ---- multiple level sets (mlsetintegration.cpp:8-158,
+   straightcutrule.cpp:679-800, cutinfo.cpp:286-415) --------------------- */
+/* One element: lsets M x (D+1) vertex values, dtts K x M domain tuples.
+   Points (xyz triples, reference coordinates) and weights (physical weight
+   = w * absdet) appended tuple by tuple.  Returns the number of points or a
+   negative OR_ERR_* code. */
+int or_mlset_rule(int D, int M, const double *lsets, int K, const int *dtts, int order, const double *F,
+                  const double *Finv, double absdet, double *pts, double *w, int cap);
+/* of_type[e] = the element's tuple of CheckIfStraightCut values is in dtts;
+   contrib[e] (nullable) = the order-0 rule has a positive weight sum.
+   lset_v: M x nv. */
+int or_mlset_mark(int D, int ne, const int32_t *el2v, const double *coords, int M, const double *lset_v, int nv,
+                  int K, const int *dtts, uint8_t *of_type, uint8_t *contrib);
+
 /* ---- marking (cutinfo.cpp:61-133) -------------------------------------- */
 /* bits: 8 arrays (indexed by COMBINED_DOMAIN_TYPE) of nwords=ceil(ne/32)
    uint32 words each, LSB-first.  returns 0 or an OR_ERR code. */

@@ -80,6 +80,34 @@ extern "C" int mom_elmat(int D, int P, const double *V, const double *pts, const
   CASE(1, 1) CASE(1, 2) CASE(1, 3) CASE(2, 1) CASE(2, 2) CASE(2, 3) CASE(3, 1) CASE(3, 2) CASE(3, 3)
   return -1;
 }
+// Moments::complement on m (in place)
+extern "C" int mom_complement(int D, int P, double *m) {
+#define CCASE(DD, PP)                          \
+  if (D == DD && P == PP) {                    \
+    xg::Moments<DD, PP>::complement(m);        \
+    return xg::Moments<DD, PP>::NM;            \
+  }
+  CCASE(1, 1) CCASE(1, 2) CCASE(1, 3) CCASE(2, 1) CCASE(2, 2) CCASE(2, 3) CCASE(3, 1) CCASE(3, 2) CCASE(3, 3)
+  return -1;
+}
+// Moments::accum over a point set
+extern "C" int mom_accum(int D, int P, const double *pts, const double *w, int n, double *m) {
+#define ACASE(DD, PP)                                  \
+  if (D == DD && P == PP) {                            \
+    for (int q = 0; q < n; ++q) {                      \
+      double l[4], s = 1;                              \
+      for (int d = 0; d < DD; ++d) {                   \
+        l[d] = pts[q * 3 + d];                         \
+        s -= l[d];                                     \
+      }                                                \
+      l[DD] = s;                                       \
+      xg::Moments<DD, PP>::accum(l, w[q], m);          \
+    }                                                  \
+    return xg::Moments<DD, PP>::NM;                    \
+  }
+  ACASE(1, 1) ACASE(1, 2) ACASE(1, 3) ACASE(2, 1) ACASE(2, 2) ACASE(2, 3) ACASE(3, 1) ACASE(3, 2) ACASE(3, 3)
+  return -1;
+}
 """
 
 
@@ -160,3 +188,33 @@ def test_nodal_table_maps_moments_to_fixed_node_weights(orc):
         phi, _ = orc.h1_shape(3, 4, x)
         W += wi * phi
     assert np.abs(A @ m - W).max() < 1e-12 * np.abs(W).max()
+
+
+@pytest.mark.parametrize("D", [1, 2, 3])
+@pytest.mark.parametrize("P", [1, 2, 3])
+def test_complement_is_whole_simplex_minus(orc, momlib, D, P):
+    """complement(0) = the moments of the whole reference simplex (the
+    oracle's base rule of order 2P summed through accum), and for a straight
+    cut the NEG moments equal complement(POS moments) (exact rules)"""
+    pts, w = orc.rule(D, 2 * P)
+    pts3 = np.ascontiguousarray(np.hstack([pts, np.zeros((len(w), 3 - D))]))
+    full = np.zeros(128)
+    nm = momlib.mom_accum(D, P, _p(pts3), _p(np.ascontiguousarray(w)), len(w), _p(full))
+    got = np.zeros(128)
+    assert momlib.mom_complement(D, P, _p(got)) == nm
+    assert np.abs(got[:nm] - full[:nm]).max() <= 2e-15 * np.abs(full[:nm]).max()
+    rng = np.random.default_rng(D * 10 + P)
+    for trial in range(5):
+        lset = rng.standard_normal(D + 1)
+        lset[0], lset[1] = -abs(lset[0]) - 0.1, abs(lset[1]) + 0.1  # cut
+        mom = {}
+        for dt in (0, 1):
+            r = orc.straight_cut_rule(D, lset, dt, 2 * P)
+            p_, w_, _ = r
+            p3 = np.ascontiguousarray(p_)
+            mm = np.zeros(128)
+            momlib.mom_accum(D, P, _p(p3), _p(np.ascontiguousarray(w_)), len(w_), _p(mm))
+            mom[dt] = mm
+        comp = mom[1].copy()
+        momlib.mom_complement(D, P, _p(comp))
+        assert np.abs(comp[:nm] - mom[0][:nm]).max() <= 1e-14 * np.abs(full[:nm]).max()
