@@ -244,6 +244,17 @@ int xg_subdiv_moments(xg_ctx *ctx, int nel, const double *lset_nodal, int lset_o
                       int order, int p, const double *finv, double *out_neg, double *out_if,
                       uint8_t *dom);
 
+/* Integral of a polynomial (as xg_integrate) on fixed-node rules: weights
+   (nel x 35 FP64, from xg_subdiv_weights with q = 4) at the nodes of the
+   degree-4 Lagrange lattice of each listed tetrahedron, mapped to the
+   physical element: sum_k |det F| W_k f(x_k), equal to the subdivision
+   rule's point sum for polynomials of degree <= 4 (IntegrateX with a P2
+   level set, python_cutint.cpp:21-110).  per_elem (nullable): FP64[nel];
+   total: FP64[1] (overwritten). */
+int xg_integrate_nodes(xg_ctx *ctx, int nel, const int32_t *elems, const int32_t *el2v, const double *coords,
+                       int q, const double *weights, int nterms, const double *coef, const int32_t *expo,
+                       double *per_elem, double *total);
+
 /* ---- element matrices on a moment-form rule ----------------------------
    (symboliccutbfi.cpp:76-304 on any rule, given by its degree-2p barycentric
    moments, e.g. from xg_subdiv_moments): the mass / stiffness matrices of

@@ -48,6 +48,7 @@ _SIGS = {
     "xg_cut_rule_count": [_vp, _i, _i, _vp, _vp, _vp, _i, _i, _vp, _vp],
     "xg_cut_rule_fill": [_vp, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp],
     "xg_integrate": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp],
+    "xg_integrate_nodes": [_vp, _i, _vp, _vp, _vp, _i, _vp, _i, _vp, _vp, _vp, _vp],
     "xg_elmat": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "xg_cut_pipeline": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _i, _vp],
     "xg_subdiv_weights": [_vp, _i, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp],
@@ -290,6 +291,23 @@ class Context:
         arr = (ctypes.c_int * len(dts))(*dts)
         check(lib().xg_cut_pipeline(self.h, D, p, order, nel, ptr(elems), ptr(el2v), ptr(coords),
                                     ptr(lset), len(dts), ctypes.cast(arr, _vp), mask, precision, ptr(out)))
+
+    def prepare_cut_pipeline(self, D, p, order, el2v, coords, lset, dts, mask, precision, out, elems=None):
+        """the same call with its ctypes arguments built once: returns a
+        function that enqueues one launch (for loops that repeat it)"""
+        nel = elems.shape[0] if elems is not None else el2v.shape[0]
+        arr = (ctypes.c_int * len(dts))(*dts)
+        args = (self.h, D, p, order, nel, ptr(elems), ptr(el2v), ptr(coords), ptr(lset), len(dts),
+                ctypes.cast(arr, _vp), mask, precision, ptr(out))
+        f = lib().xg_cut_pipeline
+        keep = (arr, elems, el2v, coords, lset, out)
+
+        def launch():
+            rc = f(*args)
+            if rc != XG_OK:
+                check(rc)
+            return keep
+        return launch
 
 
 class Batch:

@@ -226,6 +226,65 @@ std::vector<double> p1_values(py::object lset, int nv, bool &is_p1) {
   return std::vector<double>(a.data(), a.data() + a.size());
 }
 
+// Level sets that need the subdivision rule (xintegration.cpp:1626-1754):
+// P2 grid functions, or any level set with subdivlvl > 0.  Their element
+// nodal values in this library's P2 dof order (vertices, then the edges
+// (0,1),(0,2),(0,3),(1,2),(1,3),(2,3)) for the listed elements; a P1 level
+// set is lifted to P2 by its edge-midpoint averages.
+bool needs_subdivision(py::object lset, int subdivlvl) {
+  const int order = py::hasattr(lset, "order") ? lset.attr("order").cast<int>() : 1;
+  if (order > 2) throw std::runtime_error("level sets of order > 2 are not supported");
+  return order == 2 || subdivlvl > 0;
+}
+
+std::vector<double> p2_nodal_values(MA ma, py::object lset, const std::vector<int32_t> &el) {
+  if (ma->dim != 3) throw std::runtime_error("subdivision rules (P2 level sets, subdivlvl > 0): tetrahedra only");
+  const int order = py::hasattr(lset, "order") ? lset.attr("order").cast<int>() : 1;
+  py::object vec = py::hasattr(lset, "vec") ? lset.attr("vec") : lset;
+  auto a = py::array_t<double, py::array::c_style | py::array::forcecast>(vec);
+  const double *v = a.data();
+  std::vector<double> out(el.size() * 10);
+  static const int ed[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+  if (order == 2) {
+    py::tuple t = ma->pymesh.attr("dofs")(2);
+    auto dofs = py::array_t<int32_t, py::array::c_style | py::array::forcecast>(t[0]);
+    const int ndof = t[1].cast<int>();
+    if (a.size() != ndof) throw std::runtime_error("P2 level set: expected one value per P2 dof");
+    const int32_t *d = dofs.data();
+    for (size_t i = 0; i < el.size(); ++i)
+      for (int k = 0; k < 10; ++k) out[i * 10 + k] = v[d[(size_t)el[i] * 10 + k]];
+  } else {
+    if (a.size() != ma->nv) throw std::runtime_error("level set: expected one value per vertex");
+    for (size_t i = 0; i < el.size(); ++i) {
+      const int32_t *vv = &ma->h_el2v[(size_t)el[i] * 4];
+      for (int k = 0; k < 4; ++k) out[i * 10 + k] = v[vv[k]];
+      for (int k = 0; k < 6; ++k) out[i * 10 + 4 + k] = 0.5 * (v[vv[ed[k][0]]] + v[vv[ed[k][1]]]);
+    }
+  }
+  return out;
+}
+
+// F^{-1} (row-major, 3 x 3) of the listed tetrahedra, for the interface
+// weights of the subdivision rule
+std::vector<double> tet_finv(MA ma, const std::vector<int32_t> &el) {
+  auto c = py::array_t<double, py::array::c_style | py::array::forcecast>(ma->pymesh.attr("coords"));
+  const double *X = c.data();
+  std::vector<double> out(el.size() * 9);
+  for (size_t i = 0; i < el.size(); ++i) {
+    const int32_t *vv = &ma->h_el2v[(size_t)el[i] * 4];
+    double F[9];
+    for (int r = 0; r < 3; ++r)
+      for (int q = 0; q < 3; ++q) F[r * 3 + q] = X[(size_t)vv[q] * 3 + r] - X[(size_t)vv[3] * 3 + r];
+    const double A = F[4] * F[8] - F[5] * F[7], B = -(F[3] * F[8] - F[5] * F[6]), C = F[3] * F[7] - F[4] * F[6];
+    const double det = F[0] * A + F[1] * B + F[2] * C;
+    double *o = &out[i * 9];
+    o[0] = A / det; o[1] = -(F[1] * F[8] - F[2] * F[7]) / det; o[2] = (F[1] * F[5] - F[2] * F[4]) / det;
+    o[3] = B / det; o[4] = (F[0] * F[8] - F[2] * F[6]) / det; o[5] = -(F[0] * F[5] - F[2] * F[3]) / det;
+    o[6] = C / det; o[7] = -(F[0] * F[7] - F[1] * F[6]) / det; o[8] = (F[0] * F[4] - F[1] * F[3]) / det;
+  }
+  return out;
+}
+
 // ---------------------------------------------------------------- CutInfo
 class CutInformation {
  public:
@@ -270,11 +329,13 @@ class CutInformation {
     if (time_order > 0)
       py::print("Warning: you used CutInfo.Update with time_order > 0. time_order = 0 suffices to detect the "
                 "cut topology and should be prefered.");
+    if (needs_subdivision(lset, subdivlvl)) {
+      UpdateSubdivision(lset, subdivlvl);
+      return;
+    }
+    bnd_marked = true;
     bool is_p1 = true;
     std::vector<double> v = p1_values(lset, ma->nv, is_p1);
-    if (!is_p1 || subdivlvl != 0)
-      throw std::runtime_error("CutInfo.Update: this build marks P1 level sets (subdivlvl = 0); "
-                               "P2 level sets go through xg_subdiv_weights");
     d_lset.upload(v.data(), v.size());
     for (int vb = 0; vb < 2; ++vb) {
       const int ne = ma->GetNE((VorB)vb);
@@ -329,7 +390,79 @@ class CutInformation {
     return nt;
   }
 
-  BA GetElementsOfDomainType(int cdt, VorB vb) const { return vb == VOL ? elems.at(cdt) : selems.at(cdt); }
+  // P2 level sets / subdivlvl > 0 (cutinfo.cpp:80-126 with the subdivision
+  // rule): element domains from the lattice check of the level-L rule
+  // (xg_mark_p2), cut ratios part_vol[NEG] / (part_vol[NEG] + part_vol[POS])
+  // from the rule's weights on the cut elements (xg_subdiv_weights; the POS
+  // part as the NEG part of -phi).  Boundary elements are not marked.
+  void UpdateSubdivision(py::object lset, int L) {
+    if (L > 5) throw std::runtime_error("CutInfo.Update: subdivlvl <= 5 in this build");
+    const int ne = ma->ne;
+    std::vector<int32_t> all(ne);
+    for (int e = 0; e < ne; ++e) all[e] = e;
+    std::vector<double> nod = p2_nodal_values(ma, lset, all);
+    const int nw = (ne + 31) / 32;
+    DevBuf<double> d_nod;
+    d_nod.upload(nod.data(), nod.size());
+    DevBuf<uint8_t> d_dt(std::max(ne, 1));
+    DevBuf<uint32_t> bits((size_t)8 * std::max(nw, 1));
+    DevBuf<int32_t> cl(std::max(ne, 1)), nc(1);
+    check(xg_mark_p2(ctx(), ne, d_nod.p, L, d_dt.p, bits.p, cl.p, nc.p));
+    check(xg_sync(ctx()));
+    int32_t ncut = 0;
+    nc.download(&ncut, 1);
+    std::vector<uint32_t> hb((size_t)8 * nw);
+    bits.download(hb.data(), hb.size());
+    for (int c = 0; c < 8; ++c) {
+      auto &w = elems[c]->words();
+      std::copy(hb.begin() + (size_t)c * nw, hb.begin() + (size_t)(c + 1) * nw, w.begin());
+    }
+    std::vector<uint8_t> dt(ne);
+    d_dt.download(dt.data(), ne);
+    auto &cr = cut_ratio[VOL];
+    cr.assign(ne, 0.0);
+    for (int e = 0; e < ne; ++e) cr[e] = dt[e] == NEG ? 1.0 : 0.0;
+    if (ncut > 0) {
+      std::vector<int32_t> cut(ncut);
+      cl.download(cut.data(), ncut);
+      std::vector<double> nc_neg((size_t)ncut * 10), nc_pos((size_t)ncut * 10);
+      for (int i = 0; i < ncut; ++i)
+        for (int k = 0; k < 10; ++k) {
+          nc_neg[(size_t)i * 10 + k] = nod[(size_t)cut[i] * 10 + k];
+          nc_pos[(size_t)i * 10 + k] = -nod[(size_t)cut[i] * 10 + k];
+        }
+      DevBuf<double> dn, dp, wn((size_t)ncut * 35), wp((size_t)ncut * 35);
+      dn.upload(nc_neg.data(), nc_neg.size());
+      dp.upload(nc_pos.data(), nc_pos.size());
+      check(xg_subdiv_weights(ctx(), ncut, dn.p, 2, L, 0, 4, nullptr, wn.p, nullptr, nullptr));
+      check(xg_subdiv_weights(ctx(), ncut, dp.p, 2, L, 0, 4, nullptr, wp.p, nullptr, nullptr));
+      check(xg_sync(ctx()));
+      std::vector<double> hn((size_t)ncut * 35), hp((size_t)ncut * 35);
+      wn.download(hn.data(), hn.size());
+      wp.download(hp.data(), hp.size());
+      for (int i = 0; i < ncut; ++i) {
+        double a = 0.0, b = 0.0;
+        for (int k = 0; k < 35; ++k) {
+          a += hn[(size_t)i * 35 + k];
+          b += hp[(size_t)i * 35 + k];
+        }
+        cr[cut[i]] = a / (a + b);
+      }
+    }
+    DevBuf<double> d_cr(std::max(ne, 1));
+    d_cr.upload(cr.data(), ne);
+    if (ne) UpdateNodes(d_dt.p, d_cr.p);
+    for (auto &b : selems) b->Clear();
+    cut_ratio[BND].assign(ma->nse, 0.0);
+    bnd_marked = false;
+  }
+  bool bnd_marked = true;
+
+  BA GetElementsOfDomainType(int cdt, VorB vb) const {
+    if (vb == BND && !bnd_marked)
+      throw std::runtime_error("CutInfo: boundary elements are not marked for P2 level sets / subdivlvl > 0");
+    return vb == VOL ? elems.at(cdt) : selems.at(cdt);
+  }
   BA GetFacetsOfDomainType(int cdt) const { return facets.at(cdt); }
 
   // cutinfo.cpp:263-284
@@ -534,6 +667,127 @@ class MultiLevelsetCutInformation {
   }
 };
 
+// polynomial coefficient function -> (coef, exponent triples); false for a
+// general coefficient function
+bool poly_terms_of(py::object cf, std::vector<double> &coef, std::vector<int32_t> &ex) {
+  const bool poly = cf.is_none() || py::isinstance<py::float_>(cf) || py::isinstance<py::int_>(cf) ||
+                    (py::hasattr(cf, "poly_terms") && (!py::hasattr(cf, "is_poly") || cf.attr("is_poly")().cast<bool>()));
+  if (!poly) return false;
+  coef.clear();
+  ex.clear();
+  if (cf.is_none()) {
+    coef = {0.0};
+    ex = {0, 0, 0};
+  } else if (py::isinstance<py::float_>(cf) || py::isinstance<py::int_>(cf)) {
+    coef = {cf.cast<double>()};
+    ex = {0, 0, 0};
+  } else {
+    for (auto t : cf.attr("poly_terms")()) {
+      py::tuple tt = t.cast<py::tuple>();
+      coef.push_back(tt[0].cast<double>());
+      py::tuple e = tt[1].cast<py::tuple>();
+      for (int d = 0; d < 3; ++d) ex.push_back(d < (int)e.size() ? e[d].cast<int>() : 0);
+    }
+  }
+  if (coef.size() > 32) throw std::runtime_error("IntegrateX: polynomial with more than 32 terms");
+  return true;
+}
+
+// IntegrateX on the subdivision rule (P2 level sets, subdivlvl > 0): the rule
+// of each element contracted to weights at the 35 nodes of the degree-4
+// Lagrange lattice (xg_subdiv_weights), which reproduce the rule's point sum
+// for every polynomial of degree <= 4; the POS part is the NEG part of -phi
+py::object IntegrateSubdivision(MA ma, py::object lset, int dt, int L, int order, py::object cf, bool element_wise) {
+  std::vector<double> coef;
+  std::vector<int32_t> ex;
+  if (!poly_terms_of(cf, coef, ex))
+    throw std::runtime_error("IntegrateX with a P2 level set / subdivlvl > 0: polynomial integrands only");
+  for (size_t t = 0; t < coef.size(); ++t)
+    if (ex[3 * t] + ex[3 * t + 1] + ex[3 * t + 2] > 4)
+      throw std::runtime_error("IntegrateX with a P2 level set / subdivlvl > 0: polynomials of degree <= 4");
+  if (L > 5) throw std::runtime_error("IntegrateX: subdivlvl <= 5 in this build");
+  const int ne = ma->ne;
+  std::vector<int32_t> all(ne);
+  for (int e = 0; e < ne; ++e) all[e] = e;
+  std::vector<double> nod = p2_nodal_values(ma, lset, all);
+  if (dt == POS)
+    for (auto &v : nod) v = -v;
+  DevBuf<double> d_nod, d_fi, W((size_t)std::max(ne, 1) * 35), d_pe(std::max(ne, 1)), d_tot(1);
+  d_nod.upload(nod.data(), nod.size());
+  if (dt == IF) {
+    std::vector<double> fi = tet_finv(ma, all);
+    d_fi.upload(fi.data(), fi.size());
+  }
+  if (ne) {
+    check(xg_subdiv_weights(ctx(), ne, d_nod.p, 2, L, order, 4, dt == IF ? d_fi.p : nullptr,
+                            dt == IF ? nullptr : W.p, dt == IF ? W.p : nullptr, nullptr));
+  }
+  check(xg_integrate_nodes(ctx(), ne, nullptr, ma->el2v.p, ma->coords.p, 4, W.p, (int)coef.size(), coef.data(),
+                           ex.data(), d_pe.p, d_tot.p));
+  check(xg_sync(ctx()));
+  std::vector<double> pe(ne);
+  double total = 0.0;
+  d_pe.download(pe.data(), ne);
+  d_tot.download(&total, 1);
+  if (element_wise) return py::array_t<double>((py::ssize_t)pe.size(), pe.data());
+  return py::float_(total);
+}
+
+// element matrices on the subdivision rule (dCut with a P2 level set or
+// subdivlvl > 0): degree-2p moments of the rule (xg_subdiv_moments) on the
+// listed tetrahedra, then the moment-form identities (xg_elmat_moments)
+py::tuple SubdivElementMatrices(py::object mesh, py::object lset, int dt, int L, int order, int p, int mask,
+                                py::object elements, int precision) {
+  MA ma = as_mesh(mesh);
+  if (p < 1 || p > 3) throw std::runtime_error("SubdivElementMatrices: p = 1, 2, 3");
+  if (L > 5) throw std::runtime_error("SubdivElementMatrices: subdivlvl <= 5 in this build");
+  std::vector<int32_t> el;
+  if (elements.is_none()) {
+    el.resize(ma->ne);
+    for (int e = 0; e < ma->ne; ++e) el[e] = e;
+  } else if (py::isinstance<BitArray>(elements)) {
+    const BitArray &b = elements.cast<const BitArray &>();
+    for (size_t e = 0; e < b.Size(); ++e)
+      if (b.Test(e)) el.push_back((int32_t)e);
+  } else {
+    auto a = py::array_t<int32_t, py::array::c_style | py::array::forcecast>(elements);
+    el.assign(a.data(), a.data() + a.size());
+  }
+  const int nel = (int)el.size();
+  const int nm = (2 * p + 3) * (2 * p + 2) * (2 * p + 1) / 6;
+  const int nd = (p + 1) * (p + 2) * (p + 3) / 6;
+  std::vector<double> nod = p2_nodal_values(ma, lset, el);
+  if (dt == POS)
+    for (auto &v : nod) v = -v;
+  std::vector<double> fi = tet_finv(ma, el);
+  DevBuf<double> d_nod, d_fi, d_m((size_t)std::max(nel, 1) * nm);
+  DevBuf<int32_t> d_el;
+  d_nod.upload(nod.data(), nod.size());
+  d_fi.upload(fi.data(), fi.size());
+  d_el.upload(el.data(), el.size());
+  py::array_t<int32_t> elarr((py::ssize_t)nel, el.data());
+  const bool f64 = precision == XG_PREC_FP64;
+  DevBuf<double> o64;
+  DevBuf<float> o32;
+  if (f64) o64.resize((size_t)std::max(nel, 1) * nd * nd);
+  else o32.resize((size_t)std::max(nel, 1) * nd * nd);
+  if (nel) {
+    check(xg_subdiv_moments(ctx(), nel, d_nod.p, 2, L, order, p, d_fi.p, dt == IF ? nullptr : d_m.p,
+                            dt == IF ? d_m.p : nullptr, nullptr));
+    check(xg_elmat_moments(ctx(), 3, p, nel, d_el.p, ma->el2v.p, ma->coords.p, d_m.p, mask, precision,
+                           f64 ? (void *)o64.p : (void *)o32.p));
+  }
+  check(xg_sync(ctx()));
+  if (f64) {
+    py::array_t<double> h({(py::ssize_t)nel, (py::ssize_t)nd, (py::ssize_t)nd});
+    o64.download(h.mutable_data(), (size_t)nel * nd * nd);
+    return py::make_tuple(elarr, h);
+  }
+  py::array_t<float> h({(py::ssize_t)nel, (py::ssize_t)nd, (py::ssize_t)nd});
+  o32.download(h.mutable_data(), (size_t)nel * nd * nd);
+  return py::make_tuple(elarr, h);
+}
+
 // IntegrateX (python_cutint.cpp:21-225) for straight-cut (P1) level sets.
 // cf: an object with poly_terms() -> [(coef, (a, b, c)), ...] runs on the GPU
 // reduction kernel; any other cf must provide evaluate(xyz ndarray) -> values
@@ -552,12 +806,14 @@ py::object IntegrateX(py::dict lsetdom, py::object mesh, py::object cf, py::obje
     MlsetDomain md;
     make_mlset(level_set_values(ma, lset), lsetdom["domain_type"], md);
     build_mlset_rule(ma, md, order, nullptr, r);
+  } else if (needs_subdivision(lset, subdiv)) {
+    if (!ip_container.is_none())
+      throw std::runtime_error("IntegrateX: ip_container with subdivision rules is not supported");
+    return IntegrateSubdivision(ma, lset, parse_dt(lsetdom["domain_type"]), subdiv, order, cf, element_wise);
   } else {
     const int dt = parse_dt(lsetdom["domain_type"]);
     bool is_p1 = true;
     std::vector<double> v = p1_values(lset, ma->nv, is_p1);
-    if (!is_p1 || subdiv != 0)
-      throw std::runtime_error("IntegrateX: this build integrates on P1 level sets (straight cut rules)");
     DevBuf<double> d_l;
     d_l.upload(v.data(), v.size());
     build_rule(ma, d_l.p, dt, order, nullptr, r);
@@ -565,26 +821,9 @@ py::object IntegrateX(py::dict lsetdom, py::object mesh, py::object cf, py::obje
   const int D = ma->dim;
   std::vector<double> pe(ma->ne, 0.0);
   double total = 0.0;
-  const bool poly = cf.is_none() || py::isinstance<py::float_>(cf) || py::isinstance<py::int_>(cf) ||
-                    (py::hasattr(cf, "poly_terms") && (!py::hasattr(cf, "is_poly") || cf.attr("is_poly")().cast<bool>()));
-  if (poly) {
-    std::vector<double> coef;
-    std::vector<int32_t> ex;
-    if (cf.is_none()) {
-      coef = {0.0};
-      ex = {0, 0, 0};
-    } else if (py::isinstance<py::float_>(cf) || py::isinstance<py::int_>(cf)) {
-      coef = {cf.cast<double>()};
-      ex = {0, 0, 0};
-    } else {
-      for (auto t : cf.attr("poly_terms")()) {
-        py::tuple tt = t.cast<py::tuple>();
-        coef.push_back(tt[0].cast<double>());
-        py::tuple e = tt[1].cast<py::tuple>();
-        for (int d = 0; d < 3; ++d) ex.push_back(d < (int)e.size() ? e[d].cast<int>() : 0);
-      }
-    }
-    if (coef.size() > 32) throw std::runtime_error("IntegrateX: polynomial with more than 32 terms");
+  std::vector<double> coef;
+  std::vector<int32_t> ex;
+  if (poly_terms_of(cf, coef, ex)) {
     DevBuf<double> d_pe(ma->ne), d_tot(1);
     check(xg_integrate(ctx(), D, ma->ne, nullptr, ma->el2v.p, ma->coords.p, r.offsets.p, r.pts.p, (int)coef.size(),
                        coef.data(), ex.data(), d_pe.p, d_tot.p));
@@ -1064,6 +1303,10 @@ PYBIND11_MODULE(xfem, m) {
         "device scatter of element matrices into a CSR pattern (FP64 values)");
   m.def("AssembleVector", &AssembleVector, py::arg("dofs"), py::arg("elvecs"), py::arg("n"),
         "device scatter of element vectors");
+  m.def("SubdivElementMatrices", &SubdivElementMatrices, py::arg("mesh"), py::arg("levelset"),
+        py::arg("domain_type"), py::arg("subdivlvl"), py::arg("order"), py::arg("p"), py::arg("mask") = 3,
+        py::arg("elements") = py::none(), py::arg("precision") = XG_PREC_FP32,
+        "element matrices of (mass | laplace) on the subdivision rule of a P2 level set (tetrahedra)");
   m.def("MlsetElementMatrices", &MlsetElementMatrices, py::arg("mesh"), py::arg("levelsets"),
         py::arg("domain_type"), py::arg("order"), py::arg("p"), py::arg("mask") = 3,
         py::arg("elements") = py::none(), py::arg("precision") = XG_PREC_FP32,

@@ -21,11 +21,15 @@
 // Mapping: one lane per (element, domain type); a block holds 256 elements
 // of one domain type, so the number of base-rule points per sub-simplex is
 // block-uniform.  The decomposition table and the base rules are staged in
-// LDS once per block.  The block then sorts its elements by the number of
-// sub-simplices their rule sums (0..3, xg_moment_core.hpp make_plan) with a
-// counting sort in LDS, so that a wave's lanes run the same number of point
-// loops instead of all running the largest.  Occupancy is set by the VGPR
-// count of the moment accumulators.
+// LDS once per block.  Each lane gathers its element into LDS, then the
+// block sorts its elements by the number of sub-simplices their rule sums
+// (0..3, xg_moment_core.hpp make_plan) with a counting sort in LDS, so that a
+// wave's lanes run the same number of point loops instead of all running the
+// largest; the lanes then take the elements in that order from LDS (no second
+// gather).  Occupancy is set by the VGPR count of the moment accumulators.
+// (One lane per element with all its domain types in turn shares the gather
+// and the cut points, but halves the number of waves: on the 64^3 workload
+// that left most SIMDs with a single wave and was 25 % slower.)
 #include "xg_device.hpp"
 #include "xg_elmat_core.hpp"
 #include "xg_internal.hpp"
@@ -45,9 +49,11 @@ __global__ __launch_bounds__(256) void k_moment_pipeline(int nel, const int32_t 
                                                          int dts, int mask, T *__restrict__ out,
                                                          int32_t *err) {
   constexpr int N = ndof(D, P);
+  constexpr int NG = (D + 1) * (D + 1);  // geometry doubles per element: D+1 vertices (D coords) + D+1 values
   extern __shared__ char smem[];
   __shared__ int cnt[4];
   __shared__ int perm[256];
+  __shared__ double geo[NG][256];
   const int tid = threadIdx.x;
   if (tid < 4) cnt[tid] = 0;
   const MomLds S = stage_lds<D, P>(smem, order);  // block barrier: before any exit
@@ -56,14 +62,19 @@ __global__ __launch_bounds__(256) void k_moment_pipeline(int nel, const int32_t 
   const int base = (blockIdx.x - t * ntiles) * 256;
   const int dt = (dts >> (2 * t)) & 3;
   const int nitems = min(256, nel - base);
-  // counting sort of the block's elements by the number of sub-simplices
-  const int i0 = base + tid;
+  // phase 0: each lane gathers its own element into LDS and computes its
+  // sort key, the number of sub-simplices its rule sums
   int key = 0, pos = 0;
   if (tid < nitems) {
-    const int e = elems ? elems[i0] : i0;
-    double l[D + 1];
+    const int e = elems ? elems[base + tid] : base + tid;
+    double V[D + 1][3], l[D + 1];
+    ecore::load_geom<D>(el2v, coords, lset, e, V, l);
 #pragma unroll
-    for (int k = 0; k <= D; ++k) l[k] = lset[el2v[(size_t)e * (D + 1) + k]];
+    for (int k = 0; k <= D; ++k) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) geo[k * D + d][tid] = V[k][d];
+      geo[(D + 1) * D + k][tid] = l[k];
+    }
     key = make_plan<D>(S, l, dt).ns;
     pos = atomicAdd(&cnt[key], 1);
   }
@@ -72,14 +83,20 @@ __global__ __launch_bounds__(256) void k_moment_pipeline(int nel, const int32_t 
     int off = 0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) off += (k < key) ? cnt[k] : 0;
-    perm[off + pos] = i0;
+    perm[off + pos] = tid;
   }
   __syncthreads();
   if (tid >= nitems) return;  // no cross-lane operations below
-  const int i = perm[tid];
-  const int e = elems ? elems[i] : i;
+  // phase 1: the lanes take the elements in sorted order, from LDS
+  const int li = perm[tid];
+  const int i = base + li;
   double V[D + 1][3], l[D + 1];
-  ecore::load_geom<D>(el2v, coords, lset, e, V, l);
+#pragma unroll
+  for (int k = 0; k <= D; ++k) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) V[k][d] = d < D ? geo[k * D + d][li] : 0.0;
+    l[k] = geo[(D + 1) * D + k][li];
+  }
   double F[D * D], Finv[D * D], absdet;
   jacobian<D>(V, F, Finv, absdet);
   bool bad;

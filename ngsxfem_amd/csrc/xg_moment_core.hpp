@@ -259,40 +259,65 @@ __device__ __forceinline__ Plan make_plan(const MomLds &S, const double *l, int 
   return pl;
 }
 
+// What the domain types of one element share: the element domain, the
+// sign pattern and the cut points of the sign-changing edges (their order,
+// lexicographic in the edge, is the same in every table entry of that
+// pattern).
+struct ElemCut {
+  int eldom;  // CheckIfStraightCut of the element
+  int msk;    // sign pattern (bit i: l_i >= 0), valid when eldom == IF
+  double tc0, tc1, tc2, tc3;
+};
+
+template <int D>
+__device__ __forceinline__ ElemCut elem_cut(const MomLds &S, const double *l) {
+  ElemCut c;
+  c.eldom = check_if_straight_cut<D>(l);
+  c.msk = 0;
+  c.tc0 = c.tc1 = c.tc2 = c.tc3 = 0.0;
+  if (c.eldom != IF || !cut_is_valid<D>(l)) return c;
+  c.msk = sign_mask<D>(l);
+  PackedEntry E;
+  const uint32_t *src = reinterpret_cast<const uint32_t *>(&S.tab[IF * 16 + c.msk]);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) E.w[q] = src[q];
+  const int nc = E.nc();
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (k >= nc) break;
+    const int a = E.ce(k, 0), b = E.ce(k, 1);
+    double la = l[0], lb = l[0];
+#pragma unroll
+    for (int j = 1; j <= D; ++j) {
+      la = (j == a) ? l[j] : la;
+      lb = (j == b) ? l[j] : lb;
+    }
+    const double t = la / (la - lb);  // straightcutrule.cpp:50
+    c.tc0 = (k == 0) ? t : c.tc0;
+    c.tc1 = (k == 1) ? t : c.tc1;
+    c.tc2 = (k == 2) ? t : c.tc2;
+    c.tc3 = (k == 3) ? t : c.tc3;
+  }
+  return c;
+}
+
 // The degree-2P moments of the cut rule of one (element, domain type) (rule
-// topology, sub-simplex volumes, interface factor, point loop).  Returns true
-// where the reference throws (straightcutrule.cpp:45-49, :148, :193).
+// topology, sub-simplex volumes, interface factor, point loop), given the
+// element's shared cut data.  Returns true where the reference throws
+// (straightcutrule.cpp:45-49, :148, :193).
 template <int D, int P>
-__device__ __forceinline__ bool moment_accumulate(const MomLds &S, const double *l, const double *Finv, int dt,
-                                                  double *m) {
+__device__ __forceinline__ bool moment_accumulate_ec(const MomLds &S, const double *l, const ElemCut &ec,
+                                                     const double *Finv, int dt, double *m) {
   using MM = Moments<D, P>;
   const Plan pl = make_plan<D>(S, l, dt);
   const int nvs = (dt == IF) ? D : D + 1;  // vertices per sub-simplex (wave-uniform)
   PackedEntry E;
 #pragma unroll
   for (int q = 0; q < 6; ++q) E.w[q] = 0u;
-  double tc0 = 0.0, tc1 = 0.0, tc2 = 0.0, tc3 = 0.0;
   if (pl.cut) {
-    const uint32_t *src = reinterpret_cast<const uint32_t *>(&S.tab[pl.edt * 16 + sign_mask<D>(l)]);
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(&S.tab[pl.edt * 16 + ec.msk]);
 #pragma unroll
     for (int q = 0; q < 6; ++q) E.w[q] = src[q];
-    const int nc = E.nc();
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      if (c >= nc) break;
-      const int a = E.ce(c, 0), b = E.ce(c, 1);
-      double la = l[0], lb = l[0];
-#pragma unroll
-      for (int k = 1; k <= D; ++k) {
-        la = (k == a) ? l[k] : la;
-        lb = (k == b) ? l[k] : lb;
-      }
-      const double tc = la / (la - lb);
-      tc0 = (c == 0) ? tc : tc0;
-      tc1 = (c == 1) ? tc : tc1;
-      tc2 = (c == 2) ? tc : tc2;
-      tc3 = (c == 3) ? tc : tc3;
-    }
   }
   const int ns = pl.ns;
   const double ifac = (dt == IF && ns > 0) ? interface_factor<D>(l, Finv) : 1.0;
@@ -316,7 +341,7 @@ __device__ __forceinline__ bool moment_accumulate(const MomLds &S, const double 
 #pragma unroll
       for (int k = 0; k <= D; ++k) {
         if (k < nvs) {
-          piece_vertex<D>(E.v(s, k), E, tc0, tc1, tc2, tc3, W[k]);
+          piece_vertex<D>(E.v(s, k), E, ec.tc0, ec.tc1, ec.tc2, ec.tc3, W[k]);
         } else {
 #pragma unroll
           for (int d = 0; d < 3; ++d) W[k][d] = 0.0;
@@ -335,15 +360,18 @@ __device__ __forceinline__ bool moment_accumulate(const MomLds &S, const double 
   return pl.bad;
 }
 
-// One (element, domain type): rule topology, moments, identities; writes the
-// ndof x ndof matrix to out.  Returns true where the reference throws
-// (straightcutrule.cpp:45-49, :148, :193).
+template <int D, int P>
+__device__ __forceinline__ bool moment_accumulate(const MomLds &S, const double *l, const double *Finv, int dt,
+                                                  double *m) {
+  const ElemCut ec = elem_cut<D>(S, l);
+  return moment_accumulate_ec<D, P>(S, l, ec, Finv, dt, m);
+}
+
+// the mass / stiffness entries of one (element, domain type) from its
+// moments (generated identities); writes the ndof x ndof matrix to out
 template <int D, int P, typename T>
-__device__ __forceinline__ bool moment_elmat(const MomLds &S, const double *l, const double *Finv, double absdet,
-                                             int dt, int mask, T *out) {
+__device__ __forceinline__ void moment_forms(const double *m, const double *Finv, double absdet, int mask, T *out) {
   using MM = Moments<D, P>;
-  double m[MM::NM];
-  const bool bad = moment_accumulate<D, P>(S, l, Finv, dt, m);
   double mk[MM::NK];
   MM::lower(m, mk);
   // Gram matrix of the physical barycentric gradients: grad lam_i = row i
@@ -372,6 +400,17 @@ __device__ __forceinline__ bool moment_elmat(const MomLds &S, const double *l, c
     }
   const double sm = (mask & 1) ? absdet : 0.0, sk = (mask & 2) ? absdet : 0.0;
   MM::template forms<T>(m, mk, Hs, sm, sk, out);
+}
+
+// One (element, domain type): rule topology, moments, identities; writes the
+// ndof x ndof matrix to out.  Returns true where the reference throws
+// (straightcutrule.cpp:45-49, :148, :193).
+template <int D, int P, typename T>
+__device__ __forceinline__ bool moment_elmat(const MomLds &S, const double *l, const double *Finv, double absdet,
+                                             int dt, int mask, T *out) {
+  double m[Moments<D, P>::NM];
+  const bool bad = moment_accumulate<D, P>(S, l, Finv, dt, m);
+  moment_forms<D, P, T>(m, Finv, absdet, mask, out);
   return bad;
 }
 

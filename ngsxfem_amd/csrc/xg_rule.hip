@@ -172,6 +172,55 @@ __global__ __launch_bounds__(256) void k_integrate(int nel, const int32_t *__res
   if (lane == 0 && total) atomicAdd(total, acc_all);
 }
 
+// integrals on fixed-node rules: W (nel x N) weights at the N nodes of the
+// degree-Q Lagrange lattice of the reference tetrahedron (xg_subdiv_weights);
+// one thread per element
+template <int Q>
+__global__ __launch_bounds__(256) void k_integrate_nodes(int nel, const int32_t *__restrict__ elems,
+                                                         const int32_t *__restrict__ el2v,
+                                                         const double *__restrict__ coords,
+                                                         const double *__restrict__ W, Poly f,
+                                                         double *__restrict__ per_elem, double *total) {
+  constexpr Lattice<3, Q> LAT{};
+  constexpr int N = Lattice<3, Q>::N;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double s = 0.0;
+  if (i < nel) {
+    const int e = elems ? elems[i] : i;
+    double V[4][3];
+#pragma unroll
+    for (int k = 0; k <= 3; ++k) {
+      const int v = el2v[(size_t)e * 4 + k];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) V[k][d] = coords[(size_t)v * 3 + d];
+    }
+    double F[9], Finv[9], absdet;
+    jacobian<3>(V, F, Finv, absdet);
+    for (int k = 0; k < N; ++k) {
+      double x[3];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        double a = V[3][r];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a += F[r * 3 + c] * ((double)LAT.a[k][c] / Q);
+        x[r] = a;
+      }
+      double fv = 0.0;
+      for (int t = 0; t < f.n; ++t) {
+        double m = f.c[t];
+        for (int d = 0; d < 3; ++d)
+          for (int j = 0; j < f.ex[t][d]; ++j) m *= x[d];
+        fv += m;
+      }
+      s += absdet * W[(size_t)i * N + k] * fv;
+    }
+    if (per_elem) per_elem[i] = s;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0 && total) atomicAdd(total, s);
+}
+
 int grid_for_waves(int nel) {
   long long blocks = ((long long)nel * 64 + 255) / 256;
   if (blocks > 65536) blocks = 65536;
@@ -249,6 +298,30 @@ extern "C" int xg_integrate(xg_ctx *ctx, int D, int nel, const int32_t *elems, c
     case 2: k_integrate<2><<<grid, 256, 0, ctx->stream>>>(nel, elems, el2v, coords, offsets, p, f, per_elem, total); break;
     default: k_integrate<3><<<grid, 256, 0, ctx->stream>>>(nel, elems, el2v, coords, offsets, p, f, per_elem, total); break;
   }
+  XG_LAUNCH_CHECK();
+  return XG_OK;
+}
+
+extern "C" int xg_integrate_nodes(xg_ctx *ctx, int nel, const int32_t *elems, const int32_t *el2v,
+                                  const double *coords, int q, const double *weights, int nterms,
+                                  const double *coef, const int32_t *expo, double *per_elem, double *total) {
+  if (!ctx || nel < 0 || q != 4 || nterms < 0 || nterms > 32 || (nterms > 0 && (!coef || !expo)))
+    return set_error(XG_ERR_ARG, "xg_integrate_nodes: bad argument (q == 4)");
+  if (total) XG_CHECK(hipMemsetAsync(total, 0, sizeof(double), ctx->stream));
+  if (nel == 0) return XG_OK;
+  if (!el2v || !coords || !weights) return set_error(XG_ERR_ARG, "xg_integrate_nodes: null pointer");
+  Poly f;
+  f.n = nterms;
+  for (int t = 0; t < nterms; ++t) {
+    f.c[t] = coef[t];
+    for (int d = 0; d < 3; ++d) {
+      if (expo[3 * t + d] < 0 || expo[3 * t + d] > 64)
+        return set_error(XG_ERR_ARG, "xg_integrate_nodes: bad exponent");
+      f.ex[t][d] = expo[3 * t + d];
+    }
+  }
+  k_integrate_nodes<4><<<(nel + 255) / 256, 256, 0, ctx->stream>>>(nel, elems, el2v, coords, weights, f, per_elem,
+                                                                   total);
   XG_LAUNCH_CHECK();
   return XG_OK;
 }

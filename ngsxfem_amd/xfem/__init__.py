@@ -33,6 +33,7 @@ from .xfem import (ANY, BND, CDOM_IF, CDOM_NEG, CDOM_POS, FALLBACK, FIRST, HASNE
                    CutElementMatrices, CutElementVectors, CutInfo, DOMAIN_TYPE, GetDofsOfElements,
                    GetDofsOfFacets, GetElementsWithNeighborFacets, GetElementsWithSharedVertex,
                    GetFacetsWithNeighborTypes, IntegrateX, MlsetElementMatrices, MultiLevelsetCutInfo, NODE_TYPE,
+                   SubdivElementMatrices,
                    NT_CELL, NT_EDGE, NT_ELEMENT, NT_FACE, NT_FACET, NT_VERTEX, PerturbP1, QUAD_DIRECTION_POLICY,
                    VorB)
 
@@ -227,21 +228,34 @@ class GridFunction:
     def __init__(self, space, name="gf"):
         self.space = space
         self.order = space.order
-        self.vec = np.zeros(space.mesh.nv if space.order == 1 else 0)
+        self.vec = np.zeros(space.ndof)
 
     def Set(self, cf):
-        """nodal interpolation at the vertices (P1)"""
-        self.vec = _eval_at_vertices(cf, self.space.mesh)
+        """nodal interpolation: at the vertices (P1), at the vertices and edge
+        midpoints (P2, this build's dof numbering)"""
+        if self.order == 1:
+            self.vec = _eval_at_vertices(cf, self.space.mesh)
+        elif self.order == 2:
+            mesh = self.space.mesh
+            e2v, _ = mesh.edges()
+            X = np.vstack([mesh.coords, 0.5 * (mesh.coords[e2v[:, 0]] + mesh.coords[e2v[:, 1]])])
+            self.vec = _eval_at_points(cf, X)
+        else:
+            raise NotImplementedError("GridFunction.Set: orders 1 and 2")
+
+
+def _eval_at_points(cf, P):
+    X = np.zeros((P.shape[0], 3))
+    X[:, :P.shape[1]] = P
+    if isinstance(cf, CoefficientFunction):
+        return np.asarray(cf.evaluate(X), dtype=np.float64)
+    if callable(cf):
+        return np.array([cf(*p) for p in P], dtype=np.float64)
+    return np.full(P.shape[0], float(cf))
 
 
 def _eval_at_vertices(cf, mesh):
-    X = np.zeros((mesh.nv, 3))
-    X[:, :mesh.dim] = mesh.coords
-    if isinstance(cf, CoefficientFunction):
-        return cf.evaluate(X)
-    if callable(cf):
-        return np.array([cf(*p) for p in mesh.coords], dtype=np.float64)
-    return np.full(mesh.nv, float(cf))
+    return _eval_at_points(cf, mesh.coords)
 
 
 def InterpolateToP1(gf_or_cf, gf_p1, eps_perturbation=None, heapsize=1000000):
@@ -402,8 +416,6 @@ class _CutIntegral:
             lset = np.full(mesh.nv, -1.0)
             return lset, NEG, (m.order if m.order is not None else 2 * p), m.definedonelements
         d = m.lsetdom
-        if d.get("subdivlvl", 0):
-            raise NotImplementedError("subdivlvl > 0 element matrices: use the P2 subdivision kernels")
         if isinstance(d["levelset"], (tuple, list)):
             return tuple(d["levelset"]), d["domain_type"], d.get("order", 2 * p), m.definedonelements
         return d["levelset"], int(d["domain_type"]), d.get("order", 2 * p), m.definedonelements
@@ -412,10 +424,15 @@ class _CutIntegral:
         lset, dt, order, els = self._rule_args(mesh, p)
         mask = (1 if "mass" in self.integrand.parts else 0) | (2 if "laplace" in self.integrand.parts else 0)
         coefs = self.integrand.parts
+        L = self.measure.lsetdom.get("subdivlvl", 0) if isinstance(self.measure, _CutDifferentialSymbol) else 0
         if isinstance(lset, (tuple, list)):
             # several level sets: dt holds the domain tuples
             def kernel(msk):
                 return MlsetElementMatrices(mesh, tuple(lset), dt, order, p, msk, els, precision)
+        elif L > 0 or getattr(lset, "order", 1) == 2:
+            # subdivision rule (P2 level sets, subdivlvl > 0)
+            def kernel(msk):
+                return SubdivElementMatrices(mesh, lset, dt, L, order, p, msk, els, precision)
         else:
             def kernel(msk):
                 return CutElementMatrices(mesh, lset, dt, order, p, msk, els, precision)
@@ -435,6 +452,8 @@ class _CutIntegral:
     def integrate(self, mesh, element_wise=False):
         lset, dt, order, els = self._rule_args(mesh, 0)
         lsd = {"levelset": lset, "domain_type": dt, "order": order if order is not None else 5}
+        if isinstance(self.measure, _CutDifferentialSymbol) and self.measure.lsetdom.get("subdivlvl", 0):
+            lsd["subdivlvl"] = self.measure.lsetdom["subdivlvl"]
         r = IntegrateX(levelset_domain=lsd, mesh=mesh, cf=self.integrand, element_wise=True)
         if els is not None:
             r = r * np.asarray(els.numpy(), dtype=np.float64)
