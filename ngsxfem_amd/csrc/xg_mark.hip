@@ -34,9 +34,26 @@ __device__ __forceinline__ void load_lset(const int32_t *__restrict__ el2v,
   }
 }
 
+// Coordinate d of point idx of the cut point array: reference vertex k < D
+// is e_k, vertex D the origin, D+1+c the c-th cut point (the layout of
+// cut_point_array).  Selects over registers, no indexed private arrays.
+template <int D>
+__device__ __forceinline__ double point_coord(int idx, int d, const double (&cp)[4][3]) {
+  double v = (idx == d && d < D) ? 1.0 : 0.0;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) v = (idx == D + 1 + c) ? cp[c][d] : v;
+  return v;
+}
+
 // part_vol[NEG], part_vol[POS] from the order-0 rules (cutinfo.cpp:87-101):
 // the weight sums of StraightCutIntegrationRule(..., np, order 0).  Returns
-// false when the reference would throw.
+// false when the reference would throw.  Same decomposition (the compile-time
+// table of decompose(), by sign pattern), same cut points
+// (straightcutrule.cpp:50) and the same volume arithmetic
+// (SimpleX::GetVolume, :66-72) as cut_point_array / simplex_volume, with the
+// point array held in registers: the indexed arrays of the generic helpers
+// would live in per-lane scratch memory, which every wave of this
+// memory-bound kernel would then have to set up.
 template <int D>
 __device__ __forceinline__ bool part_volumes(const double *l, double *pv) {
   const double w0 = xg_rule_data_d[xg_rule_off_d[D][1]][3];  // order-0 weight = |ref simplex|
@@ -47,91 +64,252 @@ __device__ __forceinline__ bool part_volumes(const double *l, double *pv) {
     pv[eldom] = w0;
     return true;
   }
+  // decompose(): every sign-changing pair must be a straight cut (no exact
+  // zero next to a negative value, straightcutrule.cpp:45-49)
+  bool hz = false, hn = false;
+#pragma unroll
+  for (int i = 0; i <= D; ++i) {
+    hz |= l[i] == 0.0;
+    hn |= l[i] < 0.0;
+  }
+  const bool pairs_ok = !(hz && hn);
+  const int msk = sign_mask<D>(l);
+  // cut points of the sign-changing edges (same list for both parts)
+  const DecompEntry &E0 = xg_decomp_d.e[D - 1][POS][msk];
+  double cp[4][3];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int i = E0.ce[c][0], j = E0.ce[c][1];
+    double li = l[0], lj = l[0];
+#pragma unroll
+    for (int k = 1; k <= D; ++k) {
+      li = (k == i) ? l[k] : li;
+      lj = (k == j) ? l[k] : lj;
+    }
+    const double t = (c < E0.nc) ? li / (li - lj) : 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double pi = (d == i && i < D) ? 1.0 : 0.0;
+      const double pj = (d == j && j < D) ? 1.0 : 0.0;
+      cp[c][d] = pi + t * (pj - pi);
+    }
+  }
   bool ok = true;
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     const int np = t == 0 ? POS : NEG;
-    Decomp dc;
-    if (!decompose<D>(l, np, dc)) {
+    const DecompEntry &E = xg_decomp_d.e[D - 1][np][msk];
+    if (!(pairs_ok && E.ok)) {
       ok = false;
       continue;
     }
-    double P[D + 1 + 4][3];
-    cut_point_array<D>(l, dc, P);
     double s = 0.0;
-    for (int k = 0; k < dc.ns; ++k) s += w0 * simplex_volume(dc.nv, dc.v[k], P);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (k >= E.ns) break;
+      double Q[4][3];
+#pragma unroll
+      for (int m = 0; m <= D; ++m)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) Q[m][d] = point_coord<D>(E.v[k][m], d, cp);
+      double vol;
+      if constexpr (D == 1) {
+        const double a[3] = {Q[1][0] - Q[0][0], Q[1][1] - Q[0][1], Q[1][2] - Q[0][2]};
+        vol = norm3(a);
+      } else if constexpr (D == 2) {
+        double a[3], b[3], c[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          a[d] = Q[2][d] - Q[0][d];
+          b[d] = Q[1][d] - Q[0][d];
+        }
+        c[0] = a[1] * b[2] - a[2] * b[1];
+        c[1] = a[2] * b[0] - a[0] * b[2];
+        c[2] = a[0] * b[1] - a[1] * b[0];
+        vol = norm3(c);
+      } else {
+        double a[3], b[3], c[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          a[d] = Q[3][d] - Q[0][d];
+          b[d] = Q[2][d] - Q[0][d];
+          c[d] = Q[1][d] - Q[0][d];
+        }
+        vol = fabs(det3(a, b, c));
+      }
+      s += w0 * vol;
+    }
     pv[np] = s;
   }
   return ok;
 }
 
-template <int D>
-__global__ __launch_bounds__(BLK) void k_mark(int ne, const int32_t *__restrict__ el2v,
-                                              const double *__restrict__ lset,
-                                              uint8_t *__restrict__ dt_out,
-                                              double *__restrict__ cut_ratio,
-                                              uint32_t *__restrict__ bits, int nw,
-                                              int32_t *__restrict__ blk_if, int32_t *err) {
-  __shared__ int s_cnt[BLK / 64];
-  const int e = blockIdx.x * BLK + threadIdx.x;
-  const bool valid = e < ne;
-  int cdt = 0;
-  if (valid) {
-    double l[D + 1];
-    load_lset<D>(el2v, lset, e, l);
-    double pv[2];
-    if (!part_volumes<D>(l, pv)) atomicOr(err, 1);
-    if (cut_ratio) cut_ratio[e] = pv[NEG] / (pv[NEG] + pv[POS]);
-    // cutinfo.cpp:104-124
-    if (pv[NEG] > 0.0)
-      cdt = (pv[POS] > 0.0) ? 4 : 1;
-    else
-      cdt = 2;
-    if (dt_out) dt_out[e] = (uint8_t)(cdt == 4 ? IF : (cdt == 1 ? NEG : POS));
-  }
-  const int lane = threadIdx.x & 63;
-  const int wave_e0 = e - lane;  // multiple of 64
+// Marking kernel.  A block of NB threads marks NB * EPT consecutive
+// elements, EPT per thread (element base + k * NB + tid, so every wave's
+// elements stay 64 consecutive ones for the bit-array ballots): a thread
+// first issues the element rows of all its elements, then all level-set
+// gathers, so that EPT rows and (D+1) * EPT values per lane are in flight at
+// once -- the kernel is bound by memory latency otherwise.  Elements with a
+// sign change (the only ones that need the cut decomposition) are queued in
+// LDS and processed densely by the first lanes of the block: they cluster in
+// few elements per block, and without the queue every wave holding one of
+// them would run the whole decomposition path.
+constexpr int MARK_BLK = 256;
+constexpr int MARK_EPT = 4;
+
+template <int D, int NB, int EPT>
+__global__ __launch_bounds__(NB) void k_mark(int ne, const int32_t *__restrict__ el2v,
+                                             const double *__restrict__ lset,
+                                             uint8_t *__restrict__ dt_out,
+                                             double *__restrict__ cut_ratio,
+                                             uint32_t *__restrict__ bits, int nw,
+                                             int32_t *__restrict__ blk_if, int32_t *err) {
+  constexpr int NE = NB * EPT;  // elements per block
+  __shared__ int s_cnt[NB / 64];
+  __shared__ int s_nq;
+  __shared__ int16_t s_qi[NE];  // block-local element of queue slot q
+  __shared__ uint8_t s_cdt[NE];  // combined domain bits of queued elements
+  const int tid = threadIdx.x, lane = tid & 63;
+  if (tid == 0) s_nq = 0;
+  const int base = blockIdx.x * NE;
+  // element rows of all EPT elements, then their level-set values.  The
+  // loads are unconditional (lanes past the end read the last element
+  // again): a load under a branch is waited for at the branch's end, which
+  // would serialise the EPT round trips.
+  int vi[EPT][D + 1];
 #pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    // combined arrays (cutinfo.cpp:72-78,127-132): bit set iff (c & cdt) != 0,
-    // CDOM_ANY everywhere, CDOM_NO nowhere
-    const bool on = valid && ((c == 7) || (c & cdt));
-    const unsigned long long m = __ballot(on);
-    if (lane == 0) {
-      const int w = wave_e0 >> 5;
-      uint32_t *row = bits + (size_t)c * nw;
-      if (w < nw) row[w] = (uint32_t)(m & 0xffffffffull);
-      if (w + 1 < nw) row[w + 1] = (uint32_t)(m >> 32);
+  for (int k = 0; k < EPT; ++k) {
+    const int ec = min(base + k * NB + tid, ne - 1);
+    if constexpr (D == 3) {
+      const int4 v = reinterpret_cast<const int4 *>(el2v)[ec];
+      vi[k][0] = v.x;
+      vi[k][1] = v.y;
+      vi[k][2] = v.z;
+      vi[k][3] = v.w;
+    } else {
+#pragma unroll
+      for (int i = 0; i <= D; ++i) vi[k][i] = el2v[(size_t)ec * (D + 1) + i];
     }
   }
-  const unsigned long long mif = __ballot(valid && cdt == 4);
-  if (lane == 0) s_cnt[threadIdx.x >> 6] = __popcll(mif);
+  double l[EPT][D + 1];
+#pragma unroll
+  for (int k = 0; k < EPT; ++k)
+#pragma unroll
+    for (int i = 0; i <= D; ++i) l[k][i] = lset[vi[k][i]];
+  __syncthreads();  // s_nq cleared
+  int cdt[EPT];
+  bool cand[EPT];
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int e = base + k * NB + tid;
+    cdt[k] = 0;
+    cand[k] = false;
+    if (e < ne) {
+      const int eldom = check_if_straight_cut<D>(l[k]);
+      if (eldom != IF) {
+        // one part holds the whole order-0 weight: cut ratio w0/(w0+0) = 1
+        // or 0/(0+w0) = 0 (cutinfo.cpp:87-124)
+        cdt[k] = eldom == NEG ? 1 : 2;
+        if (cut_ratio) cut_ratio[e] = eldom == NEG ? 1.0 : 0.0;
+        if (dt_out) dt_out[e] = (uint8_t)eldom;
+      } else {
+        cand[k] = true;
+      }
+    }
+    const unsigned long long mc = __ballot(cand[k]);
+    if (mc) {  // wave-uniform
+      int q0 = 0;
+      if (lane == 0) q0 = atomicAdd(&s_nq, __popcll(mc));
+      q0 = __shfl(q0, 0, 64);
+      if (cand[k])
+        s_qi[q0 + __builtin_amdgcn_mbcnt_hi((uint32_t)(mc >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mc, 0))] =
+            (int16_t)(k * NB + tid);
+    }
+  }
   __syncthreads();
-  if (threadIdx.x == 0 && blk_if) {
+  const int nq = s_nq;
+  for (int q = tid; q < nq; q += NB) {
+    const int li = s_qi[q];
+    const int eq = base + li;
+    double lq[D + 1];
+    load_lset<D>(el2v, lset, eq, lq);  // cache hits: gathered above
+    double pv[2];
+    if (!part_volumes<D>(lq, pv)) atomicOr(err, 1);
+    if (cut_ratio) cut_ratio[eq] = pv[NEG] / (pv[NEG] + pv[POS]);
+    // cutinfo.cpp:104-124
+    int c;
+    if (pv[NEG] > 0.0)
+      c = (pv[POS] > 0.0) ? 4 : 1;
+    else
+      c = 2;
+    if (dt_out) dt_out[eq] = (uint8_t)(c == 4 ? IF : (c == 1 ? NEG : POS));
+    s_cdt[li] = (uint8_t)c;
+  }
+  __syncthreads();
+  int nif = 0;
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int e = base + k * NB + tid;
+    const bool valid = e < ne;
+    if (cand[k]) cdt[k] = s_cdt[k * NB + tid];
+    const int wave_e0 = e - lane;  // multiple of 64
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      // combined arrays (cutinfo.cpp:72-78,127-132): bit set iff
+      // (c & cdt) != 0, CDOM_ANY everywhere, CDOM_NO nowhere
+      const bool on = valid && ((c == 7) || (c & cdt[k]));
+      const unsigned long long m = __ballot(on);
+      if (lane == 0) {
+        const int w = wave_e0 >> 5;
+        uint32_t *row = bits + (size_t)c * nw;
+        if (w < nw) row[w] = (uint32_t)(m & 0xffffffffull);
+        if (w + 1 < nw) row[w + 1] = (uint32_t)(m >> 32);
+      }
+    }
+    nif += __popcll(__ballot(valid && cdt[k] == 4));
+  }
+  if (lane == 0) s_cnt[tid >> 6] = nif;
+  __syncthreads();
+  if (tid == 0 && blk_if) {
     int s = 0;
 #pragma unroll
-    for (int k = 0; k < BLK / 64; ++k) s += s_cnt[k];
+    for (int k = 0; k < NB / 64; ++k) s += s_cnt[k];
     blk_if[blockIdx.x] = s;
   }
 }
 
-// writes the IF elements of block b to cut_list[blk_off[b] ...] in ascending
+// writes the IF elements of block b (NB * EPT elements per block, as in the
+// marking pass that counted them) to cut_list[blk_off[b] ...] in ascending
 // order
-__global__ __launch_bounds__(BLK) void k_compact(int ne, const uint32_t *__restrict__ if_bits,
-                                                 const int32_t *__restrict__ blk_off,
-                                                 int32_t *__restrict__ cut_list) {
-  __shared__ int s_cnt[BLK / 64];
-  const int e = blockIdx.x * BLK + threadIdx.x;
-  const bool on = e < ne && ((if_bits[e >> 5] >> (e & 31)) & 1u);
-  const unsigned long long m = __ballot(on);
+template <int NB, int EPT>
+__global__ __launch_bounds__(NB) void k_compact(int ne, const uint32_t *__restrict__ if_bits,
+                                                const int32_t *__restrict__ blk_off,
+                                                int32_t *__restrict__ cut_list) {
+  __shared__ int s_cnt[EPT][NB / 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                              __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
-  if (lane == 0) s_cnt[wv] = __popcll(m);
+  bool on[EPT];
+  int below[EPT];
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    const int e = blockIdx.x * NB * EPT + k * NB + threadIdx.x;
+    on[k] = e < ne && ((if_bits[e >> 5] >> (e & 31)) & 1u);
+    const unsigned long long m = __ballot(on[k]);
+    below[k] = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+    if (lane == 0) s_cnt[k][wv] = __popcll(m);
+  }
   __syncthreads();
   int off = blk_off[blockIdx.x];
-  for (int k = 0; k < wv; ++k) off += s_cnt[k];
-  if (on) cut_list[off + below] = e;
+#pragma unroll
+  for (int k = 0; k < EPT; ++k) {
+    int o = off, tot = 0;
+#pragma unroll
+    for (int j = 0; j < NB / 64; ++j) {
+      o += (j < wv) ? s_cnt[k][j] : 0;
+      tot += s_cnt[k][j];
+    }
+    if (on[k]) cut_list[o + below[k]] = blockIdx.x * NB * EPT + k * NB + threadIdx.x;
+    off += tot;
+  }
 }
 
 // ---- marking for P2 level sets (per-element nodal values, Lattice<3,2> order)
@@ -329,6 +507,59 @@ __global__ __launch_bounds__(SCAN_BLK) void k_scan_add(int32_t *data, int n, con
     if (base + k < n) data[base + k] += add;
 }
 
+// Exclusive scan of the n per-block counts of a marking pass, in place, by a
+// single block: thread t owns the run of ceil(n / SCAN_BLK) consecutive
+// counts starting at t * ceil(n / SCAN_BLK), read 8 at a time so that the
+// loads of a run are in flight together.  data[n] = total (and *total_out).
+// One launch instead of the three of the tiled scan: for the block counts of
+// a few million elements the tiled passes are launch-latency bound.
+constexpr int SCAN_ONE_BLOCK_MAX = 8 * SCAN_BLK;
+
+__global__ __launch_bounds__(SCAN_BLK) void k_scan_counts(int32_t *data, int n, int32_t *total_out) {
+  __shared__ int s_wave[SCAN_BLK / 64];
+  const int per = (n + SCAN_BLK - 1) / SCAN_BLK;
+  const int b = min(n, (int)threadIdx.x * per), e = min(n, b + per);
+  int s = 0;
+  for (int i = b; i < e; i += 8) {
+    int v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = (i + k < e) ? data[i + k] : 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += v[k];
+  }
+  int total;
+  int off = block_exclusive_scan(s, s_wave, total);
+  for (int i = b; i < e; i += 8) {
+    int v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = (i + k < e) ? data[i + k] : 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (i + k < e) {
+        data[i + k] = off;
+        off += v[k];
+      }
+  }
+  if (threadIdx.x == 0) {
+    data[n] = total;
+    if (total_out) *total_out = total;
+  }
+}
+
+// exclusive scan of the n block counts at blk (blk[n] = total, also stored
+// to *n_cut when given); sums: (n + SCAN_TILE - 1) / SCAN_TILE ints of
+// scratch, used by the tiled path only
+void scan_block_counts(hipStream_t st, int32_t *blk, int n, int32_t *sums, int32_t *n_cut) {
+  if (n <= SCAN_ONE_BLOCK_MAX) {
+    k_scan_counts<<<1, SCAN_BLK, 0, st>>>(blk, n, n_cut);
+    return;
+  }
+  const int nt = (n + SCAN_TILE - 1) / SCAN_TILE;
+  k_scan_tiles<<<nt, SCAN_BLK, 0, st>>>(blk, n, sums);
+  k_scan_sums<<<1, SCAN_BLK, 0, st>>>(sums, nt, blk + n, n_cut);
+  k_scan_add<<<nt, SCAN_BLK, 0, st>>>(blk, n, sums);
+}
+
 // ------------------------------------------------- facets (cutinfo.cpp:476-538)
 __device__ __forceinline__ bool bit_of(const uint32_t *b, int e) { return (b[e >> 5] >> (e & 31)) & 1u; }
 
@@ -403,19 +634,16 @@ size_t mark_scratch_ints(int ne) {
 int mark_compact(hipStream_t st, int32_t *d_err, int D, int ne, const int32_t *el2v, const double *lset,
                  uint8_t *dt, double *cr, uint32_t *bits, int32_t *scratch, int32_t *cut_list, int32_t *n_cut) {
   const int nw = (ne + 31) / 32;
-  const int nblk = (ne + BLK - 1) / BLK;
+  const int nblk = (ne + MARK_BLK * MARK_EPT - 1) / (MARK_BLK * MARK_EPT);
   int32_t *blk = scratch, *sums = scratch + nblk + 1;
   switch (D) {
-    case 1: k_mark<1><<<nblk, BLK, 0, st>>>(ne, el2v, lset, dt, cr, bits, nw, blk, d_err); break;
-    case 2: k_mark<2><<<nblk, BLK, 0, st>>>(ne, el2v, lset, dt, cr, bits, nw, blk, d_err); break;
-    default: k_mark<3><<<nblk, BLK, 0, st>>>(ne, el2v, lset, dt, cr, bits, nw, blk, d_err); break;
+    case 1: k_mark<1, MARK_BLK, MARK_EPT><<<nblk, MARK_BLK, 0, st>>>(ne, el2v, lset, dt, cr, bits, nw, blk, d_err); break;
+    case 2: k_mark<2, MARK_BLK, MARK_EPT><<<nblk, MARK_BLK, 0, st>>>(ne, el2v, lset, dt, cr, bits, nw, blk, d_err); break;
+    default: k_mark<3, MARK_BLK, MARK_EPT><<<nblk, MARK_BLK, 0, st>>>(ne, el2v, lset, dt, cr, bits, nw, blk, d_err); break;
   }
   XG_LAUNCH_CHECK();
-  const int nt = (nblk + SCAN_TILE - 1) / SCAN_TILE;
-  k_scan_tiles<<<nt, SCAN_BLK, 0, st>>>(blk, nblk, sums);
-  k_scan_sums<<<1, SCAN_BLK, 0, st>>>(sums, nt, blk + nblk, n_cut);
-  k_scan_add<<<nt, SCAN_BLK, 0, st>>>(blk, nblk, sums);
-  k_compact<<<nblk, BLK, 0, st>>>(ne, bits + (size_t)4 * nw, blk, cut_list);
+  scan_block_counts(st, blk, nblk, sums, n_cut);
+  k_compact<MARK_BLK, MARK_EPT><<<nblk, MARK_BLK, 0, st>>>(ne, bits + (size_t)4 * nw, blk, cut_list);
   XG_LAUNCH_CHECK();
   return XG_OK;
 }
@@ -440,7 +668,7 @@ extern "C" int xg_mark(xg_ctx *ctx, int D, int ne, const int32_t *el2v, const do
     if (n_cut) XG_CHECK(hipMemsetAsync(n_cut, 0, sizeof(int32_t), ctx->stream));
     return XG_OK;
   }
-  const int nblk = (ne + BLK - 1) / BLK;
+  const int nblk = (ne + MARK_BLK * MARK_EPT - 1) / (MARK_BLK * MARK_EPT);
   const bool compact = cut_list || n_cut;
   // block counts live after the scan scratch: [0, nblk] counts (+ total)
   int32_t *blk = nullptr;
@@ -451,27 +679,18 @@ extern "C" int xg_mark(xg_ctx *ctx, int D, int ne, const int32_t *el2v, const do
     blk = (int32_t *)((char *)ctx->ws + ((scan_ws + 255) / 256) * 256);
   }
   switch (D) {
-    case 1: k_mark<1><<<nblk, BLK, 0, ctx->stream>>>(ne, el2v, lset_v, dt, cut_ratio, bits, nw, blk, ctx->d_err); break;
-    case 2: k_mark<2><<<nblk, BLK, 0, ctx->stream>>>(ne, el2v, lset_v, dt, cut_ratio, bits, nw, blk, ctx->d_err); break;
-    default: k_mark<3><<<nblk, BLK, 0, ctx->stream>>>(ne, el2v, lset_v, dt, cut_ratio, bits, nw, blk, ctx->d_err); break;
+    case 1: k_mark<1, MARK_BLK, MARK_EPT><<<nblk, MARK_BLK, 0, ctx->stream>>>(ne, el2v, lset_v, dt, cut_ratio, bits, nw, blk, ctx->d_err); break;
+    case 2: k_mark<2, MARK_BLK, MARK_EPT><<<nblk, MARK_BLK, 0, ctx->stream>>>(ne, el2v, lset_v, dt, cut_ratio, bits, nw, blk, ctx->d_err); break;
+    default: k_mark<3, MARK_BLK, MARK_EPT><<<nblk, MARK_BLK, 0, ctx->stream>>>(ne, el2v, lset_v, dt, cut_ratio, bits, nw, blk, ctx->d_err); break;
   }
   XG_LAUNCH_CHECK();
   if (!compact) return XG_OK;
   // scan of block counts: blk[nblk] = total.  The scan scratch is the start
   // of ctx->ws, disjoint from blk.
-  {
-    const int n = nblk;
-    const int nt = (n + SCAN_TILE - 1) / SCAN_TILE;
-    int32_t *sums = (int32_t *)ctx->ws;
-    k_scan_tiles<<<nt, SCAN_BLK, 0, ctx->stream>>>(blk, n, sums);
-    XG_LAUNCH_CHECK();
-    k_scan_sums<<<1, SCAN_BLK, 0, ctx->stream>>>(sums, nt, blk + n, n_cut);
-    XG_LAUNCH_CHECK();
-    k_scan_add<<<nt, SCAN_BLK, 0, ctx->stream>>>(blk, n, sums);
-    XG_LAUNCH_CHECK();
-  }
+  scan_block_counts(ctx->stream, blk, nblk, (int32_t *)ctx->ws, n_cut);
+  XG_LAUNCH_CHECK();
   if (cut_list) {
-    k_compact<<<nblk, BLK, 0, ctx->stream>>>(ne, bits + (size_t)4 * nw, blk, cut_list);
+    k_compact<MARK_BLK, MARK_EPT><<<nblk, MARK_BLK, 0, ctx->stream>>>(ne, bits + (size_t)4 * nw, blk, cut_list);
     XG_LAUNCH_CHECK();
   }
   return XG_OK;
@@ -515,17 +734,10 @@ extern "C" int xg_mark_p2(xg_ctx *ctx, int ne, const double *lset_nodal, int L, 
   k_bits_from_dt<<<nblk, BLK, 0, ctx->stream>>>(ne, dt, bits, nw, blk);
   XG_LAUNCH_CHECK();
   if (!compact) return XG_OK;
-  {
-    const int n = nblk;
-    const int nt = (n + SCAN_TILE - 1) / SCAN_TILE;
-    int32_t *sums = (int32_t *)ctx->ws;
-    k_scan_tiles<<<nt, SCAN_BLK, 0, ctx->stream>>>(blk, n, sums);
-    k_scan_sums<<<1, SCAN_BLK, 0, ctx->stream>>>(sums, nt, blk + n, n_cut);
-    k_scan_add<<<nt, SCAN_BLK, 0, ctx->stream>>>(blk, n, sums);
-    XG_LAUNCH_CHECK();
-  }
+  scan_block_counts(ctx->stream, blk, nblk, (int32_t *)ctx->ws, n_cut);
+  XG_LAUNCH_CHECK();
   if (cut_list) {
-    k_compact<<<nblk, BLK, 0, ctx->stream>>>(ne, bits + (size_t)4 * nw, blk, cut_list);
+    k_compact<BLK, 1><<<nblk, BLK, 0, ctx->stream>>>(ne, bits + (size_t)4 * nw, blk, cut_list);
     XG_LAUNCH_CHECK();
   }
   return XG_OK;
