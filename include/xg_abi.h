@@ -121,9 +121,10 @@ int xg_mark_from_items(xg_ctx *ctx, int n, int k, const int32_t *item2node, cons
 int xg_elements_with_neighbor_facets(xg_ctx *ctx, int nf, const int32_t *f2el,
                                      const uint32_t *facet_mark, int ne, uint32_t *out);
 /* GetElementsWithSharedVertex (cutinfo.cpp:566-596): elements with a vertex
-   of a marked element (the marked elements included); out: ne bits.  Uses
-   ceil(nv/32) words of the context scratch. */
-int xg_elements_with_shared_vertex(xg_ctx *ctx, int D, int ne, const int32_t *el2v,
+   of a marked element (the marked elements included); el2v: ne x k vertex
+   numbers (k = D+1 for simplices, 4 / 8 for quadrilaterals / hexahedra);
+   out: ne bits.  Uses ceil(nv/32) words of the context scratch. */
+int xg_elements_with_shared_vertex(xg_ctx *ctx, int k, int ne, const int32_t *el2v,
                                    const uint32_t *elem_mark, int nv, uint32_t *out);
 /* cut_neighboring_node / dom_of_node of one node type (cutinfo.cpp:135-224)
    from the element domains dt (uint8, NEG 0 / POS 1 / IF 2, as written by
@@ -149,6 +150,35 @@ int xg_cut_rule_count(xg_ctx *ctx, int D, int nel, const int32_t *elems,
 int xg_cut_rule_fill(xg_ctx *ctx, int D, int nel, const int32_t *elems,
                      const int32_t *el2v, const double *coords, const double *lset_v,
                      int dt, int order, const int32_t *offsets, double *pts);
+
+/* ---- quadrilaterals and hexahedra ---------------------------------------
+   StraightCutIntegrationRule, quad / hex branch (cutint/straightcutrule.cpp:
+   582-677) with LevelsetCutQuadrilateral (:206-501): tensor product along
+   the last coordinate, dimension swaps, simplex fallback.  D = 2 (ET_QUAD)
+   or 3 (ET_HEX); el2v: nel x 2^D vertices in the reference element's order
+   ((0,0),(1,0),(1,1),(0,1), for hexes that square at z = 0 then at z = 1);
+   policy: SWAP_DIMENSIONS_POLICY (0 FIRST_ALLOWED, 1 FIND_OPTIMAL,
+   2 ALWAYS_NONE).  Points on the unit element; interface weights include
+   |F^{-T} n_ref| of the multilinear element map at the point.  Cases where
+   the reference throws ("if_scale_factor larger than bound", a NaN bound, a
+   fallback simplex it cannot cut) -> XG_ERR_DEGENERATE at xg_sync. */
+int xg_quad_rule_count(xg_ctx *ctx, int D, int nel, const int32_t *elems, const int32_t *el2v,
+                       const double *lset_v, int dt, int order, int policy, int32_t *offsets);
+int xg_quad_rule_fill(xg_ctx *ctx, int D, int nel, const int32_t *elems, const int32_t *el2v,
+                      const double *coords, const double *lset_v, int dt, int order, int policy,
+                      const int32_t *offsets, double *pts);
+/* CutInformation::Update on a quad / hex mesh (xfem/cutinfo.cpp:61-133):
+   order-0 part volumes of that rule (FIND_OPTIMAL in the reference's
+   CutInfo, any policy here).  dt and bits required; outputs as xg_mark. */
+int xg_quad_mark(xg_ctx *ctx, int D, int ne, const int32_t *el2v, const double *lset_v, int policy,
+                 uint8_t *dt, double *cut_ratio, uint32_t *bits, int32_t *cut_list, int32_t *n_cut);
+/* IntegrateX sums (cutint/python_cutint.cpp:66-98) on a quad / hex mesh:
+   sum |det F(xi)| w f(x(xi)) with the multilinear element map; arguments as
+   xg_integrate. */
+int xg_quad_integrate(xg_ctx *ctx, int D, int nel, const int32_t *elems, const int32_t *el2v,
+                      const double *coords, const int32_t *offsets, const double *pts,
+                      int nterms, const double *coef, const int32_t *expo, double *per_elem,
+                      double *total);
 
 /* ---- several level sets on simplices ------------------------------------
    CreateMultiLevelsetCutIntegrationRule (cutint/mlsetintegration.cpp:8-158)

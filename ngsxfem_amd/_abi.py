@@ -48,6 +48,10 @@ _SIGS = {
     "xg_cut_rule_count": [_vp, _i, _i, _vp, _vp, _vp, _i, _i, _vp, _vp],
     "xg_cut_rule_fill": [_vp, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp],
     "xg_integrate": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp],
+    "xg_quad_rule_count": [_vp, _i, _i, _vp, _vp, _vp, _i, _i, _i, _vp],
+    "xg_quad_rule_fill": [_vp, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _vp, _vp],
+    "xg_quad_mark": [_vp, _i, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp],
+    "xg_quad_integrate": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp],
     "xg_integrate_nodes": [_vp, _i, _vp, _vp, _vp, _i, _vp, _i, _vp, _vp, _vp, _vp],
     "xg_elmat": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "xg_cut_pipeline": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _i, _vp],
@@ -189,7 +193,8 @@ class Context:
                                                      ptr(out)))
 
     def elements_with_shared_vertex(self, D, el2v, elem_mark, nv, out):
-        check(lib().xg_elements_with_shared_vertex(self.h, D, el2v.shape[0], ptr(el2v), ptr(elem_mark), nv,
+        """D: element dimension (the vertex count per element is el2v's width)"""
+        check(lib().xg_elements_with_shared_vertex(self.h, el2v.shape[1], el2v.shape[0], ptr(el2v), ptr(elem_mark), nv,
                                                    ptr(out)))
 
     def node_domains(self, el2node, dt, cut_ratio, nn, cutnb=None, dom=None):
@@ -245,6 +250,31 @@ class Context:
         check(lib().xg_integrate(self.h, D, nel, ptr(elems), ptr(el2v), ptr(coords), ptr(offsets),
                                  ptr(pts), n, ctypes.cast(coef, _vp), ctypes.cast(ex, _vp),
                                  ptr(per_elem), ptr(total)))
+
+    # quadrilaterals / hexahedra (el2v: 2^D vertices per element); policy
+    # 0 FIRST_ALLOWED, 1 FIND_OPTIMAL, 2 ALWAYS_NONE
+    def quad_rule_count(self, D, el2v, lset, dt, order, policy, offsets, elems=None):
+        nel = elems.shape[0] if elems is not None else el2v.shape[0]
+        check(lib().xg_quad_rule_count(self.h, D, nel, ptr(elems), ptr(el2v), ptr(lset), dt, order, policy,
+                                       ptr(offsets)))
+
+    def quad_rule_fill(self, D, el2v, coords, lset, dt, order, policy, offsets, pts, elems=None):
+        nel = elems.shape[0] if elems is not None else el2v.shape[0]
+        check(lib().xg_quad_rule_fill(self.h, D, nel, ptr(elems), ptr(el2v), ptr(coords), ptr(lset), dt, order,
+                                      policy, ptr(offsets), ptr(pts)))
+
+    def quad_mark(self, D, el2v, lset, policy, dt, bits, cut_ratio=None, cut_list=None, n_cut=None):
+        check(lib().xg_quad_mark(self.h, D, el2v.shape[0], ptr(el2v), ptr(lset), policy, ptr(dt), ptr(cut_ratio),
+                                 ptr(bits), ptr(cut_list), ptr(n_cut)))
+
+    def quad_integrate(self, D, el2v, coords, offsets, pts, terms, total, per_elem=None, elems=None):
+        nel = elems.shape[0] if elems is not None else el2v.shape[0]
+        n = len(terms)
+        coef = (ctypes.c_double * max(n, 1))(*[t[0] for t in terms])
+        ex = (ctypes.c_int32 * max(3 * n, 1))(*[e for t in terms for e in t[1]])
+        check(lib().xg_quad_integrate(self.h, D, nel, ptr(elems), ptr(el2v), ptr(coords), ptr(offsets),
+                                      ptr(pts), n, ctypes.cast(coef, _vp), ctypes.cast(ex, _vp),
+                                      ptr(per_elem), ptr(total)))
 
     def elmat(self, D, p, el2v, coords, offsets, pts, mask, precision, out, elems=None):
         nel = elems.shape[0] if elems is not None else el2v.shape[0]

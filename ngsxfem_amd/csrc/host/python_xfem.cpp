@@ -123,9 +123,11 @@ using BA = std::shared_ptr<BitArray>;
 struct MeshAccess {
   py::object pymesh;
   int dim = 0, nv = 0, ne = 0, nse = 0;
+  int nvel = 0, nvbnd = 0;  // vertices per element / boundary element
+  bool quad = false;        // quadrilaterals / hexahedra (2^dim vertices)
   DevBuf<double> coords;
   DevBuf<int32_t> el2v, bnd2v, f2el, el2f, el2e;
-  int nf = -1, nedges = -1;
+  int nf = -1, nedges = -1, nf_el = 0, ne_el = 0;
   std::vector<int32_t> h_el2v;
 
   explicit MeshAccess(py::object m) : pymesh(m) {
@@ -134,12 +136,16 @@ struct MeshAccess {
     dim = (int)c.shape(1);
     nv = (int)c.shape(0);
     ne = (int)e.shape(0);
+    nvel = (int)e.shape(1);
+    quad = dim >= 2 && nvel == (1 << dim);
+    if (!quad && nvel != dim + 1) throw std::runtime_error("mesh: simplices or quadrilaterals / hexahedra only");
     coords.upload(c.data(), (size_t)nv * dim);
-    el2v.upload(e.data(), (size_t)ne * (dim + 1));
-    h_el2v.assign(e.data(), e.data() + (size_t)ne * (dim + 1));
+    el2v.upload(e.data(), (size_t)ne * nvel);
+    h_el2v.assign(e.data(), e.data() + (size_t)ne * nvel);
     auto b = py::array_t<int32_t, py::array::c_style | py::array::forcecast>(m.attr("bnd2v"));
     nse = (int)b.shape(0);
-    if (nse) bnd2v.upload(b.data(), (size_t)nse * dim);
+    nvbnd = b.ndim() == 2 ? (int)b.shape(1) : dim;
+    if (nse) bnd2v.upload(b.data(), (size_t)nse * nvbnd);
   }
   int GetNE(VorB vb) const { return vb == VOL ? ne : nse; }
   void ensure_facets() {
@@ -149,6 +155,7 @@ struct MeshAccess {
     nf = (int)f.shape(0);
     f2el.upload(f.data(), (size_t)nf * 2);
     auto ef = py::array_t<int32_t, py::array::c_style | py::array::forcecast>(t[1]);
+    nf_el = (int)ef.shape(1);
     el2f.upload(ef.data(), (size_t)ef.size());
   }
   void ensure_edges() {
@@ -157,6 +164,7 @@ struct MeshAccess {
     auto ev = py::array_t<int32_t, py::array::c_style | py::array::forcecast>(t[0]);
     auto ee = py::array_t<int32_t, py::array::c_style | py::array::forcecast>(t[1]);
     nedges = (int)ev.shape(0);
+    ne_el = (int)ee.shape(1);
     el2e.upload(ee.data(), (size_t)ee.size());
   }
   // NGSolve's GetNNodes for NT_VERTEX, NT_EDGE, NT_FACE, NT_CELL
@@ -182,17 +190,17 @@ struct MeshAccess {
   // that type are the elements themselves
   const int32_t *el2node(int nt, int &k) {
     if (nt == NT_VERTEX) {
-      k = dim + 1;
+      k = nvel;
       return el2v.p;
     }
     if (nt == NT_EDGE && dim >= 2) {
       ensure_edges();
-      k = dim == 2 ? 3 : 6;
+      k = ne_el;
       return el2e.p;
     }
     if (nt == NT_FACE && dim == 3) {
       ensure_facets();
-      k = 4;
+      k = nf_el;
       return el2f.p;
     }
     k = 0;
@@ -217,6 +225,23 @@ std::shared_ptr<MeshAccess> as_mesh(py::object m) {
 
 // vertex values of a P1 level set: a grid function stand-in (attribute
 // `vec` and `order`) or any array of vertex values
+// the paths built for simplices only (element matrices, subdivision and
+// multi-level-set rules); quadrilateral / hexahedral meshes have the straight
+// cut rule, marking and IntegrateX
+void simplices_only(MA ma, const char *what) {
+  if (ma->quad)
+    throw std::runtime_error(std::string(what) + ": simplicial meshes only in this build "
+                             "(quadrilaterals / hexahedra: CutInfo and Integrate with P1 level sets)");
+}
+
+int parse_policy(py::dict lsetdom) {
+  if (!lsetdom.contains("quad_dir_policy")) return FIND_OPTIMAL;  // xintegration.hpp:39
+  py::object o = lsetdom["quad_dir_policy"];
+  const int v = py::isinstance<py::int_>(o) ? o.cast<int>() : (int)o.cast<SWAP_DIMENSIONS_POLICY>();
+  if (v < 0 || v > 2) throw std::runtime_error("quad_dir_policy: FIRST, OPTIMAL or FALLBACK");
+  return v;
+}
+
 std::vector<double> p1_values(py::object lset, int nv, bool &is_p1) {
   py::object vec = py::hasattr(lset, "vec") ? lset.attr("vec") : lset;
   int order = py::hasattr(lset, "order") ? lset.attr("order").cast<int>() : 1;
@@ -350,8 +375,13 @@ class CutInformation {
       DevBuf<double> cr(ne);
       DevBuf<uint8_t> dt(ne);
       if (D < 1) continue;  // point boundaries of 1D meshes
-      check(xg_mark(ctx(), D, ne, vb == VOL ? ma->el2v.p : ma->bnd2v.p, d_lset.p, dt.p, cr.p, bits.p, nullptr,
-                    nullptr));
+      const bool quad_el = vb == VOL ? ma->quad : (ma->quad && D == 2);  // hexes have quadrilateral faces
+      if (quad_el)
+        check(xg_quad_mark(ctx(), D, ne, vb == VOL ? ma->el2v.p : ma->bnd2v.p, d_lset.p, FIND_OPTIMAL, dt.p, cr.p,
+                           bits.p, nullptr, nullptr));
+      else
+        check(xg_mark(ctx(), D, ne, vb == VOL ? ma->el2v.p : ma->bnd2v.p, d_lset.p, dt.p, cr.p, bits.p, nullptr,
+                      nullptr));
       if (vb == VOL) UpdateNodes(dt.p, cr.p);
       check(xg_sync(ctx()));
       std::vector<uint32_t> hb((size_t)8 * nw);
@@ -396,6 +426,7 @@ class CutInformation {
   // from the rule's weights on the cut elements (xg_subdiv_weights; the POS
   // part as the NEG part of -phi).  Boundary elements are not marked.
   void UpdateSubdivision(py::object lset, int L) {
+    simplices_only(ma, "CutInfo with a P2 level set / subdivlvl > 0");
     if (L > 5) throw std::runtime_error("CutInfo.Update: subdivlvl <= 5 in this build");
     const int ne = ma->ne;
     std::vector<int32_t> all(ne);
@@ -509,6 +540,7 @@ struct DeviceRule {
 };
 
 void build_rule(MA ma, const double *d_lset, int dt, int order, const std::vector<int32_t> *elems, DeviceRule &r) {
+  simplices_only(ma, "straight cut rule on simplices");
   r.nel = elems ? (int)elems->size() : ma->ne;
   if (elems) r.elems.upload(elems->data(), elems->size());
   r.offsets.resize(r.nel + 1);
@@ -521,6 +553,22 @@ void build_rule(MA ma, const double *d_lset, int dt, int order, const std::vecto
   r.pts.resize((size_t)std::max(tot, 1) * 4);
   check(xg_cut_rule_fill(ctx(), ma->dim, r.nel, elems ? r.elems.p : nullptr, ma->el2v.p, ma->coords.p, d_lset, dt,
                          order, r.offsets.p, r.pts.p));
+  check(xg_sync(ctx()));
+}
+
+// the straight cut rule of a quadrilateral / hexahedral mesh
+// (LevelsetCutQuadrilateral, straightcutrule.cpp:206-501)
+void build_quad_rule(MA ma, const double *d_lset, int dt, int order, int policy, DeviceRule &r) {
+  r.nel = ma->ne;
+  r.offsets.resize(r.nel + 1);
+  check(xg_quad_rule_count(ctx(), ma->dim, r.nel, nullptr, ma->el2v.p, d_lset, dt, order, policy, r.offsets.p));
+  check(xg_sync(ctx()));
+  int32_t tot = 0;
+  hipcheck(hipMemcpy(&tot, r.offsets.p + r.nel, sizeof(int32_t), hipMemcpyDeviceToHost), "rule total");
+  r.total = tot;
+  r.pts.resize((size_t)std::max(tot, 1) * 4);
+  check(xg_quad_rule_fill(ctx(), ma->dim, r.nel, nullptr, ma->el2v.p, ma->coords.p, d_lset, dt, order, policy,
+                          r.offsets.p, r.pts.p));
   check(xg_sync(ctx()));
 }
 
@@ -585,6 +633,7 @@ void make_mlset(const std::vector<std::vector<double>> &vals, py::object dts, Ml
 }
 
 void build_mlset_rule(MA ma, const MlsetDomain &md, int order, const std::vector<int32_t> *elems, DeviceRule &r) {
+  simplices_only(ma, "several level sets");
   r.nel = elems ? (int)elems->size() : ma->ne;
   if (elems) r.elems.upload(elems->data(), elems->size());
   r.offsets.resize(r.nel + 1);
@@ -619,6 +668,7 @@ class MultiLevelsetCutInformation {
   std::vector<Kept> kept;
 
   MultiLevelsetCutInformation(MA m, py::object l) : ma(m) {
+    simplices_only(ma, "MultiLevelsetCutInfo");
     check_lsets(l);
     vals = level_set_values(ma, l);
   }
@@ -698,6 +748,7 @@ bool poly_terms_of(py::object cf, std::vector<double> &coef, std::vector<int32_t
 // Lagrange lattice (xg_subdiv_weights), which reproduce the rule's point sum
 // for every polynomial of degree <= 4; the POS part is the NEG part of -phi
 py::object IntegrateSubdivision(MA ma, py::object lset, int dt, int L, int order, py::object cf, bool element_wise) {
+  simplices_only(ma, "IntegrateX with a P2 level set / subdivlvl > 0");
   std::vector<double> coef;
   std::vector<int32_t> ex;
   if (!poly_terms_of(cf, coef, ex))
@@ -739,6 +790,7 @@ py::object IntegrateSubdivision(MA ma, py::object lset, int dt, int L, int order
 py::tuple SubdivElementMatrices(py::object mesh, py::object lset, int dt, int L, int order, int p, int mask,
                                 py::object elements, int precision) {
   MA ma = as_mesh(mesh);
+  simplices_only(ma, "SubdivElementMatrices");
   if (p < 1 || p > 3) throw std::runtime_error("SubdivElementMatrices: p = 1, 2, 3");
   if (L > 5) throw std::runtime_error("SubdivElementMatrices: subdivlvl <= 5 in this build");
   std::vector<int32_t> el;
@@ -816,14 +868,27 @@ py::object IntegrateX(py::dict lsetdom, py::object mesh, py::object cf, py::obje
     std::vector<double> v = p1_values(lset, ma->nv, is_p1);
     DevBuf<double> d_l;
     d_l.upload(v.data(), v.size());
-    build_rule(ma, d_l.p, dt, order, nullptr, r);
+    if (ma->quad)
+      build_quad_rule(ma, d_l.p, dt, order, parse_policy(lsetdom), r);
+    else
+      build_rule(ma, d_l.p, dt, order, nullptr, r);
   }
   const int D = ma->dim;
   std::vector<double> pe(ma->ne, 0.0);
   double total = 0.0;
   std::vector<double> coef;
   std::vector<int32_t> ex;
-  if (poly_terms_of(cf, coef, ex)) {
+  const bool poly = poly_terms_of(cf, coef, ex);
+  if (ma->quad && (!poly || !ip_container.is_none()))
+    throw std::runtime_error("IntegrateX on quadrilaterals / hexahedra: polynomial integrands, no ip_container");
+  if (ma->quad) {
+    DevBuf<double> d_pe(ma->ne), d_tot(1);
+    check(xg_quad_integrate(ctx(), D, ma->ne, nullptr, ma->el2v.p, ma->coords.p, r.offsets.p, r.pts.p,
+                            (int)coef.size(), coef.data(), ex.data(), d_pe.p, d_tot.p));
+    check(xg_sync(ctx()));
+    d_pe.download(pe.data(), ma->ne);
+    d_tot.download(&total, 1);
+  } else if (poly) {
     DevBuf<double> d_pe(ma->ne), d_tot(1);
     check(xg_integrate(ctx(), D, ma->ne, nullptr, ma->el2v.p, ma->coords.p, r.offsets.p, r.pts.p, (int)coef.size(),
                        coef.data(), ex.data(), d_pe.p, d_tot.p));
@@ -882,6 +947,7 @@ py::object IntegrateX(py::dict lsetdom, py::object mesh, py::object cf, py::obje
 py::tuple CutElementMatrices(py::object mesh, py::object lset, int dt, int order, int p, int mask,
                              py::object elements, int precision) {
   MA ma = as_mesh(mesh);
+  simplices_only(ma, "CutElementMatrices");
   bool is_p1 = true;
   std::vector<double> v = p1_values(lset, ma->nv, is_p1);
   if (!is_p1) throw std::runtime_error("CutElementMatrices: P1 level sets only");
@@ -931,6 +997,7 @@ py::tuple CutElementMatrices(py::object mesh, py::object lset, int dt, int order
 py::tuple MlsetElementMatrices(py::object mesh, py::object lsets, py::object dts, int order, int p, int mask,
                                py::object elements, int precision) {
   MA ma = as_mesh(mesh);
+  simplices_only(ma, "MlsetElementMatrices");
   MlsetDomain md;
   make_mlset(level_set_values(ma, lsets), dts, md);
   std::vector<int32_t> el;
@@ -993,6 +1060,7 @@ std::vector<int32_t> element_list(py::object elements, int ne) {
 // (elements, nel x ndof element vectors)
 py::tuple CutElementVectors(py::object mesh, py::object lset, int dt, int order, int p, py::object elements) {
   MA ma = as_mesh(mesh);
+  simplices_only(ma, "CutElementVectors");
   bool is_p1 = true;
   std::vector<double> v = p1_values(lset, ma->nv, is_p1);
   if (!is_p1) throw std::runtime_error("CutElementVectors: P1 level sets only");
@@ -1230,7 +1298,7 @@ PYBIND11_MODULE(xfem, m) {
           auto ret = std::make_shared<BitArray>(ma->ne);
           DevBuf<uint32_t> da, dout(ret->words().size());
           da.upload(a->words().data(), a->words().size());
-          check(xg_elements_with_shared_vertex(ctx(), ma->dim, ma->ne, ma->el2v.p, da.p, ma->nv, dout.p));
+          check(xg_elements_with_shared_vertex(ctx(), ma->nvel, ma->ne, ma->el2v.p, da.p, ma->nv, dout.p));
           check(xg_sync(ctx()));
           dout.download(ret->words().data(), ret->words().size());
           return ret;

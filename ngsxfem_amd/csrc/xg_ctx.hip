@@ -109,6 +109,10 @@ extern "C" int xg_sync(xg_ctx *ctx) {
     if (flag & 2) return xg::set_error(XG_ERR_SUBDIV, "subdivision rule: did not expect this.. -3-");
     if (flag & 4) return xg::set_error(XG_ERR_CAPACITY, "output capacity exceeded");
     if (flag & 8) return xg::set_error(XG_ERR_ARG, "node or dof number out of range");
+    if (flag & 16)
+      return xg::set_error(XG_ERR_DEGENERATE,
+                           "quadrilateral cut rule: if_scale_factor larger than bound, or a NaN direction bound "
+                           "(straightcutrule.cpp:283-287, :438)");
     return xg::set_error(XG_ERR_DEGENERATE,
                          "You tried to cut a simplex with a plain geometry lset function "
                          "(a vertex level-set value is exactly 0 next to a negative one)");

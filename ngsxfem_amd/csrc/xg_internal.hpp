@@ -30,6 +30,8 @@ int ensure_ws(xg_ctx *ctx, size_t bytes);
 int scan_exclusive_i32(xg_ctx *ctx, int32_t *data, int n, bool with_total);
 // graph-capturable marking + compaction (kernel launches only)
 size_t mark_scratch_ints(int ne);
+int bits_and_cut_list_from_dt(xg_ctx *ctx, int ne, const uint8_t *dt, uint32_t *bits, int32_t *cut_list,
+                              int32_t *n_cut);
 int mark_compact(hipStream_t st, int32_t *d_err, int D, int ne, const int32_t *el2v, const double *lset,
                  uint8_t *dt, double *cr, uint32_t *bits, int32_t *scratch, int32_t *cut_list, int32_t *n_cut);
 // fused cut rule + element matrices in moment form (xg_moment.hip); dts packed

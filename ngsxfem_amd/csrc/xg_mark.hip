@@ -650,6 +650,38 @@ int mark_compact(hipStream_t st, int32_t *d_err, int D, int ne, const int32_t *e
 
 }  // namespace xg
 
+namespace xg {
+
+// the 8 bit arrays, the cut list (IF elements, ascending) and its length
+// from per-element domain types (k_bits_from_dt, block counts, scan,
+// compaction); cut_list and n_cut nullable
+int bits_and_cut_list_from_dt(xg_ctx *ctx, int ne, const uint8_t *dt, uint32_t *bits, int32_t *cut_list,
+                              int32_t *n_cut) {
+  const int nw = (ne + 31) / 32;
+  const int nblk = (ne + BLK - 1) / BLK;
+  const bool compact = cut_list || n_cut;
+  int32_t *blk = nullptr;
+  if (compact) {
+    // block counts after the scan scratch, as in xg_mark
+    const size_t scan_ws = sizeof(int32_t) * (size_t)((nblk + 1 + SCAN_TILE - 1) / SCAN_TILE + 1);
+    int r = ensure_ws(ctx, scan_ws + sizeof(int32_t) * (size_t)(nblk + 1) + 256);
+    if (r) return r;
+    blk = (int32_t *)((char *)ctx->ws + ((scan_ws + 255) / 256) * 256);
+  }
+  k_bits_from_dt<<<nblk, BLK, 0, ctx->stream>>>(ne, dt, bits, nw, blk);
+  XG_LAUNCH_CHECK();
+  if (!compact) return XG_OK;
+  scan_block_counts(ctx->stream, blk, nblk, (int32_t *)ctx->ws, n_cut);
+  XG_LAUNCH_CHECK();
+  if (cut_list) {
+    k_compact<BLK, 1><<<nblk, BLK, 0, ctx->stream>>>(ne, bits + (size_t)4 * nw, blk, cut_list);
+    XG_LAUNCH_CHECK();
+  }
+  return XG_OK;
+}
+
+}  // namespace xg
+
 extern "C" int xg_p1_perturb(xg_ctx *ctx, int n, double *v, double eps) {
   if (!ctx || n < 0 || (n > 0 && !v)) return set_error(XG_ERR_ARG, "xg_p1_perturb: bad argument");
   if (n == 0) return XG_OK;
@@ -717,28 +749,9 @@ extern "C" int xg_mark_p2(xg_ctx *ctx, int ne, const double *lset_nodal, int L, 
     if (n_cut) XG_CHECK(hipMemsetAsync(n_cut, 0, sizeof(int32_t), ctx->stream));
     return XG_OK;
   }
-  const int nw = (ne + 31) / 32;
   const int nblk = (ne + BLK - 1) / BLK;
-  const bool compact = cut_list || n_cut;
-  // block counts after the scan scratch, as in xg_mark
-  int32_t *blk = nullptr;
-  size_t scan_ws = 0;
-  if (compact) {
-    scan_ws = sizeof(int32_t) * (size_t)((nblk + 1 + SCAN_TILE - 1) / SCAN_TILE + 1);
-    int r = ensure_ws(ctx, scan_ws + sizeof(int32_t) * (size_t)(nblk + 1) + 256);
-    if (r) return r;
-    blk = (int32_t *)((char *)ctx->ws + ((scan_ws + 255) / 256) * 256);
-  }
   k_mark_p2_bern<<<nblk, BLK, 0, ctx->stream>>>(ne, lset_nodal, dt);
   k_mark_p2_lattice<<<nblk, BLK, 0, ctx->stream>>>(ne, lset_nodal, L, dt);
-  k_bits_from_dt<<<nblk, BLK, 0, ctx->stream>>>(ne, dt, bits, nw, blk);
   XG_LAUNCH_CHECK();
-  if (!compact) return XG_OK;
-  scan_block_counts(ctx->stream, blk, nblk, (int32_t *)ctx->ws, n_cut);
-  XG_LAUNCH_CHECK();
-  if (cut_list) {
-    k_compact<BLK, 1><<<nblk, BLK, 0, ctx->stream>>>(ne, bits + (size_t)4 * nw, blk, cut_list);
-    XG_LAUNCH_CHECK();
-  }
-  return XG_OK;
+  return xg::bits_and_cut_list_from_dt(ctx, ne, dt, bits, cut_list, n_cut);
 }

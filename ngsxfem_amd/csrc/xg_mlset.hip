@@ -60,35 +60,6 @@ __device__ __forceinline__ double lsw_eval(const double *c, const double *p) {
   return fabs(v) > 1e-14 ? v : 1e-14;  // update_initial_coefs
 }
 
-__device__ __forceinline__ int straight_cut(const double *v, int n) {
-  bool hp = false, hn = false;
-  for (int i = 0; i < n; ++i) {
-    hp |= v[i] > 0.0;
-    hn |= v[i] < 0.0;
-  }
-  if (hp == hn) return IF;
-  return hn ? NEG : POS;
-}
-
-__device__ __forceinline__ double simplex_measure(int dim, const double (*P)[3]) {
-  if (dim == 0) return 1.0;
-  if (dim == 1) {
-    double a[3] = {P[1][0] - P[0][0], P[1][1] - P[0][1], P[1][2] - P[0][2]};
-    return norm3(a);
-  }
-  double a[3], b[3];
-  for (int d = 0; d < 3; ++d) {
-    a[d] = P[2][d] - P[0][d];
-    b[d] = P[1][d] - P[0][d];
-  }
-  if (dim == 2) {
-    double c[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
-    return norm3(c);
-  }
-  double c[3] = {P[3][0] - P[0][0], P[3][1] - P[0][1], P[3][2] - P[0][2]};
-  return fabs(det3(c, a, b));
-}
-
 // exclusive prefix sum of v over the wave; total in *tot
 __device__ __forceinline__ int wave_scan(int v, int *tot) {
   const int lane = threadIdx.x & 63;
@@ -100,35 +71,6 @@ __device__ __forceinline__ int wave_scan(int v, int *tot) {
   }
   *tot = __shfl(x, 63, 64);
   return x - v;
-}
-
-// children of simplex P (dimension dim, vertex values v) cut by a level set
-// with target domain type dt (LevelsetCutSimplex::Decompose through the
-// decomposition table); returns the number of children (0 where the table
-// has no entry: the reference throws)
-__device__ __forceinline__ int children(int dim, const double (*P)[3], const double *v, int dt, double (*out)[4][3],
-                                        bool &bad) {
-  int mask = 0;
-  for (int i = 0; i <= dim; ++i) mask |= (v[i] >= 0.0 ? 1 : 0) << i;
-  const DecompEntry &E = xg_decomp_d.e[dim - 1][dt][mask];
-  if (!E.ok) {
-    bad = true;
-    return 0;
-  }
-  double C[4][3];
-  for (int c = 0; c < E.nc; ++c) {
-    const int a = E.ce[c][0], b = E.ce[c][1];
-    // CalcIFPolytopEUsingLset: a zero next to a negative value throws
-    if ((v[a] == 0.0 && v[b] < 0.0) || (v[b] == 0.0 && v[a] < 0.0)) bad = true;
-    const double t = v[a] / (v[a] - v[b]);
-    for (int d = 0; d < 3; ++d) C[c][d] = P[a][d] + t * (P[b][d] - P[a][d]);
-  }
-  for (int s = 0; s < E.ns; ++s)
-    for (int k = 0; k < E.nv; ++k) {
-      const int code = E.v[s][k];
-      for (int d = 0; d < 3; ++d) out[s][k][d] = code <= dim ? P[code][d] : C[code - (dim + 1)][d];
-    }
-  return E.ns;
 }
 
 template <int D, int MODE>

@@ -152,22 +152,24 @@ extern "C" int xg_elements_with_neighbor_facets(xg_ctx *ctx, int nf, const int32
   return xg_mark_from_items(ctx, nf, 2, f2el, facet_mark, ne, out);
 }
 
-extern "C" int xg_elements_with_shared_vertex(xg_ctx *ctx, int D, int ne, const int32_t *el2v,
+extern "C" int xg_elements_with_shared_vertex(xg_ctx *ctx, int k, int ne, const int32_t *el2v,
                                               const uint32_t *elem_mark, int nv, uint32_t *out) {
-  if (!ctx || D < 1 || D > 3 || ne < 0 || nv < 0 || (ne > 0 && (!el2v || !elem_mark || !out)))
-    return set_error(XG_ERR_ARG, "xg_elements_with_shared_vertex: bad argument");
+  if (!ctx || (k != 2 && k != 3 && k != 4 && k != 8) || ne < 0 || nv < 0 ||
+      (ne > 0 && (!el2v || !elem_mark || !out)))
+    return set_error(XG_ERR_ARG, "xg_elements_with_shared_vertex: bad argument (k in 2, 3, 4, 8)");
   if (ne == 0) return XG_OK;
   // vertex marks in the context scratch
   const int nwv = (nv + 31) / 32;
   int r = xg::ensure_ws(ctx, sizeof(uint32_t) * (size_t)(nwv > 0 ? nwv : 1));
   if (r) return r;
   uint32_t *vmark = (uint32_t *)ctx->ws;
-  r = xg_mark_from_items(ctx, ne, D + 1, el2v, elem_mark, nv, vmark);
+  r = xg_mark_from_items(ctx, ne, k, el2v, elem_mark, nv, vmark);
   if (r) return r;
-  switch (D) {
-    case 1: k_elems_touching<2><<<nblk(ne), BLK, 0, ctx->stream>>>(ne, el2v, vmark, nv, out); break;
-    case 2: k_elems_touching<3><<<nblk(ne), BLK, 0, ctx->stream>>>(ne, el2v, vmark, nv, out); break;
-    default: k_elems_touching<4><<<nblk(ne), BLK, 0, ctx->stream>>>(ne, el2v, vmark, nv, out); break;
+  switch (k) {
+    case 2: k_elems_touching<2><<<nblk(ne), BLK, 0, ctx->stream>>>(ne, el2v, vmark, nv, out); break;
+    case 3: k_elems_touching<3><<<nblk(ne), BLK, 0, ctx->stream>>>(ne, el2v, vmark, nv, out); break;
+    case 4: k_elems_touching<4><<<nblk(ne), BLK, 0, ctx->stream>>>(ne, el2v, vmark, nv, out); break;
+    default: k_elems_touching<8><<<nblk(ne), BLK, 0, ctx->stream>>>(ne, el2v, vmark, nv, out); break;
   }
   XG_LAUNCH_CHECK();
   return XG_OK;

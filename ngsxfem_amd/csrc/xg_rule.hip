@@ -115,12 +115,6 @@ __global__ __launch_bounds__(256) void k_rule_fill(int nel, const int32_t *__res
   }
 }
 
-struct Poly {
-  int n;
-  double c[32];
-  int ex[32][3];
-};
-
 template <int D>
 __global__ __launch_bounds__(256) void k_integrate(int nel, const int32_t *__restrict__ elems,
                                                    const int32_t *__restrict__ el2v,

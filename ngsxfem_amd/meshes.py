@@ -21,6 +21,14 @@ import itertools
 import numpy as np
 
 
+# local edges / faces of the reference quadrilateral and hexahedron (vertex
+# order (0,0),(1,0),(1,1),(0,1), for the hexahedron that square at z = 0 then
+# at z = 1); faces in cyclic vertex order
+QUAD_EDGES = [(0, 1), (1, 2), (2, 3), (3, 0)]
+HEX_EDGES = [(0, 1), (1, 2), (2, 3), (3, 0), (4, 5), (5, 6), (6, 7), (7, 4), (0, 4), (1, 5), (2, 6), (3, 7)]
+HEX_FACES = [(0, 1, 2, 3), (4, 5, 6, 7), (0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 6), (3, 0, 4, 7)]
+
+
 class Mesh:
     """Simplex mesh held as flat arrays.
 
@@ -35,6 +43,8 @@ class Mesh:
             self.coords = self.coords[:, None]
         self.el2v = np.ascontiguousarray(el2v, dtype=np.int32)
         self.dim = self.coords.shape[1]
+        # quadrilaterals / hexahedra: 2^D vertices per element (D >= 2)
+        self.is_quad = self.dim >= 2 and self.el2v.shape[1] == 2 ** self.dim
         self._bnd2v = None if bnd2v is None else np.ascontiguousarray(bnd2v, dtype=np.int32)
         self._facets = None
 
@@ -69,13 +79,19 @@ class Mesh:
         e2v, el2e = self.edges()
         return np.ascontiguousarray(np.hstack([self.el2v, self.nv + el2e]).astype(np.int32)), self.nv + len(e2v)
 
+    def _local_edges(self):
+        D = self.dim
+        if self.is_quad:
+            return QUAD_EDGES if D == 2 else HEX_EDGES
+        return [(i, j) for i in range(D + 1) for j in range(i + 1, D + 1)]
+
     def edges(self):
         """(edge2v (nedge, 2) sorted vertex pairs in ascending order, el2e
         (ne, nE)): the element's edges in the local order (0,1),(0,2),...,
-        (1,2),... .  In 1D the edges are the elements."""
+        (1,2),... (quads / hexes: the edges of the reference element, see
+        QUAD_EDGES / HEX_EDGES).  In 1D the edges are the elements."""
         if getattr(self, "_edges", None) is None:
-            D = self.dim
-            pairs = [(i, j) for i in range(D + 1) for j in range(i + 1, D + 1)]
+            pairs = self._local_edges()
             ev = np.stack([np.sort(self.el2v[:, [i, j]], axis=1) for i, j in pairs], axis=1)  # ne x nE x 2
             key = ev[..., 0].astype(np.int64) * self.nv + ev[..., 1]
             uniq, inv = np.unique(key.ravel(), return_inverse=True)
@@ -95,24 +111,29 @@ class Mesh:
         return self.ne if str(vb).endswith("VOL") else self.nse
 
     def facets(self):
-        """(facet2v (nf, D) sorted vertex tuples, el2f (ne, D+1), f2el (nf, 2)).
+        """(facet2v (nf, nvf), el2f (ne, nF), f2el (nf, 2)).
 
-        Facet k of an element is the one opposite local vertex k.  f2el holds
-        the two neighbouring elements (-1 on the boundary)."""
+        Simplices: facet k of an element is the one opposite local vertex k,
+        its vertices sorted.  Quads / hexes: the facets of the reference
+        element (QUAD_EDGES / HEX_FACES), vertices in the cyclic order of the
+        first element that has the facet.  f2el holds the two neighbouring
+        elements (-1 on the boundary)."""
         if self._facets is None:
             D = self.dim
             ne = self.ne
-            loc = []
-            for k in range(D + 1):
-                cols = [c for c in range(D + 1) if c != k]
-                loc.append(np.sort(self.el2v[:, cols], axis=1))
-            allf = np.concatenate(loc, axis=0)  # (D+1)*ne rows, local facet k block k
-            uniq, inv = np.unique(allf, axis=0, return_inverse=True)
+            if self.is_quad:
+                locf = QUAD_EDGES if D == 2 else HEX_FACES
+            else:
+                locf = [tuple(c for c in range(D + 1) if c != k) for k in range(D + 1)]
+            nF = len(locf)
+            rows = np.concatenate([self.el2v[:, list(f)] for f in locf], axis=0)  # block k: local facet k
+            allf = np.sort(rows, axis=1)
+            uniq, first_idx, inv = np.unique(allf, axis=0, return_index=True, return_inverse=True)
             inv = inv.reshape(-1)
-            el2f = inv.reshape(D + 1, ne).T.copy().astype(np.int32)
+            el2f = inv.reshape(nF, ne).T.copy().astype(np.int32)
             nf = uniq.shape[0]
             f2el = -np.ones((nf, 2), np.int64)
-            elid = np.tile(np.arange(ne), D + 1)
+            elid = np.tile(np.arange(ne), nF)
             order = np.argsort(inv, kind="stable")
             sinv = inv[order]
             sel = elid[order]
@@ -121,7 +142,8 @@ class Mesh:
             f2el[sinv[first], 0] = sel[first]
             second = ~first
             f2el[sinv[second], 1] = sel[second]
-            self._facets = (uniq.astype(np.int32), el2f, f2el.astype(np.int32))
+            f2v = rows[first_idx] if self.is_quad else uniq
+            self._facets = (np.ascontiguousarray(f2v.astype(np.int32)), el2f, f2el.astype(np.int32))
         return self._facets
 
     def GetNFacets(self):
@@ -156,9 +178,38 @@ def Make1DMesh(n, mapping=None):
     return Mesh(x[:, None], el, bnd)
 
 
+def _structured_quads(nx, ny, mapping):
+    """nx x ny unit-square quadrilaterals; vertex (a, b) = a * (ny+1) + b, each
+    element's vertices in the reference order (0,0),(1,0),(1,1),(0,1)"""
+    i, j = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), indexing="ij")
+    coords = np.stack([i.ravel() / nx, j.ravel() / ny], axis=1).astype(np.float64)
+    if mapping is not None:
+        coords = np.array([mapping(*p) for p in coords], dtype=np.float64)
+    ci, cj = [a.ravel() for a in np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")]
+    vid = lambda a, b: a * (ny + 1) + b  # noqa: E731
+    el2v = np.stack([vid(ci, cj), vid(ci + 1, cj), vid(ci + 1, cj + 1), vid(ci, cj + 1)], axis=1)
+    return Mesh(coords, el2v)
+
+
+def _structured_hexes(nx, ny, nz, mapping):
+    """unit-cube hexahedra; vertex (a, b, c) = (a (ny+1) + b) (nz+1) + c, each
+    element's vertices in the reference order: the square (0,0),(1,0),(1,1),
+    (0,1) at z = 0, then at z = 1"""
+    i, j, k = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    coords = np.stack([i.ravel() / nx, j.ravel() / ny, k.ravel() / nz], axis=1).astype(np.float64)
+    if mapping is not None:
+        coords = np.array([mapping(*p) for p in coords], dtype=np.float64)
+    ci, cj, ck = [a.ravel() for a in np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")]
+    vid = lambda a, b, c: (a * (ny + 1) + b) * (nz + 1) + c  # noqa: E731
+    el2v = np.stack([vid(ci, cj, ck), vid(ci + 1, cj, ck), vid(ci + 1, cj + 1, ck), vid(ci, cj + 1, ck),
+                     vid(ci, cj, ck + 1), vid(ci + 1, cj, ck + 1), vid(ci + 1, cj + 1, ck + 1),
+                     vid(ci, cj + 1, ck + 1)], axis=1)
+    return Mesh(coords, el2v)
+
+
 def MakeStructured2DMesh(quads=False, nx=10, ny=10, mapping=None):
     if quads:
-        raise NotImplementedError("quadrilateral meshes are outside the simplex hot path")
+        return _structured_quads(nx, ny, mapping)
     i, j = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), indexing="ij")
     coords = np.stack([i.ravel() / nx, j.ravel() / ny], axis=1).astype(np.float64)
     if mapping is not None:
@@ -189,7 +240,7 @@ _KUHN = [p for p in itertools.permutations(range(3))]
 
 def MakeStructured3DMesh(hexes=False, nx=10, ny=None, nz=None, mapping=None):
     if hexes:
-        raise NotImplementedError("hexahedral meshes are outside the simplex hot path")
+        return _structured_hexes(nx, nx if ny is None else ny, nx if nz is None else nz, mapping)
     ny = nx if ny is None else ny
     nz = nx if nz is None else nz
     i, j, k = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")

@@ -36,6 +36,12 @@ def _load():
     lib.or_straight_cut_rule.argtypes = [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp,
                                          _dp, _dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     lib.or_mark.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _dp, _bp, _dp, _up]
+    lib.or_quad_cut_rule.argtypes = [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
+                                     ctypes.c_int]
+    lib.or_quad_mark.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _dp, ctypes.c_int, _bp, _dp, _up]
+    lib.or_quad_integrate_poly.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp,
+                                           ctypes.POINTER(ctypes.c_int)]
+    lib.or_quad_integrate_poly.restype = ctypes.c_double
     lib.or_mlset_rule.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _ip, ctypes.c_int, _dp, _dp,
                                   ctypes.c_double, _dp, _dp, ctypes.c_int]
     lib.or_mlset_mark.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _dp, ctypes.c_int, _dp, ctypes.c_int,
@@ -138,6 +144,62 @@ def mark(D, el2v, lset_v):
     if r:
         raise RuntimeError("oracle mark error %d" % r)
     return dt, cr, bits
+
+
+QUAD_POLICY = {"FIRST": 0, "OPTIMAL": 1, "FALLBACK": 2}
+
+
+def quad_cut_rule(D, vals, dt, order, pol=1, V=None, cap=40000):
+    """Straight cut rule of a quad (D=2) / hex (D=3) element
+    (LevelsetCutQuadrilateral): vals = 2^D vertex values in the reference
+    element's order, pol 0 FIRST / 1 OPTIMAL / 2 FALLBACK, V (2^D, D)
+    physical vertices for the interface weights.  Returns (pts[n,3], w[n]) or
+    None for 'no rule'."""
+    v = _f64(vals)
+    pts = np.zeros((cap, 3))
+    w = np.zeros(cap)
+    Vf = _f64(V).reshape(-1) if V is not None else None
+    n = lib().or_quad_cut_rule(D, _d(v), dt, order, pol, _d(Vf) if Vf is not None else None, _d(pts), _d(w), cap)
+    if n == -1:
+        return None
+    if n < -1:
+        raise RuntimeError("oracle quad_cut_rule error %d" % n)
+    return pts[:n].copy(), w[:n].copy()
+
+
+def quad_mark(D, el2v, lset_v, pol=1):
+    el2v = np.ascontiguousarray(el2v, dtype=np.int32)
+    l = _f64(lset_v)
+    ne = el2v.shape[0]
+    nw = (ne + 31) // 32
+    dt = np.zeros(ne, np.uint8)
+    cr = np.zeros(ne)
+    bits = np.zeros((8, nw), np.uint32)
+    r = lib().or_quad_mark(D, ne, el2v.ctypes.data_as(_ip), _d(l), pol, dt.ctypes.data_as(_bp), _d(cr),
+                           bits.ctypes.data_as(_up))
+    if r:
+        raise RuntimeError("oracle quad_mark error %d" % r)
+    return dt, cr, bits
+
+
+def quad_integrate_poly(D, V, pts, w, terms):
+    coef = _f64([t[0] for t in terms])
+    ex = np.ascontiguousarray([e for t in terms for e in t[1]], dtype=np.int32)
+    return lib().or_quad_integrate_poly(D, _d(_f64(V)), _d(_f64(pts)), _d(_f64(w)), len(w), len(terms), _d(coef),
+                                        ex.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+
+
+def quad_integrate_mesh(coords, el2v, lset_v, dt, order, terms=((1.0, (0, 0, 0)),), pol=1, element_wise=False):
+    """IntegrateX on a quad / hex mesh with the oracle's rules (element loop
+    in Python: small meshes only)."""
+    coords = _f64(coords)
+    D = coords.shape[1]
+    out = np.zeros(len(el2v))
+    for e, vs in enumerate(np.asarray(el2v)):
+        r = quad_cut_rule(D, np.asarray(lset_v)[vs], dt, order, pol, V=coords[vs])
+        if r is not None:
+            out[e] = quad_integrate_poly(D, coords[vs], r[0], r[1], terms)
+    return out if element_wise else float(out.sum())
 
 
 def mlset_rule(D, lsets, dtts, order, V=None, cap=20000):

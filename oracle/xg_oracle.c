@@ -7,6 +7,7 @@
  *
  * Reference files restated (paths relative to the ngsxfem source tree):
  *   cutint/straightcutrule.cpp:11-204, 503-677   straight cut rule
+ *   cutint/straightcutrule.cpp:74-129, 206-501    quadrilaterals / hexahedra
  *   xfem/cutinfo.cpp:61-133                       element marking
  *   xfem/symboliccutbfi.cpp:76-304                element matrix B^T D B
  *   cutint/python_cutint.cpp:66-98                IntegrateX sums
@@ -727,6 +728,572 @@ int or_mark(int D, int ne, const int32_t *el2v, const double *lset_v,
   /* combined arrays: index = COMBINED_DOMAIN_TYPE; bit set iff (cdt & c) != 0,
      i.e. UNCUT = NEG|POS, HASNEG = NEG|IF, HASPOS = POS|IF, ANY = all, NO = none */
   return 0;
+}
+
+/* ====================================================================== */
+/* quadrilaterals and hexahedra: LevelsetCutQuadrilateral                  */
+/* (straightcutrule.cpp:74-78, 102-129, 206-501, 503-556, 582-677)         */
+/* The rule of a cut unit quad / hex is built as the reference does: a     */
+/* tensor product along the last coordinate xi (Gauss points on [xi0,xi1]   */
+/* times the cut rule of the codimension-1 slice at each point), after      */
+/* splitting [0,1] at the xi where an edge along xi changes sign, with the  */
+/* dimension swaps of GetDimensionSwap and the simplex fallback.            */
+/* ====================================================================== */
+
+typedef struct { double c[2][2][2]; } QL;             /* LevelsetWrapper::c */
+typedef struct { int D; double p[8][3]; } QQ;          /* Quadrilateral */
+typedef struct { int n, cap; double *x; double *w; } QR; /* growable rule */
+
+static const double QC_c = 0.999;                      /* straightcutrule.cpp:253 */
+
+static int qr_push(QR *r, const double *x, double w)
+{
+  if (r->n == r->cap) {
+    int nc = r->cap ? 2 * r->cap : 64;
+    double *nx = (double *)realloc(r->x, sizeof(double) * 3 * (size_t)nc);
+    double *nw = (double *)realloc(r->w, sizeof(double) * (size_t)nc);
+    if (!nx || !nw) { free(nx ? nx : r->x); free(nw ? nw : r->w); r->x = r->w = NULL; return OR_ERR_CAP; }
+    r->x = nx; r->w = nw; r->cap = nc;
+  }
+  memcpy(&r->x[3 * r->n], x, sizeof(double) * 3);
+  r->w[r->n++] = w;
+  return 0;
+}
+
+static void qr_free(QR *r) { free(r->x); free(r->w); r->x = r->w = NULL; r->n = r->cap = 0; }
+
+/* LevelsetWrapper::GetCoeffsFromVals, straightcutrule.cpp:503-526
+   (et: 1 = ET_SEGM, 2 = ET_QUAD, 3 = ET_HEX) */
+static void ql_from_vals(int et, const double *v, QL *L)
+{
+  memset(L, 0, sizeof(*L));
+  double (*c)[2][2] = L->c;
+  if (et == 1) {
+    c[0][0][0] = v[1]; c[1][0][0] = v[0] - v[1];
+  } else if (et == 2) {
+    c[0][0][0] = v[0]; c[1][0][0] = v[1] - c[0][0][0]; c[0][1][0] = v[3] - c[0][0][0];
+    c[1][1][0] = v[2] - c[1][0][0] - c[0][1][0] - c[0][0][0];
+  } else {
+    c[0][0][0] = v[0]; c[1][0][0] = v[1] - v[0]; c[0][1][0] = v[3] - v[0]; c[0][0][1] = v[4] - v[0];
+    c[1][1][0] = v[2] - c[1][0][0] - c[0][1][0] - c[0][0][0];
+    c[1][0][1] = v[5] - c[1][0][0] - c[0][0][1] - c[0][0][0];
+    c[0][1][1] = v[7] - c[0][1][0] - c[0][0][1] - c[0][0][0];
+    c[1][1][1] = v[6] - c[1][1][0] - c[1][0][1] - c[0][1][1] - c[1][0][0] - c[0][1][0] - c[0][0][1] - c[0][0][0];
+  }
+}
+
+/* operator(), :542-546: sum over (i,j,k) in loop order of
+   c[i][j][k] * p0^i * p1^j * p2^k (powers 0 and 1 are exact) */
+static double ql_eval(const QL *L, const double *p)
+{
+  double v = 0.0;
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int k = 0; k < 2; ++k) {
+        double t = L->c[i][j][k];
+        if (i) t *= p[0];
+        if (j) t *= p[1];
+        if (k) t *= p[2];
+        v += t;
+      }
+  return v;
+}
+
+/* GetGrad, :528-534 */
+static void ql_grad(const QL *L, const double *p, double g[3])
+{
+  const double (*c)[2][2] = (const double (*)[2][2])L->c;
+  g[0] = c[1][0][0] + c[1][1][0] * p[1] + c[1][0][1] * p[2] + c[1][1][1] * p[1] * p[2];
+  g[1] = c[0][1][0] + c[1][1][0] * p[0] + c[0][1][1] * p[2] + c[1][1][1] * p[0] * p[2];
+  g[2] = c[0][0][1] + c[0][1][1] * p[1] + c[1][0][1] * p[0] + c[1][1][1] * p[0] * p[1];
+}
+
+/* Quadrilateral(bnds), straightcutrule.hpp:72-90 */
+static void qq_box(int D, const double *lo, const double *hi, QQ *q)
+{
+  memset(q, 0, sizeof(*q));
+  q->D = D;
+  const int nv = D == 2 ? 4 : 8;
+  for (int v = 0; v < nv; ++v) {
+    const int b = v & 3;
+    q->p[v][0] = (b == 1 || b == 2) ? hi[0] : lo[0];
+    q->p[v][1] = (b >= 2) ? hi[1] : lo[1];
+    if (D == 3) q->p[v][2] = (v >= 4) ? hi[2] : lo[2];
+  }
+}
+
+static void qq_unit(int D, QQ *q)
+{
+  const double lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+  qq_box(D, lo, hi, q);
+}
+
+static int qq_check(const QL *L, const QQ *q, double eps)
+{
+  const int nv = q->D == 2 ? 4 : 8;
+  double v[8];
+  for (int k = 0; k < nv; ++k) v[k] = ql_eval(L, q->p[k]);
+  /* CheckIfStraightCut with epsilon, :11-24 */
+  int haspos = 0, hasneg = 0;
+  for (int k = 0; k < nv; ++k) {
+    if (!haspos && v[k] > eps) haspos = 1;
+    if (!hasneg && v[k] < -eps) hasneg = 1;
+    if (haspos && hasneg) break;
+  }
+  if ((hasneg && haspos) || (!hasneg && !haspos)) return OR_IF;
+  return hasneg ? OR_NEG : OR_POS;
+}
+
+/* Quadrilateral::GetVolume / GetPlainIntegrationRule, :74-78, 102-129:
+   tensor Gauss rule of ET_QUAD / ET_HEX (x slowest), mapped by
+   points[0] + A ip with A = (p1 - p0, p3 - p0, p4 - p0) */
+static int qq_plain(const QQ *q, int order, QR *r)
+{
+  double a[3], b[3], c[3], tf;
+  for (int d = 0; d < 3; ++d) { a[d] = q->p[3][d] - q->p[0][d]; b[d] = q->p[1][d] - q->p[0][d]; }
+  if (q->D == 2) {
+    cross3(a, b, c);
+    tf = norm3(c);
+  } else {
+    double e4[3];
+    for (int d = 0; d < 3; ++d) e4[d] = q->p[4][d] - q->p[0][d];
+    tf = fabs(det3(e4, a, b));
+  }
+  double A[3][3] = {{0}};
+  for (int d = 0; d < 3; ++d) {
+    A[d][0] = q->p[1][d] - q->p[0][d];
+    A[d][1] = q->p[3][d] - q->p[0][d];
+    A[d][2] = q->D == 3 ? q->p[4][d] - q->p[0][d] : 0.0;
+  }
+  if (q->D == 2) { A[2][0] = A[2][1] = 0.0; }
+  const Rule *s = get_rule(1, order);
+  const int n = s->n;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j)
+      for (int k = 0; k < (q->D == 3 ? n : 1); ++k) {
+        const double ip[3] = {s->xi[i], s->xi[j], q->D == 3 ? s->xi[k] : 0.0};
+        const double wq = q->D == 3 ? s->w[i] * s->w[j] * s->w[k] : s->w[i] * s->w[j];
+        double x[3];
+        for (int d = 0; d < 3; ++d) x[d] = q->p[0][d] + (A[d][0] * ip[0] + A[d][1] * ip[1] + A[d][2] * ip[2]);
+        int e = qr_push(r, x, wq * tf);
+        if (e) return e;
+      }
+  return 0;
+}
+
+/* LevelsetCutSimplex(LevelsetWrapper(lv, ET_SEGM), dt, SimpleX(ET_SEGM))
+   ::GetIntegrationRule, :41-64 (D == 1 branch), :131-204.  The unit
+   segment's points are (1,0,0) and the origin. */
+static int q_segment_rule(const double lv[2], int dt, int order, QR *r)
+{
+  if (or_check_if_straight_cut(2, lv) != OR_IF) return OR_ERR_PLAIN;
+  const double P[2][3] = {{1, 0, 0}, {0, 0, 0}};
+  const double t = lv[0] / (lv[0] - lv[1]);
+  double C[3];
+  for (int d = 0; d < 3; ++d) C[d] = P[0][d] + t * (P[1][d] - P[0][d]);
+  if (dt == OR_IF) return qr_push(r, C, 1.0); /* ET_POINT rule, volume 1 */
+  int R[2], nr = 0;
+  for (int i = 0; i < 2; ++i)
+    if ((dt == OR_POS && lv[i] >= 0.0) || (dt == OR_NEG && lv[i] < 0.0)) R[nr++] = i;
+  if (nr != 1) return OR_ERR_NOT_IMPL;
+  Simp s;
+  s.np = 2;
+  memcpy(s.p[0], C, sizeof(C));
+  memcpy(s.p[1], P[R[0]], sizeof(C));
+  double pts[3 * 16], w[16];
+  int n = simp_rule(&s, order, pts, w, 16, 0);
+  if (n < 0) return n;
+  for (int k = 0; k < n; ++k) {
+    int e = qr_push(r, &pts[3 * k], w[k]);
+    if (e) return e;
+  }
+  return 0;
+}
+
+static int qc_rule(const QL *L, int D, int dt, int order, int pol, int consider, QR *r);
+
+/* GetTensorProductAlongXiIntegrationRule, :256-302 */
+static int qc_tensor(const QL *L, const QQ *q, int dt, int order, int pol, QR *r)
+{
+  const int D = q->D, xi = D - 1;
+  const double xi0 = q->p[0][xi], xi1 = xi == 1 ? q->p[2][1] : q->p[4][2];
+  const double Cb = 1. / sqrt(1 - QC_c * QC_c);
+  const Rule *s = get_rule(1, order);
+  for (int i1 = 0; i1 < s->n; ++i1) {
+    const double xs = xi0 + s->xi[i1] * (xi1 - xi0);
+    QR inner = {0, 0, NULL, NULL};
+    int e;
+    if (D == 2) {
+      const double pa[3] = {q->p[0][0], xs, 0}, pb[3] = {q->p[2][0], xs, 0};
+      const double lv[2] = {ql_eval(L, pb), ql_eval(L, pa)};
+      e = q_segment_rule(lv, dt, order, &inner);
+    } else {
+      const double p0[3] = {q->p[0][0], q->p[0][1], xs}, p1[3] = {q->p[2][0], q->p[0][1], xs};
+      const double p2[3] = {q->p[2][0], q->p[2][1], xs}, p3[3] = {q->p[0][0], q->p[2][1], xs};
+      const double lv[4] = {ql_eval(L, p0), ql_eval(L, p1), ql_eval(L, p2), ql_eval(L, p3)};
+      QL L2;
+      ql_from_vals(2, lv, &L2);
+      e = qc_rule(&L2, 2, dt, order, pol, 1, &inner);
+    }
+    if (e) { qr_free(&inner); return e; }
+    for (int k = 0; k < inner.n; ++k) {
+      const double *p2 = &inner.x[3 * k];
+      double ip[3] = {0, 0, 0};
+      ip[xi] = xs;
+      ip[0] = q->p[0][0] + p2[0] * (q->p[2][0] - q->p[0][0]);
+      if (D == 3) ip[1] = q->p[0][1] + p2[1] * (q->p[2][1] - q->p[0][1]);
+      double f = 1;
+      if (dt == OR_IF) {
+        double g[3];
+        ql_grad(L, ip, g);
+        if (D == 2)
+          f = norm3(g) / fabs(g[0]);
+        else
+          f = norm3(g) / sqrt(g[0] * g[0] + g[1] * g[1]);
+        if (isnan(f) || f > Cb) { qr_free(&inner); return OR_ERR_BOUND; }
+      }
+      e = qr_push(r, ip, inner.w[k] * s->w[i1] * (xi1 - xi0) * f);
+      if (e) { qr_free(&inner); return e; }
+    }
+    qr_free(&inner);
+  }
+  return 0;
+}
+
+/* HasTopologyChangeAlongXi / Decompose / GetIntegrationRuleAlongXi,
+   :206-251, 459-473, on the unit quad / hex */
+static int qc_along_xi(const QL *L, int D, int dt, int order, int pol, QR *r)
+{
+  QQ q;
+  qq_unit(D, &q);
+  const int dq = qq_check(L, &q, 0.0);
+  if (dq != OR_IF) return dq == dt ? qq_plain(&q, order, r) : 0;
+  static const int e2[2][2] = {{1, 2}, {0, 3}};
+  static const int e3[4][2] = {{0, 4}, {1, 5}, {2, 6}, {3, 7}};
+  const int ne = D == 2 ? 2 : 4;
+  double xs[6] = {0.0, 1.0};
+  int nx = 2, topo = 0;
+  for (int k = 0; k < ne; ++k) {
+    const int a = D == 2 ? e2[k][0] : e3[k][0], b = D == 2 ? e2[k][1] : e3[k][1];
+    const double vals[2] = {ql_eval(L, q.p[b]), ql_eval(L, q.p[a])};
+    if (or_check_if_straight_cut(2, vals) != OR_IF) continue;
+    topo = 1;
+    /* unit_line.CalcIFPolytopEUsingLset(vals).points[0][0] */
+    const double t = vals[0] / (vals[0] - vals[1]);
+    const double x = 1.0 + t * (0.0 - 1.0);
+    if (isnan(x)) continue; /* a NaN key compares equal to the set's members: not inserted */
+    int dup = 0;
+    for (int m = 0; m < nx; ++m) dup |= xs[m] == x;
+    if (!dup) xs[nx++] = x;
+  }
+  if (!topo) return qc_tensor(L, &q, dt, order, pol, r);
+  for (int i = 1; i < nx; ++i) /* ascending, as the std::set iteration */
+    for (int j = i; j > 0 && xs[j] < xs[j - 1]; --j) { double t = xs[j]; xs[j] = xs[j - 1]; xs[j - 1] = t; }
+  for (int i = 0; i + 1 < nx; ++i) {
+    QQ sq;
+    double lo[3] = {q.p[0][0], 0, 0}, hi[3] = {q.p[2][0], 0, 0};
+    if (D == 2) { lo[1] = xs[i]; hi[1] = xs[i + 1]; }
+    else { lo[1] = q.p[0][1]; hi[1] = q.p[2][1]; lo[2] = xs[i]; hi[2] = xs[i + 1]; }
+    qq_box(D, lo, hi, &sq);
+    const int ds = qq_check(L, &sq, 1e-15);
+    int e = 0;
+    if (ds == OR_IF) e = qc_tensor(L, &sq, dt, order, pol, r);
+    else if (ds == dt) e = qq_plain(&sq, order, r);
+    if (e) return e;
+  }
+  return 0;
+}
+
+/* GetFallbackIntegrationRule, :475-491 */
+static int qc_fallback(const QL *L, int D, int dt, int order, QR *r)
+{
+  static const int t2[2][3] = {{0, 1, 3}, {2, 1, 3}};
+  static const int t3[6][4] = {{3, 0, 1, 5}, {3, 1, 2, 5}, {3, 5, 2, 6}, {4, 5, 0, 3}, {4, 7, 5, 3}, {7, 6, 5, 3}};
+  QQ q;
+  qq_unit(D, &q);
+  const int ns = D == 2 ? 2 : 6;
+  double pts[3 * 4096], w[4096];
+  for (int s = 0; s < ns; ++s) {
+    double P[4][3], l[4];
+    for (int k = 0; k <= D; ++k) {
+      const int v = D == 2 ? t2[s][k] : t3[s][k];
+      memcpy(P[k], q.p[v], sizeof(P[k]));
+      const double d = ql_eval(L, P[k]); /* update_initial_coefs, :548-556 */
+      l[k] = fabs(d) > 1e-14 ? d : 1e-14;
+    }
+    const int ds = or_check_if_straight_cut(D + 1, l);
+    int n = 0;
+    if (ds != OR_IF && ds == dt) {
+      Simp sp;
+      sp.np = D + 1;
+      memcpy(sp.p, P, sizeof(P));
+      n = simp_rule(&sp, order, pts, w, 4096, 0);
+    } else if (ds == OR_IF) {
+      Simp sub[MAXSUB];
+      int nsub = 0;
+      int e = decompose_pts(D, P, l, dt, sub, &nsub);
+      if (e) return e;
+      for (int k = 0; k < nsub && n >= 0; ++k) n = simp_rule(&sub[k], order, pts, w, 4096, n);
+    }
+    if (n < 0) return n;
+    for (int k = 0; k < n; ++k) {
+      int e = qr_push(r, &pts[3 * k], w[k]);
+      if (e) return e;
+    }
+  }
+  return 0;
+}
+
+enum { QS_ID, QS_XY, QS_XZ, QS_YZ, QS_NONE };
+
+/* GetSufficientCritsQBound / GetExactCritsQBound2D / GetDimensionSwap,
+   :353-457 (pol: 0 FIRST_ALLOWED, 1 FIND_OPTIMAL, 2 ALWAYS_NONE) */
+static int qc_swap(const QL *L, int D, int pol, int consider, int *sw)
+{
+  if (pol == 2) { *sw = QS_NONE; return 0; }
+  if (!consider) { *sw = QS_ID; return 0; }
+  const double (*c)[2][2] = (const double (*)[2][2])L->c;
+  if (D == 2) {
+    int allow[2] = {1, 1};
+    double h = -c[1][0][0] / c[1][1][0];
+    if ((h > 0) && (h < 1)) allow[1] = 0;
+    h = -c[0][1][0] / c[1][1][0];
+    if ((h > 0) && (h < 1)) allow[0] = 0;
+    double qm[2] = {0, 0};
+    static const double cs[4][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}};
+    for (int k = 0; k < 4; ++k) {
+      double g[3];
+      ql_grad(L, cs[k], g);
+      const double qy = fabs(g[1]) / norm3(g), qx = fabs(g[0]) / norm3(g);
+      if (qy > qm[1]) qm[1] = qy;
+      if (qx > qm[0]) qm[0] = qx;
+    }
+    for (int i = 0; i < 2; ++i) if (!allow[i]) qm[i] = 2;
+    if (pol == 0) {
+      *sw = qm[1] < QC_c ? QS_ID : (qm[0] < QC_c ? QS_XY : QS_NONE);
+    } else {
+      if ((qm[0] < QC_c) && (qm[1] < QC_c)) *sw = qm[1] <= qm[0] ? QS_ID : QS_XY;
+      else if (qm[1] < QC_c) *sw = QS_ID;
+      else if (qm[0] < QC_c) *sw = QS_XY;
+      else *sw = QS_NONE;
+    }
+    return 0;
+  }
+  static const double cs[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0},
+                                  {0, 0, 1}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+  double g[8][3];
+  for (int k = 0; k < 8; ++k) ql_grad(L, cs[k], g[k]);
+  double Vsq = 0;
+  for (int d = 0; d < 3; ++d) {
+    double mx = 0;
+    for (int k = 0; k < 8; ++k) {
+      const double v = g[k][d] * g[k][d];
+      if (v > mx) mx = v;
+    }
+    Vsq += mx;
+  }
+  const double V = sqrt(Vsq);
+  double qm[3] = {0, 0, 0};
+  for (int k = 0; k < 8; ++k)
+    for (int d = 0; d < 3; ++d) {
+      const double qe = V * V / (V * V - g[k][d] * g[k][d]);
+      if (qe > qm[d]) qm[d] = qe;
+    }
+  for (int d = 0; d < 3; ++d) qm[d] = sqrt(1 - 1 / qm[d]);
+  for (int d = 0; d < 3; ++d) if (isnan(qm[d])) return OR_ERR_BOUND;
+  if (pol == 0) {
+    *sw = qm[2] < QC_c ? QS_ID : (qm[1] < QC_c ? QS_YZ : (qm[0] < QC_c ? QS_XZ : QS_NONE));
+  } else {
+    /* min_element with comparator <=: the last of equal minima */
+    int md = 0;
+    for (int d = 1; d < 3; ++d) if (qm[d] <= qm[md]) md = d;
+    if (qm[md] < QC_c) *sw = md == 0 ? QS_XZ : (md == 1 ? QS_YZ : QS_ID);
+    else *sw = QS_NONE;
+  }
+  return 0;
+}
+
+/* LevelsetCutQuadrilateral::GetIntegrationRule, :493-501, with the
+   permuted variants :304-351 (coefficients transposed, the rule computed
+   on the unit quad / hex, coordinates swapped back) */
+static int qc_rule(const QL *L, int D, int dt, int order, int pol, int consider, QR *r)
+{
+  int sw;
+  int e = qc_swap(L, D, pol, consider, &sw);
+  if (e) return e;
+  if (sw == QS_ID) return qc_along_xi(L, D, dt, order, pol, r);
+  if (sw == QS_NONE) return qc_fallback(L, D, dt, order, r);
+  QL Lr;
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int k = 0; k < 2; ++k)
+        Lr.c[i][j][k] = sw == QS_XY ? L->c[j][i][k] : (sw == QS_XZ ? L->c[k][j][i] : L->c[i][k][j]);
+  QR rr = {0, 0, NULL, NULL};
+  /* X_Y: GetIntegrationRuleAlongXi; X_Z, Y_Z: GetIntegrationRule of the
+     rotated quad without further swaps, i.e. along xi as well */
+  e = qc_along_xi(&Lr, D, dt, order, pol, &rr);
+  for (int k = 0; k < rr.n && !e; ++k) {
+    const double *p = &rr.x[3 * k];
+    double x[3];
+    if (sw == QS_XY) { x[0] = p[1]; x[1] = p[0]; x[2] = p[2]; }
+    else if (sw == QS_XZ) { x[0] = p[2]; x[1] = p[1]; x[2] = p[0]; }
+    else { x[0] = p[0]; x[1] = p[2]; x[2] = p[1]; }
+    e = qr_push(r, x, rr.w[k]);
+  }
+  qr_free(&rr);
+  return e;
+}
+
+/* multilinear element map of ET_QUAD / ET_HEX: F = dx/dxi at xi, |det F| */
+static void q_jacobian(int D, const double *V, const double *xi, double *F, double *absdet)
+{
+  const int nv = D == 2 ? 4 : 8;
+  for (int k = 0; k < D * D; ++k) F[k] = 0.0;
+  for (int v = 0; v < nv; ++v) {
+    const int b = v & 3;
+    const int a[3] = {(b == 1 || b == 2), b >= 2, v >= 4};
+    double dN[3];
+    for (int c = 0; c < D; ++c) {
+      double t = 1.0;
+      for (int m = 0; m < D; ++m) {
+        const double f = a[m] ? xi[m] : 1.0 - xi[m];
+        const double df = a[m] ? 1.0 : -1.0;
+        t *= (m == c) ? df : f;
+      }
+      dN[c] = t;
+    }
+    for (int rr = 0; rr < D; ++rr)
+      for (int c = 0; c < D; ++c) F[rr * D + c] += V[v * D + rr] * dN[c];
+  }
+  if (D == 2) *absdet = fabs(F[0] * F[3] - F[1] * F[2]);
+  else {
+    const double c1[3] = {F[0], F[3], F[6]}, c2[3] = {F[1], F[4], F[7]}, c3[3] = {F[2], F[5], F[8]};
+    *absdet = fabs(det3(c1, c2, c3));
+  }
+}
+
+static void q_map(int D, const double *V, const double *xi, double *x)
+{
+  const int nv = D == 2 ? 4 : 8;
+  for (int d = 0; d < 3; ++d) x[d] = 0.0;
+  for (int v = 0; v < nv; ++v) {
+    const int b = v & 3;
+    const int a[3] = {(b == 1 || b == 2), b >= 2, v >= 4};
+    double N = 1.0;
+    for (int m = 0; m < D; ++m) N *= a[m] ? xi[m] : 1.0 - xi[m];
+    for (int d = 0; d < D; ++d) x[d] += N * V[v * D + d];
+  }
+}
+
+int or_quad_cut_rule(int D, const double *vals, int dt, int order, int pol, const double *V, double *pts,
+                     double *w, int cap)
+{
+  const int nv = D == 2 ? 4 : 8;
+  const int eldom = or_check_if_straight_cut(nv, vals);
+  QR r = {0, 0, NULL, NULL};
+  if (eldom != OR_IF) {
+    if (eldom != dt) return -1;
+    QQ q;
+    qq_unit(D, &q);
+    int e = qq_plain(&q, order, &r);
+    if (e) { qr_free(&r); return e; }
+  } else {
+    QL L;
+    ql_from_vals(D, vals, &L);
+    int e = qc_rule(&L, D, dt, order, pol, 1, &r);
+    if (e) { qr_free(&r); return e; }
+    if (r.n == 0) { qr_free(&r); return -1; }
+    if (dt == OR_IF) {
+      /* TransformQuadUntrafoToIRInterface, :558-578 */
+      for (int k = 0; k < r.n; ++k) {
+        const double *p = &r.x[3 * k];
+        double g[3];
+        ql_grad(&L, p, g);
+        const double gl = norm3(g);
+        for (int d = 0; d < 3; ++d) g[d] /= gl;
+        double F[9], Finv[9], ad;
+        if (V) {
+          q_jacobian(D, V, p, F, &ad);
+          if (D == 2) {
+            const double det = F[0] * F[3] - F[1] * F[2];
+            Finv[0] = F[3] / det; Finv[1] = -F[1] / det; Finv[2] = -F[2] / det; Finv[3] = F[0] / det;
+          } else {
+            const double det = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+                               F[2] * (F[3] * F[7] - F[4] * F[6]);
+            Finv[0] = (F[4] * F[8] - F[5] * F[7]) / det; Finv[1] = (F[2] * F[7] - F[1] * F[8]) / det;
+            Finv[2] = (F[1] * F[5] - F[2] * F[4]) / det; Finv[3] = (F[5] * F[6] - F[3] * F[8]) / det;
+            Finv[4] = (F[0] * F[8] - F[2] * F[6]) / det; Finv[5] = (F[2] * F[3] - F[0] * F[5]) / det;
+            Finv[6] = (F[3] * F[7] - F[4] * F[6]) / det; Finv[7] = (F[1] * F[6] - F[0] * F[7]) / det;
+            Finv[8] = (F[0] * F[4] - F[1] * F[3]) / det;
+          }
+        }
+        double nn[3] = {0, 0, 0};
+        for (int rr = 0; rr < D; ++rr) {
+          double acc = 0.0;
+          for (int c = 0; c < D; ++c) acc += (V ? Finv[c * D + rr] : (c == rr ? 1.0 : 0.0)) * g[c];
+          nn[rr] = acc;
+        }
+        r.w[k] = r.w[k] * norm3(nn);
+      }
+    }
+  }
+  if (r.n > cap) { qr_free(&r); return OR_ERR_CAP; }
+  memcpy(pts, r.x, sizeof(double) * 3 * (size_t)r.n);
+  memcpy(w, r.w, sizeof(double) * (size_t)r.n);
+  const int n = r.n;
+  qr_free(&r);
+  return n;
+}
+
+/* CutInformation::Update (cutinfo.cpp:61-133) on a quad / hex mesh (el2v:
+   ne x 2^D vertices in the order of the reference element) */
+int or_quad_mark(int D, int ne, const int32_t *el2v, const double *lset_v, int pol, uint8_t *dt_out,
+                 double *cut_ratio, uint32_t *bits)
+{
+  const int nv = D == 2 ? 4 : 8;
+  const int nw = (ne + 31) / 32;
+  memset(bits, 0, sizeof(uint32_t) * 8 * (size_t)nw);
+  double pts[3 * 4096], w[4096];
+  for (int e = 0; e < ne; ++e) {
+    double l[8];
+    for (int k = 0; k < nv; ++k) l[k] = lset_v[el2v[e * nv + k]];
+    double pv[2] = {0.0, 0.0};
+    const int order_np[2] = {OR_POS, OR_NEG};
+    for (int t = 0; t < 2; ++t) {
+      const int np = order_np[t];
+      const int n = or_quad_cut_rule(D, l, np, 0, pol, NULL, pts, w, 4096);
+      if (n < -1) return n;
+      for (int q = 0; q < n; ++q) pv[np] += w[q];
+    }
+    if (cut_ratio) cut_ratio[e] = pv[OR_NEG] / (pv[OR_NEG] + pv[OR_POS]);
+    const int cdt = pv[OR_NEG] > 0.0 ? (pv[OR_POS] > 0.0 ? 4 : 1) : 2;
+    if (dt_out) dt_out[e] = (uint8_t)(cdt == 4 ? OR_IF : (cdt == 1 ? OR_NEG : OR_POS));
+    for (int c = 0; c < 8; ++c)
+      if ((c & cdt) || c == 7) bits[(size_t)c * nw + (e >> 5)] |= 1u << (e & 31);
+  }
+  return 0;
+}
+
+/* IntegrateX sums (python_cutint.cpp:66-98) on a quad / hex element:
+   sum_q |det F(xi_q)| w_q f(x(xi_q)), f a polynomial as or_integrate_poly */
+double or_quad_integrate_poly(int D, const double *V, const double *pts, const double *w, int n, int nterms,
+                              const double *coef, const int32_t *expo)
+{
+  double s = 0.0;
+  for (int q = 0; q < n; ++q) {
+    double F[9], ad, x[3];
+    q_jacobian(D, V, &pts[3 * q], F, &ad);
+    q_map(D, V, &pts[3 * q], x);
+    double fv = 0.0;
+    for (int t = 0; t < nterms; ++t) {
+      double m = coef[t];
+      for (int d = 0; d < 3; ++d)
+        for (int k = 0; k < expo[3 * t + d]; ++k) m *= x[d];
+      fv += m;
+    }
+    s += ad * w[q] * fv;
+  }
+  return s;
 }
 
 /* ====================================================================== */

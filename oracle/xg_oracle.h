@@ -36,6 +36,9 @@ extern "C" {
 #define OR_ERR_NOT_IMPL (-3)   /* "Cutting this part of a tetraeder is not implemented yet!" :193 */
 #define OR_ERR_CAP (-4)        /* output capacity exceeded */
 #define OR_ERR_SUBDIV (-5)     /* "did not expect this.. -3-" xintegration.cpp:1386 */
+#define OR_ERR_BOUND (-6)      /* "if_scale_factor larger than bound" (straightcutrule.cpp:286),
+                                  "Sufficient Criterion calculated nan Bound!" (:438) */
+#define OR_ERR_PLAIN (-7)      /* "You tried to cut a simplex with a plain geometry lset function" (:45-49) */
 
 /* ---- base rules ------------------------------------------------------ */
 int or_rule_npts(int D, int order);
@@ -77,6 +80,22 @@ int or_mlset_mark(int D, int ne, const int32_t *el2v, const double *coords, int 
    uint32 words each, LSB-first.  returns 0 or an OR_ERR code. */
 int or_mark(int D, int ne, const int32_t *el2v, const double *lset_v,
             uint8_t *dt_out, double *cut_ratio, uint32_t *bits);
+
+/* ---- quadrilaterals / hexahedra (straightcutrule.cpp:74-129, 206-501) -- */
+/* vals: 2^D vertex values in the reference element's vertex order (ET_QUAD:
+   (0,0),(1,0),(1,1),(0,1); ET_HEX: the same square at z=0, then at z=1);
+   pol: 0 FIRST_ALLOWED, 1 FIND_OPTIMAL, 2 ALWAYS_NONE.  V (nullable): 2^D x D
+   physical vertex coordinates, used for the interface weights
+   |F^{-T} n_ref| (identity map when NULL).  Points as xyz triples on the
+   unit quad / hex.  returns the number of points, -1 for no rule, or an
+   OR_ERR code. */
+int or_quad_cut_rule(int D, const double *vals, int dt, int order, int pol, const double *V, double *pts,
+                     double *w, int cap);
+/* CutInformation::Update on a quad / hex mesh; outputs as or_mark */
+int or_quad_mark(int D, int ne, const int32_t *el2v, const double *lset_v, int pol, uint8_t *dt_out,
+                 double *cut_ratio, uint32_t *bits);
+double or_quad_integrate_poly(int D, const double *V, const double *pts, const double *w, int n, int nterms,
+                              const double *coef, const int32_t *expo);
 
 /* ---- H1 Lagrange basis ------------------------------------------------- */
 int or_h1_ndof(int D, int p);
