@@ -57,7 +57,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="launch each path 3 times plus a 256 MiB copy, for rocprofv3 --pmc runs")
-    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5"],
+    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5", "extras"],
                     help="c2: 3D plain straight-cut pipeline (headline); c4: learned quadrature on P2-cut tets")
     ap.add_argument("--c4-n", type=int, default=2000000)
     ap.add_argument("--c4-train", type=int, default=100000)
@@ -338,6 +338,117 @@ def bench_c3(args):
     ctx.close()
 
 
+def bench_extras(args):
+    """The paths beside the headline one, each timed on its own with events
+    (inputs resident, 10 launches): node and dof marking on the 64^3 Kuhn
+    mesh (1.57M tets), several level sets on the 32^3 Kuhn mesh, straight cuts
+    of a 64^3 hexahedral mesh.  Memory-bound ones carry the HBM fraction of
+    their algorithmic bytes."""
+    import torch
+
+    from ngsxfem_amd import _abi
+    from ngsxfem_amd import workloads as W
+    from ngsxfem_amd.meshes import MakeStructured3DMesh
+
+    stream = torch.cuda.current_stream()
+    ctx = _abi.Context(0, stream=stream)
+    up = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+
+    def timed(f, reps=10):
+        f()
+        ctx.sync()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            f()
+        b.record(stream)
+        ctx.sync()
+        return a.elapsed_time(b) / reps
+
+    out = {}
+    # ---- node and dof marking (cutinfo.cpp:135-224, 598-647) on the 3D plain mesh
+    mesh, lset = W.config2(64)
+    ne, nv = mesh.ne, mesh.nv
+    e2v, el2e = mesh.edges()
+    f2v, el2f, f2el = mesh.facets()
+    d_el, d_l = up(mesh.el2v), up(lset)
+    dt = torch.empty(ne, dtype=torch.uint8, device="cuda")
+    cr = torch.empty(ne, dtype=torch.float64, device="cuda")
+    bits = torch.empty(8 * ((ne + 31) // 32), dtype=torch.int32, device="cuda")
+    ctx.mark(3, d_el, d_l, dt, cr, bits)
+    tabs = {"vertex": (d_el, nv), "edge": (up(el2e), len(e2v)), "face": (up(el2f), len(f2v))}
+    nodes = {}
+    for name, (tab, nn) in tabs.items():
+        cut = torch.zeros((nn + 31) // 32, dtype=torch.int32, device="cuda")
+        dom = torch.empty(nn, dtype=torch.uint8, device="cuda")
+        ms = timed(lambda: ctx.node_domains(tab, dt, None, nn, cut, dom))
+        # element table + dt read, one flag bit and one domain byte per node written
+        alg = ne * (4 * tab.shape[1] + 1) + nn * (1 + 1 / 8)
+        nodes[name] = {"nodes": nn, "ms": ms, "hbm_frac_algorithmic": alg / (ms * 1e-3) / PEAK_HBM}
+    dofs, ndof = mesh.dofs(2)
+    d_dofs = up(dofs)
+    ifbits = bits[4 * ((ne + 31) // 32):5 * ((ne + 31) // 32)]
+    dmark = torch.empty((ndof + 31) // 32, dtype=torch.int32, device="cuda")
+    ms = timed(lambda: ctx.mark_from_items(d_dofs, ifbits, ndof, dmark))
+    nodes["p2_dofs_of_if_elements"] = {"dofs": ndof, "ms": ms,
+                                       "hbm_frac_algorithmic": (ne * (40 + 1 / 8) + ndof / 8) / (ms * 1e-3) / PEAK_HBM}
+    out["node_and_dof_marking"] = {"mesh": "64^3 x 6 Kuhn tets (%d)" % ne, "paths": nodes}
+    # ---- several level sets (mlsetintegration.cpp) on 32^3 Kuhn tets
+    m2, _ = W.config2(32)
+    X = m2.coords
+    l1 = W.sphere_lset(X)
+    l2 = W.p1_perturb(X @ np.array([0.6, -0.5, 0.62]) - 0.31)
+    d2e, d2x = up(m2.el2v), up(X)
+    L = [up(l1), up(l2)]
+    ml = {}
+    for name, dtts in (("NEG_NEG", [[0, 0]]), ("IF_NEG", [[2, 0]]), ("IF_IF", [[2, 2]])):
+        off = torch.empty(m2.ne + 1, dtype=torch.int32, device="cuda")
+        ctx.mlset_rule_count(3, d2e, L, dtts, 4, off)
+        ctx.sync()
+        tot = int(off[-1].item())
+        pts = torch.empty(max(tot, 1) * 4, dtype=torch.float64, device="cuda")
+
+        def rule():
+            ctx.mlset_rule_count(3, d2e, L, dtts, 4, off)
+            ctx.mlset_rule_fill(3, d2e, d2x, L, dtts, 4, off, pts)
+        ml[name] = {"points": tot, "ms": timed(rule)}
+    ofb = torch.empty((m2.ne + 31) // 32, dtype=torch.int32, device="cuda")
+    cob = torch.empty((m2.ne + 31) // 32, dtype=torch.int32, device="cuda")
+    ml["mark_IF_IF"] = {"ms": timed(lambda: ctx.mlset_mark(3, d2e, d2x, L, [[2, 2]], of_type=ofb, contrib=cob))}
+    out["several_level_sets"] = {"mesh": "32^3 x 6 Kuhn tets (%d), sphere + plane, order 4" % m2.ne, "paths": ml}
+    # ---- quad / hex straight cuts on 64^3 hexahedra
+    h = MakeStructured3DMesh(hexes=True, nx=64, ny=64, nz=64)
+    hl = W.sphere_lset(h.coords)
+    he, hx, hls = up(h.el2v), up(h.coords), up(hl)
+    hdt = torch.empty(h.ne, dtype=torch.uint8, device="cuda")
+    hcr = torch.empty(h.ne, dtype=torch.float64, device="cuda")
+    hbits = torch.empty(8 * ((h.ne + 31) // 32), dtype=torch.int32, device="cuda")
+    hcut = torch.empty(h.ne, dtype=torch.int32, device="cuda")
+    hnc = torch.empty(1, dtype=torch.int32, device="cuda")
+    qd = {"mark": {"ms": timed(lambda: ctx.quad_mark(3, he, hls, 1, hdt, hbits, hcr, hcut, hnc))}}
+    ncut = int(hnc.item())
+    cl = hcut[:ncut].clone()
+    for name, dtv in (("rule_IF", 2), ("rule_NEG", 0)):
+        off = torch.empty(ncut + 1, dtype=torch.int32, device="cuda")
+        ctx.quad_rule_count(3, he, hls, dtv, 4, 1, off, elems=cl)
+        ctx.sync()
+        tot = int(off[-1].item())
+        pts = torch.empty(max(tot, 1) * 4, dtype=torch.float64, device="cuda")
+
+        def rule():
+            ctx.quad_rule_count(3, he, hls, dtv, 4, 1, off, elems=cl)
+            ctx.quad_rule_fill(3, he, hx, hls, dtv, 4, 1, off, pts, elems=cl)
+        ms = timed(rule)
+        qd[name] = {"cut_elements": ncut, "points": tot, "ms": ms, "cut_elem_per_s": ncut / (ms * 1e-3)}
+    out["quad_hex"] = {"mesh": "64^3 hexahedra (%d), the 3D plain level set, order 4, FIND_OPTIMAL" % h.ne,
+                       "paths": qd}
+    print(json.dumps({"metric": METRIC, "config": {"workload": "secondary paths: node / dof marking, several "
+                                                               "level sets, quad / hex straight cuts"},
+                      "paths": out}), flush=True)
+    ctx.close()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -350,6 +461,8 @@ def main():
         return bench_c3(args)
     if args.config == "c5":
         return bench_c5(args)
+    if args.config == "extras":
+        return bench_extras(args)
     import torch
 
     from ngsxfem_amd import _abi
