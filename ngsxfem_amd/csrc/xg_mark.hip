@@ -151,26 +151,38 @@ __device__ __forceinline__ bool part_volumes(const double *l, double *pv) {
 // gathers, so that EPT rows and (D+1) * EPT values per lane are in flight at
 // once -- the kernel is bound by memory latency otherwise.  Elements with a
 // sign change (the only ones that need the cut decomposition) are queued in
-// LDS and processed densely by the first lanes of the block: they cluster in
-// few elements per block, and without the queue every wave holding one of
-// them would run the whole decomposition path.
+// LDS per wave and processed densely by the wave's first lanes: one pass of
+// the decomposition code for up to 64 of them instead of one per group of 64
+// elements holding any, and no block barrier that would hold the other waves
+// while one decomposes.  The 8 bit-array words of each group of 64 elements
+// (two 32-bit halves of 8 ballots) go out in one store instruction per wave,
+// one word per lane.
 constexpr int MARK_BLK = 256;
 constexpr int MARK_EPT = 4;
 
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// (launch bounds: at most 80 VGPRs, 6 waves per SIMD; the decomposition
+// code of the queued elements otherwise sets the register count and leaves
+// 5, and 8 waves need spills -- 6 measured fastest)
 template <int D, int NB, int EPT>
-__global__ __launch_bounds__(NB) void k_mark(int ne, const int32_t *__restrict__ el2v,
+__global__ __launch_bounds__(NB, 6) void k_mark(int ne, const int32_t *__restrict__ el2v,
                                              const double *__restrict__ lset,
                                              uint8_t *__restrict__ dt_out,
                                              double *__restrict__ cut_ratio,
                                              uint32_t *__restrict__ bits, int nw,
                                              int32_t *__restrict__ blk_if, int32_t *err) {
+  static_assert(EPT * 16 <= 64, "one bit-array word per lane: 8 arrays x 2 halves x EPT groups");
   constexpr int NE = NB * EPT;  // elements per block
-  __shared__ int s_cnt[NB / 64];
-  __shared__ int s_nq;
-  __shared__ int16_t s_qi[NE];  // block-local element of queue slot q
-  __shared__ uint8_t s_cdt[NE];  // combined domain bits of queued elements
-  const int tid = threadIdx.x, lane = tid & 63;
-  if (tid == 0) s_nq = 0;
+  constexpr int NW = NB / 64;   // waves per block
+  __shared__ int s_cnt[NW];
+  __shared__ int16_t s_qi[NW][64 * EPT];  // the wave's queued element slots (k * 64 + lane)
+  __shared__ uint8_t s_cdt[NW][64 * EPT];  // combined domain bits of the queued slots
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int base = blockIdx.x * NE;
   // element rows of all EPT elements, then their level-set values.  The
   // loads are unconditional (lanes past the end read the last element
@@ -196,9 +208,9 @@ __global__ __launch_bounds__(NB) void k_mark(int ne, const int32_t *__restrict__
   for (int k = 0; k < EPT; ++k)
 #pragma unroll
     for (int i = 0; i <= D; ++i) l[k][i] = lset[vi[k][i]];
-  __syncthreads();  // s_nq cleared
   int cdt[EPT];
   bool cand[EPT];
+  int nq = 0;  // wave-uniform
 #pragma unroll
   for (int k = 0; k < EPT; ++k) {
     const int e = base + k * NB + tid;
@@ -217,63 +229,61 @@ __global__ __launch_bounds__(NB) void k_mark(int ne, const int32_t *__restrict__
       }
     }
     const unsigned long long mc = __ballot(cand[k]);
-    if (mc) {  // wave-uniform
-      int q0 = 0;
-      if (lane == 0) q0 = atomicAdd(&s_nq, __popcll(mc));
-      q0 = __shfl(q0, 0, 64);
-      if (cand[k])
-        s_qi[q0 + __builtin_amdgcn_mbcnt_hi((uint32_t)(mc >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mc, 0))] =
-            (int16_t)(k * NB + tid);
+    if (cand[k])
+      s_qi[wv][nq + __builtin_amdgcn_mbcnt_hi((uint32_t)(mc >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mc, 0))] =
+          (int16_t)(k * 64 + lane);
+    nq += __popcll(mc);
+  }
+  wave_sync_lds();
+  for (int q0 = 0; q0 < nq; q0 += 64) {  // wave-uniform
+    const int q = q0 + lane;
+    if (q < nq) {
+      const int slot = s_qi[wv][q];
+      const int eq = base + (slot >> 6) * NB + wv * 64 + (slot & 63);
+      double lq[D + 1];
+      load_lset<D>(el2v, lset, eq, lq);  // cache hits: gathered above
+      double pv[2];
+      if (!part_volumes<D>(lq, pv)) atomicOr(err, 1);
+      if (cut_ratio) cut_ratio[eq] = pv[NEG] / (pv[NEG] + pv[POS]);
+      // cutinfo.cpp:104-124
+      int c;
+      if (pv[NEG] > 0.0)
+        c = (pv[POS] > 0.0) ? 4 : 1;
+      else
+        c = 2;
+      if (dt_out) dt_out[eq] = (uint8_t)(c == 4 ? IF : (c == 1 ? NEG : POS));
+      s_cdt[wv][slot] = (uint8_t)c;
     }
   }
-  __syncthreads();
-  const int nq = s_nq;
-  for (int q = tid; q < nq; q += NB) {
-    const int li = s_qi[q];
-    const int eq = base + li;
-    double lq[D + 1];
-    load_lset<D>(el2v, lset, eq, lq);  // cache hits: gathered above
-    double pv[2];
-    if (!part_volumes<D>(lq, pv)) atomicOr(err, 1);
-    if (cut_ratio) cut_ratio[eq] = pv[NEG] / (pv[NEG] + pv[POS]);
-    // cutinfo.cpp:104-124
-    int c;
-    if (pv[NEG] > 0.0)
-      c = (pv[POS] > 0.0) ? 4 : 1;
-    else
-      c = 2;
-    if (dt_out) dt_out[eq] = (uint8_t)(c == 4 ? IF : (c == 1 ? NEG : POS));
-    s_cdt[li] = (uint8_t)c;
-  }
-  __syncthreads();
+  wave_sync_lds();
+  // combined arrays (cutinfo.cpp:72-78,127-132): bit set iff (c & cdt) != 0,
+  // CDOM_ANY everywhere, CDOM_NO nowhere.  Lane L stores half (L & 1) of
+  // array (L >> 1) & 7 for group L >> 4.
+  const int mk = lane >> 4, mc = (lane >> 1) & 7, mh = lane & 1;
+  uint32_t word = 0;
   int nif = 0;
 #pragma unroll
   for (int k = 0; k < EPT; ++k) {
-    const int e = base + k * NB + tid;
-    const bool valid = e < ne;
-    if (cand[k]) cdt[k] = s_cdt[k * NB + tid];
-    const int wave_e0 = e - lane;  // multiple of 64
+    const bool valid = base + k * NB + tid < ne;
+    if (cand[k]) cdt[k] = s_cdt[wv][k * 64 + lane];
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-      // combined arrays (cutinfo.cpp:72-78,127-132): bit set iff
-      // (c & cdt) != 0, CDOM_ANY everywhere, CDOM_NO nowhere
-      const bool on = valid && ((c == 7) || (c & cdt[k]));
-      const unsigned long long m = __ballot(on);
-      if (lane == 0) {
-        const int w = wave_e0 >> 5;
-        uint32_t *row = bits + (size_t)c * nw;
-        if (w < nw) row[w] = (uint32_t)(m & 0xffffffffull);
-        if (w + 1 < nw) row[w + 1] = (uint32_t)(m >> 32);
-      }
+      const unsigned long long m = __ballot(valid && ((c == 7) || (c & cdt[k])));
+      const uint32_t h = mh ? (uint32_t)(m >> 32) : (uint32_t)(m & 0xffffffffull);
+      word = (mk == k && mc == c) ? h : word;
     }
     nif += __popcll(__ballot(valid && cdt[k] == 4));
   }
-  if (lane == 0) s_cnt[tid >> 6] = nif;
+  {
+    const int w = ((base + mk * NB + wv * 64) >> 5) + mh;
+    if (lane < 16 * EPT && w < nw) bits[(size_t)mc * nw + w] = word;
+  }
+  if (lane == 0) s_cnt[wv] = nif;
   __syncthreads();
   if (tid == 0 && blk_if) {
     int s = 0;
 #pragma unroll
-    for (int k = 0; k < NB / 64; ++k) s += s_cnt[k];
+    for (int k = 0; k < NW; ++k) s += s_cnt[k];
     blk_if[blockIdx.x] = s;
   }
 }
