@@ -424,18 +424,25 @@ __global__ __launch_bounds__(64) void k_subdiv(int nel, const double *__restrict
   constexpr int NM = Moments<3, PM>::NM;
   extern __shared__ char smem[];
   SdLds S;
-  const int vol_nq = rule_cnt(3, order), if_nq = rule_cnt(2, order);
+  // the moments are integrals of degree-2 PM polynomials over simplices:
+  // once the order reaches 2 PM every exact rule gives them, so the minimal
+  // symmetric rule of degree 2 PM (tet 24 instead of 64 points, triangle 12
+  // instead of 16 for PM = 3) replaces the conical one of the order
+  const bool minimal = order >= 2 * PM;
+  const int vol_nq = minimal ? xg_minrule_cnt_d[3][PM] : rule_cnt(3, order);
+  const int if_nq = minimal ? xg_minrule_cnt_d[2][PM] : rule_cnt(2, order);
   sd_bytes(L, vol_nq, if_nq, &S, smem);
   const int lane = threadIdx.x & 63;
   const int n = 1 << L;
   const int nlat = sd_nlat(L);
   const int loff = xg_lat_off_d[L];
   {
-    const int ov = rule_off(3, order), oi = rule_off(2, order);
+    const double(*src_v)[4] = minimal ? &xg_minrule_data_d[xg_minrule_off_d[3][PM]] : &xg_rule_data_d[rule_off(3, order)];
+    const double(*src_i)[4] = minimal ? &xg_minrule_data_d[xg_minrule_off_d[2][PM]] : &xg_rule_data_d[rule_off(2, order)];
     for (int k = lane; k < 4 * (vol_nq + if_nq); k += 64) {
       const int q = k >> 2, c = k & 3;
-      if (q < vol_nq) S.rule_vol[q][c] = xg_rule_data_d[ov + q][c];
-      else S.rule_if[q - vol_nq][c] = xg_rule_data_d[oi + q - vol_nq][c];
+      if (q < vol_nq) S.rule_vol[q][c] = src_v[q][c];
+      else S.rule_if[q - vol_nq][c] = src_i[q - vol_nq][c];
     }
     wave_sync();
   }
@@ -725,7 +732,9 @@ __global__ __launch_bounds__(64) void k_subdiv(int nel, const double *__restrict
 template <int LO, int PM, int WHICH, int OUT>
 int launch_subdiv(xg_ctx *ctx, int nel, const double *lset_nodal, int L, int order, const double *finv,
                   double *out_neg, double *out_if, uint8_t *dom) {
-  const size_t lds = sd_bytes(L, xg_rule_cnt[3][order / 2 + 1], xg_rule_cnt[2][order / 2 + 1], nullptr, nullptr);
+  const bool minimal = order >= 2 * PM;  // as in k_subdiv
+  const size_t lds = sd_bytes(L, minimal ? xg_minrule_cnt[3][PM] : xg_rule_cnt[3][order / 2 + 1],
+                              minimal ? xg_minrule_cnt[2][PM] : xg_rule_cnt[2][order / 2 + 1], nullptr, nullptr);
   if (lds > 65536)
     XG_CHECK(hipFuncSetAttribute((const void *)k_subdiv<LO, PM, WHICH, OUT>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
