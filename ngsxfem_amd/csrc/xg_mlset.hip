@@ -40,12 +40,20 @@ struct MlsetArgs {
   int8_t dtt[MAXDOM][MAXLS];
 };
 
-// simplex buffers of one wave
-struct MlLds {
-  double sa[SCAP][4][3];
-  double sb[SCAP][4][3];
-  int cnt[64];
-};
+// simplex buffers of one wave, in dynamic LDS: two buffers of scap
+// sub-simplices each (scap = min(3^M, SCAP): every cut by one level set
+// leaves at most three children per simplex), then two counters.  Sizing by
+// M keeps the block small enough for many waves per CU; a fixed 2 x 256
+// simplices (49 KB) allowed three.
+__host__ __device__ inline size_t ml_lds_bytes(int scap) {
+  return sizeof(double) * 2 * (size_t)scap * 12 + sizeof(int) * 2;
+}
+
+inline int ml_scap(int M) {
+  int c = 1;
+  for (int j = 0; j < M && c < SCAP; ++j) c *= 3;
+  return c < SCAP ? c : SCAP;
+}
 
 enum { MODE_COUNT = 0, MODE_FILL = 1, MODE_WSUM = 2 };
 
@@ -78,8 +86,11 @@ __global__ __launch_bounds__(64) void k_mlset(int nel, const int32_t *__restrict
                                               const int32_t *__restrict__ el2v, const double *__restrict__ coords,
                                               MlsetArgs A, int order, int32_t *__restrict__ offsets,
                                               double4 *__restrict__ pts, double *__restrict__ wsum,
-                                              int32_t *err) {
-  __shared__ MlLds L;
+                                              int32_t *err, int scap) {
+  extern __shared__ double ml_smem[];
+  double(*const buf_a)[4][3] = reinterpret_cast<double(*)[4][3]>(ml_smem);
+  double(*const buf_b)[4][3] = buf_a + scap;
+  int *const lcnt = reinterpret_cast<int *>(buf_b + scap);
   const int lane = threadIdx.x;
   for (int i = blockIdx.x; i < nel; i += gridDim.x) {
     const int e = elems ? elems[i] : i;
@@ -134,8 +145,8 @@ __global__ __launch_bounds__(64) void k_mlset(int nel, const int32_t *__restrict
       // level 0: the element cut by the first cutting level set (its
       // element values, no clamp)
       int dim = D, ncur = 0;
-      double(*cur)[4][3] = L.sa;
-      double(*nxt)[4][3] = L.sb;
+      double(*cur)[4][3] = buf_a;
+      double(*nxt)[4][3] = buf_b;
       if (lane == 0) {
         double P[4][3];
         for (int k = 0; k <= D; ++k)
@@ -149,12 +160,12 @@ __global__ __launch_bounds__(64) void k_mlset(int nel, const int32_t *__restrict
           hn |= lv[cl[0]][k] < 0.0;
         }
         if (hz && hn) b0 = true;
-        L.cnt[0] = b0 ? 0 : children(D, P, lv[cl[0]], dt0, cur, b0);
-        L.cnt[1] = b0;
+        lcnt[0] = b0 ? 0 : children(D, P, lv[cl[0]], dt0, cur, b0);
+        lcnt[1] = b0;
       }
       __syncthreads();
-      ncur = L.cnt[0];
-      bad |= L.cnt[1] != 0;
+      ncur = lcnt[0];
+      bad |= lcnt[1] != 0;
       __syncthreads();
       if (A.dtt[t][cl[0]] == IF) --dim;
       bool over = false;
@@ -193,7 +204,7 @@ __global__ __launch_bounds__(64) void k_mlset(int nel, const int32_t *__restrict
           }
           int tot = 0;
           const int off = wave_scan(nch, &tot);
-          if (nn + tot > SCAP) {
+          if (nn + tot > scap) {
             over = true;
             break;
           }
@@ -351,10 +362,12 @@ int launch_ml(xg_ctx *ctx, int D, int nel, const int32_t *elems, const int32_t *
               const MlsetArgs &A, int order, int32_t *offsets, double *pts, double *wsum) {
   const int g = grid_for(nel);
   double4 *o = reinterpret_cast<double4 *>(pts);
+  const int scap = ml_scap(A.M);
+  const size_t lds = ml_lds_bytes(scap);
   switch (D) {
-    case 1: k_mlset<1, MODE><<<g, 64, 0, ctx->stream>>>(nel, elems, el2v, coords, A, order, offsets, o, wsum, ctx->d_err); break;
-    case 2: k_mlset<2, MODE><<<g, 64, 0, ctx->stream>>>(nel, elems, el2v, coords, A, order, offsets, o, wsum, ctx->d_err); break;
-    default: k_mlset<3, MODE><<<g, 64, 0, ctx->stream>>>(nel, elems, el2v, coords, A, order, offsets, o, wsum, ctx->d_err); break;
+    case 1: k_mlset<1, MODE><<<g, 64, lds, ctx->stream>>>(nel, elems, el2v, coords, A, order, offsets, o, wsum, ctx->d_err, scap); break;
+    case 2: k_mlset<2, MODE><<<g, 64, lds, ctx->stream>>>(nel, elems, el2v, coords, A, order, offsets, o, wsum, ctx->d_err, scap); break;
+    default: k_mlset<3, MODE><<<g, 64, lds, ctx->stream>>>(nel, elems, el2v, coords, A, order, offsets, o, wsum, ctx->d_err, scap); break;
   }
   XG_LAUNCH_CHECK();
   return XG_OK;
