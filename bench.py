@@ -643,7 +643,8 @@ def main():
         "roofline": roof, "cpu_baseline": cpu, "paths": paths,
         "roofline_plain_mfma": roof_mfma,
         "marking": {"ms": mark_ms, "tets_per_s": ne / (mark_ms * 1e-3),
-                    "hbm_frac_algorithmic": ne * 26.4 / (mark_ms * 1e-3) / PEAK_HBM},
+                    "hbm_frac_algorithmic": ne * 26.4 / (mark_ms * 1e-3) / PEAK_HBM,
+                    "k_mark_traffic": read_traffic("marking_c2_pmc.txt")},
     }
     print(json.dumps(res), flush=True)
     ctx.close()
