@@ -180,6 +180,14 @@ int xg_quad_integrate(xg_ctx *ctx, int D, int nel, const int32_t *elems, const i
                       int nterms, const double *coef, const int32_t *expo, double *per_elem,
                       double *total);
 
+/* CalcElementMatrix (xfem/symboliccutbfi.cpp:76-304) of the Q1 space of a
+   quad / hex mesh on a rule from xg_quad_rule_fill (offsets, pts): mask 1 mass,
+   2 stiffness, 3 their sum; out: nel x 2^D x 2^D (FP64 for XG_PREC_FP64, FP32
+   for XG_PREC_FP32), dofs in the element's vertex order. */
+int xg_quad_elmat(xg_ctx *ctx, int D, int nel, const int32_t *elems, const int32_t *el2v,
+                  const double *coords, const int32_t *offsets, const double *pts, int mask,
+                  int precision, void *out);
+
 /* ---- several level sets on simplices ------------------------------------
    CreateMultiLevelsetCutIntegrationRule (cutint/mlsetintegration.cpp:8-158)
    with StraightCutsIntegrationRule (cutint/straightcutrule.cpp:679-800).

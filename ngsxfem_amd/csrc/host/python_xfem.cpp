@@ -558,16 +558,19 @@ void build_rule(MA ma, const double *d_lset, int dt, int order, const std::vecto
 
 // the straight cut rule of a quadrilateral / hexahedral mesh
 // (LevelsetCutQuadrilateral, straightcutrule.cpp:206-501)
-void build_quad_rule(MA ma, const double *d_lset, int dt, int order, int policy, DeviceRule &r) {
-  r.nel = ma->ne;
+void build_quad_rule(MA ma, const double *d_lset, int dt, int order, int policy, DeviceRule &r,
+                     const std::vector<int32_t> *elems = nullptr) {
+  r.nel = elems ? (int)elems->size() : ma->ne;
+  if (elems) r.elems.upload(elems->data(), elems->size());
+  const int32_t *el = elems ? r.elems.p : nullptr;
   r.offsets.resize(r.nel + 1);
-  check(xg_quad_rule_count(ctx(), ma->dim, r.nel, nullptr, ma->el2v.p, d_lset, dt, order, policy, r.offsets.p));
+  check(xg_quad_rule_count(ctx(), ma->dim, r.nel, el, ma->el2v.p, d_lset, dt, order, policy, r.offsets.p));
   check(xg_sync(ctx()));
   int32_t tot = 0;
   hipcheck(hipMemcpy(&tot, r.offsets.p + r.nel, sizeof(int32_t), hipMemcpyDeviceToHost), "rule total");
   r.total = tot;
   r.pts.resize((size_t)std::max(tot, 1) * 4);
-  check(xg_quad_rule_fill(ctx(), ma->dim, r.nel, nullptr, ma->el2v.p, ma->coords.p, d_lset, dt, order, policy,
+  check(xg_quad_rule_fill(ctx(), ma->dim, r.nel, el, ma->el2v.p, ma->coords.p, d_lset, dt, order, policy,
                           r.offsets.p, r.pts.p));
   check(xg_sync(ctx()));
 }
@@ -994,6 +997,43 @@ py::tuple CutElementMatrices(py::object mesh, py::object lset, int dt, int order
 // element matrices on the rule of several level sets (the dCut forms of a
 // LevelsetIntegrationDomain with several level sets): GPU mlset rule, then
 // the element-matrix kernel on the materialised rule
+std::vector<int32_t> element_list(py::object elements, int ne);
+
+// element matrices of the Q1 space on a quad / hex mesh: the straight cut
+// rule of the elements (xg_quad_rule_*), then mass / stiffness on it
+// (xg_quad_elmat)
+py::tuple QuadElementMatrices(py::object mesh, py::object lset, int dt, int order, int policy, int mask,
+                              py::object elements, int precision) {
+  MA ma = as_mesh(mesh);
+  if (!ma->quad) throw std::runtime_error("QuadElementMatrices: a quadrilateral / hexahedral mesh");
+  bool is_p1 = true;
+  std::vector<double> v = p1_values(lset, ma->nv, is_p1);
+  if (!is_p1) throw std::runtime_error("QuadElementMatrices: P1 (Q1) level sets only");
+  DevBuf<double> d_l;
+  d_l.upload(v.data(), v.size());
+  std::vector<int32_t> el = element_list(elements, ma->ne);
+  DeviceRule r;
+  build_quad_rule(ma, d_l.p, dt, order, policy, r, &el);
+  const int nel = (int)el.size(), nd = ma->nvel;
+  py::array_t<int32_t> elarr((py::ssize_t)nel, el.data());
+  if (precision == XG_PREC_FP64) {
+    DevBuf<double> out((size_t)std::max(nel, 1) * nd * nd);
+    check(xg_quad_elmat(ctx(), ma->dim, nel, r.elems.p, ma->el2v.p, ma->coords.p, r.offsets.p, r.pts.p, mask,
+                        precision, out.p));
+    check(xg_sync(ctx()));
+    py::array_t<double> h({(py::ssize_t)nel, (py::ssize_t)nd, (py::ssize_t)nd});
+    out.download(h.mutable_data(), (size_t)nel * nd * nd);
+    return py::make_tuple(elarr, h);
+  }
+  DevBuf<float> out((size_t)std::max(nel, 1) * nd * nd);
+  check(xg_quad_elmat(ctx(), ma->dim, nel, r.elems.p, ma->el2v.p, ma->coords.p, r.offsets.p, r.pts.p, mask,
+                      XG_PREC_FP32, out.p));
+  check(xg_sync(ctx()));
+  py::array_t<float> h({(py::ssize_t)nel, (py::ssize_t)nd, (py::ssize_t)nd});
+  out.download(h.mutable_data(), (size_t)nel * nd * nd);
+  return py::make_tuple(elarr, h);
+}
+
 py::tuple MlsetElementMatrices(py::object mesh, py::object lsets, py::object dts, int order, int p, int mask,
                                py::object elements, int precision) {
   MA ma = as_mesh(mesh);
@@ -1382,4 +1422,7 @@ PYBIND11_MODULE(xfem, m) {
   m.def("CutElementMatrices", &CutElementMatrices, py::arg("mesh"), py::arg("levelset"), py::arg("domain_type"),
         py::arg("order"), py::arg("p"), py::arg("mask") = 3, py::arg("elements") = py::none(),
         py::arg("precision") = XG_PREC_FP32);
+  m.def("QuadElementMatrices", &QuadElementMatrices, py::arg("mesh"), py::arg("lset"), py::arg("domain_type"),
+        py::arg("order"), py::arg("policy") = (int)FIND_OPTIMAL, py::arg("mask") = 3,
+        py::arg("elements") = py::none(), py::arg("precision") = (int)XG_PREC_FP64);
 }

@@ -721,6 +721,89 @@ int waves_grid(int nel) {
   return (int)(blocks > 65536 ? 65536 : (blocks < 1 ? 1 : blocks));
 }
 
+// SymbolicCutBilinearFormIntegrator::CalcElementMatrix (xfem/symboliccutbfi.cpp:
+// 76-304) for the Q1 space of quads / hexes on a materialised cut rule: one
+// lane per element, its points in turn; out = sum_q |det F| w_q (sm phi_a
+// phi_b + sk grad phi_a . grad phi_b), grad phi = F^{-T} grad_ref phi, the
+// upper triangle accumulated in FP64 and mirrored (row = test dof)
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_quad_elmat(int nel, const int32_t *__restrict__ elems,
+                                                    const int32_t *__restrict__ el2v,
+                                                    const double *__restrict__ coords,
+                                                    const int32_t *__restrict__ offsets,
+                                                    const double4 *__restrict__ pts, int mask, T *__restrict__ out) {
+  constexpr int NV = 1 << D;
+  constexpr int NU = NV * (NV + 1) / 2;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nel) return;
+  const int e = elems ? elems[i] : i;
+  double V[NV][3];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const int v = el2v[(size_t)e * NV + k];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) V[k][d] = d < D ? coords[(size_t)v * D + d] : 0.0;
+  }
+  const double sm = (mask & 1) ? 1.0 : 0.0, sk = (mask & 2) ? 1.0 : 0.0;
+  double A[NU];
+#pragma unroll
+  for (int k = 0; k < NU; ++k) A[k] = 0.0;
+  for (int q = offsets[i]; q < offsets[i + 1]; ++q) {
+    const double4 p = pts[q];
+    const double xi[3] = {p.x, p.y, p.z};
+    double F[D * D], Finv[D * D], x[3];
+    elem_map<D>(V, xi, F, x);
+    const double det = det_inv<D>(F, Finv);
+    const double wq = fabs(det) * p.w;
+    double phi[NV], g[NV][D];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const int b = v & 3;
+      const int a[3] = {(b == 1 || b == 2), b >= 2, v >= 4};
+      double N = 1.0;
+#pragma unroll
+      for (int m = 0; m < D; ++m) N *= a[m] ? xi[m] : 1.0 - xi[m];
+      phi[v] = N;
+      double gr[D];
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        double t = 1.0;
+#pragma unroll
+        for (int m = 0; m < D; ++m) t *= (m == c) ? (a[m] ? 1.0 : -1.0) : (a[m] ? xi[m] : 1.0 - xi[m]);
+        gr[c] = t;
+      }
+      // physical gradient: (F^{-T} gr)_r = sum_c Finv[c][r] gr[c]
+#pragma unroll
+      for (int r = 0; r < D; ++r) {
+        double acc = 0.0;
+#pragma unroll
+        for (int c = 0; c < D; ++c) acc += Finv[c * D + r] * gr[c];
+        g[v][r] = acc;
+      }
+    }
+    int u = 0;
+#pragma unroll
+    for (int a = 0; a < NV; ++a)
+#pragma unroll
+      for (int b = a; b < NV; ++b) {
+        double gg = 0.0;
+#pragma unroll
+        for (int r = 0; r < D; ++r) gg += g[a][r] * g[b][r];
+        A[u++] += wq * (sm * (phi[a] * phi[b]) + sk * gg);
+      }
+  }
+  T *o = out + (size_t)i * NV * NV;
+  int u = 0;
+#pragma unroll
+  for (int a = 0; a < NV; ++a)
+#pragma unroll
+    for (int b = a; b < NV; ++b) {
+      o[a * NV + b] = (T)A[u];
+      o[b * NV + a] = (T)A[u];
+      ++u;
+    }
+}
+
 bool quad_args_ok(int D, int dt, int order, int pol) {
   return (D == 2 || D == 3) && dt >= 0 && dt <= 2 && order >= 0 && order / 2 + 1 <= XG_RULE_NMAX && pol >= 0 &&
          pol <= 2;
@@ -809,6 +892,27 @@ extern "C" int xg_quad_integrate(xg_ctx *ctx, int D, int nel, const int32_t *ele
     k_quad_integrate<2><<<grid, 256, 0, ctx->stream>>>(nel, elems, el2v, coords, offsets, p, f, per_elem, total);
   else
     k_quad_integrate<3><<<grid, 256, 0, ctx->stream>>>(nel, elems, el2v, coords, offsets, p, f, per_elem, total);
+  XG_LAUNCH_CHECK();
+  return XG_OK;
+}
+
+extern "C" int xg_quad_elmat(xg_ctx *ctx, int D, int nel, const int32_t *elems, const int32_t *el2v,
+                             const double *coords, const int32_t *offsets, const double *pts, int mask,
+                             int precision, void *out) {
+  if (!ctx || (D != 2 && D != 3) || nel < 0 || mask < 1 || mask > 3 ||
+      (precision != XG_PREC_FP32 && precision != XG_PREC_FP64))
+    return set_error(XG_ERR_ARG, "xg_quad_elmat: bad argument");
+  if (nel == 0) return XG_OK;
+  if (!el2v || !coords || !offsets || !pts || !out) return set_error(XG_ERR_ARG, "xg_quad_elmat: null pointer");
+  const int grid = (nel + 255) / 256;
+  const double4 *p = reinterpret_cast<const double4 *>(pts);
+  if (precision == XG_PREC_FP64) {
+    if (D == 2) k_quad_elmat<2, double><<<grid, 256, 0, ctx->stream>>>(nel, elems, el2v, coords, offsets, p, mask, (double *)out);
+    else k_quad_elmat<3, double><<<grid, 256, 0, ctx->stream>>>(nel, elems, el2v, coords, offsets, p, mask, (double *)out);
+  } else {
+    if (D == 2) k_quad_elmat<2, float><<<grid, 256, 0, ctx->stream>>>(nel, elems, el2v, coords, offsets, p, mask, (float *)out);
+    else k_quad_elmat<3, float><<<grid, 256, 0, ctx->stream>>>(nel, elems, el2v, coords, offsets, p, mask, (float *)out);
+  }
   XG_LAUNCH_CHECK();
   return XG_OK;
 }

@@ -33,7 +33,7 @@ from .xfem import (ANY, BND, CDOM_IF, CDOM_NEG, CDOM_POS, FALLBACK, FIRST, HASNE
                    CutElementMatrices, CutElementVectors, CutInfo, DOMAIN_TYPE, GetDofsOfElements,
                    GetDofsOfFacets, GetElementsWithNeighborFacets, GetElementsWithSharedVertex,
                    GetFacetsWithNeighborTypes, IntegrateX, MlsetElementMatrices, MultiLevelsetCutInfo, NODE_TYPE,
-                   SubdivElementMatrices,
+                   QuadElementMatrices, SubdivElementMatrices,
                    NT_CELL, NT_EDGE, NT_ELEMENT, NT_FACE, NT_FACET, NT_VERTEX, PerturbP1, QUAD_DIRECTION_POLICY,
                    VorB)
 
@@ -425,7 +425,17 @@ class _CutIntegral:
         mask = (1 if "mass" in self.integrand.parts else 0) | (2 if "laplace" in self.integrand.parts else 0)
         coefs = self.integrand.parts
         L = self.measure.lsetdom.get("subdivlvl", 0) if isinstance(self.measure, _CutDifferentialSymbol) else 0
-        if isinstance(lset, (tuple, list)):
+        if getattr(mesh, "is_quad", False):
+            # quadrilaterals / hexahedra: the Q1 space on the quad / hex cut rule
+            if p != 1 or isinstance(lset, (tuple, list)) or L > 0:
+                raise NotImplementedError("quad / hex meshes: order-1 spaces, one P1 level set, subdivlvl 0")
+            pol = self.measure.lsetdom.get("quad_dir_policy", OPTIMAL) if isinstance(
+                self.measure, _CutDifferentialSymbol) else OPTIMAL
+
+            def kernel(msk):
+                return QuadElementMatrices(mesh, lset, dt, order if order is not None else 2, int(pol), msk, els,
+                                           precision)
+        elif isinstance(lset, (tuple, list)):
             # several level sets: dt holds the domain tuples
             def kernel(msk):
                 return MlsetElementMatrices(mesh, tuple(lset), dt, order, p, msk, els, precision)

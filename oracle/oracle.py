@@ -42,6 +42,8 @@ def _load():
     lib.or_quad_integrate_poly.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp,
                                            ctypes.POINTER(ctypes.c_int)]
     lib.or_quad_integrate_poly.restype = ctypes.c_double
+    lib.or_quad_elmat.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp]
+    lib.or_quad_elmat.restype = None
     lib.or_mlset_rule.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _ip, ctypes.c_int, _dp, _dp,
                                   ctypes.c_double, _dp, _dp, ctypes.c_int]
     lib.or_mlset_mark.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _dp, ctypes.c_int, _dp, ctypes.c_int,
@@ -187,6 +189,15 @@ def quad_integrate_poly(D, V, pts, w, terms):
     ex = np.ascontiguousarray([e for t in terms for e in t[1]], dtype=np.int32)
     return lib().or_quad_integrate_poly(D, _d(_f64(V)), _d(_f64(pts)), _d(_f64(w)), len(w), len(terms), _d(coef),
                                         ex.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+
+
+def quad_elmat(D, V, pts, w, mask=3):
+    """Q1 element matrix on a quad / hex element and a unit-element rule"""
+    nv = 2 ** D
+    out = np.zeros((nv, nv))
+    if len(w):
+        lib().or_quad_elmat(D, _d(_f64(V)), _d(_f64(pts)), _d(_f64(w)), len(w), mask, _d(out))
+    return out
 
 
 def quad_integrate_mesh(coords, el2v, lset_v, dt, order, terms=((1.0, (0, 0, 0)),), pol=1, element_wise=False):

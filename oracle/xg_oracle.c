@@ -1296,6 +1296,58 @@ double or_quad_integrate_poly(int D, const double *V, const double *pts, const d
   return s;
 }
 
+/* CalcElementMatrix (symboliccutbfi.cpp:76-304) of the Q1 space on a quad /
+   hex element and a rule on the unit element: mass (mask 1), stiffness
+   (mask 2) or their sum, out (2^D x 2^D) overwritten */
+void or_quad_elmat(int D, const double *V, const double *pts, const double *w, int n, int mask, double *out)
+{
+  const int nv = D == 2 ? 4 : 8;
+  memset(out, 0, sizeof(double) * (size_t)(nv * nv));
+  for (int q = 0; q < n; ++q) {
+    const double *xi = &pts[3 * q];
+    double F[9], ad, Finv[9];
+    q_jacobian(D, V, xi, F, &ad);
+    if (D == 2) {
+      const double det = F[0] * F[3] - F[1] * F[2];
+      Finv[0] = F[3] / det; Finv[1] = -F[1] / det; Finv[2] = -F[2] / det; Finv[3] = F[0] / det;
+    } else {
+      const double det = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+                         F[2] * (F[3] * F[7] - F[4] * F[6]);
+      Finv[0] = (F[4] * F[8] - F[5] * F[7]) / det; Finv[1] = (F[2] * F[7] - F[1] * F[8]) / det;
+      Finv[2] = (F[1] * F[5] - F[2] * F[4]) / det; Finv[3] = (F[5] * F[6] - F[3] * F[8]) / det;
+      Finv[4] = (F[0] * F[8] - F[2] * F[6]) / det; Finv[5] = (F[2] * F[3] - F[0] * F[5]) / det;
+      Finv[6] = (F[3] * F[7] - F[4] * F[6]) / det; Finv[7] = (F[1] * F[6] - F[0] * F[7]) / det;
+      Finv[8] = (F[0] * F[4] - F[1] * F[3]) / det;
+    }
+    double phi[8], g[8][3];
+    for (int v = 0; v < nv; ++v) {
+      const int b = v & 3;
+      const int a[3] = {(b == 1 || b == 2), b >= 2, v >= 4};
+      double N = 1.0;
+      for (int m = 0; m < D; ++m) N *= a[m] ? xi[m] : 1.0 - xi[m];
+      phi[v] = N;
+      double gr[3];
+      for (int c = 0; c < D; ++c) {
+        double t = 1.0;
+        for (int m = 0; m < D; ++m) t *= (m == c) ? (a[m] ? 1.0 : -1.0) : (a[m] ? xi[m] : 1.0 - xi[m]);
+        gr[c] = t;
+      }
+      for (int r = 0; r < D; ++r) {
+        double acc = 0.0;
+        for (int c = 0; c < D; ++c) acc += Finv[c * D + r] * gr[c];
+        g[v][r] = acc;
+      }
+    }
+    const double wq = ad * w[q];
+    for (int a = 0; a < nv; ++a)
+      for (int b = 0; b < nv; ++b) {
+        double gg = 0.0;
+        for (int r = 0; r < D; ++r) gg += g[a][r] * g[b][r];
+        out[a * nv + b] += wq * (((mask & 1) ? phi[a] * phi[b] : 0.0) + ((mask & 2) ? gg : 0.0));
+      }
+  }
+}
+
 /* ====================================================================== */
 /* H1 Lagrange basis of degree p on the D-simplex                          */
 /* dof order: vertex nodes, then edge nodes (edges (0,1),(0,2),(0,3),(1,2),*/

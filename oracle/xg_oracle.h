@@ -96,6 +96,9 @@ int or_quad_mark(int D, int ne, const int32_t *el2v, const double *lset_v, int p
                  double *cut_ratio, uint32_t *bits);
 double or_quad_integrate_poly(int D, const double *V, const double *pts, const double *w, int n, int nterms,
                               const double *coef, const int32_t *expo);
+/* Q1 element matrix (mass: mask 1, stiffness: 2, sum: 3) on a rule of the
+   unit element; out: 2^D x 2^D */
+void or_quad_elmat(int D, const double *V, const double *pts, const double *w, int n, int mask, double *out);
 
 /* ---- H1 Lagrange basis ------------------------------------------------- */
 int or_h1_ndof(int D, int p);

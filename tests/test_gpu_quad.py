@@ -257,11 +257,83 @@ def test_cutinfo_quads(X, orc, D, n):
 
 
 def test_simplex_only_paths_raise(X):
+    """Q2 spaces and several level sets are not built on quads: they raise"""
     mesh = MakeStructured2DMesh(quads=True, nx=4, ny=4)
     g = p1(X, mesh, X.x - 0.5)
-    V = X.H1(mesh, order=1)
+    V = X.H1(mesh, order=2)
     u, v = V.TnT()
     a = X.BilinearForm(V)
-    a += u * v * X.dCut(g, X.NEG, order=2)
+    a += u * v * X.dCut(g, X.NEG, order=4)
     with pytest.raises(Exception):
         a.Assemble()
+
+
+QV2 = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], float)
+
+
+def test_quad_elmat_uncut_known(dev, orc):
+    """Q1 mass (1/36 [4 2 1 2; ...]) and stiffness (1/6 [4 -1 -2 -1; ...]) of
+    the unit square, and of the unit cube (sum of all mass entries 1, K 1 = 0)"""
+    torch, _abi, ctx = dev
+    for D, n in ((2, 1), (3, 1)):
+        mesh = MakeStructured2DMesh(quads=True, nx=n, ny=n) if D == 2 else \
+            MakeStructured3DMesh(hexes=True, nx=n, ny=n, nz=n)
+        lset = -np.ones(mesh.nv)
+        off, pts = gpu_rule(dev, mesh, lset, NEG, 2, OPTIMAL)
+        nv = 2 ** D
+        for mask in (1, 2):
+            out = torch.empty(nv * nv, dtype=torch.float64, device="cuda")
+            ctx.quad_elmat(D, up(torch, mesh.el2v), up(torch, mesh.coords), up(torch, off), up(torch, pts.reshape(-1)),
+                           mask, _abi.XG_PREC_FP64, out)
+            ctx.sync()
+            m = out.cpu().numpy().reshape(nv, nv)
+            if D == 2:
+                ref = np.array([[4, 2, 1, 2], [2, 4, 2, 1], [1, 2, 4, 2], [2, 1, 2, 4]]) / 36 if mask == 1 else \
+                    np.array([[4, -1, -2, -1], [-1, 4, -1, -2], [-2, -1, 4, -1], [-1, -2, -1, 4]]) / 6
+                assert np.abs(m - ref).max() < 1e-15
+            elif mask == 1:
+                assert abs(m.sum() - 1.0) < 1e-14 and abs(m[0, 0] - 1 / 27) < 1e-15
+            else:
+                assert np.abs(m.sum(axis=1)).max() < 1e-14 and abs(m[0, 0] - 1 / 3) < 1e-14
+
+
+@pytest.mark.parametrize("D,n", [(2, 8), (3, 4)])
+@pytest.mark.parametrize("pol", [FIRST, OPTIMAL, FALLBACK])
+def test_quad_elmat_parity(dev, orc, D, n, pol):
+    torch, _abi, ctx = dev
+    mesh = warped(D, n)
+    lset = level_sets(mesh, "sphere")
+    nv = 2 ** D
+    for dt in (NEG, POS, IF):
+        off, pts = gpu_rule(dev, mesh, lset, dt, 2, pol)
+        for mask in (1, 2, 3):
+            out = torch.empty(mesh.ne * nv * nv, dtype=torch.float64, device="cuda")
+            ctx.quad_elmat(D, up(torch, mesh.el2v), up(torch, mesh.coords), up(torch, off), up(torch, pts.reshape(-1)),
+                           mask, _abi.XG_PREC_FP64, out)
+            ctx.sync()
+            got = out.cpu().numpy().reshape(mesh.ne, nv, nv)
+            for e in range(mesh.ne):
+                q = pts[off[e]:off[e + 1]]
+                ref = orc.quad_elmat(D, mesh.coords[mesh.el2v[e]], q[:, :3], q[:, 3], mask)
+                assert np.abs(got[e] - ref).max() <= 1e-14 * max(1.0, np.abs(ref).max()), (dt, mask, e)
+
+
+@pytest.mark.parametrize("N_", [1, 10])
+def test_dcut_forms_on_quads(X, N_):
+    """a dCut bilinear form on quads: the assembled mass matrix sums to the
+    measure of the part (7/8, 1/8 for 1 - 2x - 2y; test_straightcutrule.py
+    :106-126), the stiffness matrix annihilates constants, and the interface
+    mass sums to the interface length 1/sqrt(2)"""
+    mesh = MakeStructured2DMesh(quads=True, nx=N_, ny=N_)
+    g = p1(X, mesh, 1 - 2 * X.x - 2 * X.y)
+    V = X.H1(mesh, order=1)
+    u, v = V.TnT()
+    for dt, ref in ((X.NEG, 7 / 8), (X.POS, 1 / 8), (X.IF, 1 / math.sqrt(2))):
+        a = X.BilinearForm(V)
+        a += u * v * X.dCut(g, dt, order=2)
+        a.Assemble(precision=X.PREC_FP64)
+        assert abs(a.mat.sum() - ref) < 1e-13
+    a = X.BilinearForm(V)
+    a += X.grad(u) * X.grad(v) * X.dCut(g, X.NEG, order=2)
+    a.Assemble(precision=X.PREC_FP64)
+    assert np.abs(a.mat @ np.ones(mesh.nv)).max() < 1e-13
