@@ -188,6 +188,12 @@ int xg_quad_elmat(xg_ctx *ctx, int D, int nel, const int32_t *elems, const int32
                   const double *coords, const int32_t *offsets, const double *pts, int mask,
                   int precision, void *out);
 
+/* element vectors of f = 1 (SymbolicCutLinearFormIntegrator, xfem/
+   symboliccutlfi.cpp:57-140) for the Q1 space on such a rule; out: nel x 2^D
+   FP64. */
+int xg_quad_elvec(xg_ctx *ctx, int D, int nel, const int32_t *elems, const int32_t *el2v,
+                  const double *coords, const int32_t *offsets, const double *pts, double *out);
+
 /* ---- several level sets on simplices ------------------------------------
    CreateMultiLevelsetCutIntegrationRule (cutint/mlsetintegration.cpp:8-158)
    with StraightCutsIntegrationRule (cutint/straightcutrule.cpp:679-800).

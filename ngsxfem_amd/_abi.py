@@ -53,6 +53,7 @@ _SIGS = {
     "xg_quad_mark": [_vp, _i, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp],
     "xg_quad_integrate": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp],
     "xg_quad_elmat": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
+    "xg_quad_elvec": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp],
     "xg_integrate_nodes": [_vp, _i, _vp, _vp, _vp, _i, _vp, _i, _vp, _vp, _vp, _vp],
     "xg_elmat": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "xg_cut_pipeline": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _i, _vp],
@@ -281,6 +282,11 @@ class Context:
         nel = elems.shape[0] if elems is not None else el2v.shape[0]
         check(lib().xg_quad_elmat(self.h, D, nel, ptr(elems), ptr(el2v), ptr(coords), ptr(offsets), ptr(pts), mask,
                                   precision, ptr(out)))
+
+    def quad_elvec(self, D, el2v, coords, offsets, pts, out, elems=None):
+        nel = elems.shape[0] if elems is not None else el2v.shape[0]
+        check(lib().xg_quad_elvec(self.h, D, nel, ptr(elems), ptr(el2v), ptr(coords), ptr(offsets), ptr(pts),
+                                  ptr(out)))
 
     def elmat(self, D, p, el2v, coords, offsets, pts, mask, precision, out, elems=None):
         nel = elems.shape[0] if elems is not None else el2v.shape[0]

@@ -1100,6 +1100,25 @@ std::vector<int32_t> element_list(py::object elements, int ne) {
 // (elements, nel x ndof element vectors)
 py::tuple CutElementVectors(py::object mesh, py::object lset, int dt, int order, int p, py::object elements) {
   MA ma = as_mesh(mesh);
+  if (ma->quad) {
+    // the Q1 space on the quad / hex cut rule (FIND_OPTIMAL)
+    if (p != 1) throw std::runtime_error("CutElementVectors on quads / hexes: order 1");
+    bool is_p1q = true;
+    std::vector<double> vq = p1_values(lset, ma->nv, is_p1q);
+    if (!is_p1q) throw std::runtime_error("CutElementVectors: P1 level sets only");
+    DevBuf<double> d_lq;
+    d_lq.upload(vq.data(), vq.size());
+    std::vector<int32_t> elq = element_list(elements, ma->ne);
+    DeviceRule r;
+    build_quad_rule(ma, d_lq.p, dt, order, FIND_OPTIMAL, r, &elq);
+    const int nel = (int)elq.size(), nd = ma->nvel;
+    DevBuf<double> out((size_t)std::max(nel, 1) * nd);
+    check(xg_quad_elvec(ctx(), ma->dim, nel, r.elems.p, ma->el2v.p, ma->coords.p, r.offsets.p, r.pts.p, out.p));
+    check(xg_sync(ctx()));
+    py::array_t<double> h({(py::ssize_t)nel, (py::ssize_t)nd});
+    out.download(h.mutable_data(), (size_t)nel * nd);
+    return py::make_tuple(py::array_t<int32_t>((py::ssize_t)nel, elq.data()), h);
+  }
   simplices_only(ma, "CutElementVectors");
   bool is_p1 = true;
   std::vector<double> v = p1_values(lset, ma->nv, is_p1);

@@ -804,6 +804,51 @@ __global__ __launch_bounds__(256) void k_quad_elmat(int nel, const int32_t *__re
     }
 }
 
+// SymbolicCutLinearFormIntegrator (xfem/symboliccutlfi.cpp:57-140) with f = 1
+// for the Q1 space on a materialised quad / hex rule: out_a = sum_q
+// |det F| w_q phi_a(xi_q), one lane per element
+template <int D>
+__global__ __launch_bounds__(256) void k_quad_elvec(int nel, const int32_t *__restrict__ elems,
+                                                    const int32_t *__restrict__ el2v,
+                                                    const double *__restrict__ coords,
+                                                    const int32_t *__restrict__ offsets,
+                                                    const double4 *__restrict__ pts, double *__restrict__ out) {
+  constexpr int NV = 1 << D;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nel) return;
+  const int e = elems ? elems[i] : i;
+  double V[NV][3];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const int v = el2v[(size_t)e * NV + k];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) V[k][d] = d < D ? coords[(size_t)v * D + d] : 0.0;
+  }
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+  for (int q = offsets[i]; q < offsets[i + 1]; ++q) {
+    const double4 p = pts[q];
+    const double xi[3] = {p.x, p.y, p.z};
+    double F[D * D], x[3];
+    elem_map<D>(V, xi, F, x);
+    double Finv[D * D];
+    const double det = det_inv<D>(F, Finv);
+    const double wq = fabs(det) * p.w;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const int b = v & 3;
+      const int a[3] = {(b == 1 || b == 2), b >= 2, v >= 4};
+      double N = 1.0;
+#pragma unroll
+      for (int m = 0; m < D; ++m) N *= a[m] ? xi[m] : 1.0 - xi[m];
+      acc[v] += wq * N;
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v) out[(size_t)i * NV + v] = acc[v];
+}
+
 bool quad_args_ok(int D, int dt, int order, int pol) {
   return (D == 2 || D == 3) && dt >= 0 && dt <= 2 && order >= 0 && order / 2 + 1 <= XG_RULE_NMAX && pol >= 0 &&
          pol <= 2;
@@ -913,6 +958,19 @@ extern "C" int xg_quad_elmat(xg_ctx *ctx, int D, int nel, const int32_t *elems, 
     if (D == 2) k_quad_elmat<2, float><<<grid, 256, 0, ctx->stream>>>(nel, elems, el2v, coords, offsets, p, mask, (float *)out);
     else k_quad_elmat<3, float><<<grid, 256, 0, ctx->stream>>>(nel, elems, el2v, coords, offsets, p, mask, (float *)out);
   }
+  XG_LAUNCH_CHECK();
+  return XG_OK;
+}
+
+extern "C" int xg_quad_elvec(xg_ctx *ctx, int D, int nel, const int32_t *elems, const int32_t *el2v,
+                             const double *coords, const int32_t *offsets, const double *pts, double *out) {
+  if (!ctx || (D != 2 && D != 3) || nel < 0) return set_error(XG_ERR_ARG, "xg_quad_elvec: bad argument");
+  if (nel == 0) return XG_OK;
+  if (!el2v || !coords || !offsets || !pts || !out) return set_error(XG_ERR_ARG, "xg_quad_elvec: null pointer");
+  const int grid = (nel + 255) / 256;
+  const double4 *p = reinterpret_cast<const double4 *>(pts);
+  if (D == 2) k_quad_elvec<2><<<grid, 256, 0, ctx->stream>>>(nel, elems, el2v, coords, offsets, p, out);
+  else k_quad_elvec<3><<<grid, 256, 0, ctx->stream>>>(nel, elems, el2v, coords, offsets, p, out);
   XG_LAUNCH_CHECK();
   return XG_OK;
 }

@@ -337,3 +337,27 @@ def test_dcut_forms_on_quads(X, N_):
     a += X.grad(u) * X.grad(v) * X.dCut(g, X.NEG, order=2)
     a.Assemble(precision=X.PREC_FP64)
     assert np.abs(a.mat @ np.ones(mesh.nv)).max() < 1e-13
+
+
+@pytest.mark.parametrize("D,n", [(2, 8), (3, 4)])
+def test_linear_form_on_quads(X, orc, D, n):
+    """f * v * dCut on quads / hexes: the element vector of f = 1 equals the
+    row sums of the oracle's mass matrix on the same rule (the Q1 basis sums
+    to one), and the assembled vector sums to the measure of the part"""
+    mesh = warped(D, n)
+    lset = level_sets(mesh, "sphere")
+    g = X.GridFunction(X.H1(mesh, order=1))
+    g.vec = lset.copy()
+    V = X.H1(mesh, order=1)
+    u, v = V.TnT()
+    for dt in (X.NEG, X.IF):
+        f = X.LinearForm(V)
+        f += 1.0 * v * X.dCut(g, dt, order=2)
+        f.Assemble()
+        vol = X.Integrate({"levelset": g, "domain_type": dt}, cf=X.CoefficientFunction(1), mesh=mesh, order=2)
+        assert abs(f.vec.sum() - vol) < 1e-13 * max(1.0, vol)
+        e_ids, vecs = f.integrals[0].element_vectors(mesh, 1)
+        for k, e in enumerate(e_ids[:40]):
+            r = orc.quad_cut_rule(D, lset[mesh.el2v[e]], int(dt), 2, 1, V=mesh.coords[mesh.el2v[e]])
+            ref = np.zeros(2 ** D) if r is None else orc.quad_elmat(D, mesh.coords[mesh.el2v[e]], r[0], r[1], 1).sum(1)
+            assert np.abs(vecs[k] - ref).max() <= 1e-14 * max(1.0, np.abs(ref).max())
