@@ -56,19 +56,26 @@ __global__ __launch_bounds__(256) void k_moment_pipeline(int nel, const int32_t 
   __shared__ double geo[NG][256];
   const int tid = threadIdx.x;
   if (tid < 4) cnt[tid] = 0;
-  const MomLds S = stage_lds<D, P>(smem, order);  // block barrier: before any exit
   const int ntiles = (nel + 255) >> 8;
   const int t = blockIdx.x / ntiles;  // domain-type slot (block-uniform)
   const int base = (blockIdx.x - t * ntiles) * 256;
   const int dt = (dts >> (2 * t)) & 3;
   const int nitems = min(256, nel - base);
-  // phase 0: each lane gathers its own element into LDS and computes its
-  // sort key, the number of sub-simplices its rule sums
+  // phase 0: each lane gathers its own element (issued before the table
+  // staging, whose loads and barrier then overlap the gather's dependent
+  // round trips), stores it into LDS and computes its sort key, the number
+  // of sub-simplices its rule sums
+  // (unconditional: lanes past the end load the last element again, so no
+  // branch end waits for the loads before the staging)
+  double V[D + 1][3], l[D + 1];
+  {
+    const int ic = min(base + tid, nel - 1);
+    const int e = elems ? elems[ic] : ic;
+    ecore::load_geom<D>(el2v, coords, lset, e, V, l);
+  }
+  const MomLds S = stage_lds<D, P>(smem, order);  // block barrier: before any exit
   int key = 0, pos = 0;
   if (tid < nitems) {
-    const int e = elems ? elems[base + tid] : base + tid;
-    double V[D + 1][3], l[D + 1];
-    ecore::load_geom<D>(el2v, coords, lset, e, V, l);
 #pragma unroll
     for (int k = 0; k <= D; ++k) {
 #pragma unroll
@@ -90,7 +97,6 @@ __global__ __launch_bounds__(256) void k_moment_pipeline(int nel, const int32_t 
   // phase 1: the lanes take the elements in sorted order, from LDS
   const int li = perm[tid];
   const int i = base + li;
-  double V[D + 1][3], l[D + 1];
 #pragma unroll
   for (int k = 0; k <= D; ++k) {
 #pragma unroll
