@@ -66,6 +66,15 @@ def parse():
     return ap.parse_args()
 
 
+def read_profile_value(name, key):
+    """a "key: value" figure from a committed profile summary, or None"""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    m = re.search(re.escape(key) + r"\s*[:=]\s*([0-9.eE+-]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
 def read_traffic(name):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
     scaled by the copy-kernel calibration) from the committed summary."""
@@ -613,7 +622,8 @@ def main():
                 "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": ach / PEAK_HBM,
                 "traffic": read_traffic("moments_c2_pmc.txt"), "algorithmic_bytes_per_launch": bytes_alg,
                 "avg_launch_ms": launch_mom, "isolated_launch_ms": iso_mom,
-                "fp64_valu_frac": fp64_ops_moments(npts, n_cut) / (launch_mom * 1e-3) / PEAK_FP64_VALU}
+                "fp64_valu_frac": fp64_ops_moments(npts, n_cut) / (launch_mom * 1e-3) / PEAK_FP64_VALU,
+                "fp64_valu_frac_measured": read_profile_value("fp64_valu_pmc.txt", "moment_pipeline_fp64_frac_measured")}
     roof = roof_mom if best == "moments" else roof_mfma
 
     # ---- CPU baseline: FP64 restatement of the reference algorithm ----
