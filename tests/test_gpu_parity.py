@@ -99,6 +99,17 @@ def test_mark_fuzz_special_values(orc, dev, D):
     assert_mark_parity(orc, dev, D, el2v, l)
 
 
+@pytest.mark.parametrize("ne", [1, 5, 63, 64, 65, 255, 1023, 1024, 1025, 4097])
+def test_mark_small_and_ragged_sizes(orc, dev, ne):
+    """element counts around the wave (64), block (1024 elements: 256
+    threads x 4) and bit-array word boundaries, with a cut in every block"""
+    rng = np.random.default_rng(ne)
+    D = 3
+    el2v = rng.integers(0, 50, size=(ne, D + 1)).astype(np.int32)
+    lset = rng.uniform(-1, 1, 50)
+    assert_mark_parity(orc, dev, D, el2v, lset)
+
+
 def test_mark_degenerate_raises(orc, dev):
     torch, _abi, ctx = dev
     el2v = np.array([[0, 1, 2, 3]], np.int32)
