@@ -98,6 +98,17 @@ int xg_mark(xg_ctx *ctx, int D, int ne, const int32_t *el2v, const double *lset_
 int xg_mark_p2(xg_ctx *ctx, int ne, const double *lset_nodal, int L, uint8_t *dt, uint32_t *bits,
                int32_t *cut_list, int32_t *n_cut);
 
+/* a12 on the boundary triangles of a tetrahedral mesh with a P2 level set
+   (cutinfo.cpp:80-124, BND elements, with the subdivision rule on triangles,
+   xintegration.cpp:620-838,1019-1061 and CutSimplex<2>::MakeQuad
+   :1391-1545): the order-0 NEG and POS areas of the level-L rule decide the
+   domain (IF iff both > 0, else the nonzero side) and the cut ratio
+   NEG / (NEG + POS).  lset_nodal: ne x 6 FP64 nodal values (vertices, then
+   the edges (0,1), (0,2), (1,2)); dt, cut_ratio, bits required; cut_list /
+   n_cut as in xg_mark.  L <= 5. */
+int xg_mark_p2_trig(xg_ctx *ctx, int ne, const double *lset_nodal, int L, uint8_t *dt,
+                    double *cut_ratio, uint32_t *bits, int32_t *cut_list, int32_t *n_cut);
+
 /* ---- a13: GetFacetsWithNeighborTypes (xfem/cutinfo.cpp:476-538) ------- */
 /* facet f has neighbours f2el[f][0], f2el[f][1] (-1 = none).  a, b: element
    bit arrays; result bit f = use_and ? (a(L)&b(R) | a(R)&b(L))

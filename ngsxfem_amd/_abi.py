@@ -37,6 +37,7 @@ _SIGS = {
     "xg_p1_perturb": [_vp, _i, _vp, ctypes.c_double],
     "xg_mark": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "xg_mark_p2": [_vp, _i, _vp, _i, _vp, _vp, _vp, _vp],
+    "xg_mark_p2_trig": [_vp, _i, _vp, _i, _vp, _vp, _vp, _vp, _vp],
     "xg_facets_with_neighbor_types": [_vp, _i, _vp, _vp, _vp, _i, _i, _i, _vp],
     "xg_mark_from_items": [_vp, _i, _i, _vp, _vp, _i, _vp],
     "xg_mlset_rule_count": [_vp, _i, _i, _vp, _vp, _i, _vp, _i, _vp, _i, _vp],
@@ -180,6 +181,11 @@ class Context:
     def mark_p2(self, lset_nodal, L, dt, bits, cut_list=None, n_cut=None):
         ne = lset_nodal.shape[0]
         check(lib().xg_mark_p2(self.h, ne, ptr(lset_nodal), L, ptr(dt), ptr(bits), ptr(cut_list), ptr(n_cut)))
+
+    def mark_p2_trig(self, lset_nodal, L, dt, cut_ratio, bits, cut_list=None, n_cut=None):
+        ne = lset_nodal.shape[0]
+        check(lib().xg_mark_p2_trig(self.h, ne, ptr(lset_nodal), L, ptr(dt), ptr(cut_ratio), ptr(bits),
+                                    ptr(cut_list), ptr(n_cut)))
 
     def facets_with_neighbor_types(self, f2el, a, b, bnd_val_a, bnd_val_b, use_and, out):
         check(lib().xg_facets_with_neighbor_types(self.h, f2el.shape[0], ptr(f2el), ptr(a), ptr(b),

@@ -128,7 +128,7 @@ struct MeshAccess {
   DevBuf<double> coords;
   DevBuf<int32_t> el2v, bnd2v, f2el, el2f, el2e;
   int nf = -1, nedges = -1, nf_el = 0, ne_el = 0;
-  std::vector<int32_t> h_el2v;
+  std::vector<int32_t> h_el2v, h_bnd2v;
 
   explicit MeshAccess(py::object m) : pymesh(m) {
     auto c = py::array_t<double, py::array::c_style | py::array::forcecast>(m.attr("coords"));
@@ -146,6 +146,7 @@ struct MeshAccess {
     nse = (int)b.shape(0);
     nvbnd = b.ndim() == 2 ? (int)b.shape(1) : dim;
     if (nse) bnd2v.upload(b.data(), (size_t)nse * nvbnd);
+    h_bnd2v.assign(b.data(), b.data() + (size_t)nse * nvbnd);
   }
   int GetNE(VorB vb) const { return vb == VOL ? ne : nse; }
   void ensure_facets() {
@@ -358,7 +359,6 @@ class CutInformation {
       UpdateSubdivision(lset, subdivlvl);
       return;
     }
-    bnd_marked = true;
     bool is_p1 = true;
     std::vector<double> v = p1_values(lset, ma->nv, is_p1);
     d_lset.upload(v.data(), v.size());
@@ -424,7 +424,7 @@ class CutInformation {
   // rule): element domains from the lattice check of the level-L rule
   // (xg_mark_p2), cut ratios part_vol[NEG] / (part_vol[NEG] + part_vol[POS])
   // from the rule's weights on the cut elements (xg_subdiv_weights; the POS
-  // part as the NEG part of -phi).  Boundary elements are not marked.
+  // part as the NEG part of -phi); boundary triangles by xg_mark_p2_trig.
   void UpdateSubdivision(py::object lset, int L) {
     simplices_only(ma, "CutInfo with a P2 level set / subdivlvl > 0");
     if (L > 5) throw std::runtime_error("CutInfo.Update: subdivlvl <= 5 in this build");
@@ -471,6 +471,12 @@ class CutInformation {
       std::vector<double> hn((size_t)ncut * 35), hp((size_t)ncut * 35);
       wn.download(hn.data(), hn.size());
       wp.download(hp.data(), hp.size());
+      // The element domain is decided on the part volumes, not on the
+      // lattice (cutinfo.cpp:104-124): a lattice-cut element whose leaves
+      // all fall to one side (exact zeros, or values below the leaves'
+      // relative 1e-14 threshold, xdecompose.hpp:34-85) has an empty rule on
+      // the other side and is marked uncut.
+      bool moved = false;
       for (int i = 0; i < ncut; ++i) {
         double a = 0.0, b = 0.0;
         for (int k = 0; k < 35; ++k) {
@@ -478,20 +484,72 @@ class CutInformation {
           b += hp[(size_t)i * 35 + k];
         }
         cr[cut[i]] = a / (a + b);
+        const uint8_t d = a > 0.0 ? (b > 0.0 ? (uint8_t)IF : (uint8_t)NEG) : (uint8_t)POS;
+        if (d != IF) {
+          const int e = cut[i];
+          dt[e] = d;
+          const uint32_t m = 1u << (e & 31);
+          for (int c = 0; c < 8; ++c) {
+            const bool on = c == 7 || (c & (d == NEG ? 1 : 2));
+            uint32_t &w = elems[c]->words()[e >> 5];
+            w = on ? (w | m) : (w & ~m);
+          }
+          moved = true;
+        }
       }
+      if (moved) d_dt.upload(dt.data(), ne);
     }
     DevBuf<double> d_cr(std::max(ne, 1));
     d_cr.upload(cr.data(), ne);
     if (ne) UpdateNodes(d_dt.p, d_cr.p);
-    for (auto &b : selems) b->Clear();
-    cut_ratio[BND].assign(ma->nse, 0.0);
-    bnd_marked = false;
+    UpdateSubdivisionBnd(lset, L);
   }
-  bool bnd_marked = true;
+
+  // Boundary triangles (cutinfo.cpp:80-124, vb = BND): the subdivision rule
+  // on each triangle with the level set's restriction (6 nodal values).
+  void UpdateSubdivisionBnd(py::object lset, int L) {
+    const int nse = ma->nse;
+    cut_ratio[BND].assign(nse, 0.0);
+    if (nse == 0) {
+      for (auto &b : selems) b->Clear();
+      return;
+    }
+    const int order = py::hasattr(lset, "order") ? lset.attr("order").cast<int>() : 1;
+    py::object vec = py::hasattr(lset, "vec") ? lset.attr("vec") : lset;
+    auto a = py::array_t<double, py::array::c_style | py::array::forcecast>(vec);
+    const double *v = a.data();
+    std::vector<double> nod((size_t)nse * 6);
+    if (order == 2) {
+      py::tuple t = ma->pymesh.attr("bnd_dofs")(2);
+      auto dofs = py::array_t<int32_t, py::array::c_style | py::array::forcecast>(t[0]);
+      const int32_t *d = dofs.data();
+      for (size_t i = 0; i < (size_t)nse * 6; ++i) nod[i] = v[d[i]];
+    } else {
+      static const int ed[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+      for (int i = 0; i < nse; ++i) {
+        const int32_t *vv = &ma->h_bnd2v[(size_t)i * 3];
+        for (int k = 0; k < 3; ++k) nod[(size_t)i * 6 + k] = v[vv[k]];
+        for (int k = 0; k < 3; ++k)
+          nod[(size_t)i * 6 + 3 + k] = 0.5 * (v[vv[ed[k][0]]] + v[vv[ed[k][1]]]);
+      }
+    }
+    const int nw = (nse + 31) / 32;
+    DevBuf<double> d_nod, d_cr(nse);
+    d_nod.upload(nod.data(), nod.size());
+    DevBuf<uint8_t> d_dt(nse);
+    DevBuf<uint32_t> bits((size_t)8 * nw);
+    check(xg_mark_p2_trig(ctx(), nse, d_nod.p, L, d_dt.p, d_cr.p, bits.p, nullptr, nullptr));
+    check(xg_sync(ctx()));
+    std::vector<uint32_t> hb((size_t)8 * nw);
+    bits.download(hb.data(), hb.size());
+    for (int c = 0; c < 8; ++c) {
+      auto &w = selems[c]->words();
+      std::copy(hb.begin() + (size_t)c * nw, hb.begin() + (size_t)(c + 1) * nw, w.begin());
+    }
+    d_cr.download(cut_ratio[BND].data(), nse);
+  }
 
   BA GetElementsOfDomainType(int cdt, VorB vb) const {
-    if (vb == BND && !bnd_marked)
-      throw std::runtime_error("CutInfo: boundary elements are not marked for P2 level sets / subdivlvl > 0");
     return vb == VOL ? elems.at(cdt) : selems.at(cdt);
   }
   BA GetFacetsOfDomainType(int cdt) const { return facets.at(cdt); }

@@ -395,6 +395,144 @@ __global__ __launch_bounds__(BLK) void k_mark_p2_lattice(int ne, const double *_
   }
 }
 
+// ---- boundary triangles of a tetrahedral mesh with a P2 level set
+//
+// CutInformation::Update sums the order-0 NEG and POS rules of each boundary
+// element (cutinfo.cpp:80-124); with a P2 level set these are the
+// subdivision rule on the triangle (xintegration.cpp:620-838 lattice check
+// and four children, :1019-1061 uncut nodes, :1391-1545 cut leaves after
+// Simplex::CheckIfCut, xdecompose.hpp:34-85).  For order 0 only areas count,
+// and a node the lattice finds uncut has leaves whose vertices all carry its
+// sign, so the adaptive tree gives the areas of the uniform level-L
+// refinement: one wave per triangle, lanes over the n x n cells of the
+// lattice (n = 2^L), each cell holding an upward (a,b),(a+1,b),(a,b+1) and a
+// downward (a+1,b),(a,b+1),(a+1,b+1) leaf.  Coordinates are exact dyadics,
+// so the values equal those of the nested evaluation.  Domain from the part
+// areas (IF iff both > 0), cut ratio NEG / (NEG + POS).
+XG_DEV double tri_area(const double *a, const double *b, const double *c) {
+  const double u0 = b[0] - a[0], u1 = b[1] - a[1], w0 = c[0] - a[0], w1 = c[1] - a[1];
+  return 0.5 * fabs(u0 * w1 - u1 * w0);
+}
+
+// areas of one leaf (vertices V, level-set values v) to NEG (an) and POS (ap)
+XG_DEV void trig_leaf(const double (*V)[2], const double *v, double &an, double &ap) {
+  bool hp = false, hn = false;
+  double sp = 0.0, sn = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if (v[i] >= 0.0) {
+      sp += v[i];
+      hp = true;
+    } else {
+      sn -= v[i];
+      hn = true;
+    }
+  }
+  // a leaf whose lattice check (its vertices) finds one sign goes whole to it
+  if (!(hp && hn)) {
+    (hp ? ap : an) += tri_area(V[0], V[1], V[2]);
+    return;
+  }
+  const double s = sp + sn;
+  if (sp / s < 1e-14) hp = false;
+  if (sn / s < 1e-14) hn = false;
+  if (!(hp && hn)) {
+    (hp ? ap : an) += tri_area(V[0], V[1], V[2]);
+    return;
+  }
+  // CutSimplex<2>::MakeQuad: zero counts as POS; an edge is cut where the
+  // signs differ or a zero meets a negative value
+  constexpr int EA[3] = {0, 0, 1}, EB[3] = {1, 2, 2};
+  int neg[3], pos[3], nneg = 0, npos = 0, v2c[3] = {-1, -1, -1}, nc = 0;
+  double cp[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    if (v[j] < 0.0) neg[nneg++] = j;
+    else pos[npos++] = j;
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const int l = EA[j], r = EB[j];
+    const bool cut = v[l] * v[r] < 0.0 || (v[l] == 0.0 && v[r] < 0.0) || (v[r] == 0.0 && v[l] < 0.0);
+    if (cut && nc < 2) {
+      const double t = v[l] / (v[l] - v[r]);
+      cp[nc][0] = (1 - t) * V[l][0] + t * V[r][0];
+      cp[nc][1] = (1 - t) * V[l][1] + t * V[r][1];
+      if (v2c[l] == -1) v2c[l] = nc;
+      if (v2c[r] == -1) v2c[r] = nc;
+      ++nc;
+    }
+  }
+  const bool minor_neg = nneg <= npos;
+  const int mi = minor_neg ? neg[0] : pos[0];
+  const int m0 = minor_neg ? pos[0] : neg[0], m1 = minor_neg ? pos[1] : neg[1];
+  const double amin = tri_area(V[mi], cp[0], cp[1]);
+  const double *q2 = cp[v2c[m0] < 0 ? 0 : v2c[m0]], *q3 = cp[v2c[m1] < 0 ? 0 : v2c[m1]];
+  const double amaj = tri_area(V[m0], V[m1], q2) + tri_area(V[m1], q2, q3);
+  (minor_neg ? an : ap) += amin;
+  (minor_neg ? ap : an) += amaj;
+}
+
+__global__ __launch_bounds__(BLK) void k_mark_p2_trig(int ne, const double *__restrict__ lset_nodal, int L,
+                                                      uint8_t *__restrict__ dt_out,
+                                                      double *__restrict__ cut_ratio) {
+  const int lane = threadIdx.x & 63;
+  const int e = __builtin_amdgcn_readfirstlane((blockIdx.x * BLK + threadIdx.x) >> 6);
+  if (e >= ne) return;
+  double c[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) c[k] = lset_nodal[(size_t)e * 6 + k];
+  const int n = 1 << L;
+  const double inv_n = 1.0 / n;
+  // lattice check of the whole triangle (xintegration.cpp:620-755)
+  bool hp = false, hn = false;
+  for (int q = lane; q < (n + 1) * (n + 1); q += 64) {
+    const int a = q % (n + 1), b = q / (n + 1);
+    if (a + b > n) continue;
+    const double x[2] = {a * inv_n, b * inv_n};
+    const double v = lagrange_eval2<2>(c, x);
+    if (v >= 0.0) hp = true;
+    else hn = true;
+  }
+  const bool anyp = __ballot(hp) != 0ull, anyn = __ballot(hn) != 0ull;
+  if (!(anyp && anyn)) {
+    if (lane == 0) {
+      dt_out[e] = anyp ? POS : NEG;
+      cut_ratio[e] = anyp ? 0.0 : 1.0;
+    }
+    return;
+  }
+  double an = 0.0, ap = 0.0;
+  for (int q = lane; q < n * n; q += 64) {
+    const int a = q % n, b = q / n;
+    if (a + b > n - 1) continue;
+    const double x00[2] = {a * inv_n, b * inv_n}, x10[2] = {(a + 1) * inv_n, b * inv_n},
+                 x01[2] = {a * inv_n, (b + 1) * inv_n};
+    const double v00 = lagrange_eval2<2>(c, x00), v10 = lagrange_eval2<2>(c, x10),
+                 v01 = lagrange_eval2<2>(c, x01);
+    {
+      const double V[3][2] = {{x00[0], x00[1]}, {x10[0], x10[1]}, {x01[0], x01[1]}};
+      const double v[3] = {v00, v10, v01};
+      trig_leaf(V, v, an, ap);
+    }
+    if (a + b <= n - 2) {
+      const double x11[2] = {(a + 1) * inv_n, (b + 1) * inv_n};
+      const double V[3][2] = {{x10[0], x10[1]}, {x01[0], x01[1]}, {x11[0], x11[1]}};
+      const double v[3] = {v10, v01, lagrange_eval2<2>(c, x11)};
+      trig_leaf(V, v, an, ap);
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    an += __shfl_xor(an, o, 64);
+    ap += __shfl_xor(ap, o, 64);
+  }
+  if (lane == 0) {
+    dt_out[e] = an > 0.0 ? (ap > 0.0 ? IF : NEG) : POS;
+    cut_ratio[e] = an / (an + ap);
+  }
+}
+
 // bit arrays and block IF counts from the domain types (as k_mark)
 __global__ __launch_bounds__(BLK) void k_bits_from_dt(int ne, const uint8_t *__restrict__ dt,
                                                       uint32_t *__restrict__ bits, int nw,
@@ -749,6 +887,21 @@ extern "C" int xg_facets_with_neighbor_types(xg_ctx *ctx, int nf, const int32_t 
                                                       out, nwf);
   XG_LAUNCH_CHECK();
   return XG_OK;
+}
+
+extern "C" int xg_mark_p2_trig(xg_ctx *ctx, int ne, const double *lset_nodal, int L, uint8_t *dt,
+                               double *cut_ratio, uint32_t *bits, int32_t *cut_list, int32_t *n_cut) {
+  if (!ctx || ne < 0 || L < 0 || L > XG_LAT_LMAX || !dt || !cut_ratio || !bits || (ne > 0 && !lset_nodal))
+    return set_error(XG_ERR_ARG, "xg_mark_p2_trig: bad argument (dt, cut_ratio and bits required, L <= 5)");
+  if (ne == 0) {
+    if (n_cut) XG_CHECK(hipMemsetAsync(n_cut, 0, sizeof(int32_t), ctx->stream));
+    return XG_OK;
+  }
+  const long long waves = ne;
+  k_mark_p2_trig<<<(unsigned)((waves * 64 + BLK - 1) / BLK), BLK, 0, ctx->stream>>>(ne, lset_nodal, L, dt,
+                                                                                  cut_ratio);
+  XG_LAUNCH_CHECK();
+  return xg::bits_and_cut_list_from_dt(ctx, ne, dt, bits, cut_list, n_cut);
 }
 
 extern "C" int xg_mark_p2(xg_ctx *ctx, int ne, const double *lset_nodal, int L, uint8_t *dt, uint32_t *bits,

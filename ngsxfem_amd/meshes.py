@@ -79,6 +79,28 @@ class Mesh:
         e2v, el2e = self.edges()
         return np.ascontiguousarray(np.hstack([self.el2v, self.nv + el2e]).astype(np.int32)), self.nv + len(e2v)
 
+    def bnd_dofs(self, p):
+        """global dof numbers of the degree-p Lagrange space on the boundary
+        elements, in their basis' dof order (vertices, then the edges
+        (0,1),(0,2),(1,2) of the boundary simplex): (nse x ndof table, ndof)"""
+        if p == 1:
+            return self.bnd2v, self.nv
+        if p != 2 or self.is_quad:
+            raise NotImplementedError("boundary dof numbering for p = 1, 2 on simplices")
+        e2v, _ = self.edges()
+        key = e2v[:, 0].astype(np.int64) * self.nv + e2v[:, 1]
+        nb = self.bnd2v.shape[1]
+        pairs = [(i, j) for i in range(nb) for j in range(i + 1, nb)]
+        cols = []
+        for i, j in pairs:
+            ev = np.sort(self.bnd2v[:, [i, j]], axis=1).astype(np.int64)
+            k = ev[:, 0] * self.nv + ev[:, 1]
+            pos = np.searchsorted(key, k)
+            assert np.all(key[np.minimum(pos, len(key) - 1)] == k), "boundary edge not in the mesh"
+            cols.append(self.nv + pos)
+        tab = np.hstack([self.bnd2v, np.stack(cols, axis=1)]).astype(np.int32)
+        return np.ascontiguousarray(tab), self.nv + len(e2v)
+
     def _local_edges(self):
         D = self.dim
         if self.is_quad:

@@ -62,6 +62,7 @@ def _load():
     lib.or_cut_pipeline.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip,
                                     _ip, _dp, _dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                     ctypes.c_int, _dp, ctypes.c_int]
+    lib.or_subdiv_trig_parts.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _dp]
     lib.or_subdiv_rule3.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp,
                                     ctypes.POINTER(_dp), ctypes.POINTER(_dp), ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_int)]
@@ -351,6 +352,17 @@ def subdiv_rule3(lset_nodal, lset_order, L, order, Finv=None, cap=200000):
     if dom < 0:
         raise RuntimeError("oracle subdiv error %d" % dom)
     return dom, [(P[r][:n[r]].copy(), W[r][:n[r]].copy()) for r in range(3)]
+
+
+def subdiv_trig_parts(lset_nodal, lset_order, L):
+    """(lattice domain, NEG area, POS area) of the order-0 subdivision rule on
+    the reference triangle (or_subdiv_trig_parts)"""
+    c = _f64(lset_nodal)
+    part = np.zeros(2)
+    dom = lib().or_subdiv_trig_parts(_d(c), lset_order, L, _d(part))
+    if dom < 0:
+        raise RuntimeError("oracle subdiv error %d" % dom)
+    return dom, part[0], part[1]
 
 
 def subdiv_moments3(lset_nodal, lset_order, L, order, q, Finv=None):

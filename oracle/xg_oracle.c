@@ -2152,3 +2152,132 @@ double or_mlp_loss_grad(int I, int H, int O, const double *params, int batch,
   free(h1); free(h2); free(y); free(dy); free(d2); free(d1);
   return loss / batch;
 }
+
+/* ---- order-0 subdivision rule on a triangle (boundary elements of a
+   tetrahedral mesh with a P2 level set): the part areas that
+   CutInformation::Update sums (xfem/cutinfo.cpp:80-124) from
+   NumericalIntegrationStrategy<ET_TRIG>::MakeQuadRule
+   (cutint/xintegration.cpp:620-755 lattice check, :797-838 four children,
+   :1019-1061 uncut nodes) and CutSimplex<2>::MakeQuad (:1391-1545) on the
+   cut leaves after Simplex::CheckIfCut (cutint/xdecompose.hpp:34-85).
+   Reference triangle (1,0), (0,1), (0,0); nodal values in the dof order
+   vertices, edges (0,1), (0,2), (1,2). */
+typedef struct { const double *c; int lo; double part[2]; int err; } Sub2;
+
+static double lset_eval2(const Sub2 *S, const double *x)
+{
+  double phi[6];
+  or_h1_shape(2, S->lo, x, phi, NULL);
+  int nd = or_h1_ndof(2, S->lo);
+  double v = 0.0;
+  for (int k = 0; k < nd; ++k) v += phi[k] * S->c[k];
+  return v;
+}
+
+static double area2(const double *a, const double *b, const double *c)
+{
+  double u0 = b[0] - a[0], u1 = b[1] - a[1], w0 = c[0] - a[0], w1 = c[1] - a[1];
+  return 0.5 * fabs(u0 * w1 - u1 * w0);
+}
+
+static int lattice_check2(const Sub2 *S, const double V[3][2], int lvl)
+{
+  int haspos = 0, hasneg = 0, np1 = 1 << lvl;
+  double dx = 1.0 / np1;
+  for (int i1 = 0; i1 <= np1; ++i1)
+    for (int i0 = 0; i0 + i1 <= np1; ++i0) {
+      int I[2] = {i0, i1};
+      double pos[2] = {V[0][0], V[0][1]};
+      for (int j = 0; j < 2; ++j)
+        for (int d = 0; d < 2; ++d) pos[d] += I[j] * dx * (V[j + 1][d] - V[0][d]);
+      double v = lset_eval2(S, pos);
+      if (v >= 0.0) haspos = 1; else hasneg = 1;
+      if (haspos && hasneg) return OR_IF;
+    }
+  return haspos ? OR_POS : OR_NEG;
+}
+
+static void cut_leaf2(Sub2 *S, const double V[3][2])
+{
+  static const int edge[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+  double vv[3], cp[2][2];
+  int zero[3], pos[3], npos = 0, neg[3], nneg = 0, v2c[3] = {-1, -1, -1}, nc = 0;
+  for (int j = 0; j < 3; ++j) {
+    vv[j] = lset_eval2(S, V[j]);
+    zero[j] = vv[j] == 0.0;
+    if (vv[j] < 0.0) neg[nneg++] = j; else pos[npos++] = j;
+  }
+  for (int j = 0; j < 3; ++j) {
+    int l = edge[j][0], r = edge[j][1];
+    int cut = vv[l] * vv[r] < 0.0 || (zero[l] && vv[r] < 0.0) || (zero[r] && vv[l] < 0.0);
+    if (!cut) continue;
+    if (nc >= 2) { S->err = OR_ERR_SUBDIV; return; }
+    double t = vv[l] / (vv[l] - vv[r]);
+    for (int d = 0; d < 2; ++d) cp[nc][d] = (1 - t) * V[l][d] + t * V[r][d];
+    if (v2c[l] == -1) v2c[l] = nc;
+    if (v2c[r] == -1) v2c[r] = nc;
+    nc++;
+  }
+  if (nc != 2) { S->err = OR_ERR_SUBDIV; return; }
+  int minor_neg = nneg <= npos;           /* the one-vertex side */
+  const int *mi = minor_neg ? neg : pos, *ma = minor_neg ? pos : neg;
+  double a_min = area2(V[mi[0]], cp[0], cp[1]);
+  /* DecomposePrismIntoSimplices<2> on (major0, major1, cut(major0), cut(major1)) */
+  const double *q[4] = {V[ma[0]], V[ma[1]], cp[v2c[ma[0]]], cp[v2c[ma[1]]]};
+  double a_maj = area2(q[0], q[1], q[2]) + area2(q[1], q[2], q[3]);
+  S->part[minor_neg ? 0 : 1] += a_min;
+  S->part[minor_neg ? 1 : 0] += a_maj;
+}
+
+static void make_quad_rule2(Sub2 *S, const double V[3][2], int lvl)
+{
+  static const double baryc[6][3] = {{0.0, 0.0, 1.0}, {0.5, 0.0, 0.5}, {1.0, 0.0, 0.0},
+                                     {0.0, 0.5, 0.5}, {0.5, 0.5, 0.0}, {0.0, 1.0, 0.0}};
+  static const int trigs[4][3] = {{0, 1, 3}, {1, 2, 4}, {1, 3, 4}, {3, 4, 5}};
+  int dt = lattice_check2(S, V, lvl);
+  if (dt == OR_IF) {
+    if (lvl > 0) {
+      for (int i = 0; i < 4 && !S->err; ++i) {
+        double nv[3][2];
+        for (int j = 0; j < 3; ++j)
+          for (int c = 0; c < 2; ++c) {
+            nv[j][c] = 0.0;
+            for (int d = 0; d < 3; ++d) nv[j][c] += baryc[trigs[i][j]][d] * V[d][c];
+          }
+        make_quad_rule2(S, nv, lvl - 1);
+      }
+      return;
+    }
+    /* leaf: Simplex::CheckIfCut with its relative 1e-14 threshold */
+    int haspos = 0, hasneg = 0;
+    double sp = 0.0, sn = 0.0;
+    for (int i = 0; i < 3; ++i) {
+      double v = lset_eval2(S, V[i]);
+      if (v >= 0.0) { sp += v; haspos = 1; } else { sn -= v; hasneg = 1; }
+    }
+    double s = sp + sn;
+    sp /= s;
+    sn /= s;
+    if (sp < 1e-14) haspos = 0;
+    if (sn < 1e-14) hasneg = 0;
+    if (haspos && hasneg) { cut_leaf2(S, V); return; }
+    if (!haspos && !hasneg) { S->err = OR_ERR_SUBDIV; return; }
+    S->part[haspos ? 1 : 0] += area2(V[0], V[1], V[2]);
+    return;
+  }
+  S->part[dt == OR_NEG ? 0 : 1] += area2(V[0], V[1], V[2]);
+}
+
+int or_subdiv_trig_parts(const double *lset_nodal, int lset_order, int L, double part[2])
+{
+  Sub2 S;
+  memset(&S, 0, sizeof(S));
+  S.c = lset_nodal;
+  S.lo = lset_order;
+  double V[3][2] = {{1.0, 0.0}, {0.0, 1.0}, {0.0, 0.0}};
+  int dom = lattice_check2(&S, V, L);
+  make_quad_rule2(&S, V, L);
+  part[0] = S.part[0];
+  part[1] = S.part[1];
+  return S.err ? S.err : dom;
+}

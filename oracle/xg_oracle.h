@@ -151,6 +151,11 @@ int or_subdiv_rule3(const double *lset_nodal, int lset_order, int L, int order,
    reference tet, for the NEG part (out_neg) and the interface (out_if,
    interface weights as above).  Avoids materialising large rules.
    returns the element domain or error. */
+/* Order-0 subdivision rule on the reference triangle (boundary elements of
+   a tetrahedral mesh with a P2 level set, xfem/cutinfo.cpp:80-124 with
+   cutint/xintegration.cpp:620-838,1019-1061,1391-1545): part[0] / part[1]
+   = NEG / POS areas; returns the lattice domain (0/1/2) or < 0 on error. */
+int or_subdiv_trig_parts(const double *lset_nodal, int lset_order, int L, double part[2]);
 int or_subdiv_moments3(const double *lset_nodal, int lset_order, int L,
                        int order, int q, const double *Finv, double *out_neg,
                        double *out_if);

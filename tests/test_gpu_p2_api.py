@@ -37,7 +37,7 @@ def test_cutinfo_p2_domains_and_ratios(X, orc, L):
     mesh, g = sphere_p2(X)
     ci = X.CutInfo(mesh, g, subdivlvl=L)
     nod = nodal(mesh, g)
-    dom = np.array([orc.subdiv_rule3(v, 2, L, 0)[0] for v in nod])
+    dom = np.array([part_volume_domain(orc, v, L) for v in nod])
     for dt, cdt in ((NEG, X.NEG), (POS, X.POS), (IF, X.IF)):
         assert np.array_equal(ci.GetElementsOfType(cdt).numpy(), dom == dt)
     assert (dom == IF).sum() > 20 and (dom == NEG).sum() > 0
@@ -47,8 +47,8 @@ def test_cutinfo_p2_domains_and_ratios(X, orc, L):
         vn, vp = parts[0][1].sum(), parts[1][1].sum()
         assert abs(cr[e] - vn / (vn + vp)) < 1e-12
     assert np.all(cr[dom == NEG] == 1.0) and np.all(cr[dom == POS] == 0.0)
-    with pytest.raises(Exception):
-        ci.GetElementsOfType(X.IF, X.BND)
+    # the sphere stays inside the cube: every boundary triangle is POS
+    assert ci.GetElementsOfType(X.POS, X.BND).numpy().all()
 
 
 @pytest.mark.parametrize("dt", [NEG, POS, IF])
@@ -95,3 +95,65 @@ def test_dcut_p2_element_matrices(X, orc, p, dt):
         pts, w = parts[dt]
         ref = orc.elmat(3, p, V, np.ascontiguousarray(pts), np.ascontiguousarray(w), mask=3)
         assert np.abs(mats[k] - ref).max() <= 1e-12 * np.abs(ref).max(), e
+
+
+def part_volume_domain(orc, nod, L):
+    """cutinfo.cpp:104-124 on the oracle's order-0 NEG and POS rules"""
+    _, parts = orc.subdiv_rule3(nod, 2, L, 0)
+    vn, vp = parts[NEG][1].sum(), parts[POS][1].sum()
+    return (IF if vp > 0.0 else NEG) if vn > 0.0 else POS
+
+
+@pytest.mark.parametrize("L", [1, 2, 3])
+def test_cutinfo_p2_lattice_cut_but_one_sided(X, orc, L):
+    """Elements the lattice check calls cut, whose leaves all fall to one side
+    (exact zeros on a negative level set; tiny negative values on a positive
+    one, below the leaves' relative 1e-14 threshold): the reference marks
+    them by their part volumes, so they are uncut"""
+    mesh = X.MakeStructured3DMesh(nx=3, ny=3, nz=3)
+    dofs, _ = mesh.dofs(2)
+    for sign, odd in ((-1.0, 0.0), (1.0, -1e-17)):
+        g = X.GridFunction(X.H1(mesh, order=2))
+        g.vec[:] = sign
+        g.vec[dofs[5, 4:7]] = odd
+        g.vec[dofs[40, 9]] = odd
+        nod = np.ascontiguousarray(g.vec[dofs])
+        lattice = np.array([orc.subdiv_rule3(v, 2, L, 0)[0] for v in nod])
+        want = np.array([part_volume_domain(orc, v, L) for v in nod])
+        assert (lattice == IF).sum() > 0 and not np.any(want == IF)
+        ci = X.CutInfo(mesh, g, subdivlvl=L)
+        for dt, cdt in ((NEG, X.NEG), (POS, X.POS), (IF, X.IF)):
+            assert np.array_equal(ci.GetElementsOfType(cdt).numpy(), want == dt), (sign, dt)
+        assert np.array_equal(ci.GetElementsOfType(X.HASNEG).numpy(), want != POS)
+        cr = ci.GetCutRatios()
+        assert np.all(cr[want == NEG] == 1.0) and np.all(cr[want == POS] == 0.0)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("L", [0, 2, 3])
+def test_cutinfo_p2_boundary_elements(X, orc, L, order):
+    """Boundary triangles (cutinfo.cpp:80-124 with vb = BND) of a mesh cut by
+    a sphere centred on a face: domains from the part areas of the oracle's
+    triangle subdivision rule on each boundary triangle's restriction, bit
+    for bit, cut ratios to 1e-12 (P2 level set, or P1 with subdivlvl)"""
+    mesh = X.MakeStructured3DMesh(nx=4, ny=4, nz=4)
+    g = X.GridFunction(X.H1(mesh, order=order))
+    g.Set(lambda x, y, z: (x - 0.45) ** 2 + (y - 0.52) ** 2 + z * z - 0.4 ** 2)
+    ci = X.CutInfo(mesh, g, subdivlvl=L)
+    if order == 2:
+        tab, _ = mesh.bnd_dofs(2)
+        nod = g.vec[tab]
+    else:
+        b = mesh.bnd2v
+        v = g.vec[b]
+        nod = np.hstack([v, 0.5 * (v[:, [0, 0, 1]] + v[:, [1, 2, 2]])])
+    want, wcr = [], []
+    for c in nod:
+        _, an, ap = orc.subdiv_trig_parts(np.ascontiguousarray(c), 2, L)
+        want.append((IF if ap > 0 else NEG) if an > 0 else POS)
+        wcr.append(an / (an + ap))
+    want, wcr = np.array(want), np.array(wcr)
+    assert (want == IF).sum() > 5 and (want == NEG).sum() > 0
+    for dt, cdt in ((NEG, X.NEG), (POS, X.POS), (IF, X.IF)):
+        assert np.array_equal(ci.GetElementsOfType(cdt, X.BND).numpy(), want == dt), dt
+    assert np.abs(np.asarray(ci.GetCutRatios(X.BND)) - wcr).max() <= 1e-12

@@ -435,3 +435,44 @@ def test_mark_p2_matches_subdivision_domain(orc, dev, L):
         want = np.array([(c == 7) or bool(c & (4 if d == IF else (1 if d == NEG else 2))) for d in ref])
         got = np.array([(b[c, e >> 5] >> (e & 31)) & 1 for e in range(ne)], bool)
         assert np.array_equal(got, want), c
+
+
+@pytest.mark.parametrize("L", [0, 2, 4, 5])
+def test_mark_p2_trig_matches_oracle(orc, dev, L):
+    """P2 marking of boundary triangles (xg_mark_p2_trig) == the oracle's
+    order-0 triangle subdivision rule: domains from the part areas bit for
+    bit, cut ratios to 1e-12, bit arrays and cut list; cut, barely cut,
+    uncut, zero-valued and one-sided lattice-cut elements, a ragged count"""
+    torch, _abi, ctx = dev
+    rng = np.random.default_rng(900 + L)
+    vals = rng.normal(size=(150, 6))
+    vals[50:100] += rng.uniform(-3, 3, (50, 1))
+    vals[100:] = vals[100:] * 0.02 + rng.uniform(-0.01, 0.01, (50, 1))
+    extra = np.array([np.zeros(6), np.r_[np.zeros(3), np.ones(3)], np.r_[-np.ones(5), 0.0],
+                      np.r_[np.ones(5), -1e-17], np.r_[np.ones(3), -np.ones(3) * 1e-3]])
+    vals = np.ascontiguousarray(np.vstack([vals, extra]))
+    ne = vals.shape[0]
+    ref_dt, ref_cr = np.zeros(ne, np.uint8), np.zeros(ne)
+    for e in range(ne):
+        _, an, ap = orc.subdiv_trig_parts(vals[e], 2, L)
+        ref_dt[e] = (IF if ap > 0 else NEG) if an > 0 else POS
+        ref_cr[e] = an / (an + ap)
+    nw = (ne + 31) // 32
+    dt = torch.zeros(ne, dtype=torch.uint8, device="cuda")
+    cr = torch.zeros(ne, dtype=torch.float64, device="cuda")
+    bits = torch.zeros(8 * nw, dtype=torch.int32, device="cuda")
+    cl = torch.full((ne,), -1, dtype=torch.int32, device="cuda")
+    nc = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ctx.mark_p2_trig(up(torch, vals), L, dt, cr, bits, cl, nc)
+    ctx.sync()
+    g = dt.cpu().numpy()
+    assert np.array_equal(g, ref_dt)
+    assert (ref_dt == IF).sum() > 10 and (ref_dt == NEG).sum() > 0 and (ref_dt == POS).sum() > 0
+    assert np.abs(cr.cpu().numpy() - ref_cr).max() <= 1e-12
+    n = int(nc.item())
+    assert np.array_equal(cl.cpu().numpy()[:n], np.nonzero(ref_dt == IF)[0])
+    b = bits.cpu().numpy().view(np.uint32).reshape(8, nw)
+    for c in range(8):
+        want = np.array([(c == 7) or bool(c & (4 if d == IF else (1 if d == NEG else 2))) for d in ref_dt])
+        got = np.array([(b[c, e >> 5] >> (e & 31)) & 1 for e in range(ne)], bool)
+        assert np.array_equal(got, want), c
