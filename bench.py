@@ -452,8 +452,26 @@ def bench_extras(args):
         qd[name] = {"cut_elements": ncut, "points": tot, "ms": ms, "cut_elem_per_s": ncut / (ms * 1e-3)}
     out["quad_hex"] = {"mesh": "64^3 hexahedra (%d), the 3D plain level set, order 4, FIND_OPTIMAL" % h.ne,
                        "paths": qd}
+    # ---- boundary triangles with a P2 level set (a sphere centred on the
+    # face z = 0), subdivision level 4
+    b2v = mesh.bnd2v
+    P = mesh.coords[b2v]                                                   # nse x 3 x 3
+    Q = np.concatenate([P, 0.5 * (P[:, [0, 0, 1]] + P[:, [1, 2, 2]])], axis=1)
+    phi = ((Q - np.array([0.5, 0.5, 0.0])) ** 2).sum(axis=2) - 0.35 ** 2   # nse x 6 nodal values
+    nse = phi.shape[0]
+    d_phi = up(phi)
+    bdt = torch.empty(nse, dtype=torch.uint8, device="cuda")
+    bcr = torch.empty(nse, dtype=torch.float64, device="cuda")
+    bbits = torch.empty(8 * ((nse + 31) // 32), dtype=torch.int32, device="cuda")
+    bcut = torch.empty(nse, dtype=torch.int32, device="cuda")
+    bnc = torch.empty(1, dtype=torch.int32, device="cuda")
+    ms = timed(lambda: ctx.mark_p2_trig(d_phi, 4, bdt, bcr, bbits, bcut, bnc))
+    out["p2_boundary_marking"] = {"mesh": "boundary of the 64^3 Kuhn mesh (%d triangles), P2 sphere on a face, "
+                                          "subdivision level 4" % nse,
+                                  "cut": int(bnc.item()), "ms": ms, "triangles_per_s": nse / (ms * 1e-3)}
     print(json.dumps({"metric": METRIC, "config": {"workload": "secondary paths: node / dof marking, several "
-                                                               "level sets, quad / hex straight cuts"},
+                                                               "level sets, quad / hex straight cuts, P2 "
+                                                               "boundary marking"},
                       "paths": out}), flush=True)
     ctx.close()
 

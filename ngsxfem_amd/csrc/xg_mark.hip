@@ -404,8 +404,8 @@ __global__ __launch_bounds__(BLK) void k_mark_p2_lattice(int ne, const double *_
 // Simplex::CheckIfCut, xdecompose.hpp:34-85).  For order 0 only areas count,
 // and a node the lattice finds uncut has leaves whose vertices all carry its
 // sign, so the adaptive tree gives the areas of the uniform level-L
-// refinement: one wave per triangle, lanes over the n x n cells of the
-// lattice (n = 2^L), each cell holding an upward (a,b),(a+1,b),(a,b+1) and a
+// refinement: the triangles the Bernstein certificate leaves undecided, one
+// wave each in turn, lanes over the n x n cells of the lattice (n = 2^L), each cell holding an upward (a,b),(a+1,b),(a,b+1) and a
 // downward (a+1,b),(a,b+1),(a+1,b+1) leaf.  Coordinates are exact dyadics,
 // so the values equal those of the nested evaluation.  Domain from the part
 // areas (IF iff both > 0), cut ratio NEG / (NEG + POS).
@@ -473,63 +473,102 @@ XG_DEV void trig_leaf(const double (*V)[2], const double *v, double &an, double 
   (minor_neg ? ap : an) += amaj;
 }
 
+// pass 1, one lane per triangle: the Bernstein-Bezier certificate of
+// k_mark_p2_bern (coefficients: vertex values, 2 phi_ij - (phi_i + phi_j) / 2
+// on the edges); certified triangles get their domain and cut ratio
+__global__ __launch_bounds__(BLK) void k_mark_p2_trig_bern(int ne, const double *__restrict__ lset_nodal,
+                                                           uint8_t *__restrict__ dt_out,
+                                                           double *__restrict__ cut_ratio) {
+  const int e = blockIdx.x * BLK + threadIdx.x;
+  if (e >= ne) return;
+  double c[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) c[k] = lset_nodal[(size_t)e * 6 + k];
+  constexpr int EI[3] = {0, 0, 1}, EJ[3] = {1, 2, 2};
+  double bb[6];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    bb[k] = c[k];
+    bb[3 + k] = 2.0 * c[3 + k] - 0.5 * (c[EI[k]] + c[EJ[k]]);
+  }
+  double mx = 0.0, mn = bb[0], mxb = bb[0];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    mx = fmax(mx, fabs(bb[k]));
+    mn = fmin(mn, bb[k]);
+    mxb = fmax(mxb, bb[k]);
+  }
+  const double tol = 1e-12 * mx;
+  uint8_t d = XG_UNDECIDED;
+  if (mx > 0.0 && mn > tol) d = POS;
+  else if (mx > 0.0 && mxb < -tol) d = NEG;
+  dt_out[e] = d;
+  if (d != XG_UNDECIDED) cut_ratio[e] = d == POS ? 0.0 : 1.0;
+}
+
+// pass 2, one wave per triangle (undecided ones are clustered along the
+// cut, so a wave per group of them would serialise): waves of certified
+// triangles return after one byte load; lattice check and leaves
+// lane-parallel
 __global__ __launch_bounds__(BLK) void k_mark_p2_trig(int ne, const double *__restrict__ lset_nodal, int L,
                                                       uint8_t *__restrict__ dt_out,
                                                       double *__restrict__ cut_ratio) {
   const int lane = threadIdx.x & 63;
   const int e = __builtin_amdgcn_readfirstlane((blockIdx.x * BLK + threadIdx.x) >> 6);
-  if (e >= ne) return;
-  double c[6];
-#pragma unroll
-  for (int k = 0; k < 6; ++k) c[k] = lset_nodal[(size_t)e * 6 + k];
+  if (e >= ne || dt_out[e] != XG_UNDECIDED) return;
   const int n = 1 << L;
   const double inv_n = 1.0 / n;
-  // lattice check of the whole triangle (xintegration.cpp:620-755)
-  bool hp = false, hn = false;
-  for (int q = lane; q < (n + 1) * (n + 1); q += 64) {
-    const int a = q % (n + 1), b = q / (n + 1);
-    if (a + b > n) continue;
-    const double x[2] = {a * inv_n, b * inv_n};
-    const double v = lagrange_eval2<2>(c, x);
-    if (v >= 0.0) hp = true;
-    else hn = true;
-  }
-  const bool anyp = __ballot(hp) != 0ull, anyn = __ballot(hn) != 0ull;
-  if (!(anyp && anyn)) {
-    if (lane == 0) {
-      dt_out[e] = anyp ? POS : NEG;
-      cut_ratio[e] = anyp ? 0.0 : 1.0;
-    }
-    return;
-  }
-  double an = 0.0, ap = 0.0;
-  for (int q = lane; q < n * n; q += 64) {
-    const int a = q % n, b = q / n;
-    if (a + b > n - 1) continue;
-    const double x00[2] = {a * inv_n, b * inv_n}, x10[2] = {(a + 1) * inv_n, b * inv_n},
-                 x01[2] = {a * inv_n, (b + 1) * inv_n};
-    const double v00 = lagrange_eval2<2>(c, x00), v10 = lagrange_eval2<2>(c, x10),
-                 v01 = lagrange_eval2<2>(c, x01);
-    {
-      const double V[3][2] = {{x00[0], x00[1]}, {x10[0], x10[1]}, {x01[0], x01[1]}};
-      const double v[3] = {v00, v10, v01};
-      trig_leaf(V, v, an, ap);
-    }
-    if (a + b <= n - 2) {
-      const double x11[2] = {(a + 1) * inv_n, (b + 1) * inv_n};
-      const double V[3][2] = {{x10[0], x10[1]}, {x01[0], x01[1]}, {x11[0], x11[1]}};
-      const double v[3] = {v10, v01, lagrange_eval2<2>(c, x11)};
-      trig_leaf(V, v, an, ap);
-    }
-  }
+  {
+    double c[6];
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    an += __shfl_xor(an, o, 64);
-    ap += __shfl_xor(ap, o, 64);
-  }
-  if (lane == 0) {
-    dt_out[e] = an > 0.0 ? (ap > 0.0 ? IF : NEG) : POS;
-    cut_ratio[e] = an / (an + ap);
+    for (int k = 0; k < 6; ++k) c[k] = lset_nodal[(size_t)e * 6 + k];
+    // lattice check of the whole triangle (xintegration.cpp:620-755)
+    bool hp = false, hn = false;
+    for (int q = lane; q < (n + 1) * (n + 1); q += 64) {
+      const int a = q % (n + 1), b = q / (n + 1);
+      if (a + b > n) continue;
+      const double x[2] = {a * inv_n, b * inv_n};
+      const double v = lagrange_eval2<2>(c, x);
+      if (v >= 0.0) hp = true;
+      else hn = true;
+    }
+    const bool anyp = __ballot(hp) != 0ull, anyn = __ballot(hn) != 0ull;
+    if (!(anyp && anyn)) {
+      if (lane == 0) {
+        dt_out[e] = anyp ? POS : NEG;
+        cut_ratio[e] = anyp ? 0.0 : 1.0;
+      }
+      return;
+    }
+    double an = 0.0, ap = 0.0;
+    for (int q = lane; q < n * n; q += 64) {
+      const int a = q % n, b = q / n;
+      if (a + b > n - 1) continue;
+      const double x00[2] = {a * inv_n, b * inv_n}, x10[2] = {(a + 1) * inv_n, b * inv_n},
+                   x01[2] = {a * inv_n, (b + 1) * inv_n};
+      const double v00 = lagrange_eval2<2>(c, x00), v10 = lagrange_eval2<2>(c, x10),
+                   v01 = lagrange_eval2<2>(c, x01);
+      {
+        const double V[3][2] = {{x00[0], x00[1]}, {x10[0], x10[1]}, {x01[0], x01[1]}};
+        const double v[3] = {v00, v10, v01};
+        trig_leaf(V, v, an, ap);
+      }
+      if (a + b <= n - 2) {
+        const double x11[2] = {(a + 1) * inv_n, (b + 1) * inv_n};
+        const double V[3][2] = {{x10[0], x10[1]}, {x01[0], x01[1]}, {x11[0], x11[1]}};
+        const double v[3] = {v10, v01, lagrange_eval2<2>(c, x11)};
+        trig_leaf(V, v, an, ap);
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      an += __shfl_xor(an, o, 64);
+      ap += __shfl_xor(ap, o, 64);
+    }
+    if (lane == 0) {
+      dt_out[e] = an > 0.0 ? (ap > 0.0 ? IF : NEG) : POS;
+      cut_ratio[e] = an / (an + ap);
+    }
   }
 }
 
@@ -897,9 +936,10 @@ extern "C" int xg_mark_p2_trig(xg_ctx *ctx, int ne, const double *lset_nodal, in
     if (n_cut) XG_CHECK(hipMemsetAsync(n_cut, 0, sizeof(int32_t), ctx->stream));
     return XG_OK;
   }
-  const long long waves = ne;
-  k_mark_p2_trig<<<(unsigned)((waves * 64 + BLK - 1) / BLK), BLK, 0, ctx->stream>>>(ne, lset_nodal, L, dt,
-                                                                                  cut_ratio);
+  const int nblk = (ne + BLK - 1) / BLK;
+  k_mark_p2_trig_bern<<<nblk, BLK, 0, ctx->stream>>>(ne, lset_nodal, dt, cut_ratio);
+  k_mark_p2_trig<<<(unsigned)(((long long)ne * 64 + BLK - 1) / BLK), BLK, 0, ctx->stream>>>(ne, lset_nodal, L, dt,
+                                                                                       cut_ratio);
   XG_LAUNCH_CHECK();
   return xg::bits_and_cut_list_from_dt(ctx, ne, dt, bits, cut_list, n_cut);
 }
