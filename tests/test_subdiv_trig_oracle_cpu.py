@@ -73,3 +73,14 @@ def test_lattice_cut_but_one_sided(orc, L):
     assert dom == IF and an == 0.0 and abs(ap - 0.5) < 1e-15
     # at level 0 the lattice holds only the vertices
     assert orc.subdiv_trig_parts(np.r_[-np.ones(5), 0.0], 2, 0)[0] == NEG
+
+
+@pytest.mark.parametrize("L", [1, 2, 3])
+def test_tet_lattice_cut_but_one_sided(orc, L):
+    """the same on tetrahedra (or_subdiv_rule3): the lattice domain is IF,
+    but one order-0 rule is empty and the other holds the whole volume 1/6,
+    so CutInformation::Update (part volumes) marks the element uncut"""
+    dom, parts = orc.subdiv_rule3(np.r_[-np.ones(9), 0.0], 2, L, 0)
+    assert dom == IF and len(parts[POS][1]) == 0 and abs(parts[NEG][1].sum() - 1 / 6) < 1e-15
+    dom, parts = orc.subdiv_rule3(np.r_[np.ones(9), -1e-17], 2, L, 0)
+    assert dom == IF and len(parts[NEG][1]) == 0 and abs(parts[POS][1].sum() - 1 / 6) < 1e-15
