@@ -1,15 +1,17 @@
 """Benchmark of the cut-element hot path on one MI355X.
 
-Workload (default): the 3D plain configuration -- unit cube, 64^3 cells x 6
-Kuhn tets = 1,572,864 tets, P1 level set |x - (0.5,0.5,0.5)| - 0.35 +
-0.01 sin(7x) sin(7y) sin(7z) (perturbed as InterpolateToP1 does), order-4
-straight cut rules on NEG and IF, P2 H1 mass + stiffness element matrices.
+Workload (default): the large mixed configuration -- unit cube, 128^3 cells x
+6 Kuhn tets = 12,582,912 tets, P2 level set of the union (min) of 64 spheres
+(radii U[0.05, 0.15], centres U[0.2, 0.8]^3, MT19937 seed 7); P2 marking at
+subdivision level 4 (timed on its own); one step = the level-4 subdivision
+rule of every cut element on NEG and IF contracted into FP64 degree-6
+moments, and the P3 mass + stiffness element matrices from them
+(xg_subdiv_pipeline, FP32 store), inputs resident in HBM.  value = cut
+elements / second.  The 3D plain configuration (64^3 x 6 Kuhn tets, P1 level
+set, order-4 straight cut rules, P2 matrices) is measured in the same run and
+reported under "c2".
 
-One step = the fused rule + element-matrix pass (xg_cut_pipeline) over all cut
-elements for both domain types, inputs resident in HBM.  value = cut
-elements / second.  Marking (xg_mark) is timed separately.
-
-    python bench.py [--steps K] [--warmup W] [--precision fp32|fp64]
+    python bench.py [--steps K] [--warmup W] [--config c5|c2|c3|c4|extras]
 """
 import argparse
 import json
@@ -47,7 +49,7 @@ def fp64_ops_moments(npts, n_cut, D=3, p=2):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=64, help="cells per direction")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"], help="element-matrix store type")
@@ -57,8 +59,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="launch each path 3 times plus a 256 MiB copy, for rocprofv3 --pmc runs")
-    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5", "extras"],
-                    help="c2: 3D plain straight-cut pipeline (headline); c4: learned quadrature on P2-cut tets")
+    ap.add_argument("--config", default="c5", choices=["c2", "c3", "c4", "c5", "extras"],
+                    help="c5 (default): large mixed P2-level-set pipeline, with the 3D plain straight-cut "
+                         "pipeline (c2) beside it; c2 / c3 / c4 / extras: one configuration alone")
     ap.add_argument("--c4-n", type=int, default=2000000)
     ap.add_argument("--c4-train", type=int, default=100000)
     ap.add_argument("--c5-n", type=int, default=128)
@@ -158,7 +161,7 @@ def bench_c4(args):
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
-        cores = min(16, os.cpu_count() or 1)
+        cores = cpu_threads()
         ns = 4000
         xs = x[:ns].cpu().numpy()
         t1 = time.perf_counter()
@@ -188,13 +191,51 @@ def bench_c4(args):
     ctx.close()
 
 
-def bench_c5(args):
+def c5_fp64_ops(stats, p=3):
+    """FP64 operations (FMA = 2) of the exact-rule subdivision kernel from its
+    per-element counts (table rows, volume tets and interface triangles
+    integrated point by point, cut leaves), per pass:
+      lattice      969 P2 evaluations (10 shape functions: 22 products for
+                   the shapes, 10 FMA) = 42 flops each;
+      table rows   NM additions each;
+      volume tets  24 points x (mapping 27 + degree-3 monomials 30 + weighted
+                   monomials 21 + NM FMA) + determinant 17;
+      IF triangles 12 points x (mapping 18 + 30 + 21 + 2 NM) + 40 (cross
+                   product, normal, interface transform);
+      cut leaves   6 edges x 8 (cut points) + 4 P2 evaluations x 42."""
+    nm = {1: 10, 2: 35, 3: 84}[p]
+    nq_v, nq_i = {1: (4, 3), 2: (14, 6), 3: (24, 12)}[p]
+    rows, vol, itf, leaves = (float(stats[..., k].sum()) for k in range(4))
+    passes = stats.shape[0] * stats.shape[1]
+    return (passes * 969 * 42 + rows * nm + vol * (nq_v * (27 + 30 + 21 + 2 * nm) + 17)
+            + itf * (nq_i * (18 + 30 + 21 + 2 * nm) + 40) + leaves * (48 + 4 * 42))
+
+
+def cpu_threads():
+    """host threads for the CPU baseline: every CPU this process may run on"""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def measure_c5(args):
     """Large mixed: 128^3 cells x 6 Kuhn tets (12.6M tets), P2 level set of the
-    union of 64 random spheres, P3 element matrices.  Steps: P2 marking
-    (Bernstein certificate + lattice check), then on the cut elements the
-    subdivision rule (level L) contracted to degree-6 moments and the P3
-    mass + stiffness matrices from the moment identities (FP64 arithmetic,
-    FP32 store)."""
+    union of 64 random spheres (MT19937 seed 7), P3 element matrices.
+    Marking (P2, level 4) is timed on its own; one step is the fused
+    subdivision rule (level 4, order 6) + P3 mass + stiffness matrices on NEG
+    and IF of every cut element (xg_subdiv_pipeline: FP64 moments and
+    identities, FP32 store)."""
     import torch
 
     from ngsxfem_amd import _abi
@@ -204,7 +245,8 @@ def bench_c5(args):
     stream = torch.cuda.current_stream()
     ctx = _abi.Context(0, stream=stream)
     dev = torch.device("cuda:0")
-    n, L, p = args.c5_n, args.c5_L, 3
+    n, L, p, order = args.c5_n, args.c5_L, 3, 6
+    nd, nm = 20, 84
     t0 = time.time()
     mesh = MakeStructured3DMesh(nx=n, ny=n, nz=n)
     centres, radii = W.config5_spheres(7)
@@ -212,7 +254,7 @@ def bench_c5(args):
     d_x = torch.from_numpy(mesh.coords).to(dev)
     lp2 = W.p2_nodal_values_torch(d_el, d_x, centres, radii)
     torch.cuda.synchronize()
-    print("# workload built in %.1fs: %d tets" % (time.time() - t0, mesh.ne), file=sys.stderr, flush=True)
+    print("# large mixed workload built in %.1fs: %d tets" % (time.time() - t0, mesh.ne), file=sys.stderr, flush=True)
     ne = mesh.ne
     nw = (ne + 31) // 32
     dt = torch.empty(ne, dtype=torch.uint8, device=dev)
@@ -235,29 +277,113 @@ def bench_c5(args):
     ms_mark = timed(lambda: ctx.mark_p2(lp2, L, dt, bits, cut, ncut), 5)
     n_cut = int(ncut.item())
     cl = cut[:n_cut].clone()
-    lcut = lp2[cl.long()].contiguous()
-    nm = 84
-    mom = torch.empty(n_cut * nm, dtype=torch.float64, device=dev)
-    out = torch.empty(n_cut * 400, dtype=torch.float32, device=dev)
-
-    def plain():
-        ctx.subdiv_moments(lcut, 2, L, 2 * p, p, mom, None, None)
-        ctx.elmat_moments(3, p, mom, 3, _abi.XG_PREC_FP32, out, el2v=d_el, coords=d_x, elems=cl)
-
-    ms_plain = timed(plain, args.steps)
-    ms_sub = timed(lambda: ctx.subdiv_moments(lcut, 2, L, 2 * p, p, mom, None, None), 2)
     counts = np.bincount(dt.cpu().numpy(), minlength=3)
-    res = {"metric": METRIC, "value": n_cut / (ms_plain * 1e-3), "unit": "cut-elem/s", "n_gpus": 1,
-           "steps": args.steps, "ms_per_step": ms_plain, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "fp64+fp32", "data": "synthetic",
+    dts = [_abi.XG_NEG, _abi.XG_IF]
+    out = torch.empty(n_cut * 2 * nd * nd, dtype=torch.float32, device=dev)
+
+    def step():
+        ctx.subdiv_pipeline(p, L, order, cl, d_el, d_x, lp2, dts, 3, _abi.XG_PREC_FP32, out)
+
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+    torch.cuda.synchronize()
+    g0 = torch.cuda.Event(enable_timing=True)
+    g1 = torch.cuda.Event(enable_timing=True)
+    g0.record(stream)
+    for _ in range(args.steps):
+        step()
+    g1.record(stream)
+    ctx.sync()
+    torch.cuda.synchronize()
+    ms = g0.elapsed_time(g1) / args.steps
+    print("# large mixed step %.3f ms over %d steps" % (ms, args.steps), file=sys.stderr, flush=True)
+
+    # the dominant kernel alone (NEG pass: k_subdiv_exact<2,3,NEG,1>), its
+    # launches back to back between two events on the stream it runs on
+    mom = torch.empty(n_cut * nm, dtype=torch.float64, device=dev)
+    stats = torch.zeros(n_cut * 2 * 4, dtype=torch.int32, device=dev)
+    ms_neg = timed(lambda: ctx.subdiv_moments_mesh(p, L, order, cl, d_el, d_x, lp2, [_abi.XG_NEG], mom), 3)
+    ms_if = timed(lambda: ctx.subdiv_moments_mesh(p, L, order, cl, d_el, d_x, lp2, [_abi.XG_IF], mom), 3)
+    mom2 = torch.empty(n_cut * 2 * nm, dtype=torch.float64, device=dev)
+    ctx.subdiv_moments_mesh(p, L, order, cl, d_el, d_x, lp2, dts, mom2, stats)
+    ctx.sync()
+    st = stats.cpu().numpy().reshape(n_cut, 2, 4)
+    ops_neg = c5_fp64_ops(st[:, :1])
+    ops_all = c5_fp64_ops(st)
+
+    # error of the FP32 step against the same pipeline with FP64 output
+    ref = torch.empty(n_cut * 2 * nd * nd, dtype=torch.float64, device=dev)
+    ctx.subdiv_pipeline(p, L, order, cl, d_el, d_x, lp2, dts, 3, _abi.XG_PREC_FP64, ref)
+    ctx.sync()
+    r64 = ref.view(-1, nd * nd)
+    rn = torch.linalg.vector_norm(r64, dim=1)
+    nzr = rn > 0
+    dn = torch.linalg.vector_norm(out.view(-1, nd * nd).double() - r64, dim=1)
+    err32 = float((dn[nzr] / rn[nzr]).max().item())
+
+    roof = {"bound": "fp64-valu", "kernel": "k_subdiv_exact<2,3,NEG,1>",
+            "achieved": ops_neg / (ms_neg * 1e-3) / 1e12, "peak": PEAK_FP64_VALU / 1e12, "unit": "TFLOP/s",
+            "frac": ops_neg / (ms_neg * 1e-3) / PEAK_FP64_VALU, "traffic": None,
+            "algorithmic_flops_per_launch": ops_neg, "avg_launch_ms": ms_neg,
+            "counts_per_cut_element": {"table_rows": float(st[:, 0, 0].mean()),
+                                       "volume_tets": float(st[:, 0, 1].mean()),
+                                       "interface_triangles": float(st[:, 1, 2].mean()),
+                                       "cut_leaves": float(st[:, 0, 3].mean())},
+            "note": "traffic null: the kernel reads 80 B of nodal values and writes 672 B of moments per element "
+                    "(0.3 GB per launch); its bound is the FP64 vector ALU, flops counted from the kernel's own "
+                    "per-element piece counts"}
+    res = {"metric": METRIC, "value": n_cut / (ms * 1e-3), "unit": "cut-elem/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "fp64+fp32", "data": "synthetic",
            "config": {"workload": "large mixed: unit cube %d^3 x 6 Kuhn tets (%d tets), P2 level set = min of 64 "
-                                  "spheres (MT19937 seed 7), subdivision rule L=%d (order 6) -> degree-6 moments -> "
-                                  "P3 mass+stiffness elmat (FP64 identities, FP32 store)" % (n, ne, L),
+                                  "spheres r U[0.05,0.15] centres U[0.2,0.8]^3 (MT19937 seed 7), subdivision rule "
+                                  "L=%d order 6 on NEG+IF -> FP64 degree-6 moments -> P3 mass+stiffness elmat "
+                                  "(FP64 identities, FP32 store)" % (n, ne, L),
                       "cut_elements": n_cut, "neg": int(counts[0]), "pos": int(counts[1])},
-           "marking": {"ms": ms_mark, "tets_per_s": ne / (ms_mark * 1e-3)},
-           "subdivision_ms": ms_sub}
-    print(json.dumps(res), flush=True)
+           "roofline": roof,
+           "max_rel_err_vs_fp64": err32,
+           "paths": {"subdivision_neg_ms": ms_neg, "subdivision_if_ms": ms_if,
+                     "elmat_and_rest_ms": ms - ms_neg - ms_if,
+                     "fp64_tflops_both_passes": ops_all / ((ms_neg + ms_if) * 1e-3) / 1e12},
+           "marking": {"ms": ms_mark, "tets_per_s": ne / (ms_mark * 1e-3), "kernel": "xg_mark_p2 (L=4)"}}
+
+    # ---- CPU baseline: the oracle (reference algorithm: materialised level-4
+    # subdivision rule + P3 element matrices per point, FP64) on a bounded
+    # sample of the cut elements, all host threads
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = cpu_threads()
+        cln = cl.cpu().numpy()
+        vals = lp2.cpu().numpy()
+        rng = np.random.default_rng(11)
+        order_ = rng.permutation(n_cut)
+        probe = np.sort(cln[order_[:threads]]).astype(np.int32)
+        t1 = time.perf_counter()
+        O.subdiv_pipeline(p, L, order, probe, mesh.el2v, mesh.coords, vals, dts, 3, nthreads=threads)
+        per = (time.perf_counter() - t1) / len(probe)
+        ns = int(min(n_cut, max(threads, args.cpu_seconds / max(per, 1e-9))))
+        sample = np.sort(cln[order_[:ns]]).astype(np.int32)
+        t1 = time.perf_counter()
+        ref_s = O.subdiv_pipeline(p, L, order, sample, mesh.el2v, mesh.coords, vals, dts, 3, nthreads=threads)
+        dt_ = time.perf_counter() - t1
+        # the GPU's FP32 step output on the same elements against the oracle
+        pos = np.searchsorted(cln, sample)
+        g = out.view(n_cut, 2, nd, nd)[torch.from_numpy(pos).to(dev).long()].double().cpu().numpy()
+        scale = np.abs(ref_s).max(axis=(-2, -1)) + 1e-300
+        err_o = float((np.abs(g - ref_s).max(axis=(-2, -1)) / scale).max())
+        res["cpu_baseline"] = {"value": ns / dt_, "unit": "cut-elem/s", "cores": threads, "kind": "port",
+                               "sample": "%d random cut elements of %d (oracle: level-4 subdivision rule "
+                                         "materialised, P3 elmat per point, NEG+IF, FP64, %d threads, %.1fs; %s, "
+                                         "os.cpu_count()=%s)" % (ns, n_cut, threads, dt_, cpu_model(),
+                                                                 os.cpu_count())}
+        res["max_rel_err_vs_oracle_sample"] = err_o
     ctx.close()
+    return res
+
+
+def bench_c5(args):
+    print(json.dumps(measure_c5(args)), flush=True)
 
 
 def bench_c3(args):
@@ -314,7 +440,7 @@ def bench_c3(args):
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
-        cores = min(16, os.cpu_count() or 1)
+        cores = cpu_threads()
         t_cpu, done = 0.0, 0
         for m in range(len(ps)):
             if t_cpu > args.cpu_seconds:
@@ -476,20 +602,9 @@ def bench_extras(args):
     ctx.close()
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (every element is independent, so the path "
-              "shards by contiguous element ranges, one range per GPU); run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
-    if args.config == "c4":
-        return bench_c4(args)
-    if args.config == "c3":
-        return bench_c3(args)
-    if args.config == "c5":
-        return bench_c5(args)
-    if args.config == "extras":
-        return bench_extras(args)
+def measure_c2(args):
+    """3D plain configuration (C2 of the header): fused straight-cut rule +
+    P2 element matrices; returns the result dict"""
     import torch
 
     from ngsxfem_amd import _abi
@@ -648,7 +763,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
-        cores = min(16, os.cpu_count() or 1)
+        cores = cpu_threads()
         cl = cut_list.cpu().numpy()
         reps, t_cpu = 0, 0.0
         while t_cpu < args.cpu_seconds and reps < 100:
@@ -674,8 +789,32 @@ def main():
                     "hbm_frac_algorithmic": ne * 26.4 / (mark_ms * 1e-3) / PEAK_HBM,
                     "k_mark_traffic": read_traffic("marking_c2_pmc.txt")},
     }
-    print(json.dumps(res), flush=True)
     ctx.close()
+    return res
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (every element is independent, so the path "
+              "shards by contiguous element ranges, one range per GPU); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    if args.config == "c4":
+        return bench_c4(args)
+    if args.config == "c3":
+        return bench_c3(args)
+    if args.config == "c2":
+        print(json.dumps(measure_c2(args)), flush=True)
+        return
+    if args.config == "extras":
+        return bench_extras(args)
+    # default: the large mixed workload (the largest configuration) as the
+    # headline, the 3D plain workload beside it
+    res = measure_c5(args)
+    c2 = measure_c2(args)
+    res["c2"] = {k: c2[k] for k in ("value", "ms_per_step", "config", "roofline", "cpu_baseline", "paths",
+                                    "roofline_plain_mfma", "marking")}
+    print(json.dumps(res), flush=True)
 
 
 if __name__ == "__main__":

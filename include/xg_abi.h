@@ -299,6 +299,35 @@ int xg_subdiv_moments(xg_ctx *ctx, int nel, const double *lset_nodal, int lset_o
                       int order, int p, const double *finv, double *out_neg, double *out_if,
                       uint8_t *dom);
 
+/* The same subdivision rule contracted into FP64 barycentric moments of
+   degree 2p (sum_points w lam(x)^alpha, as xg_subdiv_moments) for listed
+   mesh elements and any of NEG / POS / IF (dts, host array, ndt <= 3):
+   moments: nel x ndt x NM FP64, element-major.  Arguments, order >= 2p and
+   stats as xg_subdiv_pipeline.  POS keeps the zero-valued side as the
+   reference's leaves do (xintegration.cpp:1159-1164). */
+int xg_subdiv_moments_mesh(xg_ctx *ctx, int p, int L, int order, int nel, const int32_t *elems,
+                           const int32_t *el2v, const double *coords, const double *lset_nodal, int ndt,
+                           const int *dts, double *moments, int32_t *stats);
+/* Fused subdivision rule + element matrices on a tetrahedral mesh with a P2
+   level set: dCut(lset_p2, dt, subdivlvl=L) forms assembled element by
+   element (CutIntegrationRule, xintegration.cpp:1626-1754, with
+   NumericalIntegrationStrategy :617-1063 and CutSimplex<3>::MakeQuad
+   :1084-1388; CalcElementMatrix, symboliccutbfi.cpp:76-304).  For each listed
+   element elems[i] (required) and each dts[t] (NEG, POS or IF; host array,
+   ndt <= 3) the level-L rule is contracted into FP64 degree-2p moments and
+   the degree-p mass (mask 1) / stiffness (mask 2) matrix formed from them.
+   lset_nodal: ne x 10 FP64 nodal values indexed by element id (Lattice<3,2>
+   dof order); el2v / coords: the mesh (interface weights and the element
+   map).  order >= 2p (the reference's intorder = 2p, symboliccutbfi.cpp:
+   107-111: the base rules are then exact for the moments, uncut tree nodes
+   come from a table).  out: nel x ndt x ndof x ndof, FP32 or FP64 by
+   precision.  stats (nullable): int32 nel x ndt x 4 -- table rows, volume
+   tets and interface triangles integrated point by point, cut leaves.
+   Uses the context workspace (moments of up to 2^21 elements at a time). */
+int xg_subdiv_pipeline(xg_ctx *ctx, int p, int L, int order, int nel, const int32_t *elems,
+                       const int32_t *el2v, const double *coords, const double *lset_nodal, int ndt,
+                       const int *dts, int mask, int precision, void *out, int32_t *stats);
+
 /* Integral of a polynomial (as xg_integrate) on fixed-node rules: weights
    (nel x 35 FP64, from xg_subdiv_weights with q = 4) at the nodes of the
    degree-4 Lagrange lattice of each listed tetrahedron, mapped to the

@@ -60,6 +60,8 @@ _SIGS = {
     "xg_cut_pipeline": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _i, _vp],
     "xg_subdiv_weights": [_vp, _i, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp],
     "xg_subdiv_moments": [_vp, _i, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp],
+    "xg_subdiv_pipeline": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _i, _vp, _vp],
+    "xg_subdiv_moments_mesh": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp],
     "xg_elmat_moments": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "xg_cut_elvec": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp],
     "xg_assemble_csr": [_vp, _i, _i, _vp, _vp, _i, _vp, _vp, _vp],
@@ -310,6 +312,23 @@ class Context:
         nel = lset_nodal.shape[0]
         check(lib().xg_subdiv_moments(self.h, nel, ptr(lset_nodal), lset_order, L, order, p, ptr(finv),
                                       ptr(out_neg), ptr(out_if), ptr(dom)))
+
+    def subdiv_pipeline(self, p, L, order, elems, el2v, coords, lset_nodal, dts, mask, precision, out,
+                        stats=None):
+        """level-L subdivision rule + degree-p element matrices of the listed
+        elements (xg_subdiv_pipeline); out: nel x ndt x ndof^2"""
+        arr = (ctypes.c_int * len(dts))(*dts)
+        check(lib().xg_subdiv_pipeline(self.h, p, L, order, elems.shape[0], ptr(elems), ptr(el2v), ptr(coords),
+                                       ptr(lset_nodal), len(dts), ctypes.cast(arr, _vp), mask, precision, ptr(out),
+                                       ptr(stats)))
+
+    def subdiv_moments_mesh(self, p, L, order, elems, el2v, coords, lset_nodal, dts, moments, stats=None):
+        """degree-2p moments of the level-L subdivision rule of the listed
+        elements (xg_subdiv_moments_mesh); moments: nel x ndt x NM FP64"""
+        arr = (ctypes.c_int * len(dts))(*dts)
+        check(lib().xg_subdiv_moments_mesh(self.h, p, L, order, elems.shape[0], ptr(elems), ptr(el2v), ptr(coords),
+                                           ptr(lset_nodal), len(dts), ctypes.cast(arr, _vp), ptr(moments),
+                                           ptr(stats)))
 
     def cut_elvec(self, D, p, order, el2v, coords, lset, dts, precision, out, elems=None):
         nel = elems.shape[0] if elems is not None else el2v.shape[0]

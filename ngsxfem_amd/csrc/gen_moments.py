@@ -213,11 +213,12 @@ def gen(D, P):
             cnt += 1
             L.append("    const double %s = %s * l[%d];" % (nm, have[b], k))
             have[a] = nm
-    for a in monP:
-        L.append("    const double w%d = w * %s;" % (idxP[a], have[a]))
+    # split alpha = beta + gamma, |beta| = |gamma| = P: take beta greedily
+    # from the front; the updates are grouped by beta and each weighted
+    # monomial w q_beta is formed right before its group, so only one of them
+    # is live at a time (register pressure of the degree-6 moments)
+    groups = {}
     for a in mon2:
-        # split alpha = beta + gamma, |beta| = |gamma| = P: take beta
-        # greedily from the front
         beta = []
         left = P
         for i in range(nv):
@@ -226,7 +227,15 @@ def gen(D, P):
             left -= t
         beta = tuple(beta)
         gamma = tuple(x - y for x, y in zip(a, beta))
-        L.append("    m[%d] = fma(w%d, %s, m[%d]);" % (idx2[a], idxP[beta], have[gamma], idx2[a]))
+        groups.setdefault(beta, []).append((a, gamma))
+    for b in monP:
+        if b not in groups:
+            continue
+        L.append("    {")
+        L.append("      const double wb = w * %s;" % have[b])
+        for a, gamma in groups[b]:
+            L.append("      m[%d] = fma(wb, %s, m[%d]);" % (idx2[a], have[gamma], idx2[a]))
+        L.append("    }")
     L.append("  }")
     # ---- complement: m <- (moments of the whole reference simplex) - m.
     # int_T lam^alpha = |T| D! alpha! / (|alpha| + D)! with |T| = 1/D! (the

@@ -86,6 +86,9 @@ extern "C" int xg_ctx_destroy(xg_ctx *ctx) {
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   else hipDeviceSynchronize();
   if (ctx->ws) hipFree(ctx->ws);
+  if (ctx->sdx_verts) hipFree(ctx->sdx_verts);
+  for (int k = 0; k < 4; ++k)
+    if (ctx->sdx_mom[k]) hipFree(ctx->sdx_mom[k]);
   if (ctx->d_err) hipFree(ctx->d_err);
   if (ctx->h_err) hipHostFree(ctx->h_err);
   delete ctx;

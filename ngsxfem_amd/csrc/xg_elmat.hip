@@ -30,7 +30,8 @@ namespace {
 
 using namespace xg::ecore;
 
-template <int D, int P, typename T>
+// SPLIT = 1: split-BF16 products (T = float, elmat_one_bf16x3)
+template <int D, int P, typename T, int SPLIT = 0>
 __global__ __launch_bounds__(128) void k_pipeline(int nel, const int32_t *__restrict__ elems,
                                                   const int32_t *__restrict__ el2v,
                                                   const double *__restrict__ coords,
@@ -72,7 +73,8 @@ __global__ __launch_bounds__(128) void k_pipeline(int nel, const int32_t *__rest
       const int npts = build_stage<D, T>(&s_stage[wv], l, Finv, dt, nq_vol, nq_if, nq, nv1, bad);
       if (bad && lane == 0) atomicOr(err, 1);
       StagedSrc<D, T> src{&s_stage[wv], s_rule[dt == IF ? 1 : 0], nq, nv1};
-      elmat_one<D, P, T>(src, npts, Finv, absdet, mask, sB, sW, out + ((size_t)i * ndt + t) * NN);
+      if constexpr (SPLIT) elmat_one_bf16x3<D, P>(src, npts, Finv, absdet, mask, sB, sW, out + ((size_t)i * ndt + t) * NN);
+      else elmat_one<D, P, T>(src, npts, Finv, absdet, mask, sB, sW, out + ((size_t)i * ndt + t) * NN);
     }
   }
 }
@@ -90,7 +92,7 @@ struct CsrSrc {
   }
 };
 
-template <int D, int P, typename T>
+template <int D, int P, typename T, int SPLIT = 0>
 __global__ __launch_bounds__(128) void k_elmat_csr(int nel, const int32_t *__restrict__ elems,
                                                    const int32_t *__restrict__ el2v,
                                                    const double *__restrict__ coords,
@@ -115,7 +117,8 @@ __global__ __launch_bounds__(128) void k_elmat_csr(int nel, const int32_t *__res
     double F[D * D], Finv[D * D], absdet;
     jacobian<D>(V, F, Finv, absdet);
     CsrSrc<T> src{pts, offsets[i]};
-    elmat_one<D, P, T>(src, offsets[i + 1] - offsets[i], Finv, absdet, mask, sB, sW, out + (size_t)i * NN);
+    if constexpr (SPLIT) elmat_one_bf16x3<D, P>(src, offsets[i + 1] - offsets[i], Finv, absdet, mask, sB, sW, out + (size_t)i * NN);
+    else elmat_one<D, P, T>(src, offsets[i + 1] - offsets[i], Finv, absdet, mask, sB, sW, out + (size_t)i * NN);
   }
 }
 
@@ -133,7 +136,7 @@ struct WeightSrc {
   }
 };
 
-template <int P, int Q, typename T>
+template <int P, int Q, typename T, int SPLIT = 0>
 __global__ __launch_bounds__(128) void k_elmat_weights(int nel, const int32_t *__restrict__ elems,
                                                        const int32_t *__restrict__ el2v,
                                                        const double *__restrict__ coords,
@@ -160,7 +163,8 @@ __global__ __launch_bounds__(128) void k_elmat_weights(int nel, const int32_t *_
       jacobian<3>(V, F, Finv, absdet);
     }
     WeightSrc<Q, T> src{weights + (size_t)i * NQ};
-    elmat_one<3, P, T>(src, NQ, Finv, absdet, mask, sB, sW, out + (size_t)i * NN);
+    if constexpr (SPLIT) elmat_one_bf16x3<3, P>(src, NQ, Finv, absdet, mask, sB, sW, out + (size_t)i * NN);
+    else elmat_one<3, P, T>(src, NQ, Finv, absdet, mask, sB, sW, out + (size_t)i * NN);
   }
 }
 
@@ -171,23 +175,23 @@ int grid_waves(int nel) {
   return (int)blocks;
 }
 
-template <int D, int P, typename T>
+template <int D, int P, typename T, int SPLIT = 0>
 int launch_pipeline(xg_ctx *ctx, int order, int nel, const int32_t *elems, const int32_t *el2v,
                     const double *coords, const double *lset, int ndt, int dts, int mask, void *out) {
   using C = Cfg<D, P, T>;
   const size_t lds = C::LDS_PER_WAVE * C::WAVES;
-  k_pipeline<D, P, T><<<grid_waves(nel), 64 * C::WAVES, lds, ctx->stream>>>(
+  k_pipeline<D, P, T, SPLIT><<<grid_waves(nel), 64 * C::WAVES, lds, ctx->stream>>>(
       nel, elems, el2v, coords, lset, order, ndt, dts, mask, (T *)out, ctx->d_err);
   XG_LAUNCH_CHECK();
   return XG_OK;
 }
 
-template <int D, int P, typename T>
+template <int D, int P, typename T, int SPLIT = 0>
 int launch_csr(xg_ctx *ctx, int nel, const int32_t *elems, const int32_t *el2v, const double *coords,
                const int32_t *offsets, const double *pts, int mask, void *out) {
   using C = Cfg<D, P, T>;
   const size_t lds = C::LDS_PER_WAVE * C::WAVES;
-  k_elmat_csr<D, P, T><<<grid_waves(nel), 64 * C::WAVES, lds, ctx->stream>>>(
+  k_elmat_csr<D, P, T, SPLIT><<<grid_waves(nel), 64 * C::WAVES, lds, ctx->stream>>>(
       nel, elems, el2v, coords, offsets, reinterpret_cast<const double4 *>(pts), mask, (T *)out);
   XG_LAUNCH_CHECK();
   return XG_OK;
@@ -235,15 +239,18 @@ extern "C" int xg_cut_pipeline(xg_ctx *ctx, int D, int p, int order, int nel, co
   if (precision == XG_PREC_FP32) XG_DISPATCH(D, p, float, CALL_P);
   if (precision == XG_PREC_FP64) XG_DISPATCH(D, p, double, CALL_P);
 #undef CALL_P
+#define CALL_S(DD, PP, TT) launch_pipeline<DD, PP, TT, 1>(ctx, order, nel, elems, el2v, coords, lset_v, ndt, packed, mask, out)
+  if (precision == XG_PREC_BF16X3) XG_DISPATCH(D, p, float, CALL_S);
+#undef CALL_S
   return set_error(XG_ERR_UNSUPPORTED, "xg_cut_pipeline: unsupported precision");
 }
 
-template <int P, int Q, typename T>
+template <int P, int Q, typename T, int SPLIT = 0>
 int launch_weights(xg_ctx *ctx, int nel, const int32_t *elems, const int32_t *el2v, const double *coords,
                    const double *weights, int mask, void *out) {
   using C = Cfg<3, P, T>;
   const size_t lds = C::LDS_PER_WAVE * C::WAVES;
-  k_elmat_weights<P, Q, T><<<grid_waves(nel), 64 * C::WAVES, lds, ctx->stream>>>(nel, elems, el2v, coords, weights,
+  k_elmat_weights<P, Q, T, SPLIT><<<grid_waves(nel), 64 * C::WAVES, lds, ctx->stream>>>(nel, elems, el2v, coords, weights,
                                                                                 mask, (T *)out);
   XG_LAUNCH_CHECK();
   return XG_OK;
@@ -261,6 +268,9 @@ extern "C" int xg_elmat_weights(xg_ctx *ctx, int p, int q, int nel, const int32_
   if (precision == XG_PREC_FP64)
     return p == 1 ? launch_weights<1, 4, double>(ctx, nel, elems, el2v, coords, weights, mask, out)
                   : launch_weights<2, 4, double>(ctx, nel, elems, el2v, coords, weights, mask, out);
+  if (precision == XG_PREC_BF16X3)
+    return p == 1 ? launch_weights<1, 4, float, 1>(ctx, nel, elems, el2v, coords, weights, mask, out)
+                  : launch_weights<2, 4, float, 1>(ctx, nel, elems, el2v, coords, weights, mask, out);
   return set_error(XG_ERR_UNSUPPORTED, "xg_elmat_weights: unsupported precision");
 }
 
@@ -275,5 +285,8 @@ extern "C" int xg_elmat(xg_ctx *ctx, int D, int p, int nel, const int32_t *elems
   if (precision == XG_PREC_FP32) XG_DISPATCH(D, p, float, CALL_C);
   if (precision == XG_PREC_FP64) XG_DISPATCH(D, p, double, CALL_C);
 #undef CALL_C
+#define CALL_CS(DD, PP, TT) launch_csr<DD, PP, TT, 1>(ctx, nel, elems, el2v, coords, offsets, pts, mask, out)
+  if (precision == XG_PREC_BF16X3) XG_DISPATCH(D, p, float, CALL_CS);
+#undef CALL_CS
   return set_error(XG_ERR_UNSUPPORTED, "xg_elmat: unsupported precision");
 }

@@ -173,10 +173,15 @@ __device__ __forceinline__ void elmat_one(const SRC &src, int npts, const double
   const int lane = threadIdx.x & 63;
   const int c = lane >> 4, ii = lane & 15;
   acc_t acc[NB][NB];
+  double dacc[NB][NB][sizeof(T) == 4 ? 4 : 1];
 #pragma unroll
   for (int a = 0; a < NB; ++a)
 #pragma unroll
-    for (int b = 0; b < NB; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+    for (int b = 0; b < NB; ++b) {
+      acc[a][b] = acc_t{0, 0, 0, 0};
+#pragma unroll
+      for (int v = 0; v < (sizeof(T) == 4 ? 4 : 1); ++v) dacc[a][b][v] = 0.0;
+    }
   T FinvT[D * D];
 #pragma unroll
   for (int k = 0; k < D * D; ++k) FinvT[k] = (T)Finv[k];
@@ -244,6 +249,19 @@ __device__ __forceinline__ void elmat_one(const SRC &src, int npts, const double
         for (int bi = 0; bi < NB; ++bi)
 #pragma unroll
           for (int bj = bi; bj < NB; ++bj) acc[bi][bj] = Mfma<T>::run(av[u][bi], av[u][bj] * wv[u], acc[bi][bj]);
+      if constexpr (sizeof(T) == 4) {
+        // the FP32 sums of a group of 8 points join FP64 accumulators, so
+        // the rounding of long point sums does not grow with the rule size
+#pragma unroll
+        for (int bi = 0; bi < NB; ++bi)
+#pragma unroll
+          for (int bj = bi; bj < NB; ++bj)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              dacc[bi][bj][v] += (double)acc[bi][bj][v];
+              acc[bi][bj][v] = 0.0f;
+            }
+      }
     }
     wave_sync();
   }
@@ -256,8 +274,180 @@ __device__ __forceinline__ void elmat_one(const SRC &src, int npts, const double
       for (int v = 0; v < 4; ++v) {
         const int gi = bi * 16 + Mfma<T>::row(c, v), gj = bj * 16 + ii;
         if (gi < N && gj < N) {
-          out[gi * N + gj] = acc[bi][bj][v];
-          if (bi < bj) out[gj * N + gi] = acc[bi][bj][v];
+          const T val = (sizeof(T) == 4) ? (T)dacc[bi][bj][v] : acc[bi][bj][v];
+          out[gi * N + gj] = val;
+          if (bi < bj) out[gj * N + gi] = val;
+        }
+      }
+}
+
+// ---------------------------------------------------------------- split BF16
+// x = hi + mid + lo, each a BF16 (8 significant bits, round to nearest
+// even); the two subtractions are exact in FP32, so hi + mid + lo carries 24
+// bits of x
+typedef short shortx8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ uint32_t bf16_bits(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+__device__ __forceinline__ float bf16_value(uint32_t h) { return __uint_as_float(h << 16); }
+
+__device__ __forceinline__ void split3(const float *x, shortx8 &hi, shortx8 &mid, shortx8 &lo) {
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const uint32_t h = bf16_bits(x[u]);
+    const float r1 = x[u] - bf16_value(h);
+    const uint32_t m = bf16_bits(r1);
+    const float r2 = r1 - bf16_value(m);
+    hi[u] = (short)h;
+    mid[u] = (short)m;
+    lo[u] = (short)bf16_bits(r2);
+  }
+}
+
+__device__ __forceinline__ floatx4 mfma_bf16(const shortx8 &a, const shortx8 &b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                 0, 0, 0);
+}
+
+// elmat_one with split-BF16 products on v_mfma_f32_16x16x32_bf16: K = 32 =
+// 4 components x 8 points, so one MFMA covers a whole group of 8 points.
+// Each lane stores sqrt(|w|) * (value or gradient component) in FP32 (the
+// weight sign in sW), the contraction lane (dof, component) splits its 8
+// values into hi / mid / lo BF16 vectors and runs the six products
+// hi.hi + hi.mid + mid.hi + hi.lo + mid.mid + lo.hi (the dropped ones are
+// below 2^-26 relative) with FP32 accumulation; the B operand is the A
+// operand with the sign of the points of negative weight flipped.  Each
+// group's FP32 sums join FP64 accumulators.  Output FP32.
+template <int D, int P, typename SRC>
+__device__ __forceinline__ void elmat_one_bf16x3(const SRC &src, int npts, const double *Finv, double absdet,
+                                                 int mask, float *sB, float *sW, float *out) {
+  using C = Cfg<D, P, float>;
+  constexpr int N = C::N, NB = C::NB, CH = C::CH, PAD = C::PAD;
+  const int lane = threadIdx.x & 63;
+  const int c = lane >> 4, ii = lane & 15;
+  floatx4 acc[NB][NB];
+  double dacc[NB][NB][4];
+#pragma unroll
+  for (int a = 0; a < NB; ++a)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      acc[a][b] = floatx4{0, 0, 0, 0};
+#pragma unroll
+      for (int v = 0; v < 4; ++v) dacc[a][b][v] = 0.0;
+    }
+  float FinvT[D * D];
+#pragma unroll
+  for (int k = 0; k < D * D; ++k) FinvT[k] = (float)Finv[k];
+  const bool m_mass = mask & 1, m_lap = mask & 2;
+
+  for (int c0 = 0; c0 < npts; c0 += CH) {
+    const int nch = min(CH, npts - c0);
+    if (lane < CH) {
+      const int g = c0 + lane;
+      float phi[N], dphi[N][D];
+      float sw = 0.0f, sgn = 1.0f;
+      if (g < npts) {
+        double x[3], wr;
+        src.point(g, x, wr);
+        const double wp = wr * absdet;
+        sgn = wp < 0.0 ? -1.0f : 1.0f;
+        sw = (float)sqrt(fabs(wp));
+        float lam[D + 1];
+        double s = 1.0;
+#pragma unroll
+        for (int m = 0; m < D; ++m) {
+          lam[m] = (float)x[m];
+          s -= x[m];
+        }
+        lam[D] = (float)s;
+        h1_shape<D, P, float>(lam, phi, dphi);
+      } else {
+#pragma unroll
+        for (int n = 0; n < N; ++n) {
+          phi[n] = 0.0f;
+#pragma unroll
+          for (int d = 0; d < D; ++d) dphi[n][d] = 0.0f;
+        }
+      }
+      sW[lane] = sgn;
+#pragma unroll
+      for (int n = 0; n < N; ++n) {
+        sB[(0 * N + n) * PAD + lane] = m_mass ? sw * phi[n] : 0.0f;
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+          float gr = 0.0f;
+#pragma unroll
+          for (int cc = 0; cc < D; ++cc) gr += FinvT[cc * D + r] * dphi[n][cc];
+          sB[((1 + r) * N + n) * PAD + lane] = m_lap ? sw * gr : 0.0f;
+        }
+      }
+    }
+    wave_sync();
+    const int nch8 = min(CH, (nch + 7) & ~7);
+    for (int p0 = 0; p0 < nch8; p0 += 8) {
+      // sign flips of the group's negative-weight points (BF16 sign bit)
+      short flip[8];
+      bool anyneg = false;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const bool neg = sW[p0 + u] < 0.0f;
+        flip[u] = neg ? (short)0x8000 : (short)0;
+        anyneg |= neg;
+      }
+      shortx8 ah[NB], am[NB], al[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const int dof = b * 16 + ii;  // dofs >= N are zero rows of the MFMA tile
+        float xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xv[u] = dof < N ? sB[(c * N + dof) * PAD + p0 + u] : 0.0f;
+        split3(xv, ah[b], am[b], al[b]);
+      }
+#pragma unroll
+      for (int bi = 0; bi < NB; ++bi)
+#pragma unroll
+        for (int bj = bi; bj < NB; ++bj) {
+          shortx8 bh = ah[bj], bm = am[bj], bl = al[bj];
+          if (anyneg) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              bh[u] ^= flip[u];
+              bm[u] ^= flip[u];
+              bl[u] ^= flip[u];
+            }
+          }
+          floatx4 t = acc[bi][bj];
+          t = mfma_bf16(ah[bi], bh, t);
+          t = mfma_bf16(ah[bi], bm, t);
+          t = mfma_bf16(am[bi], bh, t);
+          t = mfma_bf16(ah[bi], bl, t);
+          t = mfma_bf16(am[bi], bm, t);
+          t = mfma_bf16(al[bi], bh, t);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            dacc[bi][bj][v] += (double)t[v];
+            t[v] = 0.0f;
+          }
+          acc[bi][bj] = t;
+        }
+    }
+    wave_sync();
+  }
+#pragma unroll
+  for (int bi = 0; bi < NB; ++bi)
+#pragma unroll
+    for (int bj = bi; bj < NB; ++bj)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int gi = bi * 16 + 4 * c + v, gj = bj * 16 + ii;
+        if (gi < N && gj < N) {
+          const float val = (float)dacc[bi][bj][v];
+          out[gi * N + gj] = val;
+          if (bi < bj) out[gj * N + gi] = val;
         }
       }
 }

@@ -17,6 +17,10 @@ struct xg_ctx {
   // the reference throws; read and cleared by xg_sync()
   int32_t *d_err = nullptr;
   int32_t *h_err = nullptr;  // pinned host mirror
+  // exact-rule subdivision tables, built on first use: vertices of every
+  // refinement-tree node of depth <= 5, and their degree-2p moments (p = 1..3)
+  void *sdx_verts = nullptr;
+  double *sdx_mom[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 namespace xg {
@@ -39,6 +43,11 @@ int mark_compact(hipStream_t st, int32_t *d_err, int D, int ne, const int32_t *e
 int launch_moment_pipeline(xg_ctx *ctx, int D, int p, int order, int nel, const int32_t *elems,
                            const int32_t *el2v, const double *coords, const double *lset, int ndt, int dts,
                            int mask, bool fp64_out, void *out);
+
+// degree-p (tetrahedra) mass / stiffness matrices from degree-2p moments
+// stored element-major with ndt sets per element (xg_moment.hip)
+int elmat_from_moments(xg_ctx *ctx, int p, int nel, int ndt, const int32_t *elems, const int32_t *el2v,
+                       const double *coords, const double *mom, int mask, bool fp64_out, void *out);
 
 }  // namespace xg
 

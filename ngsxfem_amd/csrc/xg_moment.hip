@@ -116,15 +116,18 @@ __global__ __launch_bounds__(256) void k_moment_pipeline(int nel, const int32_t 
 // element matrices from given moments (one lane per element): the rule is
 // whatever produced the moments (subdivision rule, learned rule)
 template <int D, int P, typename T>
-__global__ __launch_bounds__(256) void k_elmat_from_moments(int nel, const int32_t *__restrict__ elems,
+__global__ __launch_bounds__(256) void k_elmat_from_moments(int nel, int ndt, const int32_t *__restrict__ elems,
                                                             const int32_t *__restrict__ el2v,
                                                             const double *__restrict__ coords,
                                                             const double *__restrict__ mom, int mask,
                                                             T *__restrict__ out) {
+  // thread g = (element g / ndt, domain type g % ndt): moments and matrices
+  // are stored element-major, ndt consecutive sets per element
   using MM = Moments<D, P>;
   constexpr int N = ndof(D, P);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nel) return;
+  const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long long)nel * ndt) return;
+  const int i = (int)(g / ndt);
   double Finv[D * D], absdet = 1.0;
 #pragma unroll
   for (int k = 0; k < D * D; ++k) Finv[k] = (k % (D + 1) == 0) ? 1.0 : 0.0;
@@ -136,7 +139,7 @@ __global__ __launch_bounds__(256) void k_elmat_from_moments(int nel, const int32
   }
   double m[MM::NM];
 #pragma unroll
-  for (int k = 0; k < MM::NM; ++k) m[k] = mom[(size_t)i * MM::NM + k];
+  for (int k = 0; k < MM::NM; ++k) m[k] = mom[(size_t)g * MM::NM + k];
   double mk[MM::NK];
   MM::lower(m, mk);
   // Gram matrix of the physical barycentric gradients: grad lam_i = row i
@@ -164,14 +167,16 @@ __global__ __launch_bounds__(256) void k_elmat_from_moments(int nel, const int32
       Hs[h++] = acc;
     }
   const double sm = (mask & 1) ? absdet : 0.0, sk = (mask & 2) ? absdet : 0.0;
-  MM::template forms<T>(m, mk, Hs, sm, sk, out + (size_t)i * (N * N));
+  MM::template forms<T>(m, mk, Hs, sm, sk, out + (size_t)g * (N * N));
 }
 
 template <int D, int P, typename T>
-int launch_from_moments(xg_ctx *ctx, int nel, const int32_t *elems, const int32_t *el2v, const double *coords,
-                        const double *mom, int mask, void *out) {
-  const int blocks = (nel + 255) / 256;
-  k_elmat_from_moments<D, P, T><<<blocks, 256, 0, ctx->stream>>>(nel, elems, el2v, coords, mom, mask, (T *)out);
+int launch_from_moments(xg_ctx *ctx, int nel, int ndt, const int32_t *elems, const int32_t *el2v,
+                        const double *coords, const double *mom, int mask, void *out) {
+  const long long blocks = ((long long)nel * ndt + 255) / 256;
+  if (blocks > 0x7fffffffLL) return set_error(XG_ERR_ARG, "xg_elmat_moments: too many elements");
+  k_elmat_from_moments<D, P, T><<<(unsigned)blocks, 256, 0, ctx->stream>>>(nel, ndt, elems, el2v, coords, mom,
+                                                                           mask, (T *)out);
   XG_LAUNCH_CHECK();
   return XG_OK;
 }
@@ -206,6 +211,20 @@ int launch_moment_pipeline(xg_ctx *ctx, int D, int p, int order, int nel, const 
   return set_error(XG_ERR_ARG, "xg_cut_pipeline: bad D or p");
 }
 
+int elmat_from_moments(xg_ctx *ctx, int p, int nel, int ndt, const int32_t *elems, const int32_t *el2v,
+                       const double *coords, const double *mom, int mask, bool fp64_out, void *out) {
+  if (p == 1)
+    return fp64_out ? launch_from_moments<3, 1, double>(ctx, nel, ndt, elems, el2v, coords, mom, mask, out)
+                    : launch_from_moments<3, 1, float>(ctx, nel, ndt, elems, el2v, coords, mom, mask, out);
+  if (p == 2)
+    return fp64_out ? launch_from_moments<3, 2, double>(ctx, nel, ndt, elems, el2v, coords, mom, mask, out)
+                    : launch_from_moments<3, 2, float>(ctx, nel, ndt, elems, el2v, coords, mom, mask, out);
+  if (p == 3)
+    return fp64_out ? launch_from_moments<3, 3, double>(ctx, nel, ndt, elems, el2v, coords, mom, mask, out)
+                    : launch_from_moments<3, 3, float>(ctx, nel, ndt, elems, el2v, coords, mom, mask, out);
+  return set_error(XG_ERR_ARG, "element matrices from moments: bad p");
+}
+
 int launch_moment_elvec(xg_ctx *ctx, int D, int p, int order, int nel, const int32_t *elems, const int32_t *el2v,
                         const double *coords, const double *lset, int ndt, int dts, bool fp64_out, void *out) {
 #define XG_VCALL(DD, PP)                                                                                          \
@@ -232,8 +251,8 @@ extern "C" int xg_elmat_moments(xg_ctx *ctx, int D, int p, int nel, const int32_
   const bool f64 = precision == XG_PREC_FP64;
 #define XG_ECALL(DD, PP)                                                                          \
   if (D == DD && p == PP)                                                                         \
-    return f64 ? launch_from_moments<DD, PP, double>(ctx, nel, elems, el2v, coords, moments, mask, out) \
-               : launch_from_moments<DD, PP, float>(ctx, nel, elems, el2v, coords, moments, mask, out);
+    return f64 ? launch_from_moments<DD, PP, double>(ctx, nel, 1, elems, el2v, coords, moments, mask, out) \
+               : launch_from_moments<DD, PP, float>(ctx, nel, 1, elems, el2v, coords, moments, mask, out);
   XG_ECALL(1, 1) XG_ECALL(1, 2) XG_ECALL(1, 3)
   XG_ECALL(2, 1) XG_ECALL(2, 2) XG_ECALL(2, 3)
   XG_ECALL(3, 1) XG_ECALL(3, 2) XG_ECALL(3, 3)

@@ -763,6 +763,728 @@ int subdiv_moments_impl(xg_ctx *ctx, int nel, const double *lset_nodal, int L, i
   }
 }
 
+// ======================================================================
+// Exact-rule subdivision kernel (order >= 2 PM).
+//
+// Once the base rules integrate degree-2 PM polynomials exactly, the moments
+// of every uncut node of the refinement tree are the exact integrals of
+// lam^alpha over that node, independent of the level set: they are read from
+// a table (one row per node of depth <= 5, built once per context) instead of
+// being summed point by point.  The same table gives the complement of a cut
+// leaf whose kept side holds three of its vertices: that part (the
+// reference's prism of three tets, xintegration.cpp:1164-1238) has the moments
+// of the whole leaf minus those of the one-vertex corner tet on the other
+// side, so one tet is integrated instead of three.  Only the finest-lattice
+// signs are kept in LDS (one bit per point); the FP64 values at the vertices
+// of a cut leaf are evaluated again, with the same arithmetic, when the leaf
+// is cut.  Node vertices come from a table indexed by (depth, path).  One
+// accumulator set per lane: a launch integrates one volume side (SIDE = NEG or
+// POS) and / or the interface (WHICH bit 0 / bit 1).
+constexpr int SDX_DMAX = XG_LAT_LMAX;                                   // table depth
+constexpr int SDX_NODES = ((1 << (3 * (SDX_DMAX + 1))) - 1) / 7;        // 37449
+constexpr int SDX_CH = 128;                                             // candidates per chunk
+__host__ __device__ constexpr int sdx_lvl_off(int k) { return ((1 << (3 * k)) - 1) / 7; }
+
+// vertices of every node of depth <= 5 in level-5 lattice units (root
+// (32,0,0), (0,32,0), (0,0,32), (0,0,0)), packed x | y << 8 | z << 16
+__global__ void k_sdx_node_verts(uint4 *__restrict__ out) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= SDX_NODES) return;
+  int k = 0;
+  while (id >= sdx_lvl_off(k + 1)) ++k;
+  int W[4][3];
+  node_vertices(id - sdx_lvl_off(k), k, 1 << SDX_DMAX, W);
+  out[id] = make_uint4(pack3(W[0][0], W[0][1], W[0][2]), pack3(W[1][0], W[1][1], W[1][2]),
+                       pack3(W[2][0], W[2][1], W[2][2]), pack3(W[3][0], W[3][1], W[3][2]));
+}
+
+// exact degree-2PM moments of every node (minimal exact rule, FP64): one wave
+// per node, rows of NM values
+template <int PM>
+__global__ __launch_bounds__(64) void k_sdx_node_moments(const uint4 *__restrict__ verts, double *__restrict__ out) {
+  constexpr int NM = Moments<3, PM>::NM;
+  const int id = blockIdx.x, lane = threadIdx.x & 63;
+  const uint4 pv = verts[id];
+  const uint32_t pk[4] = {pv.x, pv.y, pv.z, pv.w};
+  double V[4][3];
+  constexpr double inv = 1.0 / (1 << SDX_DMAX);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    int w[3];
+    unpack3(pk[j], w);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) V[j][d] = w[d] * inv;
+  }
+  const double tf = tet_tf(V[0], V[1], V[2], V[3], false);
+  const int nq = xg_minrule_cnt_d[3][PM];
+  const double(*rule)[4] = &xg_minrule_data_d[xg_minrule_off_d[3][PM]];
+  double m[NM];
+#pragma unroll
+  for (int a = 0; a < NM; ++a) m[a] = 0.0;
+  for (int q = lane; q < nq; q += 64) accum_tet<PM>(m, V[0], V[1], V[2], V[3], tf, rule, q);
+#pragma unroll
+  for (int a = 0; a < NM; ++a) {
+    double s = m[a];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) out[(size_t)id * NM + a] = s;
+  }
+}
+
+struct SdxLds {
+  uint32_t *latbits;                     // sign bits (v >= 0) of the finest lattice
+  double *red;                           // 2 NM
+  double (*rule_vol)[4], (*rule_if)[4];  // minimal exact rules of degree 2 PM
+  double (*pool)[8][3];                  // 64 cut leaves x (4 vertices, 4 cut points)
+  double *voltf, *ifw;                   // 192 / 128 piece weight factors (signed)
+  uint32_t *volrec, *ifrec;              // 192 / 128 piece records
+  uint4 *nodeV;                          // SDX_CH candidate vertices (level-L units)
+  uint32_t *flags;                       // SDX_CH
+  uint16_t *cur, *nxt;                   // sd_maxcut(L) paths
+  uint16_t *fillq;                       // 2 SDX_CH table rows
+  uint16_t *leafq;                       // SDX_CH
+};
+
+__host__ __device__ inline size_t sdx_bytes(int L, int nq_vol, int nq_if, int NM, SdxLds *s, char *base) {
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = o;
+    o = sd_align(o + bytes);
+    return at;
+  };
+  const size_t a_lb = take(sizeof(uint32_t) * 2 * ((sd_nlat(L) + 63) / 64));
+  const size_t a_red = take(sizeof(double) * 2 * NM);
+  const size_t a_rv = take(sizeof(double) * 4 * nq_vol);
+  const size_t a_ri = take(sizeof(double) * 4 * nq_if);
+  const size_t a_cp = take(sizeof(double) * 64 * 8 * 3);
+  const size_t a_vtf = take(sizeof(double) * 192);
+  const size_t a_ifw = take(sizeof(double) * 128);
+  const size_t a_vrec = take(sizeof(uint32_t) * 192);
+  const size_t a_irec = take(sizeof(uint32_t) * 128);
+  const size_t a_nv = take(sizeof(uint4) * SDX_CH);
+  const size_t a_fl = take(sizeof(uint32_t) * SDX_CH);
+  const size_t a_cur = take(sizeof(uint16_t) * sd_maxcut(L));
+  const size_t a_nxt = take(sizeof(uint16_t) * sd_maxcut(L));
+  const size_t a_fq = take(sizeof(uint16_t) * 2 * SDX_CH);
+  const size_t a_lq = take(sizeof(uint16_t) * SDX_CH);
+  if (s) {
+    s->latbits = (uint32_t *)(base + a_lb);
+    s->red = (double *)(base + a_red);
+    s->rule_vol = (double(*)[4])(base + a_rv);
+    s->rule_if = (double(*)[4])(base + a_ri);
+    s->pool = (double(*)[8][3])(base + a_cp);
+    s->voltf = (double *)(base + a_vtf);
+    s->ifw = (double *)(base + a_ifw);
+    s->volrec = (uint32_t *)(base + a_vrec);
+    s->ifrec = (uint32_t *)(base + a_irec);
+    s->nodeV = (uint4 *)(base + a_nv);
+    s->flags = (uint32_t *)(base + a_fl);
+    s->cur = (uint16_t *)(base + a_cur);
+    s->nxt = (uint16_t *)(base + a_nxt);
+    s->fillq = (uint16_t *)(base + a_fq);
+    s->leafq = (uint16_t *)(base + a_lq);
+  }
+  return o;
+}
+
+// The pieces of a cut leaf for the kept volume side SIDE (CutSimplex<3>::
+// MakeQuad, xintegration.cpp:1084-1388), as index lists into
+// [leaf vertices 0..3, cut points 4..7]: `nvol` tets with sign `vsign`
+// (-1: the listed corner tet is subtracted from the whole leaf, whose table
+// row the caller adds, `comp`), and the interface triangles (3 points + the
+// first POS vertex, which orients the normal).  Returns false where the
+// reference throws.
+struct LeafPieces2 {
+  int nvol, nif;
+  bool comp;
+  uint32_t vol[3];
+  uint32_t itf[2];
+};
+
+template <int SIDE>
+__device__ bool leaf_pieces(const double (*V)[3], const double *vv, double (*cp)[3], LeafPieces2 &P) {
+  constexpr int edge[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+  P.nvol = 0;
+  P.nif = 0;
+  P.comp = false;
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) cp[j][d] = 0.0;
+  int posm = 0, negm = 0, zerom = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (vv[j] > 0) posm |= 1 << j;
+    else if (vv[j] < 0) negm |= 1 << j;
+    else {
+      posm |= 1 << j;  // a zero vertex value counts as POS (xintegration.cpp:1159-1164)
+      zerom |= 1 << j;
+    }
+  }
+  const int npos = __popc(posm), nneg = __popc(negm);
+  int ncut = 0;
+  uint32_t v2c1 = 0xFFFFu, v2c2 = 0xFFFFu;
+#pragma unroll
+  for (int e = 0; e < 6; ++e) {
+    const int lv = edge[e][0], rv = edge[e][1];
+    const double vl = vv[lv], vr = vv[rv];
+    bool hascut = (vl * vr < 0);
+    if (((zerom >> lv) & 1) && vr < 0) hascut = true;
+    if (((zerom >> rv) & 1) && vl < 0) hascut = true;
+    if (hascut) {
+      if (ncut >= 4) return false;
+      const double c = vl / (vl - vr);
+      double q[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) q[d] = (1 - c) * V[lv][d] + c * V[rv][d];
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        if (t == ncut)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) cp[t][d] = q[d];
+      if (nib(v2c1, lv) == 15) v2c1 = set_nib(v2c1, lv, ncut); else v2c2 = set_nib(v2c2, lv, ncut);
+      if (nib(v2c1, rv) == 15) v2c1 = set_nib(v2c1, rv, ncut); else v2c2 = set_nib(v2c2, rv, ncut);
+      ncut++;
+    }
+  }
+  if (ncut == 3) {
+    const bool minor_pos = nneg > npos;
+    const int nmin = minor_pos ? npos : nneg;
+    if (nmin != 1) return false;
+    const int minm = minor_pos ? posm : negm;
+    // the one-vertex corner tet [minor vertex, cut 0, cut 1, cut 2] is the
+    // minor side; the major side (the reference's prism) is leaf - corner
+    P.vol[0] = pack4i(kth_bit(minm, 0), 4, 5, 6);
+    P.nvol = 1;
+    P.comp = (SIDE == POS) ? !minor_pos : minor_pos;
+    P.itf[0] = pack4i(4, 5, 6, kth_bit(posm, 0));
+    P.nif = 1;
+    return true;
+  }
+  if (ncut == 4) {
+    if (npos != 2 || nneg != 2) return false;
+    // the kept side's prism (the reference's side loop, own = kept vertices,
+    // oth = the other side's first vertex for the orientation switch)
+    const int ownm = (SIDE == POS) ? posm : negm, othm = (SIDE == POS) ? negm : posm;
+    const int o0 = kth_bit(ownm, 0), o1 = kth_bit(ownm, 1), t0 = kth_bit(othm, 0);
+    const int c1 = nib(v2c1, o0), c2 = nib(v2c2, o0);
+    int c3 = -1;
+#pragma unroll
+    for (int l = 0; l < 4; ++l)
+      if (c1 != l && c2 != l) c3 = l;
+    int c4 = 6 - c3 - c2 - c1;
+    if ((nib(v2c1, t0) == c1 && nib(v2c2, t0) == c4) || (nib(v2c1, t0) == c2 && nib(v2c2, t0) == c3)) {
+      const int tt = c3;
+      c3 = c4;
+      c4 = tt;
+    }
+    const int q1 = 4 + c1, q2 = 4 + c2, q4 = 4 + c3, q5 = 4 + c4;
+    P.vol[0] = pack4i(o0, q1, q2, o1);
+    P.vol[1] = pack4i(q1, q2, o1, q4);
+    P.vol[2] = pack4i(q2, o1, q4, q5);
+    P.nvol = 3;
+    const int n0 = kth_bit(negm, 0), n1 = kth_bit(negm, 1), p0 = kth_bit(posm, 0), p1 = kth_bit(posm, 1);
+    int diag1, diag2, ndiag1, ndiag2;
+    if (nib(v2c1, n0) == nib(v2c1, p0) || nib(v2c2, n0) == nib(v2c1, p0)) {
+      diag1 = nib(v2c1, p0);
+      ndiag1 = nib(v2c2, p0);
+    } else {
+      diag1 = nib(v2c2, p0);
+      ndiag1 = nib(v2c1, p0);
+    }
+    if (nib(v2c1, n1) == nib(v2c1, p1) || nib(v2c2, n1) == nib(v2c1, p1)) {
+      diag2 = nib(v2c1, p1);
+      ndiag2 = nib(v2c2, p1);
+    } else {
+      diag2 = nib(v2c2, p1);
+      ndiag2 = nib(v2c1, p1);
+    }
+    P.itf[0] = pack4i(4 + diag1, 4 + ndiag1, 4 + diag2, p0);
+    P.itf[1] = pack4i(4 + diag2, 4 + ndiag2, 4 + diag1, p0);
+    P.nif = 2;
+    return true;
+  }
+  return false;
+}
+
+__device__ __forceinline__ void unpack_uint4(const uint4 &pv, int shift, int (*W)[3]) {
+  const uint32_t pk[4] = {pv.x, pv.y, pv.z, pv.w};
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    unpack3(pk[j], W[j]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) W[j][d] >>= shift;
+  }
+}
+
+__device__ __forceinline__ bool lat_bit(const uint32_t *bits, int idx) { return (bits[idx >> 5] >> (idx & 31)) & 1u; }
+
+// stats (nullable, int32 x 4 per element): table rows added, volume tets and
+// interface triangles integrated point by point, cut leaves
+template <int LO, int PM, int SIDE, int WHICH>
+__global__ __launch_bounds__(64) void k_subdiv_exact(int nel, const int32_t *__restrict__ elems,
+                                                     const double *__restrict__ lset_nodal,
+                                                     const int32_t *__restrict__ el2v,
+                                                     const double *__restrict__ coords,
+                                                     const double *__restrict__ finv_all, int L,
+                                                     const uint4 *__restrict__ nverts,
+                                                     const double *__restrict__ nmom, double *__restrict__ out_vol,
+                                                     double *__restrict__ out_if, int out_stride,
+                                                     uint8_t *__restrict__ dom_out, int32_t *__restrict__ stats,
+                                                     int stats_stride, int32_t *err) {
+  constexpr int NL = ndof(3, LO);
+  constexpr int NM = Moments<3, PM>::NM;
+  // minimal exact rules of degree 2 PM: tets 4 / 14 / 24 points, triangles
+  // 3 / 6 / 12, processed in blocks of SDX_VB / SDX_IB points per lane task
+  constexpr int SDX_VB = PM == 1 ? 4 : (PM == 2 ? 7 : 8), SDX_NVB = PM == 1 ? 1 : (PM == 2 ? 2 : 3);
+  constexpr int SDX_IB = PM == 1 ? 3 : (PM == 2 ? 6 : 4), SDX_NIB = PM == 1 ? 1 : (PM == 2 ? 1 : 3);
+  extern __shared__ char smem[];
+  SdxLds S;
+  const int vol_nq = xg_minrule_cnt_d[3][PM];
+  const int if_nq = xg_minrule_cnt_d[2][PM];
+  sdx_bytes(L, vol_nq, if_nq, NM, &S, smem);
+  const int lane = threadIdx.x & 63;
+  const int n = 1 << L;
+  const int nlat = sd_nlat(L);
+  const int loff = xg_lat_off_d[L];
+  const int shift = SDX_DMAX - L;
+  {
+    const double(*src_v)[4] = &xg_minrule_data_d[xg_minrule_off_d[3][PM]];
+    const double(*src_i)[4] = &xg_minrule_data_d[xg_minrule_off_d[2][PM]];
+    for (int k = lane; k < 4 * (vol_nq + if_nq); k += 64) {
+      const int q = k >> 2, c = k & 3;
+      if (q < vol_nq) S.rule_vol[q][c] = src_v[q][c];
+      else S.rule_if[q - vol_nq][c] = src_i[q - vol_nq][c];
+    }
+    wave_sync();
+  }
+  const double inv_n = 1.0 / n;
+  for (int i = blockIdx.x; i < nel; i += gridDim.x) {
+    const int e = elems ? elems[i] : i;
+    double cl[NL];
+#pragma unroll
+    for (int k = 0; k < NL; ++k) cl[k] = lset_nodal[(size_t)e * NL + k];
+    double Fi[9];
+    Geo geo;
+    geo.Finv = nullptr;
+    geo.absdet = 1.0;
+    if (WHICH & 2) {
+      if (el2v && coords) {
+        double V[4][3], F[9];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int v = el2v[(size_t)e * 4 + k];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) V[k][d] = coords[(size_t)v * 3 + d];
+        }
+        jacobian<3>(V, F, Fi, geo.absdet);
+        geo.Finv = Fi;
+      } else if (finv_all) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Fi[k] = finv_all[(size_t)i * 9 + k];
+        const double dinv = Fi[0] * (Fi[4] * Fi[8] - Fi[5] * Fi[7]) - Fi[1] * (Fi[3] * Fi[8] - Fi[5] * Fi[6]) +
+                            Fi[2] * (Fi[3] * Fi[7] - Fi[4] * Fi[6]);
+        geo.absdet = 1.0 / fabs(dinv);
+        geo.Finv = Fi;
+      }
+    }
+    // 1. signs of the level set on the finest lattice (one ballot per 64 points)
+    for (int q0 = 0; q0 < nlat; q0 += 64) {
+      const int q = q0 + lane;
+      bool nonneg = false;
+      if (q < nlat) {
+        const unsigned pk = xg_lat_d[loff + q];
+        const double x[3] = {(double)(pk & 255u) * inv_n, (double)((pk >> 8) & 255u) * inv_n,
+                             (double)(pk >> 16) * inv_n};
+        nonneg = lagrange_eval3<LO>(cl, x) >= 0.0;
+      }
+      const unsigned long long b = __ballot(nonneg);
+      if (lane == 0) {
+        S.latbits[(q0 >> 5)] = (uint32_t)b;
+        S.latbits[(q0 >> 5) + 1] = (uint32_t)(b >> 32);
+      }
+    }
+    double mvol[(WHICH & 1) ? NM : 1], mif[(WHICH & 2) ? NM : 1];
+#pragma unroll
+    for (int k = 0; k < ((WHICH & 1) ? NM : 1); ++k) mvol[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < ((WHICH & 2) ? NM : 1); ++k) mif[k] = 0.0;
+    // table rows summed lane-parallel: lane holds moments lane and lane + 64
+    double tacc0 = 0.0, tacc1 = 0.0;
+    int st_rows = 0, st_vol = 0, st_if = 0, st_leaf = 0;
+    bool ok = true;
+    int dom = -1;
+    int ncur = 1;
+    wave_sync();
+    for (int k = 0; k <= L && ncur > 0; ++k) {
+      const int r = L - k;
+      const int ncand = (k == 0) ? 1 : 8 * ncur;
+      const int np = xg_lat_cnt_d[r], poff = xg_lat_off_d[r];
+      const int koff = sdx_lvl_off(k);
+      int nnxt = 0;
+      for (int cbase = 0; cbase < ncand; cbase += SDX_CH) {
+        const int nc = min(SDX_CH, ncand - cbase);
+        for (int c = lane; c < nc; c += 64) {
+          const int gc = cbase + c;
+          const int path = (k == 0) ? 0 : (int)S.cur[gc >> 3] * 8 + (gc & 7);
+          S.nodeV[c] = nverts[koff + path];
+          S.flags[c] = 0u;
+        }
+        wave_sync();
+        // lattice checks above the leaves: sign bits of every (candidate,
+        // lattice point) pair (a leaf's lattice is its 4 vertices, read below)
+        if (r > 0) {
+          for (int q = lane; q < nc * np; q += 64) {
+            const int c = q / np, t = q - c * np;
+            const unsigned pk = xg_lat_d[poff + t];
+            const int I0 = (int)(pk & 255u), I1 = (int)((pk >> 8) & 255u), I2 = (int)(pk >> 16);
+            int W[4][3];
+            unpack_uint4(S.nodeV[c], shift, W);
+            int P[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+              P[d] = W[0][d] + ((I0 * (W[1][d] - W[0][d]) + I1 * (W[2][d] - W[0][d]) + I2 * (W[3][d] - W[0][d])) >> r);
+            atomicOr(&S.flags[c], lat_bit(S.latbits, lat_index(P[0], P[1], P[2], n)) ? 1u : 2u);
+          }
+          wave_sync();
+        }
+        // classification and compaction
+        int nfill = 0, nleaf = 0;
+        for (int c0 = 0; c0 < nc; c0 += 64) {
+          const int c = c0 + lane;
+          const bool valid = c < nc;
+          int W[4][3] = {};
+          if (valid) unpack_uint4(S.nodeV[c], shift, W);
+          uint32_t f = 0u;
+          if (valid) {
+            if (r > 0) f = S.flags[c];
+            else
+#pragma unroll
+              for (int j = 0; j < 4; ++j) f |= lat_bit(S.latbits, lat_index(W[j][0], W[j][1], W[j][2], n)) ? 1u : 2u;
+          }
+          const int cls = (f == 3u) ? IF : ((f == 1u) ? POS : NEG);
+          if (k == 0) dom = __shfl(cls, 0, 64);
+          int lc = -2;
+          if (valid && cls == IF && r == 0) {
+            double vv[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const double x[3] = {(double)W[j][0] * inv_n, (double)W[j][1] * inv_n, (double)W[j][2] * inv_n};
+              vv[j] = lagrange_eval3<LO>(cl, x);
+            }
+            lc = leaf_check(vv);
+            if (lc < 0) ok = false;
+          }
+          const int path = (k == 0 || !valid) ? 0 : (int)S.cur[(cbase + c) >> 3] * 8 + ((cbase + c) & 7);
+          const bool wantfill = (WHICH & 1) && valid && (cls == SIDE || lc == SIDE);
+          const bool leafcut = valid && lc == IF;
+          const bool wantnext = valid && cls == IF && r > 0;
+          int cnt;
+          const int sf = wave_slot(wantfill, nfill, lane, cnt);
+          if (wantfill) S.fillq[sf] = (uint16_t)(koff + path);
+          nfill += cnt;
+          const int sl = wave_slot(leafcut, nleaf, lane, cnt);
+          if (leafcut) S.leafq[sl] = (uint16_t)c;
+          nleaf += cnt;
+          const int sn = wave_slot(wantnext, nnxt, lane, cnt);
+          if (wantnext) S.nxt[sn] = (uint16_t)path;
+          nnxt += cnt;
+        }
+        wave_sync();
+        st_leaf += nleaf;
+        // cut leaves, 64 at a time: pieces one lane per leaf (its 4 vertices
+        // and up to 4 cut points into the LDS pool), then the pieces' base-
+        // rule points lane-parallel over (piece, block of points) tasks
+        for (int l0 = 0; l0 < nleaf; l0 += 64) {
+          const int l = l0 + lane;
+          LeafPieces2 PC;
+          PC.nvol = 0;
+          PC.nif = 0;
+          PC.comp = false;
+          if (l < nleaf) {
+            const int c = S.leafq[l];
+            int W[4][3];
+            unpack_uint4(S.nodeV[c], shift, W);
+            double V[4][3], vv[4], cpt[4][3];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+#pragma unroll
+              for (int d = 0; d < 3; ++d) V[j][d] = W[j][d] * inv_n;
+              vv[j] = lagrange_eval3<LO>(cl, V[j]);
+            }
+            if (!leaf_pieces<SIDE>(V, vv, cpt, PC)) {
+              ok = false;
+              PC.nvol = 0;
+              PC.nif = 0;
+              PC.comp = false;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+              for (int d = 0; d < 3; ++d) {
+                S.pool[lane][j][d] = V[j][d];
+                S.pool[lane][4 + j][d] = cpt[j][d];
+              }
+          }
+          if (!(WHICH & 1)) {
+            PC.nvol = 0;
+            PC.comp = false;
+          }
+          if (!(WHICH & 2)) PC.nif = 0;
+          // complement: the whole leaf's table row joins this chunk's rows
+          {
+            int cnt;
+            const int sf = wave_slot(PC.comp, nfill, lane, cnt);
+            if (PC.comp) {
+              const int gc = cbase + S.leafq[l];
+              S.fillq[sf] = (uint16_t)(koff + ((k == 0) ? 0 : (int)S.cur[gc >> 3] * 8 + (gc & 7)));
+            }
+            nfill += cnt;
+          }
+          int tot_vol, tot_if;
+          const int bvol = wave_excl_scan(PC.nvol, lane, tot_vol);
+          const int bif = wave_excl_scan(PC.nif, lane, tot_if);
+#pragma unroll
+          for (int q = 0; q < 3; ++q)
+            if (q < PC.nvol) S.volrec[bvol + q] = (uint32_t)lane | (PC.vol[q] << 8) | (PC.comp ? 0x80u : 0u);
+#pragma unroll
+          for (int q = 0; q < 2; ++q)
+            if (q < PC.nif) S.ifrec[bif + q] = (uint32_t)lane | (PC.itf[q] << 8);
+          st_vol += tot_vol;
+          st_if += tot_if;
+          wave_sync();
+          // weight factors: FillSimplexWithRule (|det|/6)*6, negative for a
+          // subtracted corner; FillSimplexCoDim1WithRule |cross| and the
+          // interface transform
+          for (int q = lane; q < tot_vol; q += 64) {
+            const uint32_t rec = S.volrec[q];
+            const double(*pl)[3] = S.pool[rec & 63];
+            const double tf = tet_tf(pl[(rec >> 8) & 7], pl[(rec >> 11) & 7], pl[(rec >> 14) & 7],
+                                     pl[(rec >> 17) & 7], true);
+            S.voltf[q] = (rec & 0x80u) ? -tf : tf;
+          }
+          for (int q = lane; q < tot_if; q += 64) {
+            const uint32_t rec = S.ifrec[q];
+            const double(*pl)[3] = S.pool[rec & 63];
+            const double *s0 = pl[(rec >> 8) & 7], *s1 = pl[(rec >> 11) & 7], *s2 = pl[(rec >> 14) & 7];
+            const double *pp = pl[(rec >> 17) & 7];
+            double a[3], b[3], cc[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              a[d] = s1[d] - s0[d];
+              b[d] = s2[d] - s0[d];
+            }
+            cc[0] = a[1] * b[2] - a[2] * b[1];
+            cc[1] = a[2] * b[0] - a[0] * b[2];
+            cc[2] = a[0] * b[1] - a[1] * b[0];
+            const double tf = norm3(cc);
+            const double na = norm3(a), nb = norm3(b);
+            const double rel = na > nb ? na : nb;
+            double wf = 0.0;
+            if (!(tf < 1e-14 * rel)) {
+#pragma unroll
+              for (int d = 0; d < 3; ++d) cc[d] /= tf;
+              const double dd0 = pp[0] - s0[0], dd1 = pp[1] - s0[1], dd2 = pp[2] - s0[2];
+              if (dd0 * cc[0] + dd1 * cc[1] + dd2 * cc[2] >= 0)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) cc[d] *= -1.0;
+              wf = tf * geo.if_factor(cc);
+            }
+            S.ifw[q] = wf;
+          }
+          wave_sync();
+          // tasks (piece, block of SDX_VB points): the piece's vertices are
+          // read once per block
+          if (WHICH & 1) {
+            for (int q = lane; q < tot_vol * SDX_NVB; q += 64) {
+              const int pc = q / SDX_NVB, t0 = (q - pc * SDX_NVB) * SDX_VB;
+              const uint32_t rec = S.volrec[pc];
+              const double(*pl)[3] = S.pool[rec & 63];
+              double p0[3], p1[3], p2[3], p3[3];
+#pragma unroll
+              for (int d = 0; d < 3; ++d) {
+                p0[d] = pl[(rec >> 8) & 7][d];
+                p1[d] = pl[(rec >> 11) & 7][d];
+                p2[d] = pl[(rec >> 14) & 7][d];
+                p3[d] = pl[(rec >> 17) & 7][d];
+              }
+              const double tf = S.voltf[pc];
+              for (int t = t0; t < t0 + SDX_VB; ++t) accum_tet<PM>(mvol, p0, p1, p2, p3, tf, S.rule_vol, t);
+            }
+          }
+          if (WHICH & 2) {
+            for (int q = lane; q < tot_if * SDX_NIB; q += 64) {
+              const int pc = q / SDX_NIB, t0 = (q - pc * SDX_NIB) * SDX_IB;
+              const uint32_t rec = S.ifrec[pc];
+              const double(*pl)[3] = S.pool[rec & 63];
+              double s0[3], s1[3], s2[3];
+#pragma unroll
+              for (int d = 0; d < 3; ++d) {
+                s0[d] = pl[(rec >> 8) & 7][d];
+                s1[d] = pl[(rec >> 11) & 7][d];
+                s2[d] = pl[(rec >> 14) & 7][d];
+              }
+              const double wf = S.ifw[pc];
+              for (int t = t0; t < t0 + SDX_IB; ++t) {
+                const double *rr = S.rule_if[t];
+                const double ow = 1.0 - rr[0] - rr[1];
+                double lam[4];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) lam[d] = ow * s0[d] + rr[0] * s1[d] + rr[1] * s2[d];
+                lam[3] = 1.0 - lam[0] - lam[1] - lam[2];
+                Moments<3, PM>::accum(lam, rr[3] * wf, mif);
+              }
+            }
+          }
+          wave_sync();
+        }
+        // table rows of the chunk: uncut nodes of the kept side and the
+        // complemented leaves
+        if (WHICH & 1) {
+          st_rows += nfill;
+          int f = 0;
+          for (; f + 4 <= nfill; f += 4) {
+            const int r0 = S.fillq[f], r1 = S.fillq[f + 1], r2 = S.fillq[f + 2], r3 = S.fillq[f + 3];
+            double a0 = 0, a1 = 0, a2 = 0, a3 = 0, b0 = 0, b1 = 0, b2 = 0, b3 = 0;
+            if (lane < NM) {
+              a0 = nmom[(size_t)r0 * NM + lane];
+              a1 = nmom[(size_t)r1 * NM + lane];
+              a2 = nmom[(size_t)r2 * NM + lane];
+              a3 = nmom[(size_t)r3 * NM + lane];
+            }
+            if (lane + 64 < NM) {
+              b0 = nmom[(size_t)r0 * NM + 64 + lane];
+              b1 = nmom[(size_t)r1 * NM + 64 + lane];
+              b2 = nmom[(size_t)r2 * NM + 64 + lane];
+              b3 = nmom[(size_t)r3 * NM + 64 + lane];
+            }
+            tacc0 += (a0 + a1) + (a2 + a3);
+            tacc1 += (b0 + b1) + (b2 + b3);
+          }
+          for (; f < nfill; ++f) {
+            const int r0 = S.fillq[f];
+            if (lane < NM) tacc0 += nmom[(size_t)r0 * NM + lane];
+            if (lane + 64 < NM) tacc1 += nmom[(size_t)r0 * NM + 64 + lane];
+          }
+        }
+        wave_sync();
+      }
+      uint16_t *tmp = S.cur;
+      S.cur = S.nxt;
+      S.nxt = tmp;
+      ncur = nnxt;
+    }
+    // wave reduction of the per-lane moments, plus the table sums
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+      double a = (WHICH & 1) ? mvol[(WHICH & 1) ? k : 0] : 0.0, b = (WHICH & 2) ? mif[(WHICH & 2) ? k : 0] : 0.0;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        if (WHICH & 1) a += __shfl_xor(a, o, 64);
+        if (WHICH & 2) b += __shfl_xor(b, o, 64);
+      }
+      if (lane == 0) {
+        S.red[k] = a;
+        S.red[NM + k] = b;
+      }
+    }
+    wave_sync();
+    if (WHICH & 1) {
+      if (lane < NM) S.red[lane] += tacc0;
+      if (lane + 64 < NM) S.red[64 + lane] += tacc1;
+    }
+    wave_sync();
+    for (int k = lane; k < NM; k += 64) {
+      if ((WHICH & 1) && out_vol) out_vol[(size_t)i * out_stride + k] = S.red[k];
+      if ((WHICH & 2) && out_if) out_if[(size_t)i * out_stride + k] = S.red[NM + k];
+    }
+    const bool allok = __ballot(!ok) == 0ull;
+    if (lane == 0) {
+      if (dom_out) dom_out[i] = (uint8_t)dom;
+      if (stats) {
+        stats[(size_t)i * stats_stride + 0] = st_rows;
+        stats[(size_t)i * stats_stride + 1] = st_vol;
+        stats[(size_t)i * stats_stride + 2] = st_if;
+        stats[(size_t)i * stats_stride + 3] = st_leaf;
+      }
+      if (!allok) atomicOr(err, 2);
+    }
+    wave_sync();
+  }
+}
+
+template <int LO, int PM, int SIDE, int WHICH>
+int launch_subdiv_exact(xg_ctx *ctx, int nel, const int32_t *elems, const double *lset_nodal, const int32_t *el2v,
+                        const double *coords, const double *finv, int L, double *out_vol, double *out_if,
+                        int out_stride, uint8_t *dom, int32_t *stats, int stats_stride) {
+  constexpr int NM = Moments<3, PM>::NM;
+  const size_t lds = sdx_bytes(L, xg_minrule_cnt[3][PM], xg_minrule_cnt[2][PM], NM, nullptr, nullptr);
+  if (lds > 65536)
+    XG_CHECK(hipFuncSetAttribute((const void *)k_subdiv_exact<LO, PM, SIDE, WHICH>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const int blocks = nel < (1 << 22) ? nel : (1 << 22);
+  k_subdiv_exact<LO, PM, SIDE, WHICH><<<blocks, 64, lds, ctx->stream>>>(
+      nel, elems, lset_nodal, el2v, coords, finv, L, (const uint4 *)ctx->sdx_verts, ctx->sdx_mom[PM], out_vol,
+      out_if, out_stride, dom, stats, stats_stride, ctx->d_err);
+  XG_LAUNCH_CHECK();
+  return XG_OK;
+}
+
+// node tables of the exact-rule kernel (once per context and degree)
+int ensure_sdx_tables(xg_ctx *ctx, int PM) {
+  if (!ctx->sdx_verts) {
+    XG_CHECK(hipMalloc(&ctx->sdx_verts, sizeof(uint4) * SDX_NODES));
+    k_sdx_node_verts<<<(SDX_NODES + 255) / 256, 256, 0, ctx->stream>>>((uint4 *)ctx->sdx_verts);
+    XG_LAUNCH_CHECK();
+  }
+  if (!ctx->sdx_mom[PM]) {
+    const int NM = PM == 1 ? Moments<3, 1>::NM : (PM == 2 ? Moments<3, 2>::NM : Moments<3, 3>::NM);
+    XG_CHECK(hipMalloc(&ctx->sdx_mom[PM], sizeof(double) * NM * (size_t)SDX_NODES));
+    const uint4 *v = (const uint4 *)ctx->sdx_verts;
+    if (PM == 1) k_sdx_node_moments<1><<<SDX_NODES, 64, 0, ctx->stream>>>(v, ctx->sdx_mom[1]);
+    if (PM == 2) k_sdx_node_moments<2><<<SDX_NODES, 64, 0, ctx->stream>>>(v, ctx->sdx_mom[2]);
+    if (PM == 3) k_sdx_node_moments<3><<<SDX_NODES, 64, 0, ctx->stream>>>(v, ctx->sdx_mom[3]);
+    XG_LAUNCH_CHECK();
+  }
+  return XG_OK;
+}
+
+// one pass of the exact-rule kernel: volume side `vol_side` (NEG / POS, or
+// -1 = none) and / or the interface
+template <int LO, int PM>
+int subdiv_exact_pass(xg_ctx *ctx, int nel, const int32_t *elems, const double *lset_nodal, const int32_t *el2v,
+                      const double *coords, const double *finv, int L, int vol_side, bool with_if, double *out_vol,
+                      double *out_if, int out_stride, uint8_t *dom, int32_t *stats, int stats_stride) {
+  if (vol_side >= 0) {
+    const int rc = ensure_sdx_tables(ctx, PM);
+    if (rc != XG_OK) return rc;
+  }
+#define XG_SDX(SIDE, WHICH)                                                                                        \
+  return launch_subdiv_exact<LO, PM, SIDE, WHICH>(ctx, nel, elems, lset_nodal, el2v, coords, finv, L, out_vol,    \
+                                                  out_if, out_stride, dom, stats, stats_stride)
+  if (vol_side == NEG && !with_if) XG_SDX(NEG, 1);
+  if (vol_side == POS && !with_if) XG_SDX(POS, 1);
+  if constexpr (PM < 3) {
+    if (vol_side == NEG && with_if) XG_SDX(NEG, 3);
+    if (vol_side == POS && with_if) XG_SDX(POS, 3);
+  }
+  if (vol_side < 0 && with_if) XG_SDX(NEG, 2);
+#undef XG_SDX
+  return set_error(XG_ERR_ARG, "subdivision rule: bad pass");
+}
+
+int subdiv_exact(xg_ctx *ctx, int lo, int p, int nel, const int32_t *elems, const double *lset_nodal,
+                 const int32_t *el2v, const double *coords, const double *finv, int L, int vol_side, bool with_if,
+                 double *out_vol, double *out_if, int out_stride, uint8_t *dom, int32_t *stats, int stats_stride) {
+#define XG_SDXP(LO, PM)                                                                                    \
+  if (lo == LO && p == PM)                                                                                 \
+    return subdiv_exact_pass<LO, PM>(ctx, nel, elems, lset_nodal, el2v, coords, finv, L, vol_side, with_if, \
+                                     out_vol, out_if, out_stride, dom, stats, stats_stride);
+  XG_SDXP(1, 1) XG_SDXP(1, 2) XG_SDXP(1, 3) XG_SDXP(2, 1) XG_SDXP(2, 2) XG_SDXP(2, 3)
+#undef XG_SDXP
+  return set_error(XG_ERR_ARG, "subdivision rule: bad level-set order or degree");
+}
+
 }  // namespace
 
 extern "C" int xg_subdiv_weights(xg_ctx *ctx, int nel, const double *lset_nodal, int lset_order, int L,
@@ -785,6 +1507,19 @@ extern "C" int xg_subdiv_moments(xg_ctx *ctx, int nel, const double *lset_nodal,
     return set_error(XG_ERR_ARG, "xg_subdiv_moments: bad argument (L <= 5, lset_order 1|2, 1 <= p <= 3)");
   if (nel == 0) return XG_OK;
   if (!lset_nodal) return set_error(XG_ERR_ARG, "xg_subdiv_moments: null level set");
+  if (order >= 2 * p) {
+    // exact base rules: node-moment tables + complemented leaves
+    const int nm = p == 1 ? 10 : (p == 2 ? 35 : 84);
+    const bool want_vol = out_neg || !out_if;
+    if (p < 3 || !want_vol || !out_if)
+      return subdiv_exact(ctx, lset_order, p, nel, nullptr, lset_nodal, nullptr, nullptr, finv, L,
+                          want_vol ? NEG : -1, out_if != nullptr, out_neg, out_if, nm, dom, nullptr, 0);
+    const int rc = subdiv_exact(ctx, lset_order, p, nel, nullptr, lset_nodal, nullptr, nullptr, finv, L, NEG, false,
+                                out_neg, nullptr, nm, dom, nullptr, 0);
+    if (rc != XG_OK) return rc;
+    return subdiv_exact(ctx, lset_order, p, nel, nullptr, lset_nodal, nullptr, nullptr, finv, L, -1, true, nullptr,
+                        out_if, nm, nullptr, nullptr, 0);
+  }
   if (lset_order == 1 && p == 1) return subdiv_moments_impl<1, 1>(ctx, nel, lset_nodal, L, order, finv, out_neg, out_if, dom);
   if (lset_order == 1 && p == 2) return subdiv_moments_impl<1, 2>(ctx, nel, lset_nodal, L, order, finv, out_neg, out_if, dom);
   if (lset_order == 1 && p == 3) return subdiv_moments_impl<1, 3>(ctx, nel, lset_nodal, L, order, finv, out_neg, out_if, dom);
@@ -792,4 +1527,73 @@ extern "C" int xg_subdiv_moments(xg_ctx *ctx, int nel, const double *lset_nodal,
   if (lset_order == 2 && p == 2) return subdiv_moments_impl<2, 2>(ctx, nel, lset_nodal, L, order, finv, out_neg, out_if, dom);
   if (lset_order == 2 && p == 3) return subdiv_moments_impl<2, 3>(ctx, nel, lset_nodal, L, order, finv, out_neg, out_if, dom);
   return set_error(XG_ERR_ARG, "xg_subdiv_moments: bad argument");
+}
+
+static int subdiv_mesh_check(const char *who, int p, int L, int order, int nel, int ndt, const int *dts) {
+  if (p < 1 || p > 3 || L < 0 || L > XG_LAT_LMAX || nel < 0 || ndt < 1 || ndt > 3 || !dts)
+    return set_error(XG_ERR_ARG, std::string(who) + ": bad argument (1 <= p <= 3, L <= 5, 1 <= ndt <= 3)");
+  if (order < 2 * p)
+    return set_error(XG_ERR_UNSUPPORTED, std::string(who) + ": order >= 2p (exact base rules) only");
+  for (int t = 0; t < ndt; ++t)
+    if (dts[t] < 0 || dts[t] > 2) return set_error(XG_ERR_ARG, std::string(who) + ": bad domain type");
+  return XG_OK;
+}
+
+static int subdiv_mesh_moments(xg_ctx *ctx, int p, int L, int nel, const int32_t *elems, const int32_t *el2v,
+                               const double *coords, const double *lset_nodal, int ndt, const int *dts,
+                               double *mom, int32_t *stats) {
+  const int nm = p == 1 ? 10 : (p == 2 ? 35 : 84);
+  for (int t = 0; t < ndt; ++t) {
+    const int d = dts[t];
+    const int rc = subdiv_exact(ctx, 2, p, nel, elems, lset_nodal, el2v, coords, nullptr, L, d == IF ? -1 : d,
+                                d == IF, d == IF ? nullptr : mom + (size_t)t * nm,
+                                d == IF ? mom + (size_t)t * nm : nullptr, ndt * nm, nullptr,
+                                stats ? stats + (size_t)t * 4 : nullptr, ndt * 4);
+    if (rc != XG_OK) return rc;
+  }
+  return XG_OK;
+}
+
+extern "C" int xg_subdiv_moments_mesh(xg_ctx *ctx, int p, int L, int order, int nel, const int32_t *elems,
+                                      const int32_t *el2v, const double *coords, const double *lset_nodal, int ndt,
+                                      const int *dts, double *moments, int32_t *stats) {
+  if (!ctx) return set_error(XG_ERR_ARG, "xg_subdiv_moments_mesh: null context");
+  int rc = subdiv_mesh_check("xg_subdiv_moments_mesh", p, L, order, nel, ndt, dts);
+  if (rc != XG_OK || nel == 0) return rc;
+  if (!elems || !el2v || !coords || !lset_nodal || !moments)
+    return set_error(XG_ERR_ARG, "xg_subdiv_moments_mesh: null pointer (elems, el2v, coords, lset_nodal, moments)");
+  return subdiv_mesh_moments(ctx, p, L, nel, elems, el2v, coords, lset_nodal, ndt, dts, moments, stats);
+}
+
+extern "C" int xg_subdiv_pipeline(xg_ctx *ctx, int p, int L, int order, int nel, const int32_t *elems,
+                                  const int32_t *el2v, const double *coords, const double *lset_nodal, int ndt,
+                                  const int *dts, int mask, int precision, void *out, int32_t *stats) {
+  if (!ctx) return set_error(XG_ERR_ARG, "xg_subdiv_pipeline: null context");
+  int rc = subdiv_mesh_check("xg_subdiv_pipeline", p, L, order, nel, ndt, dts);
+  if (rc != XG_OK) return rc;
+  if (mask < 1 || mask > 3) return set_error(XG_ERR_ARG, "xg_subdiv_pipeline: bad form mask");
+  if (precision != XG_PREC_FP32 && precision != XG_PREC_FP64)
+    return set_error(XG_ERR_UNSUPPORTED, "xg_subdiv_pipeline: FP32 or FP64 output");
+  if (nel == 0) return XG_OK;
+  if (!elems || !el2v || !coords || !lset_nodal || !out)
+    return set_error(XG_ERR_ARG, "xg_subdiv_pipeline: null pointer (elems, el2v, coords, lset_nodal, out)");
+  const int nm = p == 1 ? 10 : (p == 2 ? 35 : 84);
+  const int nd = p == 1 ? 4 : (p == 2 ? 10 : 20);
+  const size_t osz = precision == XG_PREC_FP64 ? sizeof(double) : sizeof(float);
+  // moments of up to 2^21 elements at a time in the context workspace, then
+  // the element matrices from them
+  const int chunk = nel < (1 << 21) ? nel : (1 << 21);
+  rc = ensure_ws(ctx, sizeof(double) * (size_t)chunk * ndt * nm);
+  if (rc != XG_OK) return rc;
+  double *mom = (double *)ctx->ws;
+  for (int off = 0; off < nel; off += chunk) {
+    const int m = nel - off < chunk ? nel - off : chunk;
+    rc = subdiv_mesh_moments(ctx, p, L, m, elems + off, el2v, coords, lset_nodal, ndt, dts, mom,
+                             stats ? stats + (size_t)off * ndt * 4 : nullptr);
+    if (rc != XG_OK) return rc;
+    rc = elmat_from_moments(ctx, p, m, ndt, elems + off, el2v, coords, mom, mask, precision == XG_PREC_FP64,
+                            (char *)out + (size_t)off * ndt * nd * nd * osz);
+    if (rc != XG_OK) return rc;
+  }
+  return XG_OK;
 }

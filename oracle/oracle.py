@@ -62,6 +62,9 @@ def _load():
     lib.or_cut_pipeline.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip,
                                     _ip, _dp, _dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                     ctypes.c_int, _dp, ctypes.c_int]
+    lib.or_subdiv_pipeline.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip, _dp,
+                                       _dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int, _dp,
+                                       ctypes.c_int]
     lib.or_subdiv_trig_parts.argtypes = [_dp, ctypes.c_int, ctypes.c_int, _dp]
     lib.or_subdiv_rule3.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp,
                                     ctypes.POINTER(_dp), ctypes.POINTER(_dp), ctypes.c_int,
@@ -352,6 +355,27 @@ def subdiv_rule3(lset_nodal, lset_order, L, order, Finv=None, cap=200000):
     if dom < 0:
         raise RuntimeError("oracle subdiv error %d" % dom)
     return dom, [(P[r][:n[r]].copy(), W[r][:n[r]].copy()) for r in range(3)]
+
+
+def subdiv_pipeline(p, L, order, elems, el2v, coords, lset_nodal, dts, mask=3, nthreads=1):
+    """or_subdiv_pipeline: level-L subdivision rule + degree-p element
+    matrices per listed element; returns nel x ndt x ndof x ndof"""
+    nd = h1_ndof(3, p)
+    el2v = np.ascontiguousarray(el2v, dtype=np.int32)
+    coords = _f64(coords)
+    ln = _f64(lset_nodal)
+    if elems is None:
+        nel, ep = el2v.shape[0], None
+    else:
+        elems = np.ascontiguousarray(elems, dtype=np.int32)
+        nel, ep = len(elems), elems.ctypes.data_as(_ip)
+    dtsa = (ctypes.c_int * len(dts))(*dts)
+    out = np.zeros((nel, len(dts), nd, nd))
+    r = lib().or_subdiv_pipeline(p, L, order, nel, ep, el2v.ctypes.data_as(_ip), _d(coords), _d(ln), len(dts),
+                                 dtsa, mask, _d(out), nthreads)
+    if r:
+        raise RuntimeError("oracle subdivision pipeline error %d" % r)
+    return out
 
 
 def subdiv_trig_parts(lset_nodal, lset_order, L):

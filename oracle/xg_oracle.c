@@ -2071,6 +2071,80 @@ int or_subdiv_moments3_batch(int nel, const double *lset_nodal, int lset_order, 
   return err;
 }
 
+/* Threaded subdivision pipeline (the CPU baseline of P2 level sets on a mesh,
+   and the checker for sampled elements): per listed tetrahedron the
+   subdivision rule of level L (CutIntegrationRule with subdivlvl = L,
+   xintegration.cpp:1626-1754) is materialised for every domain type of dts,
+   then CalcElementMatrix (symboliccutbfi.cpp:76-304) runs on it with the
+   physical element map; interface weights carry the F^{-T} n transform of
+   xintegration.cpp:1698-1716. */
+typedef struct {
+  int p, L, order, nel, ndt, mask, lo, hi, err, cap;
+  const int32_t *elems, *el2v;
+  const double *coords, *lset_nodal;
+  const int *dts;
+  double *out;
+} SubPipeJob;
+
+static void *subpipe_worker(void *arg)
+{
+  SubPipeJob *J = (SubPipeJob *)arg;
+  const int nd = or_h1_ndof(3, J->p);
+  double *P[3], *Wt[3];
+  for (int r = 0; r < 3; ++r) {
+    P[r] = (double *)malloc(sizeof(double) * 3 * (size_t)J->cap);
+    Wt[r] = (double *)malloc(sizeof(double) * (size_t)J->cap);
+  }
+  for (int i = J->lo; i < J->hi; ++i) {
+    const int e = J->elems ? J->elems[i] : i;
+    double V[12], F[9], Finv[9], absdet;
+    for (int k = 0; k < 4; ++k) {
+      const int v = J->el2v[(size_t)e * 4 + k];
+      for (int d = 0; d < 3; ++d) V[k * 3 + d] = J->coords[(size_t)v * 3 + d];
+    }
+    or_jacobian(3, V, F, Finv, &absdet);
+    int n[3];
+    const int dom = or_subdiv_rule3(J->lset_nodal + (size_t)e * 10, 2, J->L, J->order, Finv, P, Wt, J->cap, n);
+    if (dom < 0) { J->err = dom; break; }
+    for (int t = 0; t < J->ndt; ++t) {
+      double *o = J->out + ((size_t)i * J->ndt + t) * nd * nd;
+      memset(o, 0, sizeof(double) * nd * nd);
+      const int r = J->dts[t];
+      if (n[r] > 0) or_elmat_add(3, J->p, V, P[r], Wt[r], n[r], J->mask, o);
+    }
+  }
+  for (int r = 0; r < 3; ++r) { free(P[r]); free(Wt[r]); }
+  return NULL;
+}
+
+int or_subdiv_pipeline(int p, int L, int order, int nel, const int32_t *elems, const int32_t *el2v,
+                       const double *coords, const double *lset_nodal, int ndt, const int *dts, int mask,
+                       double *out, int nthreads)
+{
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  get_rule(3, order);
+  get_rule(2, order);
+  SubPipeJob jobs[256];
+  pthread_t th[256];
+  for (int t = 0; t < nthreads; ++t) {
+    SubPipeJob *J = &jobs[t];
+    J->p = p; J->L = L; J->order = order; J->nel = nel; J->ndt = ndt; J->mask = mask;
+    J->elems = elems; J->el2v = el2v; J->coords = coords; J->lset_nodal = lset_nodal;
+    J->dts = dts; J->out = out; J->err = 0; J->cap = 400000;
+    J->lo = (int)((long long)nel * t / nthreads);
+    J->hi = (int)((long long)nel * (t + 1) / nthreads);
+    if (nthreads > 1) pthread_create(&th[t], NULL, subpipe_worker, J);
+    else subpipe_worker(J);
+  }
+  int err = 0;
+  for (int t = 0; t < nthreads; ++t) {
+    if (nthreads > 1) pthread_join(th[t], NULL);
+    if (jobs[t].err) err = jobs[t].err;
+  }
+  return err;
+}
+
 /* ====================================================================== */
 /* MLP (FP64 oracle for the learned rule's kernels)                        */
 /* ====================================================================== */

@@ -165,6 +165,15 @@ int or_subdiv_moments3_batch(int nel, const double *lset_nodal, int lset_order,
                              int L, int order, int q, double *out_neg,
                              double *out_if, int nthreads);
 
+/* Rule + element matrix on tetrahedral meshes with a P2 level set
+   (lset_nodal: ne x 10 nodal values, indexed by element id): for each listed
+   element (elems nullable) the level-L subdivision rule of every dts[t] and
+   the degree-p mass/stiffness matrix on it (physical map from coords/el2v).
+   out: nel x ndt x ndof^2.  returns 0 or an OR_ERR code. */
+int or_subdiv_pipeline(int p, int L, int order, int nel, const int32_t *elems, const int32_t *el2v,
+                       const double *coords, const double *lset_nodal, int ndt, const int *dts, int mask,
+                       double *out, int nthreads);
+
 /* ---- learned rule: MLP 10 -> H -> H -> O with tanh (FP64) -------------- */
 /* params layout: W1[H][I], b1[H], W2[H][H], b2[H], W3[O][H], b3[O] */
 int or_mlp_nparams(int I, int H, int O);

@@ -71,6 +71,30 @@ def test_mark_config1(orc, dev):
     assert int(np.sum(dt == NEG)) == 2356
 
 
+def test_p1_perturbation_config1(orc, dev):
+    """InterpolateToP1's perturbation (utils/p1interpol.cpp:104-107) through
+    xg_p1_perturb on C1's raw vertex values |x| - 1: exactly the 12 vertices
+    with |v| < 1e-14 (FP64 gives |v| ~ 2.2e-16 at points such as (0.6, 0.8))
+    become +1e-14, bit for bit; marking the perturbed values gives 270 cut
+    and 2356 NEG triangles, as the oracle does"""
+    torch, _abi, ctx = dev
+    mesh = MakeStructured2DMesh(nx=60, ny=60, mapping=lambda x, y: (-1.5 + 3 * x, -1.5 + 3 * y))
+    x, y = mesh.coords[:, 0], mesh.coords[:, 1]
+    raw = np.sqrt(x * x + y * y) - 1.0
+    d = up(torch, raw)
+    ctx.p1_perturb(d, 1e-14)
+    ctx.sync()
+    g = d.cpu().numpy()
+    changed = np.nonzero(g.view(np.int64) != raw.view(np.int64))[0]
+    assert len(changed) == 12
+    assert np.all(np.abs(raw[changed]) < 1e-14) and np.all(g[changed] == 1e-14)
+    assert np.sum(np.abs(raw) < 1e-14) == 12
+    dt, cr, bits, cl = gpu_mark(dev, 2, mesh.el2v, g)
+    assert int(np.sum(dt == IF)) == 270 and int(np.sum(dt == NEG)) == 2356
+    dt_o, cr_o, bits_o = orc.mark(2, mesh.el2v, g)
+    assert np.array_equal(dt, dt_o) and np.array_equal(bits, bits_o)
+
+
 def test_mark_config2_full(orc, dev):
     mesh, lset = W.config2(64)
     dt = assert_mark_parity(orc, dev, 3, mesh.el2v, lset)
@@ -198,9 +222,12 @@ def rel_fro(a, b):
     return num / np.maximum(den, 1e-300)
 
 
-@pytest.mark.parametrize("D,p,n", [(2, 1, 0), (2, 2, 0), (2, 3, 0), (3, 1, 8), (3, 2, 8), (3, 3, 8)])
-@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("D,p,n", [(2, 1, 0), (2, 2, 0), (2, 3, 0), (3, 1, 8), (3, 2, 8), (3, 3, 8), (3, 2, 64),
+                                   (3, 3, 64)])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
 def test_pipeline_elmat_parity(orc, dev, D, p, n, prec):
+    """FP64 MFMA 1e-12; FP32 MFMA and split-BF16 MFMA (both with FP64 sums of
+    8-point groups) 1e-6 -- on the full 64^3 3D plain mesh too, P2 and P3"""
     torch, _abi, ctx = dev
     if D == 2:
         mesh, lset = W.config1()
@@ -214,7 +241,7 @@ def test_pipeline_elmat_parity(orc, dev, D, p, n, prec):
     dts = [NEG, IF, POS]
     ref = orc.cut_pipeline(D, p, order, elems, mesh.el2v, mesh.coords, lset, dts, mask=3, nthreads=8)
     nd = ref.shape[-1]
-    P = _abi.XG_PREC_FP64 if prec == "fp64" else _abi.XG_PREC_FP32
+    P = {"fp64": _abi.XG_PREC_FP64, "fp32": _abi.XG_PREC_FP32, "bf16x3": _abi.XG_PREC_BF16X3}[prec]
     out = torch.zeros(len(elems) * len(dts) * nd * nd,
                       dtype=torch.float64 if prec == "fp64" else torch.float32, device="cuda")
     ctx.cut_pipeline(D, p, order, up(torch, mesh.el2v), up(torch, mesh.coords), up(torch, lset), dts, 3, P,
@@ -228,7 +255,7 @@ def test_pipeline_elmat_parity(orc, dev, D, p, n, prec):
     assert err.max() < tol, "max rel err %g" % err.max()
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
 def test_csr_elmat_matches_pipeline(orc, dev, prec):
     torch, _abi, ctx = dev
     mesh, lset = W.config2(8)
@@ -236,7 +263,7 @@ def test_csr_elmat_matches_pipeline(orc, dev, prec):
     dto, _, _ = orc.mark(D, mesh.el2v, lset)
     elems = np.nonzero(dto == IF)[0].astype(np.int32)
     off, ns, pts = gpu_rule(dev, D, mesh, lset, NEG, order, elems)
-    P = _abi.XG_PREC_FP64 if prec == "fp64" else _abi.XG_PREC_FP32
+    P = {"fp64": _abi.XG_PREC_FP64, "fp32": _abi.XG_PREC_FP32, "bf16x3": _abi.XG_PREC_BF16X3}[prec]
     tdt = torch.float64 if prec == "fp64" else torch.float32
     out = torch.zeros(len(elems) * 100, dtype=tdt, device="cuda")
     d_el, d_x, d_e = up(torch, mesh.el2v), up(torch, mesh.coords), up(torch, elems)

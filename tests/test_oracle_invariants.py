@@ -419,3 +419,52 @@ def test_mlp_gradient_finite_differences(orc):
         assert abs((lp - lm) / (2 * h) - g[k]) < 1e-7 * max(1, abs(g[k]))
     y = orc.mlp_forward(I, H, O, P, x)
     assert abs(0.5 * np.sum((y - t) ** 2) / 5 - loss) < 1e-12
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_subdiv_pipeline_linear_equals_straight_pipeline(orc, p):
+    """or_subdiv_pipeline with a linear level set given by P2 nodal values
+    equals the straight-cut pipeline (pinned by the reference's KATs) on
+    physical tets, NEG / POS / IF, at levels 0 and 2: the subdivision rule
+    of a linear level set covers the same polytope, and the moments of exact
+    rules do not depend on how it is split"""
+    from ngsxfem_amd.meshes import MakeStructured3DMesh
+    from ngsxfem_amd.workloads import p1_perturb
+    mesh = MakeStructured3DMesh(nx=3, ny=3, nz=3)
+    X = mesh.coords
+    lin = p1_perturb(X @ np.array([0.7, -0.4, 0.55]) - 0.31)
+    # P2 nodal values of the same linear function: vertex values and their
+    # edge midpoint averages (Lattice<3,2> dof order)
+    V = lin[mesh.el2v]
+    pairs = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+    vals = np.concatenate([V, np.stack([0.5 * (V[:, i] + V[:, j]) for i, j in pairs], axis=1)], axis=1)
+    dto, _, _ = orc.mark(3, mesh.el2v, lin)
+    cut = np.nonzero(dto == IF)[0].astype(np.int32)
+    assert len(cut) > 10
+    ref = orc.cut_pipeline(3, p, 2 * p, cut, mesh.el2v, X, lin, [NEG, POS, IF], mask=3, nthreads=4)
+    for L in (0, 2):
+        got = orc.subdiv_pipeline(p, L, 2 * p, cut, mesh.el2v, X, vals, [NEG, POS, IF], 3, nthreads=4)
+        scale = np.abs(ref).max(axis=(-2, -1), keepdims=True)
+        assert (np.abs(got - ref) / scale).max() < 1e-11, L
+
+
+def test_subdiv_pipeline_sides_partition_the_element(orc):
+    """NEG + POS element matrices of the subdivision rule = the uncut
+    element's matrix (the tree assigns every region to one side; zero values
+    count as POS, xintegration.cpp:1159-1164), for random quadrics on
+    physical tets"""
+    from ngsxfem_amd.workloads import random_quadric_p2
+    rng = np.random.default_rng(4)
+    nel = 6
+    vals = random_quadric_p2(nel, seed=9)
+    V = np.zeros((nel, 4, 3))
+    V[:, :3] = np.eye(3)[None] + 0.2 * rng.standard_normal((nel, 3, 3))
+    X = V.reshape(-1, 3)
+    el2v = np.arange(4 * nel, dtype=np.int32).reshape(nel, 4)
+    for p in (1, 3):
+        g = orc.subdiv_pipeline(p, 3, 2 * p, None, el2v, X, vals, [NEG, POS], 3, nthreads=4)
+        full = orc.subdiv_pipeline(p, 0, 2 * p, None, el2v, X, np.ones_like(vals), [POS], 3)[:, 0]
+        for e in range(nel):
+            ref = orc.elmat(3, p, V[e], orc.rule(3, 2 * p)[0], orc.rule(3, 2 * p)[1], mask=3)
+            np.testing.assert_allclose(full[e], ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+            np.testing.assert_allclose(g[e, 0] + g[e, 1], ref, rtol=0, atol=1e-12 * np.abs(ref).max())
