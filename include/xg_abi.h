@@ -417,6 +417,19 @@ int xg_mlp_train_step(xg_ctx *ctx, int I, int H, int O, float *params, float *ad
 int xg_moment_residual(xg_ctx *ctx, int nel, int nq, const float *pred, double scale, const double *ref,
                        double thr, double *rel, uint8_t *accept, double *out);
 
+/* Learned rule as a correction of a cheap rule, checked without the plain
+   rule of full degree.  No reference counterpart (the rebuild's learned
+   quadrature).  out = base + corr: base = the degree-2p moments of a cheap
+   plain rule (the subdivision rule at a low level), corr = the MLP's FP32
+   correction (nel x NM); the check lowers out to degree-2 moments through
+   sum lam = 1 and compares them with ref2 (nel x 10 FP64, the exact degree-2
+   moments of the full plain rule, xg_subdiv_moments with p = 1, a fraction of
+   its cost): rel[e] = |lowered - ref2| / |ref2| (nullable), accept[e] =
+   rel <= thr.  Rejected elements fall back to the plain rule (the caller
+   computes them).  p = 1..3. */
+int xg_learned_moments(xg_ctx *ctx, int nel, int p, const double *base, const float *corr,
+                       const double *ref2, double thr, double *out, double *rel, uint8_t *accept);
+
 #ifdef __cplusplus
 }
 #endif

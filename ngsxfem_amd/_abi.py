@@ -61,6 +61,7 @@ _SIGS = {
     "xg_subdiv_weights": [_vp, _i, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp],
     "xg_subdiv_moments": [_vp, _i, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp],
     "xg_subdiv_pipeline": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _i, _vp, _vp],
+    "xg_learned_moments": [_vp, _i, _i, _vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp],
     "xg_subdiv_moments_mesh": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp],
     "xg_elmat_moments": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "xg_cut_elvec": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp],

@@ -170,6 +170,46 @@ __global__ __launch_bounds__(256) void k_elmat_from_moments(int nel, int ndt, co
   MM::template forms<T>(m, mk, Hs, sm, sk, out + (size_t)g * (N * N));
 }
 
+// learned moments m = base + corr (FP64 + FP32 correction) and their check:
+// the degree-2 moments of m, obtained through sum lam = 1 (lower, once per
+// degree step), against the plain rule's exact degree-2 moments ref2
+template <int P>
+__global__ __launch_bounds__(256) void k_learned_check(int nel, const double *__restrict__ base,
+                                                       const float *__restrict__ corr, const double *__restrict__ ref2,
+                                                       double thr, double *__restrict__ out, double *__restrict__ rel,
+                                                       uint8_t *__restrict__ accept) {
+  using MM = Moments<3, P>;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nel) return;
+  double m[MM::NM];
+#pragma unroll
+  for (int k = 0; k < MM::NM; ++k) {
+    m[k] = base[(size_t)i * MM::NM + k] + (double)corr[(size_t)i * MM::NM + k];
+    out[(size_t)i * MM::NM + k] = m[k];
+  }
+  double m2[10];
+  if constexpr (P == 1) {
+#pragma unroll
+    for (int k = 0; k < 10; ++k) m2[k] = m[k];
+  } else if constexpr (P == 2) {
+    MM::lower(m, m2);
+  } else {
+    double m4[Moments<3, 2>::NM];
+    MM::lower(m, m4);
+    Moments<3, 2>::lower(m4, m2);
+  }
+  double num = 0.0, den = 0.0;
+#pragma unroll
+  for (int k = 0; k < 10; ++k) {
+    const double r = ref2[(size_t)i * 10 + k];
+    num += (m2[k] - r) * (m2[k] - r);
+    den += r * r;
+  }
+  const double q = den > 0.0 ? sqrt(num / den) : (num > 0.0 ? INFINITY : 0.0);
+  if (rel) rel[i] = q;
+  accept[i] = q <= thr ? 1 : 0;
+}
+
 template <int D, int P, typename T>
 int launch_from_moments(xg_ctx *ctx, int nel, int ndt, const int32_t *elems, const int32_t *el2v,
                         const double *coords, const double *mom, int mask, void *out) {
@@ -351,6 +391,19 @@ extern "C" int xg_assemble_vec(xg_ctx *ctx, int nel, int ndof, const int32_t *do
     k_assemble_vec<double><<<(unsigned)blocks, 256, 0, ctx->stream>>>(nent, dofs, (const double *)elvec, out);
   else
     k_assemble_vec<float><<<(unsigned)blocks, 256, 0, ctx->stream>>>(nent, dofs, (const float *)elvec, out);
+  XG_LAUNCH_CHECK();
+  return XG_OK;
+}
+
+extern "C" int xg_learned_moments(xg_ctx *ctx, int nel, int p, const double *base, const float *corr,
+                                  const double *ref2, double thr, double *out, double *rel, uint8_t *accept) {
+  if (!ctx || nel < 0 || p < 1 || p > 3) return set_error(XG_ERR_ARG, "xg_learned_moments: bad argument");
+  if (nel == 0) return XG_OK;
+  if (!base || !corr || !ref2 || !out || !accept) return set_error(XG_ERR_ARG, "xg_learned_moments: null pointer");
+  const int blocks = (nel + 255) / 256;
+  if (p == 1) k_learned_check<1><<<blocks, 256, 0, ctx->stream>>>(nel, base, corr, ref2, thr, out, rel, accept);
+  if (p == 2) k_learned_check<2><<<blocks, 256, 0, ctx->stream>>>(nel, base, corr, ref2, thr, out, rel, accept);
+  if (p == 3) k_learned_check<3><<<blocks, 256, 0, ctx->stream>>>(nel, base, corr, ref2, thr, out, rel, accept);
   XG_LAUNCH_CHECK();
   return XG_OK;
 }

@@ -1485,6 +1485,23 @@ int subdiv_exact(xg_ctx *ctx, int lo, int p, int nel, const int32_t *elems, cons
   return set_error(XG_ERR_ARG, "subdivision rule: bad level-set order or degree");
 }
 
+// fixed-node weights of the degree-4 Lagrange lattice from degree-4 moments,
+// in place: W_k = sum_alpha nodal34[k][alpha] m_alpha (one element per
+// thread, its 35 moments read before any weight is written)
+__global__ __launch_bounds__(128) void k_moments_to_nodal4(int nel, double *__restrict__ a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nel) return;
+  double m[35];
+#pragma unroll
+  for (int k = 0; k < 35; ++k) m[k] = a[(size_t)i * 35 + k];
+  for (int k = 0; k < 35; ++k) {
+    double w = 0.0;
+#pragma unroll
+    for (int q = 0; q < 35; ++q) w = fma(nodal34[k][q], m[q], w);
+    a[(size_t)i * 35 + k] = w;
+  }
+}
+
 }  // namespace
 
 extern "C" int xg_subdiv_weights(xg_ctx *ctx, int nel, const double *lset_nodal, int lset_order, int L,
@@ -1495,6 +1512,21 @@ extern "C" int xg_subdiv_weights(xg_ctx *ctx, int nel, const double *lset_nodal,
     return set_error(XG_ERR_ARG, "xg_subdiv_weights: bad argument (L <= 5, lset_order 1|2, q == 4)");
   if (nel == 0) return XG_OK;
   if (!lset_nodal) return set_error(XG_ERR_ARG, "xg_subdiv_weights: null level set");
+  if (order >= 4) {
+    // exact base rules: degree-4 moments from the exact-rule kernel, then
+    // the nodal weights (the degree-4 lattice integrates exactly what the
+    // moments do)
+    const bool want_vol = out_neg || !out_if;
+    int rc = subdiv_exact(ctx, lset_order, 2, nel, nullptr, lset_nodal, nullptr, nullptr, finv, L,
+                          want_vol ? NEG : -1, out_if != nullptr, out_neg, out_if, 35, dom, nullptr, 0);
+    if (rc != XG_OK) return rc;
+    for (double *a : {out_neg, out_if})
+      if (a) {
+        k_moments_to_nodal4<<<(nel + 127) / 128, 128, 0, ctx->stream>>>(nel, a);
+        XG_LAUNCH_CHECK();
+      }
+    return XG_OK;
+  }
   if (lset_order == 2)
     return launch_subdiv<2, 2, 3, 1>(ctx, nel, lset_nodal, L, order, finv, out_neg, out_if, dom);
   return launch_subdiv<1, 2, 3, 1>(ctx, nel, lset_nodal, L, order, finv, out_neg, out_if, dom);

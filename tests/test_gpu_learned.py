@@ -5,9 +5,10 @@ C ABI, checked against the FP64 oracle.
   backward (itself pinned by finite differences): 1e-4 relative.
 * BF16 MFMA inference vs the FP32 forward: BF16 rounding of activations and
   weights, 3e-2 relative (Frobenius) -- also validates the MFMA lane layout.
-* A trained network beats the untrained one by > 10x and beats the plain
-  rule at subdivision level 1 (the cheap plain alternative) against the
-  level-4 plain rule.
+* The trained correction network brings the level-1 subdivision moments
+  closer to the level-4 ones than level 1 alone (> 3x) and than an untrained
+  network (> 10x); the degree-2 check passes the plain rule itself to 1e-12;
+  threshold 0 makes every element fall back to the level-4 rule.
 * Element matrices from fixed-node weights equal those on the materialised
   subdivision rule (exactness of the degree-4 lattice for P2 integrands).
 """
@@ -82,34 +83,68 @@ def test_mlp_forward_fp32_and_bf16(orc, dev):
 
 
 def test_learned_rule_gain(orc, dev):
-    """training on the GPU reduces the error against the level-4 plain rule by
-    > 10x and beats the level-1 plain rule"""
+    """the correction network, trained on the GPU, brings the level-1
+    subdivision moments closer to the level-4 ones than the level-1 rule
+    alone (and far closer than an untrained network); the degree-2 check is
+    exact for the plain rule itself; threshold 0 sends every element to the
+    plain fallback, an infinite one accepts all"""
     torch, _abi, ctx = dev
     from ngsxfem_amd.learned import LearnedRule, random_cut_quadrics
-    lr_ = LearnedRule(ctx, seed=0)
-    untrained = LearnedRule(ctx, seed=0)
+    lr_ = LearnedRule(ctx, p=2, base_L=1, L=4, seed=0)
+    untrained = LearnedRule(ctx, p=2, base_L=1, L=4, seed=0)
     untrained.fold()
-    hist = lr_.train(n_train=40000, epochs=8, batch=512, lr=2e-3, L=4, seed=3)
+    lr_.train(n_train=40000, epochs=8, batch=512, lr=2e-3, seed=3)
     xv = random_cut_quadrics(4000, 99)
-    w4, _, _ = lr_.plain_weights(xv, L=4)
-    w1, _, _ = lr_.plain_weights(xv, L=1)
-    p = lr_.predict(xv).to(torch.float64)
-    pu = untrained.predict(xv).to(torch.float64)
+    m4 = lr_.plain_moments(xv, 4)
+    m1 = lr_.plain_moments(xv, 1)
+    c = lr_.correction(xv).to(torch.float64)
+    cu = untrained.correction(xv).to(torch.float64)
     ctx.sync()
+
     def err(a):
-        return float(torch.linalg.norm(a - w4) / torch.linalg.norm(w4))
-    e_nn, e_un, e_l1 = err(p), err(pu), err(w1)
-    print("rel err vs L4: trained %.3e untrained %.3e plain L1 %.3e" % (e_nn, e_un, e_l1))
+        return float(torch.linalg.norm(a - m4) / torch.linalg.norm(m4))
+    e_nn, e_un, e_l1 = err(m1 + c), err(m1 + cu), err(m1)
+    print("rel err vs L4: corrected %.3e untrained %.3e plain L1 %.3e" % (e_nn, e_un, e_l1))
+    assert e_nn < e_l1 / 3
     assert e_nn < e_un / 10
-    assert e_nn < e_l1
-    # fallback: threshold 0 -> everything falls back to the plain weights
-    out, acc, rel = lr_.with_fallback(p.to(torch.float32).contiguous(), w4, thr=0.0)
+    # the check lowers degree-4 moments to degree 2 through sum lam = 1: the
+    # plain level-4 moments pass it against the level-4 degree-2 moments
+    n = xv.shape[0]
+    ref2 = lr_.plain_moments(xv, 4, p=1)
+    out = torch.empty_like(m4)
+    rel = torch.empty(n, dtype=torch.float64, device="cuda")
+    acc = torch.empty(n, dtype=torch.uint8, device="cuda")
+    zero = torch.zeros(n, 35, dtype=torch.float32, device="cuda")
+    _abi.check(_abi.lib().xg_learned_moments(ctx.h, n, 2, _abi.ptr(m4), _abi.ptr(zero), _abi.ptr(ref2), 1e-12,
+                                             _abi.ptr(out), _abi.ptr(rel), _abi.ptr(acc)))
     ctx.sync()
-    assert int(acc.sum().item()) == 0
-    assert torch.equal(out, w4)
-    out, acc, rel = lr_.with_fallback(p.to(torch.float32).contiguous(), w4, thr=1e30)
+    assert float(rel.max()) < 1e-12 and int(acc.sum()) == n
+    assert torch.equal(out, m4)
+    # fallback extremes through LearnedRule.moments
+    out0, acc0, _ = lr_.moments(xv, thr=0.0)
     ctx.sync()
-    assert int(acc.sum().item()) == w4.shape[0]
+    assert int(acc0.sum().item()) == 0 and torch.equal(out0, m4)
+    out1, acc1, _ = lr_.moments(xv, thr=float("inf"))
+    ctx.sync()
+    assert int(acc1.sum().item()) == n
+    assert torch.allclose(out1, m1 + lr_.correction(xv).to(torch.float64), rtol=0, atol=1e-14)
+
+
+def test_learned_moments_elmat(orc, dev):
+    """element matrices from learned-with-fallback moments at threshold 0
+    are the plain level-4 matrices (every element falls back)"""
+    torch, _abi, ctx = dev
+    from ngsxfem_amd.learned import LearnedRule, random_cut_quadrics
+    lr_ = LearnedRule(ctx, p=2, base_L=1, L=4, seed=1)
+    lr_.fold()
+    xv = random_cut_quadrics(512, 7)
+    m, acc, rel = lr_.moments(xv, thr=0.0)
+    a = torch.empty(512 * 100, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    ctx.elmat_moments(3, 2, m, 3, _abi.XG_PREC_FP64, a, nel=512)
+    ctx.elmat_moments(3, 2, lr_.plain_moments(xv, 4), 3, _abi.XG_PREC_FP64, b, nel=512)
+    ctx.sync()
+    assert torch.equal(a, b)
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
