@@ -88,6 +88,20 @@ int xg_mark(xg_ctx *ctx, int D, int ne, const int32_t *el2v, const double *lset_
             uint8_t *dt, double *cut_ratio, uint32_t *bits, int32_t *cut_list,
             int32_t *n_cut);
 
+/* xg_mark for callers that cannot allocate device memory (code under HIP
+   graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+   counterpart in the reference.  Same arguments and outputs as xg_mark, bit
+   for bit; the block counts and scan partials that xg_mark keeps in the
+   context workspace live in a fixed file-scope device array instead
+   (1 MiB per GPU, held until the process ends), so this call never
+   allocates.  ne <= XG_MARK_NOALLOC_MAX_NE, else XG_ERR_CAPACITY.  Calls
+   that overlap on one GPU (several streams or contexts) share that array
+   and race: serialise them. */
+#define XG_MARK_NOALLOC_MAX_NE (1 << 28)
+int xg_mark_noalloc(xg_ctx *ctx, int D, int ne, const int32_t *el2v, const double *lset_v,
+                    uint8_t *dt, double *cut_ratio, uint32_t *bits, int32_t *cut_list,
+                    int32_t *n_cut);
+
 /* ---- a12 for P2 level sets (cutinfo.cpp:61-133 with the subdivision rule of
    xintegration.cpp:617-752): the element domain is the lattice check of the
    level-L subdivision rule (IF iff the level set takes values >= 0 and < 0 on

@@ -61,6 +61,7 @@ _SIGS = {
     "xg_subdiv_weights": [_vp, _i, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp],
     "xg_subdiv_moments": [_vp, _i, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp],
     "xg_subdiv_pipeline": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _i, _vp, _vp],
+    "xg_mark_noalloc": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "xg_learned_moments": [_vp, _i, _i, _vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp],
     "xg_subdiv_moments_mesh": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp],
     "xg_elmat_moments": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _vp],
@@ -180,6 +181,12 @@ class Context:
         ne = el2v.shape[0]
         check(lib().xg_mark(self.h, D, ne, ptr(el2v), ptr(lset), ptr(dt), ptr(cut_ratio), ptr(bits),
                             ptr(cut_list), ptr(n_cut)))
+
+    def mark_noalloc(self, D, el2v, lset, dt=None, cut_ratio=None, bits=None, cut_list=None, n_cut=None):
+        """xg_mark without allocation (fixed device scratch)"""
+        ne = el2v.shape[0]
+        check(lib().xg_mark_noalloc(self.h, D, ne, ptr(el2v), ptr(lset), ptr(dt), ptr(cut_ratio), ptr(bits),
+                                    ptr(cut_list), ptr(n_cut)))
 
     def mark_p2(self, lset_nodal, L, dt, bits, cut_list=None, n_cut=None):
         ne = lset_nodal.shape[0]

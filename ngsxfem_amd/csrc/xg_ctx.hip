@@ -76,6 +76,14 @@ extern "C" int xg_ctx_create(int device, xg_ctx **out) {
     delete c;
     return xg::check_hip(e, "hipMemset(err)");
   }
+  // per-device set-up of the no-alloc entry point's scratch address
+  const int r = xg::noalloc_scratch(device, &c->noalloc_scratch);
+  if (r != XG_OK) {
+    hipFree(c->d_err);
+    hipHostFree(c->h_err);
+    delete c;
+    return r;
+  }
   *out = c;
   return XG_OK;
 }

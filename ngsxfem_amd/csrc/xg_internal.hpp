@@ -21,6 +21,8 @@ struct xg_ctx {
   // refinement-tree node of depth <= 5, and their degree-2p moments (p = 1..3)
   void *sdx_verts = nullptr;
   double *sdx_mom[4] = {nullptr, nullptr, nullptr, nullptr};
+  // device address of the no-alloc marking scratch on this device
+  int32_t *noalloc_scratch = nullptr;
 };
 
 namespace xg {
@@ -44,6 +46,8 @@ int launch_moment_pipeline(xg_ctx *ctx, int D, int p, int order, int nel, const 
                            const int32_t *el2v, const double *coords, const double *lset, int ndt, int dts,
                            int mask, bool fp64_out, void *out);
 
+// device address of the no-alloc marking scratch on `device` (xg_mark.hip)
+int noalloc_scratch(int device, int32_t **out);
 // degree-p (tetrahedra) mass / stiffness matrices from degree-2p moments
 // stored element-major with ndt sets per element (xg_moment.hip)
 int elmat_from_moments(xg_ctx *ctx, int p, int nel, int ndt, const int32_t *elems, const int32_t *el2v,

@@ -10,6 +10,8 @@
 // few million vertices).  The 8 COMBINED_DOMAIN_TYPE bit arrays are built
 // with one wave ballot per array: lane 0 stores the 64-bit mask, i.e. two
 // uint32 words, so no atomics touch the bit arrays.
+#include <mutex>
+
 #include "xg_device.hpp"
 #include "xg_internal.hpp"
 
@@ -907,6 +909,88 @@ extern "C" int xg_mark(xg_ctx *ctx, int D, int ne, const int32_t *el2v, const do
   // scan of block counts: blk[nblk] = total.  The scan scratch is the start
   // of ctx->ws, disjoint from blk.
   scan_block_counts(ctx->stream, blk, nblk, (int32_t *)ctx->ws, n_cut);
+  XG_LAUNCH_CHECK();
+  if (cut_list) {
+    k_compact<MARK_BLK, MARK_EPT><<<nblk, MARK_BLK, 0, ctx->stream>>>(ne, bits + (size_t)4 * nw, blk, cut_list);
+    XG_LAUNCH_CHECK();
+  }
+  return XG_OK;
+}
+
+// ---------------------------------------------------------------- no-alloc
+// xg_mark without the context workspace: the block counts and the scan
+// partials live in one file-scope device array.  Every GPU that loads the
+// library holds its own copy; its address is looked up once per device
+// (hipGetSymbolAddress with that device current) and only that address is
+// passed to kernels.  k_mark writes every block count it later reads and the
+// scan writes its partials before reading them, so nothing has to start at
+// zero.  Calls that overlap on one GPU share the array and race.
+namespace {
+constexpr int NOALLOC_MAX_NE = XG_MARK_NOALLOC_MAX_NE;
+constexpr int NOALLOC_MAX_BLK = NOALLOC_MAX_NE / (MARK_BLK * MARK_EPT);
+constexpr int NOALLOC_INTS = NOALLOC_MAX_BLK + 1 + (NOALLOC_MAX_BLK + 1 + SCAN_TILE - 1) / SCAN_TILE + 1;
+static __device__ __align__(256) int32_t scratch[NOALLOC_INTS];
+
+constexpr int NOALLOC_MAX_DEV = 64;
+std::mutex g_scratch_mu;
+int32_t *g_scratch_ptr[NOALLOC_MAX_DEV] = {};
+
+__global__ void k_zero_i32(int32_t *p) { *p = 0; }
+}  // namespace
+
+namespace xg {
+// the device address of `scratch` on `device` (looked up on first use, with
+// that device current)
+int noalloc_scratch(int device, int32_t **out) {
+  if (device < 0 || device >= NOALLOC_MAX_DEV) return set_error(XG_ERR_ARG, "no-alloc scratch: device id too large");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = 0;
+    XG_CHECK(hipGetDevice(&cur));
+    if (cur != device) XG_CHECK(hipSetDevice(device));
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != device) XG_CHECK(hipSetDevice(cur));
+    if (e != hipSuccess) return check_hip(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[device] = (int32_t *)ptr;
+  }
+  *out = g_scratch_ptr[device];
+  return XG_OK;
+}
+}  // namespace xg
+
+extern "C" int xg_mark_noalloc(xg_ctx *ctx, int D, int ne, const int32_t *el2v, const double *lset_v, uint8_t *dt,
+                               double *cut_ratio, uint32_t *bits, int32_t *cut_list, int32_t *n_cut) {
+  if (!ctx || D < 1 || D > 3 || ne < 0 || !bits || (ne > 0 && (!el2v || !lset_v)))
+    return set_error(XG_ERR_ARG, "xg_mark_noalloc: bad argument");
+  if (ne > NOALLOC_MAX_NE)
+    return set_error(XG_ERR_CAPACITY, "xg_mark_noalloc: more than XG_MARK_NOALLOC_MAX_NE (2^28) elements; "
+                                      "use xg_mark");
+  int32_t *scr = ctx->noalloc_scratch;
+  if (!scr) {
+    const int r = noalloc_scratch(ctx->device, &scr);
+    if (r != XG_OK) return r;
+    ctx->noalloc_scratch = scr;
+  }
+  if (ne == 0) {
+    if (n_cut) {
+      k_zero_i32<<<1, 1, 0, ctx->stream>>>(n_cut);
+      XG_LAUNCH_CHECK();
+    }
+    return XG_OK;
+  }
+  const int nw = (ne + 31) / 32;
+  const int nblk = (ne + MARK_BLK * MARK_EPT - 1) / (MARK_BLK * MARK_EPT);
+  const bool compact = cut_list || n_cut;
+  int32_t *blk = compact ? scr : nullptr, *sums = scr + nblk + 1;
+  switch (D) {
+    case 1: k_mark<1, MARK_BLK, MARK_EPT><<<nblk, MARK_BLK, 0, ctx->stream>>>(ne, el2v, lset_v, dt, cut_ratio, bits, nw, blk, ctx->d_err); break;
+    case 2: k_mark<2, MARK_BLK, MARK_EPT><<<nblk, MARK_BLK, 0, ctx->stream>>>(ne, el2v, lset_v, dt, cut_ratio, bits, nw, blk, ctx->d_err); break;
+    default: k_mark<3, MARK_BLK, MARK_EPT><<<nblk, MARK_BLK, 0, ctx->stream>>>(ne, el2v, lset_v, dt, cut_ratio, bits, nw, blk, ctx->d_err); break;
+  }
+  XG_LAUNCH_CHECK();
+  if (!compact) return XG_OK;
+  scan_block_counts(ctx->stream, blk, nblk, sums, n_cut);
   XG_LAUNCH_CHECK();
   if (cut_list) {
     k_compact<MARK_BLK, MARK_EPT><<<nblk, MARK_BLK, 0, ctx->stream>>>(ne, bits + (size_t)4 * nw, blk, cut_list);
