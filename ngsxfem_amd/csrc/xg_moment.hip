@@ -327,8 +327,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_assemble_csr(long long nent, int ndof, const int32_t *__restrict__ dofs,
                                                       const T *__restrict__ elmat,
                                                       const int32_t *__restrict__ row_ptr,
-                                                      const int32_t *__restrict__ col_idx, double *values,
-                                                      int32_t *err) {
+                                                      const int32_t *__restrict__ col_idx, double scale,
+                                                      double *values, int32_t *err) {
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nent) return;
   const int nn = ndof * ndof;
@@ -346,7 +346,7 @@ __global__ __launch_bounds__(256) void k_assemble_csr(long long nent, int ndof, 
     atomicOr(err, 4);  // entry missing from the sparsity pattern
     return;
   }
-  atomicAdd(&values[lo], (double)elmat[q]);
+  atomicAdd(&values[lo], scale * (double)elmat[q]);
 }
 
 template <typename T>
@@ -360,8 +360,9 @@ __global__ __launch_bounds__(256) void k_assemble_vec(long long nent, const int3
 
 }  // namespace
 
-extern "C" int xg_assemble_csr(xg_ctx *ctx, int nel, int ndof, const int32_t *dofs, const void *elmat,
-                               int precision, const int32_t *row_ptr, const int32_t *col_idx, double *values) {
+extern "C" int xg_assemble_csr_scaled(xg_ctx *ctx, int nel, int ndof, const int32_t *dofs, const void *elmat,
+                                      int precision, double scale, const int32_t *row_ptr, const int32_t *col_idx,
+                                      double *values) {
   if (!ctx || nel < 0 || ndof < 1 || ndof > 64 || (precision != XG_PREC_FP32 && precision != XG_PREC_FP64))
     return set_error(XG_ERR_ARG, "xg_assemble_csr: bad argument");
   if (nel == 0) return XG_OK;
@@ -371,12 +372,17 @@ extern "C" int xg_assemble_csr(xg_ctx *ctx, int nel, int ndof, const int32_t *do
   if (blocks > 0x7fffffffLL) return set_error(XG_ERR_ARG, "xg_assemble_csr: too many entries");
   if (precision == XG_PREC_FP64)
     k_assemble_csr<double><<<(unsigned)blocks, 256, 0, ctx->stream>>>(nent, ndof, dofs, (const double *)elmat, row_ptr,
-                                                                      col_idx, values, ctx->d_err);
+                                                                      col_idx, scale, values, ctx->d_err);
   else
     k_assemble_csr<float><<<(unsigned)blocks, 256, 0, ctx->stream>>>(nent, ndof, dofs, (const float *)elmat, row_ptr,
-                                                                     col_idx, values, ctx->d_err);
+                                                                     col_idx, scale, values, ctx->d_err);
   XG_LAUNCH_CHECK();
   return XG_OK;
+}
+
+extern "C" int xg_assemble_csr(xg_ctx *ctx, int nel, int ndof, const int32_t *dofs, const void *elmat, int precision,
+                               const int32_t *row_ptr, const int32_t *col_idx, double *values) {
+  return xg_assemble_csr_scaled(ctx, nel, ndof, dofs, elmat, precision, 1.0, row_ptr, col_idx, values);
 }
 
 extern "C" int xg_assemble_vec(xg_ctx *ctx, int nel, int ndof, const int32_t *dofs, const void *elvec, int precision,

@@ -29,8 +29,8 @@ from ..meshes import Make1DMesh, MakeStructured2DMesh, MakeStructured3DMesh, Mes
 load_hip_runtime()  # before the host module loads libxg.so: one HIP runtime per process
 
 from .xfem import (ANY, BND, CDOM_IF, CDOM_NEG, CDOM_POS, FALLBACK, FIRST, HASNEG, HASPOS, IF, NEG, NO,  # noqa: F401
-                   OPTIMAL, POS, UNCUT, VOL, AssembleCSR, AssembleVector, BitArray, COMBINED_DOMAIN_TYPE,
-                   CutElementMatrices, CutElementVectors, CutInfo, DOMAIN_TYPE, GetDofsOfElements,
+                   OPTIMAL, POS, UNCUT, VOL, AssembleCSR, AssembleCutFormsDevice, AssembleVector, BitArray,
+                   COMBINED_DOMAIN_TYPE, CutElementMatrices, CutElementVectors, CutInfo, DOMAIN_TYPE, GetDofsOfElements,
                    GetDofsOfFacets, GetElementsWithNeighborFacets, GetElementsWithSharedVertex,
                    GetFacetsWithNeighborTypes, IntegrateX, MlsetElementMatrices, MultiLevelsetCutInfo, NODE_TYPE,
                    QuadElementMatrices, SubdivElementMatrices,
@@ -499,14 +499,19 @@ def _csr_pattern(dofs, n):
 
 
 class BilinearForm:
-    """Collects cut integrals; Assemble() computes the element matrices on the
-    GPU (xg_cut_pipeline) and scatters them on the GPU (xg_assemble_csr) into
-    a CSR matrix (scipy) of the P1 or P2 space."""
+    """Collects cut integrals; Assemble() builds the CSR matrix (scipy) of the
+    P1 or P2 space.  On simplex meshes with one level set the whole assembly
+    stays on the GPU (AssembleCutFormsDevice: the matrix graph of the dof
+    table, the element matrices of every integral and their scatter); other
+    meshes and several level sets compute the element matrices on the GPU and
+    scatter them there from host copies.  `elmats` gives the element matrices
+    of each integral on request (computed again on the GPU)."""
 
     def __init__(self, space, symmetric=False, **kwargs):
         self.space = space
         self.integrals = []
-        self.elmats = []
+        self._elmats = None
+        self._prec = None
         self.mat = None
 
     def __iadd__(self, o):
@@ -516,19 +521,56 @@ class BilinearForm:
             raise TypeError("BilinearForm += expects cut integrals")
         return self
 
+    @property
+    def elmats(self):
+        if self._elmats is None:
+            mesh = self.space.mesh
+            self._elmats = [ci.element_matrices(mesh, self.space.order, self._prec) for ci in self.integrals]
+        return self._elmats
+
+    def _device_terms(self, mesh, p):
+        """(lset, dt, order, p, mask, elements, coef, subdivlvl) per integral and
+        form part, or None when an integral needs the host path"""
+        if getattr(mesh, "is_quad", False):
+            return None
+        terms = []
+        for ci in self.integrals:
+            lset, dt, order, els = ci._rule_args(mesh, p)
+            if isinstance(lset, (tuple, list)):
+                return None
+            L = ci.measure.lsetdom.get("subdivlvl", 0) if isinstance(ci.measure, _CutDifferentialSymbol) else 0
+            if (L > 0 or getattr(lset, "order", 1) == 2) and order < 2 * p:
+                return None
+            parts = ci.integrand.parts
+            mass, lap = "mass" in parts, "laplace" in parts
+            if mass and lap and parts["mass"] == parts["laplace"]:
+                terms.append((lset, int(dt), int(order), p, 3, els, float(parts["mass"]), int(L)))
+            else:
+                if mass:
+                    terms.append((lset, int(dt), int(order), p, 1, els, float(parts["mass"]), int(L)))
+                if lap:
+                    terms.append((lset, int(dt), int(order), p, 2, els, float(parts["laplace"]), int(L)))
+        return terms
+
     def Assemble(self, precision=None):
         from scipy.sparse import csr_matrix
         prec = ngsxfemglobals.elmat_precision if precision is None else precision
+        self._prec = prec
+        self._elmats = None
         mesh = self.space.mesh
         p = self.space.order
-        self.elmats = [ci.element_matrices(mesh, p, prec) for ci in self.integrals]
         if p in (1, 2):
             dofs, n = self.space.dofs()
-            row_ptr, col_idx = _csr_pattern(dofs, n)
-            values = np.zeros(len(col_idx))
-            for els, mats in self.elmats:
-                values += AssembleCSR(np.ascontiguousarray(dofs[els]), np.asarray(mats, dtype=np.float64),
-                                      row_ptr, col_idx)
+            terms = self._device_terms(mesh, p)
+            if terms is not None:
+                row_ptr, col_idx, values = AssembleCutFormsDevice(mesh, np.ascontiguousarray(dofs, dtype=np.int32),
+                                                                  int(n), terms, int(prec))
+            else:
+                row_ptr, col_idx = _csr_pattern(dofs, n)
+                values = np.zeros(len(col_idx))
+                for els, mats in self.elmats:
+                    values += AssembleCSR(np.ascontiguousarray(dofs[els]), np.asarray(mats, dtype=np.float64),
+                                          row_ptr, col_idx)
             self.mat = csr_matrix((values, col_idx, row_ptr), shape=(n, n))
         return self
 

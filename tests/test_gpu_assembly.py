@@ -21,6 +21,16 @@ NEG, POS, IF = 0, 1, 2
 
 
 @pytest.fixture(scope="module")
+def dev_abi():
+    import torch
+    assert torch.cuda.is_available()
+    from ngsxfem_amd import _abi
+    ctx = _abi.Context(0)
+    yield torch, _abi, ctx
+    ctx.close()
+
+
+@pytest.fixture(scope="module")
 def X():
     import torch
     assert torch.cuda.is_available()
@@ -100,3 +110,38 @@ def test_p2_mass_integrates_quadratics(X):
     cq = xyz[:, 1]
     assert abs(cp @ (m.mat @ cq) - (1.0 / 3.0) * 0.5) < 1e-13
     assert abs(np.ones(n) @ (m.mat @ np.ones(n)) - 1.0) < 1e-13
+
+
+def test_csr_pattern_device_matches_numpy(dev_abi):
+    """xg_csr_pattern (radix sort + unique of the couplings) equals the sorted
+    unique pattern from numpy, skipped (negative) dofs left out; P1 and P2
+    dof tables of a 5^3 Kuhn mesh, and a dof >= n is reported"""
+    torch, _abi, ctx = dev_abi
+    from ngsxfem_amd.meshes import MakeStructured3DMesh
+    mesh = MakeStructured3DMesh(nx=5, ny=5, nz=5)
+    for p in (1, 2):
+        dofs, n = mesh.dofs(p)
+        dofs = dofs.astype(np.int32).copy()
+        dofs[::7, 0] = -1  # skipped dofs
+        nd = dofs.shape[1]
+        rows = np.repeat(dofs, nd, axis=1).ravel().astype(np.int64)
+        cols = np.tile(dofs, (1, nd)).ravel().astype(np.int64)
+        ok = (rows >= 0) & (cols >= 0)
+        key = np.unique(rows[ok] * n + cols[ok])
+        ref_rp = np.zeros(n + 1, np.int64)
+        np.add.at(ref_rp, key // n + 1, 1)
+        ref_rp = np.cumsum(ref_rp)
+        d = torch.from_numpy(dofs).cuda()
+        rp = torch.full((n + 1,), -1, dtype=torch.int32, device="cuda")
+        ci = torch.full((dofs.shape[0] * nd * nd,), -1, dtype=torch.int32, device="cuda")
+        nnz = torch.zeros(1, dtype=torch.int32, device="cuda")
+        ctx.csr_pattern(d, n, rp, ci, nnz)
+        ctx.sync()
+        m = int(nnz.item())
+        assert m == len(key)
+        assert np.array_equal(rp.cpu().numpy(), ref_rp)
+        assert np.array_equal(ci.cpu().numpy()[:m], (key % n).astype(np.int32))
+    bad = torch.from_numpy(np.array([[0, 1, 2, 99]], np.int32)).cuda()
+    ctx.csr_pattern(bad, 10, rp[:11], ci, nnz)
+    with pytest.raises(RuntimeError):
+        ctx.sync()
