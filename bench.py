@@ -199,10 +199,11 @@ def bench_c4(args):
     ctx.close()
 
 
-def c5_fp64_ops(stats, p=3):
+def c5_fp64_ops(stats, launches, p=3):
     """FP64 operations (FMA = 2) of the exact-rule subdivision kernel from its
-    per-element counts (table rows, volume tets and interface triangles
-    integrated point by point, cut leaves), per pass:
+    per-element counts (stats[..., 0..3]: table rows, volume tets and
+    interface triangles integrated point by point, cut leaves) and the number
+    of launches per element (each evaluates the lattice once):
       lattice      969 P2 evaluations (10 shape functions: 22 products for
                    the shapes, 10 FMA) = 42 flops each;
       table rows   NM additions each;
@@ -214,8 +215,7 @@ def c5_fp64_ops(stats, p=3):
     nm = {1: 10, 2: 35, 3: 84}[p]
     nq_v, nq_i = {1: (4, 3), 2: (14, 6), 3: (24, 12)}[p]
     rows, vol, itf, leaves = (float(stats[..., k].sum()) for k in range(4))
-    passes = stats.shape[0] * stats.shape[1]
-    return (passes * 969 * 42 + rows * nm + vol * (nq_v * (27 + 30 + 21 + 2 * nm) + 17)
+    return (launches * 969 * 42 + rows * nm + vol * (nq_v * (27 + 30 + 21 + 2 * nm) + 17)
             + itf * (nq_i * (18 + 30 + 21 + 2 * nm) + 40) + leaves * (48 + 4 * 42))
 
 
@@ -307,18 +307,35 @@ def measure_c5(args):
     ms = g0.elapsed_time(g1) / args.steps
     print("# large mixed step %.3f ms over %d steps" % (ms, args.steps), file=sys.stderr, flush=True)
 
-    # the dominant kernel alone (NEG pass: k_subdiv_exact<2,3,NEG,1>), its
-    # launches back to back between two events on the stream it runs on
-    mom = torch.empty(n_cut * nm, dtype=torch.float64, device=dev)
-    stats = torch.zeros(n_cut * 2 * 4, dtype=torch.int32, device=dev)
-    ms_neg = timed(lambda: ctx.subdiv_moments_mesh(p, L, order, cl, d_el, d_x, lp2, [_abi.XG_NEG], mom), 3)
-    ms_if = timed(lambda: ctx.subdiv_moments_mesh(p, L, order, cl, d_el, d_x, lp2, [_abi.XG_IF], mom), 3)
+    # the dominant kernel alone: the NEG pass k_subdiv_exact<2,3,NEG,1> (the
+    # step runs it and the interface pass), its launches back to back between
+    # two events on the stream it runs on
     mom2 = torch.empty(n_cut * 2 * nm, dtype=torch.float64, device=dev)
+    stats = torch.zeros(n_cut * 2 * 4, dtype=torch.int32, device=dev)
+    ms_neg = timed(lambda: ctx.subdiv_moments_mesh(p, L, order, cl, d_el, d_x, lp2, [_abi.XG_NEG], mom2), 3)
+    ms_if = timed(lambda: ctx.subdiv_moments_mesh(p, L, order, cl, d_el, d_x, lp2, [_abi.XG_IF], mom2), 3)
     ctx.subdiv_moments_mesh(p, L, order, cl, d_el, d_x, lp2, dts, mom2, stats)
     ctx.sync()
     st = stats.cpu().numpy().reshape(n_cut, 2, 4)
-    ops_neg = c5_fp64_ops(st[:, :1])
-    ops_all = c5_fp64_ops(st)
+    ops = c5_fp64_ops(st[:, :1], launches=n_cut)
+    ops_if = c5_fp64_ops(st[:, 1:], launches=n_cut)
+    ms_kern = ms_neg
+    if args.pmc_probe:
+        # counter runs (rocprofv3 --pmc): a 256 MiB device copy of known size
+        # (calibration of FETCH_SIZE / WRITE_SIZE), then 3 kernel launches
+        a = torch.empty(64 << 20, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        for _ in range(3):
+            ctx.subdiv_moments_mesh(p, L, order, cl, d_el, d_x, lp2, [_abi.XG_NEG], mom2)
+        ctx.sync()
+        torch.cuda.synchronize()
+        print(json.dumps({"pmc_probe": True, "copy_bytes": 2 * a.numel() * 4, "cut_elements": n_cut,
+                          "algorithmic_flops_per_launch": ops,
+                          "algorithmic_bytes_per_launch": n_cut * (4 + 80 + 16 + 96 + nm * 8)}), flush=True)
+        ctx.close()
+        return None
 
     # error of the FP32 step against the same pipeline with FP64 output
     ref = torch.empty(n_cut * 2 * nd * nd, dtype=torch.float64, device=dev)
@@ -330,17 +347,21 @@ def measure_c5(args):
     dn = torch.linalg.vector_norm(out.view(-1, nd * nd).double() - r64, dim=1)
     err32 = float((dn[nzr] / rn[nzr]).max().item())
 
-    roof = {"bound": "fp64-valu", "kernel": "k_subdiv_exact<2,3,NEG,1>",
-            "achieved": ops_neg / (ms_neg * 1e-3) / 1e12, "peak": PEAK_FP64_VALU / 1e12, "unit": "TFLOP/s",
-            "frac": ops_neg / (ms_neg * 1e-3) / PEAK_FP64_VALU, "traffic": None,
-            "algorithmic_flops_per_launch": ops_neg, "avg_launch_ms": ms_neg,
+    roof = {"bound": "fp64-valu", "kernel": "k_subdiv_exact<2,3,NEG,1> (the NEG pass)",
+            "achieved": ops / (ms_kern * 1e-3) / 1e12, "peak": PEAK_FP64_VALU / 1e12, "unit": "TFLOP/s",
+            "frac": ops / (ms_kern * 1e-3) / PEAK_FP64_VALU, "traffic": read_traffic("subdiv_c5_pmc.txt"),
+            "traffic_source": "profiles/subdiv_c5_pmc.txt (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the same "
+                              "kernel, copy-calibrated)",
+            "algorithmic_flops_per_launch": ops, "avg_launch_ms": ms_kern,
             "counts_per_cut_element": {"table_rows": float(st[:, 0, 0].mean()),
                                        "volume_tets": float(st[:, 0, 1].mean()),
                                        "interface_triangles": float(st[:, 1, 2].mean()),
                                        "cut_leaves": float(st[:, 0, 3].mean())},
-            "note": "traffic null: the kernel reads 80 B of nodal values and writes 672 B of moments per element "
-                    "(0.3 GB per launch); its bound is the FP64 vector ALU, flops counted from the kernel's own "
-                    "per-element piece counts"}
+            "interface_pass": {"kernel": "k_subdiv_exact<2,3,NEG,2>", "avg_launch_ms": ms_if,
+                               "algorithmic_flops_per_launch": ops_if,
+                               "frac": ops_if / (ms_if * 1e-3) / PEAK_FP64_VALU},
+            "note": "flops counted from the kernel's own per-element piece counts (bench.py c5_fp64_ops); the "
+                    "kernel reads 176 B and writes 672 B per cut element, so HBM is not its bound"}
     res = {"metric": METRIC, "value": n_cut / (ms * 1e-3), "unit": "cut-elem/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "fp64+fp32", "data": "synthetic",
@@ -351,9 +372,8 @@ def measure_c5(args):
                       "cut_elements": n_cut, "neg": int(counts[0]), "pos": int(counts[1])},
            "roofline": roof,
            "max_rel_err_vs_fp64": err32,
-           "paths": {"subdivision_neg_ms": ms_neg, "subdivision_if_ms": ms_if,
-                     "elmat_and_rest_ms": ms - ms_neg - ms_if,
-                     "fp64_tflops_both_passes": ops_all / ((ms_neg + ms_if) * 1e-3) / 1e12},
+           "paths": {"subdivision_neg_pass_ms": ms_neg, "subdivision_if_pass_ms": ms_if,
+                     "elmat_and_rest_ms": ms - ms_neg - ms_if},
            "marking": {"ms": ms_mark, "tets_per_s": ne / (ms_mark * 1e-3), "kernel": "xg_mark_p2 (L=4)"}}
 
     # ---- CPU baseline: the oracle (reference algorithm: materialised level-4
@@ -391,7 +411,9 @@ def measure_c5(args):
 
 
 def bench_c5(args):
-    print(json.dumps(measure_c5(args)), flush=True)
+    res = measure_c5(args)
+    if res is not None:
+        print(json.dumps(res), flush=True)
 
 
 def bench_c3(args):
@@ -748,6 +770,38 @@ def measure_c2(args):
                         "ms": ms_mom, "cut_elem_per_s": n_cut / (ms_mom * 1e-3),
                         "max_rel_err_vs_fp64": max_rel_err(out_mom),
                         "arith": "FP64 moments + FP64 identities, %s store" % args.precision}
+    # the split-BF16 MFMA contraction (XG_PREC_BF16X3) beside FP32 MFMA, on
+    # P2 (the workload's order 4) and on P3 (order 6, 20 dofs) of the same cut
+    # elements, each against the FP64 MFMA result
+    for pp, oo in ((2, 4), (3, 6)):
+        ndp = {2: 10, 3: 20}[pp]
+        r64p = torch.empty(n_cut * 2 * ndp * ndp, dtype=torch.float64, device=dev)
+        ctx.cut_pipeline(D, pp, oo, d_el, d_x, d_l, dts, 3, _abi.XG_PREC_FP64, r64p, elems=cut_list)
+        ctx.sync()
+        rr = r64p.view(-1, ndp * ndp)
+        rn = torch.linalg.vector_norm(rr, dim=1)
+        nzp = rn > 0
+        for name, pr in (("fp32", _abi.XG_PREC_FP32), ("bf16x3", _abi.XG_PREC_BF16X3)):
+            o = torch.empty(n_cut * 2 * ndp * ndp, dtype=torch.float32, device=dev)
+            step = ctx.prepare_cut_pipeline(D, pp, oo, d_el, d_x, d_l, dts, 3, pr, o, elems=cut_list)
+            for _ in range(2):
+                step()
+            ctx.sync()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(5):
+                step()
+            e1.record(stream)
+            ctx.sync()
+            msp = e0.elapsed_time(e1) / 5
+            dd = torch.linalg.vector_norm(o.view(-1, ndp * ndp).double() - rr, dim=1)
+            paths["mfma_p%d_%s" % (pp, name)] = {
+                "kernel": "k_pipeline<3,%d,float%s>" % (pp, ",split-BF16" if name == "bf16x3" else ""),
+                "ms": msp, "cut_elem_per_s": n_cut / (msp * 1e-3),
+                "max_rel_err_vs_fp64": float((dd[nzp] / rn[nzp]).max().item()),
+                "arith": "FP32 MFMA" if name == "fp32" else "split-BF16 hi/mid/lo MFMA (6 products), FP32 sums",
+                "note": "rule order %d, FP64 sums of 8-point groups" % oo}
     best = "moments" if args.method == "moments" else "plain_mfma"
     ms_per_step = paths[best]["ms"]
     value = n_cut / (ms_per_step * 1e-3)
@@ -819,6 +873,8 @@ def main():
     # default: the large mixed workload (the largest configuration) as the
     # headline, the 3D plain workload beside it
     res = measure_c5(args)
+    if res is None:
+        return
     c2 = measure_c2(args)
     res["c2"] = {k: c2[k] for k in ("value", "ms_per_step", "config", "roofline", "cpu_baseline", "paths",
                                     "roofline_plain_mfma", "marking")}

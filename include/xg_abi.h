@@ -286,6 +286,19 @@ int xg_cut_elvec(xg_ctx *ctx, int D, int p, int order, int nel, const int32_t *e
    missing pattern entry is reported by xg_sync (XG_ERR_CAPACITY). */
 int xg_assemble_csr(xg_ctx *ctx, int nel, int ndof, const int32_t *dofs, const void *elmat, int precision,
                     const int32_t *row_ptr, const int32_t *col_idx, double *values);
+/* the same scatter with every element-matrix entry multiplied by scale
+   (the coefficient of the form, e.g. 2 * u * v * dCut(...)) */
+int xg_assemble_csr_scaled(xg_ctx *ctx, int nel, int ndof, const int32_t *dofs, const void *elmat, int precision,
+                           double scale, const int32_t *row_ptr, const int32_t *col_idx, double *values);
+/* Sparsity pattern of the element dof couplings on the device (the matrix
+   graph NGSolve's BilinearForm::Assemble builds before adding the element
+   matrices): every (dofs[e][a], dofs[e][b]) of the nel x ndof table (entries
+   < 0 skipped), sorted and made unique.  row_ptr: int32[n + 1]; col_idx:
+   int32[cap] (columns ascending per row); nnz: device int32[1].  Uses the
+   context workspace (about 32 B per coupling); nnz > cap -> XG_ERR_CAPACITY
+   and a dof >= n -> XG_ERR_ARG at xg_sync.  nel * ndof^2 < 2^31. */
+int xg_csr_pattern(xg_ctx *ctx, int nel, int ndof, const int32_t *dofs, int n, int32_t *row_ptr,
+                   int32_t *col_idx, long long cap, int32_t *nnz);
 int xg_assemble_vec(xg_ctx *ctx, int nel, int ndof, const int32_t *dofs, const void *elvec, int precision,
                     double *out);
 
@@ -322,6 +335,15 @@ int xg_subdiv_moments(xg_ctx *ctx, int nel, const double *lset_nodal, int lset_o
 int xg_subdiv_moments_mesh(xg_ctx *ctx, int p, int L, int order, int nel, const int32_t *elems,
                            const int32_t *el2v, const double *coords, const double *lset_nodal, int ndt,
                            const int *dts, double *moments, int32_t *stats);
+/* The same rule as fixed-node weights on the degree-4 Lagrange lattice of
+   each element (as xg_subdiv_weights, q = 4: they integrate every
+   polynomial of degree <= 4 exactly like the rule) for listed mesh elements
+   and NEG / POS / IF: weights nel x ndt x 35 FP64.  order >= 4; arguments as
+   xg_subdiv_moments_mesh. */
+int xg_subdiv_weights_mesh(xg_ctx *ctx, int L, int order, int nel, const int32_t *elems,
+                           const int32_t *el2v, const double *coords, const double *lset_nodal, int ndt,
+                           const int *dts, double *weights, int32_t *stats);
+
 /* Fused subdivision rule + element matrices on a tetrahedral mesh with a P2
    level set: dCut(lset_p2, dt, subdivlvl=L) forms assembled element by
    element (CutIntegrationRule, xintegration.cpp:1626-1754, with

@@ -63,6 +63,9 @@ _SIGS = {
     "xg_subdiv_pipeline": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _i, _vp, _vp],
     "xg_mark_noalloc": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "xg_learned_moments": [_vp, _i, _i, _vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp],
+    "xg_subdiv_weights_mesh": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp],
+    "xg_assemble_csr_scaled": [_vp, _i, _i, _vp, _vp, _i, ctypes.c_double, _vp, _vp, _vp],
+    "xg_csr_pattern": [_vp, _i, _i, _vp, _i, _vp, _vp, ctypes.c_longlong, _vp],
     "xg_subdiv_moments_mesh": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp],
     "xg_elmat_moments": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "xg_cut_elvec": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp],
@@ -337,6 +340,20 @@ class Context:
         check(lib().xg_subdiv_moments_mesh(self.h, p, L, order, elems.shape[0], ptr(elems), ptr(el2v), ptr(coords),
                                            ptr(lset_nodal), len(dts), ctypes.cast(arr, _vp), ptr(moments),
                                            ptr(stats)))
+
+    def subdiv_weights_mesh(self, L, order, elems, el2v, coords, lset_nodal, dts, weights, stats=None):
+        """degree-4 fixed-node weights of the level-L subdivision rule of the
+        listed elements (xg_subdiv_weights_mesh); weights: nel x ndt x 35"""
+        arr = (ctypes.c_int * len(dts))(*dts)
+        check(lib().xg_subdiv_weights_mesh(self.h, L, order, elems.shape[0], ptr(elems), ptr(el2v), ptr(coords),
+                                           ptr(lset_nodal), len(dts), ctypes.cast(arr, _vp), ptr(weights),
+                                           ptr(stats)))
+
+    def csr_pattern(self, dofs, n, row_ptr, col_idx, nnz):
+        """matrix graph of the element dof table on the device (xg_csr_pattern)"""
+        nel, ndof = dofs.shape
+        check(lib().xg_csr_pattern(self.h, nel, ndof, ptr(dofs), n, ptr(row_ptr), ptr(col_idx), col_idx.numel(),
+                                   ptr(nnz)))
 
     def cut_elvec(self, D, p, order, el2v, coords, lset, dts, precision, out, elems=None):
         nel = elems.shape[0] if elems is not None else el2v.shape[0]

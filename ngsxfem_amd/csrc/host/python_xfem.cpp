@@ -423,8 +423,8 @@ class CutInformation {
   // P2 level sets / subdivlvl > 0 (cutinfo.cpp:80-126 with the subdivision
   // rule): element domains from the lattice check of the level-L rule
   // (xg_mark_p2), cut ratios part_vol[NEG] / (part_vol[NEG] + part_vol[POS])
-  // from the rule's weights on the cut elements (xg_subdiv_weights; the POS
-  // part as the NEG part of -phi); boundary triangles by xg_mark_p2_trig.
+  // from the rule's NEG and POS parts on the cut elements
+  // (xg_subdiv_moments_mesh); boundary triangles by xg_mark_p2_trig.
   void UpdateSubdivision(py::object lset, int L) {
     simplices_only(ma, "CutInfo with a P2 level set / subdivlvl > 0");
     if (L > 5) throw std::runtime_error("CutInfo.Update: subdivlvl <= 5 in this build");
@@ -456,21 +456,20 @@ class CutInformation {
     if (ncut > 0) {
       std::vector<int32_t> cut(ncut);
       cl.download(cut.data(), ncut);
-      std::vector<double> nc_neg((size_t)ncut * 10), nc_pos((size_t)ncut * 10);
-      for (int i = 0; i < ncut; ++i)
-        for (int k = 0; k < 10; ++k) {
-          nc_neg[(size_t)i * 10 + k] = nod[(size_t)cut[i] * 10 + k];
-          nc_pos[(size_t)i * 10 + k] = -nod[(size_t)cut[i] * 10 + k];
-        }
-      DevBuf<double> dn, dp, wn((size_t)ncut * 35), wp((size_t)ncut * 35);
-      dn.upload(nc_neg.data(), nc_neg.size());
-      dp.upload(nc_pos.data(), nc_pos.size());
-      check(xg_subdiv_weights(ctx(), ncut, dn.p, 2, L, 0, 4, nullptr, wn.p, nullptr, nullptr));
-      check(xg_subdiv_weights(ctx(), ncut, dp.p, 2, L, 0, 4, nullptr, wp.p, nullptr, nullptr));
+      // NEG and POS part volumes of the subdivision rule (exact base rules of
+      // degree 2: the degree-2 moments of each side; zero level-set values
+      // count as POS, xintegration.cpp:1159-1164); volume = sum_alpha
+      // (2 / alpha!) m_alpha = the integral of (lam_0 + .. + lam_3)^2
+      DevBuf<int32_t> d_cut;
+      d_cut.upload(cut.data(), cut.size());
+      DevBuf<double> mom((size_t)ncut * 2 * 10);
+      const int sides[2] = {NEG, POS};
+      check(xg_subdiv_moments_mesh(ctx(), 1, L, 2, ncut, d_cut.p, ma->el2v.p, ma->coords.p, d_nod.p, 2, sides, mom.p,
+                                   nullptr));
       check(xg_sync(ctx()));
-      std::vector<double> hn((size_t)ncut * 35), hp((size_t)ncut * 35);
-      wn.download(hn.data(), hn.size());
-      wp.download(hp.data(), hp.size());
+      std::vector<double> hm((size_t)ncut * 2 * 10);
+      mom.download(hm.data(), hm.size());
+      static const double mult[10] = {1, 2, 2, 2, 1, 2, 2, 1, 2, 1};
       // The element domain is decided on the part volumes, not on the
       // lattice (cutinfo.cpp:104-124): a lattice-cut element whose leaves
       // all fall to one side (exact zeros, or values below the leaves'
@@ -479,9 +478,9 @@ class CutInformation {
       bool moved = false;
       for (int i = 0; i < ncut; ++i) {
         double a = 0.0, b = 0.0;
-        for (int k = 0; k < 35; ++k) {
-          a += hn[(size_t)i * 35 + k];
-          b += hp[(size_t)i * 35 + k];
+        for (int k = 0; k < 10; ++k) {
+          a += mult[k] * hm[((size_t)i * 2 + 0) * 10 + k];
+          b += mult[k] * hm[((size_t)i * 2 + 1) * 10 + k];
         }
         cr[cut[i]] = a / (a + b);
         const uint8_t d = a > 0.0 ? (b > 0.0 ? (uint8_t)IF : (uint8_t)NEG) : (uint8_t)POS;
@@ -806,8 +805,9 @@ bool poly_terms_of(py::object cf, std::vector<double> &coef, std::vector<int32_t
 
 // IntegrateX on the subdivision rule (P2 level sets, subdivlvl > 0): the rule
 // of each element contracted to weights at the 35 nodes of the degree-4
-// Lagrange lattice (xg_subdiv_weights), which reproduce the rule's point sum
-// for every polynomial of degree <= 4; the POS part is the NEG part of -phi
+// Lagrange lattice (xg_subdiv_weights_mesh), which reproduce the rule's point
+// sum for every polynomial of degree <= 4; below order 4 (inexact base rules)
+// xg_subdiv_weights with the POS part as the NEG part of -phi
 py::object IntegrateSubdivision(MA ma, py::object lset, int dt, int L, int order, py::object cf, bool element_wise) {
   simplices_only(ma, "IntegrateX with a P2 level set / subdivlvl > 0");
   std::vector<double> coef;
@@ -822,15 +822,22 @@ py::object IntegrateSubdivision(MA ma, py::object lset, int dt, int L, int order
   std::vector<int32_t> all(ne);
   for (int e = 0; e < ne; ++e) all[e] = e;
   std::vector<double> nod = p2_nodal_values(ma, lset, all);
-  if (dt == POS)
-    for (auto &v : nod) v = -v;
+  const bool exact = order >= 4;  // exact base rules: the mesh entry, POS kept as such
+  if (dt == POS && !exact)
+    for (auto &v : nod) v = -v;  // lower orders: the POS part as the NEG part of -phi
   DevBuf<double> d_nod, d_fi, W((size_t)std::max(ne, 1) * 35), d_pe(std::max(ne, 1)), d_tot(1);
   d_nod.upload(nod.data(), nod.size());
-  if (dt == IF) {
-    std::vector<double> fi = tet_finv(ma, all);
-    d_fi.upload(fi.data(), fi.size());
-  }
-  if (ne) {
+  if (ne && exact) {
+    DevBuf<int32_t> d_all;
+    d_all.upload(all.data(), all.size());
+    const int dts[1] = {dt};
+    check(xg_subdiv_weights_mesh(ctx(), L, order, ne, d_all.p, ma->el2v.p, ma->coords.p, d_nod.p, 1, dts, W.p,
+                                 nullptr));
+  } else if (ne) {
+    if (dt == IF) {
+      std::vector<double> fi = tet_finv(ma, all);
+      d_fi.upload(fi.data(), fi.size());
+    }
     check(xg_subdiv_weights(ctx(), ne, d_nod.p, 2, L, order, 4, dt == IF ? d_fi.p : nullptr,
                             dt == IF ? nullptr : W.p, dt == IF ? W.p : nullptr, nullptr));
   }
@@ -846,8 +853,10 @@ py::object IntegrateSubdivision(MA ma, py::object lset, int dt, int L, int order
 }
 
 // element matrices on the subdivision rule (dCut with a P2 level set or
-// subdivlvl > 0): degree-2p moments of the rule (xg_subdiv_moments) on the
-// listed tetrahedra, then the moment-form identities (xg_elmat_moments)
+// subdivlvl > 0): with exact base rules (order >= 2p) the fused mesh entry
+// xg_subdiv_pipeline; below, the degree-2p moments of the rule
+// (xg_subdiv_moments) on the listed tetrahedra, then the moment-form
+// identities (xg_elmat_moments)
 py::tuple SubdivElementMatrices(py::object mesh, py::object lset, int dt, int L, int order, int p, int mask,
                                 py::object elements, int precision) {
   MA ma = as_mesh(mesh);
@@ -869,9 +878,40 @@ py::tuple SubdivElementMatrices(py::object mesh, py::object lset, int dt, int L,
   const int nel = (int)el.size();
   const int nm = (2 * p + 3) * (2 * p + 2) * (2 * p + 1) / 6;
   const int nd = (p + 1) * (p + 2) * (p + 3) / 6;
+  if (order >= 2 * p) {
+    // exact base rules: the fused mesh entry (nodal values of all elements,
+    // indexed by element id; the POS side keeps zero values as the
+    // reference's leaves do)
+    std::vector<int32_t> all(ma->ne);
+    for (int e = 0; e < ma->ne; ++e) all[e] = e;
+    std::vector<double> nod_all = p2_nodal_values(ma, lset, all);
+    DevBuf<double> d_all;
+    d_all.upload(nod_all.data(), nod_all.size());
+    DevBuf<int32_t> d_el;
+    d_el.upload(el.data(), el.size());
+    py::array_t<int32_t> elarr((py::ssize_t)nel, el.data());
+    const int dts[1] = {dt};
+    const size_t cnt = (size_t)nel * nd * nd;
+    if (precision == XG_PREC_FP64) {
+      DevBuf<double> o(std::max(cnt, (size_t)1));
+      if (nel) check(xg_subdiv_pipeline(ctx(), p, L, order, nel, d_el.p, ma->el2v.p, ma->coords.p, d_all.p, 1, dts,
+                                        mask, precision, o.p, nullptr));
+      check(xg_sync(ctx()));
+      py::array_t<double> h({(py::ssize_t)nel, (py::ssize_t)nd, (py::ssize_t)nd});
+      o.download(h.mutable_data(), cnt);
+      return py::make_tuple(elarr, h);
+    }
+    DevBuf<float> o(std::max(cnt, (size_t)1));
+    if (nel) check(xg_subdiv_pipeline(ctx(), p, L, order, nel, d_el.p, ma->el2v.p, ma->coords.p, d_all.p, 1, dts,
+                                      mask, XG_PREC_FP32, o.p, nullptr));
+    check(xg_sync(ctx()));
+    py::array_t<float> h({(py::ssize_t)nel, (py::ssize_t)nd, (py::ssize_t)nd});
+    o.download(h.mutable_data(), cnt);
+    return py::make_tuple(elarr, h);
+  }
   std::vector<double> nod = p2_nodal_values(ma, lset, el);
   if (dt == POS)
-    for (auto &v : nod) v = -v;
+    for (auto &v : nod) v = -v;  // inexact base rules: the POS part as the NEG part of -phi
   std::vector<double> fi = tet_finv(ma, el);
   DevBuf<double> d_nod, d_fi, d_m((size_t)std::max(nel, 1) * nm);
   DevBuf<int32_t> d_el;
@@ -1225,6 +1265,102 @@ py::array_t<double> AssembleCSR(py::array_t<int32_t, py::array::c_style | py::ar
   return h;
 }
 
+// BilinearForm.Assemble with everything on the device.  NGSolve's
+// BilinearForm::Assemble builds the matrix graph of the space and adds every
+// element's CalcElementMatrix (symboliccutbfi.cpp:37-46) into it; here the
+// graph of the whole element dof table is built on the GPU
+// (xg_csr_pattern), each cut integral's element matrices stay in device
+// memory (xg_cut_pipeline for P1 level sets, xg_subdiv_pipeline for P2 ones
+// or subdivlvl > 0 with exact base rules) and are scattered with their
+// coefficient (xg_assemble_csr_scaled).  Only the CSR arrays return.
+// terms: [(lset, dt, order, p, mask, elements, coef, subdivlvl), ...];
+// precision: the element-matrix arithmetic of xg_cut_pipeline (P1 terms).
+py::tuple AssembleCutFormsDevice(py::object mesh, py::array_t<int32_t, py::array::c_style | py::array::forcecast> dofs,
+                                 int n, py::list terms, int precision) {
+  MA ma = as_mesh(mesh);
+  simplices_only(ma, "AssembleCutFormsDevice");
+  if (dofs.ndim() != 2) throw std::runtime_error("AssembleCutFormsDevice: dofs must be ne x ndof");
+  const int ne = (int)dofs.shape(0), nd = (int)dofs.shape(1);
+  if (ne != ma->ne) throw std::runtime_error("AssembleCutFormsDevice: one dof row per element");
+  DevBuf<int32_t> d_dofs, rp((size_t)n + 1), nnz(1);
+  d_dofs.upload(dofs.data(), dofs.size());
+  const long long cap = (long long)ne * nd * nd;
+  DevBuf<int32_t> ci((size_t)std::max(cap, 1LL));
+  check(xg_csr_pattern(ctx(), ne, nd, d_dofs.p, n, rp.p, ci.p, cap, nnz.p));
+  check(xg_sync(ctx()));
+  int32_t h_nnz = 0;
+  nnz.download(&h_nnz, 1);
+  DevBuf<double> vals((size_t)std::max(h_nnz, 1));
+  if (h_nnz) hipcheck(hipMemset(vals.p, 0, sizeof(double) * h_nnz), "hipMemset");
+  const int32_t *hd = dofs.data();
+  for (py::handle th : terms) {
+    py::tuple t = th.cast<py::tuple>();
+    py::object lset = t[0];
+    const int dt = t[1].cast<int>(), order = t[2].cast<int>(), p = t[3].cast<int>(), mask = t[4].cast<int>();
+    const double coef = t[6].cast<double>();
+    const int L = t[7].cast<int>();
+    std::vector<int32_t> el = element_list(t[5], ma->ne);
+    const int nel = (int)el.size();
+    if (nel == 0) continue;
+    int pd = 1;
+    for (int k = 1; k <= ma->dim; ++k) pd = pd * (p + k) / k;
+    if (pd != nd) throw std::runtime_error("AssembleCutFormsDevice: the form's order does not match the dof table");
+    std::vector<int32_t> edofs((size_t)nel * nd);
+    for (int i = 0; i < nel; ++i)
+      std::copy(hd + (size_t)el[i] * nd, hd + (size_t)(el[i] + 1) * nd, edofs.begin() + (size_t)i * nd);
+    DevBuf<int32_t> d_el, d_edofs;
+    d_el.upload(el.data(), el.size());
+    d_edofs.upload(edofs.data(), edofs.size());
+    const int dts[1] = {dt};
+    const bool subdiv = needs_subdivision(lset, L);
+    int mprec = precision & ~XG_METHOD_MOMENTS;
+    if (subdiv) {
+      if (order < 2 * p)
+        throw std::runtime_error("AssembleCutFormsDevice: subdivision rules with order >= 2p (exact base rules)");
+      mprec = mprec == XG_PREC_FP64 ? XG_PREC_FP64 : XG_PREC_FP32;
+    }
+    const bool f64 = mprec == XG_PREC_FP64;
+    DevBuf<double> o64;
+    DevBuf<float> o32;
+    void *out = nullptr;
+    if (f64) {
+      o64.resize((size_t)nel * nd * nd);
+      out = o64.p;
+    } else {
+      o32.resize((size_t)nel * nd * nd);
+      out = o32.p;
+    }
+    if (subdiv) {
+      std::vector<int32_t> all(ma->ne);
+      for (int e = 0; e < ma->ne; ++e) all[e] = e;
+      std::vector<double> nod = p2_nodal_values(ma, lset, all);
+      DevBuf<double> d_nod;
+      d_nod.upload(nod.data(), nod.size());
+      check(xg_subdiv_pipeline(ctx(), p, L, order, nel, d_el.p, ma->el2v.p, ma->coords.p, d_nod.p, 1, dts, mask, mprec,
+                               out, nullptr));
+      check(xg_assemble_csr_scaled(ctx(), nel, nd, d_edofs.p, out, mprec, coef, rp.p, ci.p, vals.p));
+      check(xg_sync(ctx()));
+    } else {
+      bool is_p1 = true;
+      std::vector<double> v = p1_values(lset, ma->nv, is_p1);
+      DevBuf<double> d_l;
+      d_l.upload(v.data(), v.size());
+      check(xg_cut_pipeline(ctx(), ma->dim, p, order, nel, d_el.p, ma->el2v.p, ma->coords.p, d_l.p, 1, dts, mask,
+                            precision, out));
+      check(xg_assemble_csr_scaled(ctx(), nel, nd, d_edofs.p, out, f64 ? XG_PREC_FP64 : XG_PREC_FP32, coef, rp.p,
+                                   ci.p, vals.p));
+      check(xg_sync(ctx()));
+    }
+  }
+  check(xg_sync(ctx()));
+  py::array_t<int32_t> h_rp((py::ssize_t)n + 1), h_ci((py::ssize_t)h_nnz);
+  py::array_t<double> h_v((py::ssize_t)h_nnz);
+  rp.download(h_rp.mutable_data(), (size_t)n + 1);
+  ci.download(h_ci.mutable_data(), (size_t)h_nnz);
+  vals.download(h_v.mutable_data(), (size_t)h_nnz);
+  return py::make_tuple(h_rp, h_ci, h_v);
+}
+
 py::array_t<double> AssembleVector(py::array_t<int32_t, py::array::c_style | py::array::forcecast> dofs,
                                    py::array_t<double, py::array::c_style | py::array::forcecast> elvecs, int n) {
   const int nel = (int)dofs.shape(0), nd = (int)dofs.shape(1);
@@ -1484,6 +1620,9 @@ PYBIND11_MODULE(xfem, m) {
   m.def("CutElementVectors", &CutElementVectors, py::arg("mesh"), py::arg("levelset"), py::arg("domain_type"),
         py::arg("order"), py::arg("p"), py::arg("elements") = py::none(),
         "element vectors of the source term f = 1 on the straight cut rule (FP64)");
+  m.def("AssembleCutFormsDevice", &AssembleCutFormsDevice, py::arg("mesh"), py::arg("dofs"), py::arg("n"),
+        py::arg("terms"), py::arg("precision"),
+        "BilinearForm.Assemble on the device: pattern, element matrices and scatter; returns CSR arrays");
   m.def("AssembleCSR", &AssembleCSR, py::arg("dofs"), py::arg("elmats"), py::arg("row_ptr"), py::arg("col_idx"),
         "device scatter of element matrices into a CSR pattern (FP64 values)");
   m.def("AssembleVector", &AssembleVector, py::arg("dofs"), py::arg("elvecs"), py::arg("n"),

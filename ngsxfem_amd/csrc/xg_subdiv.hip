@@ -782,6 +782,7 @@ int subdiv_moments_impl(xg_ctx *ctx, int nel, const double *lset_nodal, int L, i
 // POS) and / or the interface (WHICH bit 0 / bit 1).
 constexpr int SDX_DMAX = XG_LAT_LMAX;                                   // table depth
 constexpr int SDX_NODES = ((1 << (3 * (SDX_DMAX + 1))) - 1) / 7;        // 37449
+constexpr int SDX_LATQ = 214;  // entries of the level-0..3 lattice offset tables (4 + 10 + 35 + 165)
 constexpr int SDX_CH = 128;                                             // candidates per chunk
 __host__ __device__ constexpr int sdx_lvl_off(int k) { return ((1 << (3 * k)) - 1) / 7; }
 
@@ -843,6 +844,7 @@ struct SdxLds {
   uint16_t *cur, *nxt;                   // sd_maxcut(L) paths
   uint16_t *fillq;                       // 2 SDX_CH table rows
   uint16_t *leafq;                       // SDX_CH
+  uint32_t *latq;                        // lattice offsets of levels 0..3 (the pair checks)
 };
 
 __host__ __device__ inline size_t sdx_bytes(int L, int nq_vol, int nq_if, int NM, SdxLds *s, char *base) {
@@ -867,7 +869,9 @@ __host__ __device__ inline size_t sdx_bytes(int L, int nq_vol, int nq_if, int NM
   const size_t a_nxt = take(sizeof(uint16_t) * sd_maxcut(L));
   const size_t a_fq = take(sizeof(uint16_t) * 2 * SDX_CH);
   const size_t a_lq = take(sizeof(uint16_t) * SDX_CH);
+  const size_t a_lo = take(sizeof(uint32_t) * SDX_LATQ);
   if (s) {
+    s->latq = (uint32_t *)(base + a_lo);
     s->latbits = (uint32_t *)(base + a_lb);
     s->red = (double *)(base + a_red);
     s->rule_vol = (double(*)[4])(base + a_rv);
@@ -1022,7 +1026,7 @@ __device__ __forceinline__ bool lat_bit(const uint32_t *bits, int idx) { return 
 // stats (nullable, int32 x 4 per element): table rows added, volume tets and
 // interface triangles integrated point by point, cut leaves
 template <int LO, int PM, int SIDE, int WHICH>
-__global__ __launch_bounds__(64) void k_subdiv_exact(int nel, const int32_t *__restrict__ elems,
+__global__ __launch_bounds__(64, 2) void k_subdiv_exact(int nel, const int32_t *__restrict__ elems,
                                                      const double *__restrict__ lset_nodal,
                                                      const int32_t *__restrict__ el2v,
                                                      const double *__restrict__ coords,
@@ -1056,6 +1060,7 @@ __global__ __launch_bounds__(64) void k_subdiv_exact(int nel, const int32_t *__r
       if (q < vol_nq) S.rule_vol[q][c] = src_v[q][c];
       else S.rule_if[q - vol_nq][c] = src_i[q - vol_nq][c];
     }
+    for (int k = lane; k < SDX_LATQ; k += 64) S.latq[k] = xg_lat_d[k];
     wave_sync();
   }
   const double inv_n = 1.0 / n;
@@ -1105,12 +1110,12 @@ __global__ __launch_bounds__(64) void k_subdiv_exact(int nel, const int32_t *__r
       }
     }
     double mvol[(WHICH & 1) ? NM : 1], mif[(WHICH & 2) ? NM : 1];
+    // table sums, lane-parallel over the moments: lane and lane + 64
+    double tacc0 = 0.0, tacc1 = 0.0;
 #pragma unroll
     for (int k = 0; k < ((WHICH & 1) ? NM : 1); ++k) mvol[k] = 0.0;
 #pragma unroll
     for (int k = 0; k < ((WHICH & 2) ? NM : 1); ++k) mif[k] = 0.0;
-    // table rows summed lane-parallel: lane holds moments lane and lane + 64
-    double tacc0 = 0.0, tacc1 = 0.0;
     int st_rows = 0, st_vol = 0, st_if = 0, st_leaf = 0;
     bool ok = true;
     int dom = -1;
@@ -1136,7 +1141,7 @@ __global__ __launch_bounds__(64) void k_subdiv_exact(int nel, const int32_t *__r
         if (r > 0) {
           for (int q = lane; q < nc * np; q += 64) {
             const int c = q / np, t = q - c * np;
-            const unsigned pk = xg_lat_d[poff + t];
+            const unsigned pk = (r <= 3) ? S.latq[poff + t] : xg_lat_d[poff + t];
             const int I0 = (int)(pk & 255u), I1 = (int)((pk >> 8) & 255u), I2 = (int)(pk >> 16);
             int W[4][3];
             unpack_uint4(S.nodeV[c], shift, W);
@@ -1339,8 +1344,9 @@ __global__ __launch_bounds__(64) void k_subdiv_exact(int nel, const int32_t *__r
           wave_sync();
         }
         // table rows of the chunk: uncut nodes of the kept side and the
-        // complemented leaves
-        if (WHICH & 1) {
+        // complemented leaves, summed lane-parallel over the moments (lane
+        // holds moments lane and lane + 64), four rows per round trip
+        if constexpr ((WHICH & 1) != 0) {
           st_rows += nfill;
           int f = 0;
           for (; f + 4 <= nfill; f += 4) {
@@ -1362,9 +1368,9 @@ __global__ __launch_bounds__(64) void k_subdiv_exact(int nel, const int32_t *__r
             tacc1 += (b0 + b1) + (b2 + b3);
           }
           for (; f < nfill; ++f) {
-            const int r0 = S.fillq[f];
-            if (lane < NM) tacc0 += nmom[(size_t)r0 * NM + lane];
-            if (lane + 64 < NM) tacc1 += nmom[(size_t)r0 * NM + 64 + lane];
+            const size_t row = (size_t)S.fillq[f] * NM;
+            if (lane < NM) tacc0 += nmom[row + lane];
+            if (lane + 64 < NM) tacc1 += nmom[row + 64 + lane];
           }
         }
         wave_sync();
@@ -1595,6 +1601,25 @@ extern "C" int xg_subdiv_moments_mesh(xg_ctx *ctx, int p, int L, int order, int 
   if (!elems || !el2v || !coords || !lset_nodal || !moments)
     return set_error(XG_ERR_ARG, "xg_subdiv_moments_mesh: null pointer (elems, el2v, coords, lset_nodal, moments)");
   return subdiv_mesh_moments(ctx, p, L, nel, elems, el2v, coords, lset_nodal, ndt, dts, moments, stats);
+}
+
+extern "C" int xg_subdiv_weights_mesh(xg_ctx *ctx, int L, int order, int nel, const int32_t *elems,
+                                      const int32_t *el2v, const double *coords, const double *lset_nodal, int ndt,
+                                      const int *dts, double *weights, int32_t *stats) {
+  if (!ctx) return set_error(XG_ERR_ARG, "xg_subdiv_weights_mesh: null context");
+  int rc = subdiv_mesh_check("xg_subdiv_weights_mesh", 2, L, order, nel, ndt, dts);
+  if (rc != XG_OK || nel == 0) return rc;
+  if (!elems || !el2v || !coords || !lset_nodal || !weights)
+    return set_error(XG_ERR_ARG, "xg_subdiv_weights_mesh: null pointer (elems, el2v, coords, lset_nodal, weights)");
+  rc = subdiv_mesh_moments(ctx, 2, L, nel, elems, el2v, coords, lset_nodal, ndt, dts, weights, stats);
+  if (rc != XG_OK) return rc;
+  // degree-4 moments -> weights of the degree-4 lattice, each (element, dt)
+  // row in place
+  const long long rows = (long long)nel * ndt;
+  if (rows > 0x7fffffffLL) return set_error(XG_ERR_ARG, "xg_subdiv_weights_mesh: too many elements");
+  k_moments_to_nodal4<<<(unsigned)((rows + 127) / 128), 128, 0, ctx->stream>>>((int)rows, weights);
+  XG_LAUNCH_CHECK();
+  return XG_OK;
 }
 
 extern "C" int xg_subdiv_pipeline(xg_ctx *ctx, int p, int L, int order, int nel, const int32_t *elems,

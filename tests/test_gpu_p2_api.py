@@ -157,3 +157,60 @@ def test_cutinfo_p2_boundary_elements(X, orc, L, order):
     for dt, cdt in ((NEG, X.NEG), (POS, X.POS), (IF, X.IF)):
         assert np.array_equal(ci.GetElementsOfType(cdt, X.BND).numpy(), want == dt), dt
     assert np.abs(np.asarray(ci.GetCutRatios(X.BND)) - wcr).max() <= 1e-12
+
+
+@pytest.mark.parametrize("L", [2, 3])
+def test_cutinfo_p2_zero_planes_count_as_pos(X, orc, L):
+    """A P2 level set that vanishes on whole lattice planes, x * (y - h) with
+    h = 2^-L scaled to a cell of the mesh: subtrees whose lattice values are
+    all exactly zero belong to POS (xintegration.cpp:1159-1164, the lattice
+    check's v >= 0), so NEG + POS part volumes make up every element and the
+    cut ratios and domains are the oracle's (NEG / POS rules of the tree)"""
+    mesh = X.MakeStructured3DMesh(nx=2, ny=2, nz=2)
+    h = 0.5 * 2.0 ** -L
+    g = X.GridFunction(X.H1(mesh, order=2))
+    g.Set(lambda x, y, z: (x - 0.5) * (y - h))
+    dofs, _ = mesh.dofs(2)
+    nod = np.ascontiguousarray(g.vec[dofs])
+    ci = X.CutInfo(mesh, g, subdivlvl=L)
+    cr = np.asarray(ci.GetCutRatios())
+    for e in range(mesh.ne):
+        _, parts = orc.subdiv_rule3(nod[e], 2, L, 0)
+        vn, vp = parts[NEG][1].sum(), parts[POS][1].sum()
+        assert abs(vn + vp - 1.0 / 6.0) < 1e-14
+        want = (IF if vp > 0.0 else NEG) if vn > 0.0 else POS
+        for dt, cdt in ((NEG, X.NEG), (POS, X.POS), (IF, X.IF)):
+            assert ci.GetElementsOfType(cdt).numpy()[e] == (want == dt), (e, dt)
+        assert abs(cr[e] - vn / (vn + vp)) <= 1e-12, e
+    assert (np.asarray(ci.GetElementsOfType(X.IF).numpy())).sum() > 0
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_dcut_p2_pos_keeps_zero_planes(X, orc, p):
+    """dCut(lset_p2, POS, subdivlvl=L) on the same zero planes: the POS
+    element matrices are the oracle's POS rule's, and NEG + POS = the uncut
+    matrix of every element"""
+    L = 2
+    mesh = X.MakeStructured3DMesh(nx=2, ny=2, nz=2)
+    g = X.GridFunction(X.H1(mesh, order=2))
+    g.Set(lambda x, y, z: (x - 0.5) * (y - 0.125))
+    dofs, _ = mesh.dofs(2)
+    nod = np.ascontiguousarray(g.vec[dofs])
+    V_ = X.H1(mesh, order=p)
+    u, v = V_.TnT()
+    mats = {}
+    for dt in (X.NEG, X.POS):
+        a = X.BilinearForm(V_)
+        a += (X.grad(u) * X.grad(v) + u * v) * X.dCut(g, dt, subdivlvl=L, order=2 * p)
+        a.Assemble(precision=X.PREC_FP64)
+        e_ids, m = a.elmats[0]
+        mats[int(dt)] = np.asarray(m)
+    for e in range(mesh.ne):
+        Vx = mesh.coords[mesh.el2v[e]]
+        F, Fi, ad = orc.jacobian(Vx)
+        _, parts = orc.subdiv_rule3(nod[e], 2, L, 2 * p, Finv=Fi)
+        pts, w = parts[POS]
+        ref = orc.elmat(3, p, Vx, np.ascontiguousarray(pts), np.ascontiguousarray(w), mask=3)
+        assert np.abs(mats[POS][e] - ref).max() <= 1e-12 * np.abs(ref).max(), e
+        full = orc.elmat(3, p, Vx, orc.rule(3, 2 * p)[0], orc.rule(3, 2 * p)[1], mask=3)
+        assert np.abs(mats[NEG][e] + mats[POS][e] - full).max() <= 1e-12 * np.abs(full).max(), e
