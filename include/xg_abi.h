@@ -466,6 +466,17 @@ int xg_moment_residual(xg_ctx *ctx, int nel, int nq, const float *pred, double s
 int xg_learned_moments(xg_ctx *ctx, int nel, int p, const double *base, const float *corr,
                        const double *ref2, double thr, double *out, double *rel, uint8_t *accept);
 
+/* Per-kernel timing of this context (a measurement aid; no counterpart in
+   the reference).  While enabled, the subdivision kernels record a HIP event
+   pair on the context's stream around each launch, named "sdx_tree",
+   "sdx_leaves_vol", "sdx_leaves_if", "sdx_rows" and "subdiv_exact" (the
+   one-kernel path).  xg_timing_enable(ctx, 1) clears the records and starts
+   recording, 0 stops; xg_timing_read synchronises the stream and returns the
+   summed milliseconds and the launch count recorded under `name` (0 and 0
+   for a name never recorded). */
+int xg_timing_enable(xg_ctx *ctx, int on);
+int xg_timing_read(xg_ctx *ctx, const char *name, double *total_ms, int *launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -65,6 +65,8 @@ _SIGS = {
     "xg_learned_moments": [_vp, _i, _i, _vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp],
     "xg_subdiv_weights_mesh": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp],
     "xg_assemble_csr_scaled": [_vp, _i, _i, _vp, _vp, _i, ctypes.c_double, _vp, _vp, _vp],
+    "xg_timing_enable": [_vp, _i],
+    "xg_timing_read": [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)],
     "xg_csr_pattern": [_vp, _i, _i, _vp, _i, _vp, _vp, ctypes.c_longlong, _vp],
     "xg_subdiv_moments_mesh": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp],
     "xg_elmat_moments": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _vp],
@@ -184,6 +186,16 @@ class Context:
         ne = el2v.shape[0]
         check(lib().xg_mark(self.h, D, ne, ptr(el2v), ptr(lset), ptr(dt), ptr(cut_ratio), ptr(bits),
                             ptr(cut_list), ptr(n_cut)))
+
+    def timing_enable(self, on=True):
+        """per-kernel HIP event timing of the subdivision kernels (cleared when enabled)"""
+        check(lib().xg_timing_enable(self.h, 1 if on else 0))
+
+    def timing_read(self, name):
+        """(summed ms, launches) recorded under a kernel name"""
+        ms, n = ctypes.c_double(0.0), ctypes.c_int(0)
+        check(lib().xg_timing_read(self.h, name.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
 
     def mark_noalloc(self, D, el2v, lset, dt=None, cut_ratio=None, bits=None, cut_list=None, n_cut=None):
         """xg_mark without allocation (fixed device scratch)"""

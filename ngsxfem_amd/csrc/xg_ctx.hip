@@ -29,24 +29,88 @@ int check_hip(hipError_t e, const char *what) {
   return XG_ERR_HIP;
 }
 
-int ensure_ws(xg_ctx *ctx, size_t bytes) {
-  if (ctx->ws_size >= bytes) return XG_OK;
-  if (ctx->ws) {
+int ensure_buf(xg_ctx *ctx, void **buf, size_t *size, size_t bytes) {
+  if (*size >= bytes) return XG_OK;
+  if (*buf) {
     XG_CHECK(hipStreamSynchronize(ctx->stream));
-    XG_CHECK(hipFree(ctx->ws));
-    ctx->ws = nullptr;
-    ctx->ws_size = 0;
+    XG_CHECK(hipFree(*buf));
+    *buf = nullptr;
+    *size = 0;
   }
   size_t sz = 1 << 20;
   while (sz < bytes) sz <<= 1;
-  XG_CHECK(hipMalloc(&ctx->ws, sz));
-  ctx->ws_size = sz;
+  XG_CHECK(hipMalloc(buf, sz));
+  *size = sz;
   return XG_OK;
+}
+
+int ensure_ws(xg_ctx *ctx, size_t bytes) { return ensure_buf(ctx, &ctx->ws, &ctx->ws_size, bytes); }
+
+KernelTimer::KernelTimer(xg_ctx *c, const char *name) : ctx(c) {
+  if (!ctx->timing) return;
+  for (size_t k = 0; k < ctx->tnames.size(); ++k)
+    if (ctx->tnames[k] == name) slot = (int)k;
+  if (slot < 0) {
+    slot = (int)ctx->tnames.size();
+    ctx->tnames.push_back(name);
+    ctx->tevents.emplace_back();
+  }
+  if (hipEventCreate(&a) != hipSuccess || hipEventRecord(a, ctx->stream) != hipSuccess) {
+    if (a) hipEventDestroy(a);
+    a = nullptr;
+  }
+}
+
+KernelTimer::~KernelTimer() {
+  if (!a) return;
+  hipEvent_t b = nullptr;
+  if (hipEventCreate(&b) != hipSuccess || hipEventRecord(b, ctx->stream) != hipSuccess) {
+    if (b) hipEventDestroy(b);
+    hipEventDestroy(a);
+    return;
+  }
+  ctx->tevents[slot].emplace_back(a, b);
+}
+
+void timing_clear(xg_ctx *ctx) {
+  if (ctx->tnames.empty()) return;
+  hipStreamSynchronize(ctx->stream);
+  for (auto &v : ctx->tevents)
+    for (auto &e : v) {
+      hipEventDestroy(e.first);
+      hipEventDestroy(e.second);
+    }
+  ctx->tnames.clear();
+  ctx->tevents.clear();
 }
 
 }  // namespace xg
 
 extern "C" const char *xg_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int xg_timing_enable(xg_ctx *ctx, int on) {
+  if (!ctx) return xg::set_error(XG_ERR_ARG, "xg_timing_enable: null ctx");
+  if (on) xg::timing_clear(ctx);
+  ctx->timing = on != 0;
+  return XG_OK;
+}
+
+extern "C" int xg_timing_read(xg_ctx *ctx, const char *name, double *total_ms, int *launches) {
+  if (!ctx || !name || !total_ms || !launches) return xg::set_error(XG_ERR_ARG, "xg_timing_read: null argument");
+  *total_ms = 0.0;
+  *launches = 0;
+  XG_CHECK(hipStreamSynchronize(ctx->stream));
+  for (size_t k = 0; k < ctx->tnames.size(); ++k) {
+    if (ctx->tnames[k] != name) continue;
+    for (auto &e : ctx->tevents[k]) {
+      float ms = 0.0f;
+      XG_CHECK(hipEventElapsedTime(&ms, e.first, e.second));
+      *total_ms += ms;
+      *launches += 1;
+    }
+  }
+  return XG_OK;
+}
 
 extern "C" int xg_ctx_create(int device, xg_ctx **out) {
   if (!out) return xg::set_error(XG_ERR_ARG, "xg_ctx_create: null out");
@@ -93,7 +157,9 @@ extern "C" int xg_ctx_destroy(xg_ctx *ctx) {
   hipSetDevice(ctx->device);
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
   else hipDeviceSynchronize();
+  xg::timing_clear(ctx);
   if (ctx->ws) hipFree(ctx->ws);
+  if (ctx->sdx_ws) hipFree(ctx->sdx_ws);
   if (ctx->sdx_verts) hipFree(ctx->sdx_verts);
   for (int k = 0; k < 4; ++k)
     if (ctx->sdx_mom[k]) hipFree(ctx->sdx_mom[k]);

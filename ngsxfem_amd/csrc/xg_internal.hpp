@@ -4,6 +4,8 @@
 #include <stdint.h>
 
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "../../include/xg_abi.h"
 
@@ -23,14 +25,33 @@ struct xg_ctx {
   double *sdx_mom[4] = {nullptr, nullptr, nullptr, nullptr};
   // device address of the no-alloc marking scratch on this device
   int32_t *noalloc_scratch = nullptr;
+  // grow-only lists of the staged subdivision kernels (node ids per element)
+  void *sdx_ws = nullptr;
+  size_t sdx_ws_size = 0;
+  // per-kernel timing (xg_timing_enable): event pairs by kernel name
+  bool timing = false;
+  std::vector<std::string> tnames;
+  std::vector<std::vector<std::pair<hipEvent_t, hipEvent_t>>> tevents;
 };
 
 namespace xg {
 
 int set_error(int code, const std::string &msg);
 int check_hip(hipError_t e, const char *what);
+// records a HIP event pair on the context's stream around the launches in
+// its scope while the context's timing is enabled
+struct KernelTimer {
+  xg_ctx *ctx;
+  int slot = -1;
+  hipEvent_t a = nullptr;
+  KernelTimer(xg_ctx *c, const char *name);
+  ~KernelTimer();
+};
+void timing_clear(xg_ctx *ctx);
 // make sure ctx->ws holds at least `bytes`
 int ensure_ws(xg_ctx *ctx, size_t bytes);
+// the same for another grow-only buffer of the context
+int ensure_buf(xg_ctx *ctx, void **buf, size_t *size, size_t bytes);
 // exclusive scan of n int32 values in place (device), total written to
 // data[n] when `with_total` (data must then hold n+1 entries)
 int scan_exclusive_i32(xg_ctx *ctx, int32_t *data, int n, bool with_total);

@@ -34,6 +34,8 @@
 //  4. the per-lane accumulators are reduced across the wave at the end.
 // FP64 throughout; the cut decisions use the same values and comparisons as
 // the reference, so they are identical to the oracle's.
+#include <cstdlib>
+
 #include "moments_table.h"
 #include "xg_device.hpp"
 #include "xg_internal.hpp"
@@ -845,6 +847,9 @@ struct SdxLds {
   uint16_t *fillq;                       // 2 SDX_CH table rows
   uint16_t *leafq;                       // SDX_CH
   uint32_t *latq;                        // lattice offsets of levels 0..3 (the pair checks)
+  double *cl;                            // 20: the element's level-set nodal values
+  double *fi;                            // 10: F^{-1} (row major) and |det F|
+  double *tab;                           // 128: table row sums, moments lane and lane + 64
 };
 
 __host__ __device__ inline size_t sdx_bytes(int L, int nq_vol, int nq_if, int NM, SdxLds *s, char *base) {
@@ -870,8 +875,14 @@ __host__ __device__ inline size_t sdx_bytes(int L, int nq_vol, int nq_if, int NM
   const size_t a_fq = take(sizeof(uint16_t) * 2 * SDX_CH);
   const size_t a_lq = take(sizeof(uint16_t) * SDX_CH);
   const size_t a_lo = take(sizeof(uint32_t) * SDX_LATQ);
+  const size_t a_cl = take(sizeof(double) * 20);
+  const size_t a_fi = take(sizeof(double) * 10);
+  const size_t a_tab = take(sizeof(double) * 128);
   if (s) {
     s->latq = (uint32_t *)(base + a_lo);
+    s->cl = (double *)(base + a_cl);
+    s->fi = (double *)(base + a_fi);
+    s->tab = (double *)(base + a_tab);
     s->latbits = (uint32_t *)(base + a_lb);
     s->red = (double *)(base + a_red);
     s->rule_vol = (double(*)[4])(base + a_rv);
@@ -1026,7 +1037,8 @@ __device__ __forceinline__ bool lat_bit(const uint32_t *bits, int idx) { return 
 // stats (nullable, int32 x 4 per element): table rows added, volume tets and
 // interface triangles integrated point by point, cut leaves
 template <int LO, int PM, int SIDE, int WHICH>
-__global__ __launch_bounds__(64, 2) void k_subdiv_exact(int nel, const int32_t *__restrict__ elems,
+__global__ __launch_bounds__(64, 2) void k_subdiv_exact(int nel, const int32_t *__restrict__ sub,
+                                                     const int32_t *__restrict__ elems,
                                                      const double *__restrict__ lset_nodal,
                                                      const int32_t *__restrict__ el2v,
                                                      const double *__restrict__ coords,
@@ -1064,35 +1076,47 @@ __global__ __launch_bounds__(64, 2) void k_subdiv_exact(int nel, const int32_t *
     wave_sync();
   }
   const double inv_n = 1.0 / n;
-  for (int i = blockIdx.x; i < nel; i += gridDim.x) {
+  for (int j = blockIdx.x; j < nel; j += gridDim.x) {
+    const int i = sub ? sub[j] : j;
     const int e = elems ? elems[i] : i;
-    double cl[NL];
-#pragma unroll
-    for (int k = 0; k < NL; ++k) cl[k] = lset_nodal[(size_t)e * NL + k];
-    double Fi[9];
-    Geo geo;
-    geo.Finv = nullptr;
-    geo.absdet = 1.0;
+    // the element's nodal values and geometry live in LDS (the lanes'
+    // registers hold the moment sums)
+    if (lane < NL) S.cl[lane] = lset_nodal[(size_t)e * NL + lane];
+    if (lane < 128) S.tab[lane] = 0.0;
+    if (lane + 64 < 128) S.tab[lane + 64] = 0.0;
+    bool have_geo = false;
     if (WHICH & 2) {
       if (el2v && coords) {
-        double V[4][3], F[9];
+        have_geo = true;
+        if (lane == 0) {
+          double V[4][3], F[9], Fi[9], ad;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const int v = el2v[(size_t)e * 4 + k];
+          for (int k = 0; k < 4; ++k) {
+            const int v = el2v[(size_t)e * 4 + k];
 #pragma unroll
-          for (int d = 0; d < 3; ++d) V[k][d] = coords[(size_t)v * 3 + d];
+            for (int d = 0; d < 3; ++d) V[k][d] = coords[(size_t)v * 3 + d];
+          }
+          jacobian<3>(V, F, Fi, ad);
+#pragma unroll
+          for (int k = 0; k < 9; ++k) S.fi[k] = Fi[k];
+          S.fi[9] = ad;
         }
-        jacobian<3>(V, F, Fi, geo.absdet);
-        geo.Finv = Fi;
       } else if (finv_all) {
+        have_geo = true;
+        if (lane == 0) {
+          double Fi[9];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) Fi[k] = finv_all[(size_t)i * 9 + k];
-        const double dinv = Fi[0] * (Fi[4] * Fi[8] - Fi[5] * Fi[7]) - Fi[1] * (Fi[3] * Fi[8] - Fi[5] * Fi[6]) +
-                            Fi[2] * (Fi[3] * Fi[7] - Fi[4] * Fi[6]);
-        geo.absdet = 1.0 / fabs(dinv);
-        geo.Finv = Fi;
+          for (int k = 0; k < 9; ++k) Fi[k] = finv_all[(size_t)i * 9 + k];
+          const double dinv = Fi[0] * (Fi[4] * Fi[8] - Fi[5] * Fi[7]) - Fi[1] * (Fi[3] * Fi[8] - Fi[5] * Fi[6]) +
+                              Fi[2] * (Fi[3] * Fi[7] - Fi[4] * Fi[6]);
+#pragma unroll
+          for (int k = 0; k < 9; ++k) S.fi[k] = Fi[k];
+          S.fi[9] = 1.0 / fabs(dinv);
+        }
       }
     }
+    wave_sync();
+    const double *cl = S.cl;
     // 1. signs of the level set on the finest lattice (one ballot per 64 points)
     for (int q0 = 0; q0 < nlat; q0 += 64) {
       const int q = q0 + lane;
@@ -1110,8 +1134,6 @@ __global__ __launch_bounds__(64, 2) void k_subdiv_exact(int nel, const int32_t *
       }
     }
     double mvol[(WHICH & 1) ? NM : 1], mif[(WHICH & 2) ? NM : 1];
-    // table sums, lane-parallel over the moments: lane and lane + 64
-    double tacc0 = 0.0, tacc1 = 0.0;
 #pragma unroll
     for (int k = 0; k < ((WHICH & 1) ? NM : 1); ++k) mvol[k] = 0.0;
 #pragma unroll
@@ -1293,6 +1315,9 @@ __global__ __launch_bounds__(64, 2) void k_subdiv_exact(int nel, const int32_t *
               if (dd0 * cc[0] + dd1 * cc[1] + dd2 * cc[2] >= 0)
 #pragma unroll
                 for (int d = 0; d < 3; ++d) cc[d] *= -1.0;
+              Geo geo;
+              geo.Finv = have_geo ? S.fi : nullptr;
+              geo.absdet = have_geo ? S.fi[9] : 1.0;
               wf = tf * geo.if_factor(cc);
             }
             S.ifw[q] = wf;
@@ -1364,13 +1389,13 @@ __global__ __launch_bounds__(64, 2) void k_subdiv_exact(int nel, const int32_t *
               b2 = nmom[(size_t)r2 * NM + 64 + lane];
               b3 = nmom[(size_t)r3 * NM + 64 + lane];
             }
-            tacc0 += (a0 + a1) + (a2 + a3);
-            tacc1 += (b0 + b1) + (b2 + b3);
+            S.tab[lane] += (a0 + a1) + (a2 + a3);
+            S.tab[64 + lane] += (b0 + b1) + (b2 + b3);
           }
           for (; f < nfill; ++f) {
             const size_t row = (size_t)S.fillq[f] * NM;
-            if (lane < NM) tacc0 += nmom[row + lane];
-            if (lane + 64 < NM) tacc1 += nmom[row + 64 + lane];
+            if (lane < NM) S.tab[lane] += nmom[row + lane];
+            if (lane + 64 < NM) S.tab[64 + lane] += nmom[row + 64 + lane];
           }
         }
         wave_sync();
@@ -1396,8 +1421,8 @@ __global__ __launch_bounds__(64, 2) void k_subdiv_exact(int nel, const int32_t *
     }
     wave_sync();
     if (WHICH & 1) {
-      if (lane < NM) S.red[lane] += tacc0;
-      if (lane + 64 < NM) S.red[64 + lane] += tacc1;
+      if (lane < NM) S.red[lane] += S.tab[lane];
+      if (lane + 64 < NM) S.red[64 + lane] += S.tab[64 + lane];
     }
     wave_sync();
     for (int k = lane; k < NM; k += 64) {
@@ -1422,15 +1447,17 @@ __global__ __launch_bounds__(64, 2) void k_subdiv_exact(int nel, const int32_t *
 template <int LO, int PM, int SIDE, int WHICH>
 int launch_subdiv_exact(xg_ctx *ctx, int nel, const int32_t *elems, const double *lset_nodal, const int32_t *el2v,
                         const double *coords, const double *finv, int L, double *out_vol, double *out_if,
-                        int out_stride, uint8_t *dom, int32_t *stats, int stats_stride) {
+                        int out_stride, uint8_t *dom, int32_t *stats, int stats_stride,
+                        const int32_t *sub = nullptr) {
   constexpr int NM = Moments<3, PM>::NM;
   const size_t lds = sdx_bytes(L, xg_minrule_cnt[3][PM], xg_minrule_cnt[2][PM], NM, nullptr, nullptr);
   if (lds > 65536)
     XG_CHECK(hipFuncSetAttribute((const void *)k_subdiv_exact<LO, PM, SIDE, WHICH>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const int blocks = nel < (1 << 22) ? nel : (1 << 22);
+  KernelTimer tm(ctx, "subdiv_exact");
   k_subdiv_exact<LO, PM, SIDE, WHICH><<<blocks, 64, lds, ctx->stream>>>(
-      nel, elems, lset_nodal, el2v, coords, finv, L, (const uint4 *)ctx->sdx_verts, ctx->sdx_mom[PM], out_vol,
+      nel, sub, elems, lset_nodal, el2v, coords, finv, L, (const uint4 *)ctx->sdx_verts, ctx->sdx_mom[PM], out_vol,
       out_if, out_stride, dom, stats, stats_stride, ctx->d_err);
   XG_LAUNCH_CHECK();
   return XG_OK;
@@ -1460,14 +1487,15 @@ int ensure_sdx_tables(xg_ctx *ctx, int PM) {
 template <int LO, int PM>
 int subdiv_exact_pass(xg_ctx *ctx, int nel, const int32_t *elems, const double *lset_nodal, const int32_t *el2v,
                       const double *coords, const double *finv, int L, int vol_side, bool with_if, double *out_vol,
-                      double *out_if, int out_stride, uint8_t *dom, int32_t *stats, int stats_stride) {
+                      double *out_if, int out_stride, uint8_t *dom, int32_t *stats, int stats_stride,
+                      const int32_t *sub = nullptr) {
   if (vol_side >= 0) {
     const int rc = ensure_sdx_tables(ctx, PM);
     if (rc != XG_OK) return rc;
   }
 #define XG_SDX(SIDE, WHICH)                                                                                        \
   return launch_subdiv_exact<LO, PM, SIDE, WHICH>(ctx, nel, elems, lset_nodal, el2v, coords, finv, L, out_vol,    \
-                                                  out_if, out_stride, dom, stats, stats_stride)
+                                                  out_if, out_stride, dom, stats, stats_stride, sub)
   if (vol_side == NEG && !with_if) XG_SDX(NEG, 1);
   if (vol_side == POS && !with_if) XG_SDX(POS, 1);
   if constexpr (PM < 3) {
@@ -1477,6 +1505,729 @@ int subdiv_exact_pass(xg_ctx *ctx, int nel, const int32_t *elems, const double *
   if (vol_side < 0 && with_if) XG_SDX(NEG, 2);
 #undef XG_SDX
   return set_error(XG_ERR_ARG, "subdivision rule: bad pass");
+}
+
+// ---------------------------------------------------------------- staged path
+// The one-kernel path above keeps the element's moment sums (2 x 84 FP64 for
+// P3) in registers for the whole refinement walk, so the walk's latency-bound
+// integer and LDS work runs at two waves per SIMD.  The staged path splits
+// it into three kernels over a per-element list of uint16 node ids:
+//   1. k_sdx_tree: the walk (lattice sign checks, classification) at high
+//      occupancy; writes the cut leaves from the front of the element's list
+//      and the uncut nodes of the kept side (table rows) from the back
+//   2. k_sdx_leaves: the cut leaves' pieces and their base-rule points
+//      (the register-heavy part), once per requested part (volume side or
+//      interface) on the same lists; appends the complemented leaves after
+//      the cut leaves
+//   3. k_sdx_rows: the table rows (uncut nodes + complemented leaves),
+//      added to the moments
+// An element whose lists do not fit (2 cut leaves + rows > cap) is queued
+// and handled by the one-kernel path.
+
+struct SdcLds {
+  uint32_t *latbits;  // sign bits (v >= 0) of the finest lattice, lattice order
+  uint16_t *rowst;    // (n+1)^2: lattice index of (0, b, c)
+  uint4 *nodeV;       // SDX_CH candidate vertices (level-L units)
+  uint32_t *flags;    // SDX_CH
+  uint16_t *cur, *nxt;
+  uint32_t *latq;     // lattice offsets of levels 0..3
+  double *cl;         // 20
+};
+
+__host__ __device__ inline size_t sdc_bytes(int L, SdcLds *s, char *base) {
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = o;
+    o = sd_align(o + bytes);
+    return at;
+  };
+  const int n1 = (1 << L) + 1;
+  const size_t a_lb = take(sizeof(uint32_t) * 2 * ((sd_nlat(L) + 63) / 64));
+  const size_t a_rs = take(sizeof(uint16_t) * n1 * n1);
+  const size_t a_nv = take(sizeof(uint4) * SDX_CH);
+  const size_t a_fl = take(sizeof(uint32_t) * SDX_CH);
+  const size_t a_cur = take(sizeof(uint16_t) * sd_maxcut(L));
+  const size_t a_nxt = take(sizeof(uint16_t) * sd_maxcut(L));
+  const size_t a_lq = take(sizeof(uint32_t) * SDX_LATQ);
+  const size_t a_cl = take(sizeof(double) * 20);
+  if (s) {
+    s->latbits = (uint32_t *)(base + a_lb);
+    s->rowst = (uint16_t *)(base + a_rs);
+    s->nodeV = (uint4 *)(base + a_nv);
+    s->flags = (uint32_t *)(base + a_fl);
+    s->cur = (uint16_t *)(base + a_cur);
+    s->nxt = (uint16_t *)(base + a_nxt);
+    s->latq = (uint32_t *)(base + a_lq);
+    s->cl = (double *)(base + a_cl);
+  }
+  return o;
+}
+
+// the lattice index of (a, b, c) through the row table (lat_index without
+// the cubic terms)
+__device__ __forceinline__ int lat_index_rs(const uint16_t *rowst, int a, int b, int c, int n) {
+  return (int)rowst[c * (n + 1) + b] + a;
+}
+
+// q / d for 0 <= q < 2^24, d >= 1 (float estimate, one correction step)
+__device__ __forceinline__ int small_div(int q, int d, float rd, int &rem) {
+  int c = (int)((float)q * rd);
+  int t = q - c * d;
+  if (t >= d) {
+    ++c;
+    t -= d;
+  } else if (t < 0) {
+    --c;
+    t += d;
+  }
+  rem = t;
+  return c;
+}
+
+// SIDE: NEG or POS (rows of that side), or -1 (cut leaves only)
+template <int LO, int SIDE>
+__global__ __launch_bounds__(64) void k_sdx_tree(int nel, const int32_t *__restrict__ elems,
+                                                 const double *__restrict__ lset_nodal, int L,
+                                                 const uint4 *__restrict__ nverts, uint16_t *__restrict__ lists,
+                                                 int cap, int4 *__restrict__ cnt, uint8_t *__restrict__ dom_out,
+                                                 int32_t *__restrict__ over, int32_t *__restrict__ nover,
+                                                 int32_t *err) {
+  constexpr int NL = ndof(3, LO);
+  extern __shared__ char smem[];
+  SdcLds S;
+  sdc_bytes(L, &S, smem);
+  const int lane = threadIdx.x & 63;
+  const int n = 1 << L;
+  const int nlat = sd_nlat(L);
+  const int loff = xg_lat_off_d[L];
+  const int shift = SDX_DMAX - L;
+  for (int k = lane; k < SDX_LATQ; k += 64) S.latq[k] = xg_lat_d[k];
+  for (int k = lane; k < (n + 1) * (n + 1); k += 64) {
+    const int c = k / (n + 1), b = k - c * (n + 1);
+    S.rowst[k] = (uint16_t)((b + c <= n) ? lat_index(0, b, c, n) : 0);
+  }
+  wave_sync();
+  const double inv_n = 1.0 / n;
+  for (int i = blockIdx.x; i < nel; i += gridDim.x) {
+    const int e = elems ? elems[i] : i;
+    if (lane < NL) S.cl[lane] = lset_nodal[(size_t)e * NL + lane];
+    wave_sync();
+    const double *cl = S.cl;
+    // signs of the level set on the finest lattice (one ballot per 64 points)
+    for (int q0 = 0; q0 < nlat; q0 += 64) {
+      const int q = q0 + lane;
+      bool nonneg = false;
+      if (q < nlat) {
+        const unsigned pk = xg_lat_d[loff + q];
+        const double x[3] = {(double)(pk & 255u) * inv_n, (double)((pk >> 8) & 255u) * inv_n,
+                             (double)(pk >> 16) * inv_n};
+        nonneg = lagrange_eval3<LO>(cl, x) >= 0.0;
+      }
+      const unsigned long long b = __ballot(nonneg);
+      if (lane == 0) {
+        S.latbits[(q0 >> 5)] = (uint32_t)b;
+        S.latbits[(q0 >> 5) + 1] = (uint32_t)(b >> 32);
+      }
+    }
+    uint16_t *lst = lists + (size_t)i * cap;
+    int nl = 0, nf = 0;
+    bool ok = true, overflow = false;
+    int dom = -1;
+    int ncur = 1;
+    wave_sync();
+    for (int k = 0; k <= L && ncur > 0 && !overflow; ++k) {
+      const int r = L - k;
+      const int ncand = (k == 0) ? 1 : 8 * ncur;
+      const int np = xg_lat_cnt_d[r], poff = xg_lat_off_d[r];
+      const float rnp = 1.0f / (float)np;
+      const int koff = sdx_lvl_off(k);
+      int nnxt = 0;
+      for (int cbase = 0; cbase < ncand && !overflow; cbase += SDX_CH) {
+        const int nc = min(SDX_CH, ncand - cbase);
+        for (int c = lane; c < nc; c += 64) {
+          const int gc = cbase + c;
+          const int path = (k == 0) ? 0 : (int)S.cur[gc >> 3] * 8 + (gc & 7);
+          S.nodeV[c] = nverts[koff + path];
+          S.flags[c] = 0u;
+        }
+        wave_sync();
+        // lattice checks above the leaves: the sign bits of every
+        // (candidate, lattice point) pair, OR-ed per candidate with ballots
+        // (the pairs of one pass cover consecutive candidates)
+        if (r > 0) {
+          const int npairs = nc * np;
+          for (int q0 = 0; q0 < npairs; q0 += 64) {
+            const int q = q0 + lane;
+            int c = -1;
+            uint32_t bit = 0u;
+            if (q < npairs) {
+              int t;
+              c = small_div(q, np, rnp, t);
+              const unsigned pk = (r <= 3) ? S.latq[poff + t] : xg_lat_d[poff + t];
+              const int I0 = (int)(pk & 255u), I1 = (int)((pk >> 8) & 255u), I2 = (int)(pk >> 16);
+              int W[4][3];
+              unpack_uint4(S.nodeV[c], shift, W);
+              int P[3];
+#pragma unroll
+              for (int d = 0; d < 3; ++d)
+                P[d] = W[0][d] + ((I0 * (W[1][d] - W[0][d]) + I1 * (W[2][d] - W[0][d]) + I2 * (W[3][d] - W[0][d])) >> r);
+              bit = lat_bit(S.latbits, lat_index_rs(S.rowst, P[0], P[1], P[2], n)) ? 1u : 2u;
+            }
+            int trem;
+            const int c_lo = small_div(q0, np, rnp, trem);
+            const int qhi = min(q0 + 63, npairs - 1);
+            const int c_hi = small_div(qhi, np, rnp, trem);
+            for (int cc = c_lo; cc <= c_hi; ++cc) {
+              const unsigned long long b1 = __ballot(c == cc && bit == 1u);
+              const unsigned long long b2 = __ballot(c == cc && bit == 2u);
+              if (lane == 0) S.flags[cc] |= (b1 ? 1u : 0u) | (b2 ? 2u : 0u);
+            }
+          }
+          wave_sync();
+        }
+        // classification and the lists
+        for (int c0 = 0; c0 < nc; c0 += 64) {
+          const int c = c0 + lane;
+          const bool valid = c < nc;
+          int W[4][3] = {};
+          if (valid) unpack_uint4(S.nodeV[c], shift, W);
+          uint32_t f = 0u;
+          if (valid) {
+            if (r > 0) f = S.flags[c];
+            else
+#pragma unroll
+              for (int j = 0; j < 4; ++j)
+                f |= lat_bit(S.latbits, lat_index_rs(S.rowst, W[j][0], W[j][1], W[j][2], n)) ? 1u : 2u;
+          }
+          const int cls = (f == 3u) ? IF : ((f == 1u) ? POS : NEG);
+          if (k == 0) dom = __shfl(cls, 0, 64);
+          int lc = -2;
+          if (valid && cls == IF && r == 0) {
+            double vv[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const double x[3] = {(double)W[j][0] * inv_n, (double)W[j][1] * inv_n, (double)W[j][2] * inv_n};
+              vv[j] = lagrange_eval3<LO>(cl, x);
+            }
+            lc = leaf_check(vv);
+            if (lc < 0) ok = false;
+          }
+          const int path = (k == 0 || !valid) ? 0 : (int)S.cur[(cbase + c) >> 3] * 8 + ((cbase + c) & 7);
+          const bool wantfill = SIDE >= 0 && valid && (cls == SIDE || lc == SIDE);
+          const bool leafcut = valid && lc == IF;
+          const bool wantnext = valid && cls == IF && r > 0;
+          int cf, cl2, cn;
+          const int sf = wave_slot(wantfill, nf, lane, cf);
+          const int sl = wave_slot(leafcut, nl, lane, cl2);
+          const int sn = wave_slot(wantnext, nnxt, lane, cn);
+          // room for the rows, the cut leaves and their complements
+          if (2 * (nl + cl2) + nf + cf > cap) {
+            overflow = true;
+            break;
+          }
+          if (wantfill) lst[cap - 1 - sf] = (uint16_t)(koff + path);
+          if (leafcut) lst[sl] = (uint16_t)(koff + path);
+          if (wantnext) S.nxt[sn] = (uint16_t)path;
+          nf += cf;
+          nl += cl2;
+          nnxt += cn;
+        }
+        wave_sync();
+      }
+      uint16_t *tmp = S.cur;
+      S.cur = S.nxt;
+      S.nxt = tmp;
+      ncur = nnxt;
+    }
+    const bool allok = __ballot(!ok) == 0ull;
+    if (lane == 0) {
+      if (dom_out) dom_out[i] = (uint8_t)dom;
+      if (overflow) {
+        cnt[i] = make_int4(-1, 0, 0, 0);
+        over[atomicAdd(nover, 1)] = i;
+      } else {
+        cnt[i] = make_int4(nl, nf, 0, 0);
+        if (!allok) atomicOr(err, 2);
+      }
+    }
+    wave_sync();
+  }
+}
+
+// the cut leaves of each listed element: pieces and base-rule points (as in
+// k_subdiv_exact), moments written (not added); the complemented leaves of a
+// volume pass are appended after the cut leaves (cnt.z)
+template <int LO, int PM, int SIDE, int WHICH>
+__global__ __launch_bounds__(64, 2) void k_sdx_leaves(int nel, const int32_t *__restrict__ elems,
+                                                     const double *__restrict__ lset_nodal,
+                                                     const int32_t *__restrict__ el2v,
+                                                     const double *__restrict__ coords,
+                                                     const double *__restrict__ finv_all, int L,
+                                                     const uint4 *__restrict__ nverts,
+                                                     uint16_t *__restrict__ lists, int cap,
+                                                     int4 *__restrict__ cnt, double *__restrict__ out_vol,
+                                                     double *__restrict__ out_if, int out_stride,
+                                                     int32_t *__restrict__ stats, int stats_stride, int32_t *err) {
+  constexpr int NL = ndof(3, LO);
+  constexpr int NM = Moments<3, PM>::NM;
+  constexpr int SDX_VB = PM == 1 ? 4 : (PM == 2 ? 7 : 8), SDX_NVB = PM == 1 ? 1 : (PM == 2 ? 2 : 3);
+  constexpr int SDX_IB = PM == 1 ? 3 : (PM == 2 ? 6 : 4), SDX_NIB = PM == 1 ? 1 : (PM == 2 ? 1 : 3);
+  extern __shared__ char smem[];
+  SdxLds S;
+  const int vol_nq = xg_minrule_cnt_d[3][PM];
+  const int if_nq = xg_minrule_cnt_d[2][PM];
+  sdx_bytes(L, vol_nq, if_nq, NM, &S, smem);
+  // leaf points lane-minor: (point j, coordinate d, leaf lane l)
+  double *pool = &S.pool[0][0][0];
+  const int lane = threadIdx.x & 63;
+  const int n = 1 << L;
+  const int shift = SDX_DMAX - L;
+  {
+    const double(*src_v)[4] = &xg_minrule_data_d[xg_minrule_off_d[3][PM]];
+    const double(*src_i)[4] = &xg_minrule_data_d[xg_minrule_off_d[2][PM]];
+    for (int k = lane; k < 4 * (vol_nq + if_nq); k += 64) {
+      const int q = k >> 2, c = k & 3;
+      if (q < vol_nq) S.rule_vol[q][c] = src_v[q][c];
+      else S.rule_if[q - vol_nq][c] = src_i[q - vol_nq][c];
+    }
+    wave_sync();
+  }
+  const double inv_n = 1.0 / n;
+  for (int i = blockIdx.x; i < nel; i += gridDim.x) {
+    const int4 c4 = cnt[i];
+    if (c4.x < 0) continue;  // queued for the one-kernel path
+    const int e = elems ? elems[i] : i;
+    if (lane < NL) S.cl[lane] = lset_nodal[(size_t)e * NL + lane];
+    bool have_geo = false;
+    if (WHICH & 2) {
+      if (el2v && coords) {
+        have_geo = true;
+        if (lane == 0) {
+          double V[4][3], F[9], Fi[9], ad;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const int v = el2v[(size_t)e * 4 + k];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) V[k][d] = coords[(size_t)v * 3 + d];
+          }
+          jacobian<3>(V, F, Fi, ad);
+#pragma unroll
+          for (int k = 0; k < 9; ++k) S.fi[k] = Fi[k];
+          S.fi[9] = ad;
+        }
+      } else if (finv_all) {
+        have_geo = true;
+        if (lane == 0) {
+          double Fi[9];
+#pragma unroll
+          for (int k = 0; k < 9; ++k) Fi[k] = finv_all[(size_t)i * 9 + k];
+          const double dinv = Fi[0] * (Fi[4] * Fi[8] - Fi[5] * Fi[7]) - Fi[1] * (Fi[3] * Fi[8] - Fi[5] * Fi[6]) +
+                              Fi[2] * (Fi[3] * Fi[7] - Fi[4] * Fi[6]);
+#pragma unroll
+          for (int k = 0; k < 9; ++k) S.fi[k] = Fi[k];
+          S.fi[9] = 1.0 / fabs(dinv);
+        }
+      }
+    }
+    wave_sync();
+    const double *cl = S.cl;
+    double mvol[(WHICH & 1) ? NM : 1], mif[(WHICH & 2) ? NM : 1];
+#pragma unroll
+    for (int k = 0; k < ((WHICH & 1) ? NM : 1); ++k) mvol[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < ((WHICH & 2) ? NM : 1); ++k) mif[k] = 0.0;
+    const int nleaf = c4.x;
+    uint16_t *lst = lists + (size_t)i * cap;
+    int ncomp = 0, st_vol = 0, st_if = 0;
+    bool ok = true;
+    for (int l0 = 0; l0 < nleaf; l0 += 64) {
+      const int l = l0 + lane;
+      LeafPieces2 PC;
+      PC.nvol = 0;
+      PC.nif = 0;
+      PC.comp = false;
+      int id = 0;
+      if (l < nleaf) {
+        id = lst[l];
+        int W[4][3];
+        unpack_uint4(nverts[id], shift, W);
+        double V[4][3], vv[4], cpt[4][3];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) V[j][d] = W[j][d] * inv_n;
+          vv[j] = lagrange_eval3<LO>(cl, V[j]);
+        }
+        if (!leaf_pieces<SIDE>(V, vv, cpt, PC)) {
+          ok = false;
+          PC.nvol = 0;
+          PC.nif = 0;
+          PC.comp = false;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            pool[(j * 3 + d) * 64 + lane] = V[j][d];
+            pool[((4 + j) * 3 + d) * 64 + lane] = cpt[j][d];
+          }
+      }
+      if (!(WHICH & 1)) {
+        PC.nvol = 0;
+        PC.comp = false;
+      }
+      if (!(WHICH & 2)) PC.nif = 0;
+      {
+        int cc;
+        const int sc = wave_slot(PC.comp, ncomp, lane, cc);
+        if (PC.comp) lst[nleaf + sc] = (uint16_t)id;
+        ncomp += cc;
+      }
+      int tot_vol, tot_if;
+      const int bvol = wave_excl_scan(PC.nvol, lane, tot_vol);
+      const int bif = wave_excl_scan(PC.nif, lane, tot_if);
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        if (q < PC.nvol) S.volrec[bvol + q] = (uint32_t)lane | (PC.vol[q] << 8) | (PC.comp ? 0x80u : 0u);
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+        if (q < PC.nif) S.ifrec[bif + q] = (uint32_t)lane | (PC.itf[q] << 8);
+      st_vol += tot_vol;
+      st_if += tot_if;
+      wave_sync();
+      // weight factors (as in k_subdiv_exact)
+      for (int q = lane; q < tot_vol; q += 64) {
+        const uint32_t rec = S.volrec[q];
+        const int ll = rec & 63;
+        double P4[4][3];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          const int j = (rec >> (8 + 3 * m)) & 7;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) P4[m][d] = pool[(j * 3 + d) * 64 + ll];
+        }
+        const double tf = tet_tf(P4[0], P4[1], P4[2], P4[3], true);
+        S.voltf[q] = (rec & 0x80u) ? -tf : tf;
+      }
+      for (int q = lane; q < tot_if; q += 64) {
+        const uint32_t rec = S.ifrec[q];
+        const int ll = rec & 63;
+        double s0[3], s1[3], s2[3], pp[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          s0[d] = pool[(((rec >> 8) & 7) * 3 + d) * 64 + ll];
+          s1[d] = pool[(((rec >> 11) & 7) * 3 + d) * 64 + ll];
+          s2[d] = pool[(((rec >> 14) & 7) * 3 + d) * 64 + ll];
+          pp[d] = pool[(((rec >> 17) & 7) * 3 + d) * 64 + ll];
+        }
+        double a[3], b[3], cc[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          a[d] = s1[d] - s0[d];
+          b[d] = s2[d] - s0[d];
+        }
+        cc[0] = a[1] * b[2] - a[2] * b[1];
+        cc[1] = a[2] * b[0] - a[0] * b[2];
+        cc[2] = a[0] * b[1] - a[1] * b[0];
+        const double tf = norm3(cc);
+        const double na = norm3(a), nb = norm3(b);
+        const double rel = na > nb ? na : nb;
+        double wf = 0.0;
+        if (!(tf < 1e-14 * rel)) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) cc[d] /= tf;
+          const double dd0 = pp[0] - s0[0], dd1 = pp[1] - s0[1], dd2 = pp[2] - s0[2];
+          if (dd0 * cc[0] + dd1 * cc[1] + dd2 * cc[2] >= 0)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) cc[d] *= -1.0;
+          Geo geo;
+          geo.Finv = have_geo ? S.fi : nullptr;
+          geo.absdet = have_geo ? S.fi[9] : 1.0;
+          wf = tf * geo.if_factor(cc);
+        }
+        S.ifw[q] = wf;
+      }
+      wave_sync();
+      if (WHICH & 1) {
+        for (int q = lane; q < tot_vol * SDX_NVB; q += 64) {
+          const int pc = q / SDX_NVB, t0 = (q - pc * SDX_NVB) * SDX_VB;
+          const uint32_t rec = S.volrec[pc];
+          const int ll = rec & 63;
+          double p0[3], p1[3], p2[3], p3[3];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            p0[d] = pool[(((rec >> 8) & 7) * 3 + d) * 64 + ll];
+            p1[d] = pool[(((rec >> 11) & 7) * 3 + d) * 64 + ll];
+            p2[d] = pool[(((rec >> 14) & 7) * 3 + d) * 64 + ll];
+            p3[d] = pool[(((rec >> 17) & 7) * 3 + d) * 64 + ll];
+          }
+          const double tf = S.voltf[pc];
+          // the next point's rule entry is read while this one accumulates
+          double nr[4];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) nr[c] = S.rule_vol[t0][c];
+          for (int t = t0; t < t0 + SDX_VB; ++t) {
+            const double r0 = nr[0], r1 = nr[1], r2 = nr[2], rw = nr[3];
+            const int tn = t + 1 < t0 + SDX_VB ? t + 1 : t;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) nr[c] = S.rule_vol[tn][c];
+            const double ow = 1.0 - r0 - r1 - r2;
+            double lam[4];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) lam[d] = ow * p0[d] + r0 * p1[d] + r1 * p2[d] + r2 * p3[d];
+            lam[3] = 1.0 - lam[0] - lam[1] - lam[2];
+            Moments<3, PM>::accum(lam, rw * tf, mvol);
+          }
+        }
+      }
+      if (WHICH & 2) {
+        for (int q = lane; q < tot_if * SDX_NIB; q += 64) {
+          const int pc = q / SDX_NIB, t0 = (q - pc * SDX_NIB) * SDX_IB;
+          const uint32_t rec = S.ifrec[pc];
+          const int ll = rec & 63;
+          double s0[3], s1[3], s2[3];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            s0[d] = pool[(((rec >> 8) & 7) * 3 + d) * 64 + ll];
+            s1[d] = pool[(((rec >> 11) & 7) * 3 + d) * 64 + ll];
+            s2[d] = pool[(((rec >> 14) & 7) * 3 + d) * 64 + ll];
+          }
+          const double wf = S.ifw[pc];
+          double nr[3] = {S.rule_if[t0][0], S.rule_if[t0][1], S.rule_if[t0][3]};
+          for (int t = t0; t < t0 + SDX_IB; ++t) {
+            const double r0 = nr[0], r1 = nr[1], rw = nr[2];
+            const int tn = t + 1 < t0 + SDX_IB ? t + 1 : t;
+            nr[0] = S.rule_if[tn][0];
+            nr[1] = S.rule_if[tn][1];
+            nr[2] = S.rule_if[tn][3];
+            const double ow = 1.0 - r0 - r1;
+            double lam[4];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) lam[d] = ow * s0[d] + r0 * s1[d] + r1 * s2[d];
+            lam[3] = 1.0 - lam[0] - lam[1] - lam[2];
+            Moments<3, PM>::accum(lam, rw * wf, mif);
+          }
+        }
+      }
+      wave_sync();
+    }
+    // wave reduction of the per-lane moments
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+      double a = (WHICH & 1) ? mvol[(WHICH & 1) ? k : 0] : 0.0, b = (WHICH & 2) ? mif[(WHICH & 2) ? k : 0] : 0.0;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        if (WHICH & 1) a += __shfl_xor(a, o, 64);
+        if (WHICH & 2) b += __shfl_xor(b, o, 64);
+      }
+      if (lane == 0) {
+        S.red[k] = a;
+        S.red[NM + k] = b;
+      }
+    }
+    wave_sync();
+    for (int k = lane; k < NM; k += 64) {
+      if ((WHICH & 1) && out_vol) out_vol[(size_t)i * out_stride + k] = S.red[k];
+      if ((WHICH & 2) && out_if) out_if[(size_t)i * out_stride + k] = S.red[NM + k];
+    }
+    const bool allok = __ballot(!ok) == 0ull;
+    if (lane == 0) {
+      if (WHICH & 1) cnt[i].z = ncomp;
+      if (stats) {
+        stats[(size_t)i * stats_stride + 0] = (WHICH & 1) ? c4.y + ncomp : 0;
+        stats[(size_t)i * stats_stride + 1] = st_vol;
+        stats[(size_t)i * stats_stride + 2] = st_if;
+        stats[(size_t)i * stats_stride + 3] = nleaf;
+      }
+      if (!allok) atomicOr(err, 2);
+    }
+    wave_sync();
+  }
+}
+
+// table rows of each listed element (uncut nodes of the kept side from the
+// back of the list, complemented leaves after the cut leaves) added to its
+// moments; one wave per element, lane = moment (and moment + 64)
+template <int PM>
+__global__ __launch_bounds__(256) void k_sdx_rows(int nel, const uint16_t *__restrict__ lists, int cap,
+                                                  const int4 *__restrict__ cnt, const double *__restrict__ nmom,
+                                                  double *__restrict__ out, int out_stride) {
+  constexpr int NM = Moments<3, PM>::NM;
+  const int lane = threadIdx.x & 63;
+  const int nw = gridDim.x * (blockDim.x >> 6);
+  for (int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < nel; i += nw) {
+    const int4 c4 = cnt[i];
+    if (c4.x < 0) continue;
+    const uint16_t *lst = lists + (size_t)i * cap;
+    const int nrow = c4.y + c4.z;
+    double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+    for (int r0 = 0; r0 < nrow; r0 += 64) {
+      // 64 row ids per round, one per lane, then broadcast
+      const int rr = r0 + lane;
+      int id = 0;
+      if (rr < nrow) id = rr < c4.y ? lst[cap - 1 - rr] : lst[c4.x + rr - c4.y];
+      const int m = min(64, nrow - r0);
+      int j = 0;
+      for (; j + 2 <= m; j += 2) {
+        const size_t ra = (size_t)__shfl(id, j, 64) * NM, rb = (size_t)__shfl(id, j + 1, 64) * NM;
+        if (lane < NM) {
+          a0 += nmom[ra + lane];
+          b0 += nmom[rb + lane];
+        }
+        if (lane + 64 < NM) {
+          a1 += nmom[ra + 64 + lane];
+          b1 += nmom[rb + 64 + lane];
+        }
+      }
+      if (j < m) {
+        const size_t ra = (size_t)__shfl(id, j, 64) * NM;
+        if (lane < NM) a0 += nmom[ra + lane];
+        if (lane + 64 < NM) a1 += nmom[ra + 64 + lane];
+      }
+    }
+    if (lane < NM) out[(size_t)i * out_stride + lane] += a0 + b0;
+    if (lane + 64 < NM) out[(size_t)i * out_stride + 64 + lane] += a1 + b1;
+  }
+}
+
+__global__ void k_zero_i32(int32_t *p) { *p = 0; }
+
+constexpr int SDC_CAP = 4096;       // list entries per element (uint16)
+constexpr int SDC_SUB = 1 << 18;    // elements per staged round
+
+template <int LO>
+int launch_sdx_tree(xg_ctx *ctx, int side, int nel, const int32_t *elems, const double *lset, int L,
+                    uint16_t *lists, int cap, int4 *cnt, uint8_t *dom, int32_t *over, int32_t *nover) {
+  const size_t lds = sdc_bytes(L, nullptr, nullptr);
+  const int blocks = nel < (1 << 22) ? nel : (1 << 22);
+  KernelTimer tm(ctx, "sdx_tree");
+#define XG_TREE(S)                                                                                               \
+  {                                                                                                              \
+    if (lds > 65536)                                                                                             \
+      XG_CHECK(hipFuncSetAttribute((const void *)k_sdx_tree<LO, S>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                   (int)lds));                                                                   \
+    k_sdx_tree<LO, S><<<blocks, 64, lds, ctx->stream>>>(nel, elems, lset, L, (const uint4 *)ctx->sdx_verts,      \
+                                                         lists, cap, cnt, dom, over, nover, ctx->d_err);         \
+  }
+  if (side == NEG) XG_TREE(NEG)
+  else if (side == POS) XG_TREE(POS)
+  else XG_TREE(-1)
+#undef XG_TREE
+  XG_LAUNCH_CHECK();
+  return XG_OK;
+}
+
+template <int LO, int PM, int SIDE, int WHICH>
+int launch_sdx_leaves(xg_ctx *ctx, int nel, const int32_t *elems, const double *lset, const int32_t *el2v,
+                      const double *coords, const double *finv, int L, uint16_t *lists, int cap, int4 *cnt,
+                      double *out_vol, double *out_if, int out_stride, int32_t *stats, int stats_stride) {
+  constexpr int NM = Moments<3, PM>::NM;
+  const size_t lds = sdx_bytes(L, xg_minrule_cnt[3][PM], xg_minrule_cnt[2][PM], NM, nullptr, nullptr);
+  if (lds > 65536)
+    XG_CHECK(hipFuncSetAttribute((const void *)k_sdx_leaves<LO, PM, SIDE, WHICH>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const int blocks = nel < (1 << 22) ? nel : (1 << 22);
+  KernelTimer tm(ctx, (WHICH & 1) ? "sdx_leaves_vol" : "sdx_leaves_if");
+  k_sdx_leaves<LO, PM, SIDE, WHICH><<<blocks, 64, lds, ctx->stream>>>(
+      nel, elems, lset, el2v, coords, finv, L, (const uint4 *)ctx->sdx_verts, lists, cap, cnt, out_vol, out_if,
+      out_stride, stats, stats_stride, ctx->d_err);
+  XG_LAUNCH_CHECK();
+  return XG_OK;
+}
+
+template <int PM>
+int launch_sdx_rows(xg_ctx *ctx, int nel, const uint16_t *lists, int cap, const int4 *cnt, double *out,
+                    int out_stride) {
+  const int blocks = (nel + 3) / 4 < (1 << 20) ? (nel + 3) / 4 : (1 << 20);
+  KernelTimer tm(ctx, "sdx_rows");
+  k_sdx_rows<PM><<<blocks, 256, 0, ctx->stream>>>(nel, lists, cap, cnt, ctx->sdx_mom[PM], out, out_stride);
+  XG_LAUNCH_CHECK();
+  return XG_OK;
+}
+
+// Moments of the parts dts[0..ndt) (NEG / POS / IF) of nel elements, written
+// at mom + i * mstride + t * NM, through the staged kernels; the elements
+// whose lists overflow go through the one-kernel path.
+template <int LO, int PM>
+int subdiv_staged(xg_ctx *ctx, int nel, const int32_t *elems, const double *lset, const int32_t *el2v,
+                  const double *coords, const double *finv, int L, int ndt, const int *dts, double *mom,
+                  int mstride, int32_t *stats) {
+  constexpr int NM = Moments<3, PM>::NM;
+  int rc = ensure_sdx_tables(ctx, PM);
+  if (rc != XG_OK) return rc;
+  // list capacity: SDC_CAP, or smaller through XG_SDX_CAP (16 and up; the
+  // elements that do not fit go through the one-kernel path)
+  int cap = SDC_CAP;
+  if (const char *ev = getenv("XG_SDX_CAP")) {
+    const int v = atoi(ev);
+    if (v >= 16 && v < SDC_CAP) cap = v;
+  }
+  const int sub = nel < SDC_SUB ? nel : SDC_SUB;
+  const size_t b_lists = sd_align(sizeof(uint16_t) * (size_t)sub * cap);
+  const size_t b_cnt = sd_align(sizeof(int4) * (size_t)sub);
+  const size_t b_over = sd_align(sizeof(int32_t) * ((size_t)sub + 1));
+  rc = ensure_buf(ctx, &ctx->sdx_ws, &ctx->sdx_ws_size, b_lists + b_cnt + b_over);
+  if (rc != XG_OK) return rc;
+  uint16_t *lists = (uint16_t *)ctx->sdx_ws;
+  int4 *cnt = (int4 *)((char *)ctx->sdx_ws + b_lists);
+  int32_t *nover = (int32_t *)((char *)ctx->sdx_ws + b_lists + b_cnt);
+  int32_t *over = nover + 1;
+  for (int off = 0; off < nel; off += sub) {
+    const int m = nel - off < sub ? nel - off : sub;
+    const int32_t *el = elems ? elems + off : nullptr;
+    const double *fi = finv ? finv + (size_t)off * 9 : nullptr;
+    double *mo = mom + (size_t)off * mstride;
+    int32_t *st = stats ? stats + (size_t)off * ndt * 4 : nullptr;
+    // volume sides in the order requested; the interface rides on the first
+    // tree (its cut leaves do not depend on the side)
+    bool has_if = false, if_done = false;
+    for (int t = 0; t < ndt; ++t) has_if = has_if || dts[t] == IF;
+    for (int t = 0; t < ndt || (has_if && !if_done && t == ndt); ++t) {
+      int side;
+      if (t < ndt) {
+        if (dts[t] == IF) continue;
+        side = dts[t];
+      } else {
+        side = -1;  // interface only
+      }
+      k_zero_i32<<<1, 1, 0, ctx->stream>>>(nover);
+      XG_LAUNCH_CHECK();
+      rc = launch_sdx_tree<LO>(ctx, side, m, el, lset, L, lists, cap, cnt, nullptr, over, nover);
+      if (rc != XG_OK) return rc;
+      int h_nover = 0;
+      XG_CHECK(hipMemcpyAsync(&h_nover, nover, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+      XG_CHECK(hipStreamSynchronize(ctx->stream));
+      for (int u = 0; u < ndt; ++u) {
+        const bool vol = dts[u] == side;
+        const bool itf = dts[u] == IF && !if_done;
+        if (!vol && !itf) continue;
+        double *o = mo + (size_t)u * NM;
+        int32_t *su = st ? st + (size_t)u * 4 : nullptr;
+        if (vol) {
+          if (side == NEG)
+            rc = launch_sdx_leaves<LO, PM, NEG, 1>(ctx, m, el, lset, el2v, coords, fi, L, lists, cap, cnt, o, nullptr,
+                                                   mstride, su, ndt * 4);
+          else
+            rc = launch_sdx_leaves<LO, PM, POS, 1>(ctx, m, el, lset, el2v, coords, fi, L, lists, cap, cnt, o, nullptr,
+                                                   mstride, su, ndt * 4);
+          if (rc != XG_OK) return rc;
+          rc = launch_sdx_rows<PM>(ctx, m, lists, cap, cnt, o, mstride);
+        } else {
+          rc = launch_sdx_leaves<LO, PM, NEG, 2>(ctx, m, el, lset, el2v, coords, fi, L, lists, cap, cnt, nullptr, o,
+                                                 mstride, su, ndt * 4);
+        }
+        if (rc != XG_OK) return rc;
+        if (h_nover > 0) {
+          rc = subdiv_exact_pass<LO, PM>(ctx, h_nover, el, lset, el2v, coords, fi, L, vol ? side : -1, !vol,
+                                         vol ? o : nullptr, vol ? nullptr : o, mstride, nullptr, su, ndt * 4, over);
+          if (rc != XG_OK) return rc;
+        }
+        if (itf) if_done = true;
+      }
+      if (t == ndt) break;
+    }
+  }
+  return XG_OK;
 }
 
 int subdiv_exact(xg_ctx *ctx, int lo, int p, int nel, const int32_t *elems, const double *lset_nodal,
@@ -1581,15 +2332,11 @@ static int subdiv_mesh_moments(xg_ctx *ctx, int p, int L, int nel, const int32_t
                                const double *coords, const double *lset_nodal, int ndt, const int *dts,
                                double *mom, int32_t *stats) {
   const int nm = p == 1 ? 10 : (p == 2 ? 35 : 84);
-  for (int t = 0; t < ndt; ++t) {
-    const int d = dts[t];
-    const int rc = subdiv_exact(ctx, 2, p, nel, elems, lset_nodal, el2v, coords, nullptr, L, d == IF ? -1 : d,
-                                d == IF, d == IF ? nullptr : mom + (size_t)t * nm,
-                                d == IF ? mom + (size_t)t * nm : nullptr, ndt * nm, nullptr,
-                                stats ? stats + (size_t)t * 4 : nullptr, ndt * 4);
-    if (rc != XG_OK) return rc;
-  }
-  return XG_OK;
+  if (p == 1)
+    return subdiv_staged<2, 1>(ctx, nel, elems, lset_nodal, el2v, coords, nullptr, L, ndt, dts, mom, ndt * nm, stats);
+  if (p == 2)
+    return subdiv_staged<2, 2>(ctx, nel, elems, lset_nodal, el2v, coords, nullptr, L, ndt, dts, mom, ndt * nm, stats);
+  return subdiv_staged<2, 3>(ctx, nel, elems, lset_nodal, el2v, coords, nullptr, L, ndt, dts, mom, ndt * nm, stats);
 }
 
 extern "C" int xg_subdiv_moments_mesh(xg_ctx *ctx, int p, int L, int order, int nel, const int32_t *elems,

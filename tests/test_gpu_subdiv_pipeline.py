@@ -132,6 +132,33 @@ def test_pipeline_zero_lattice_values(orc, dev):
         np.testing.assert_allclose(g[:, 0] + g[:, 1], full, rtol=0, atol=1e-13 * np.abs(full).max())
 
 
+@pytest.mark.parametrize("cap", ["16", "400"])
+def test_pipeline_list_overflow(orc, dev, monkeypatch, cap):
+    """elements whose node lists (2 cut leaves + table rows) exceed the list
+    capacity go through the one-kernel path: with the capacity lowered to 16
+    (every element) or 400 (some elements) the matrices and the piece counts
+    equal those of the default capacity, and the oracle's"""
+    torch, _abi, ctx = dev
+    mesh = MakeStructured3DMesh(nx=6, ny=6, nz=6)
+    c, r = W.config5_spheres(7)
+    vals = p2_values(mesh, c, r)
+    cut = lattice_cut(vals)
+    cut = cut[:: max(1, len(cut) // 40)]
+    dts = [NEG, IF, POS]
+    g0, st0 = run_pipeline(dev, 3, 4, cut, mesh, vals, dts, _abi.XG_PREC_FP64, stats=True)
+    monkeypatch.setenv("XG_SDX_CAP", cap)
+    g1, st1 = run_pipeline(dev, 3, 4, cut, mesh, vals, dts, _abi.XG_PREC_FP64, stats=True)
+    monkeypatch.delenv("XG_SDX_CAP")
+    if cap == "400":
+        # some elements surely fit (2 leaves + rows + complements < 400),
+        # some surely do not (2 leaves > 400)
+        assert (2 * st0[:, 0, 3] + st0[:, 0, 0] < 400).any() and (2 * st0[:, 0, 3] > 400).any()
+    np.testing.assert_array_equal(st0, st1)
+    ref = orc.subdiv_pipeline(3, 4, 6, cut, mesh.el2v, mesh.coords, vals, dts, 3, nthreads=8)
+    assert rel_err(g0, ref).max() < 1e-12
+    assert rel_err(g1, ref).max() < 1e-12
+
+
 @pytest.fixture(scope="module")
 def large_mixed(dev):
     """the large mixed workload: 128^3 x 6 Kuhn tets, union of 64 spheres
