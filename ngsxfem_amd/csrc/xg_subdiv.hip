@@ -1584,6 +1584,15 @@ __device__ __forceinline__ int small_div(int q, int d, float rd, int &rem) {
   return c;
 }
 
+__global__ void k_zero_i32(int32_t *p) { *p = 0; }
+
+// next element of a persistent wave: lane 0 takes it from the work counter
+__device__ __forceinline__ int next_item(int32_t *queue, int lane) {
+  int nx = 0;
+  if (lane == 0) nx = atomicAdd(queue, 1);
+  return __shfl(nx, 0, 64);
+}
+
 // SIDE: NEG or POS (rows of that side), or -1 (cut leaves only)
 template <int LO, int SIDE>
 __global__ __launch_bounds__(64) void k_sdx_tree(int nel, const int32_t *__restrict__ elems,
@@ -1767,7 +1776,8 @@ __global__ __launch_bounds__(64, 2) void k_sdx_leaves(int nel, const int32_t *__
                                                      uint16_t *__restrict__ lists, int cap,
                                                      int4 *__restrict__ cnt, double *__restrict__ out_vol,
                                                      double *__restrict__ out_if, int out_stride,
-                                                     int32_t *__restrict__ stats, int stats_stride, int32_t *err) {
+                                                     int32_t *__restrict__ stats, int stats_stride, int32_t *err,
+                                                     int32_t *__restrict__ queue) {
   constexpr int NL = ndof(3, LO);
   constexpr int NM = Moments<3, PM>::NM;
   constexpr int SDX_VB = PM == 1 ? 4 : (PM == 2 ? 7 : 8), SDX_NVB = PM == 1 ? 1 : (PM == 2 ? 2 : 3);
@@ -1793,7 +1803,10 @@ __global__ __launch_bounds__(64, 2) void k_sdx_leaves(int nel, const int32_t *__
     wave_sync();
   }
   const double inv_n = 1.0 / n;
-  for (int i = blockIdx.x; i < nel; i += gridDim.x) {
+  // elements from the work counter (persistent waves, the heavy elements do
+  // not wait behind a fixed stride) or strided by the grid
+  for (int i = queue ? next_item(queue, lane) : blockIdx.x; i < nel;
+       i = queue ? next_item(queue, lane) : i + gridDim.x) {
     const int4 c4 = cnt[i];
     if (c4.x < 0) continue;  // queued for the one-kernel path
     const int e = elems ? elems[i] : i;
@@ -2060,7 +2073,8 @@ __global__ __launch_bounds__(256) void k_sdx_rows(int nel, const uint16_t *__res
     if (c4.x < 0) continue;
     const uint16_t *lst = lists + (size_t)i * cap;
     const int nrow = c4.y + c4.z;
-    double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+    // four rows in flight per lane (moment lane, moment lane + 64)
+    double a[4] = {0.0, 0.0, 0.0, 0.0}, b[4] = {0.0, 0.0, 0.0, 0.0};
     for (int r0 = 0; r0 < nrow; r0 += 64) {
       // 64 row ids per round, one per lane, then broadcast
       const int rr = r0 + lane;
@@ -2068,29 +2082,26 @@ __global__ __launch_bounds__(256) void k_sdx_rows(int nel, const uint16_t *__res
       if (rr < nrow) id = rr < c4.y ? lst[cap - 1 - rr] : lst[c4.x + rr - c4.y];
       const int m = min(64, nrow - r0);
       int j = 0;
-      for (; j + 2 <= m; j += 2) {
-        const size_t ra = (size_t)__shfl(id, j, 64) * NM, rb = (size_t)__shfl(id, j + 1, 64) * NM;
-        if (lane < NM) {
-          a0 += nmom[ra + lane];
-          b0 += nmom[rb + lane];
-        }
-        if (lane + 64 < NM) {
-          a1 += nmom[ra + 64 + lane];
-          b1 += nmom[rb + 64 + lane];
+      for (; j + 4 <= m; j += 4) {
+        size_t ro[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) ro[u] = (size_t)__shfl(id, j + u, 64) * NM;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (lane < NM) a[u] += nmom[ro[u] + lane];
+          if (lane + 64 < NM) b[u] += nmom[ro[u] + 64 + lane];
         }
       }
-      if (j < m) {
+      for (; j < m; ++j) {
         const size_t ra = (size_t)__shfl(id, j, 64) * NM;
-        if (lane < NM) a0 += nmom[ra + lane];
-        if (lane + 64 < NM) a1 += nmom[ra + 64 + lane];
+        if (lane < NM) a[0] += nmom[ra + lane];
+        if (lane + 64 < NM) b[0] += nmom[ra + 64 + lane];
       }
     }
-    if (lane < NM) out[(size_t)i * out_stride + lane] += a0 + b0;
-    if (lane + 64 < NM) out[(size_t)i * out_stride + 64 + lane] += a1 + b1;
+    if (lane < NM) out[(size_t)i * out_stride + lane] += (a[0] + a[1]) + (a[2] + a[3]);
+    if (lane + 64 < NM) out[(size_t)i * out_stride + 64 + lane] += (b[0] + b[1]) + (b[2] + b[3]);
   }
 }
-
-__global__ void k_zero_i32(int32_t *p) { *p = 0; }
 
 constexpr int SDC_CAP = 4096;       // list entries per element (uint16)
 constexpr int SDC_SUB = 1 << 18;    // elements per staged round
@@ -2120,17 +2131,27 @@ int launch_sdx_tree(xg_ctx *ctx, int side, int nel, const int32_t *elems, const 
 template <int LO, int PM, int SIDE, int WHICH>
 int launch_sdx_leaves(xg_ctx *ctx, int nel, const int32_t *elems, const double *lset, const int32_t *el2v,
                       const double *coords, const double *finv, int L, uint16_t *lists, int cap, int4 *cnt,
-                      double *out_vol, double *out_if, int out_stride, int32_t *stats, int stats_stride) {
+                      double *out_vol, double *out_if, int out_stride, int32_t *stats, int stats_stride,
+                      int32_t *queue) {
   constexpr int NM = Moments<3, PM>::NM;
+  const void *kern = (const void *)k_sdx_leaves<LO, PM, SIDE, WHICH>;
   const size_t lds = sdx_bytes(L, xg_minrule_cnt[3][PM], xg_minrule_cnt[2][PM], NM, nullptr, nullptr);
-  if (lds > 65536)
-    XG_CHECK(hipFuncSetAttribute((const void *)k_sdx_leaves<LO, PM, SIDE, WHICH>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const int blocks = nel < (1 << 22) ? nel : (1 << 22);
+  if (lds > 65536) XG_CHECK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  int blocks = nel < (1 << 22) ? nel : (1 << 22);
+  if (queue) {
+    // as many persistent waves as fit on the device at once
+    int per_cu = 0, ncu = 0;
+    XG_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64, lds));
+    XG_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    const long long res = (long long)(per_cu > 0 ? per_cu : 1) * (ncu > 0 ? ncu : 1);
+    if (res < blocks) blocks = (int)res;
+    k_zero_i32<<<1, 1, 0, ctx->stream>>>(queue);
+    XG_LAUNCH_CHECK();
+  }
   KernelTimer tm(ctx, (WHICH & 1) ? "sdx_leaves_vol" : "sdx_leaves_if");
   k_sdx_leaves<LO, PM, SIDE, WHICH><<<blocks, 64, lds, ctx->stream>>>(
       nel, elems, lset, el2v, coords, finv, L, (const uint4 *)ctx->sdx_verts, lists, cap, cnt, out_vol, out_if,
-      out_stride, stats, stats_stride, ctx->d_err);
+      out_stride, stats, stats_stride, ctx->d_err, queue);
   XG_LAUNCH_CHECK();
   return XG_OK;
 }
@@ -2165,13 +2186,17 @@ int subdiv_staged(xg_ctx *ctx, int nel, const int32_t *elems, const double *lset
   const int sub = nel < SDC_SUB ? nel : SDC_SUB;
   const size_t b_lists = sd_align(sizeof(uint16_t) * (size_t)sub * cap);
   const size_t b_cnt = sd_align(sizeof(int4) * (size_t)sub);
-  const size_t b_over = sd_align(sizeof(int32_t) * ((size_t)sub + 1));
+  const size_t b_over = sd_align(sizeof(int32_t) * ((size_t)sub + 2));
+  // persistent waves on a work counter (XG_SDX_QUEUE=0: one block per element)
+  bool use_queue = true;
+  if (const char *ev = getenv("XG_SDX_QUEUE")) use_queue = atoi(ev) != 0;
   rc = ensure_buf(ctx, &ctx->sdx_ws, &ctx->sdx_ws_size, b_lists + b_cnt + b_over);
   if (rc != XG_OK) return rc;
   uint16_t *lists = (uint16_t *)ctx->sdx_ws;
   int4 *cnt = (int4 *)((char *)ctx->sdx_ws + b_lists);
   int32_t *nover = (int32_t *)((char *)ctx->sdx_ws + b_lists + b_cnt);
-  int32_t *over = nover + 1;
+  int32_t *queue = use_queue ? nover + 1 : nullptr;
+  int32_t *over = nover + 2;
   for (int off = 0; off < nel; off += sub) {
     const int m = nel - off < sub ? nel - off : sub;
     const int32_t *el = elems ? elems + off : nullptr;
@@ -2206,15 +2231,15 @@ int subdiv_staged(xg_ctx *ctx, int nel, const int32_t *elems, const double *lset
         if (vol) {
           if (side == NEG)
             rc = launch_sdx_leaves<LO, PM, NEG, 1>(ctx, m, el, lset, el2v, coords, fi, L, lists, cap, cnt, o, nullptr,
-                                                   mstride, su, ndt * 4);
+                                                   mstride, su, ndt * 4, queue);
           else
             rc = launch_sdx_leaves<LO, PM, POS, 1>(ctx, m, el, lset, el2v, coords, fi, L, lists, cap, cnt, o, nullptr,
-                                                   mstride, su, ndt * 4);
+                                                   mstride, su, ndt * 4, queue);
           if (rc != XG_OK) return rc;
           rc = launch_sdx_rows<PM>(ctx, m, lists, cap, cnt, o, mstride);
         } else {
           rc = launch_sdx_leaves<LO, PM, NEG, 2>(ctx, m, el, lset, el2v, coords, fi, L, lists, cap, cnt, nullptr, o,
-                                                 mstride, su, ndt * 4);
+                                                 mstride, su, ndt * 4, queue);
         }
         if (rc != XG_OK) return rc;
         if (h_nover > 0) {

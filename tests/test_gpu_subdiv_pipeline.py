@@ -159,6 +159,24 @@ def test_pipeline_list_overflow(orc, dev, monkeypatch, cap):
     assert rel_err(g1, ref).max() < 1e-12
 
 
+def test_pipeline_queue_modes_agree(dev, monkeypatch):
+    """the leaf kernels take elements from a work counter (persistent waves)
+    or one block per element (XG_SDX_QUEUE=0): each element is still one
+    wave's work, so the two give the same bits"""
+    torch, _abi, ctx = dev
+    mesh = MakeStructured3DMesh(nx=6, ny=6, nz=6)
+    c, r = W.config5_spheres(7)
+    vals = p2_values(mesh, c, r)
+    cut = lattice_cut(vals)
+    dts = [NEG, IF, POS]
+    g0, st0 = run_pipeline(dev, 3, 3, cut, mesh, vals, dts, _abi.XG_PREC_FP64, stats=True)
+    monkeypatch.setenv("XG_SDX_QUEUE", "0")
+    g1, st1 = run_pipeline(dev, 3, 3, cut, mesh, vals, dts, _abi.XG_PREC_FP64, stats=True)
+    monkeypatch.delenv("XG_SDX_QUEUE")
+    np.testing.assert_array_equal(st0, st1)
+    np.testing.assert_array_equal(g0, g1)
+
+
 @pytest.fixture(scope="module")
 def large_mixed(dev):
     """the large mixed workload: 128^3 x 6 Kuhn tets, union of 64 spheres
