@@ -219,6 +219,20 @@ def c5_fp64_ops(stats, launches, p=3):
             + itf * (nq_i * (18 + 30 + 21 + 2 * nm) + 40) + leaves * (48 + 4 * 42))
 
 
+def c5_leaf_ops(stats, part, p=3):
+    """FP64 operations (FMA = 2) of the cut-leaf kernel k_sdx_leaves for one
+    part from the per-element counts (stats[..., 0..3] as c5_fp64_ops):
+    part "vol": the volume tets point by point + the cut leaves' pieces;
+    part "if": the interface triangles + the cut leaves' pieces."""
+    nm = {1: 10, 2: 35, 3: 84}[p]
+    nq_v, nq_i = {1: (4, 3), 2: (14, 6), 3: (24, 12)}[p]
+    vol, itf, leaves = (float(stats[..., k].sum()) for k in (1, 2, 3))
+    pieces = leaves * (48 + 4 * 42)
+    if part == "vol":
+        return vol * (nq_v * (27 + 30 + 21 + 2 * nm) + 17) + pieces
+    return itf * (nq_i * (18 + 30 + 21 + 2 * nm) + 40) + pieces
+
+
 def cpu_threads():
     """host threads for the CPU baseline: every CPU this process may run on"""
     try:
@@ -307,19 +321,29 @@ def measure_c5(args):
     ms = g0.elapsed_time(g1) / args.steps
     print("# large mixed step %.3f ms over %d steps" % (ms, args.steps), file=sys.stderr, flush=True)
 
-    # the dominant kernel alone: the NEG pass k_subdiv_exact<2,3,NEG,1> (the
-    # step runs it and the interface pass), its launches back to back between
-    # two events on the stream it runs on
+    # the subdivision kernels of the step (walk, NEG leaf pieces, interface
+    # leaf pieces, table rows; the one-kernel path for list overflows), each
+    # launch between its own pair of events on the context's stream
+    # (xg_timing_*), 3 repetitions of the step's subdivision part
     mom2 = torch.empty(n_cut * 2 * nm, dtype=torch.float64, device=dev)
     stats = torch.zeros(n_cut * 2 * 4, dtype=torch.int32, device=dev)
-    ms_neg = timed(lambda: ctx.subdiv_moments_mesh(p, L, order, cl, d_el, d_x, lp2, [_abi.XG_NEG], mom2), 3)
-    ms_if = timed(lambda: ctx.subdiv_moments_mesh(p, L, order, cl, d_el, d_x, lp2, [_abi.XG_IF], mom2), 3)
+    ms_sub = timed(lambda: ctx.subdiv_moments_mesh(p, L, order, cl, d_el, d_x, lp2, dts, mom2), 3)
+    ctx.timing_enable(True)
+    for _ in range(3):
+        ctx.subdiv_moments_mesh(p, L, order, cl, d_el, d_x, lp2, dts, mom2)
+    kt = {}
+    for k in ("sdx_tree", "sdx_leaves_vol", "sdx_leaves_if", "sdx_rows", "subdiv_exact"):
+        tot, nl = ctx.timing_read(k)
+        kt[k] = {"ms_per_step": tot / 3, "launches_per_step": nl / 3}
+    ctx.timing_enable(False)
     ctx.subdiv_moments_mesh(p, L, order, cl, d_el, d_x, lp2, dts, mom2, stats)
     ctx.sync()
     st = stats.cpu().numpy().reshape(n_cut, 2, 4)
-    ops = c5_fp64_ops(st[:, :1], launches=n_cut)
-    ops_if = c5_fp64_ops(st[:, 1:], launches=n_cut)
-    ms_kern = ms_neg
+    ops = c5_leaf_ops(st[:, 0], "vol")
+    ops_if = c5_leaf_ops(st[:, 1], "if")
+    ms_kern = kt["sdx_leaves_vol"]["ms_per_step"]
+    ms_kif = kt["sdx_leaves_if"]["ms_per_step"]
+    rows_bytes = float(st[:, 0, 0].sum()) * nm * 8
     if args.pmc_probe:
         # counter runs (rocprofv3 --pmc): a 256 MiB device copy of known size
         # (calibration of FETCH_SIZE / WRITE_SIZE), then 3 kernel launches
@@ -328,12 +352,12 @@ def measure_c5(args):
         for _ in range(3):
             b.copy_(a)
         for _ in range(3):
-            ctx.subdiv_moments_mesh(p, L, order, cl, d_el, d_x, lp2, [_abi.XG_NEG], mom2)
+            ctx.subdiv_moments_mesh(p, L, order, cl, d_el, d_x, lp2, dts, mom2)
         ctx.sync()
         torch.cuda.synchronize()
         print(json.dumps({"pmc_probe": True, "copy_bytes": 2 * a.numel() * 4, "cut_elements": n_cut,
-                          "algorithmic_flops_per_launch": ops,
-                          "algorithmic_bytes_per_launch": n_cut * (4 + 80 + 16 + 96 + nm * 8)}), flush=True)
+                          "leaves_vol_flops_per_step": ops, "leaves_if_flops_per_step": ops_if,
+                          "kernels": kt}), flush=True)
         ctx.close()
         return None
 
@@ -347,21 +371,31 @@ def measure_c5(args):
     dn = torch.linalg.vector_norm(out.view(-1, nd * nd).double() - r64, dim=1)
     err32 = float((dn[nzr] / rn[nzr]).max().item())
 
-    roof = {"bound": "fp64-valu", "kernel": "k_subdiv_exact<2,3,NEG,1> (the NEG pass)",
+    roof = {"bound": "fp64-valu", "kernel": "k_sdx_leaves<2,3,NEG,1> (NEG cut-leaf pieces, the step's largest)",
             "achieved": ops / (ms_kern * 1e-3) / 1e12, "peak": PEAK_FP64_VALU / 1e12, "unit": "TFLOP/s",
             "frac": ops / (ms_kern * 1e-3) / PEAK_FP64_VALU, "traffic": read_traffic("subdiv_c5_pmc.txt"),
             "traffic_source": "profiles/subdiv_c5_pmc.txt (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the same "
-                              "kernel, copy-calibrated)",
+                              "kernel, copy-calibrated, summed over its launches of one step)",
             "algorithmic_flops_per_launch": ops, "avg_launch_ms": ms_kern,
+            "launch_note": "one step launches it once per 262,144 elements (2 launches); flops and ms are per step",
             "counts_per_cut_element": {"table_rows": float(st[:, 0, 0].mean()),
                                        "volume_tets": float(st[:, 0, 1].mean()),
                                        "interface_triangles": float(st[:, 1, 2].mean()),
                                        "cut_leaves": float(st[:, 0, 3].mean())},
-            "interface_pass": {"kernel": "k_subdiv_exact<2,3,NEG,2>", "avg_launch_ms": ms_if,
-                               "algorithmic_flops_per_launch": ops_if,
-                               "frac": ops_if / (ms_if * 1e-3) / PEAK_FP64_VALU},
-            "note": "flops counted from the kernel's own per-element piece counts (bench.py c5_fp64_ops); the "
-                    "kernel reads 176 B and writes 672 B per cut element, so HBM is not its bound"}
+            "kernels": {
+                "k_sdx_leaves<2,3,NEG,2> (interface)": {
+                    "ms": ms_kif, "bound": "fp64-valu", "algorithmic_flops": ops_if,
+                    "frac": ops_if / (ms_kif * 1e-3) / PEAK_FP64_VALU},
+                "k_sdx_tree<2,NEG> (refinement walk, lists)": {
+                    "ms": kt["sdx_tree"]["ms_per_step"], "bound": "VALU issue (integer lattice checks)"},
+                "k_sdx_rows<3> (table rows)": {
+                    "ms": kt["sdx_rows"]["ms_per_step"], "bound": "L2 (the depth <= 4 rows, 3.1 MB)",
+                    "bytes": rows_bytes,
+                    "GB/s": rows_bytes / (kt["sdx_rows"]["ms_per_step"] * 1e-3) / 1e9},
+                "k_subdiv_exact (list overflows)": {"ms": kt["subdiv_exact"]["ms_per_step"],
+                                                    "launches": kt["subdiv_exact"]["launches_per_step"]}},
+            "note": "flops counted from the kernels' own per-element piece counts (bench.py c5_leaf_ops); each "
+                    "kernel time is the sum of its launches' event pairs on the stream (xg_timing_read)"}
     res = {"metric": METRIC, "value": n_cut / (ms * 1e-3), "unit": "cut-elem/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "fp64+fp32", "data": "synthetic",
@@ -372,8 +406,7 @@ def measure_c5(args):
                       "cut_elements": n_cut, "neg": int(counts[0]), "pos": int(counts[1])},
            "roofline": roof,
            "max_rel_err_vs_fp64": err32,
-           "paths": {"subdivision_neg_pass_ms": ms_neg, "subdivision_if_pass_ms": ms_if,
-                     "elmat_and_rest_ms": ms - ms_neg - ms_if},
+           "paths": {"subdivision_ms": ms_sub, "elmat_and_rest_ms": ms - ms_sub},
            "marking": {"ms": ms_mark, "tets_per_s": ne / (ms_mark * 1e-3), "kernel": "xg_mark_p2 (L=4)"}}
 
     # ---- CPU baseline: the oracle (reference algorithm: materialised level-4
