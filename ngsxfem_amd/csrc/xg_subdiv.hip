@@ -1660,10 +1660,33 @@ __global__ __launch_bounds__(64) void k_sdx_tree(int nel, const int32_t *__restr
           S.flags[c] = 0u;
         }
         wave_sync();
-        // lattice checks above the leaves: the sign bits of every
-        // (candidate, lattice point) pair, OR-ed per candidate with ballots
-        // (the pairs of one pass cover consecutive candidates)
-        if (r > 0) {
+        // lattice checks above the leaves.  Up to 35 lattice points per
+        // candidate (r <= 2): one lane per candidate walks them.  More: the
+        // sign bits of every (candidate, lattice point) pair, OR-ed per
+        // candidate with ballots (the pairs of one pass cover consecutive
+        // candidates)
+        if (r > 0 && r <= 2) {
+          for (int c = lane; c < nc; c += 64) {
+            int W[4][3];
+            unpack_uint4(S.nodeV[c], shift, W);
+            int E[3][3];
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+#pragma unroll
+              for (int d = 0; d < 3; ++d) E[m][d] = W[m + 1][d] - W[0][d];
+            uint32_t f = 0u;
+            for (int t = 0; t < np; ++t) {
+              const unsigned pk = S.latq[poff + t];
+              const int I0 = (int)(pk & 255u), I1 = (int)((pk >> 8) & 255u), I2 = (int)(pk >> 16);
+              int P[3];
+#pragma unroll
+              for (int d = 0; d < 3; ++d) P[d] = W[0][d] + ((I0 * E[0][d] + I1 * E[1][d] + I2 * E[2][d]) >> r);
+              f |= lat_bit(S.latbits, lat_index_rs(S.rowst, P[0], P[1], P[2], n)) ? 1u : 2u;
+            }
+            S.flags[c] = f;
+          }
+          wave_sync();
+        } else if (r > 0) {
           const int npairs = nc * np;
           for (int q0 = 0; q0 < npairs; q0 += 64) {
             const int q = q0 + lane;
@@ -2024,21 +2047,34 @@ __global__ __launch_bounds__(64, 2) void k_sdx_leaves(int nel, const int32_t *__
       }
       wave_sync();
     }
-    // wave reduction of the per-lane moments
+    // wave reduction of the per-lane moments through LDS (the point pool is
+    // free now): SDX_RB moments per round, lane kk of the round sums column
+    // kk, columns padded to 65 doubles so the summing lanes hit distinct
+    // banks (the staged kernels run one part per launch)
+    static_assert(WHICH == 1 || WHICH == 2, "one part per launch");
+    constexpr int SDX_RB = 23;  // 23 x 65 doubles fit the 64 x 24 pool
 #pragma unroll
-    for (int k = 0; k < NM; ++k) {
-      double a = (WHICH & 1) ? mvol[(WHICH & 1) ? k : 0] : 0.0, b = (WHICH & 2) ? mif[(WHICH & 2) ? k : 0] : 0.0;
+    for (int k0 = 0; k0 < NM; k0 += SDX_RB) {
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        if (WHICH & 1) a += __shfl_xor(a, o, 64);
-        if (WHICH & 2) b += __shfl_xor(b, o, 64);
+      for (int kk = 0; kk < SDX_RB; ++kk) {
+        if (k0 + kk < NM) {
+          if constexpr ((WHICH & 1) != 0) pool[kk * 65 + lane] = mvol[k0 + kk < NM ? k0 + kk : 0];
+          else pool[kk * 65 + lane] = mif[k0 + kk < NM ? k0 + kk : 0];
+        }
       }
-      if (lane == 0) {
-        S.red[k] = a;
-        S.red[NM + k] = b;
+      wave_sync();
+      if (lane < SDX_RB && k0 + lane < NM) {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        for (int j = 0; j < 64; j += 4) {
+          s0 += pool[lane * 65 + j];
+          s1 += pool[lane * 65 + j + 1];
+          s2 += pool[lane * 65 + j + 2];
+          s3 += pool[lane * 65 + j + 3];
+        }
+        S.red[((WHICH & 1) ? 0 : NM) + k0 + lane] = (s0 + s1) + (s2 + s3);
       }
+      wave_sync();
     }
-    wave_sync();
     for (int k = lane; k < NM; k += 64) {
       if ((WHICH & 1) && out_vol) out_vol[(size_t)i * out_stride + k] = S.red[k];
       if ((WHICH & 2) && out_if) out_if[(size_t)i * out_stride + k] = S.red[NM + k];
@@ -2104,7 +2140,7 @@ __global__ __launch_bounds__(256) void k_sdx_rows(int nel, const uint16_t *__res
 }
 
 constexpr int SDC_CAP = 4096;       // list entries per element (uint16)
-constexpr int SDC_SUB = 1 << 18;    // elements per staged round
+constexpr int SDC_SUB = 1 << 19;    // elements per staged round
 
 template <int LO>
 int launch_sdx_tree(xg_ctx *ctx, int side, int nel, const int32_t *elems, const double *lset, int L,
