@@ -1034,6 +1034,34 @@ __device__ __forceinline__ void unpack_uint4(const uint4 &pv, int shift, int (*W
 
 __device__ __forceinline__ bool lat_bit(const uint32_t *bits, int idx) { return (bits[idx >> 5] >> (idx & 31)) & 1u; }
 
+// Sum of the wave's per-lane arrays m[0..NM) into dst[0..NM), through LDS
+// scratch of at least 23 x 65 doubles: 23 entries per round, lane kk of the
+// round sums column kk (columns padded to 65 doubles so the summing lanes
+// hit distinct banks).  Cheaper than 6 shuffle steps per entry.  Every lane
+// of the wave calls it; it ends with dst written and visible to the wave.
+template <int NM>
+__device__ __forceinline__ void wave_sum_lds(const double *m, double *scr, double *dst, int lane) {
+  constexpr int RB = 23;
+#pragma unroll
+  for (int k0 = 0; k0 < NM; k0 += RB) {
+#pragma unroll
+    for (int kk = 0; kk < RB; ++kk)
+      if (k0 + kk < NM) scr[kk * 65 + lane] = m[k0 + kk < NM ? k0 + kk : 0];
+    wave_sync();
+    if (lane < RB && k0 + lane < NM) {
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      for (int j = 0; j < 64; j += 4) {
+        s0 += scr[lane * 65 + j];
+        s1 += scr[lane * 65 + j + 1];
+        s2 += scr[lane * 65 + j + 2];
+        s3 += scr[lane * 65 + j + 3];
+      }
+      dst[k0 + lane] = (s0 + s1) + (s2 + s3);
+    }
+    wave_sync();
+  }
+}
+
 // stats (nullable, int32 x 4 per element): table rows added, volume tets and
 // interface triangles integrated point by point, cut leaves
 template <int LO, int PM, int SIDE, int WHICH>
@@ -1405,21 +1433,10 @@ __global__ __launch_bounds__(64, 2) void k_subdiv_exact(int nel, const int32_t *
       S.nxt = tmp;
       ncur = nnxt;
     }
-    // wave reduction of the per-lane moments, plus the table sums
-#pragma unroll
-    for (int k = 0; k < NM; ++k) {
-      double a = (WHICH & 1) ? mvol[(WHICH & 1) ? k : 0] : 0.0, b = (WHICH & 2) ? mif[(WHICH & 2) ? k : 0] : 0.0;
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        if (WHICH & 1) a += __shfl_xor(a, o, 64);
-        if (WHICH & 2) b += __shfl_xor(b, o, 64);
-      }
-      if (lane == 0) {
-        S.red[k] = a;
-        S.red[NM + k] = b;
-      }
-    }
-    wave_sync();
+    // wave reduction of the per-lane moments (through LDS: the point pool is
+    // free now), plus the table sums
+    if constexpr ((WHICH & 1) != 0) wave_sum_lds<NM>(mvol, &S.pool[0][0][0], S.red, lane);
+    if constexpr ((WHICH & 2) != 0) wave_sum_lds<NM>(mif, &S.pool[0][0][0], S.red + NM, lane);
     if (WHICH & 1) {
       if (lane < NM) S.red[lane] += S.tab[lane];
       if (lane + 64 < NM) S.red[64 + lane] += S.tab[64 + lane];
@@ -2048,33 +2065,10 @@ __global__ __launch_bounds__(64, 2) void k_sdx_leaves(int nel, const int32_t *__
       wave_sync();
     }
     // wave reduction of the per-lane moments through LDS (the point pool is
-    // free now): SDX_RB moments per round, lane kk of the round sums column
-    // kk, columns padded to 65 doubles so the summing lanes hit distinct
-    // banks (the staged kernels run one part per launch)
+    // free now; the staged kernels run one part per launch)
     static_assert(WHICH == 1 || WHICH == 2, "one part per launch");
-    constexpr int SDX_RB = 23;  // 23 x 65 doubles fit the 64 x 24 pool
-#pragma unroll
-    for (int k0 = 0; k0 < NM; k0 += SDX_RB) {
-#pragma unroll
-      for (int kk = 0; kk < SDX_RB; ++kk) {
-        if (k0 + kk < NM) {
-          if constexpr ((WHICH & 1) != 0) pool[kk * 65 + lane] = mvol[k0 + kk < NM ? k0 + kk : 0];
-          else pool[kk * 65 + lane] = mif[k0 + kk < NM ? k0 + kk : 0];
-        }
-      }
-      wave_sync();
-      if (lane < SDX_RB && k0 + lane < NM) {
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-        for (int j = 0; j < 64; j += 4) {
-          s0 += pool[lane * 65 + j];
-          s1 += pool[lane * 65 + j + 1];
-          s2 += pool[lane * 65 + j + 2];
-          s3 += pool[lane * 65 + j + 3];
-        }
-        S.red[((WHICH & 1) ? 0 : NM) + k0 + lane] = (s0 + s1) + (s2 + s3);
-      }
-      wave_sync();
-    }
+    if constexpr ((WHICH & 1) != 0) wave_sum_lds<NM>(mvol, pool, S.red, lane);
+    else wave_sum_lds<NM>(mif, pool, S.red + NM, lane);
     for (int k = lane; k < NM; k += 64) {
       if ((WHICH & 1) && out_vol) out_vol[(size_t)i * out_stride + k] = S.red[k];
       if ((WHICH & 2) && out_if) out_if[(size_t)i * out_stride + k] = S.red[NM + k];
@@ -2096,46 +2090,88 @@ __global__ __launch_bounds__(64, 2) void k_sdx_leaves(int nel, const int32_t *__
 
 // table rows of each listed element (uncut nodes of the kept side from the
 // back of the list, complemented leaves after the cut leaves) added to its
-// moments; one wave per element, lane = moment (and moment + 64)
+// moments; one wave per element.  Rows of an even length (p = 1, 3): R =
+// NM / 4 lanes per row, each loading 4 moments as two 16-byte loads, so a
+// wave reads 64 / R rows per instruction (3 for p = 3), four rounds of them
+// in flight; the rows' partial sums meet in LDS.  p = 2 (35 moments, rows not
+// 16-byte aligned): lane = moment (and moment + 64), four rows in flight.
 template <int PM>
 __global__ __launch_bounds__(256) void k_sdx_rows(int nel, const uint16_t *__restrict__ lists, int cap,
                                                   const int4 *__restrict__ cnt, const double *__restrict__ nmom,
                                                   double *__restrict__ out, int out_stride) {
   constexpr int NM = Moments<3, PM>::NM;
-  const int lane = threadIdx.x & 63;
+  constexpr int R = (NM + 3) / 4, G = 64 / R;  // lanes per row, rows per instruction
+  __shared__ double scr[4][G][4 * R];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int nw = gridDim.x * (blockDim.x >> 6);
   for (int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < nel; i += nw) {
     const int4 c4 = cnt[i];
     if (c4.x < 0) continue;
     const uint16_t *lst = lists + (size_t)i * cap;
     const int nrow = c4.y + c4.z;
-    // four rows in flight per lane (moment lane, moment lane + 64)
-    double a[4] = {0.0, 0.0, 0.0, 0.0}, b[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int r0 = 0; r0 < nrow; r0 += 64) {
-      // 64 row ids per round, one per lane, then broadcast
-      const int rr = r0 + lane;
-      int id = 0;
-      if (rr < nrow) id = rr < c4.y ? lst[cap - 1 - rr] : lst[c4.x + rr - c4.y];
-      const int m = min(64, nrow - r0);
-      int j = 0;
-      for (; j + 4 <= m; j += 4) {
-        size_t ro[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) ro[u] = (size_t)__shfl(id, j + u, 64) * NM;
+    if constexpr (NM % 2 == 0) {
+      const int slot = lane / R, sub = lane - slot * R, c0 = 4 * sub;
+      const bool act = slot < G;
+      double a[4][4] = {};
+      for (int r0 = 0; r0 < nrow; r0 += 4 * G) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          if (lane < NM) a[u] += nmom[ro[u] + lane];
-          if (lane + 64 < NM) b[u] += nmom[ro[u] + 64 + lane];
+          const int rr = r0 + u * G + slot;
+          if (act && rr < nrow) {
+            const int id = rr < c4.y ? lst[cap - 1 - rr] : lst[c4.x + rr - c4.y];
+            const double2 *src = reinterpret_cast<const double2 *>(nmom + (size_t)id * NM + c0);
+            const double2 x = src[0];
+            a[u][0] += x.x;
+            a[u][1] += x.y;
+            if (c0 + 2 < NM) {
+              const double2 y = src[1];
+              a[u][2] += y.x;
+              a[u][3] += y.y;
+            }
+          }
         }
       }
-      for (; j < m; ++j) {
-        const size_t ra = (size_t)__shfl(id, j, 64) * NM;
-        if (lane < NM) a[0] += nmom[ra + lane];
-        if (lane + 64 < NM) b[0] += nmom[ra + 64 + lane];
+      if (act) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) scr[w][slot][c0 + c] = (a[0][c] + a[1][c]) + (a[2][c] + a[3][c]);
       }
+      wave_sync();
+      for (int k = lane; k < NM; k += 64) {
+        double t = 0.0;
+#pragma unroll
+        for (int g = 0; g < G; ++g) t += scr[w][g][k];
+        out[(size_t)i * out_stride + k] += t;
+      }
+      wave_sync();
+    } else {
+      // four rows in flight per lane (moment lane, moment lane + 64)
+      double a[4] = {0.0, 0.0, 0.0, 0.0}, b[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int r0 = 0; r0 < nrow; r0 += 64) {
+        // 64 row ids per round, one per lane, then broadcast
+        const int rr = r0 + lane;
+        int id = 0;
+        if (rr < nrow) id = rr < c4.y ? lst[cap - 1 - rr] : lst[c4.x + rr - c4.y];
+        const int m = min(64, nrow - r0);
+        int j = 0;
+        for (; j + 4 <= m; j += 4) {
+          size_t ro[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) ro[u] = (size_t)__shfl(id, j + u, 64) * NM;
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            if (lane < NM) a[u] += nmom[ro[u] + lane];
+            if (lane + 64 < NM) b[u] += nmom[ro[u] + 64 + lane];
+          }
+        }
+        for (; j < m; ++j) {
+          const size_t ra = (size_t)__shfl(id, j, 64) * NM;
+          if (lane < NM) a[0] += nmom[ra + lane];
+          if (lane + 64 < NM) b[0] += nmom[ra + 64 + lane];
+        }
+      }
+      if (lane < NM) out[(size_t)i * out_stride + lane] += (a[0] + a[1]) + (a[2] + a[3]);
+      if (lane + 64 < NM) out[(size_t)i * out_stride + 64 + lane] += (b[0] + b[1]) + (b[2] + b[3]);
     }
-    if (lane < NM) out[(size_t)i * out_stride + lane] += (a[0] + a[1]) + (a[2] + a[3]);
-    if (lane + 64 < NM) out[(size_t)i * out_stride + 64 + lane] += (b[0] + b[1]) + (b[2] + b[3]);
   }
 }
 
