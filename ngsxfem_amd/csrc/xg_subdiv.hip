@@ -1549,6 +1549,7 @@ struct SdcLds {
   uint16_t *cur, *nxt;
   uint32_t *latq;     // lattice offsets of levels 0..3
   double *cl;         // 20
+  uint16_t *fq;       // SDX_CH: a chunk's table rows
 };
 
 __host__ __device__ inline size_t sdc_bytes(int L, SdcLds *s, char *base) {
@@ -1567,7 +1568,9 @@ __host__ __device__ inline size_t sdc_bytes(int L, SdcLds *s, char *base) {
   const size_t a_nxt = take(sizeof(uint16_t) * sd_maxcut(L));
   const size_t a_lq = take(sizeof(uint32_t) * SDX_LATQ);
   const size_t a_cl = take(sizeof(double) * 20);
+  const size_t a_fq = take(sizeof(uint16_t) * SDX_CH);
   if (s) {
+    s->fq = (uint16_t *)(base + a_fq);
     s->latbits = (uint32_t *)(base + a_lb);
     s->rowst = (uint16_t *)(base + a_rs);
     s->nodeV = (uint4 *)(base + a_nv);
@@ -1610,15 +1613,19 @@ __device__ __forceinline__ int next_item(int32_t *queue, int lane) {
   return __shfl(nx, 0, 64);
 }
 
-// SIDE: NEG or POS (rows of that side), or -1 (cut leaves only)
-template <int LO, int SIDE>
+// SIDE: NEG or POS (the uncut nodes of that side: their table rows are
+// summed here, lane-parallel over the moments, and written to out), or -1
+// (cut leaves only)
+template <int LO, int PM, int SIDE>
 __global__ __launch_bounds__(64) void k_sdx_tree(int nel, const int32_t *__restrict__ elems,
                                                  const double *__restrict__ lset_nodal, int L,
                                                  const uint4 *__restrict__ nverts, uint16_t *__restrict__ lists,
                                                  int cap, int4 *__restrict__ cnt, uint8_t *__restrict__ dom_out,
                                                  int32_t *__restrict__ over, int32_t *__restrict__ nover,
-                                                 int32_t *err) {
+                                                 const double *__restrict__ nmom, double *__restrict__ out,
+                                                 int out_stride, int32_t *err) {
   constexpr int NL = ndof(3, LO);
+  constexpr int NM = Moments<3, PM>::NM;
   extern __shared__ char smem[];
   SdcLds S;
   sdc_bytes(L, &S, smem);
@@ -1639,7 +1646,9 @@ __global__ __launch_bounds__(64) void k_sdx_tree(int nel, const int32_t *__restr
     if (lane < NL) S.cl[lane] = lset_nodal[(size_t)e * NL + lane];
     wave_sync();
     const double *cl = S.cl;
-    // signs of the level set on the finest lattice (one ballot per 64 points)
+    // signs of the level set on the finest lattice (one ballot per 64
+    // points); the root's flags (its lattice is the whole lattice) follow
+    uint32_t rootf = 0u;
     for (int q0 = 0; q0 < nlat; q0 += 64) {
       const int q = q0 + lane;
       bool nonneg = false;
@@ -1650,12 +1659,16 @@ __global__ __launch_bounds__(64) void k_sdx_tree(int nel, const int32_t *__restr
         nonneg = lagrange_eval3<LO>(cl, x) >= 0.0;
       }
       const unsigned long long b = __ballot(nonneg);
+      const unsigned long long bn = __ballot(q < nlat && !nonneg);
+      rootf |= (b ? 1u : 0u) | (bn ? 2u : 0u);
       if (lane == 0) {
         S.latbits[(q0 >> 5)] = (uint32_t)b;
         S.latbits[(q0 >> 5) + 1] = (uint32_t)(b >> 32);
       }
     }
     uint16_t *lst = lists + (size_t)i * cap;
+    // table-row sums of the kept side: moments lane and lane + 64
+    double ta = 0.0, tb = 0.0;
     int nl = 0, nf = 0;
     bool ok = true, overflow = false;
     int dom = -1;
@@ -1682,7 +1695,10 @@ __global__ __launch_bounds__(64) void k_sdx_tree(int nel, const int32_t *__restr
         // sign bits of every (candidate, lattice point) pair, OR-ed per
         // candidate with ballots (the pairs of one pass cover consecutive
         // candidates)
-        if (r > 0 && r <= 2) {
+        if (k == 0 && r > 0) {
+          if (lane == 0) S.flags[0] = rootf;
+          wave_sync();
+        } else if (r > 0 && r <= 2) {
           for (int c = lane; c < nc; c += 64) {
             int W[4][3];
             unpack_uint4(S.nodeV[c], shift, W);
@@ -1735,6 +1751,7 @@ __global__ __launch_bounds__(64) void k_sdx_tree(int nel, const int32_t *__restr
           wave_sync();
         }
         // classification and the lists
+        int nfc = 0;  // this chunk's table rows (S.fq)
         for (int c0 = 0; c0 < nc; c0 += 64) {
           const int c = c0 + lane;
           const bool valid = c < nc;
@@ -1766,22 +1783,42 @@ __global__ __launch_bounds__(64) void k_sdx_tree(int nel, const int32_t *__restr
           const bool leafcut = valid && lc == IF;
           const bool wantnext = valid && cls == IF && r > 0;
           int cf, cl2, cn;
-          const int sf = wave_slot(wantfill, nf, lane, cf);
+          const int sf = wave_slot(wantfill, nfc, lane, cf);
           const int sl = wave_slot(leafcut, nl, lane, cl2);
           const int sn = wave_slot(wantnext, nnxt, lane, cn);
-          // room for the rows, the cut leaves and their complements
-          if (2 * (nl + cl2) + nf + cf > cap) {
+          // room for the cut leaves and their complements
+          if (2 * (nl + cl2) > cap) {
             overflow = true;
             break;
           }
-          if (wantfill) lst[cap - 1 - sf] = (uint16_t)(koff + path);
+          if (wantfill) S.fq[sf] = (uint16_t)(koff + path);
           if (leafcut) lst[sl] = (uint16_t)(koff + path);
           if (wantnext) S.nxt[sn] = (uint16_t)path;
-          nf += cf;
+          nfc += cf;
           nl += cl2;
           nnxt += cn;
         }
         wave_sync();
+        // the chunk's table rows, four per round trip
+        if constexpr (SIDE >= 0) {
+          if (!overflow) {
+            int f = 0;
+            for (; f + 4 <= nfc; f += 4) {
+              const size_t r0 = (size_t)S.fq[f] * NM, r1 = (size_t)S.fq[f + 1] * NM;
+              const size_t r2 = (size_t)S.fq[f + 2] * NM, r3 = (size_t)S.fq[f + 3] * NM;
+              if (lane < NM) ta += (nmom[r0 + lane] + nmom[r1 + lane]) + (nmom[r2 + lane] + nmom[r3 + lane]);
+              if (lane + 64 < NM)
+                tb += (nmom[r0 + 64 + lane] + nmom[r1 + 64 + lane]) + (nmom[r2 + 64 + lane] + nmom[r3 + 64 + lane]);
+            }
+            for (; f < nfc; ++f) {
+              const size_t r0 = (size_t)S.fq[f] * NM;
+              if (lane < NM) ta += nmom[r0 + lane];
+              if (lane + 64 < NM) tb += nmom[r0 + 64 + lane];
+            }
+          }
+          wave_sync();
+        }
+        nf += nfc;
       }
       uint16_t *tmp = S.cur;
       S.cur = S.nxt;
@@ -1795,8 +1832,15 @@ __global__ __launch_bounds__(64) void k_sdx_tree(int nel, const int32_t *__restr
         cnt[i] = make_int4(-1, 0, 0, 0);
         over[atomicAdd(nover, 1)] = i;
       } else {
-        cnt[i] = make_int4(nl, nf, 0, 0);
+        // (.y: rows left for k_sdx_rows, none; .w: the rows summed here)
+        cnt[i] = make_int4(nl, 0, 0, nf);
         if (!allok) atomicOr(err, 2);
+      }
+    }
+    if constexpr (SIDE >= 0) {
+      if (!overflow) {
+        if (lane < NM) out[(size_t)i * out_stride + lane] = ta;
+        if (lane + 64 < NM) out[(size_t)i * out_stride + 64 + lane] = tb;
       }
     }
     wave_sync();
@@ -2070,14 +2114,15 @@ __global__ __launch_bounds__(64, 2) void k_sdx_leaves(int nel, const int32_t *__
     if constexpr ((WHICH & 1) != 0) wave_sum_lds<NM>(mvol, pool, S.red, lane);
     else wave_sum_lds<NM>(mif, pool, S.red + NM, lane);
     for (int k = lane; k < NM; k += 64) {
-      if ((WHICH & 1) && out_vol) out_vol[(size_t)i * out_stride + k] = S.red[k];
+      // (the volume side adds to the walk's table-row sums)
+      if ((WHICH & 1) && out_vol) out_vol[(size_t)i * out_stride + k] += S.red[k];
       if ((WHICH & 2) && out_if) out_if[(size_t)i * out_stride + k] = S.red[NM + k];
     }
     const bool allok = __ballot(!ok) == 0ull;
     if (lane == 0) {
       if (WHICH & 1) cnt[i].z = ncomp;
       if (stats) {
-        stats[(size_t)i * stats_stride + 0] = (WHICH & 1) ? c4.y + ncomp : 0;
+        stats[(size_t)i * stats_stride + 0] = (WHICH & 1) ? c4.y + c4.w + ncomp : 0;
         stats[(size_t)i * stats_stride + 1] = st_vol;
         stats[(size_t)i * stats_stride + 2] = st_if;
         stats[(size_t)i * stats_stride + 3] = nleaf;
@@ -2178,19 +2223,21 @@ __global__ __launch_bounds__(256) void k_sdx_rows(int nel, const uint16_t *__res
 constexpr int SDC_CAP = 4096;       // list entries per element (uint16)
 constexpr int SDC_SUB = 1 << 19;    // elements per staged round
 
-template <int LO>
+template <int LO, int PM>
 int launch_sdx_tree(xg_ctx *ctx, int side, int nel, const int32_t *elems, const double *lset, int L,
-                    uint16_t *lists, int cap, int4 *cnt, uint8_t *dom, int32_t *over, int32_t *nover) {
+                    uint16_t *lists, int cap, int4 *cnt, uint8_t *dom, int32_t *over, int32_t *nover, double *out,
+                    int out_stride) {
   const size_t lds = sdc_bytes(L, nullptr, nullptr);
   const int blocks = nel < (1 << 22) ? nel : (1 << 22);
   KernelTimer tm(ctx, "sdx_tree");
 #define XG_TREE(S)                                                                                               \
   {                                                                                                              \
     if (lds > 65536)                                                                                             \
-      XG_CHECK(hipFuncSetAttribute((const void *)k_sdx_tree<LO, S>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                                   (int)lds));                                                                   \
-    k_sdx_tree<LO, S><<<blocks, 64, lds, ctx->stream>>>(nel, elems, lset, L, (const uint4 *)ctx->sdx_verts,      \
-                                                         lists, cap, cnt, dom, over, nover, ctx->d_err);         \
+      XG_CHECK(hipFuncSetAttribute((const void *)k_sdx_tree<LO, PM, S>,                                          \
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                       \
+    k_sdx_tree<LO, PM, S><<<blocks, 64, lds, ctx->stream>>>(nel, elems, lset, L, (const uint4 *)ctx->sdx_verts,  \
+                                                             lists, cap, cnt, dom, over, nover, ctx->sdx_mom[PM], \
+                                                             out, out_stride, ctx->d_err);                         \
   }
   if (side == NEG) XG_TREE(NEG)
   else if (side == POS) XG_TREE(POS)
@@ -2289,7 +2336,11 @@ int subdiv_staged(xg_ctx *ctx, int nel, const int32_t *elems, const double *lset
       }
       k_zero_i32<<<1, 1, 0, ctx->stream>>>(nover);
       XG_LAUNCH_CHECK();
-      rc = launch_sdx_tree<LO>(ctx, side, m, el, lset, L, lists, cap, cnt, nullptr, over, nover);
+      // the volume side's table-row sums go straight to its moments
+      double *ovol = nullptr;
+      for (int u = 0; u < ndt; ++u)
+        if (dts[u] == side) ovol = mo + (size_t)u * NM;
+      rc = launch_sdx_tree<LO, PM>(ctx, side, m, el, lset, L, lists, cap, cnt, nullptr, over, nover, ovol, mstride);
       if (rc != XG_OK) return rc;
       int h_nover = 0;
       XG_CHECK(hipMemcpyAsync(&h_nover, nover, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
