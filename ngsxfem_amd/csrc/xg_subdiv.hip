@@ -1667,7 +1667,8 @@ __global__ __launch_bounds__(64) void k_sdx_tree(int nel, const int32_t *__restr
       }
     }
     uint16_t *lst = lists + (size_t)i * cap;
-    // table-row sums of the kept side: moments lane and lane + 64
+    // table-row sums of the kept side: moments lane and lane + 64 (16-byte
+    // loads of a few rows per instruction measured slower here: registers)
     double ta = 0.0, tb = 0.0;
     int nl = 0, nf = 0;
     bool ok = true, overflow = false;
@@ -1717,6 +1718,33 @@ __global__ __launch_bounds__(64) void k_sdx_tree(int nel, const int32_t *__restr
               f |= lat_bit(S.latbits, lat_index_rs(S.rowst, P[0], P[1], P[2], n)) ? 1u : 2u;
             }
             S.flags[c] = f;
+          }
+          wave_sync();
+        } else if (r > 0 && nc <= 8) {
+          // a few candidates with many lattice points: 8 lanes per candidate
+          const int cand = lane >> 3, sub = lane & 7;
+          uint32_t f = 0u;
+          if (cand < nc) {
+            int W[4][3];
+            unpack_uint4(S.nodeV[cand], shift, W);
+            int E[3][3];
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+#pragma unroll
+              for (int d = 0; d < 3; ++d) E[m][d] = W[m + 1][d] - W[0][d];
+            for (int t = sub; t < np; t += 8) {
+              const unsigned pk = (r <= 3) ? S.latq[poff + t] : xg_lat_d[poff + t];
+              const int I0 = (int)(pk & 255u), I1 = (int)((pk >> 8) & 255u), I2 = (int)(pk >> 16);
+              int P[3];
+#pragma unroll
+              for (int d = 0; d < 3; ++d) P[d] = W[0][d] + ((I0 * E[0][d] + I1 * E[1][d] + I2 * E[2][d]) >> r);
+              f |= lat_bit(S.latbits, lat_index_rs(S.rowst, P[0], P[1], P[2], n)) ? 1u : 2u;
+            }
+          }
+          for (int cc = 0; cc < nc; ++cc) {
+            const unsigned long long b1 = __ballot(cand == cc && (f & 1u));
+            const unsigned long long b2 = __ballot(cand == cc && (f & 2u));
+            if (lane == 0) S.flags[cc] = (b1 ? 1u : 0u) | (b2 ? 2u : 0u);
           }
           wave_sync();
         } else if (r > 0) {
