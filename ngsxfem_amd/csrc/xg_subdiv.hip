@@ -1040,7 +1040,8 @@ __device__ __forceinline__ bool lat_bit(const uint32_t *bits, int idx) { return 
 // hit distinct banks).  Cheaper than 6 shuffle steps per entry.  Every lane
 // of the wave calls it; it ends with dst written and visible to the wave.
 template <int NM>
-__device__ __forceinline__ void wave_sum_lds(const double *m, double *scr, double *dst, int lane) {
+__device__ __forceinline__ void wave_sum_lds(const double *m, double *scr, double *dst, int lane,
+                                             double *dst_hi = nullptr, int split = 64) {
   constexpr int RB = 23;
 #pragma unroll
   for (int k0 = 0; k0 < NM; k0 += RB) {
@@ -1049,14 +1050,26 @@ __device__ __forceinline__ void wave_sum_lds(const double *m, double *scr, doubl
       if (k0 + kk < NM) scr[kk * 65 + lane] = m[k0 + kk < NM ? k0 + kk : 0];
     wave_sync();
     if (lane < RB && k0 + lane < NM) {
-      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-      for (int j = 0; j < 64; j += 4) {
-        s0 += scr[lane * 65 + j];
-        s1 += scr[lane * 65 + j + 1];
-        s2 += scr[lane * 65 + j + 2];
-        s3 += scr[lane * 65 + j + 3];
+      if (!dst_hi) {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        for (int j = 0; j < 64; j += 4) {
+          s0 += scr[lane * 65 + j];
+          s1 += scr[lane * 65 + j + 1];
+          s2 += scr[lane * 65 + j + 2];
+          s3 += scr[lane * 65 + j + 3];
+        }
+        dst[k0 + lane] = (s0 + s1) + (s2 + s3);
+      } else {
+        // two sums: lanes below `split` into dst, the others into dst_hi
+        double s0 = 0.0, s1 = 0.0, h0 = 0.0, h1 = 0.0;
+        for (int j = 0; j < 64; j += 2) {
+          const double v0 = scr[lane * 65 + j], v1 = scr[lane * 65 + j + 1];
+          if (j < split) s0 += v0; else h0 += v0;
+          if (j + 1 < split) s1 += v1; else h1 += v1;
+        }
+        dst[k0 + lane] = s0 + s1;
+        dst_hi[k0 + lane] = h0 + h1;
       }
-      dst[k0 + lane] = (s0 + s1) + (s2 + s3);
     }
     wave_sync();
   }
@@ -1670,7 +1683,7 @@ __global__ __launch_bounds__(64) void k_sdx_tree(int nel, const int32_t *__restr
     // table-row sums of the kept side: moments lane and lane + 64 (16-byte
     // loads of a few rows per instruction measured slower here: registers)
     double ta = 0.0, tb = 0.0;
-    int nl = 0, nf = 0;
+    int nl = 0, nf = 0, n22 = 0;
     bool ok = true, overflow = false;
     int dom = -1;
     int ncur = 1;
@@ -1796,6 +1809,7 @@ __global__ __launch_bounds__(64) void k_sdx_tree(int nel, const int32_t *__restr
           const int cls = (f == 3u) ? IF : ((f == 1u) ? POS : NEG);
           if (k == 0) dom = __shfl(cls, 0, 64);
           int lc = -2;
+          bool lc22 = false;
           if (valid && cls == IF && r == 0) {
             double vv[4];
 #pragma unroll
@@ -1805,6 +1819,11 @@ __global__ __launch_bounds__(64) void k_sdx_tree(int nel, const int32_t *__restr
             }
             lc = leaf_check(vv);
             if (lc < 0) ok = false;
+            // two vertices on each side (zero counts POS): a 2-2 cut leaf
+            int npos = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) npos += vv[j] >= 0.0 ? 1 : 0;
+            if (lc == IF && npos == 2) lc22 = true;
           }
           const int path = (k == 0 || !valid) ? 0 : (int)S.cur[(cbase + c) >> 3] * 8 + ((cbase + c) & 7);
           const bool wantfill = SIDE >= 0 && valid && (cls == SIDE || lc == SIDE);
@@ -1825,6 +1844,7 @@ __global__ __launch_bounds__(64) void k_sdx_tree(int nel, const int32_t *__restr
           nfc += cf;
           nl += cl2;
           nnxt += cn;
+          n22 += __popcll(__ballot(lc22));
         }
         wave_sync();
         // the chunk's table rows, four per round trip
@@ -1860,8 +1880,9 @@ __global__ __launch_bounds__(64) void k_sdx_tree(int nel, const int32_t *__restr
         cnt[i] = make_int4(-1, 0, 0, 0);
         over[atomicAdd(nover, 1)] = i;
       } else {
-        // (.y: rows left for k_sdx_rows, none; .w: the rows summed here)
-        cnt[i] = make_int4(nl, 0, 0, nf);
+        // (.y: the 2-2 cut leaves, .z: complemented leaves (the leaf
+        // kernel), .w: the table rows summed here)
+        cnt[i] = make_int4(nl, n22, 0, nf);
         if (!allok) atomicOr(err, 2);
       }
     }
@@ -1889,9 +1910,15 @@ __global__ __launch_bounds__(64, 2) void k_sdx_leaves(int nel, const int32_t *__
                                                      int4 *__restrict__ cnt, double *__restrict__ out_vol,
                                                      double *__restrict__ out_if, int out_stride,
                                                      int32_t *__restrict__ stats, int stats_stride, int32_t *err,
-                                                     int32_t *__restrict__ queue) {
+                                                     int32_t *__restrict__ queue, int32_t *__restrict__ stats_if) {
   constexpr int NL = ndof(3, LO);
   constexpr int NM = Moments<3, PM>::NM;
+  // WHICH == 3 (both parts in one launch, p = 3): lanes [0, nn) integrate the
+  // volume pieces and lanes [nn, 64) the interface pieces, each lane into
+  // one set of 84 sums, with one task loop (blocks of 4 points, a triangle
+  // being a tet whose fourth rule coordinate is 0); nn follows the element's
+  // expected point split
+  static_assert(WHICH != 3 || PM == 3, "both parts in one launch: p = 3");
   constexpr int SDX_VB = PM == 1 ? 4 : (PM == 2 ? 7 : 8), SDX_NVB = PM == 1 ? 1 : (PM == 2 ? 2 : 3);
   constexpr int SDX_IB = PM == 1 ? 3 : (PM == 2 ? 6 : 4), SDX_NIB = PM == 1 ? 1 : (PM == 2 ? 1 : 3);
   extern __shared__ char smem[];
@@ -1956,12 +1983,22 @@ __global__ __launch_bounds__(64, 2) void k_sdx_leaves(int nel, const int32_t *__
     }
     wave_sync();
     const double *cl = S.cl;
-    double mvol[(WHICH & 1) ? NM : 1], mif[(WHICH & 2) ? NM : 1];
+    double mvol[(WHICH & 1) ? NM : 1], mif[(WHICH == 2) ? NM : 1];
 #pragma unroll
     for (int k = 0; k < ((WHICH & 1) ? NM : 1); ++k) mvol[k] = 0.0;
 #pragma unroll
-    for (int k = 0; k < ((WHICH & 2) ? NM : 1); ++k) mif[k] = 0.0;
+    for (int k = 0; k < ((WHICH == 2) ? NM : 1); ++k) mif[k] = 0.0;
     const int nleaf = c4.x;
+    int nn = 64;  // lanes of the volume role (WHICH == 3)
+    if constexpr (WHICH == 3) {
+      // points: 24 per volume tet, 12 per triangle; a 1-3 cut leaf has 1
+      // tet and 1 triangle, a 2-2 cut leaf 3 tets and 2 triangles (c4.y:
+      // the walk's count of 2-2 leaves)
+      const int n22 = c4.y, n13 = nleaf - n22;
+      const double vp = 24.0 * (n13 + 3 * n22), ip = 12.0 * (n13 + 2 * n22);
+      nn = (vp + ip > 0.0) ? (int)(64.0 * vp / (vp + ip) + 0.5) : 44;
+      nn = nn < 16 ? 16 : (nn > 56 ? 56 : nn);
+    }
     uint16_t *lst = lists + (size_t)i * cap;
     int ncomp = 0, st_vol = 0, st_if = 0;
     bool ok = true;
@@ -2073,7 +2110,43 @@ __global__ __launch_bounds__(64, 2) void k_sdx_leaves(int nel, const int32_t *__
         S.ifw[q] = wf;
       }
       wave_sync();
-      if (WHICH & 1) {
+      if constexpr (WHICH == 3) {
+        // one task loop for both roles: (piece, block of 4 points) tasks
+        const int ni = 64 - nn;
+        const int nvt = tot_vol * 6, nit = tot_if * 3;
+        const int rounds = max((nvt + nn - 1) / nn, (nit + ni - 1) / ni);
+        const bool isv = lane < nn;
+        const int rl = isv ? lane : lane - nn, rs = isv ? nn : ni;
+        for (int rd = 0; rd < rounds; ++rd) {
+          const int q = rd * rs + rl;
+          if (isv ? (q < nvt) : (q < nit)) {
+            const int pc = isv ? q / 6 : q / 3;
+            const int t0 = 4 * (isv ? q - pc * 6 : q - pc * 3);
+            const uint32_t rec = isv ? S.volrec[pc] : S.ifrec[pc];
+            const double wf = isv ? S.voltf[pc] : S.ifw[pc];
+            const double(*rule)[4] = isv ? S.rule_vol : S.rule_if;
+            const int ll = rec & 63;
+            double p0[3], p1[3], p2[3], p3[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              p0[d] = pool[(((rec >> 8) & 7) * 3 + d) * 64 + ll];
+              p1[d] = pool[(((rec >> 11) & 7) * 3 + d) * 64 + ll];
+              p2[d] = pool[(((rec >> 14) & 7) * 3 + d) * 64 + ll];
+              p3[d] = pool[(((rec >> 17) & 7) * 3 + d) * 64 + ll];
+            }
+            for (int t = t0; t < t0 + 4; ++t) {
+              const double r0 = rule[t][0], r1 = rule[t][1], r2 = rule[t][2], rw = rule[t][3];
+              const double ow = 1.0 - r0 - r1 - r2;
+              double lam[4];
+#pragma unroll
+              for (int d = 0; d < 3; ++d) lam[d] = ow * p0[d] + r0 * p1[d] + r1 * p2[d] + r2 * p3[d];
+              lam[3] = 1.0 - lam[0] - lam[1] - lam[2];
+              Moments<3, PM>::accum(lam, rw * wf, mvol);
+            }
+          }
+        }
+      }
+      if (WHICH == 1) {
         for (int q = lane; q < tot_vol * SDX_NVB; q += 64) {
           const int pc = q / SDX_NVB, t0 = (q - pc * SDX_NVB) * SDX_VB;
           const uint32_t rec = S.volrec[pc];
@@ -2105,7 +2178,7 @@ __global__ __launch_bounds__(64, 2) void k_sdx_leaves(int nel, const int32_t *__
           }
         }
       }
-      if (WHICH & 2) {
+      if constexpr (WHICH == 2) {
         for (int q = lane; q < tot_if * SDX_NIB; q += 64) {
           const int pc = q / SDX_NIB, t0 = (q - pc * SDX_NIB) * SDX_IB;
           const uint32_t rec = S.ifrec[pc];
@@ -2137,9 +2210,9 @@ __global__ __launch_bounds__(64, 2) void k_sdx_leaves(int nel, const int32_t *__
       wave_sync();
     }
     // wave reduction of the per-lane moments through LDS (the point pool is
-    // free now; the staged kernels run one part per launch)
-    static_assert(WHICH == 1 || WHICH == 2, "one part per launch");
-    if constexpr ((WHICH & 1) != 0) wave_sum_lds<NM>(mvol, pool, S.red, lane);
+    // free now)
+    if constexpr (WHICH == 3) wave_sum_lds<NM>(mvol, pool, S.red, lane, S.red + NM, nn);
+    else if constexpr (WHICH == 1) wave_sum_lds<NM>(mvol, pool, S.red, lane);
     else wave_sum_lds<NM>(mif, pool, S.red + NM, lane);
     for (int k = lane; k < NM; k += 64) {
       // (the volume side adds to the walk's table-row sums)
@@ -2150,10 +2223,16 @@ __global__ __launch_bounds__(64, 2) void k_sdx_leaves(int nel, const int32_t *__
     if (lane == 0) {
       if (WHICH & 1) cnt[i].z = ncomp;
       if (stats) {
-        stats[(size_t)i * stats_stride + 0] = (WHICH & 1) ? c4.y + c4.w + ncomp : 0;
+        stats[(size_t)i * stats_stride + 0] = (WHICH & 1) ? c4.w + ncomp : 0;
         stats[(size_t)i * stats_stride + 1] = st_vol;
-        stats[(size_t)i * stats_stride + 2] = st_if;
+        stats[(size_t)i * stats_stride + 2] = (WHICH == 3) ? 0 : st_if;
         stats[(size_t)i * stats_stride + 3] = nleaf;
+      }
+      if (WHICH == 3 && stats_if) {
+        stats_if[(size_t)i * stats_stride + 0] = 0;
+        stats_if[(size_t)i * stats_stride + 1] = 0;
+        stats_if[(size_t)i * stats_stride + 2] = st_if;
+        stats_if[(size_t)i * stats_stride + 3] = nleaf;
       }
       if (!allok) atomicOr(err, 2);
     }
@@ -2181,7 +2260,8 @@ __global__ __launch_bounds__(256) void k_sdx_rows(int nel, const uint16_t *__res
     const int4 c4 = cnt[i];
     if (c4.x < 0) continue;
     const uint16_t *lst = lists + (size_t)i * cap;
-    const int nrow = c4.y + c4.z;
+    // the complemented leaves (.z) after the cut leaves (.x)
+    const int nrow = c4.z;
     if constexpr (NM % 2 == 0) {
       const int slot = lane / R, sub = lane - slot * R, c0 = 4 * sub;
       const bool act = slot < G;
@@ -2191,7 +2271,7 @@ __global__ __launch_bounds__(256) void k_sdx_rows(int nel, const uint16_t *__res
         for (int u = 0; u < 4; ++u) {
           const int rr = r0 + u * G + slot;
           if (act && rr < nrow) {
-            const int id = rr < c4.y ? lst[cap - 1 - rr] : lst[c4.x + rr - c4.y];
+            const int id = lst[c4.x + rr];
             const double2 *src = reinterpret_cast<const double2 *>(nmom + (size_t)id * NM + c0);
             const double2 x = src[0];
             a[u][0] += x.x;
@@ -2223,7 +2303,7 @@ __global__ __launch_bounds__(256) void k_sdx_rows(int nel, const uint16_t *__res
         // 64 row ids per round, one per lane, then broadcast
         const int rr = r0 + lane;
         int id = 0;
-        if (rr < nrow) id = rr < c4.y ? lst[cap - 1 - rr] : lst[c4.x + rr - c4.y];
+        if (rr < nrow) id = lst[c4.x + rr];
         const int m = min(64, nrow - r0);
         int j = 0;
         for (; j + 4 <= m; j += 4) {
@@ -2279,7 +2359,7 @@ template <int LO, int PM, int SIDE, int WHICH>
 int launch_sdx_leaves(xg_ctx *ctx, int nel, const int32_t *elems, const double *lset, const int32_t *el2v,
                       const double *coords, const double *finv, int L, uint16_t *lists, int cap, int4 *cnt,
                       double *out_vol, double *out_if, int out_stride, int32_t *stats, int stats_stride,
-                      int32_t *queue) {
+                      int32_t *queue, int32_t *stats_if = nullptr) {
   constexpr int NM = Moments<3, PM>::NM;
   const void *kern = (const void *)k_sdx_leaves<LO, PM, SIDE, WHICH>;
   const size_t lds = sdx_bytes(L, xg_minrule_cnt[3][PM], xg_minrule_cnt[2][PM], NM, nullptr, nullptr);
@@ -2295,10 +2375,10 @@ int launch_sdx_leaves(xg_ctx *ctx, int nel, const int32_t *elems, const double *
     k_zero_i32<<<1, 1, 0, ctx->stream>>>(queue);
     XG_LAUNCH_CHECK();
   }
-  KernelTimer tm(ctx, (WHICH & 1) ? "sdx_leaves_vol" : "sdx_leaves_if");
+  KernelTimer tm(ctx, WHICH == 3 ? "sdx_leaves_both" : ((WHICH & 1) ? "sdx_leaves_vol" : "sdx_leaves_if"));
   k_sdx_leaves<LO, PM, SIDE, WHICH><<<blocks, 64, lds, ctx->stream>>>(
       nel, elems, lset, el2v, coords, finv, L, (const uint4 *)ctx->sdx_verts, lists, cap, cnt, out_vol, out_if,
-      out_stride, stats, stats_stride, ctx->d_err, queue);
+      out_stride, stats, stats_stride, ctx->d_err, queue, stats_if);
   XG_LAUNCH_CHECK();
   return XG_OK;
 }
@@ -2373,6 +2453,39 @@ int subdiv_staged(xg_ctx *ctx, int nel, const int32_t *elems, const double *lset
       int h_nover = 0;
       XG_CHECK(hipMemcpyAsync(&h_nover, nover, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
       XG_CHECK(hipStreamSynchronize(ctx->stream));
+      // p = 3 with the volume side and the interface both requested: one
+      // launch for both (lane roles), then the rows, then the overflow path
+      int u_vol = -1, u_if = -1;
+      for (int u = 0; u < ndt; ++u) {
+        if (dts[u] == side) u_vol = u;
+        if (dts[u] == IF && !if_done) u_if = u;
+      }
+      if constexpr (PM == 3) {
+        if (u_vol >= 0 && u_if >= 0 && !getenv("XG_SDX_NOFUSE")) {
+          double *ov = mo + (size_t)u_vol * NM, *oi = mo + (size_t)u_if * NM;
+          int32_t *sv = st ? st + (size_t)u_vol * 4 : nullptr, *si = st ? st + (size_t)u_if * 4 : nullptr;
+          if (side == NEG)
+            rc = launch_sdx_leaves<LO, PM, NEG, 3>(ctx, m, el, lset, el2v, coords, fi, L, lists, cap, cnt, ov, oi,
+                                                   mstride, sv, ndt * 4, queue, si);
+          else
+            rc = launch_sdx_leaves<LO, PM, POS, 3>(ctx, m, el, lset, el2v, coords, fi, L, lists, cap, cnt, ov, oi,
+                                                   mstride, sv, ndt * 4, queue, si);
+          if (rc != XG_OK) return rc;
+          rc = launch_sdx_rows<PM>(ctx, m, lists, cap, cnt, ov, mstride);
+          if (rc != XG_OK) return rc;
+          if (h_nover > 0) {
+            rc = subdiv_exact_pass<LO, PM>(ctx, h_nover, el, lset, el2v, coords, fi, L, side, false, ov, nullptr,
+                                           mstride, nullptr, sv, ndt * 4, over);
+            if (rc != XG_OK) return rc;
+            rc = subdiv_exact_pass<LO, PM>(ctx, h_nover, el, lset, el2v, coords, fi, L, -1, true, nullptr, oi,
+                                           mstride, nullptr, si, ndt * 4, over);
+            if (rc != XG_OK) return rc;
+          }
+          if_done = true;
+          if (t == ndt) break;
+          continue;
+        }
+      }
       for (int u = 0; u < ndt; ++u) {
         const bool vol = dts[u] == side;
         const bool itf = dts[u] == IF && !if_done;
