@@ -219,18 +219,18 @@ def c5_fp64_ops(stats, launches, p=3):
             + itf * (nq_i * (18 + 30 + 21 + 2 * nm) + 40) + leaves * (48 + 4 * 42))
 
 
-def c5_leaf_ops(stats, part, p=3):
-    """FP64 operations (FMA = 2) of the cut-leaf kernel k_sdx_leaves for one
-    part from the per-element counts (stats[..., 0..3] as c5_fp64_ops):
-    part "vol": the volume tets point by point + the cut leaves' pieces;
-    part "if": the interface triangles + the cut leaves' pieces."""
+def c5_leaf_ops(st_vol, st_if, p=3):
+    """FP64 operations (FMA = 2) of the cut-leaf kernel k_sdx_leaves (volume
+    side and interface in one launch) from the per-element counts of the two
+    parts (stats[..., 0..3] as c5_fp64_ops): the volume tets and interface
+    triangles point by point, the cut leaves' pieces once."""
     nm = {1: 10, 2: 35, 3: 84}[p]
     nq_v, nq_i = {1: (4, 3), 2: (14, 6), 3: (24, 12)}[p]
-    vol, itf, leaves = (float(stats[..., k].sum()) for k in (1, 2, 3))
-    pieces = leaves * (48 + 4 * 42)
-    if part == "vol":
-        return vol * (nq_v * (27 + 30 + 21 + 2 * nm) + 17) + pieces
-    return itf * (nq_i * (18 + 30 + 21 + 2 * nm) + 40) + pieces
+    vol = float(st_vol[..., 1].sum())
+    itf = float(st_if[..., 2].sum())
+    leaves = float(st_vol[..., 3].sum())
+    return (vol * (nq_v * (27 + 30 + 21 + 2 * nm) + 17) + itf * (nq_i * (18 + 30 + 21 + 2 * nm) + 40)
+            + leaves * (48 + 4 * 42))
 
 
 def cpu_threads():
@@ -332,17 +332,15 @@ def measure_c5(args):
     for _ in range(3):
         ctx.subdiv_moments_mesh(p, L, order, cl, d_el, d_x, lp2, dts, mom2)
     kt = {}
-    for k in ("sdx_tree", "sdx_leaves_vol", "sdx_leaves_if", "sdx_rows", "subdiv_exact"):
+    for k in ("sdx_tree", "sdx_leaves_both", "sdx_rows", "subdiv_exact"):
         tot, nl = ctx.timing_read(k)
         kt[k] = {"ms_per_step": tot / 3, "launches_per_step": nl / 3}
     ctx.timing_enable(False)
     ctx.subdiv_moments_mesh(p, L, order, cl, d_el, d_x, lp2, dts, mom2, stats)
     ctx.sync()
     st = stats.cpu().numpy().reshape(n_cut, 2, 4)
-    ops = c5_leaf_ops(st[:, 0], "vol")
-    ops_if = c5_leaf_ops(st[:, 1], "if")
-    ms_kern = kt["sdx_leaves_vol"]["ms_per_step"]
-    ms_kif = kt["sdx_leaves_if"]["ms_per_step"]
+    ops = c5_leaf_ops(st[:, 0], st[:, 1])
+    ms_kern = kt["sdx_leaves_both"]["ms_per_step"]
     rows_bytes = float(st[:, 0, 0].sum()) * nm * 8
     if args.pmc_probe:
         # counter runs (rocprofv3 --pmc): a 256 MiB device copy of known size
@@ -356,7 +354,7 @@ def measure_c5(args):
         ctx.sync()
         torch.cuda.synchronize()
         print(json.dumps({"pmc_probe": True, "copy_bytes": 2 * a.numel() * 4, "cut_elements": n_cut,
-                          "leaves_vol_flops_per_step": ops, "leaves_if_flops_per_step": ops_if,
+                          "leaves_flops_per_step": ops,
                           "kernels": kt}), flush=True)
         ctx.close()
         return None
@@ -371,7 +369,8 @@ def measure_c5(args):
     dn = torch.linalg.vector_norm(out.view(-1, nd * nd).double() - r64, dim=1)
     err32 = float((dn[nzr] / rn[nzr]).max().item())
 
-    roof = {"bound": "fp64-valu", "kernel": "k_sdx_leaves<2,3,NEG,1> (NEG cut-leaf pieces, the step's largest)",
+    roof = {"bound": "fp64-valu",
+            "kernel": "k_sdx_leaves<2,3,NEG,3> (NEG and interface cut-leaf pieces, the step's largest)",
             "achieved": ops / (ms_kern * 1e-3) / 1e12, "peak": PEAK_FP64_VALU / 1e12, "unit": "TFLOP/s",
             "frac": ops / (ms_kern * 1e-3) / PEAK_FP64_VALU, "traffic": read_traffic("subdiv_c5_pmc.txt"),
             "traffic_source": "profiles/subdiv_c5_pmc.txt (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the same "
@@ -383,15 +382,11 @@ def measure_c5(args):
                                        "interface_triangles": float(st[:, 1, 2].mean()),
                                        "cut_leaves": float(st[:, 0, 3].mean())},
             "kernels": {
-                "k_sdx_leaves<2,3,NEG,2> (interface)": {
-                    "ms": ms_kif, "bound": "fp64-valu", "algorithmic_flops": ops_if,
-                    "frac": ops_if / (ms_kif * 1e-3) / PEAK_FP64_VALU},
-                "k_sdx_tree<2,NEG> (refinement walk, lists)": {
+                "k_sdx_tree<2,3,NEG> (refinement walk, node lists, the kept side's table rows)": {
                     "ms": kt["sdx_tree"]["ms_per_step"], "bound": "VALU issue (integer lattice checks)"},
-                "k_sdx_rows<3> (table rows)": {
-                    "ms": kt["sdx_rows"]["ms_per_step"], "bound": "L2 (the depth <= 4 rows, 3.1 MB)",
-                    "bytes": rows_bytes,
-                    "GB/s": rows_bytes / (kt["sdx_rows"]["ms_per_step"] * 1e-3) / 1e9},
+                "k_sdx_rows<3> (complemented leaves' table rows)": {
+                    "ms": kt["sdx_rows"]["ms_per_step"], "bound": "L2 (the depth <= 4 rows, 3.1 MB)"},
+                "table_rows_bytes_per_step": rows_bytes,
                 "k_subdiv_exact (list overflows)": {"ms": kt["subdiv_exact"]["ms_per_step"],
                                                     "launches": kt["subdiv_exact"]["launches_per_step"]}},
             "note": "flops counted from the kernels' own per-element piece counts (bench.py c5_leaf_ops); each "
