@@ -66,6 +66,7 @@ def parse():
     ap.add_argument("--c4-train", type=int, default=100000)
     ap.add_argument("--c5-n", type=int, default=128)
     ap.add_argument("--c5-L", type=int, default=4)
+    ap.add_argument("--no-c5-learned", action="store_true", help="skip the learned leg of the large mixed line")
     return ap.parse_args()
 
 
@@ -404,6 +405,9 @@ def measure_c5(args):
            "paths": {"subdivision_ms": ms_sub, "elmat_and_rest_ms": ms - ms_sub},
            "marking": {"ms": ms_mark, "tets_per_s": ne / (ms_mark * 1e-3), "kernel": "xg_mark_p2 (L=4)"}}
 
+    if not args.no_c5_learned:
+        res["learned"] = c5_learned(args, ctx, stream, cl, d_el, d_x, lp2, L, ms)
+
     # ---- CPU baseline: the oracle (reference algorithm: materialised level-4
     # subdivision rule + P3 element matrices per point, FP64) on a bounded
     # sample of the cut elements, all host threads
@@ -436,6 +440,78 @@ def measure_c5(args):
         res["max_rel_err_vs_oracle_sample"] = err_o
     ctx.close()
     return res
+
+
+def c5_learned(args, ctx, stream, cl, d_el, d_x, lp2, L, ms_plain):
+    """The learned rule on the large mixed workload: the NEG moments of the cut
+    elements from the level-1 moments + the MLP correction (p = 3, 84
+    outputs, BF16 MFMA), checked on degree 2 against the level-4 rule, the
+    rejected elements from the level-4 rule; the interface moments from the
+    plain (staged) path; then the P3 element matrices of both.  The network is
+    trained on every other cut element of this workload and judged on all."""
+    import torch
+
+    from ngsxfem_amd import _abi
+    from ngsxfem_amd.learned import LearnedRule
+
+    n = cl.shape[0]
+    lc = lp2[cl.long()].contiguous()
+    t0 = time.time()
+    lr_ = LearnedRule(ctx, p=3, base_L=1, L=L, seed=0)
+    hist = lr_.train(epochs=8, batch=1024, lr=2e-3, seed=3, x_raw=lc[::2].contiguous())
+    ctx.sync()
+    t_train = time.time() - t0
+    nd, nm = 20, 84
+    mif = torch.empty(n * nm, dtype=torch.float64, device="cuda")
+    e_neg = torch.empty(n * nd * nd, dtype=torch.float32, device="cuda")
+    e_if = torch.empty_like(e_neg)
+    bufs = {}
+
+    def step():
+        out, acc, rel = lr_.moments(lc, thr=1e-6, bufs=bufs)
+        ctx.subdiv_moments_mesh(3, L, 6, cl, d_el, d_x, lp2, [_abi.XG_IF], mif)
+        ctx.elmat_moments(3, 3, out, 3, _abi.XG_PREC_FP32, e_neg, nel=n, elems=cl, el2v=d_el, coords=d_x)
+        ctx.elmat_moments(3, 3, mif, 3, _abi.XG_PREC_FP32, e_if, nel=n, elems=cl, el2v=d_el, coords=d_x)
+
+    step()
+    ctx.sync()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(3):
+        step()
+    b.record(stream)
+    ctx.sync()
+    ms = a.elapsed_time(b) / 3
+    # accuracy of the network (before the check) and of the plain level-1
+    # rule, element matrices against level 4 (NEG, FP64)
+    m4 = lr_.plain_moments(lc, L)
+    m1 = lr_.plain_moments(lc, 1)
+    corr = lr_.correction(lc).to(torch.float64)
+
+    def em(m):
+        e = torch.empty(n * nd * nd, dtype=torch.float64, device="cuda")
+        ctx.elmat_moments(3, 3, m, 3, _abi.XG_PREC_FP64, e, nel=n, elems=cl, el2v=d_el, coords=d_x)
+        ctx.sync()
+        return e.view(n, nd * nd)
+
+    e4 = em(m4)
+    n4 = torch.linalg.vector_norm(e4, dim=1).clamp_min(1e-300)
+    q = lambda t: {"median": float(t.median()), "p99": float(torch.quantile(t.float(), 0.99)),  # noqa
+                   "max": float(t.max())}
+    err_nn = torch.linalg.vector_norm(em(m1 + corr) - e4, dim=1) / n4
+    err_l1 = torch.linalg.vector_norm(em(m1) - e4, dim=1) / n4
+    by_thr = {}
+    for thr in (1e-6, 1e-4, 1e-3, 1e-2):
+        out, acc, rel = lr_.moments(lc, thr=thr, bufs={})
+        ctx.sync()
+        by_thr["%g" % thr] = float(acc.float().mean())
+    return {"ms": ms, "cut_elem_per_s": n / (ms * 1e-3), "plain_ms": ms_plain,
+            "includes": "L1 moments (p = 3) + MLP 10-128-128-84 (BF16) + L4 degree-2 moments + check + L4 "
+                        "fallback of the rejected (NEG) + plain interface moments + both P3 elmat",
+            "network_rel_err_elmat_vs_L4": q(err_nn), "level1_rel_err_elmat_vs_L4": q(err_l1),
+            "accepted_frac_by_threshold": by_thr, "train_s": t_train, "train_loss_last": hist[-1],
+            "train_samples": int((n + 1) // 2)}
 
 
 def bench_c5(args):

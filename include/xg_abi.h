@@ -438,7 +438,7 @@ int xg_batch_run(xg_ctx *ctx, xg_batch *b, float *out, long long cap, int use_gr
 #define XG_PREC_BF16 3   /* plain BF16 MFMA, FP32 accumulation */
 int xg_mlp_nparams(int I, int H, int O);
 /* precision XG_PREC_FP32 (VALU; work: 2*B*H floats) or XG_PREC_BF16 (fused
-   MFMA kernel, I = 10, H = 128, O <= 48; work unused). */
+   MFMA kernel, I = 10, H = 128, O <= 96; work unused). */
 int xg_mlp_forward(xg_ctx *ctx, int I, int H, int O, const float *params, int B, const float *x,
                    float *y, float *work, int precision);
 /* One Adam step (FP32 forward + backward); loss: device float[1]; grad

@@ -102,7 +102,7 @@ __global__ void k_adam(int n, float *__restrict__ p, const float *__restrict__ g
 }
 
 // ------------------------------------------------------------ BF16 inference
-constexpr int MI = 10, MH = 128, MOP = 48;  // input, hidden, padded output
+constexpr int MI = 10, MH = 128, MOP = 96;  // input, hidden, padded output
 
 __device__ __forceinline__ short f2bf(float f) {
   // round to nearest even
@@ -250,7 +250,7 @@ extern "C" int xg_mlp_forward(xg_ctx *ctx, int I, int H, int O, const float *par
   if (precision != XG_PREC_BF16)
     return set_error(XG_ERR_UNSUPPORTED, "xg_mlp_forward: precision must be FP32 or BF16");
   if (I != MI || H != MH || O > MOP)
-    return set_error(XG_ERR_UNSUPPORTED, "xg_mlp_forward: BF16 path is built for 10 -> 128 -> 128 -> <=48");
+    return set_error(XG_ERR_UNSUPPORTED, "xg_mlp_forward: BF16 path is built for 10 -> 128 -> 128 -> <=96");
   // packed B operands live in the context scratch
   const size_t n2 = (size_t)8 * 4 * 64 * 8, n3 = (size_t)(MOP / 16) * 4 * 64 * 8;
   int rc = ensure_ws(ctx, sizeof(short) * (n2 + n3));

@@ -91,9 +91,14 @@ class LearnedRule:
         return m
 
     # ------------------------------------------------------------- training
-    def train(self, n_train=200000, epochs=20, batch=1024, lr=1e-3, seed=1, log=None):
+    def train(self, n_train=200000, epochs=20, batch=1024, lr=1e-3, seed=1, log=None, x_raw=None):
+        """Adam on the FP32 kernels; samples: random cut quadrics (n_train), or
+        the given P2 nodal values x_raw (n x 10 FP64, device)"""
         import torch
-        x_raw = random_cut_quadrics(n_train, seed, self.device)
+        if x_raw is None:
+            x_raw = random_cut_quadrics(n_train, seed, self.device)
+        else:
+            n_train = x_raw.shape[0]
         full = self.plain_moments(x_raw, self.L)
         base = self.plain_moments(x_raw, self.base_L)
         self.ctx.sync()

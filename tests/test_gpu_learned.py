@@ -57,9 +57,12 @@ def test_mlp_gradient_vs_oracle(orc, dev):
     assert np.array_equal(dP.cpu().numpy(), P)
 
 
-def test_mlp_forward_fp32_and_bf16(orc, dev):
+@pytest.mark.parametrize("O", [35, 84])
+def test_mlp_forward_fp32_and_bf16(orc, dev, O):
+    """FP32 and BF16 inference vs the FP64 oracle, for the P2 (35) and P3 (84)
+    moment corrections"""
     torch, _abi, ctx = dev
-    I, H, O, B = 10, 128, 35, 1000
+    I, H, B = 10, 128, 1000
     rng = np.random.default_rng(1)
     npar = orc.mlp_nparams(I, H, O)
     P = (0.3 * rng.standard_normal(npar)).astype(np.float32)
