@@ -2393,13 +2393,15 @@ int launch_sdx_rows(xg_ctx *ctx, int nel, const uint16_t *lists, int cap, const 
   return XG_OK;
 }
 
-// Moments of the parts dts[0..ndt) (NEG / POS / IF) of nel elements, written
-// at mom + i * mstride + t * NM, through the staged kernels; the elements
-// whose lists overflow go through the one-kernel path.
+// Moments of the parts dts[0..ndt) (NEG / POS / IF) of nel elements, part u
+// written at outs[u] + i * mstride (stats, nullable: stats_u[u] + i *
+// sstride), through the staged kernels; the elements whose lists overflow
+// go through the one-kernel path.  dom (nullable): each element's domain
+// type on the lattice.
 template <int LO, int PM>
 int subdiv_staged(xg_ctx *ctx, int nel, const int32_t *elems, const double *lset, const int32_t *el2v,
-                  const double *coords, const double *finv, int L, int ndt, const int *dts, double *mom,
-                  int mstride, int32_t *stats) {
+                  const double *coords, const double *finv, int L, int ndt, const int *dts,
+                  double *const *outs, int mstride, int32_t *const *stats_u, int sstride, uint8_t *dom) {
   constexpr int NM = Moments<3, PM>::NM;
   int rc = ensure_sdx_tables(ctx, PM);
   if (rc != XG_OK) return rc;
@@ -2428,8 +2430,13 @@ int subdiv_staged(xg_ctx *ctx, int nel, const int32_t *elems, const double *lset
     const int m = nel - off < sub ? nel - off : sub;
     const int32_t *el = elems ? elems + off : nullptr;
     const double *fi = finv ? finv + (size_t)off * 9 : nullptr;
-    double *mo = mom + (size_t)off * mstride;
-    int32_t *st = stats ? stats + (size_t)off * ndt * 4 : nullptr;
+    double *o_u[3] = {nullptr, nullptr, nullptr};
+    int32_t *s_u[3] = {nullptr, nullptr, nullptr};
+    for (int u = 0; u < ndt; ++u) {
+      o_u[u] = outs[u] + (size_t)off * mstride;
+      s_u[u] = (stats_u && stats_u[u]) ? stats_u[u] + (size_t)off * sstride : nullptr;
+    }
+    uint8_t *dm = dom ? dom + off : nullptr;
     // volume sides in the order requested; the interface rides on the first
     // tree (its cut leaves do not depend on the side)
     bool has_if = false, if_done = false;
@@ -2447,8 +2454,8 @@ int subdiv_staged(xg_ctx *ctx, int nel, const int32_t *elems, const double *lset
       // the volume side's table-row sums go straight to its moments
       double *ovol = nullptr;
       for (int u = 0; u < ndt; ++u)
-        if (dts[u] == side) ovol = mo + (size_t)u * NM;
-      rc = launch_sdx_tree<LO, PM>(ctx, side, m, el, lset, L, lists, cap, cnt, nullptr, over, nover, ovol, mstride);
+        if (dts[u] == side) ovol = o_u[u];
+      rc = launch_sdx_tree<LO, PM>(ctx, side, m, el, lset, L, lists, cap, cnt, dm, over, nover, ovol, mstride);
       if (rc != XG_OK) return rc;
       int h_nover = 0;
       XG_CHECK(hipMemcpyAsync(&h_nover, nover, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
@@ -2462,23 +2469,23 @@ int subdiv_staged(xg_ctx *ctx, int nel, const int32_t *elems, const double *lset
       }
       if constexpr (PM == 3) {
         if (u_vol >= 0 && u_if >= 0 && !getenv("XG_SDX_NOFUSE")) {
-          double *ov = mo + (size_t)u_vol * NM, *oi = mo + (size_t)u_if * NM;
-          int32_t *sv = st ? st + (size_t)u_vol * 4 : nullptr, *si = st ? st + (size_t)u_if * 4 : nullptr;
+          double *ov = o_u[u_vol], *oi = o_u[u_if];
+          int32_t *sv = s_u[u_vol], *si = s_u[u_if];
           if (side == NEG)
             rc = launch_sdx_leaves<LO, PM, NEG, 3>(ctx, m, el, lset, el2v, coords, fi, L, lists, cap, cnt, ov, oi,
-                                                   mstride, sv, ndt * 4, queue, si);
+                                                   mstride, sv, sstride, queue, si);
           else
             rc = launch_sdx_leaves<LO, PM, POS, 3>(ctx, m, el, lset, el2v, coords, fi, L, lists, cap, cnt, ov, oi,
-                                                   mstride, sv, ndt * 4, queue, si);
+                                                   mstride, sv, sstride, queue, si);
           if (rc != XG_OK) return rc;
           rc = launch_sdx_rows<PM>(ctx, m, lists, cap, cnt, ov, mstride);
           if (rc != XG_OK) return rc;
           if (h_nover > 0) {
             rc = subdiv_exact_pass<LO, PM>(ctx, h_nover, el, lset, el2v, coords, fi, L, side, false, ov, nullptr,
-                                           mstride, nullptr, sv, ndt * 4, over);
+                                           mstride, nullptr, sv, sstride, over);
             if (rc != XG_OK) return rc;
             rc = subdiv_exact_pass<LO, PM>(ctx, h_nover, el, lset, el2v, coords, fi, L, -1, true, nullptr, oi,
-                                           mstride, nullptr, si, ndt * 4, over);
+                                           mstride, nullptr, si, sstride, over);
             if (rc != XG_OK) return rc;
           }
           if_done = true;
@@ -2490,25 +2497,25 @@ int subdiv_staged(xg_ctx *ctx, int nel, const int32_t *elems, const double *lset
         const bool vol = dts[u] == side;
         const bool itf = dts[u] == IF && !if_done;
         if (!vol && !itf) continue;
-        double *o = mo + (size_t)u * NM;
-        int32_t *su = st ? st + (size_t)u * 4 : nullptr;
+        double *o = o_u[u];
+        int32_t *su = s_u[u];
         if (vol) {
           if (side == NEG)
             rc = launch_sdx_leaves<LO, PM, NEG, 1>(ctx, m, el, lset, el2v, coords, fi, L, lists, cap, cnt, o, nullptr,
-                                                   mstride, su, ndt * 4, queue);
+                                                   mstride, su, sstride, queue);
           else
             rc = launch_sdx_leaves<LO, PM, POS, 1>(ctx, m, el, lset, el2v, coords, fi, L, lists, cap, cnt, o, nullptr,
-                                                   mstride, su, ndt * 4, queue);
+                                                   mstride, su, sstride, queue);
           if (rc != XG_OK) return rc;
           rc = launch_sdx_rows<PM>(ctx, m, lists, cap, cnt, o, mstride);
         } else {
           rc = launch_sdx_leaves<LO, PM, NEG, 2>(ctx, m, el, lset, el2v, coords, fi, L, lists, cap, cnt, nullptr, o,
-                                                 mstride, su, ndt * 4, queue);
+                                                 mstride, su, sstride, queue);
         }
         if (rc != XG_OK) return rc;
         if (h_nover > 0) {
           rc = subdiv_exact_pass<LO, PM>(ctx, h_nover, el, lset, el2v, coords, fi, L, vol ? side : -1, !vol,
-                                         vol ? o : nullptr, vol ? nullptr : o, mstride, nullptr, su, ndt * 4, over);
+                                         vol ? o : nullptr, vol ? nullptr : o, mstride, nullptr, su, sstride, over);
           if (rc != XG_OK) return rc;
         }
         if (itf) if_done = true;
@@ -2548,6 +2555,35 @@ __global__ __launch_bounds__(128) void k_moments_to_nodal4(int nel, double *__re
   }
 }
 
+// The reference-tet entry points (level-set nodal values per element, an
+// optional F^{-1} per element for the interface weights) through the staged
+// kernels: the NEG volume (out_vol) and / or interface (out_if) moments,
+// stride nm.  Without either output the one-kernel path (domains only).
+int subdiv_ref(xg_ctx *ctx, int lo, int p, int nel, const double *lset, const double *finv, int L, double *out_vol,
+               double *out_if, int nm, uint8_t *dom) {
+  int dts[2];
+  double *outs[2];
+  int ndt = 0;
+  if (out_vol) {
+    dts[ndt] = NEG;
+    outs[ndt++] = out_vol;
+  }
+  if (out_if) {
+    dts[ndt] = IF;
+    outs[ndt++] = out_if;
+  }
+  if (ndt == 0)
+    return subdiv_exact(ctx, lo, p, nel, nullptr, lset, nullptr, nullptr, finv, L, NEG, false, nullptr, nullptr, nm,
+                        dom, nullptr, 0);
+#define XG_SREF(LO, PM)                                                                                            \
+  if (lo == LO && p == PM)                                                                                         \
+    return subdiv_staged<LO, PM>(ctx, nel, nullptr, lset, nullptr, nullptr, finv, L, ndt, dts, outs, nm, nullptr, \
+                                 0, dom);
+  XG_SREF(1, 1) XG_SREF(1, 2) XG_SREF(1, 3) XG_SREF(2, 1) XG_SREF(2, 2) XG_SREF(2, 3)
+#undef XG_SREF
+  return set_error(XG_ERR_ARG, "subdivision rule: bad level-set order or degree");
+}
+
 }  // namespace
 
 extern "C" int xg_subdiv_weights(xg_ctx *ctx, int nel, const double *lset_nodal, int lset_order, int L,
@@ -2562,9 +2598,7 @@ extern "C" int xg_subdiv_weights(xg_ctx *ctx, int nel, const double *lset_nodal,
     // exact base rules: degree-4 moments from the exact-rule kernel, then
     // the nodal weights (the degree-4 lattice integrates exactly what the
     // moments do)
-    const bool want_vol = out_neg || !out_if;
-    int rc = subdiv_exact(ctx, lset_order, 2, nel, nullptr, lset_nodal, nullptr, nullptr, finv, L,
-                          want_vol ? NEG : -1, out_if != nullptr, out_neg, out_if, 35, dom, nullptr, 0);
+    int rc = subdiv_ref(ctx, lset_order, 2, nel, lset_nodal, finv, L, out_neg, out_if, 35, dom);
     if (rc != XG_OK) return rc;
     for (double *a : {out_neg, out_if})
       if (a) {
@@ -2588,15 +2622,7 @@ extern "C" int xg_subdiv_moments(xg_ctx *ctx, int nel, const double *lset_nodal,
   if (order >= 2 * p) {
     // exact base rules: node-moment tables + complemented leaves
     const int nm = p == 1 ? 10 : (p == 2 ? 35 : 84);
-    const bool want_vol = out_neg || !out_if;
-    if (p < 3 || !want_vol || !out_if)
-      return subdiv_exact(ctx, lset_order, p, nel, nullptr, lset_nodal, nullptr, nullptr, finv, L,
-                          want_vol ? NEG : -1, out_if != nullptr, out_neg, out_if, nm, dom, nullptr, 0);
-    const int rc = subdiv_exact(ctx, lset_order, p, nel, nullptr, lset_nodal, nullptr, nullptr, finv, L, NEG, false,
-                                out_neg, nullptr, nm, dom, nullptr, 0);
-    if (rc != XG_OK) return rc;
-    return subdiv_exact(ctx, lset_order, p, nel, nullptr, lset_nodal, nullptr, nullptr, finv, L, -1, true, nullptr,
-                        out_if, nm, nullptr, nullptr, 0);
+    return subdiv_ref(ctx, lset_order, p, nel, lset_nodal, finv, L, out_neg, out_if, nm, dom);
   }
   if (lset_order == 1 && p == 1) return subdiv_moments_impl<1, 1>(ctx, nel, lset_nodal, L, order, finv, out_neg, out_if, dom);
   if (lset_order == 1 && p == 2) return subdiv_moments_impl<1, 2>(ctx, nel, lset_nodal, L, order, finv, out_neg, out_if, dom);
@@ -2621,11 +2647,20 @@ static int subdiv_mesh_moments(xg_ctx *ctx, int p, int L, int nel, const int32_t
                                const double *coords, const double *lset_nodal, int ndt, const int *dts,
                                double *mom, int32_t *stats) {
   const int nm = p == 1 ? 10 : (p == 2 ? 35 : 84);
+  double *outs[3];
+  int32_t *st[3];
+  for (int u = 0; u < ndt; ++u) {
+    outs[u] = mom + (size_t)u * nm;
+    st[u] = stats ? stats + (size_t)u * 4 : nullptr;
+  }
   if (p == 1)
-    return subdiv_staged<2, 1>(ctx, nel, elems, lset_nodal, el2v, coords, nullptr, L, ndt, dts, mom, ndt * nm, stats);
+    return subdiv_staged<2, 1>(ctx, nel, elems, lset_nodal, el2v, coords, nullptr, L, ndt, dts, outs, ndt * nm, st,
+                               ndt * 4, nullptr);
   if (p == 2)
-    return subdiv_staged<2, 2>(ctx, nel, elems, lset_nodal, el2v, coords, nullptr, L, ndt, dts, mom, ndt * nm, stats);
-  return subdiv_staged<2, 3>(ctx, nel, elems, lset_nodal, el2v, coords, nullptr, L, ndt, dts, mom, ndt * nm, stats);
+    return subdiv_staged<2, 2>(ctx, nel, elems, lset_nodal, el2v, coords, nullptr, L, ndt, dts, outs, ndt * nm, st,
+                               ndt * 4, nullptr);
+  return subdiv_staged<2, 3>(ctx, nel, elems, lset_nodal, el2v, coords, nullptr, L, ndt, dts, outs, ndt * nm, st,
+                             ndt * 4, nullptr);
 }
 
 extern "C" int xg_subdiv_moments_mesh(xg_ctx *ctx, int p, int L, int order, int nel, const int32_t *elems,
