@@ -1563,6 +1563,7 @@ struct SdcLds {
   uint32_t *latq;     // lattice offsets of levels 0..3
   double *cl;         // 20
   uint16_t *fq;       // SDX_CH: a chunk's table rows
+  uint16_t *flq;      // SDX_CH: a chunk's leaves flagged cut by their vertex signs
 };
 
 __host__ __device__ inline size_t sdc_bytes(int L, SdcLds *s, char *base) {
@@ -1582,8 +1583,10 @@ __host__ __device__ inline size_t sdc_bytes(int L, SdcLds *s, char *base) {
   const size_t a_lq = take(sizeof(uint32_t) * SDX_LATQ);
   const size_t a_cl = take(sizeof(double) * 20);
   const size_t a_fq = take(sizeof(uint16_t) * SDX_CH);
+  const size_t a_flq = take(sizeof(uint16_t) * SDX_CH);
   if (s) {
     s->fq = (uint16_t *)(base + a_fq);
+    s->flq = (uint16_t *)(base + a_flq);
     s->latbits = (uint32_t *)(base + a_lb);
     s->rowst = (uint16_t *)(base + a_rs);
     s->nodeV = (uint4 *)(base + a_nv);
@@ -1792,7 +1795,8 @@ __global__ __launch_bounds__(64) void k_sdx_tree(int nel, const int32_t *__restr
           wave_sync();
         }
         // classification and the lists
-        int nfc = 0;  // this chunk's table rows (S.fq)
+        int nfc = 0;    // this chunk's table rows (S.fq)
+        int nflag = 0;  // this chunk's leaves to check on their values (S.flq)
         for (int c0 = 0; c0 < nc; c0 += 64) {
           const int c = c0 + lane;
           const bool valid = c < nc;
@@ -1808,9 +1812,36 @@ __global__ __launch_bounds__(64) void k_sdx_tree(int nel, const int32_t *__restr
           }
           const int cls = (f == 3u) ? IF : ((f == 1u) ? POS : NEG);
           if (k == 0) dom = __shfl(cls, 0, 64);
+          // leaves whose vertex signs differ are checked on their values in a
+          // second, dense pass (flq)
+          const int path = (k == 0 || !valid) ? 0 : (int)S.cur[(cbase + c) >> 3] * 8 + ((cbase + c) & 7);
+          const bool wantfill = SIDE >= 0 && valid && cls == SIDE;
+          const bool flagged = valid && cls == IF && r == 0;
+          const bool wantnext = valid && cls == IF && r > 0;
+          int cf, cg, cn;
+          const int sf = wave_slot(wantfill, nfc, lane, cf);
+          const int sg = wave_slot(flagged, nflag, lane, cg);
+          const int sn = wave_slot(wantnext, nnxt, lane, cn);
+          if (wantfill) S.fq[sf] = (uint16_t)(koff + path);
+          if (flagged) S.flq[sg] = (uint16_t)c;
+          if (wantnext) S.nxt[sn] = (uint16_t)path;
+          nfc += cf;
+          nflag += cg;
+          nnxt += cn;
+        }
+        wave_sync();
+        // the flagged leaves: Simplex::CheckIfCut on the vertex values
+        for (int q0 = 0; q0 < nflag && !overflow; q0 += 64) {
+          const int q = q0 + lane;
+          const bool valid = q < nflag;
           int lc = -2;
           bool lc22 = false;
-          if (valid && cls == IF && r == 0) {
+          int path = 0;
+          if (valid) {
+            const int c = S.flq[q];
+            path = (k == 0) ? 0 : (int)S.cur[(cbase + c) >> 3] * 8 + ((cbase + c) & 7);
+            int W[4][3];
+            unpack_uint4(S.nodeV[c], shift, W);
             double vv[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
@@ -1825,14 +1856,11 @@ __global__ __launch_bounds__(64) void k_sdx_tree(int nel, const int32_t *__restr
             for (int j = 0; j < 4; ++j) npos += vv[j] >= 0.0 ? 1 : 0;
             if (lc == IF && npos == 2) lc22 = true;
           }
-          const int path = (k == 0 || !valid) ? 0 : (int)S.cur[(cbase + c) >> 3] * 8 + ((cbase + c) & 7);
-          const bool wantfill = SIDE >= 0 && valid && (cls == SIDE || lc == SIDE);
+          const bool wantfill = SIDE >= 0 && valid && lc == SIDE;
           const bool leafcut = valid && lc == IF;
-          const bool wantnext = valid && cls == IF && r > 0;
-          int cf, cl2, cn;
+          int cf, cl2;
           const int sf = wave_slot(wantfill, nfc, lane, cf);
           const int sl = wave_slot(leafcut, nl, lane, cl2);
-          const int sn = wave_slot(wantnext, nnxt, lane, cn);
           // room for the cut leaves and their complements
           if (2 * (nl + cl2) > cap) {
             overflow = true;
@@ -1840,10 +1868,8 @@ __global__ __launch_bounds__(64) void k_sdx_tree(int nel, const int32_t *__restr
           }
           if (wantfill) S.fq[sf] = (uint16_t)(koff + path);
           if (leafcut) lst[sl] = (uint16_t)(koff + path);
-          if (wantnext) S.nxt[sn] = (uint16_t)path;
           nfc += cf;
           nl += cl2;
-          nnxt += cn;
           n22 += __popcll(__ballot(lc22));
         }
         wave_sync();

@@ -177,6 +177,28 @@ def test_pipeline_queue_modes_agree(dev, monkeypatch):
     np.testing.assert_array_equal(g0, g1)
 
 
+def test_pipeline_fused_and_separate_leaf_kernels(orc, dev, monkeypatch):
+    """P3 with the volume side and the interface: one leaf launch with lane
+    roles (4-point tasks) or one launch per part (XG_SDX_NOFUSE=1, 8-point
+    volume tasks); the same sums in another order: equal to 1e-13, and both
+    equal the oracle to 1e-12; the piece counts are identical"""
+    torch, _abi, ctx = dev
+    mesh = MakeStructured3DMesh(nx=6, ny=6, nz=6)
+    c, r = W.config5_spheres(7)
+    vals = p2_values(mesh, c, r)
+    cut = lattice_cut(vals)
+    cut = cut[:: max(1, len(cut) // 40)]
+    dts = [NEG, IF]
+    g0, st0 = run_pipeline(dev, 3, 4, cut, mesh, vals, dts, _abi.XG_PREC_FP64, stats=True)
+    monkeypatch.setenv("XG_SDX_NOFUSE", "1")
+    g1, st1 = run_pipeline(dev, 3, 4, cut, mesh, vals, dts, _abi.XG_PREC_FP64, stats=True)
+    monkeypatch.delenv("XG_SDX_NOFUSE")
+    np.testing.assert_array_equal(st0, st1)
+    assert rel_err(g0, g1).max() < 1e-13
+    ref = orc.subdiv_pipeline(3, 4, 6, cut, mesh.el2v, mesh.coords, vals, dts, 3, nthreads=8)
+    assert rel_err(g0, ref).max() < 1e-12
+
+
 @pytest.fixture(scope="module")
 def large_mixed(dev):
     """the large mixed workload: 128^3 x 6 Kuhn tets, union of 64 spheres
