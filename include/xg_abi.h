@@ -359,7 +359,17 @@ int xg_subdiv_weights_mesh(xg_ctx *ctx, int L, int order, int nel, const int32_t
    come from a table).  out: nel x ndt x ndof x ndof, FP32 or FP64 by
    precision.  stats (nullable): int32 nel x ndt x 4 -- table rows, volume
    tets and interface triangles integrated point by point, cut leaves.
-   Uses the context workspace (moments of up to 2^21 elements at a time). */
+   Uses the context workspace (moments of up to 2^21 elements at a time).
+   With order >= 2p, xg_subdiv_moments, xg_subdiv_weights and the three mesh
+   entry points run in rounds of up to 2^19 elements over per-element node
+   lists (8 KB per element in a grow-only context buffer, up to 4 GB) and
+   synchronise the context's stream once per round (the count of elements
+   whose lists overflow, which then take the one-kernel path); the calls are
+   therefore blocking and cannot be captured into a HIP graph.  Tuning knobs
+   read from the environment at each call: XG_SDX_CAP (list capacity, 16 ..
+   4096, default 4096), XG_SDX_QUEUE=0 (one block per element instead of
+   persistent waves), XG_SDX_NOFUSE=1 (p = 3: volume side and interface in
+   separate launches). */
 int xg_subdiv_pipeline(xg_ctx *ctx, int p, int L, int order, int nel, const int32_t *elems,
                        const int32_t *el2v, const double *coords, const double *lset_nodal, int ndt,
                        const int *dts, int mask, int precision, void *out, int32_t *stats);
