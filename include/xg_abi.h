@@ -362,10 +362,11 @@ int xg_subdiv_weights_mesh(xg_ctx *ctx, int L, int order, int nel, const int32_t
    Uses the context workspace (moments of up to 2^21 elements at a time).
    With order >= 2p, xg_subdiv_moments, xg_subdiv_weights and the three mesh
    entry points run in rounds of up to 2^19 elements over per-element node
-   lists (8 KB per element in a grow-only context buffer, up to 4 GB) and
-   synchronise the context's stream once per round (the count of elements
-   whose lists overflow, which then take the one-kernel path); the calls are
-   therefore blocking and cannot be captured into a HIP graph.  Tuning knobs
+   lists (8 KB per element in a grow-only context buffer, up to 4 GB; the
+   elements whose lists overflow take the one-kernel path, whose launch reads
+   their count from device memory).  The calls stay asynchronous; growing the
+   buffer allocates, so only a call whose buffer is already large enough could
+   be captured into a HIP graph (not tested).  Tuning knobs
    read from the environment at each call: XG_SDX_CAP (list capacity, 16 ..
    4096, default 4096), XG_SDX_QUEUE=0 (one block per element instead of
    persistent waves), XG_SDX_NOFUSE=1 (p = 3: volume side and interface in

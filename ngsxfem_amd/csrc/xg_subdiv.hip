@@ -1078,7 +1078,8 @@ __device__ __forceinline__ void wave_sum_lds(const double *m, double *scr, doubl
 // stats (nullable, int32 x 4 per element): table rows added, volume tets and
 // interface triangles integrated point by point, cut leaves
 template <int LO, int PM, int SIDE, int WHICH>
-__global__ __launch_bounds__(64, 2) void k_subdiv_exact(int nel, const int32_t *__restrict__ sub,
+__global__ __launch_bounds__(64, 2) void k_subdiv_exact(int nel, const int32_t *__restrict__ nel_dev,
+                                                     const int32_t *__restrict__ sub,
                                                      const int32_t *__restrict__ elems,
                                                      const double *__restrict__ lset_nodal,
                                                      const int32_t *__restrict__ el2v,
@@ -1095,6 +1096,10 @@ __global__ __launch_bounds__(64, 2) void k_subdiv_exact(int nel, const int32_t *
   // 3 / 6 / 12, processed in blocks of SDX_VB / SDX_IB points per lane task
   constexpr int SDX_VB = PM == 1 ? 4 : (PM == 2 ? 7 : 8), SDX_NVB = PM == 1 ? 1 : (PM == 2 ? 2 : 3);
   constexpr int SDX_IB = PM == 1 ? 3 : (PM == 2 ? 6 : 4), SDX_NIB = PM == 1 ? 1 : (PM == 2 ? 1 : 3);
+  // (nel_dev: the element count from device memory, at most nel; a block
+  // with no element leaves before staging the rules)
+  const int n_el = nel_dev ? min(*nel_dev, nel) : nel;
+  if ((int)blockIdx.x >= n_el) return;
   extern __shared__ char smem[];
   SdxLds S;
   const int vol_nq = xg_minrule_cnt_d[3][PM];
@@ -1117,7 +1122,7 @@ __global__ __launch_bounds__(64, 2) void k_subdiv_exact(int nel, const int32_t *
     wave_sync();
   }
   const double inv_n = 1.0 / n;
-  for (int j = blockIdx.x; j < nel; j += gridDim.x) {
+  for (int j = blockIdx.x; j < n_el; j += gridDim.x) {
     const int i = sub ? sub[j] : j;
     const int e = elems ? elems[i] : i;
     // the element's nodal values and geometry live in LDS (the lanes'
@@ -1478,16 +1483,18 @@ template <int LO, int PM, int SIDE, int WHICH>
 int launch_subdiv_exact(xg_ctx *ctx, int nel, const int32_t *elems, const double *lset_nodal, const int32_t *el2v,
                         const double *coords, const double *finv, int L, double *out_vol, double *out_if,
                         int out_stride, uint8_t *dom, int32_t *stats, int stats_stride,
-                        const int32_t *sub = nullptr) {
+                        const int32_t *sub = nullptr, const int32_t *nel_dev = nullptr) {
   constexpr int NM = Moments<3, PM>::NM;
   const size_t lds = sdx_bytes(L, xg_minrule_cnt[3][PM], xg_minrule_cnt[2][PM], NM, nullptr, nullptr);
   if (lds > 65536)
     XG_CHECK(hipFuncSetAttribute((const void *)k_subdiv_exact<LO, PM, SIDE, WHICH>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  const int blocks = nel < (1 << 22) ? nel : (1 << 22);
+  // (a device-side count: a grid of 2048 waves strides over it)
+  const int cap_blocks = nel_dev ? 2048 : (1 << 22);
+  const int blocks = nel < cap_blocks ? nel : cap_blocks;
   KernelTimer tm(ctx, "subdiv_exact");
   k_subdiv_exact<LO, PM, SIDE, WHICH><<<blocks, 64, lds, ctx->stream>>>(
-      nel, sub, elems, lset_nodal, el2v, coords, finv, L, (const uint4 *)ctx->sdx_verts, ctx->sdx_mom[PM], out_vol,
+      nel, nel_dev, sub, elems, lset_nodal, el2v, coords, finv, L, (const uint4 *)ctx->sdx_verts, ctx->sdx_mom[PM], out_vol,
       out_if, out_stride, dom, stats, stats_stride, ctx->d_err);
   XG_LAUNCH_CHECK();
   return XG_OK;
@@ -1518,14 +1525,14 @@ template <int LO, int PM>
 int subdiv_exact_pass(xg_ctx *ctx, int nel, const int32_t *elems, const double *lset_nodal, const int32_t *el2v,
                       const double *coords, const double *finv, int L, int vol_side, bool with_if, double *out_vol,
                       double *out_if, int out_stride, uint8_t *dom, int32_t *stats, int stats_stride,
-                      const int32_t *sub = nullptr) {
+                      const int32_t *sub = nullptr, const int32_t *nel_dev = nullptr) {
   if (vol_side >= 0) {
     const int rc = ensure_sdx_tables(ctx, PM);
     if (rc != XG_OK) return rc;
   }
 #define XG_SDX(SIDE, WHICH)                                                                                        \
   return launch_subdiv_exact<LO, PM, SIDE, WHICH>(ctx, nel, elems, lset_nodal, el2v, coords, finv, L, out_vol,    \
-                                                  out_if, out_stride, dom, stats, stats_stride, sub)
+                                                  out_if, out_stride, dom, stats, stats_stride, sub, nel_dev)
   if (vol_side == NEG && !with_if) XG_SDX(NEG, 1);
   if (vol_side == POS && !with_if) XG_SDX(POS, 1);
   if constexpr (PM < 3) {
@@ -2483,9 +2490,9 @@ int subdiv_staged(xg_ctx *ctx, int nel, const int32_t *elems, const double *lset
         if (dts[u] == side) ovol = o_u[u];
       rc = launch_sdx_tree<LO, PM>(ctx, side, m, el, lset, L, lists, cap, cnt, dm, over, nover, ovol, mstride);
       if (rc != XG_OK) return rc;
-      int h_nover = 0;
-      XG_CHECK(hipMemcpyAsync(&h_nover, nover, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-      XG_CHECK(hipStreamSynchronize(ctx->stream));
+      // the overflowed elements (count in device memory) go through the
+      // one-kernel path below; its launch strides over the count, so no
+      // host round trip is needed
       // p = 3 with the volume side and the interface both requested: one
       // launch for both (lane roles), then the rows, then the overflow path
       int u_vol = -1, u_if = -1;
@@ -2506,14 +2513,12 @@ int subdiv_staged(xg_ctx *ctx, int nel, const int32_t *elems, const double *lset
           if (rc != XG_OK) return rc;
           rc = launch_sdx_rows<PM>(ctx, m, lists, cap, cnt, ov, mstride);
           if (rc != XG_OK) return rc;
-          if (h_nover > 0) {
-            rc = subdiv_exact_pass<LO, PM>(ctx, h_nover, el, lset, el2v, coords, fi, L, side, false, ov, nullptr,
-                                           mstride, nullptr, sv, sstride, over);
-            if (rc != XG_OK) return rc;
-            rc = subdiv_exact_pass<LO, PM>(ctx, h_nover, el, lset, el2v, coords, fi, L, -1, true, nullptr, oi,
-                                           mstride, nullptr, si, sstride, over);
-            if (rc != XG_OK) return rc;
-          }
+          rc = subdiv_exact_pass<LO, PM>(ctx, m, el, lset, el2v, coords, fi, L, side, false, ov, nullptr, mstride,
+                                         nullptr, sv, sstride, over, nover);
+          if (rc != XG_OK) return rc;
+          rc = subdiv_exact_pass<LO, PM>(ctx, m, el, lset, el2v, coords, fi, L, -1, true, nullptr, oi, mstride,
+                                         nullptr, si, sstride, over, nover);
+          if (rc != XG_OK) return rc;
           if_done = true;
           if (t == ndt) break;
           continue;
@@ -2539,11 +2544,10 @@ int subdiv_staged(xg_ctx *ctx, int nel, const int32_t *elems, const double *lset
                                                  mstride, su, sstride, queue);
         }
         if (rc != XG_OK) return rc;
-        if (h_nover > 0) {
-          rc = subdiv_exact_pass<LO, PM>(ctx, h_nover, el, lset, el2v, coords, fi, L, vol ? side : -1, !vol,
-                                         vol ? o : nullptr, vol ? nullptr : o, mstride, nullptr, su, sstride, over);
-          if (rc != XG_OK) return rc;
-        }
+        rc = subdiv_exact_pass<LO, PM>(ctx, m, el, lset, el2v, coords, fi, L, vol ? side : -1, !vol,
+                                       vol ? o : nullptr, vol ? nullptr : o, mstride, nullptr, su, sstride, over,
+                                       nover);
+        if (rc != XG_OK) return rc;
         if (itf) if_done = true;
       }
       if (t == ndt) break;
