@@ -34,16 +34,17 @@ PEAK_FP64_VALU = 78.6e12
 
 def fp64_ops_moments(npts, n_cut, D=3, p=2):
     """FP64 flops of the moment path (FMA = 2).  The order-4 rule counts npts
-    (27-point tets, 9-point triangles) give the numbers of pieces; the moment
-    form evaluates each piece with the minimal exact rule of degree 4 (14 /
-    6 points).  Per point the mapping (3 ow + 3 mul + 9 FMA + 3 sub) and the
-    moment update (10 products, 10 weight products, 35 FMA); per (element,
-    domain type) the degree-2 lowering (90 adds) and the identities (934
-    coefficient terms, one FMA each)."""
-    pts = npts[0] // 27 * 14 + npts[2] // 9 * 6
-    per_pt = 3 + 3 + 2 * 9 + 3 + 10 + 10 + 2 * 35
-    per_el = 90 + 2 * 934
-    return pts * per_pt + 2 * n_cut * per_el
+    (27-point tets, 9-point triangles) give the numbers of pieces.  With exact
+    rules the moment form takes each piece's moments from its vertices (the
+    complete homogeneous recursion of degree 4: per vertex 4 adds and 136
+    FMAs, plus 7 operations to scale its barycentrics, and 16 per piece for
+    its volume and weight); per (element, domain type) the alpha! factors
+    (34 products), the degree-2 lowering (90 adds) and the Bernstein-form
+    identities (508 coefficient terms, one FMA each, 20 scalings)."""
+    per_vertex = 7 + 4 + 2 * 136
+    pieces_tet, pieces_tri = npts[0] // 27, npts[2] // 9
+    per_el = 34 + 90 + 2 * 508 + 20
+    return pieces_tet * (4 * per_vertex + 16) + pieces_tri * (3 * per_vertex + 16) + 2 * n_cut * per_el
 
 
 def parse():

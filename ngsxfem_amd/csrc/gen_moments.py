@@ -180,6 +180,198 @@ def lin_expr(terms, var):
     return s
 
 
+def gen_hstep(D, P, L):
+    """Exact moments of a simplex from its vertices (rules exact for degree
+    2P): for a k-simplex T with vertices whose barycentric coordinates in the
+    element are l_0 .. l_k, and n = 2P,
+
+        int_T (t . lam)^n = |T| n! k! / (n + k)! h_n(t . l_0, .., t . l_k),
+
+    h_n the complete homogeneous symmetric polynomial (the integral of a
+    power of an affine function over a simplex).  Comparing the coefficients
+    of t^alpha (the left side is sum_alpha n!/alpha! t^alpha m_alpha):
+
+        m_alpha = |T| k! alpha! / (n + k)! [t^alpha] h_n.
+
+    [t^alpha] h_n follows from the recursion over the vertices
+    H_j <- H_j + (t . l_v) H_{j-1} (j = 1..n ascending, H_0 = 1), whose terms
+    are all products of nonnegative barycentrics (no cancellation).  hstep
+    runs one vertex, H_1 .. H_{n-1} in h (zero at the start of a simplex), H_n
+    accumulated into m; scaling every l_v by s^(1/n) multiplies the simplex's
+    contribution by s, so m collects the pieces of a domain directly;
+    hfinish applies the alpha! factors once at the end."""
+    nv = D + 1
+    n = 2 * P
+    mons = [monomials(D, k) for k in range(n + 1)]
+    idx = [{a: i for i, a in enumerate(mk)} for mk in mons]
+    off = {}
+    o = 0
+    for k in range(1, n):
+        off[k] = o
+        o += len(mons[k])
+    L.append("  static constexpr int NH = %d;  // h: coefficients of H_1 .. H_%d" % (max(o, 1), n - 1))
+    L.append("  // one vertex of the complete homogeneous recursion (l: barycentrics scaled by s^(1/%d))" % n)
+    L.append("  XG_DEV static void hstep(const double *l, double *h, double *m) {")
+    for k in range(1, n + 1):
+        for g in mons[k]:
+            tgt = ("m[%d]" % idx[n][g]) if k == n else ("h[%d]" % (off[k] + idx[k][g]))
+            expr = tgt
+            for i in range(nv):
+                if g[i] == 0:
+                    continue
+                b = list(g)
+                b[i] -= 1
+                b = tuple(b)
+                if k == 1:
+                    expr = "%s + l[%d]" % (expr, i)
+                else:
+                    expr = "fma(l[%d], h[%d], %s)" % (i, off[k - 1] + idx[k - 1][b], expr)
+            L.append("    %s = %s;" % (tgt, expr))
+    L.append("  }")
+    L.append("  XG_DEV static void hfinish(double *m) {")
+    for g in mons[n]:
+        f = 1
+        for x in g:
+            f *= math.factorial(x)
+        if f != 1:
+            L.append("    m[%d] *= %.1f;" % (idx[n][g], f))
+    L.append("  }")
+
+
+def gen_forms_bernstein(D, P, L):
+    """forms through the Bernstein basis B_c = P!/c! lam^c (|c| = P): the
+    products of Bernstein functions and of their barycentric partials are
+    single monomials,
+
+        M^B_cd = P!/c! P!/d! m_{c+d}
+        K^B_cd = P!/c! P!/d! sum_{i,j} c_i d_j Gs_ij mk_{c+d-e_i-e_j},
+
+    and the Lagrange matrices follow by the change of basis phi_a =
+    sum_c T_ac B_c:  L = T (sm M^B + sk K^B) T^T, evaluated one column of
+    X = A T^T at a time.  Same quadrature sum, reassociated by exact
+    identities, in fewer operations than the expanded coefficients."""
+    nv = D + 1
+    monP = monomials(D, P)
+    mon2 = monomials(D, 2 * P)
+    idx2 = {a: i for i, a in enumerate(mon2)}
+    monK = monomials(D, 2 * P - 2)
+    idxK = {a: i for i, a in enumerate(monK)}
+    phis = basis(D, P)
+    N = len(phis)
+    NB = len(monP)
+    assert N == NB
+    Pf = math.factorial(P)
+
+    def mult(c):
+        f = Pf
+        for x in c:
+            f //= math.factorial(x)
+        return Fraction(f)
+
+    T = [[Fraction(0)] * NB for _ in range(N)]
+    for a, p in enumerate(phis):
+        for mono, coef in p.items():
+            T[a][monP.index(mono)] = coef / mult(mono)
+    gpairs = [(i, j) for i in range(nv) for j in range(i, nv)]
+    gidx = {pr: k for k, pr in enumerate(gpairs)}
+    nterms = 0
+    # entries of A: (m index, its factor, {(g, mk index): factor})
+    ents = {}
+    for ci in range(NB):
+        for di in range(ci, NB):
+            c, d = monP[ci], monP[di]
+            kc = mult(c) * mult(d)
+            mi = idx2[tuple(x + y for x, y in zip(c, d))]
+            kt = {}
+            for i in range(nv):
+                if c[i] == 0:
+                    continue
+                for j in range(nv):
+                    if d[j] == 0:
+                        continue
+                    b = [x + y for x, y in zip(c, d)]
+                    b[i] -= 1
+                    b[j] -= 1
+                    key = (gidx[(min(i, j), max(i, j))], idxK[tuple(b)])
+                    kt[key] = kt.get(key, 0) + kc * c[i] * d[j]
+            ents[(ci, di)] = (mi, kc, {k: v for k, v in kt.items() if v != 0})
+    # common factors go into scaled copies of sm / sk; the remaining term
+    # factors are 1 or 2 (2 Gs precomputed), so every term is one FMA
+    fm = sorted({e[1] for e in ents.values()})
+    fk = {}
+    for key, (mi, kc, kt) in ents.items():
+        if kt:
+            q = min(abs(v) for v in kt.values())
+            assert all(v / q in (1, 2) for v in kt.values()), (D, P, kt)
+            fk[key] = q
+    def scal(name, f):
+        return name if f == 1 else "%s%s_" % (name, str(f).replace("/", "o"))
+    for f in fm:
+        if f != 1:
+            L.append("    const double %s = %s * sm;" % (scal("sm", f), fmt(f)))
+    for f in sorted(set(fk.values())):
+        if f != 1:
+            L.append("    const double %s = %s * sk;" % (scal("sk", f), fmt(f)))
+    need2 = sorted({g for key, (mi, kc, kt) in ents.items() for (g, b), v in kt.items() if v / fk[key] == 2})
+    for g in need2:
+        L.append("    const double Gd%d = 2.0 * Gs[%d];" % (g, g))
+    L.append("    // A = sm M^B + sk K^B (Bernstein basis, upper triangle)")
+    for (ci, di), (mi, kc, kt) in ents.items():
+        nterms += 1 + len(kt)
+        if kt:
+            q = fk[(ci, di)]
+            parts = []
+            for (g, b), v in sorted(kt.items()):
+                parts.append("%s * mk[%d]" % ("Gs[%d]" % g if v / q == 1 else "Gd%d" % g, b))
+            L.append("    const double A%d_%d = %s * m[%d] + %s * (%s);" % (ci, di, scal("sm", kc), mi, scal("sk", q),
+                                                                            " + ".join(parts)))
+        else:
+            L.append("    const double A%d_%d = %s * m[%d];" % (ci, di, scal("sm", kc), mi))
+
+    def A(c, d):
+        return "A%d_%d" % (min(c, d), max(c, d))
+
+    def chain(terms):
+        """sum of coefficient * name (coefficients exact small numbers)"""
+        parts = []
+        for name, v in terms:
+            if v == 1:
+                parts.append(name)
+            elif v == -1:
+                parts.append("-" + name)
+            else:
+                parts.append("%s * %s" % (fmt(v), name))
+        e = parts[0]
+        for t in parts[1:]:
+            e += (" - " + t[1:]) if t.startswith("-") else (" + " + t)
+        return e
+
+    L.append("    // L = T A T^T: a column b of X = A T^T where T's row b has several")
+    L.append("    // entries, otherwise the entries of A directly")
+    for b in range(N):
+        L.append("    {")
+        tb = [(d, T[b][d]) for d in range(NB) if T[b][d] != 0]
+        direct = len(tb) == 1
+        if not direct:
+            for c in range(NB):
+                nterms += len(tb)
+                L.append("      const double X%d = %s;" % (c, chain([(A(c, d), v) for d, v in tb])))
+        for a in range(b + 1):
+            ta = [(c, T[a][c]) for c in range(NB) if T[a][c] != 0]
+            if direct:
+                d0, v0 = tb[0]
+                terms = [(A(c, d0), v * v0) for c, v in ta]
+            else:
+                terms = [("X%d" % c, v) for c, v in ta]
+            nterms += len(terms)
+            L.append("      const T v%d_ = (T)(%s);" % (a, chain(terms)))
+            L.append("      out[%d] = v%d_;" % (a * N + b, a))
+            if a != b:
+                L.append("      out[%d] = v%d_;" % (b * N + a, a))
+        L.append("    }")
+    return nterms
+
+
 def gen(D, P):
     nv = D + 1
     mon2 = monomials(D, 2 * P)
@@ -263,6 +455,8 @@ def gen(D, P):
             terms.append((idx2[a], mult))
         L.append("    mk[%d] = %s;" % (idxK[b], lin_expr(terms, "m")))
     L.append("  }")
+    if P <= 2:
+        gen_hstep(D, P, L)
     # ---- forms
     # stiffness through the barycentric gradients: grad phi = sum_i
     # (d phi / d lam_i) grad lam_i for any extension of phi off the plane
@@ -278,7 +472,9 @@ def gen(D, P):
     L.append("  XG_DEV static void forms(const double *m, const double *mk, const double *Gs, double sm, double sk,")
     L.append("                           T *out) {")
     nterms = 0
-    for a in range(N):
+    if P == 2:
+        nterms = gen_forms_bernstein(D, P, L)
+    for a in (range(N) if P != 2 else []):
         for b in range(a, N):
             mp = pmul(phis[a], phis[b])
             mterms = sorted((idx2[k], v) for k, v in mp.items())

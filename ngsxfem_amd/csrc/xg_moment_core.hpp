@@ -120,6 +120,43 @@ __device__ __forceinline__ void accum_piece(const double (*rule)[4], int nq, con
   }
 }
 
+constexpr double inv_factorial(int n) { return n <= 1 ? 1.0 : inv_factorial(n - 1) / n; }
+
+// Exact moments of one sub-simplex (nvs vertices W, reference coordinates)
+// from its vertices (Moments::hstep, the complete homogeneous recursion):
+// m_alpha += s [t^alpha] h_2P(t . l_0, .., t . l_k), k = nvs - 1, s = wsc /
+// (2P + k)! (wsc = k! |T| times the interface factor), l_v the vertices'
+// barycentrics; Moments::hfinish applies the alpha! factors once per domain.
+// The barycentrics are scaled by s^(1/2P), so the recursion's top degree
+// lands in m already weighted.  For rules exact in degree 2P this equals the
+// rule's point sum to rounding (P = 2: 4 x 140 FMAs for a tetrahedron
+// against 14 x 73 operations).
+template <int D, int P>
+__device__ __forceinline__ void hpiece(const double (*W)[3], int nvs, double wsc, double *m) {
+  using MM = Moments<D, P>;
+  static_assert(P <= 2, "the recursion's registers grow with the degree");
+  constexpr double f_vol = inv_factorial(2 * P + D), f_if = inv_factorial(2 * P + D - 1);
+  const double s = wsc * (nvs == D + 1 ? f_vol : f_if);
+  double c = sqrt(s);
+  if constexpr (P == 2) c = sqrt(c);
+  double h[MM::NH];
+#pragma unroll
+  for (int k = 0; k < MM::NH; ++k) h[k] = 0.0;
+#pragma unroll
+  for (int v = 0; v <= D; ++v) {
+    if (v >= nvs) break;
+    double l[D + 1];
+    double sl = 1.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      l[d] = c * W[v][d];
+      sl -= W[v][d];
+    }
+    l[D] = c * sl;
+    MM::hstep(l, h, m);
+  }
+}
+
 // Block-shared staging in LDS: the decomposition table of dimension D (all
 // three domain types) and the base rules of dimensions D and D-1 for the
 // rule order.  Every thread of the block must call stage_lds (it ends with a
@@ -349,12 +386,22 @@ __device__ __forceinline__ bool moment_accumulate_ec(const MomLds &S, const doub
       }
       wsc = piece_volume(nvs, W) * ifac;
     }
+    if constexpr (P <= 2) {
+      // exact rules: the sub-simplex's exact moments from its vertices
+      if (S.exact) {
+        hpiece<D, P>(W, nvs, wsc, m);
+        continue;
+      }
+    }
     // one code path for volume and interface pieces (a single instantiation
     // keeps the register allocation of the two from adding up): the rule
     // coordinates past the piece's dimension are exact zeros in the rule
     // table and the unused vertices are zero, so the extra terms add exact
     // zeros
     accum_piece<D, P, D>(rule, nq, W, wsc, m);
+  }
+  if constexpr (P <= 2) {
+    if (S.exact) MM::hfinish(m);
   }
   if (pl.comp || pl.full) MM::complement(m);  // full: whole simplex - 0
   return pl.bad;

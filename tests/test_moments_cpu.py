@@ -108,6 +108,25 @@ extern "C" int mom_accum(int D, int P, const double *pts, const double *w, int n
   ACASE(1, 1) ACASE(1, 2) ACASE(1, 3) ACASE(2, 1) ACASE(2, 2) ACASE(2, 3) ACASE(3, 1) ACASE(3, 2) ACASE(3, 3)
   return -1;
 }
+// Moments::hstep over the nv vertices of one simplex (barycentrics lv,
+// nv x (D+1)), every vertex scaled by s^(1/2P), then hfinish
+extern "C" int mom_hsimplex(int D, int P, const double *lv, int nv, double s, double *m) {
+#define HCASE(DD, PP)                                           \
+  if (D == DD && P == PP) {                                     \
+    using M = xg::Moments<DD, PP>;                              \
+    double h[M::NH] = {0};                                      \
+    const double c = std::pow(s, 1.0 / (2 * PP));               \
+    for (int v = 0; v < nv; ++v) {                              \
+      double l[4];                                              \
+      for (int i = 0; i <= DD; ++i) l[i] = c * lv[v * (DD + 1) + i]; \
+      M::hstep(l, h, m);                                        \
+    }                                                           \
+    M::hfinish(m);                                              \
+    return M::NM;                                               \
+  }
+  HCASE(1, 1) HCASE(1, 2) HCASE(2, 1) HCASE(2, 2) HCASE(3, 1) HCASE(3, 2)
+  return -1;
+}
 """
 
 
@@ -124,6 +143,8 @@ def momlib(tmp_path_factory):
                    check=True, capture_output=True)
     lib = ctypes.CDLL(str(so))
     lib.mom_elmat.restype = ctypes.c_int
+    lib.mom_hsimplex.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                 ctypes.c_void_p]
     return lib
 
 
@@ -150,6 +171,49 @@ def test_moment_identities_equal_direct_quadrature(orc, momlib, D, P):
             assert momlib.mom_elmat(D, P, _p(V), _p(pts3), _p(w), npt, mask, _p(out)) == 0
             got = out.reshape(nd, nd)
             assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max(), (D, P, mask)
+
+
+@pytest.mark.parametrize("D", [1, 2, 3])
+@pytest.mark.parametrize("P", [1, 2])
+def test_vertex_recursion_gives_exact_simplex_moments(momlib, D, P):
+    """Moments::hstep / hfinish (the complete homogeneous recursion over a
+    simplex's vertices) against (a) the closed form over the whole reference
+    simplex (Moments::complement of zero: int lam^alpha = alpha! D! /
+    (2P + D)! |T|) and (b) Grundmann-Moeller rules of degree 2P + 1 on random
+    sub-simplices of dimension D and D - 1 (volume pieces and interface
+    pieces), summed through Moments::accum (the product form, an independent
+    code path)."""
+    import math
+    import sys
+    sys.path.insert(0, HERE)
+    from test_oracle_invariants import gm_rule
+    n = 2 * P
+    NM = momlib.mom_complement(D, P, _p(np.zeros(256)))
+    full = np.zeros(NM)
+    momlib.mom_complement(D, P, _p(full))
+    got = np.zeros(NM)
+    lv = np.ascontiguousarray(np.eye(D + 1)[[D] + list(range(D))])  # origin, e_1 .. e_D
+    s = (1.0 / math.factorial(D)) * math.factorial(D) / math.factorial(n + D)
+    momlib.mom_hsimplex(D, P, _p(lv), D + 1, s, _p(got))
+    assert np.abs(got - full).max() <= 1e-15 * np.abs(full).max()
+    rng = np.random.default_rng(7 * D + P)
+    for k in ([D, D - 1] if D > 1 else [D]):
+        for trial in range(6):
+            lam = rng.random((k + 1, D + 1)) ** 2
+            lam /= lam.sum(1, keepdims=True)  # vertices inside the element
+            X = lam[:, :D]  # reference coordinates
+            E = X[1:] - X[0]
+            vol = math.sqrt(abs(np.linalg.det(E @ E.T))) / math.factorial(k)
+            b, w = gm_rule(k, P)  # degree 2P + 1 on the unit k-simplex
+            pts = b @ X
+            pts3 = np.ascontiguousarray(np.hstack([pts, np.zeros((len(w), 3 - D))]))
+            ww = np.ascontiguousarray(w * vol * math.factorial(k))
+            ref = np.zeros(NM)
+            momlib.mom_accum(D, P, _p(pts3), _p(ww), len(w), _p(ref))
+            got = np.zeros(NM)
+            s = vol * math.factorial(k) / math.factorial(n + k)
+            momlib.mom_hsimplex(D, P, _p(np.ascontiguousarray(lam)), k + 1, s, _p(got))
+            assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max(), (D, P, k, trial)
 
 
 def test_moment_table_is_current():
