@@ -252,6 +252,47 @@ int xg_integrate(xg_ctx *ctx, int D, int nel, const int32_t *elems, const int32_
                  int nterms, const double *coef, const int32_t *expo, double *per_elem,
                  double *total);
 
+/* ---- coefficient functions at the points of a rule --------------------
+   NGSolve evaluates a CoefficientFunction at the mapped points of the rule
+   (symboliccutbfi.cpp:229-287 scales the proxy products by it,
+   symboliccutlfi.cpp:57-140 the test functions, python_cutint.cpp:66-98 the
+   weights).  Here a coefficient is a postfix program over the physical point
+   (host arrays, at most 192 code words, 48 constants, stack depth 16,
+   checked by the entry point: XG_ERR_ARG otherwise).  Opcodes, with the
+   immediate operand that follows some of them:
+     1 CONST k  push consts[k]          2 COORD d  push x_d (physical)
+     3 ADD  4 SUB  5 MUL  6 DIV  (a b -> a op b)   7 NEG
+     8 POWI n  a -> a^n (integer n)      9 POW  a b -> pow(a, b)
+     10 SQRT 11 SIN 12 COS 13 TAN 14 EXP 15 LOG 16 ATAN 17 TANH 18 ABS
+     19 GF1 s  push the P1 grid function gf[s] (device FP64 vertex values,
+               nv of them) at the point: sum_k lam_k gf[s][v_k]
+     20 IFPOS  a b c -> (a > 0 ? b : c)   21 MAX  22 MIN  (a b -> ...)
+   The program must leave exactly one value. */
+typedef struct xg_cf {
+  int32_t ncode;
+  const int32_t *code;
+  int32_t nconst;
+  const double *consts;
+  int32_t ngf;                /* <= 4 */
+  const double *const *gf;    /* host array of device pointers */
+} xg_cf;
+
+/* w_q <- w_q * cf(x_q) for the points of a straight cut / several-level-set
+   rule (layout of xg_cut_rule_fill, in place).  The element matrices
+   (xg_elmat), element vectors (xg_elvec) and integrals (xg_integrate with
+   f = 1) of the scaled rule are those of c u v / c grad u . grad v, f v and
+   f.  No counterpart in the reference's interface: the reference applies the
+   coefficient inside CalcElementMatrix / CalcElementVector /
+   IntegrateX; this splits it out so the three kernels can share it. */
+int xg_rule_scale_cf(xg_ctx *ctx, int D, int nel, const int32_t *elems, const int32_t *el2v,
+                     const double *coords, const int32_t *offsets, double *pts, const xg_cf *cf);
+
+/* ---- linear forms on a materialised rule: SymbolicCutLinearFormIntegrator::
+   T_CalcElementVector (xfem/symboliccutlfi.cpp:57-140) with f = 1 in the
+   weights: out[i][a] = |det F| sum_q w_q phi_a(xi_q), FP64 nel x ndof. */
+int xg_elvec(xg_ctx *ctx, int D, int p, int nel, const int32_t *elems, const int32_t *el2v,
+             const double *coords, const int32_t *offsets, const double *pts, double *out);
+
 /* ---- a14: SymbolicCutBilinearFormIntegrator::CalcElementMatrix ---------
    (xfem/symboliccutbfi.cpp:37-46, 76-304) on a materialised rule.
    out: nel x ndof x ndof (FP64 for XG_PREC_FP64, FP32 otherwise), row = test

@@ -28,6 +28,21 @@ class XgError(RuntimeError):
 
 
 # signatures: name -> argtypes (all return int)
+class _XgCf(ctypes.Structure):
+    """struct xg_cf (include/xg_abi.h): a coefficient's postfix program"""
+    _fields_ = [("ncode", ctypes.c_int32), ("code", _vp), ("nconst", ctypes.c_int32), ("consts", _vp),
+                ("ngf", ctypes.c_int32), ("gf", _vp)]
+
+    @staticmethod
+    def make(code, consts, gfs=()):
+        c = (ctypes.c_int32 * max(len(code), 1))(*code)
+        k = (ctypes.c_double * max(len(consts), 1))(*consts)
+        g = (_vp * max(len(gfs), 1))(*[ptr(t) for t in gfs])
+        s = _XgCf(len(code), ctypes.cast(c, _vp), len(consts), ctypes.cast(k, _vp), len(gfs), ctypes.cast(g, _vp))
+        s._keep = (c, k, g)
+        return s
+
+
 _SIGS = {
     "xg_ctx_create": [_i, ctypes.POINTER(_vp)],
     "xg_ctx_destroy": [_vp],
@@ -57,6 +72,8 @@ _SIGS = {
     "xg_quad_elvec": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp],
     "xg_integrate_nodes": [_vp, _i, _vp, _vp, _vp, _i, _vp, _i, _vp, _vp, _vp, _vp],
     "xg_elmat": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
+    "xg_rule_scale_cf": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp],
+    "xg_elvec": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp],
     "xg_cut_pipeline": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _i, _vp],
     "xg_subdiv_weights": [_vp, _i, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp],
     "xg_subdiv_moments": [_vp, _i, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp],
@@ -318,6 +335,19 @@ class Context:
         nel = elems.shape[0] if elems is not None else el2v.shape[0]
         check(lib().xg_quad_elvec(self.h, D, nel, ptr(elems), ptr(el2v), ptr(coords), ptr(offsets), ptr(pts),
                                   ptr(out)))
+
+    def rule_scale_cf(self, D, el2v, coords, offsets, pts, code, consts, gfs=(), elems=None):
+        """pts[q].w *= cf(x_q); code / consts: the coefficient's postfix
+        program, gfs: device FP64 vertex values of its P1 grid functions"""
+        nel = elems.shape[0] if elems is not None else el2v.shape[0]
+        cf = _XgCf.make(code, consts, gfs)
+        check(lib().xg_rule_scale_cf(self.h, D, nel, ptr(elems), ptr(el2v), ptr(coords), ptr(offsets), ptr(pts),
+                                     ctypes.byref(cf)))
+
+    def elvec(self, D, p, el2v, coords, offsets, pts, out, elems=None):
+        nel = elems.shape[0] if elems is not None else el2v.shape[0]
+        check(lib().xg_elvec(self.h, D, p, nel, ptr(elems), ptr(el2v), ptr(coords), ptr(offsets), ptr(pts),
+                             ptr(out)))
 
     def elmat(self, D, p, el2v, coords, offsets, pts, mask, precision, out, elems=None):
         nel = elems.shape[0] if elems is not None else el2v.shape[0]

@@ -632,6 +632,38 @@ void build_quad_rule(MA ma, const double *d_lset, int dt, int order, int policy,
   check(xg_sync(ctx()));
 }
 
+// ---------------------------------------------------------------- coefficient functions
+// A coefficient's postfix program (CoefficientFunction.program(): code,
+// constants, P1 grid functions) with the grid functions' vertex values on
+// the device; cf points into the vectors here.
+struct DeviceCf {
+  std::vector<int32_t> code;
+  std::vector<double> consts;
+  std::vector<std::unique_ptr<DevBuf<double>>> gfbuf;
+  std::vector<const double *> gfp;
+  xg_cf cf{};
+};
+
+void make_device_cf(py::object cfobj, int nv, DeviceCf &d) {
+  if (!py::hasattr(cfobj, "program")) throw std::runtime_error("coefficient: a CoefficientFunction expected");
+  py::tuple t = cfobj.attr("program")();
+  d.code = t[0].cast<std::vector<int32_t>>();
+  d.consts = t[1].cast<std::vector<double>>();
+  for (auto g : t[2].cast<py::list>()) {
+    auto a = py::array_t<double, py::array::c_style | py::array::forcecast>(g.attr("vec"));
+    if ((int)a.size() != nv) throw std::runtime_error("coefficient: P1 grid functions (one value per vertex)");
+    d.gfbuf.emplace_back(new DevBuf<double>());
+    d.gfbuf.back()->upload(a.data(), (size_t)a.size());
+    d.gfp.push_back(d.gfbuf.back()->p);
+  }
+  d.cf.ncode = (int32_t)d.code.size();
+  d.cf.code = d.code.data();
+  d.cf.nconst = (int32_t)d.consts.size();
+  d.cf.consts = d.consts.data();
+  d.cf.ngf = (int32_t)d.gfp.size();
+  d.cf.gf = d.gfp.data();
+}
+
 // ---------------------------------------------------------------- several level sets
 // A set of P1 level sets on the device (one array of vertex values each) and
 // a list of domain tuples (LevelsetIntegrationDomain with several level sets,
@@ -997,47 +1029,49 @@ py::object IntegrateX(py::dict lsetdom, py::object mesh, py::object cf, py::obje
     d_pe.download(pe.data(), ma->ne);
     d_tot.download(&total, 1);
   } else {
-    // host evaluation of a general coefficient function at the rule points
-    std::vector<int32_t> off(ma->ne + 1);
-    r.offsets.download(off.data(), off.size());
-    std::vector<double> p((size_t)r.total * 4);
-    r.pts.download(p.data(), p.size());
-    auto c = py::array_t<double, py::array::c_style | py::array::forcecast>(mesh.attr("coords"));
-    const double *X = c.data();
-    py::array_t<double> xyz({(py::ssize_t)r.total, (py::ssize_t)3});
-    auto xv = xyz.mutable_unchecked<2>();
-    std::vector<double> wphys(r.total);
-    for (int e = 0; e < ma->ne; ++e) {
-      const int32_t *vv = &ma->h_el2v[(size_t)e * (D + 1)];
-      double F[9], det;
-      for (int rr = 0; rr < D; ++rr)
-        for (int cc = 0; cc < D; ++cc) F[rr * D + cc] = X[(size_t)vv[cc] * D + rr] - X[(size_t)vv[D] * D + rr];
-      if (D == 1) det = F[0];
-      else if (D == 2) det = F[0] * F[3] - F[1] * F[2];
-      else
-        det = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
-              F[2] * (F[3] * F[7] - F[4] * F[6]);
-      for (int q = off[e]; q < off[e + 1]; ++q) {
-        for (int rr = 0; rr < 3; ++rr) {
-          double a = rr < D ? X[(size_t)vv[D] * D + rr] : 0.0;
-          for (int cc = 0; cc < D && rr < D; ++cc) a += F[rr * D + cc] * p[(size_t)4 * q + cc];
-          xv(q, rr) = a;
+    // any other coefficient function: its postfix program scales the rule's
+    // weights on the device, then the integral of f = 1
+    if (!ip_container.is_none()) {
+      // the rule's physical points and weights as the reference lists them
+      std::vector<int32_t> off(ma->ne + 1);
+      r.offsets.download(off.data(), off.size());
+      std::vector<double> p((size_t)r.total * 4);
+      r.pts.download(p.data(), p.size());
+      auto c = py::array_t<double, py::array::c_style | py::array::forcecast>(mesh.attr("coords"));
+      const double *X = c.data();
+      py::list l = ip_container.cast<py::list>();
+      for (int e = 0; e < ma->ne; ++e) {
+        const int32_t *vv = &ma->h_el2v[(size_t)e * (D + 1)];
+        double F[9], det;
+        for (int rr = 0; rr < D; ++rr)
+          for (int cc = 0; cc < D; ++cc) F[rr * D + cc] = X[(size_t)vv[cc] * D + rr] - X[(size_t)vv[D] * D + rr];
+        if (D == 1) det = F[0];
+        else if (D == 2) det = F[0] * F[3] - F[1] * F[2];
+        else
+          det = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+                F[2] * (F[3] * F[7] - F[4] * F[6]);
+        for (int q = off[e]; q < off[e + 1]; ++q) {
+          double xq[3] = {0.0, 0.0, 0.0};
+          for (int rr = 0; rr < D; ++rr) {
+            double a = X[(size_t)vv[D] * D + rr];
+            for (int cc = 0; cc < D; ++cc) a += F[rr * D + cc] * p[(size_t)4 * q + cc];
+            xq[rr] = a;
+          }
+          l.append(py::make_tuple(xq[0], xq[1], xq[2], std::abs(det) * p[(size_t)4 * q + 3]));
         }
-        wphys[q] = std::abs(det) * p[(size_t)4 * q + 3];
       }
     }
-    auto fv = py::array_t<double, py::array::c_style | py::array::forcecast>(cf.attr("evaluate")(xyz));
-    const double *f = fv.data();
-    for (int e = 0; e < ma->ne; ++e) {
-      double s = 0.0;
-      for (int q = off[e]; q < off[e + 1]; ++q) s += wphys[q] * f[q];
-      pe[e] = s;
-      total += s;
-    }
-    if (!ip_container.is_none()) {
-      py::list l = ip_container.cast<py::list>();
-      for (int q = 0; q < r.total; ++q) l.append(py::make_tuple(xv(q, 0), xv(q, 1), xv(q, 2), wphys[q]));
-    }
+    DeviceCf dcf;
+    make_device_cf(cf, ma->nv, dcf);
+    check(xg_rule_scale_cf(ctx(), D, ma->ne, nullptr, ma->el2v.p, ma->coords.p, r.offsets.p, r.pts.p, &dcf.cf));
+    const double one = 1.0;
+    const int32_t ex0[3] = {0, 0, 0};
+    DevBuf<double> d_pe(ma->ne), d_tot(1);
+    check(xg_integrate(ctx(), D, ma->ne, nullptr, ma->el2v.p, ma->coords.p, r.offsets.p, r.pts.p, 1, &one, ex0,
+                       d_pe.p, d_tot.p));
+    check(xg_sync(ctx()));
+    d_pe.download(pe.data(), ma->ne);
+    d_tot.download(&total, 1);
   }
   if (element_wise) return py::array_t<double>((py::ssize_t)pe.size(), pe.data());
   return py::float_(total);
@@ -1239,6 +1273,76 @@ py::tuple CutElementVectors(py::object mesh, py::object lset, int dt, int order,
   out.download(h.mutable_data(), (size_t)nel * nd);
   py::array_t<int32_t> elarr((py::ssize_t)nel, el.data());
   return py::make_tuple(elarr, h);
+}
+
+// element matrices of c u v (mask 1) / c grad u . grad v (mask 2) with a
+// coefficient function c: the straight cut rule of the elements, its weights
+// scaled by c at the physical points (xg_rule_scale_cf), then the element
+// matrices of the scaled rule (xg_elmat)
+py::tuple CutElementMatricesCF(py::object mesh, py::object lset, int dt, int order, int p, int mask,
+                               py::object elements, int precision, py::object cf) {
+  MA ma = as_mesh(mesh);
+  simplices_only(ma, "CutElementMatricesCF");
+  bool is_p1 = true;
+  std::vector<double> v = p1_values(lset, ma->nv, is_p1);
+  if (!is_p1) throw std::runtime_error("coefficient functions: straight cut rules of P1 level sets");
+  DevBuf<double> d_l;
+  d_l.upload(v.data(), v.size());
+  std::vector<int32_t> el = element_list(elements, ma->ne);
+  DeviceRule r;
+  build_rule(ma, d_l.p, dt, order, &el, r);
+  DeviceCf dcf;
+  make_device_cf(cf, ma->nv, dcf);
+  const int D = ma->dim, nel = (int)el.size();
+  int nd = 1;
+  for (int k = 1; k <= D; ++k) nd = nd * (p + k) / k;
+  py::array_t<int32_t> elarr((py::ssize_t)nel, el.data());
+  if (nel) check(xg_rule_scale_cf(ctx(), D, nel, r.elems.p, ma->el2v.p, ma->coords.p, r.offsets.p, r.pts.p, &dcf.cf));
+  if (precision == XG_PREC_FP64) {
+    DevBuf<double> out((size_t)std::max(nel, 1) * nd * nd);
+    if (nel) check(xg_elmat(ctx(), D, p, nel, r.elems.p, ma->el2v.p, ma->coords.p, r.offsets.p, r.pts.p, mask,
+                            precision, out.p));
+    check(xg_sync(ctx()));
+    py::array_t<double> h({(py::ssize_t)nel, (py::ssize_t)nd, (py::ssize_t)nd});
+    out.download(h.mutable_data(), (size_t)nel * nd * nd);
+    return py::make_tuple(elarr, h);
+  }
+  DevBuf<float> out((size_t)std::max(nel, 1) * nd * nd);
+  if (nel) check(xg_elmat(ctx(), D, p, nel, r.elems.p, ma->el2v.p, ma->coords.p, r.offsets.p, r.pts.p, mask,
+                          precision, out.p));
+  check(xg_sync(ctx()));
+  py::array_t<float> h({(py::ssize_t)nel, (py::ssize_t)nd, (py::ssize_t)nd});
+  out.download(h.mutable_data(), (size_t)nel * nd * nd);
+  return py::make_tuple(elarr, h);
+}
+
+// element vectors of f v with a coefficient function f (FP64)
+py::tuple CutElementVectorsCF(py::object mesh, py::object lset, int dt, int order, int p, py::object elements,
+                              py::object cf) {
+  MA ma = as_mesh(mesh);
+  simplices_only(ma, "CutElementVectorsCF");
+  bool is_p1 = true;
+  std::vector<double> v = p1_values(lset, ma->nv, is_p1);
+  if (!is_p1) throw std::runtime_error("coefficient functions: straight cut rules of P1 level sets");
+  DevBuf<double> d_l;
+  d_l.upload(v.data(), v.size());
+  std::vector<int32_t> el = element_list(elements, ma->ne);
+  DeviceRule r;
+  build_rule(ma, d_l.p, dt, order, &el, r);
+  DeviceCf dcf;
+  make_device_cf(cf, ma->nv, dcf);
+  const int D = ma->dim, nel = (int)el.size();
+  int nd = 1;
+  for (int k = 1; k <= D; ++k) nd = nd * (p + k) / k;
+  DevBuf<double> out((size_t)std::max(nel, 1) * nd);
+  if (nel) {
+    check(xg_rule_scale_cf(ctx(), D, nel, r.elems.p, ma->el2v.p, ma->coords.p, r.offsets.p, r.pts.p, &dcf.cf));
+    check(xg_elvec(ctx(), D, p, nel, r.elems.p, ma->el2v.p, ma->coords.p, r.offsets.p, r.pts.p, out.p));
+  }
+  check(xg_sync(ctx()));
+  py::array_t<double> h({(py::ssize_t)nel, (py::ssize_t)nd});
+  out.download(h.mutable_data(), (size_t)nel * nd);
+  return py::make_tuple(py::array_t<int32_t>((py::ssize_t)nel, el.data()), h);
 }
 
 // device scatter of element matrices into a CSR pattern (the assembly loop of
@@ -1635,6 +1739,13 @@ PYBIND11_MODULE(xfem, m) {
         py::arg("domain_type"), py::arg("order"), py::arg("p"), py::arg("mask") = 3,
         py::arg("elements") = py::none(), py::arg("precision") = XG_PREC_FP32,
         "element matrices of (mass | laplace) on the rule of several level sets");
+  m.def("CutElementMatricesCF", &CutElementMatricesCF, py::arg("mesh"), py::arg("levelset"),
+        py::arg("domain_type"), py::arg("order"), py::arg("p"), py::arg("mask"), py::arg("elements"),
+        py::arg("precision"), py::arg("cf"),
+        "element matrices of cf u v (mask 1) / cf grad u . grad v (mask 2), cf evaluated on the GPU");
+  m.def("CutElementVectorsCF", &CutElementVectorsCF, py::arg("mesh"), py::arg("levelset"), py::arg("domain_type"),
+        py::arg("order"), py::arg("p"), py::arg("elements"), py::arg("cf"),
+        "element vectors of cf v on the straight cut rule, cf evaluated on the GPU (FP64)");
   m.def("CutElementMatrices", &CutElementMatrices, py::arg("mesh"), py::arg("levelset"), py::arg("domain_type"),
         py::arg("order"), py::arg("p"), py::arg("mask") = 3, py::arg("elements") = py::none(),
         py::arg("precision") = XG_PREC_FP32);

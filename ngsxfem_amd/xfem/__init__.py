@@ -30,7 +30,8 @@ load_hip_runtime()  # before the host module loads libxg.so: one HIP runtime per
 
 from .xfem import (ANY, BND, CDOM_IF, CDOM_NEG, CDOM_POS, FALLBACK, FIRST, HASNEG, HASPOS, IF, NEG, NO,  # noqa: F401
                    OPTIMAL, POS, UNCUT, VOL, AssembleCSR, AssembleCutFormsDevice, AssembleVector, BitArray,
-                   COMBINED_DOMAIN_TYPE, CutElementMatrices, CutElementVectors, CutInfo, DOMAIN_TYPE, GetDofsOfElements,
+                   COMBINED_DOMAIN_TYPE, CutElementMatrices, CutElementMatricesCF, CutElementVectors,
+                   CutElementVectorsCF, CutInfo, DOMAIN_TYPE, GetDofsOfElements,
                    GetDofsOfFacets, GetElementsWithNeighborFacets, GetElementsWithSharedVertex,
                    GetFacetsWithNeighborTypes, IntegrateX, MlsetElementMatrices, MultiLevelsetCutInfo, NODE_TYPE,
                    QuadElementMatrices, SubdivElementMatrices,
@@ -77,23 +78,33 @@ def HAS(domain_type):
 class CoefficientFunction:
     """Scalar coefficient function of the physical coordinates.  Polynomials
     (built from x, y, z, constants, +, -, *, integer powers) are kept as
-    monomial term lists and integrated by the GPU reduction kernel; anything
-    else is evaluated on the host (``evaluate``)."""
+    monomial term lists and integrated by the GPU reduction kernel; every
+    other expression keeps its tree (constants, coordinates, + - * /, powers,
+    sqrt sin cos tan exp log atan tanh abs, IfPos, max / min, P1 grid
+    functions) and is compiled into the postfix program that the GPU
+    evaluates at the rule points (``program``, include/xg_abi.h xg_cf).
+    ``evaluate`` computes the same values with numpy (for checks)."""
 
-    def __init__(self, val=None, terms=None, fn=None):
+    def __init__(self, val=None, terms=None, expr=None):
         if terms is not None:
             self._terms = {k: v for k, v in terms.items() if v != 0.0}
-            self._fn = None
-        elif fn is not None:
+            self._expr = None
+        elif expr is not None:
             self._terms = None
-            self._fn = fn
+            self._expr = expr
         else:
             self._terms = {(0, 0, 0): float(val)} if float(val) != 0.0 else {}
-            self._fn = None
+            self._expr = None
 
     @staticmethod
     def _lift(o):
-        return o if isinstance(o, CoefficientFunction) else CoefficientFunction(float(o))
+        if isinstance(o, CoefficientFunction):
+            return o
+        if isinstance(o, GridFunction):
+            if o.order != 1:
+                raise NotImplementedError("grid functions in coefficients: P1 only")
+            return CoefficientFunction(expr=("gf", o))
+        return CoefficientFunction(float(o))
 
     def is_poly(self):
         return self._terms is not None
@@ -104,19 +115,23 @@ class CoefficientFunction:
         t = [(c, e) for e, c in sorted(self._terms.items())]
         return t if t else [(0.0, (0, 0, 0))]
 
+    def _node(self):
+        return ("poly", self._terms) if self.is_poly() else self._expr
+
     def evaluate(self, X):
         X = np.asarray(X, dtype=np.float64)
-        if self.is_poly():
-            out = np.zeros(X.shape[0])
-            for (a, b, c), co in self._terms.items():
-                out += co * X[:, 0] ** a * X[:, 1] ** b * X[:, 2] ** c
-            return out
-        return np.asarray(self._fn(X), dtype=np.float64)
+        return np.asarray(_eval_node(self._node(), X), dtype=np.float64) * np.ones(X.shape[0])
 
     def __call__(self, *p):
         X = np.zeros((1, 3))
         X[0, :len(p)] = p
         return float(self.evaluate(X)[0])
+
+    def program(self):
+        """(code, consts, grid functions) of the postfix program"""
+        code, consts, gfs = [], [], []
+        _compile_node(self._node(), code, consts, gfs)
+        return code, consts, gfs
 
     def _binop(self, o, op):
         o = self._lift(o)
@@ -134,10 +149,7 @@ class CoefficientFunction:
                         e = (e1[0] + e2[0], e1[1] + e2[1], e1[2] + e2[2])
                         t[e] = t.get(e, 0.0) + c1 * c2
                 return CoefficientFunction(terms=t)
-        a, b = self, o
-        fns = {"+": lambda X: a.evaluate(X) + b.evaluate(X), "-": lambda X: a.evaluate(X) - b.evaluate(X),
-               "*": lambda X: a.evaluate(X) * b.evaluate(X), "/": lambda X: a.evaluate(X) / b.evaluate(X)}
-        return CoefficientFunction(fn=fns[op])
+        return CoefficientFunction(expr=("bin", op, self._node(), o._node()))
 
     def __add__(self, o):
         return self._binop(o, "+")
@@ -152,7 +164,7 @@ class CoefficientFunction:
         return self._lift(o)._binop(self, "-")
 
     def __mul__(self, o):
-        if isinstance(o, (_Form, _CutDifferentialSymbol, _Measure)):
+        if isinstance(o, (_Form, _LinForm, _Proxy, _CutDifferentialSymbol, _Measure)):
             return NotImplemented
         return self._binop(o, "*")
 
@@ -165,8 +177,14 @@ class CoefficientFunction:
             return self * (1.0 / o._terms[(0, 0, 0)])
         return self._binop(o, "/")
 
+    def __rtruediv__(self, o):
+        return self._lift(o).__truediv__(self)
+
     def __neg__(self):
         return self * (-1.0)
+
+    def __abs__(self):
+        return CoefficientFunction(expr=("un", "abs", self._node()))
 
     def __pow__(self, k):
         if isinstance(k, int) and k >= 0 and self.is_poly():
@@ -174,8 +192,82 @@ class CoefficientFunction:
             for _ in range(k):
                 r = r * self
             return r
-        a = self
-        return CoefficientFunction(fn=lambda X: a.evaluate(X) ** k)
+        if isinstance(k, int):
+            return CoefficientFunction(expr=("powi", self._node(), k))
+        return CoefficientFunction(expr=("bin", "pow", self._node(), self._lift(k)._node()))
+
+
+# postfix opcodes of the device coefficient programs (include/xg_abi.h)
+_OP = {"const": 1, "coord": 2, "+": 3, "-": 4, "*": 5, "/": 6, "neg": 7, "powi": 8, "pow": 9, "sqrt": 10,
+       "sin": 11, "cos": 12, "tan": 13, "exp": 14, "log": 15, "atan": 16, "tanh": 17, "abs": 18, "gf": 19,
+       "ifpos": 20, "max": 21, "min": 22}
+_NP = {"sqrt": np.sqrt, "sin": np.sin, "cos": np.cos, "tan": np.tan, "exp": np.exp, "log": np.log,
+       "atan": np.arctan, "tanh": np.tanh, "abs": np.abs}
+
+
+def _eval_node(n, X):
+    k = n[0]
+    if k == "poly":
+        out = np.zeros(X.shape[0])
+        for (a, b, c), co in n[1].items():
+            out += co * X[:, 0] ** a * X[:, 1] ** b * X[:, 2] ** c
+        return out
+    if k == "bin":
+        a, b = _eval_node(n[2], X), _eval_node(n[3], X)
+        return {"+": lambda: a + b, "-": lambda: a - b, "*": lambda: a * b, "/": lambda: a / b,
+                "pow": lambda: np.power(a, b), "max": lambda: np.maximum(a, b),
+                "min": lambda: np.minimum(a, b)}[n[1]]()
+    if k == "un":
+        return _NP[n[1]](_eval_node(n[2], X))
+    if k == "powi":
+        return _eval_node(n[1], X) ** float(n[2])
+    if k == "ifpos":
+        return np.where(_eval_node(n[1], X) > 0, _eval_node(n[2], X), _eval_node(n[3], X))
+    if k == "gf":
+        raise NotImplementedError("grid functions are evaluated per element on the GPU (no host point location)")
+    raise ValueError(k)
+
+
+def _compile_node(n, code, consts, gfs):
+    def const(v):
+        v = float(v)
+        if v not in consts:
+            consts.append(v)
+        code.extend([_OP["const"], consts.index(v)])
+
+    k = n[0]
+    if k == "poly":
+        terms = sorted(n[1].items()) or [((0, 0, 0), 0.0)]
+        for t, ((a, b, c), co) in enumerate(terms):
+            const(co)
+            for d, e in enumerate((a, b, c)):
+                if e:
+                    code.extend([_OP["coord"], d])
+                    if e > 1:
+                        code.extend([_OP["powi"], e])
+                    code.append(_OP["*"])
+            if t:
+                code.append(_OP["+"])
+    elif k == "bin":
+        _compile_node(n[2], code, consts, gfs)
+        _compile_node(n[3], code, consts, gfs)
+        code.append(_OP[n[1]])
+    elif k == "un":
+        _compile_node(n[2], code, consts, gfs)
+        code.append(_OP[n[1]])
+    elif k == "powi":
+        _compile_node(n[1], code, consts, gfs)
+        code.extend([_OP["powi"], int(n[2])])
+    elif k == "ifpos":
+        for c in n[1:]:
+            _compile_node(c, code, consts, gfs)
+        code.append(_OP["ifpos"])
+    elif k == "gf":
+        if n[1] not in gfs:
+            gfs.append(n[1])
+        code.extend([_OP["gf"], gfs.index(n[1])])
+    else:
+        raise ValueError(k)
 
 
 x = CoefficientFunction(terms={(1, 0, 0): 1.0})
@@ -183,18 +275,40 @@ y = CoefficientFunction(terms={(0, 1, 0): 1.0})
 z = CoefficientFunction(terms={(0, 0, 1): 1.0})
 
 
-def _unary(f):
+def _unary(name):
+    f = _NP[name]
+
     def g(cf):
-        if not isinstance(cf, CoefficientFunction):
-            return f(cf)
-        return CoefficientFunction(fn=lambda X: f(cf.evaluate(X)))
+        if isinstance(cf, (int, float)):
+            return float(f(cf))
+        return CoefficientFunction(expr=("un", name, CoefficientFunction._lift(cf)._node()))
     return g
 
 
-sqrt = _unary(np.sqrt)
-sin = _unary(np.sin)
-cos = _unary(np.cos)
-exp = _unary(np.exp)
+sqrt = _unary("sqrt")
+sin = _unary("sin")
+cos = _unary("cos")
+tan = _unary("tan")
+exp = _unary("exp")
+log = _unary("log")
+atan = _unary("atan")
+tanh = _unary("tanh")
+
+
+def IfPos(cf, cf_then, cf_else):
+    """NGSolve IfPos: cf_then where cf > 0, else cf_else"""
+    L = CoefficientFunction._lift
+    return CoefficientFunction(expr=("ifpos", L(cf)._node(), L(cf_then)._node(), L(cf_else)._node()))
+
+
+def Max(a, b):
+    L = CoefficientFunction._lift
+    return CoefficientFunction(expr=("bin", "max", L(a)._node(), L(b)._node()))
+
+
+def Min(a, b):
+    L = CoefficientFunction._lift
+    return CoefficientFunction(expr=("bin", "min", L(a)._node(), L(b)._node()))
 
 
 # ------------------------------------------------------------------ spaces, grid functions
@@ -296,27 +410,50 @@ def Integrate(levelset_domain=None, *args, **kwargs):
 
 
 # ------------------------------------------------------------------ forms
+def _is_coef(o):
+    return isinstance(o, (int, float, CoefficientFunction, GridFunction))
+
+
+def _cmul(a, b):
+    """product of two form coefficients (numbers or coefficient functions)"""
+    if isinstance(a, (int, float)) and isinstance(b, (int, float)):
+        return float(a) * float(b)
+    return CoefficientFunction._lift(a) * CoefficientFunction._lift(b)
+
+
+def _as_const(c):
+    """a number for constant coefficients, None for coefficient functions"""
+    if isinstance(c, (int, float)):
+        return float(c)
+    if isinstance(c, CoefficientFunction) and c.is_poly() and set(c._terms) <= {(0, 0, 0)}:
+        return float(c._terms.get((0, 0, 0), 0.0))
+    return None
+
+
 class _Proxy:
-    def __init__(self, kind, space, deriv=0):
-        self.kind, self.space, self.deriv = kind, space, deriv
+    """trial / test function (or its gradient), times a coefficient"""
+
+    def __init__(self, kind, space, deriv=0, coef=1.0):
+        self.kind, self.space, self.deriv, self.coef = kind, space, deriv, coef
 
     def __mul__(self, o):
         if isinstance(o, _Proxy):
             if {self.kind, o.kind} != {"u", "v"} or self.deriv != o.deriv:
                 raise NotImplementedError("forms: u*v and grad(u)*grad(v) only")
-            return _Form({("mass" if self.deriv == 0 else "laplace"): 1.0}, self.space)
-        if isinstance(o, (int, float)) and self.kind == "v" and self.deriv == 0:
-            return _LinForm(float(o), self.space)
+            return _Form({("mass" if self.deriv == 0 else "laplace"): _cmul(self.coef, o.coef)}, self.space)
+        if _is_coef(o):
+            if self.kind == "v" and self.deriv == 0:
+                return _LinForm(_cmul(self.coef, o), self.space)
+            return _Proxy(self.kind, self.space, self.deriv, _cmul(self.coef, o))
         return NotImplemented
 
     def __rmul__(self, o):
-        if isinstance(o, (int, float)) and self.kind == "v" and self.deriv == 0:
-            return _LinForm(float(o), self.space)
-        return NotImplemented
+        return self.__mul__(o) if _is_coef(o) else NotImplemented
 
 
 class _LinForm:
-    """f * v with a constant f (SymbolicCutLinearFormIntegrator's source term)"""
+    """f * v with a number or a coefficient function f
+    (SymbolicCutLinearFormIntegrator's source term)"""
 
     def __init__(self, f, space):
         self.f, self.space = f, space
@@ -324,16 +461,16 @@ class _LinForm:
     def __mul__(self, o):
         if isinstance(o, (_CutDifferentialSymbol, _Measure)):
             return _SumOfIntegrals([_CutIntegral(self, o)])
-        if isinstance(o, (int, float)):
-            return _LinForm(self.f * o, self.space)
+        if _is_coef(o):
+            return _LinForm(_cmul(self.f, o), self.space)
         return NotImplemented
 
-    __rmul__ = lambda self, o: self.__mul__(o) if isinstance(o, (int, float)) else NotImplemented
+    __rmul__ = lambda self, o: self.__mul__(o) if _is_coef(o) else NotImplemented
 
 
 def grad(p):
     if isinstance(p, _Proxy):
-        return _Proxy(p.kind, p.space, 1)
+        return _Proxy(p.kind, p.space, 1, p.coef)
     raise NotImplementedError("grad of a proxy function only")
 
 
@@ -344,17 +481,17 @@ class _Form:
     def __add__(self, o):
         d = dict(self.parts)
         for k, c in o.parts.items():
-            d[k] = d.get(k, 0.0) + c
+            d[k] = (d[k] + c) if k in d else c
         return _Form(d, self.space)
 
     def __mul__(self, o):
         if isinstance(o, (_CutDifferentialSymbol, _Measure)):
             return _SumOfIntegrals([_CutIntegral(self, o)])
-        if isinstance(o, (int, float)):
-            return _Form({k: c * o for k, c in self.parts.items()}, self.space)
+        if _is_coef(o):
+            return _Form({k: _cmul(c, o) for k, c in self.parts.items()}, self.space)
         return NotImplemented
 
-    __rmul__ = lambda self, o: self.__mul__(o) if isinstance(o, (int, float)) else NotImplemented
+    __rmul__ = lambda self, o: self.__mul__(o) if _is_coef(o) else NotImplemented
 
 
 class _Measure:
@@ -423,7 +560,9 @@ class _CutIntegral:
     def element_matrices(self, mesh, p, precision):
         lset, dt, order, els = self._rule_args(mesh, p)
         mask = (1 if "mass" in self.integrand.parts else 0) | (2 if "laplace" in self.integrand.parts else 0)
-        coefs = self.integrand.parts
+        if any(_as_const(c) is None for c in self.integrand.parts.values()):
+            return self._element_matrices_cf(mesh, p, precision)
+        coefs = {k: _as_const(c) for k, c in self.integrand.parts.items()}
         L = self.measure.lsetdom.get("subdivlvl", 0) if isinstance(self.measure, _CutDifferentialSymbol) else 0
         if getattr(mesh, "is_quad", False):
             # quadrilaterals / hexahedra: the Q1 space on the quad / hex cut rule
@@ -454,10 +593,47 @@ class _CutIntegral:
         c = coefs.get("mass", coefs.get("laplace", 1.0))
         return e, c * m
 
+    def _cf_rule_args(self, mesh, p, what):
+        lset, dt, order, els = self._rule_args(mesh, p)
+        L = self.measure.lsetdom.get("subdivlvl", 0) if isinstance(self.measure, _CutDifferentialSymbol) else 0
+        if getattr(mesh, "is_quad", False) or isinstance(lset, (tuple, list)) or L > 0 or \
+                getattr(lset, "order", 1) == 2:
+            raise NotImplementedError(what + " with coefficient functions: straight cut rules of one P1 level "
+                                      "set on simplices")
+        return lset, dt, els
+
+    def _element_matrices_cf(self, mesh, p, precision):
+        """c u v / c grad u . grad v with coefficient functions: the rule's
+        weights scaled by c on the GPU, then the element matrices.  Rule order
+        as the reference's SymbolicCutBFI (symboliccutbfi.cpp:107-118): the
+        sum of the trial and test orders minus their derivative orders
+        (2p, or 2p - 2 for a form with gradients only), unless dCut sets it."""
+        lset, dt, els = self._cf_rule_args(mesh, p, "element matrices")
+        parts = self.integrand.parts
+        order = self.measure.lsetdom.get("order") if isinstance(self.measure, _CutDifferentialSymbol) else \
+            self.measure.order
+        if order is None:
+            order = 2 * p if "mass" in parts else max(2 * p - 2, 0)
+        e, tot = None, None
+        for part, c in parts.items():
+            e, m = CutElementMatricesCF(mesh, lset, dt, int(order), p, 1 if part == "mass" else 2, els, precision,
+                                        CoefficientFunction._lift(c))
+            tot = m if tot is None else tot + m
+        return e, tot
+
     def element_vectors(self, mesh, p):
+        f = self.integrand.f
+        if _as_const(f) is None:
+            # f v with a coefficient function: rule order 2p
+            # (symboliccutlfi.cpp:93-94), f evaluated on the GPU
+            lset, dt, els = self._cf_rule_args(mesh, p, "element vectors")
+            order = self.measure.lsetdom.get("order") if isinstance(self.measure, _CutDifferentialSymbol) else \
+                self.measure.order
+            return CutElementVectorsCF(mesh, lset, dt, int(order if order is not None else 2 * p), p, els,
+                                       CoefficientFunction._lift(f))
         lset, dt, order, els = self._rule_args(mesh, p)
         e, v = CutElementVectors(mesh, lset, dt, order, p, els)
-        return e, self.integrand.f * v
+        return e, _as_const(f) * v
 
     def integrate(self, mesh, element_wise=False):
         lset, dt, order, els = self._rule_args(mesh, 0)
@@ -541,7 +717,9 @@ class BilinearForm:
             L = ci.measure.lsetdom.get("subdivlvl", 0) if isinstance(ci.measure, _CutDifferentialSymbol) else 0
             if (L > 0 or getattr(lset, "order", 1) == 2) and order < 2 * p:
                 return None
-            parts = ci.integrand.parts
+            parts = {k: _as_const(c) for k, c in ci.integrand.parts.items()}
+            if any(c is None for c in parts.values()):
+                return None  # coefficient functions: element matrices on the GPU, host scatter
             mass, lap = "mass" in parts, "laplace" in parts
             if mass and lap and parts["mass"] == parts["laplace"]:
                 terms.append((lset, int(dt), int(order), p, 3, els, float(parts["mass"]), int(L)))
