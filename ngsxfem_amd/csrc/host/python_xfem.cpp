@@ -1380,17 +1380,31 @@ py::array_t<double> AssembleCSR(py::array_t<int32_t, py::array::c_style | py::ar
 // terms: [(lset, dt, order, p, mask, elements, coef, subdivlvl), ...];
 // precision: the element-matrix arithmetic of xg_cut_pipeline (P1 terms).
 py::tuple AssembleCutFormsDevice(py::object mesh, py::array_t<int32_t, py::array::c_style | py::array::forcecast> dofs,
-                                 int n, py::list terms, int precision) {
+                                 int n, py::list terms, int precision, py::object pattern_elements) {
   MA ma = as_mesh(mesh);
   simplices_only(ma, "AssembleCutFormsDevice");
   if (dofs.ndim() != 2) throw std::runtime_error("AssembleCutFormsDevice: dofs must be ne x ndof");
   const int ne = (int)dofs.shape(0), nd = (int)dofs.shape(1);
   if (ne != ma->ne) throw std::runtime_error("AssembleCutFormsDevice: one dof row per element");
+  // matrix graph: the dofs of all elements, or of the active elements of a
+  // restricted bilinear form (MeshEntityToDofTable with active_elems,
+  // utils/restrictedblf.cpp:8-64)
+  std::vector<int32_t> prow;
+  if (!pattern_elements.is_none()) {
+    std::vector<int32_t> pe = element_list(pattern_elements, ne);
+    prow.resize(pe.size() * (size_t)nd);
+    for (size_t i = 0; i < pe.size(); ++i) {
+      if (pe[i] < 0 || pe[i] >= ne) throw std::runtime_error("AssembleCutFormsDevice: element out of range");
+      std::copy(dofs.data() + (size_t)pe[i] * nd, dofs.data() + (size_t)(pe[i] + 1) * nd, prow.begin() + i * nd);
+    }
+  }
+  const int npat = pattern_elements.is_none() ? ne : (int)(prow.size() / nd);
   DevBuf<int32_t> d_dofs, rp((size_t)n + 1), nnz(1);
-  d_dofs.upload(dofs.data(), dofs.size());
-  const long long cap = (long long)ne * nd * nd;
-  DevBuf<int32_t> ci((size_t)std::max(cap, 1LL));
-  check(xg_csr_pattern(ctx(), ne, nd, d_dofs.p, n, rp.p, ci.p, cap, nnz.p));
+  if (pattern_elements.is_none()) d_dofs.upload(dofs.data(), dofs.size());
+  else d_dofs.upload(prow.data(), prow.size());
+  const long long cap = std::max((long long)npat * nd * nd, 1LL);
+  DevBuf<int32_t> ci((size_t)cap);
+  check(xg_csr_pattern(ctx(), npat, nd, d_dofs.p, n, rp.p, ci.p, cap, nnz.p));
   check(xg_sync(ctx()));
   int32_t h_nnz = 0;
   nnz.download(&h_nnz, 1);
@@ -1725,7 +1739,7 @@ PYBIND11_MODULE(xfem, m) {
         py::arg("order"), py::arg("p"), py::arg("elements") = py::none(),
         "element vectors of the source term f = 1 on the straight cut rule (FP64)");
   m.def("AssembleCutFormsDevice", &AssembleCutFormsDevice, py::arg("mesh"), py::arg("dofs"), py::arg("n"),
-        py::arg("terms"), py::arg("precision"),
+        py::arg("terms"), py::arg("precision"), py::arg("pattern_elements") = py::none(),
         "BilinearForm.Assemble on the device: pattern, element matrices and scatter; returns CSR arrays");
   m.def("AssembleCSR", &AssembleCSR, py::arg("dofs"), py::arg("elmats"), py::arg("row_ptr"), py::arg("col_idx"),
         "device scatter of element matrices into a CSR pattern (FP64 values)");

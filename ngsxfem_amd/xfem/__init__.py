@@ -704,6 +704,10 @@ class BilinearForm:
             self._elmats = [ci.element_matrices(mesh, self.space.order, self._prec) for ci in self.integrals]
         return self._elmats
 
+    def _pattern_elements(self):
+        """elements whose dof couplings make the matrix graph (None: all)"""
+        return None
+
     def _device_terms(self, mesh, p):
         """(lset, dt, order, p, mask, elements, coef, subdivlvl) per integral and
         form part, or None when an integral needs the host path"""
@@ -740,17 +744,49 @@ class BilinearForm:
         if p in (1, 2):
             dofs, n = self.space.dofs()
             terms = self._device_terms(mesh, p)
+            pat = self._pattern_elements()
             if terms is not None:
                 row_ptr, col_idx, values = AssembleCutFormsDevice(mesh, np.ascontiguousarray(dofs, dtype=np.int32),
-                                                                  int(n), terms, int(prec))
+                                                                  int(n), terms, int(prec), pat)
             else:
-                row_ptr, col_idx = _csr_pattern(dofs, n)
+                row_ptr, col_idx = _csr_pattern(dofs if pat is None else dofs[pat], n)
                 values = np.zeros(len(col_idx))
                 for els, mats in self.elmats:
                     values += AssembleCSR(np.ascontiguousarray(dofs[els]), np.asarray(mats, dtype=np.float64),
                                           row_ptr, col_idx)
             self.mat = csr_matrix((values, col_idx, row_ptr), shape=(n, n))
         return self
+
+
+class RestrictedBilinearForm(BilinearForm):
+    """A bilinear form whose matrix graph holds only the couplings of the
+    active elements (python/__init__.py:973-1037, utils/restrictedblf.cpp:
+    8-64,145-164): rows and columns of dofs outside them stay empty.  The
+    graph is built on the GPU from the active elements' dofs; an integral
+    with an element outside them has entries outside the graph, which is an
+    error here (XG_ERR_CAPACITY from the scatter).  facet_restriction only
+    concerns DG couplings, which the H1 spaces of this build do not have."""
+
+    def __init__(self, space=None, name="blf", element_restriction=None, facet_restriction=None, trialspace=None,
+                 testspace=None, **kwargs):
+        if trialspace is not None and testspace is not None:
+            if trialspace is not testspace:
+                raise NotImplementedError("RestrictedBilinearForm: one space for trial and test functions")
+            space = trialspace
+        if space is None:
+            raise Exception("No space given for RestrictedBilinearForm!")
+        super().__init__(space, **kwargs)
+        self.name = name
+        self.element_restriction = element_restriction
+        self.facet_restriction = facet_restriction
+
+    def _pattern_elements(self):
+        er = self.element_restriction
+        if er is None:
+            return None
+        if isinstance(er, BitArray):
+            return np.nonzero(np.asarray(er.numpy(), dtype=bool))[0].astype(np.int32)
+        return np.asarray(er, dtype=np.int32)
 
 
 class LinearForm:

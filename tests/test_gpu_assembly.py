@@ -145,3 +145,43 @@ def test_csr_pattern_device_matches_numpy(dev_abi):
     ctx.csr_pattern(bad, 10, rp[:11], ci, nnz)
     with pytest.raises(RuntimeError):
         ctx.sync()
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_restricted_bilinear_form(X, p):
+    """RestrictedBilinearForm (python/__init__.py:973-1037, utils/
+    restrictedblf.cpp:8-64,145-164): the matrix graph holds exactly the dof
+    pairs of the active elements (checked against a numpy set of pairs),
+    rows of dofs outside them are empty, and the values equal the full
+    BilinearForm's on the same integrals; an integral on an element outside
+    the graph is an error"""
+    mesh = X.MakeStructured3DMesh(nx=6, ny=6, nz=6)
+    lset = X.GridFunction(X.H1(mesh, order=1))
+    X.InterpolateToP1(X.sqrt((X.x - 0.5) ** 2 + (X.y - 0.5) ** 2 + (X.z - 0.5) ** 2) - 0.3, lset)
+    ci = X.CutInfo(mesh, lset)
+    act = ci.GetElementsOfType(X.HASNEG)
+    V = X.H1(mesh, order=p)
+    u, v = V.TnT()
+    form = (X.grad(u) * X.grad(v) + u * v) * X.dCut(lset, X.NEG, definedonelements=act)
+    a = X.RestrictedBilinearForm(V, element_restriction=act)
+    a += form
+    a.Assemble(precision=0)
+    b = X.BilinearForm(V)
+    b += form
+    b.Assemble(precision=0)
+    dofs, n = V.dofs()
+    ae = np.nonzero(np.asarray(act.numpy(), dtype=bool))[0]
+    d = dofs[ae]
+    pairs = np.unique(np.repeat(d, d.shape[1], axis=1).ravel().astype(np.int64) * n +
+                      np.tile(d, (1, d.shape[1])).ravel())
+    A = a.mat.tocoo()
+    got = np.unique(A.row.astype(np.int64) * n + A.col)
+    assert np.array_equal(got, pairs)
+    assert a.mat.nnz < b.mat.nnz
+    inactive = np.setdiff1d(np.arange(n), np.unique(d))
+    assert len(inactive) > 0 and np.all(np.diff(a.mat.indptr)[inactive] == 0)
+    assert abs(a.mat - b.mat).max() < 1e-14 * abs(b.mat).max()
+    c = X.RestrictedBilinearForm(V, element_restriction=act)
+    c += (u * v) * X.dCut(lset, X.POS)  # POS elements lie outside the graph
+    with pytest.raises(RuntimeError):
+        c.Assemble(precision=0)
