@@ -278,13 +278,15 @@ typedef struct xg_cf {
 } xg_cf;
 
 /* w_q <- w_q * cf(x_q) for the points of a straight cut / several-level-set
-   rule (layout of xg_cut_rule_fill, in place).  The element matrices
+   rule (layout of xg_cut_rule_fill, in place); coords holds A = D or D + 1
+   values per vertex (D + 1: boundary elements given in the ambient space,
+   x_q = sum_k lam_k X_k).  The element matrices
    (xg_elmat), element vectors (xg_elvec) and integrals (xg_integrate with
    f = 1) of the scaled rule are those of c u v / c grad u . grad v, f v and
    f.  No counterpart in the reference's interface: the reference applies the
    coefficient inside CalcElementMatrix / CalcElementVector /
    IntegrateX; this splits it out so the three kernels can share it. */
-int xg_rule_scale_cf(xg_ctx *ctx, int D, int nel, const int32_t *elems, const int32_t *el2v,
+int xg_rule_scale_cf(xg_ctx *ctx, int D, int A, int nel, const int32_t *elems, const int32_t *el2v,
                      const double *coords, const int32_t *offsets, double *pts, const xg_cf *cf);
 
 /* ---- linear forms on a materialised rule: SymbolicCutLinearFormIntegrator::

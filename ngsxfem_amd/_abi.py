@@ -72,7 +72,7 @@ _SIGS = {
     "xg_quad_elvec": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp],
     "xg_integrate_nodes": [_vp, _i, _vp, _vp, _vp, _i, _vp, _i, _vp, _vp, _vp, _vp],
     "xg_elmat": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
-    "xg_rule_scale_cf": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp],
+    "xg_rule_scale_cf": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp],
     "xg_elvec": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp],
     "xg_cut_pipeline": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _i, _vp],
     "xg_subdiv_weights": [_vp, _i, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp],
@@ -336,13 +336,14 @@ class Context:
         check(lib().xg_quad_elvec(self.h, D, nel, ptr(elems), ptr(el2v), ptr(coords), ptr(offsets), ptr(pts),
                                   ptr(out)))
 
-    def rule_scale_cf(self, D, el2v, coords, offsets, pts, code, consts, gfs=(), elems=None):
+    def rule_scale_cf(self, D, el2v, coords, offsets, pts, code, consts, gfs=(), elems=None, A=None):
         """pts[q].w *= cf(x_q); code / consts: the coefficient's postfix
-        program, gfs: device FP64 vertex values of its P1 grid functions"""
+        program, gfs: device FP64 vertex values of its P1 grid functions;
+        coords: A (default D) values per vertex"""
         nel = elems.shape[0] if elems is not None else el2v.shape[0]
         cf = _XgCf.make(code, consts, gfs)
-        check(lib().xg_rule_scale_cf(self.h, D, nel, ptr(elems), ptr(el2v), ptr(coords), ptr(offsets), ptr(pts),
-                                     ctypes.byref(cf)))
+        check(lib().xg_rule_scale_cf(self.h, D, D if A is None else A, nel, ptr(elems), ptr(el2v), ptr(coords),
+                                     ptr(offsets), ptr(pts), ctypes.byref(cf)))
 
     def elvec(self, D, p, el2v, coords, offsets, pts, out, elems=None):
         nel = elems.shape[0] if elems is not None else el2v.shape[0]

@@ -644,16 +644,30 @@ struct DeviceCf {
   xg_cf cf{};
 };
 
-void make_device_cf(py::object cfobj, int nv, DeviceCf &d) {
+// vmap (nullable): the mesh's vertices as vertices of the coefficient's
+// grid functions (the boundary submeshes' vertex copies)
+void make_device_cf(py::object cfobj, int nv, DeviceCf &d, const std::vector<int32_t> *vmap = nullptr) {
   if (!py::hasattr(cfobj, "program")) throw std::runtime_error("coefficient: a CoefficientFunction expected");
   py::tuple t = cfobj.attr("program")();
   d.code = t[0].cast<std::vector<int32_t>>();
   d.consts = t[1].cast<std::vector<double>>();
   for (auto g : t[2].cast<py::list>()) {
     auto a = py::array_t<double, py::array::c_style | py::array::forcecast>(g.attr("vec"));
-    if ((int)a.size() != nv) throw std::runtime_error("coefficient: P1 grid functions (one value per vertex)");
+    std::vector<double> vals;
+    if (vmap) {
+      if ((int)vmap->size() != nv) throw std::runtime_error("coefficient: vertex map size");
+      vals.resize(nv);
+      for (int k = 0; k < nv; ++k) {
+        const int32_t v = (*vmap)[k];
+        if (v < 0 || v >= (int)a.size()) throw std::runtime_error("coefficient: P1 grid functions (one value per vertex)");
+        vals[k] = a.data()[v];
+      }
+    } else {
+      if ((int)a.size() != nv) throw std::runtime_error("coefficient: P1 grid functions (one value per vertex)");
+      vals.assign(a.data(), a.data() + a.size());
+    }
     d.gfbuf.emplace_back(new DevBuf<double>());
-    d.gfbuf.back()->upload(a.data(), (size_t)a.size());
+    d.gfbuf.back()->upload(vals.data(), vals.size());
     d.gfp.push_back(d.gfbuf.back()->p);
   }
   d.cf.ncode = (int32_t)d.code.size();
@@ -1063,7 +1077,7 @@ py::object IntegrateX(py::dict lsetdom, py::object mesh, py::object cf, py::obje
     }
     DeviceCf dcf;
     make_device_cf(cf, ma->nv, dcf);
-    check(xg_rule_scale_cf(ctx(), D, ma->ne, nullptr, ma->el2v.p, ma->coords.p, r.offsets.p, r.pts.p, &dcf.cf));
+    check(xg_rule_scale_cf(ctx(), D, D, ma->ne, nullptr, ma->el2v.p, ma->coords.p, r.offsets.p, r.pts.p, &dcf.cf));
     const double one = 1.0;
     const int32_t ex0[3] = {0, 0, 0};
     DevBuf<double> d_pe(ma->ne), d_tot(1);
@@ -1279,8 +1293,35 @@ py::tuple CutElementVectors(py::object mesh, py::object lset, int dt, int order,
 // coefficient function c: the straight cut rule of the elements, its weights
 // scaled by c at the physical points (xg_rule_scale_cf), then the element
 // matrices of the scaled rule (xg_elmat)
+// ambient (nullable): the vertices' coordinates where the coefficient is
+// evaluated (nv x A, A = D + 1 for boundary submeshes), vmap (nullable): the
+// grid functions' vertex of each vertex
+struct AmbientCoords {
+  int A = 0;
+  DevBuf<double> buf;
+  std::vector<int32_t> vmap;
+  bool has_vmap = false;
+};
+
+void make_ambient(MA ma, py::object ambient, py::object vmap, AmbientCoords &a) {
+  a.A = ma->dim;
+  if (!ambient.is_none()) {
+    auto x = py::array_t<double, py::array::c_style | py::array::forcecast>(ambient);
+    if (x.ndim() != 2 || x.shape(0) != ma->nv || (x.shape(1) != ma->dim && x.shape(1) != ma->dim + 1))
+      throw std::runtime_error("coefficient coordinates: nv x D or nv x (D + 1)");
+    a.A = (int)x.shape(1);
+    a.buf.upload(x.data(), (size_t)x.size());
+  }
+  if (!vmap.is_none()) {
+    auto m = py::array_t<int32_t, py::array::c_style | py::array::forcecast>(vmap);
+    a.vmap.assign(m.data(), m.data() + m.size());
+    a.has_vmap = true;
+  }
+}
+
 py::tuple CutElementMatricesCF(py::object mesh, py::object lset, int dt, int order, int p, int mask,
-                               py::object elements, int precision, py::object cf) {
+                               py::object elements, int precision, py::object cf, py::object ambient,
+                               py::object vmap) {
   MA ma = as_mesh(mesh);
   simplices_only(ma, "CutElementMatricesCF");
   bool is_p1 = true;
@@ -1291,13 +1332,17 @@ py::tuple CutElementMatricesCF(py::object mesh, py::object lset, int dt, int ord
   std::vector<int32_t> el = element_list(elements, ma->ne);
   DeviceRule r;
   build_rule(ma, d_l.p, dt, order, &el, r);
+  AmbientCoords amb;
+  make_ambient(ma, ambient, vmap, amb);
   DeviceCf dcf;
-  make_device_cf(cf, ma->nv, dcf);
+  make_device_cf(cf, ma->nv, dcf, amb.has_vmap ? &amb.vmap : nullptr);
   const int D = ma->dim, nel = (int)el.size();
   int nd = 1;
   for (int k = 1; k <= D; ++k) nd = nd * (p + k) / k;
   py::array_t<int32_t> elarr((py::ssize_t)nel, el.data());
-  if (nel) check(xg_rule_scale_cf(ctx(), D, nel, r.elems.p, ma->el2v.p, ma->coords.p, r.offsets.p, r.pts.p, &dcf.cf));
+  if (nel)
+    check(xg_rule_scale_cf(ctx(), D, amb.A, nel, r.elems.p, ma->el2v.p, amb.buf.p ? amb.buf.p : ma->coords.p,
+                           r.offsets.p, r.pts.p, &dcf.cf));
   if (precision == XG_PREC_FP64) {
     DevBuf<double> out((size_t)std::max(nel, 1) * nd * nd);
     if (nel) check(xg_elmat(ctx(), D, p, nel, r.elems.p, ma->el2v.p, ma->coords.p, r.offsets.p, r.pts.p, mask,
@@ -1318,7 +1363,7 @@ py::tuple CutElementMatricesCF(py::object mesh, py::object lset, int dt, int ord
 
 // element vectors of f v with a coefficient function f (FP64)
 py::tuple CutElementVectorsCF(py::object mesh, py::object lset, int dt, int order, int p, py::object elements,
-                              py::object cf) {
+                              py::object cf, py::object ambient, py::object vmap) {
   MA ma = as_mesh(mesh);
   simplices_only(ma, "CutElementVectorsCF");
   bool is_p1 = true;
@@ -1329,20 +1374,60 @@ py::tuple CutElementVectorsCF(py::object mesh, py::object lset, int dt, int orde
   std::vector<int32_t> el = element_list(elements, ma->ne);
   DeviceRule r;
   build_rule(ma, d_l.p, dt, order, &el, r);
+  AmbientCoords amb;
+  make_ambient(ma, ambient, vmap, amb);
   DeviceCf dcf;
-  make_device_cf(cf, ma->nv, dcf);
+  make_device_cf(cf, ma->nv, dcf, amb.has_vmap ? &amb.vmap : nullptr);
   const int D = ma->dim, nel = (int)el.size();
   int nd = 1;
   for (int k = 1; k <= D; ++k) nd = nd * (p + k) / k;
   DevBuf<double> out((size_t)std::max(nel, 1) * nd);
   if (nel) {
-    check(xg_rule_scale_cf(ctx(), D, nel, r.elems.p, ma->el2v.p, ma->coords.p, r.offsets.p, r.pts.p, &dcf.cf));
+    check(xg_rule_scale_cf(ctx(), D, amb.A, nel, r.elems.p, ma->el2v.p, amb.buf.p ? amb.buf.p : ma->coords.p,
+                           r.offsets.p, r.pts.p, &dcf.cf));
     check(xg_elvec(ctx(), D, p, nel, r.elems.p, ma->el2v.p, ma->coords.p, r.offsets.p, r.pts.p, out.p));
   }
   check(xg_sync(ctx()));
   py::array_t<double> h({(py::ssize_t)nel, (py::ssize_t)nd});
   out.download(h.mutable_data(), (size_t)nel * nd);
   return py::make_tuple(py::array_t<int32_t>((py::ssize_t)nel, el.data()), h);
+}
+
+// integral of cf over the straight cut rule of a P1 level set on every
+// element (per element when element_wise), cf evaluated on the GPU at the
+// points given by ambient (nullable, see CutElementMatricesCF)
+py::object IntegrateCF(py::object mesh, py::object lset, int dt, int order, py::object cf, bool element_wise,
+                       py::object ambient, py::object vmap) {
+  MA ma = as_mesh(mesh);
+  simplices_only(ma, "IntegrateCF");
+  bool is_p1 = true;
+  std::vector<double> v = p1_values(lset, ma->nv, is_p1);
+  if (!is_p1) throw std::runtime_error("coefficient functions: straight cut rules of P1 level sets");
+  DevBuf<double> d_l;
+  d_l.upload(v.data(), v.size());
+  DeviceRule r;
+  build_rule(ma, d_l.p, dt, order, nullptr, r);
+  AmbientCoords amb;
+  make_ambient(ma, ambient, vmap, amb);
+  DeviceCf dcf;
+  make_device_cf(cf, ma->nv, dcf, amb.has_vmap ? &amb.vmap : nullptr);
+  const int D = ma->dim;
+  std::vector<double> pe(ma->ne, 0.0);
+  double total = 0.0;
+  if (ma->ne) {
+    check(xg_rule_scale_cf(ctx(), D, amb.A, ma->ne, nullptr, ma->el2v.p, amb.buf.p ? amb.buf.p : ma->coords.p,
+                           r.offsets.p, r.pts.p, &dcf.cf));
+    const double one = 1.0;
+    const int32_t ex0[3] = {0, 0, 0};
+    DevBuf<double> d_pe(ma->ne), d_tot(1);
+    check(xg_integrate(ctx(), D, ma->ne, nullptr, ma->el2v.p, ma->coords.p, r.offsets.p, r.pts.p, 1, &one, ex0,
+                       d_pe.p, d_tot.p));
+    check(xg_sync(ctx()));
+    d_pe.download(pe.data(), ma->ne);
+    d_tot.download(&total, 1);
+  }
+  if (element_wise) return py::array_t<double>((py::ssize_t)pe.size(), pe.data());
+  return py::float_(total);
 }
 
 // device scatter of element matrices into a CSR pattern (the assembly loop of
@@ -1755,11 +1840,14 @@ PYBIND11_MODULE(xfem, m) {
         "element matrices of (mass | laplace) on the rule of several level sets");
   m.def("CutElementMatricesCF", &CutElementMatricesCF, py::arg("mesh"), py::arg("levelset"),
         py::arg("domain_type"), py::arg("order"), py::arg("p"), py::arg("mask"), py::arg("elements"),
-        py::arg("precision"), py::arg("cf"),
+        py::arg("precision"), py::arg("cf"), py::arg("ambient") = py::none(), py::arg("vmap") = py::none(),
         "element matrices of cf u v (mask 1) / cf grad u . grad v (mask 2), cf evaluated on the GPU");
+  m.def("IntegrateCF", &IntegrateCF, py::arg("mesh"), py::arg("levelset"), py::arg("domain_type"), py::arg("order"),
+        py::arg("cf"), py::arg("element_wise") = false, py::arg("ambient") = py::none(), py::arg("vmap") = py::none(),
+        "integral of cf on the straight cut rule, cf evaluated on the GPU");
   m.def("CutElementVectorsCF", &CutElementVectorsCF, py::arg("mesh"), py::arg("levelset"), py::arg("domain_type"),
-        py::arg("order"), py::arg("p"), py::arg("elements"), py::arg("cf"),
-        "element vectors of cf v on the straight cut rule, cf evaluated on the GPU (FP64)");
+        py::arg("order"), py::arg("p"), py::arg("elements"), py::arg("cf"), py::arg("ambient") = py::none(),
+        py::arg("vmap") = py::none(), "element vectors of cf v on the straight cut rule, cf evaluated on the GPU (FP64)");
   m.def("CutElementMatrices", &CutElementMatrices, py::arg("mesh"), py::arg("levelset"), py::arg("domain_type"),
         py::arg("order"), py::arg("p"), py::arg("mask") = 3, py::arg("elements") = py::none(),
         py::arg("precision") = XG_PREC_FP32);

@@ -33,8 +33,10 @@ __device__ __forceinline__ void element_geometry(const int32_t *__restrict__ el2
   }
 }
 
-// w_q <- w_q * cf(x_q) for every point of every listed element
-template <int D>
+// w_q <- w_q * cf(x_q) for every point of every listed element; the
+// element's vertices have A coordinates (A = D, or D + 1 for boundary
+// elements given in the ambient space), x_q = sum_k lam_k X_k
+template <int D, int A>
 __global__ __launch_bounds__(256) void k_rule_scale_cf(int nel, const int32_t *__restrict__ elems,
                                                        const int32_t *__restrict__ el2v,
                                                        const double *__restrict__ coords,
@@ -47,7 +49,12 @@ __global__ __launch_bounds__(256) void k_rule_scale_cf(int nel, const int32_t *_
     const int e = elems ? elems[i] : i;
     int32_t vv[D + 1];
     double V[D + 1][3];
-    element_geometry<D>(el2v, coords, e, vv, V);
+#pragma unroll
+    for (int k = 0; k <= D; ++k) {
+      vv[k] = el2v[(size_t)e * (D + 1) + k];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) V[k][d] = d < A ? coords[(size_t)vv[k] * A + d] : 0.0;
+    }
     for (int q = offsets[i] + lane; q < offsets[i + 1]; q += 64) {
       double4 p = pts[q];
       const double xi[3] = {p.x, p.y, p.z};
@@ -61,7 +68,7 @@ __global__ __launch_bounds__(256) void k_rule_scale_cf(int nel, const int32_t *_
       }
       lam[D] = sl;
 #pragma unroll
-      for (int r = 0; r < D; ++r) {
+      for (int r = 0; r < A; ++r) {
         double a = V[D][r];
 #pragma unroll
         for (int c = 0; c < D; ++c) a += (V[c][r] - V[D][r]) * xi[c];
@@ -130,9 +137,10 @@ int cf_grid(int nel) {
 
 }  // namespace
 
-extern "C" int xg_rule_scale_cf(xg_ctx *ctx, int D, int nel, const int32_t *elems, const int32_t *el2v,
+extern "C" int xg_rule_scale_cf(xg_ctx *ctx, int D, int A, int nel, const int32_t *elems, const int32_t *el2v,
                                 const double *coords, const int32_t *offsets, double *pts, const xg_cf *cf) {
-  if (!ctx || D < 1 || D > 3 || nel < 0 || !cf) return set_error(XG_ERR_ARG, "xg_rule_scale_cf: bad argument");
+  if (!ctx || D < 1 || D > 3 || (A != D && A != D + 1) || A > 3 || nel < 0 || !cf)
+    return set_error(XG_ERR_ARG, "xg_rule_scale_cf: bad argument");
   cf::Prog P;
   if (const char *msg = cf::validate(cf->ncode, cf->code, cf->nconst, cf->consts, cf->ngf, cf->gf, P))
     return set_error(XG_ERR_ARG, std::string("xg_rule_scale_cf: ") + msg);
@@ -140,11 +148,15 @@ extern "C" int xg_rule_scale_cf(xg_ctx *ctx, int D, int nel, const int32_t *elem
   if (!el2v || !coords || !offsets || !pts) return set_error(XG_ERR_ARG, "xg_rule_scale_cf: null pointer");
   const int grid = cf_grid(nel);
   double4 *p = reinterpret_cast<double4 *>(pts);
-  switch (D) {
-    case 1: k_rule_scale_cf<1><<<grid, 256, 0, ctx->stream>>>(nel, elems, el2v, coords, offsets, p, P); break;
-    case 2: k_rule_scale_cf<2><<<grid, 256, 0, ctx->stream>>>(nel, elems, el2v, coords, offsets, p, P); break;
-    default: k_rule_scale_cf<3><<<grid, 256, 0, ctx->stream>>>(nel, elems, el2v, coords, offsets, p, P); break;
+#define XG_SC(DD, AA) k_rule_scale_cf<DD, AA><<<grid, 256, 0, ctx->stream>>>(nel, elems, el2v, coords, offsets, p, P)
+  switch (D * 4 + A) {
+    case 5: XG_SC(1, 1); break;
+    case 6: XG_SC(1, 2); break;
+    case 10: XG_SC(2, 2); break;
+    case 11: XG_SC(2, 3); break;
+    default: XG_SC(3, 3); break;
   }
+#undef XG_SC
   XG_LAUNCH_CHECK();
   return XG_OK;
 }

@@ -31,7 +31,7 @@ load_hip_runtime()  # before the host module loads libxg.so: one HIP runtime per
 from .xfem import (ANY, BND, CDOM_IF, CDOM_NEG, CDOM_POS, FALLBACK, FIRST, HASNEG, HASPOS, IF, NEG, NO,  # noqa: F401
                    OPTIMAL, POS, UNCUT, VOL, AssembleCSR, AssembleCutFormsDevice, AssembleVector, BitArray,
                    COMBINED_DOMAIN_TYPE, CutElementMatrices, CutElementMatricesCF, CutElementVectors,
-                   CutElementVectorsCF, CutInfo, DOMAIN_TYPE, GetDofsOfElements,
+                   CutElementVectorsCF, CutInfo, IntegrateCF, DOMAIN_TYPE, GetDofsOfElements,
                    GetDofsOfFacets, GetElementsWithNeighborFacets, GetElementsWithSharedVertex,
                    GetFacetsWithNeighborTypes, IntegrateX, MlsetElementMatrices, MultiLevelsetCutInfo, NODE_TYPE,
                    QuadElementMatrices, SubdivElementMatrices,
@@ -394,8 +394,68 @@ def Integrate_X_special_args(levelset_domain={}, cf=None, mesh=None, VOL_or_BND=
         lsd["time_order"] = time_order
     if isinstance(cf, (int, float)):
         cf = CoefficientFunction(float(cf))
+    if VOL_or_BND == BND:
+        if deformation is not None or ip_container is not None:
+            raise NotImplementedError("Integrate on the boundary: no deformation, no ip_container")
+        return _integrate_bnd(mesh, lsd["levelset"], lsd["domain_type"], lsd["order"], cf, element_wise)
     return IntegrateX(levelset_domain=lsd, mesh=mesh, cf=cf, deformation=deformation,
                       ip_container=ip_container, element_wise=element_wise, heapsize=heapsize)
+
+
+def _bnd_submesh(mesh):
+    """The boundary elements as a mesh of dimension D - 1: every boundary
+    simplex with its own vertex copies in an isometric frame of its own
+    (v0 at the origin, v1 on the first axis).  Cut rules in reference
+    coordinates depend only on the level set values, the interface weights'
+    |F^-T n| and the measures are invariant under isometries, and so are the
+    trace space's element matrices, so the (D-1)-dimensional kernels compute
+    the boundary integrals; coefficients are evaluated at the ambient
+    coordinates.  Returns (submesh, vmap = mesh vertex of each copy,
+    ambient = their coordinates); cached on the mesh."""
+    c = getattr(mesh, "_bnd_sub", None)
+    if c is not None:
+        return c
+    D = mesh.dim
+    if D not in (2, 3) or getattr(mesh, "is_quad", False):
+        raise NotImplementedError("boundary integrals: triangle and tetrahedral meshes")
+    b = np.ascontiguousarray(mesh.bnd2v, dtype=np.int32)
+    nb = b.shape[0]
+    X = mesh.coords[b]  # nb x D x D
+    e1 = X[:, 1] - X[:, 0]
+    l1 = np.sqrt(np.sum(e1 * e1, axis=1))
+    if D == 2:
+        loc = np.zeros((nb, 2, 1))
+        loc[:, 1, 0] = l1
+    else:
+        u = e1 / l1[:, None]
+        e2 = X[:, 2] - X[:, 0]
+        a = np.sum(e2 * u, axis=1)
+        h = np.sqrt(np.maximum(np.sum(e2 * e2, axis=1) - a * a, 0.0))
+        loc = np.zeros((nb, 3, 2))
+        loc[:, 1, 0] = l1
+        loc[:, 2, 0] = a
+        loc[:, 2, 1] = h
+    from ..meshes import Mesh
+    sub = Mesh(loc.reshape(nb * D, D - 1), np.arange(nb * D, dtype=np.int32).reshape(nb, D))
+    vmap = b.reshape(-1).copy()
+    amb = np.ascontiguousarray(mesh.coords[vmap])
+    mesh._bnd_sub = (sub, vmap, amb)
+    return mesh._bnd_sub
+
+
+def _bnd_lset(lset, vmap):
+    if getattr(lset, "order", 1) != 1 or isinstance(lset, (tuple, list)):
+        raise NotImplementedError("boundary integrals: one P1 level set")
+    v = np.asarray(lset.vec if hasattr(lset, "vec") else lset, dtype=np.float64)
+    return np.ascontiguousarray(v[vmap])
+
+
+def _integrate_bnd(mesh, lset, dt, order, cf, element_wise):
+    """Integrate over the boundary elements (VOL_or_BND=BND): the cut rule of
+    the level set's trace on each boundary simplex, cf on the GPU"""
+    sub, vmap, amb = _bnd_submesh(mesh)
+    return IntegrateCF(sub, _bnd_lset(lset, vmap), int(dt), int(order), CoefficientFunction._lift(cf),
+                       bool(element_wise), amb, vmap)
 
 
 def Integrate(levelset_domain=None, *args, **kwargs):
@@ -511,10 +571,12 @@ class _CutDifferentialSymbol:
 
     def __init__(self, lsetdom, definedon=None, vb=VOL, element_boundary=False, element_vb=VOL, skeleton=False,
                  deformation=None, definedonelements=None):
-        if deformation is not None or skeleton or element_boundary or vb != VOL:
-            raise NotImplementedError("dCut: VOL element integrals without deformation in this build")
+        if deformation is not None or skeleton or element_boundary or vb not in (VOL, BND):
+            raise NotImplementedError("dCut: element (VOL) and boundary (BND) integrals without deformation, "
+                                      "element_boundary or skeleton in this build")
         self.lsetdom = lsetdom
         self.definedonelements = definedonelements
+        self.vb = vb
 
     def __rmul__(self, o):
         if isinstance(o, (_Form, _LinForm)):
@@ -557,7 +619,50 @@ class _CutIntegral:
             return tuple(d["levelset"]), d["domain_type"], d.get("order", 2 * p), m.definedonelements
         return d["levelset"], int(d["domain_type"]), d.get("order", 2 * p), m.definedonelements
 
+    def is_bnd(self):
+        return isinstance(self.measure, _CutDifferentialSymbol) and self.measure.vb == BND
+
+    def dof_table(self, space):
+        """global dofs of the elements this integral runs over"""
+        return space.mesh.bnd_dofs(space.order)[0] if self.is_bnd() else space.dofs()[0]
+
+    def _bnd_args(self, mesh, p):
+        lset, dt, order, els = self._rule_args(mesh, p)
+        if self.measure.lsetdom.get("subdivlvl", 0):
+            raise NotImplementedError("boundary integrals: straight cut rules (subdivlvl 0)")
+        sub, vmap, amb = _bnd_submesh(mesh)
+        return sub, vmap, amb, _bnd_lset(lset, vmap), int(dt), els
+
+    def _cf_order(self, p):
+        d = self.measure.lsetdom if isinstance(self.measure, _CutDifferentialSymbol) else {}
+        order = d.get("order") if isinstance(self.measure, _CutDifferentialSymbol) else self.measure.order
+        if order is None:
+            order = 2 * p if "mass" in self.integrand.parts else max(2 * p - 2, 0)
+        return int(order)
+
+    def _element_matrices_bnd(self, mesh, p, precision):
+        """forms on the boundary elements (dCut(..., vb=BND)): the trace
+        space's element matrices on the boundary simplices' cut rules"""
+        sub, vmap, amb, lsub, dt, els = self._bnd_args(mesh, p)
+        parts = self.integrand.parts
+        if all(_as_const(c) is not None for c in parts.values()):
+            order = self.measure.lsetdom.get("order", 2 * p)
+            e, tot = None, None
+            for part, c in parts.items():
+                e, m = CutElementMatrices(sub, lsub, dt, int(order), p, 1 if part == "mass" else 2, els, precision)
+                tot = _as_const(c) * m if tot is None else tot + _as_const(c) * m
+            return e, tot
+        order = self._cf_order(p)
+        e, tot = None, None
+        for part, c in parts.items():
+            e, m = CutElementMatricesCF(sub, lsub, dt, order, p, 1 if part == "mass" else 2, els, precision,
+                                        CoefficientFunction._lift(c), amb, vmap)
+            tot = m if tot is None else tot + m
+        return e, tot
+
     def element_matrices(self, mesh, p, precision):
+        if self.is_bnd():
+            return self._element_matrices_bnd(mesh, p, precision)
         lset, dt, order, els = self._rule_args(mesh, p)
         mask = (1 if "mass" in self.integrand.parts else 0) | (2 if "laplace" in self.integrand.parts else 0)
         if any(_as_const(c) is None for c in self.integrand.parts.values()):
@@ -610,10 +715,7 @@ class _CutIntegral:
         (2p, or 2p - 2 for a form with gradients only), unless dCut sets it."""
         lset, dt, els = self._cf_rule_args(mesh, p, "element matrices")
         parts = self.integrand.parts
-        order = self.measure.lsetdom.get("order") if isinstance(self.measure, _CutDifferentialSymbol) else \
-            self.measure.order
-        if order is None:
-            order = 2 * p if "mass" in parts else max(2 * p - 2, 0)
+        order = self._cf_order(p)
         e, tot = None, None
         for part, c in parts.items():
             e, m = CutElementMatricesCF(mesh, lset, dt, int(order), p, 1 if part == "mass" else 2, els, precision,
@@ -623,6 +725,13 @@ class _CutIntegral:
 
     def element_vectors(self, mesh, p):
         f = self.integrand.f
+        if self.is_bnd():
+            sub, vmap, amb, lsub, dt, els = self._bnd_args(mesh, p)
+            order = int(self.measure.lsetdom.get("order", 2 * p))
+            if _as_const(f) is None:
+                return CutElementVectorsCF(sub, lsub, dt, order, p, els, CoefficientFunction._lift(f), amb, vmap)
+            e, v = CutElementVectors(sub, lsub, dt, order, p, els)
+            return e, _as_const(f) * v
         if _as_const(f) is None:
             # f v with a coefficient function: rule order 2p
             # (symboliccutlfi.cpp:93-94), f evaluated on the GPU
@@ -636,6 +745,14 @@ class _CutIntegral:
         return e, _as_const(f) * v
 
     def integrate(self, mesh, element_wise=False):
+        if self.is_bnd():
+            sub, vmap, amb, lsub, dt, els = self._bnd_args(mesh, 0)
+            order = self.measure.lsetdom.get("order", 5)
+            r = IntegrateCF(sub, lsub, dt, int(order if order is not None else 5),
+                            CoefficientFunction._lift(self.integrand), True, amb, vmap)
+            if els is not None:
+                r = r * np.asarray(els.numpy(), dtype=np.float64)
+            return r if element_wise else float(np.sum(r))
         lset, dt, order, els = self._rule_args(mesh, 0)
         lsd = {"levelset": lset, "domain_type": dt, "order": order if order is not None else 5}
         if isinstance(self.measure, _CutDifferentialSymbol) and self.measure.lsetdom.get("subdivlvl", 0):
@@ -658,7 +775,7 @@ def SymbolicCutBFI(levelset_domain, form, VOL_or_BND=VOL, element_boundary=False
                    definedonelements=None, deformation=None):
     """xfem/python_xfem.cpp:855-931"""
     return _SumOfIntegrals([_CutIntegral(form, _CutDifferentialSymbol(
-        levelset_domain, definedonelements=definedonelements, deformation=deformation,
+        levelset_domain, vb=VOL_or_BND, definedonelements=definedonelements, deformation=deformation,
         element_boundary=element_boundary, skeleton=skeleton))])
 
 
@@ -715,6 +832,8 @@ class BilinearForm:
             return None
         terms = []
         for ci in self.integrals:
+            if ci.is_bnd():
+                return None  # boundary forms: element matrices on the GPU, host scatter
             lset, dt, order, els = ci._rule_args(mesh, p)
             if isinstance(lset, (tuple, list)):
                 return None
@@ -751,8 +870,9 @@ class BilinearForm:
             else:
                 row_ptr, col_idx = _csr_pattern(dofs if pat is None else dofs[pat], n)
                 values = np.zeros(len(col_idx))
-                for els, mats in self.elmats:
-                    values += AssembleCSR(np.ascontiguousarray(dofs[els]), np.asarray(mats, dtype=np.float64),
+                for ci, (els, mats) in zip(self.integrals, self.elmats):
+                    tab = ci.dof_table(self.space)
+                    values += AssembleCSR(np.ascontiguousarray(tab[els]), np.asarray(mats, dtype=np.float64),
                                           row_ptr, col_idx)
             self.mat = csr_matrix((values, col_idx, row_ptr), shape=(n, n))
         return self
@@ -813,6 +933,7 @@ class LinearForm:
         out = np.zeros(n)
         for ci in self.integrals:
             els, vecs = ci.element_vectors(mesh, p)
-            out += AssembleVector(np.ascontiguousarray(dofs[els]), np.asarray(vecs, dtype=np.float64), n)
+            tab = ci.dof_table(self.space)
+            out += AssembleVector(np.ascontiguousarray(tab[els]), np.asarray(vecs, dtype=np.float64), n)
         self.vec = out
         return self
