@@ -1181,7 +1181,7 @@ py::tuple QuadElementMatrices(py::object mesh, py::object lset, int dt, int orde
 }
 
 py::tuple MlsetElementMatrices(py::object mesh, py::object lsets, py::object dts, int order, int p, int mask,
-                               py::object elements, int precision) {
+                               py::object elements, int precision, py::object cf) {
   MA ma = as_mesh(mesh);
   simplices_only(ma, "MlsetElementMatrices");
   MlsetDomain md;
@@ -1205,6 +1205,12 @@ py::tuple MlsetElementMatrices(py::object mesh, py::object lsets, py::object dts
   py::array_t<int32_t> elarr((py::ssize_t)nel, el.data());
   DeviceRule r;
   build_mlset_rule(ma, md, order, &el, r);
+  if (!cf.is_none() && nel) {
+    // a coefficient function: the rule's weights scaled by it
+    DeviceCf dcf;
+    make_device_cf(cf, ma->nv, dcf);
+    check(xg_rule_scale_cf(ctx(), D, D, nel, r.elems.p, ma->el2v.p, ma->coords.p, r.offsets.p, r.pts.p, &dcf.cf));
+  }
   const bool f64 = precision == XG_PREC_FP64;
   DevBuf<double> o64;
   DevBuf<float> o32;
@@ -1836,8 +1842,8 @@ PYBIND11_MODULE(xfem, m) {
         "element matrices of (mass | laplace) on the subdivision rule of a P2 level set (tetrahedra)");
   m.def("MlsetElementMatrices", &MlsetElementMatrices, py::arg("mesh"), py::arg("levelsets"),
         py::arg("domain_type"), py::arg("order"), py::arg("p"), py::arg("mask") = 3,
-        py::arg("elements") = py::none(), py::arg("precision") = XG_PREC_FP32,
-        "element matrices of (mass | laplace) on the rule of several level sets");
+        py::arg("elements") = py::none(), py::arg("precision") = XG_PREC_FP32, py::arg("cf") = py::none(),
+        "element matrices of (mass | laplace) on the rule of several level sets (times cf when given)");
   m.def("CutElementMatricesCF", &CutElementMatricesCF, py::arg("mesh"), py::arg("levelset"),
         py::arg("domain_type"), py::arg("order"), py::arg("p"), py::arg("mask"), py::arg("elements"),
         py::arg("precision"), py::arg("cf"), py::arg("ambient") = py::none(), py::arg("vmap") = py::none(),

@@ -701,6 +701,8 @@ class _CutIntegral:
     def _cf_rule_args(self, mesh, p, what):
         lset, dt, order, els = self._rule_args(mesh, p)
         L = self.measure.lsetdom.get("subdivlvl", 0) if isinstance(self.measure, _CutDifferentialSymbol) else 0
+        if what == "element matrices" and isinstance(lset, (tuple, list)) and not getattr(mesh, "is_quad", False):
+            return lset, dt, els  # several level sets: MlsetElementMatrices with cf
         if getattr(mesh, "is_quad", False) or isinstance(lset, (tuple, list)) or L > 0 or \
                 getattr(lset, "order", 1) == 2:
             raise NotImplementedError(what + " with coefficient functions: straight cut rules of one P1 level "
@@ -718,8 +720,13 @@ class _CutIntegral:
         order = self._cf_order(p)
         e, tot = None, None
         for part, c in parts.items():
-            e, m = CutElementMatricesCF(mesh, lset, dt, int(order), p, 1 if part == "mass" else 2, els, precision,
-                                        CoefficientFunction._lift(c))
+            msk = 1 if part == "mass" else 2
+            if isinstance(lset, (tuple, list)):
+                e, m = MlsetElementMatrices(mesh, tuple(lset), dt, int(order), p, msk, els, precision,
+                                            CoefficientFunction._lift(c))
+            else:
+                e, m = CutElementMatricesCF(mesh, lset, dt, int(order), p, msk, els, precision,
+                                            CoefficientFunction._lift(c))
             tot = m if tot is None else tot + m
         return e, tot
 

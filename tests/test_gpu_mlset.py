@@ -275,3 +275,32 @@ def test_dcut_element_matrices(X, orc):
         p, w = orc.mlset_rule(2, lv[:, vs], tri.as_list, 4, V=mesh.coords[vs])
         ref = orc.elmat(2, 2, mesh.coords[vs], p, w, mask=1) if len(w) else np.zeros((6, 6))
         assert np.abs(mats[k] - ref).max() <= 1e-13 * max(np.abs(ref).max(), 1e-300)
+
+
+def test_dcut_coefficient_function(X, orc):
+    """a coefficient function on the rule of several level sets: the
+    oracle's element matrices of its mlset rule with the weights scaled by
+    the coefficient at the physical points (numpy evaluation)"""
+    from ngsxfem_amd.xfem.mlset import DomainTypeArray
+    mesh = mesh2d(-1, 1, -0.5, 1.5, 8)
+    fs = [lambda x, y: y - 1, lambda x, y: 2 * x - y, lambda x, y: -2 * x - y]
+    ls = gfs(X, mesh, fs)
+    lv = np.stack([g.vec for g in ls])
+    tri = DomainTypeArray((NEG, NEG, NEG))
+    V = X.H1(mesh, order=2)
+    u, v = V.TnT()
+    c = 1 + X.x * X.y + X.sin(X.y)
+    integral = ((c * u * v) * X.dCut(levelset=ls, domain_type=tri, order=4)).parts[0]
+    els, mats = integral.element_matrices(mesh, 2, X.PREC_FP64)
+    assert len(els) > 0
+    for k, e in enumerate(els):
+        vs = mesh.el2v[e]
+        Vx = mesh.coords[vs]
+        p, w = orc.mlset_rule(2, lv[:, vs], tri.as_list, 4, V=Vx)
+        if not len(w):
+            assert np.abs(mats[k]).max() == 0.0
+            continue
+        xp = np.zeros((len(w), 3))
+        xp[:, :2] = Vx[2] + p[:, :2] @ np.stack([Vx[0] - Vx[2], Vx[1] - Vx[2]], axis=1).T
+        ref = orc.elmat(2, 2, Vx, p, w * c.evaluate(xp), mask=1)
+        assert np.abs(mats[k] - ref).max() <= 1e-13 * max(np.abs(ref).max(), 1e-300)
