@@ -75,6 +75,14 @@ int xg_rule_npts(int D, int order);
 /* ---- a2: InterpolateP1 perturbation (utils/p1interpol.cpp:104-107) ----- */
 /* v[i] = eps if |v[i]| < eps, in place (n values, device). */
 int xg_p1_perturb(xg_ctx *ctx, int n, double *v, double eps);
+/* struct xg_cf: a coefficient function's postfix program (declared below) */
+struct xg_cf;
+/* InterpolateP1::Do (utils/p1interpol.cpp:27-113) for a coefficient
+   function: out[v] = cf(x_v) at the nv vertices (coords nv x D, device),
+   then v = eps where |v| < eps.  Grid function opcodes read the vertex's own
+   value. */
+int xg_p1_interpolate_cf(xg_ctx *ctx, int D, int nv, const double *coords, const struct xg_cf *cf, double eps,
+                         double *out);
 
 /* ---- a5, a12: CutInformation::Update marking (xfem/cutinfo.cpp:61-133) */
 /* D = element dimension (1, 2, 3); el2v: ne x (D+1).

@@ -1814,6 +1814,20 @@ PYBIND11_MODULE(xfem, m) {
         py::arg("deformation") = py::none(), py::arg("ip_container") = py::none(), py::arg("element_wise") = false,
         py::arg("heapsize") = 1000000);
 
+  m.def("InterpolateCFToP1",
+        [](py::object mesh, py::object cf, double eps) {
+          MA ma = as_mesh(mesh);
+          DeviceCf dcf;
+          make_device_cf(cf, ma->nv, dcf);
+          DevBuf<double> out((size_t)std::max(ma->nv, 1));
+          check(xg_p1_interpolate_cf(ctx(), ma->dim, ma->nv, ma->coords.p, &dcf.cf, eps, out.p));
+          check(xg_sync(ctx()));
+          py::array_t<double> r(ma->nv);
+          out.download(r.mutable_data(), (size_t)ma->nv);
+          return r;
+        },
+        py::arg("mesh"), py::arg("cf"), py::arg("eps_perturbation") = 1e-14,
+        "vertex values of a coefficient function on the GPU with the P1 perturbation (InterpolateP1)");
   m.def("PerturbP1",
         [](py::array_t<double, py::array::c_style | py::array::forcecast> v, double eps) {
           DevBuf<double> d;

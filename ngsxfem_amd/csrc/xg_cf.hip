@@ -128,6 +128,28 @@ __global__ __launch_bounds__(256) void k_elvec_rule(int nel, const int32_t *__re
   }
 }
 
+// InterpolateP1::Do (utils/p1interpol.cpp:27-113): the coefficient at every
+// vertex, then the perturbation |v| < eps -> eps (:104-107); one lane per
+// vertex (a grid function opcode reads the vertex's own value)
+template <int D>
+__global__ __launch_bounds__(256) void k_p1_interpolate_cf(int nv, const double *__restrict__ coords, const cf::Prog P,
+                                                           double eps, double *__restrict__ out) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  double x[3] = {0.0, 0.0, 0.0}, lam[D + 1];
+  int32_t vv[D + 1];
+#pragma unroll
+  for (int d = 0; d < D; ++d) x[d] = coords[(size_t)v * D + d];
+#pragma unroll
+  for (int k = 0; k <= D; ++k) {
+    lam[k] = k == 0 ? 1.0 : 0.0;
+    vv[k] = v;
+  }
+  double val = cf::eval<D>(P, x, lam, vv);
+  if (fabs(val) < eps) val = eps;
+  out[v] = val;
+}
+
 int cf_grid(int nel) {
   long long blocks = ((long long)nel * 64 + 255) / 256;
   if (blocks > 65536) blocks = 65536;
@@ -157,6 +179,24 @@ extern "C" int xg_rule_scale_cf(xg_ctx *ctx, int D, int A, int nel, const int32_
     default: XG_SC(3, 3); break;
   }
 #undef XG_SC
+  XG_LAUNCH_CHECK();
+  return XG_OK;
+}
+
+extern "C" int xg_p1_interpolate_cf(xg_ctx *ctx, int D, int nv, const double *coords, const xg_cf *cf, double eps,
+                                    double *out) {
+  if (!ctx || D < 1 || D > 3 || nv < 0 || !cf) return set_error(XG_ERR_ARG, "xg_p1_interpolate_cf: bad argument");
+  cf::Prog P;
+  if (const char *msg = cf::validate(cf->ncode, cf->code, cf->nconst, cf->consts, cf->ngf, cf->gf, P))
+    return set_error(XG_ERR_ARG, std::string("xg_p1_interpolate_cf: ") + msg);
+  if (nv == 0) return XG_OK;
+  if (!coords || !out) return set_error(XG_ERR_ARG, "xg_p1_interpolate_cf: null pointer");
+  const int grid = (nv + 255) / 256;
+  switch (D) {
+    case 1: k_p1_interpolate_cf<1><<<grid, 256, 0, ctx->stream>>>(nv, coords, P, eps, out); break;
+    case 2: k_p1_interpolate_cf<2><<<grid, 256, 0, ctx->stream>>>(nv, coords, P, eps, out); break;
+    default: k_p1_interpolate_cf<3><<<grid, 256, 0, ctx->stream>>>(nv, coords, P, eps, out); break;
+  }
   XG_LAUNCH_CHECK();
   return XG_OK;
 }

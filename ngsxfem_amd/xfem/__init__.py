@@ -31,7 +31,7 @@ load_hip_runtime()  # before the host module loads libxg.so: one HIP runtime per
 from .xfem import (ANY, BND, CDOM_IF, CDOM_NEG, CDOM_POS, FALLBACK, FIRST, HASNEG, HASPOS, IF, NEG, NO,  # noqa: F401
                    OPTIMAL, POS, UNCUT, VOL, AssembleCSR, AssembleCutFormsDevice, AssembleVector, BitArray,
                    COMBINED_DOMAIN_TYPE, CutElementMatrices, CutElementMatricesCF, CutElementVectors,
-                   CutElementVectorsCF, CutInfo, IntegrateCF, DOMAIN_TYPE, GetDofsOfElements,
+                   CutElementVectorsCF, CutInfo, IntegrateCF, InterpolateCFToP1, DOMAIN_TYPE, GetDofsOfElements,
                    GetDofsOfFacets, GetElementsWithNeighborFacets, GetElementsWithSharedVertex,
                    GetFacetsWithNeighborTypes, IntegrateX, MlsetElementMatrices, MultiLevelsetCutInfo, NODE_TYPE,
                    QuadElementMatrices, SubdivElementMatrices,
@@ -375,6 +375,10 @@ def _eval_at_vertices(cf, mesh):
 def InterpolateToP1(gf_or_cf, gf_p1, eps_perturbation=None, heapsize=1000000):
     """utils/p1interpol.cpp:27-113: vertex values, |v| < eps -> +eps (GPU)"""
     eps = ngsxfemglobals.eps_P1_perturbation if eps_perturbation is None else eps_perturbation
+    if isinstance(gf_or_cf, CoefficientFunction):
+        # the coefficient's program at the vertices and the perturbation, on the GPU
+        gf_p1.vec = InterpolateCFToP1(gf_p1.space.mesh, gf_or_cf, eps)
+        return
     if isinstance(gf_or_cf, GridFunction):
         vals = np.asarray(gf_or_cf.vec, dtype=np.float64).copy()
     else:
