@@ -275,6 +275,9 @@ int xg_integrate(xg_ctx *ctx, int D, int nel, const int32_t *elems, const int32_
      19 GF1 s  push the P1 grid function gf[s] (device FP64 vertex values,
                nv of them) at the point: sum_k lam_k gf[s][v_k]
      20 IFPOS  a b c -> (a > 0 ? b : c)   21 MAX  22 MIN  (a b -> ...)
+     23 EL0 s  push gf[s][e], a piecewise constant grid function (one value
+               per element of the mesh, e the element's number; not for
+               xg_p1_interpolate_cf)
    The program must leave exactly one value. */
 typedef struct xg_cf {
   int32_t ncode;

@@ -645,8 +645,10 @@ struct DeviceCf {
 };
 
 // vmap (nullable): the mesh's vertices as vertices of the coefficient's
-// grid functions (the boundary submeshes' vertex copies)
-void make_device_cf(py::object cfobj, int nv, DeviceCf &d, const std::vector<int32_t> *vmap = nullptr) {
+// grid functions (the boundary submeshes' vertex copies); ne: the number of
+// elements a piecewise constant grid function must have (< 0: none allowed)
+void make_device_cf(py::object cfobj, int nv, DeviceCf &d, const std::vector<int32_t> *vmap = nullptr,
+                    int ne = -1) {
   if (!py::hasattr(cfobj, "program")) throw std::runtime_error("coefficient: a CoefficientFunction expected");
   py::tuple t = cfobj.attr("program")();
   d.code = t[0].cast<std::vector<int32_t>>();
@@ -654,7 +656,13 @@ void make_device_cf(py::object cfobj, int nv, DeviceCf &d, const std::vector<int
   for (auto g : t[2].cast<py::list>()) {
     auto a = py::array_t<double, py::array::c_style | py::array::forcecast>(g.attr("vec"));
     std::vector<double> vals;
-    if (vmap) {
+    const bool per_element = py::hasattr(g.attr("space"), "is_l2");
+    if (per_element) {
+      if (ne < 0 || vmap)
+        throw std::runtime_error("coefficient: piecewise constant grid functions only on the mesh's elements");
+      if ((int)a.size() != ne) throw std::runtime_error("coefficient: L2 order-0 grid functions (one value per element)");
+      vals.assign(a.data(), a.data() + a.size());
+    } else if (vmap) {
       if ((int)vmap->size() != nv) throw std::runtime_error("coefficient: vertex map size");
       vals.resize(nv);
       for (int k = 0; k < nv; ++k) {
@@ -1076,7 +1084,7 @@ py::object IntegrateX(py::dict lsetdom, py::object mesh, py::object cf, py::obje
       }
     }
     DeviceCf dcf;
-    make_device_cf(cf, ma->nv, dcf);
+    make_device_cf(cf, ma->nv, dcf, nullptr, ma->ne);
     check(xg_rule_scale_cf(ctx(), D, D, ma->ne, nullptr, ma->el2v.p, ma->coords.p, r.offsets.p, r.pts.p, &dcf.cf));
     const double one = 1.0;
     const int32_t ex0[3] = {0, 0, 0};
@@ -1208,7 +1216,7 @@ py::tuple MlsetElementMatrices(py::object mesh, py::object lsets, py::object dts
   if (!cf.is_none() && nel) {
     // a coefficient function: the rule's weights scaled by it
     DeviceCf dcf;
-    make_device_cf(cf, ma->nv, dcf);
+    make_device_cf(cf, ma->nv, dcf, nullptr, ma->ne);
     check(xg_rule_scale_cf(ctx(), D, D, nel, r.elems.p, ma->el2v.p, ma->coords.p, r.offsets.p, r.pts.p, &dcf.cf));
   }
   const bool f64 = precision == XG_PREC_FP64;
@@ -1341,7 +1349,7 @@ py::tuple CutElementMatricesCF(py::object mesh, py::object lset, int dt, int ord
   AmbientCoords amb;
   make_ambient(ma, ambient, vmap, amb);
   DeviceCf dcf;
-  make_device_cf(cf, ma->nv, dcf, amb.has_vmap ? &amb.vmap : nullptr);
+  make_device_cf(cf, ma->nv, dcf, amb.has_vmap ? &amb.vmap : nullptr, ma->ne);
   const int D = ma->dim, nel = (int)el.size();
   int nd = 1;
   for (int k = 1; k <= D; ++k) nd = nd * (p + k) / k;
@@ -1383,7 +1391,7 @@ py::tuple CutElementVectorsCF(py::object mesh, py::object lset, int dt, int orde
   AmbientCoords amb;
   make_ambient(ma, ambient, vmap, amb);
   DeviceCf dcf;
-  make_device_cf(cf, ma->nv, dcf, amb.has_vmap ? &amb.vmap : nullptr);
+  make_device_cf(cf, ma->nv, dcf, amb.has_vmap ? &amb.vmap : nullptr, ma->ne);
   const int D = ma->dim, nel = (int)el.size();
   int nd = 1;
   for (int k = 1; k <= D; ++k) nd = nd * (p + k) / k;
@@ -1416,7 +1424,7 @@ py::object IntegrateCF(py::object mesh, py::object lset, int dt, int order, py::
   AmbientCoords amb;
   make_ambient(ma, ambient, vmap, amb);
   DeviceCf dcf;
-  make_device_cf(cf, ma->nv, dcf, amb.has_vmap ? &amb.vmap : nullptr);
+  make_device_cf(cf, ma->nv, dcf, amb.has_vmap ? &amb.vmap : nullptr, ma->ne);
   const int D = ma->dim;
   std::vector<double> pe(ma->ne, 0.0);
   double total = 0.0;

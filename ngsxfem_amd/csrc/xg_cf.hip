@@ -74,7 +74,7 @@ __global__ __launch_bounds__(256) void k_rule_scale_cf(int nel, const int32_t *_
         for (int c = 0; c < D; ++c) a += (V[c][r] - V[D][r]) * xi[c];
         x[r] = a;
       }
-      p.w *= cf::eval<D>(P, x, lam, vv);
+      p.w *= cf::eval<D>(P, x, lam, vv, e);
       pts[q] = p;
     }
   }
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(256) void k_p1_interpolate_cf(int nv, const double 
     lam[k] = k == 0 ? 1.0 : 0.0;
     vv[k] = v;
   }
-  double val = cf::eval<D>(P, x, lam, vv);
+  double val = cf::eval<D>(P, x, lam, vv, -1);
   if (fabs(val) < eps) val = eps;
   out[v] = val;
 }
@@ -189,6 +189,8 @@ extern "C" int xg_p1_interpolate_cf(xg_ctx *ctx, int D, int nv, const double *co
   cf::Prog P;
   if (const char *msg = cf::validate(cf->ncode, cf->code, cf->nconst, cf->consts, cf->ngf, cf->gf, P))
     return set_error(XG_ERR_ARG, std::string("xg_p1_interpolate_cf: ") + msg);
+  if (cf::uses_elements(P))
+    return set_error(XG_ERR_ARG, "xg_p1_interpolate_cf: piecewise constant grid functions have no vertex values");
   if (nv == 0) return XG_OK;
   if (!coords || !out) return set_error(XG_ERR_ARG, "xg_p1_interpolate_cf: null pointer");
   const int grid = (nv + 255) / 256;

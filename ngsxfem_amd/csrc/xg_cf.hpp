@@ -51,7 +51,8 @@ enum Op : int32_t {
   IFPOS = 20,  // a b c -> (a > 0 ? b : c)  (NGSolve IfPos)
   MAX = 21,    // a b -> max(a, b)
   MIN = 22,    // a b -> min(a, b)
-  NOPS = 23
+  EL0 = 23,    // + grid function slot s: push u_s[e] (piecewise constant, one value per element)
+  NOPS = 24
 };
 
 constexpr int MAXCODE = 192;  // program words
@@ -69,12 +70,14 @@ struct Prog {
 };
 
 // words taken by an opcode (itself and its immediate operand)
-XG_CF_HD int op_words(int32_t op) { return (op == CONST || op == COORD || op == POWI || op == GF1) ? 2 : 1; }
+XG_CF_HD int op_words(int32_t op) {
+  return (op == CONST || op == COORD || op == POWI || op == GF1 || op == EL0) ? 2 : 1;
+}
 
 // stack effect of an opcode
 XG_CF_HD int op_delta(int32_t op) {
   switch (op) {
-    case CONST: case COORD: case GF1: return 1;
+    case CONST: case COORD: case GF1: case EL0: return 1;
     case ADD: case SUB: case MUL: case DIV: case POW: case MAX: case MIN: return -1;
     case IFPOS: return -2;
     default: return 0;
@@ -98,7 +101,8 @@ inline const char *validate(int32_t ncode, const int32_t *code, int32_t nconst, 
     const int32_t a = w == 2 ? code[pc + 1] : 0;
     if (op == CONST && (a < 0 || a >= nconst)) return "coefficient program: constant index out of range";
     if (op == COORD && (a < 0 || a > 2)) return "coefficient program: coordinate out of range";
-    if (op == GF1 && (a < 0 || a >= ngf || !gf[a])) return "coefficient program: grid function slot out of range";
+    if ((op == GF1 || op == EL0) && (a < 0 || a >= ngf || !gf[a]))
+      return "coefficient program: grid function slot out of range";
     const int need = (op == IFPOS) ? 3 : (op_delta(op) == -1 ? 2 : (op_delta(op) == 0 ? 1 : 0));
     if (depth < need) return "coefficient program: stack underflow";
     depth += op_delta(op);
@@ -114,10 +118,17 @@ inline const char *validate(int32_t ncode, const int32_t *code, int32_t nconst, 
   return nullptr;
 }
 
-// value of the program at the physical point x of an element with vertices
+// true when the program reads piecewise constant grid functions (EL0)
+inline bool uses_elements(const Prog &p) {
+  for (int pc = 0; pc < p.n; pc += op_words(p.code[pc]))
+    if (p.code[pc] == EL0) return true;
+  return false;
+}
+
+// value of the program at the physical point x of element e with vertices
 // vv (D + 1 of them) and barycentrics lam (lam_k of vertex k)
 template <int D>
-XG_CF_FN double eval(const Prog &P, const double *x, const double *lam, const int32_t *vv) {
+XG_CF_FN double eval(const Prog &P, const double *x, const double *lam, const int32_t *vv, int e) {
   double st[MAXSTACK];
   int sp = 0;
   for (int pc = 0; pc < P.n;) {
@@ -134,6 +145,7 @@ XG_CF_FN double eval(const Prog &P, const double *x, const double *lam, const in
         st[sp++] = s;
         break;
       }
+      case EL0: st[sp++] = P.gf[a][e]; break;
       case ADD: --sp; st[sp - 1] = st[sp - 1] + st[sp]; break;
       case SUB: --sp; st[sp - 1] = st[sp - 1] - st[sp]; break;
       case MUL: --sp; st[sp - 1] = st[sp - 1] * st[sp]; break;

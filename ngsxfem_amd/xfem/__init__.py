@@ -101,8 +101,10 @@ class CoefficientFunction:
         if isinstance(o, CoefficientFunction):
             return o
         if isinstance(o, GridFunction):
+            if isinstance(o.space, L2):
+                return CoefficientFunction(expr=("el", o))
             if o.order != 1:
-                raise NotImplementedError("grid functions in coefficients: P1 only")
+                raise NotImplementedError("grid functions in coefficients: P1 and L2 order 0")
             return CoefficientFunction(expr=("gf", o))
         return CoefficientFunction(float(o))
 
@@ -200,7 +202,7 @@ class CoefficientFunction:
 # postfix opcodes of the device coefficient programs (include/xg_abi.h)
 _OP = {"const": 1, "coord": 2, "+": 3, "-": 4, "*": 5, "/": 6, "neg": 7, "powi": 8, "pow": 9, "sqrt": 10,
        "sin": 11, "cos": 12, "tan": 13, "exp": 14, "log": 15, "atan": 16, "tanh": 17, "abs": 18, "gf": 19,
-       "ifpos": 20, "max": 21, "min": 22}
+       "ifpos": 20, "max": 21, "min": 22, "el": 23}
 _NP = {"sqrt": np.sqrt, "sin": np.sin, "cos": np.cos, "tan": np.tan, "exp": np.exp, "log": np.log,
        "atan": np.arctan, "tanh": np.tanh, "abs": np.abs}
 
@@ -223,7 +225,7 @@ def _eval_node(n, X):
         return _eval_node(n[1], X) ** float(n[2])
     if k == "ifpos":
         return np.where(_eval_node(n[1], X) > 0, _eval_node(n[2], X), _eval_node(n[3], X))
-    if k == "gf":
+    if k in ("gf", "el"):
         raise NotImplementedError("grid functions are evaluated per element on the GPU (no host point location)")
     raise ValueError(k)
 
@@ -262,10 +264,10 @@ def _compile_node(n, code, consts, gfs):
         for c in n[1:]:
             _compile_node(c, code, consts, gfs)
         code.append(_OP["ifpos"])
-    elif k == "gf":
+    elif k in ("gf", "el"):
         if n[1] not in gfs:
             gfs.append(n[1])
-        code.extend([_OP["gf"], gfs.index(n[1])])
+        code.extend([_OP[k], gfs.index(n[1])])
     else:
         raise ValueError(k)
 
@@ -338,15 +340,69 @@ class H1:
         return self.TrialFunction(), self.TestFunction()
 
 
+class L2:
+    """Piecewise constant space (order 0): one value per element, in the
+    element numbering.  Its grid functions enter coefficient functions
+    (CutRatioGF, IndicatorCF)."""
+    is_l2 = True
+
+    def __init__(self, mesh, order=0, **kwargs):
+        if order != 0:
+            raise NotImplementedError("L2: order 0")
+        self.mesh = mesh
+        self.order = 0
+
+    @property
+    def ndof(self):
+        return self.mesh.ne
+
+
 class GridFunction:
     def __init__(self, space, name="gf"):
         self.space = space
         self.order = space.order
         self.vec = np.zeros(space.ndof)
 
+    # a grid function is a coefficient function (P1: vertex values times
+    # barycentrics, L2 order 0: the element's value)
+    def _cf(self):
+        return CoefficientFunction._lift(self)
+
+    def __add__(self, o):
+        return self._cf().__add__(o)
+
+    def __radd__(self, o):
+        return self._cf().__radd__(o)
+
+    def __sub__(self, o):
+        return self._cf().__sub__(o)
+
+    def __rsub__(self, o):
+        return self._cf().__rsub__(o)
+
+    def __mul__(self, o):
+        return self._cf().__mul__(o)
+
+    def __rmul__(self, o):
+        return self._cf().__rmul__(o)
+
+    def __truediv__(self, o):
+        return self._cf().__truediv__(o)
+
+    def __rtruediv__(self, o):
+        return self._cf().__rtruediv__(o)
+
+    def __neg__(self):
+        return self._cf().__neg__()
+
+    def __pow__(self, k):
+        return self._cf().__pow__(k)
+
     def Set(self, cf):
         """nodal interpolation: at the vertices (P1), at the vertices and edge
         midpoints (P2, this build's dof numbering)"""
+        if isinstance(self.space, L2):
+            raise NotImplementedError("GridFunction(L2).Set: assign .vec")
         if self.order == 1:
             self.vec = _eval_at_vertices(cf, self.space.mesh)
         elif self.order == 2:
@@ -384,6 +440,51 @@ def InterpolateToP1(gf_or_cf, gf_p1, eps_perturbation=None, heapsize=1000000):
     else:
         vals = _eval_at_vertices(gf_or_cf, gf_p1.space.mesh)
     gf_p1.vec = PerturbP1(vals, eps)
+
+
+def IndicatorCF(mesh, ba, facets=False):
+    """python/__init__.py:441-453: 1 on the elements where the BitArray is
+    set, 0 elsewhere (a piecewise constant grid function's coefficient)"""
+    if facets:
+        raise NotImplementedError("IndicatorCF on facets")
+    g = GridFunction(L2(mesh, order=0))
+    g.vec = np.asarray(ba.numpy() if isinstance(ba, BitArray) else ba, dtype=np.float64).copy()
+    return CoefficientFunction._lift(g)
+
+
+def BitArrayCF(ba, mesh=None):
+    """NGSolve's BitArrayCF (elements only): the mesh comes from the caller"""
+    if mesh is None:
+        raise NotImplementedError("BitArrayCF: pass mesh= (no global mesh in this build)")
+    return IndicatorCF(mesh, ba)
+
+
+def CutRatioGF(cutinfo):
+    """python/__init__.py:455-462: the cut ratios NEG / (NEG + POS) as a
+    piecewise constant grid function"""
+    g = GridFunction(L2(cutinfo.Mesh(), order=0))
+    g.vec = np.asarray(cutinfo.GetCutRatios(VOL), dtype=np.float64).copy()
+    return g
+
+
+def kappa(mesh, lset_approx, subdivlvl=0):
+    """python/__init__.py:464-478 (deprecated there): the NEG and POS volume
+    fractions of every element as piecewise constant grid functions"""
+    print("kappa-function is deprecated - use CutRatioGF instead")
+    k1 = CutRatioGF(CutInfo(mesh, lset_approx, subdivlvl=subdivlvl))
+    k2 = GridFunction(L2(mesh, order=0))
+    k2.vec = 1.0 - k1.vec
+    return k1, k2
+
+
+def IsCut(mesh, lset_approx, subdivlvl=0):
+    """python/__init__.py:480-510 (deprecated there): 1 on elements with a
+    volume fraction strictly between 1e-16 and 1 - 1e-16"""
+    print("IsCut-function is deprecated - use CutInfo-functionality instead")
+    k1, _ = kappa(mesh, lset_approx, subdivlvl)
+    g = GridFunction(L2(mesh, order=0))
+    g.vec = ((k1.vec > 1e-16) & (k1.vec < 1.0 - 1e-16)).astype(np.float64)
+    return g
 
 
 # ------------------------------------------------------------------ Integrate
@@ -585,7 +686,7 @@ class _CutDifferentialSymbol:
     def __rmul__(self, o):
         if isinstance(o, (_Form, _LinForm)):
             return o.__mul__(self)
-        if isinstance(o, (CoefficientFunction, int, float)):
+        if isinstance(o, (CoefficientFunction, GridFunction, int, float)):
             return _SumOfIntegrals([_CutIntegral(CoefficientFunction._lift(o), self)])
         return NotImplemented
 

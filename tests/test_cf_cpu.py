@@ -32,7 +32,7 @@ extern "C" int cf_load(int ncode, const int *code, int nconst, const double *con
 // values at n points x (n x 3), barycentrics lam (n x 4) of a tetrahedron
 // with vertex numbers vv
 extern "C" void cf_eval(int n, const double *x, const double *lam, const int *vv, double *out) {
-  for (int i = 0; i < n; ++i) out[i] = eval<3>(prog, x + 3 * i, lam + 4 * i, vv);
+  for (int i = 0; i < n; ++i) out[i] = eval<3>(prog, x + 3 * i, lam + 4 * i, vv, 5);
 }
 """
 

@@ -204,3 +204,36 @@ def test_interpolate_to_p1_on_gpu(X):
     X3 = m3.coords
     ref3 = np.exp(X3[:, 1]) * np.cos(X3[:, 0]) - 1.2 + (0.5 * X3[:, 0] - X3[:, 2])
     np.testing.assert_allclose(l3.vec, np.where(np.abs(ref3) < 1e-14, 1e-14, ref3), rtol=1e-14, atol=1e-15)
+
+
+def test_piecewise_constant_coefficients(X):
+    """CutRatioGF / IndicatorCF (python/__init__.py:441-462) as coefficients:
+    the integral of the cut ratio over each element's NEG part is the ratio
+    times that part's volume; the indicator of the cut elements keeps their
+    mass matrices and zeroes the others; kappa and IsCut agree with the
+    cut ratios and the IF elements; vertex interpolation rejects them"""
+    mesh = X.MakeStructured3DMesh(nx=5, ny=5, nz=5)
+    lset = X.GridFunction(X.H1(mesh, order=1))
+    X.InterpolateToP1(X.sqrt((X.x - 0.5) ** 2 + (X.y - 0.45) ** 2 + (X.z - 0.5) ** 2) - 0.33, lset)
+    ci = X.CutInfo(mesh, lset)
+    kap = X.CutRatioGF(ci)
+    lsd = {"levelset": lset, "domain_type": X.NEG}
+    vol = X.Integrate(lsd, cf=1, mesh=mesh, order=2, element_wise=True)
+    got = X.Integrate(lsd, cf=kap * 1.0 + 0.0 * X.x, mesh=mesh, order=2, element_wise=True)
+    np.testing.assert_allclose(got, kap.vec * vol, rtol=1e-13, atol=1e-16)
+    V = X.H1(mesh, order=1)
+    u, v = V.TnT()
+    ifel = ci.GetElementsOfType(X.IF)
+    ind = X.IndicatorCF(mesh, ifel)
+    e1, m1 = ((ind * u * v) * X.dCut(lset, X.NEG)).parts[0].element_matrices(mesh, 1, 0)
+    e0, m0 = ((u * v) * X.dCut(lset, X.NEG)).parts[0].element_matrices(mesh, 1, 0)
+    assert np.array_equal(e1, e0)
+    isif = np.asarray(ifel.numpy(), dtype=bool)[e1]
+    assert np.abs(m1[~isif]).max() == 0.0
+    np.testing.assert_allclose(m1[isif], m0[isif], rtol=1e-13, atol=1e-18)
+    k1, k2 = X.kappa(mesh, lset)
+    assert np.allclose(k1.vec + k2.vec, 1.0) and np.array_equal(k1.vec, kap.vec)
+    assert int(X.IsCut(mesh, lset).vec.sum()) == ifel.NumSet()
+    g = X.GridFunction(X.H1(mesh, order=1))
+    with pytest.raises(RuntimeError):
+        X.InterpolateToP1(kap * X.x, g)
