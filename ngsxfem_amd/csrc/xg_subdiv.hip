@@ -216,6 +216,39 @@ __device__ __forceinline__ void accum_tet(double *m, const double *p0, const dou
   Moments<3, PM>::accum(lam, rr[3] * tf, m);
 }
 
+constexpr double sd_inv_factorial(int n) { return n <= 1 ? 1.0 : sd_inv_factorial(n - 1) / n; }
+
+// exact moments of one tet from its vertices (p <= 2: Moments::hstep, the
+// complete homogeneous recursion over the 4 vertices' barycentrics) added
+// with the signed weight factor tf = +-6|T| (negative: a complemented
+// leaf's piece is subtracted, by negating the sums around the update); the
+// barycentrics are scaled by (|tf| / (2p + 3)!)^(1/2p) so the recursion's
+// top degree lands in m weighted, and the alpha! factors are applied once
+// per element (Moments::hfinish).  4 x 140 FMAs for a degree-4 tet against
+// 14 points x 73 operations with the base rule; the result is that rule's
+// point sum to rounding.
+template <int PM>
+__device__ __forceinline__ void htet(double *m, const double *p0, const double *p1, const double *p2,
+                                     const double *p3, double tf) {
+  using MM = Moments<3, PM>;
+  double h[MM::NH];
+#pragma unroll
+  for (int k = 0; k < MM::NH; ++k) h[k] = 0.0;
+  double c = sqrt(fabs(tf) * sd_inv_factorial(2 * PM + 3));
+  if constexpr (PM == 2) c = sqrt(c);
+  const double sg = tf < 0.0 ? -1.0 : 1.0;
+#pragma unroll
+  for (int k = 0; k < MM::NM; ++k) m[k] *= sg;
+  const double *P[4] = {p0, p1, p2, p3};
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const double l[4] = {c * P[v][0], c * P[v][1], c * P[v][2], c * (1.0 - P[v][0] - P[v][1] - P[v][2])};
+    MM::hstep(l, h, m);
+  }
+#pragma unroll
+  for (int k = 0; k < MM::NM; ++k) m[k] *= sg;
+}
+
 __device__ __forceinline__ double tet_tf(const double *p0, const double *p1, const double *p2, const double *p3,
                                          bool measure6) {
   double a[3], b[3], c[3];
@@ -2179,7 +2212,22 @@ __global__ __launch_bounds__(64, 2) void k_sdx_leaves(int nel, const int32_t *__
           }
         }
       }
-      if (WHICH == 1) {
+      if constexpr (WHICH == 1 && PM <= 2) {
+        // p <= 2: one task per piece, its moments from its vertices
+        for (int q = lane; q < tot_vol; q += 64) {
+          const uint32_t rec = S.volrec[q];
+          const int ll = rec & 63;
+          double p0[3], p1[3], p2[3], p3[3];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            p0[d] = pool[(((rec >> 8) & 7) * 3 + d) * 64 + ll];
+            p1[d] = pool[(((rec >> 11) & 7) * 3 + d) * 64 + ll];
+            p2[d] = pool[(((rec >> 14) & 7) * 3 + d) * 64 + ll];
+            p3[d] = pool[(((rec >> 17) & 7) * 3 + d) * 64 + ll];
+          }
+          htet<PM>(mvol, p0, p1, p2, p3, S.voltf[q]);
+        }
+      } else if (WHICH == 1) {
         for (int q = lane; q < tot_vol * SDX_NVB; q += 64) {
           const int pc = q / SDX_NVB, t0 = (q - pc * SDX_NVB) * SDX_VB;
           const uint32_t rec = S.volrec[pc];
@@ -2244,6 +2292,7 @@ __global__ __launch_bounds__(64, 2) void k_sdx_leaves(int nel, const int32_t *__
     }
     // wave reduction of the per-lane moments through LDS (the point pool is
     // free now)
+    if constexpr (WHICH == 1 && PM <= 2) Moments<3, PM>::hfinish(mvol);
     if constexpr (WHICH == 3) wave_sum_lds<NM>(mvol, pool, S.red, lane, S.red + NM, nn);
     else if constexpr (WHICH == 1) wave_sum_lds<NM>(mvol, pool, S.red, lane);
     else wave_sum_lds<NM>(mif, pool, S.red + NM, lane);
