@@ -300,6 +300,19 @@ typedef struct xg_cf {
 int xg_rule_scale_cf(xg_ctx *ctx, int D, int A, int nel, const int32_t *elems, const int32_t *el2v,
                      const double *coords, const int32_t *offsets, double *pts, const xg_cf *cf);
 
+/* Element-boundary rules (SymbolicCutBilinearFormIntegrator::
+   T_CalcElementMatrixEBAdd, xfem/symboliccutbfi.cpp:488-640): given the cut
+   rules of the facets of nel elements (D = 2, 3), computed as rules of a
+   (D-1)-dimensional mesh whose element i (D + 1) + k is facet k of element i
+   (its vertices without vertex k, ascending; foffsets: nel (D + 1) + 1
+   entries, weights in that mesh's reference measure), maps the points in
+   place into the element's reference coordinates and rescales the weights to
+   the element's measure (w |det F_facet| / |det F_element|); eoffsets
+   (nel + 1) = foffsets[i (D + 1)].  xg_elmat / xg_elvec / xg_integrate then
+   give the element-boundary integrals with the volume basis. */
+int xg_facet_rule_to_element(xg_ctx *ctx, int D, int nel, const int32_t *elems, const int32_t *el2v,
+                             const double *coords, const int32_t *foffsets, double *pts, int32_t *eoffsets);
+
 /* ---- linear forms on a materialised rule: SymbolicCutLinearFormIntegrator::
    T_CalcElementVector (xfem/symboliccutlfi.cpp:57-140) with f = 1 in the
    weights: out[i][a] = |det F| sum_q w_q phi_a(xi_q), FP64 nel x ndof. */

@@ -74,6 +74,7 @@ _SIGS = {
     "xg_elmat": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "xg_rule_scale_cf": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp],
     "xg_elvec": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp],
+    "xg_facet_rule_to_element": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp],
     "xg_p1_interpolate_cf": [_vp, _i, _i, _vp, _vp, ctypes.c_double, _vp],
     "xg_cut_pipeline": [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _i, _vp],
     "xg_subdiv_weights": [_vp, _i, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp],

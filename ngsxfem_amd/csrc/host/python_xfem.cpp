@@ -1444,6 +1444,60 @@ py::object IntegrateCF(py::object mesh, py::object lset, int dt, int order, py::
   return py::float_(total);
 }
 
+// element-boundary element matrices (SymbolicCutBFI with element_boundary,
+// symboliccutbfi.cpp:488-640): the cut rules of the listed elements' facets
+// on the facet submesh fmesh (facet k of element i = its element i (D + 1) +
+// k, level set values flset per vertex copy), mapped into the elements
+// (xg_facet_rule_to_element), times cf when given, then xg_elmat with the
+// volume basis
+py::tuple ElementBoundaryMatrices(py::object mesh, py::object fmesh, py::object flset, int dt, int order, int p,
+                                  int mask, py::array_t<int32_t, py::array::c_style | py::array::forcecast> elements,
+                                  int precision, py::object cf) {
+  MA ma = as_mesh(mesh);
+  simplices_only(ma, "ElementBoundaryMatrices");
+  MA fm = as_mesh(fmesh);
+  const int D = ma->dim, nel = (int)elements.size();
+  if (D < 2 || fm->dim != D - 1 || fm->ne != nel * (D + 1))
+    throw std::runtime_error("ElementBoundaryMatrices: D + 1 facets per element, of dimension D - 1");
+  bool is_p1 = true;
+  std::vector<double> v = p1_values(flset, fm->nv, is_p1);
+  DevBuf<double> d_l;
+  d_l.upload(v.data(), v.size());
+  DeviceRule fr;
+  build_rule(fm, d_l.p, dt, order, nullptr, fr);
+  std::vector<int32_t> el(elements.data(), elements.data() + nel);
+  for (int32_t e : el)
+    if (e < 0 || e >= ma->ne) throw std::runtime_error("ElementBoundaryMatrices: element out of range");
+  DevBuf<int32_t> d_el, eoff((size_t)nel + 1);
+  d_el.upload(el.data(), el.size());
+  check(xg_facet_rule_to_element(ctx(), D, nel, d_el.p, ma->el2v.p, ma->coords.p, fr.offsets.p, fr.pts.p, eoff.p));
+  if (!cf.is_none() && nel) {
+    DeviceCf dcf;
+    make_device_cf(cf, ma->nv, dcf, nullptr, ma->ne);
+    check(xg_rule_scale_cf(ctx(), D, D, nel, d_el.p, ma->el2v.p, ma->coords.p, eoff.p, fr.pts.p, &dcf.cf));
+  }
+  int nd = 1;
+  for (int k = 1; k <= D; ++k) nd = nd * (p + k) / k;
+  py::array_t<int32_t> elarr((py::ssize_t)nel, el.data());
+  const bool f64 = precision == XG_PREC_FP64;
+  DevBuf<double> o64;
+  DevBuf<float> o32;
+  if (f64) o64.resize((size_t)std::max(nel, 1) * nd * nd);
+  else o32.resize((size_t)std::max(nel, 1) * nd * nd);
+  if (nel)
+    check(xg_elmat(ctx(), D, p, nel, d_el.p, ma->el2v.p, ma->coords.p, eoff.p, fr.pts.p, mask, precision,
+                   f64 ? (void *)o64.p : (void *)o32.p));
+  check(xg_sync(ctx()));
+  if (f64) {
+    py::array_t<double> h({(py::ssize_t)nel, (py::ssize_t)nd, (py::ssize_t)nd});
+    o64.download(h.mutable_data(), (size_t)nel * nd * nd);
+    return py::make_tuple(elarr, h);
+  }
+  py::array_t<float> h({(py::ssize_t)nel, (py::ssize_t)nd, (py::ssize_t)nd});
+  o32.download(h.mutable_data(), (size_t)nel * nd * nd);
+  return py::make_tuple(elarr, h);
+}
+
 // device scatter of element matrices into a CSR pattern (the assembly loop of
 // NGSolve's BilinearForm::Assemble); returns the FP64 values array
 py::array_t<double> AssembleCSR(py::array_t<int32_t, py::array::c_style | py::array::forcecast> dofs,
@@ -1870,6 +1924,10 @@ PYBIND11_MODULE(xfem, m) {
         py::arg("domain_type"), py::arg("order"), py::arg("p"), py::arg("mask"), py::arg("elements"),
         py::arg("precision"), py::arg("cf"), py::arg("ambient") = py::none(), py::arg("vmap") = py::none(),
         "element matrices of cf u v (mask 1) / cf grad u . grad v (mask 2), cf evaluated on the GPU");
+  m.def("ElementBoundaryMatrices", &ElementBoundaryMatrices, py::arg("mesh"), py::arg("facet_mesh"),
+        py::arg("facet_levelset"), py::arg("domain_type"), py::arg("order"), py::arg("p"), py::arg("mask"),
+        py::arg("elements"), py::arg("precision"), py::arg("cf") = py::none(),
+        "element matrices on the cut rules of the elements' boundaries (facets), volume basis");
   m.def("IntegrateCF", &IntegrateCF, py::arg("mesh"), py::arg("levelset"), py::arg("domain_type"), py::arg("order"),
         py::arg("cf"), py::arg("element_wise") = false, py::arg("ambient") = py::none(), py::arg("vmap") = py::none(),
         "integral of cf on the straight cut rule, cf evaluated on the GPU");

@@ -150,6 +150,86 @@ __global__ __launch_bounds__(256) void k_p1_interpolate_cf(int nv, const double 
   out[v] = val;
 }
 
+// Element-boundary rules (SymbolicCutBFI with element_boundary,
+// symboliccutbfi.cpp:488-640): the cut rules of the D + 1 facets of every
+// listed element, computed on a submesh of the facets (facet k of element i
+// is submesh element i (D + 1) + k: the element's vertices without vertex k,
+// in ascending order), mapped in place into the element's reference
+// coordinates (facet barycentrics mu_j -> element barycentrics
+// lam_{loc_j}), the weights rescaled from the facet's measure to the
+// element's (w |det F_facet| / |det F_element|, the element kernels
+// multiply by |det F_element|).  One wave per element, lanes over points;
+// eoff[i] = foff[i (D + 1)].
+template <int D>
+__global__ __launch_bounds__(256) void k_facet_rule_to_element(int nel, const int32_t *__restrict__ elems,
+                                                               const int32_t *__restrict__ el2v,
+                                                               const double *__restrict__ coords,
+                                                               const int32_t *__restrict__ foff, double4 *pts,
+                                                               int32_t *__restrict__ eoff) {
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int nwaves = (gridDim.x * blockDim.x) >> 6;
+  if (wid == 0 && lane == 0) eoff[nel] = foff[nel * (D + 1)];
+  for (int i = wid; i < nel; i += nwaves) {
+    const int e = elems ? elems[i] : i;
+    if (lane == 0) eoff[i] = foff[i * (D + 1)];
+    int32_t vv[D + 1];
+    double V[D + 1][3];
+    element_geometry<D>(el2v, coords, e, vv, V);
+    double F[D * D], Finv[D * D], absdet;
+    jacobian<D>(V, F, Finv, absdet);
+#pragma unroll
+    for (int k = 0; k <= D; ++k) {
+      // facet k: the element's vertices without k, ascending
+      int loc[D];
+      int c = 0;
+#pragma unroll
+      for (int j = 0; j <= D; ++j)
+        if (j != k) loc[c++] = j;
+      // |det F_facet|: length (D = 2) or twice the area (D = 3) of the facet
+      double fdet = 1.0;
+      if constexpr (D == 2) {
+        const double a0 = V[loc[1]][0] - V[loc[0]][0], a1 = V[loc[1]][1] - V[loc[0]][1];
+        fdet = sqrt(a0 * a0 + a1 * a1);
+      } else if constexpr (D == 3) {
+        double a[3], b[3], cr[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          a[d] = V[loc[1]][d] - V[loc[0]][d];
+          b[d] = V[loc[2]][d] - V[loc[0]][d];
+        }
+        cr[0] = a[1] * b[2] - a[2] * b[1];
+        cr[1] = a[2] * b[0] - a[0] * b[2];
+        cr[2] = a[0] * b[1] - a[1] * b[0];
+        fdet = sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+      }
+      const double scale = fdet / absdet;
+      const int q0 = foff[i * (D + 1) + k], q1 = foff[i * (D + 1) + k + 1];
+      for (int q = q0 + lane; q < q1; q += 64) {
+        double4 p = pts[q];
+        const double xi[3] = {p.x, p.y, p.z};
+        double mu[D], s = 1.0;
+#pragma unroll
+        for (int j = 0; j < D - 1; ++j) {
+          mu[j] = xi[j];
+          s -= xi[j];
+        }
+        mu[D - 1] = s;
+        double lam[D + 1];
+#pragma unroll
+        for (int j = 0; j <= D; ++j) lam[j] = 0.0;
+#pragma unroll
+        for (int j = 0; j < D; ++j) lam[loc[j]] = mu[j];
+        p.x = lam[0];
+        p.y = D > 1 ? lam[1] : 0.0;
+        p.z = D > 2 ? lam[2] : 0.0;
+        p.w *= scale;
+        pts[q] = p;
+      }
+    }
+  }
+}
+
 int cf_grid(int nel) {
   long long blocks = ((long long)nel * 64 + 255) / 256;
   if (blocks > 65536) blocks = 65536;
@@ -199,6 +279,26 @@ extern "C" int xg_p1_interpolate_cf(xg_ctx *ctx, int D, int nv, const double *co
     case 2: k_p1_interpolate_cf<2><<<grid, 256, 0, ctx->stream>>>(nv, coords, P, eps, out); break;
     default: k_p1_interpolate_cf<3><<<grid, 256, 0, ctx->stream>>>(nv, coords, P, eps, out); break;
   }
+  XG_LAUNCH_CHECK();
+  return XG_OK;
+}
+
+extern "C" int xg_facet_rule_to_element(xg_ctx *ctx, int D, int nel, const int32_t *elems, const int32_t *el2v,
+                                        const double *coords, const int32_t *foffsets, double *pts,
+                                        int32_t *eoffsets) {
+  if (!ctx || D < 2 || D > 3 || nel < 0 || !foffsets || !eoffsets)
+    return set_error(XG_ERR_ARG, "xg_facet_rule_to_element: bad argument");
+  if (nel == 0) {
+    XG_CHECK(hipMemcpyAsync(eoffsets, foffsets, sizeof(int32_t), hipMemcpyDeviceToDevice, ctx->stream));
+    return XG_OK;
+  }
+  if (!el2v || !coords || !pts) return set_error(XG_ERR_ARG, "xg_facet_rule_to_element: null pointer");
+  const int grid = cf_grid(nel);
+  double4 *q = reinterpret_cast<double4 *>(pts);
+  if (D == 2)
+    k_facet_rule_to_element<2><<<grid, 256, 0, ctx->stream>>>(nel, elems, el2v, coords, foffsets, q, eoffsets);
+  else
+    k_facet_rule_to_element<3><<<grid, 256, 0, ctx->stream>>>(nel, elems, el2v, coords, foffsets, q, eoffsets);
   XG_LAUNCH_CHECK();
   return XG_OK;
 }

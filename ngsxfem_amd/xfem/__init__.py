@@ -31,7 +31,7 @@ load_hip_runtime()  # before the host module loads libxg.so: one HIP runtime per
 from .xfem import (ANY, BND, CDOM_IF, CDOM_NEG, CDOM_POS, FALLBACK, FIRST, HASNEG, HASPOS, IF, NEG, NO,  # noqa: F401
                    OPTIMAL, POS, UNCUT, VOL, AssembleCSR, AssembleCutFormsDevice, AssembleVector, BitArray,
                    COMBINED_DOMAIN_TYPE, CutElementMatrices, CutElementMatricesCF, CutElementVectors,
-                   CutElementVectorsCF, CutInfo, IntegrateCF, InterpolateCFToP1, DOMAIN_TYPE, GetDofsOfElements,
+                   CutElementVectorsCF, CutInfo, ElementBoundaryMatrices, IntegrateCF, InterpolateCFToP1, DOMAIN_TYPE, GetDofsOfElements,
                    GetDofsOfFacets, GetElementsWithNeighborFacets, GetElementsWithSharedVertex,
                    GetFacetsWithNeighborTypes, IntegrateX, MlsetElementMatrices, MultiLevelsetCutInfo, NODE_TYPE,
                    QuadElementMatrices, SubdivElementMatrices,
@@ -523,7 +523,15 @@ def _bnd_submesh(mesh):
     D = mesh.dim
     if D not in (2, 3) or getattr(mesh, "is_quad", False):
         raise NotImplementedError("boundary integrals: triangle and tetrahedral meshes")
-    b = np.ascontiguousarray(mesh.bnd2v, dtype=np.int32)
+    mesh._bnd_sub = _isometric_submesh(mesh, np.ascontiguousarray(mesh.bnd2v, dtype=np.int32))
+    return mesh._bnd_sub
+
+
+def _isometric_submesh(mesh, b):
+    """simplices of dimension D - 1 given by their vertices b (nb x D) as a
+    mesh of their own: vertex copies in an isometric frame per simplex;
+    returns (submesh, vmap, ambient coordinates)"""
+    D = mesh.dim
     nb = b.shape[0]
     X = mesh.coords[b]  # nb x D x D
     e1 = X[:, 1] - X[:, 0]
@@ -544,8 +552,16 @@ def _bnd_submesh(mesh):
     sub = Mesh(loc.reshape(nb * D, D - 1), np.arange(nb * D, dtype=np.int32).reshape(nb, D))
     vmap = b.reshape(-1).copy()
     amb = np.ascontiguousarray(mesh.coords[vmap])
-    mesh._bnd_sub = (sub, vmap, amb)
-    return mesh._bnd_sub
+    return sub, vmap, amb
+
+
+def _facet_submesh(mesh, els):
+    """the D + 1 facets of the elements els (facet k of element i: its
+    vertices without vertex k, ascending; submesh element i (D + 1) + k)"""
+    D = mesh.dim
+    E = mesh.el2v[els]
+    fv = np.stack([E[:, [j for j in range(D + 1) if j != k]] for k in range(D + 1)], axis=1)
+    return _isometric_submesh(mesh, np.ascontiguousarray(fv.reshape(-1, D), dtype=np.int32))
 
 
 def _bnd_lset(lset, vmap):
@@ -676,12 +692,15 @@ class _CutDifferentialSymbol:
 
     def __init__(self, lsetdom, definedon=None, vb=VOL, element_boundary=False, element_vb=VOL, skeleton=False,
                  deformation=None, definedonelements=None):
-        if deformation is not None or skeleton or element_boundary or vb not in (VOL, BND):
-            raise NotImplementedError("dCut: element (VOL) and boundary (BND) integrals without deformation, "
-                                      "element_boundary or skeleton in this build")
+        if element_vb == BND:
+            element_boundary = True
+        if deformation is not None or skeleton or vb not in (VOL, BND) or (element_boundary and vb != VOL):
+            raise NotImplementedError("dCut: element (VOL), boundary (BND) and element-boundary integrals without "
+                                      "deformation or skeleton in this build")
         self.lsetdom = lsetdom
         self.definedonelements = definedonelements
         self.vb = vb
+        self.element_boundary = bool(element_boundary)
 
     def __rmul__(self, o):
         if isinstance(o, (_Form, _LinForm)):
@@ -727,6 +746,36 @@ class _CutIntegral:
     def is_bnd(self):
         return isinstance(self.measure, _CutDifferentialSymbol) and self.measure.vb == BND
 
+    def is_eb(self):
+        return isinstance(self.measure, _CutDifferentialSymbol) and self.measure.element_boundary
+
+    def _element_matrices_eb(self, mesh, p, precision):
+        """forms on the element boundaries (dCut(..., element_boundary=True),
+        symboliccutbfi.cpp:488-640): the cut rules of each element's facets
+        with the volume basis; rule order the sum of the trial and test
+        orders (2p) unless dCut sets it"""
+        lset, dt, order, els = self._rule_args(mesh, p)
+        if isinstance(lset, (tuple, list)) or getattr(lset, "order", 1) != 1 or \
+                self.measure.lsetdom.get("subdivlvl", 0) or getattr(mesh, "is_quad", False):
+            raise NotImplementedError("element-boundary integrals: one P1 level set on simplices")
+        if els is None:
+            el = np.arange(mesh.ne, dtype=np.int32)
+        elif isinstance(els, BitArray):
+            el = np.nonzero(np.asarray(els.numpy(), dtype=bool))[0].astype(np.int32)
+        else:
+            el = np.asarray(els, dtype=np.int32)
+        fm, vmap, _ = _facet_submesh(mesh, el)
+        fl = _bnd_lset(lset, vmap)
+        order = int(self.measure.lsetdom.get("order", 2 * p))
+        e, tot = None, None
+        for part, c in self.integrand.parts.items():
+            k = _as_const(c)
+            e, m = ElementBoundaryMatrices(mesh, fm, fl, int(dt), order, p, 1 if part == "mass" else 2, el,
+                                           precision, None if k is not None else CoefficientFunction._lift(c))
+            m = k * m if k is not None else m
+            tot = m if tot is None else tot + m
+        return e, tot
+
     def dof_table(self, space):
         """global dofs of the elements this integral runs over"""
         return space.mesh.bnd_dofs(space.order)[0] if self.is_bnd() else space.dofs()[0]
@@ -768,6 +817,8 @@ class _CutIntegral:
     def element_matrices(self, mesh, p, precision):
         if self.is_bnd():
             return self._element_matrices_bnd(mesh, p, precision)
+        if self.is_eb():
+            return self._element_matrices_eb(mesh, p, precision)
         lset, dt, order, els = self._rule_args(mesh, p)
         mask = (1 if "mass" in self.integrand.parts else 0) | (2 if "laplace" in self.integrand.parts else 0)
         if any(_as_const(c) is None for c in self.integrand.parts.values()):
@@ -836,6 +887,9 @@ class _CutIntegral:
         return e, tot
 
     def element_vectors(self, mesh, p):
+        if self.is_eb():
+            raise NotImplementedError("element-boundary linear forms (the reference's linear form integrator "
+                                      "has no element-boundary variant)")
         f = self.integrand.f
         if self.is_bnd():
             sub, vmap, amb, lsub, dt, els = self._bnd_args(mesh, p)
@@ -857,6 +911,8 @@ class _CutIntegral:
         return e, _as_const(f) * v
 
     def integrate(self, mesh, element_wise=False):
+        if self.is_eb():
+            raise NotImplementedError("Integrate over element boundaries")
         if self.is_bnd():
             sub, vmap, amb, lsub, dt, els = self._bnd_args(mesh, 0)
             order = self.measure.lsetdom.get("order", 5)
@@ -944,8 +1000,8 @@ class BilinearForm:
             return None
         terms = []
         for ci in self.integrals:
-            if ci.is_bnd():
-                return None  # boundary forms: element matrices on the GPU, host scatter
+            if ci.is_bnd() or ci.is_eb():
+                return None  # boundary / element-boundary forms: element matrices on the GPU, host scatter
             lset, dt, order, els = ci._rule_args(mesh, p)
             if isinstance(lset, (tuple, list)):
                 return None
