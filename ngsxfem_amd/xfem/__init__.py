@@ -947,6 +947,61 @@ def SymbolicCutBFI(levelset_domain, form, VOL_or_BND=VOL, element_boundary=False
         element_boundary=element_boundary, skeleton=skeleton))])
 
 
+def SymbolicCutLFI(levelset_domain, form, VOL_or_BND=VOL, element_boundary=False, skeleton=False, definedon=None,
+                   definedonelements=None, deformation=None):
+    """xfem/python_xfem.cpp:1015-1065: the linear form integrator on a level
+    set domain (f * v with a number or a coefficient function f)"""
+    if element_boundary or skeleton:
+        raise Exception("No Facet LFI with Symbolic cuts..")
+    return _SumOfIntegrals([_CutIntegral(form, _CutDifferentialSymbol(
+        levelset_domain, vb=VOL_or_BND, definedonelements=definedonelements, deformation=deformation))])
+
+
+def _wrapper_lsetdom(levelset_domain, kwargs):
+    """the level-set dictionary with the wrapper's order / time_order
+    keywords moved into it (python/__init__.py:228-237)"""
+    d = dict(levelset_domain)
+    if "order" in kwargs:
+        if "order" not in d:
+            d["order"] = kwargs["order"]
+        del kwargs["order"]
+    if "time_order" in kwargs:
+        if "time_order" not in d or d["time_order"] == -1:
+            d["time_order"] = kwargs["time_order"]
+        del kwargs["time_order"]
+    return d
+
+
+def _plain_integral(form, kwargs):
+    """NGSolve's SymbolicBFI / SymbolicLFI on uncut elements: the form on
+    dx(definedonelements=...)"""
+    if kwargs.get("VOL_or_BND", VOL) != VOL or kwargs.get("element_boundary") or kwargs.get("skeleton"):
+        raise NotImplementedError("plain (non level-set) integrators: volume elements only")
+    return _SumOfIntegrals([_CutIntegral(form, _Measure(kwargs.get("definedonelements")))])
+
+
+def SymbolicBFIWrapper(levelset_domain=None, *args, **kwargs):
+    """python/__init__.py:156-246: SymbolicCutBFI when a level-set
+    dictionary is given, otherwise the plain integrator"""
+    if levelset_domain is not None and type(levelset_domain) == dict:
+        return SymbolicCutBFI(_wrapper_lsetdom(levelset_domain, kwargs), *args, **kwargs)
+    form = levelset_domain if levelset_domain is not None else (args[0] if args else kwargs.pop("form"))
+    return _plain_integral(form, kwargs)
+
+
+def SymbolicLFIWrapper(levelset_domain=None, *args, **kwargs):
+    """python/__init__.py:248-337: SymbolicCutLFI when a level-set
+    dictionary is given, otherwise the plain integrator"""
+    if levelset_domain is not None and type(levelset_domain) == dict:
+        return SymbolicCutLFI(_wrapper_lsetdom(levelset_domain, kwargs), *args, **kwargs)
+    form = levelset_domain if levelset_domain is not None else (args[0] if args else kwargs.pop("form"))
+    return _plain_integral(form, kwargs)
+
+
+SymbolicBFI = SymbolicBFIWrapper
+SymbolicLFI = SymbolicLFIWrapper
+
+
 def _csr_pattern(dofs, n):
     """sorted CSR sparsity pattern of the element dof couplings (host)"""
     nd = dofs.shape[1]

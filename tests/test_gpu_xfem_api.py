@@ -155,3 +155,35 @@ def test_dcut_bilinear_form(X, prec):
     assert abs(one @ (m.mat @ one) - vol) < t * vol
     assert np.abs(k.mat @ one).max() < t * abs(k.mat).max() * 10
     assert abs(vol - math.pi * 0.09) < 2e-3
+
+
+def test_symbolic_wrappers(X):
+    """SymbolicBFI / SymbolicLFI (python/__init__.py:156-337) with a level-set
+    dictionary equal the dCut forms; without one, the plain integrators on
+    dx; SymbolicCutLFI refuses facet forms like the reference"""
+    mesh = X.MakeStructured2DMesh(nx=12, ny=12)
+    lset = X.GridFunction(X.H1(mesh, order=1))
+    X.InterpolateToP1(X.sqrt((X.x - 0.5) ** 2 + (X.y - 0.5) ** 2) - 0.3, lset)
+    V = X.H1(mesh, order=1)
+    u, v = V.TnT()
+    lsd = {"levelset": lset, "domain_type": X.NEG}
+    a1, a2 = X.BilinearForm(V), X.BilinearForm(V)
+    a1 += X.SymbolicBFI(levelset_domain=lsd, form=X.grad(u) * X.grad(v) + u * v)
+    a2 += (X.grad(u) * X.grad(v) + u * v) * X.dCut(lset, X.NEG)
+    a1.Assemble(precision=0)
+    a2.Assemble(precision=0)
+    assert abs(a1.mat - a2.mat).max() < 1e-14
+    f1, f2 = X.LinearForm(V), X.LinearForm(V)
+    f1 += X.SymbolicLFI(levelset_domain=lsd, form=(1 + X.x) * v)
+    f2 += (1 + X.x) * v * X.dCut(lset, X.NEG)
+    f1.Assemble()
+    f2.Assemble()
+    assert np.abs(f1.vec - f2.vec).max() < 1e-15
+    p1, p2 = X.BilinearForm(V), X.BilinearForm(V)
+    p1 += X.SymbolicBFI(form=u * v)
+    p2 += (u * v) * X.dx()
+    p1.Assemble(precision=0)
+    p2.Assemble(precision=0)
+    assert abs(p1.mat - p2.mat).max() < 1e-15 and abs(np.ones(V.ndof) @ (p1.mat @ np.ones(V.ndof)) - 1.0) < 1e-13
+    with pytest.raises(Exception):
+        X.SymbolicCutLFI(lsd, v * 1.0, element_boundary=True)
