@@ -404,6 +404,10 @@ class GridFunction:
         if isinstance(self.space, L2):
             raise NotImplementedError("GridFunction(L2).Set: assign .vec")
         if self.order == 1:
+            if isinstance(cf, (CoefficientFunction, GridFunction)):
+                # the coefficient's program at the vertices on the GPU (no perturbation)
+                self.vec = InterpolateCFToP1(self.space.mesh, CoefficientFunction._lift(cf), 0.0)
+                return
             self.vec = _eval_at_vertices(cf, self.space.mesh)
         elif self.order == 2:
             mesh = self.space.mesh

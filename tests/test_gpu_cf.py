@@ -237,3 +237,16 @@ def test_piecewise_constant_coefficients(X):
     g = X.GridFunction(X.H1(mesh, order=1))
     with pytest.raises(RuntimeError):
         X.InterpolateToP1(kap * X.x, g)
+
+
+def test_set_p1_from_grid_function_expression(X):
+    """GridFunction.Set on P1 evaluates the coefficient's program at the
+    vertices on the GPU, so expressions of grid functions interpolate too"""
+    mesh = X.MakeStructured3DMesh(nx=4, ny=4, nz=4)
+    V = X.H1(mesh, order=1)
+    h, g = X.GridFunction(V), X.GridFunction(V)
+    h.vec = np.arange(mesh.nv, dtype=np.float64) / mesh.nv
+    g.Set(2 * h + X.sin(X.x) - X.IfPos(X.y - 0.5, 1.0, 0.0))
+    xc = mesh.coords
+    ref = 2 * h.vec + np.sin(xc[:, 0]) - (xc[:, 1] > 0.5)
+    np.testing.assert_allclose(g.vec, ref, rtol=1e-15, atol=1e-15)
