@@ -372,6 +372,118 @@ def gen_forms_bernstein(D, P, L):
     return nterms
 
 
+def gen_affine(D, P, L, idx2):
+    """accum_aff: the affine moments n_beta = sum_i w_i y_i^beta, |beta| <= 2P,
+    in D affine coordinates y (lam_D = 1 - sum is not formed): the monomials
+    of degree 2..P take 16 products instead of the homogeneous ones' 30
+    (D = 3, P = 3), and the constant's weighted monomial is w itself.
+    n_beta is kept in the slot of the homogeneous moment (beta, 2P - |beta|).
+    With y = lam - 1/4 (centred coordinates) the homogeneous moments follow
+    by the dense map of gen_affine_table."""
+    n2 = 2 * P
+    zero = tuple([0] * D)
+
+    def slot(b):
+        return idx2[tuple(b) + (n2 - sum(b),)]
+    L.append("  // n_beta += w y^beta, |beta| <= 2P, in the slot of the homogeneous (beta, 2P - |beta|)")
+    L.append("  XG_DEV static void accum_aff(const double *x, double w, double *m) {")
+    have = {tuple(1 if i == k else 0 for i in range(D)): "x[%d]" % k for k in range(D)}
+    cnt = 0
+    for deg in range(2, P + 1):
+        for a in monomials(D - 1, deg):
+            k = next(i for i in range(D) if a[i] > 0)
+            b = list(a)
+            b[k] -= 1
+            b = tuple(b)
+            nm = "t%d" % cnt
+            cnt += 1
+            L.append("    const double %s = %s * x[%d];" % (nm, have[b], k))
+            have[a] = nm
+    groups = {}
+    for a in affine_monomials(D, n2):
+        beta = []
+        left = P
+        for i in range(D):
+            t = min(a[i], left)
+            beta.append(t)
+            left -= t
+        beta = tuple(beta)
+        gamma = tuple(x - y for x, y in zip(a, beta))
+        groups.setdefault(beta, []).append((a, gamma))
+    for b in affine_monomials(D, P):
+        if b not in groups:
+            continue
+        L.append("    {")
+        L.append("      const double wb = %s;" % ("w" if b == zero else "w * " + have[b]))
+        for a, gamma in groups[b]:
+            s = slot(a)
+            if gamma == zero:
+                L.append("      m[%d] += wb;" % s)
+            else:
+                L.append("      m[%d] = fma(wb, %s, m[%d]);" % (s, have[gamma], s))
+        L.append("    }")
+    L.append("  }")
+
+
+def affine_monomials(D, n):
+    """exponent tuples of length D with |beta| <= n, ascending degree"""
+    res = []
+    for deg in range(n + 1):
+        res.extend(monomials(D - 1, deg))
+    return res
+
+
+def gen_affine_table(D, P):
+    """m_alpha = sum_beta C[beta][alpha] n_beta for the centred affine
+    moments n (y_k = lam_k - 1/4, k < D; lam_D = 1/4 - sum y): C from the
+    expansion of prod_k (1/4 + y_k)^alpha_k (1/4 - sum y)^alpha_D.  Centring
+    keeps the map well conditioned (sum |C n| / max |m| about 20 on the
+    whole simplex, against 2,815 for y = lam), and its entries (multinomials
+    times powers of 1/4) are exact in FP32.  Layout [slot of beta][lane][2]:
+    the columns alpha = lane and alpha = lane + 64 (zero past NM), so a wave
+    reads one 8-byte pair per lane and row."""
+    n2 = 2 * P
+    mon2 = monomials(D, n2)
+    idx2 = {a: i for i, a in enumerate(mon2)}
+    NM = len(mon2)
+    q = Fraction(1, 4)
+    cz = Fraction(1) - D * q
+    C = [[Fraction(0)] * NM for _ in range(NM)]
+    for ai, a in enumerate(mon2):
+        p = {tuple([0] * D): Fraction(1)}
+        for k in range(D):
+            lk = {tuple([0] * D): q, tuple(1 if i == k else 0 for i in range(D)): Fraction(1)}
+            for _ in range(a[k]):
+                p = pmul(p, lk)
+        ld = {tuple([0] * D): cz}
+        for k in range(D):
+            ld[tuple(1 if i == k else 0 for i in range(D))] = Fraction(-1)
+        for _ in range(a[D]):
+            p = pmul(p, ld)
+        for beta, c in p.items():
+            C[idx2[tuple(beta) + (n2 - sum(beta),)]][ai] = c
+    L = []
+    L.append("// centred affine moments (y = lam - 1/4) -> homogeneous degree-%d moments, D = %d:" % (n2, D))
+    L.append("// m[alpha] = sum_slot xg_aff%d%d[slot][alpha %% 64][alpha / 64] n[slot]" % (D, P))
+    L.append("#ifndef XG_AFF_STORAGE")
+    L.append("#define XG_AFF_STORAGE static __device__")
+    L.append("#endif")
+    L.append("XG_AFF_STORAGE const float xg_aff%d%d[%d][64][2] __attribute__((aligned(16))) = {" % (D, P, NM))
+    for r in range(NM):
+        vals = []
+        for lane in range(64):
+            for h in range(2):
+                al = lane + 64 * h
+                c = C[r][al] if al < NM else Fraction(0)
+                f = float(c)
+                import struct
+                assert Fraction(struct.unpack("f", struct.pack("f", f))[0]) == c, "not exact in FP32"
+                vals.append("0" if c == 0 else repr(f))
+        L.append("    {" + ", ".join("{%s, %s}" % (vals[2 * j], vals[2 * j + 1]) for j in range(64)) + "},")
+    L.append("};")
+    return "\n".join(L)
+
+
 def gen(D, P):
     nv = D + 1
     mon2 = monomials(D, 2 * P)
@@ -429,6 +541,8 @@ def gen(D, P):
             L.append("      m[%d] = fma(wb, %s, m[%d]);" % (idx2[a], have[gamma], idx2[a]))
         L.append("    }")
     L.append("  }")
+    if D == 3:
+        gen_affine(D, P, L, idx2)
     # ---- complement: m <- (moments of the whole reference simplex) - m.
     # int_T lam^alpha = |T| D! alpha! / (|alpha| + D)! with |T| = 1/D! (the
     # rule weights of the reference simplex sum to its volume)
@@ -553,6 +667,7 @@ def main():
     for D in (1, 2, 3):
         for P in (1, 2, 3):
             out.append(gen(D, P))
+    out.append(gen_affine_table(3, 3))
     out.append("#ifndef XG_MOM_NO_NODAL  // host builds of the identities leave the device table out")
     out.append(gen_nodal(3, 4))
     out.append("#endif")

@@ -19,6 +19,7 @@ HARNESS = r"""
 #include <cmath>
 #define XG_DEV inline
 #define XG_MOM_NO_NODAL
+#define XG_AFF_STORAGE static
 #include "moments_table.h"
 extern "C" int mom_elmat(int D, int P, const double *V, const double *pts, const double *w, int n, int mask,
                          double *out) {
@@ -127,6 +128,22 @@ extern "C" int mom_hsimplex(int D, int P, const double *lv, int nv, double s, do
   HCASE(1, 1) HCASE(1, 2) HCASE(2, 1) HCASE(2, 2) HCASE(3, 1) HCASE(3, 2)
   return -1;
 }
+// Moments<3, 3>::accum_aff over a point set in centred coordinates
+// y = x - 1/4, then the dense map xg_aff33 to the homogeneous moments
+extern "C" int mom_accum_aff(const double *pts, const double *w, int n, double *m) {
+  using M = xg::Moments<3, 3>;
+  double nn[M::NM] = {0};
+  for (int q = 0; q < n; ++q) {
+    const double y[3] = {pts[3 * q] - 0.25, pts[3 * q + 1] - 0.25, pts[3 * q + 2] - 0.25};
+    M::accum_aff(y, w[q], nn);
+  }
+  for (int a = 0; a < M::NM; ++a) {
+    double s = 0;
+    for (int r = 0; r < M::NM; ++r) s += (double)xg::xg_aff33[r][a % 64][a / 64] * nn[r];
+    m[a] += s;
+  }
+  return M::NM;
+}
 """
 
 
@@ -214,6 +231,41 @@ def test_vertex_recursion_gives_exact_simplex_moments(momlib, D, P):
             s = vol * math.factorial(k) / math.factorial(n + k)
             momlib.mom_hsimplex(D, P, _p(np.ascontiguousarray(lam)), k + 1, s, _p(got))
             assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max(), (D, P, k, trial)
+
+
+def test_affine_accumulation_equals_homogeneous(momlib):
+    """accum_aff (monomials of three centred affine coordinates y = lam - 1/4,
+    degree <= 6, lam_3 never formed) followed by the dense map xg_aff33 gives
+    the homogeneous degree-6 moments of accum: on random points with
+    mixed-sign weights (complemented pieces), on points clustered at each
+    vertex (where the map cancels most), and on the closed form of the whole
+    reference tetrahedron (int lam^alpha = 3! alpha! / 9! |T|, a
+    Grundmann-Moeller rule of degree 7)"""
+    rng = np.random.default_rng(43)
+    corners = np.vstack([np.eye(3), np.zeros(3)])
+    for trial in range(9):
+        n = 50
+        pts = rng.random((n, 3))
+        pts = pts / (pts.sum(1, keepdims=True) + rng.random((n, 1)))
+        if trial >= 5:  # a small cluster at vertex trial - 5
+            pts = corners[trial - 5] + 0.05 * (pts - corners[trial - 5])
+        pts = np.ascontiguousarray(pts)
+        w = np.ascontiguousarray(rng.random(n) - 0.3 * (trial % 2))
+        ref = np.zeros(128)
+        nm = momlib.mom_accum(3, 3, _p(pts), _p(w), n, _p(ref))
+        got = np.zeros(128)
+        assert momlib.mom_accum_aff(_p(pts), _p(w), n, _p(got)) == nm
+        assert np.abs(got[:nm] - ref[:nm]).max() <= 1e-14 * np.abs(ref[:nm]).max(), (trial,)
+    import sys
+    sys.path.insert(0, HERE)
+    from test_oracle_invariants import gm_rule
+    b, w = gm_rule(3, 3)  # degree 7 on the unit tetrahedron (weights sum to 1/6)
+    pts = np.ascontiguousarray(b[:, 1:])
+    got = np.zeros(128)
+    nm = momlib.mom_accum_aff(_p(pts), _p(np.ascontiguousarray(w)), len(w), _p(got))
+    full = np.zeros(128)
+    momlib.mom_complement(3, 3, _p(full))
+    assert np.abs(got[:nm] - full[:nm]).max() <= 1e-14 * np.abs(full[:nm]).max()
 
 
 def test_moment_table_is_current():
