@@ -935,6 +935,43 @@ __host__ __device__ inline size_t sdx_bytes(int L, int nq_vol, int nq_if, int NM
   return o;
 }
 
+// LDS of k_sdx_leaves alone: the arrays it uses (the walk's candidate and
+// lattice arrays left out), about 19 KB at p = 3, so 8 one-wave blocks fit
+// in a CU's 160 KB (2 waves on every SIMD; the whole SdxLds, 26 KB, allowed
+// 6)
+__host__ __device__ inline size_t sdx_leaf_bytes(int nq_vol, int nq_if, int NM, SdxLds *s, char *base) {
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = o;
+    o = sd_align(o + bytes);
+    return at;
+  };
+  const size_t a_red = take(sizeof(double) * 2 * NM);
+  const size_t a_rv = take(sizeof(double) * 4 * nq_vol);
+  const size_t a_ri = take(sizeof(double) * 4 * nq_if);
+  const size_t a_cp = take(sizeof(double) * 64 * 8 * 3);
+  const size_t a_vtf = take(sizeof(double) * 192);
+  const size_t a_ifw = take(sizeof(double) * 128);
+  const size_t a_vrec = take(sizeof(uint32_t) * 192);
+  const size_t a_irec = take(sizeof(uint32_t) * 128);
+  const size_t a_cl = take(sizeof(double) * 20);
+  const size_t a_fi = take(sizeof(double) * 10);
+  if (s) {
+    *s = SdxLds{};
+    s->red = (double *)(base + a_red);
+    s->rule_vol = (double(*)[4])(base + a_rv);
+    s->rule_if = (double(*)[4])(base + a_ri);
+    s->pool = (double(*)[8][3])(base + a_cp);
+    s->voltf = (double *)(base + a_vtf);
+    s->ifw = (double *)(base + a_ifw);
+    s->volrec = (uint32_t *)(base + a_vrec);
+    s->ifrec = (uint32_t *)(base + a_irec);
+    s->cl = (double *)(base + a_cl);
+    s->fi = (double *)(base + a_fi);
+  }
+  return o;
+}
+
 // The pieces of a cut leaf for the kept volume side SIDE (CutSimplex<3>::
 // MakeQuad, xintegration.cpp:1084-1388), as index lists into
 // [leaf vertices 0..3, cut points 4..7]: `nvol` tets with sign `vsign`
@@ -2026,7 +2063,7 @@ __global__ __launch_bounds__(64, 2) void k_sdx_leaves(int nel, const int32_t *__
   SdxLds S;
   const int vol_nq = xg_minrule_cnt_d[3][PM];
   const int if_nq = xg_minrule_cnt_d[2][PM];
-  sdx_bytes(L, vol_nq, if_nq, NM, &S, smem);
+  sdx_leaf_bytes(vol_nq, if_nq, NM, &S, smem);
   // leaf points lane-minor: (point j, coordinate d, leaf lane l)
   double *pool = &S.pool[0][0][0];
   const int lane = threadIdx.x & 63;
@@ -2486,7 +2523,7 @@ int launch_sdx_leaves(xg_ctx *ctx, int nel, const int32_t *elems, const double *
                       int32_t *queue, int32_t *stats_if = nullptr) {
   constexpr int NM = Moments<3, PM>::NM;
   const void *kern = (const void *)k_sdx_leaves<LO, PM, SIDE, WHICH>;
-  const size_t lds = sdx_bytes(L, xg_minrule_cnt[3][PM], xg_minrule_cnt[2][PM], NM, nullptr, nullptr);
+  const size_t lds = sdx_leaf_bytes(xg_minrule_cnt[3][PM], xg_minrule_cnt[2][PM], NM, nullptr, nullptr);
   if (lds > 65536) XG_CHECK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   int blocks = nel < (1 << 22) ? nel : (1 << 22);
   if (queue) {
