@@ -378,7 +378,7 @@ def measure_c5(args):
             "traffic_source": "profiles/subdiv_c5_pmc.txt (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the same "
                               "kernel, copy-calibrated, summed over its launches of one step)",
             "algorithmic_flops_per_launch": ops, "avg_launch_ms": ms_kern,
-            "launch_note": "one step launches it once per 262,144 elements (2 launches); flops and ms are per step",
+            "launch_note": "one launch per step (a launch takes up to 524,288 elements); flops and ms are per step",
             "counts_per_cut_element": {"table_rows": float(st[:, 0, 0].mean()),
                                        "volume_tets": float(st[:, 0, 1].mean()),
                                        "interface_triangles": float(st[:, 1, 2].mean()),
