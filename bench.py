@@ -231,6 +231,13 @@ def c5_leaf_ops(st_vol, st_if, p=3):
     vol = float(st_vol[..., 1].sum())
     itf = float(st_if[..., 2].sum())
     leaves = float(st_vol[..., 3].sum())
+    if p == 3:
+        # centred affine moments: per point the map (25 volume / 18 interface),
+        # 16 monomials, 19 weighted monomials, 83 FMAs + 1 add; per element the
+        # 84 x 84 map to the homogeneous moments for each of the two parts
+        nel = float(st_vol[..., 0].size)
+        return (vol * (nq_v * (25 + 16 + 19 + 167) + 17) + itf * (nq_i * (18 + 16 + 19 + 167) + 40)
+                + leaves * (48 + 4 * 42) + nel * 2 * 2 * nm * nm)
     return (vol * (nq_v * (27 + 30 + 21 + 2 * nm) + 17) + itf * (nq_i * (18 + 30 + 21 + 2 * nm) + 40)
             + leaves * (48 + 4 * 42))
 
