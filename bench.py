@@ -456,7 +456,7 @@ def c5_learned(args, ctx, stream, cl, d_el, d_x, lp2, L, ms_plain):
     outputs, BF16 MFMA), checked on degree 2 against the level-4 rule, the
     rejected elements from the level-4 rule; the interface moments from the
     plain (staged) path; then the P3 element matrices of both.  The network is
-    trained on every other cut element of this workload and judged on all."""
+    trained on every other cut element of this workload and evaluated on all."""
     import torch
 
     from ngsxfem_amd import _abi
